@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput of the Bloch-equation hot path on one MI355X.
+
+Workload (one "step" = one full solve of the configuration):
+  2D massive Dirac d = (v(kx - A), v ky, Delta), v = 1, Delta = 0.1, on a
+  1024 x 512 Gauss-Legendre grid over |kx|, |ky| <= 3 (524,288 k-points);
+  cos^2 pulse A0 = 0.5, 8 optical cycles, T2 = 10 cycles transverse
+  dephasing; 4096 sampling intervals; FP64 RKF78 at rtol = atol = 1e-10 with
+  the reference's controller restart at Delta/50 every interval.
+
+Prints one JSON line: trajectories/s, the FP64 roofline of the solve kernel
+(algorithmic FLOPs from the attempted-step counter over the HIP-event time of
+the kernel on its own stream), and the CPU oracle timed on a bounded sample
+of the same workload on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--small]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X vector FP64
+
+# Algorithmic FLOPs (convention: + - * = 1, FMA = 2, / sqrt = 1, each
+# sin/cos/exp = 1).  RKF78 step algebra: 55 tableau FMAs x 3 components +
+# stage updates + 8th-order sum + embedded error + controller.
+F_STEP = 450.0
+F_RHS = {"dirac_t2": 47.0, "dirac": 23.0, "piflux": 26.0, "honeycomb": 34.0}
+F_VERTEX = {"dirac": 3.0, "piflux": 4.0, "honeycomb": 20.0}
+F_MAGNUS = 85.0
+
+
+def workload(small: bool):
+    T = 16 * math.pi
+    nx, ny = (256, 128) if small else (1024, 512)
+    return dict(T=T, nx=nx, ny=ny, N=4096, n_sub=50, v=1.0, mass=0.1, A0=0.5, n_cycles=8.0, T2=20 * math.pi,
+                rtol=1e-10, atol=1e-10)
+
+
+def make_plan(H, wl, method=0, relax=2, proposal=0, magnus_substeps=5, **launch):
+    T = wl["T"]
+    md = H.ModelDesc(H.MODEL_DIRAC2D, relax, wl["v"], wl["mass"], 0.0, 0.0, 0.0, 0.0, wl["T2"] if relax else 0.0)
+    laser = H.CosineLaser(wl["A0"], wl["n_cycles"], -T / 2)
+    tc = H.TimeIntegrationConfig(-T / 2, T / 2, wl["N"], wl["n_sub"])
+    grid = H.KGrid.box(wl["nx"], wl["ny"], -3.0, 3.0, -3.0, 3.0)
+    solver = H.Solver(method=method, rtol=wl["rtol"], atol=wl["atol"], proposal=proposal,
+                      magnus_substeps=magnus_substeps)
+    return H.Plan([H.Job(md, laser, tc, 0)], [grid], solver, **launch)
+
+
+def time_plan(plan, steps, warmup):
+    for _ in range(warmup):
+        plan.execute()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    solve_ms = []
+    t0 = time.perf_counter()
+    j = None
+    for _ in range(steps):
+        j = plan.execute()[0]
+        solve_ms.append(plan.stats.solve_ms)
+    wall = time.perf_counter() - t0
+    return wall, float(np.mean(solve_ms)), plan.stats.as_dict(), j
+
+
+def cpu_baseline(wl, target_s=10.0):
+    """The oracle (optimised build) on a seeded random sample of the workload's
+    k-points, OpenMP over the host cores (capped at 16)."""
+    import oracle as O
+    T = wl["T"]
+    kx, ky, w = O.gl_box(wl["nx"], wl["ny"], -3.0, 3.0, -3.0, 3.0)
+    m = O.model(O.MODEL_DIRAC2D, v=wl["v"], mass=wl["mass"], relax=O.RELAX_T2, relax_time=wl["T2"])
+    p = O.pulse(O.PULSE_COS2, wl["A0"], n_cycles=wl["n_cycles"], t0=-T / 2)
+    tc = O.time_grid(-T / 2, T / 2, wl["N"], wl["n_sub"])
+    sv = O.solver(rtol=wl["rtol"], atol=wl["atol"])
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rng = np.random.default_rng(20261203)
+    pilot = rng.choice(len(kx), size=threads * 2, replace=False)
+    t0 = time.perf_counter()
+    O.current_density(m, p, tc, sv, kx[pilot], ky[pilot], w[pilot], n_threads=threads, fast=True)
+    per = (time.perf_counter() - t0) / len(pilot)
+    n = int(min(len(kx), max(threads * 4, target_s / max(per, 1e-9))))
+    sel = rng.choice(len(kx), size=n, replace=False)
+    t0 = time.perf_counter()
+    O.current_density(m, p, tc, sv, kx[sel], ky[sel], w[sel], n_threads=threads, fast=True)
+    wall = time.perf_counter() - t0
+    return {"value": n / wall, "unit": "trajectories/s", "cores": threads, "kind": "port",
+            "sample": f"{n} seeded-random k-points of the {wl['nx']}x{wl['ny']} grid (seed 20261203), "
+                      f"oracle -O3 -ffast-math build, OpenMP {threads} threads, {wall:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="256x128 grid (profiling runs)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--extras", action="store_true", help="also time Magnus-4 and the carry ablation")
+    ap.add_argument("--autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it shards)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import higherharmonics_amd as H
+    wl = workload(args.small)
+    K = wl["nx"] * wl["ny"]
+    print(f"[bench] Dirac-2D {wl['nx']}x{wl['ny']} N={wl['N']} RKF78 1e-10 restart, T2 dephasing", file=sys.stderr,
+          flush=True)
+    plan = make_plan(H, wl, persistent=1)
+    tuned = plan.autotune() if args.autotune else None
+    wall, solve_ms, stats, j = time_plan(plan, args.steps, args.warmup)
+    plan.close()
+    value = args.steps * K / wall
+    attempts_per_step = stats["attempts"]
+    flops = attempts_per_step * (F_STEP + 13 * F_RHS["dirac_t2"]) + K * (wl["N"] + 1) * F_VERTEX["dirac"]
+    achieved = flops / (solve_ms * 1e-3) / 1e12
+    out = {
+        "metric": "k-point trajectories/s at <=1e-9 rel j(t) err (plain GPU RKF78, reference restart controller)",
+        "value": value,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic pulse, Gauss-Legendre k-grid)",
+        "config": {"workload": f"Dirac-2D {wl['nx']}x{wl['ny']} GL, cos^2 A0=0.5 8 cycles, T2=10 cycles, "
+                               f"4096 samples, RKF78 rtol=atol=1e-10", "trajectories": K,
+                   "attempted_steps": attempts_per_step, "rejects": stats["rejects"],
+                   "launch": tuned},
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel_ms": solve_ms, "algorithmic_flop_per_launch": flops},
+    }
+    if args.extras:
+        extras = {}
+        for name, kw in [("magnus4_T2inf_dt_T20480", dict(method=1, relax=0, magnus_substeps=5)),
+                         ("rkf78_T2inf", dict(method=0, relax=0)),
+                         ("rkf78_carry_ablation", dict(method=0, relax=2, proposal=1))]:
+            p = make_plan(H, wl, persistent=1, **kw)
+            w2, sm, st, _ = time_plan(p, max(1, args.steps // 2), 1)
+            p.close()
+            extras[name] = {"trajectories_per_s": max(1, args.steps // 2) * K / w2, "kernel_ms": sm,
+                            "attempts": st["attempts"]}
+        out["extras"] = extras
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,509 @@
+"""MI355X HigherHarmonics Bloch hot path: Python binding of libhhg_amd.so.
+
+The library's C ABI (``include/hhg_amd.h``) is the drop-in boundary; this
+module binds it with ctypes and mirrors the reference's host interface:
+
+* :class:`TimeIntegrationConfig` — sources/HHG/TimeIntegrationConfig.hpp:7-15
+* :class:`CosineLaser`, :class:`GaussLaser`, :class:`DoubleLaser` —
+  sources/HHG/Laser/{CosineLaser,GaussLaser,DoubleLaser}.hpp, with
+  ``laser_function(t)`` as Laser.hpp:32
+* :class:`DiracSystem2D`, :class:`PiFlux2D`, :class:`Honeycomb` with
+  ``compute_current_density(laser, time_config, rank, n_ranks)`` as
+  sources/HHG/Systems/{DiracSystem,PiFlux,Honeycomb}.hpp
+* :class:`Plan` — a reusable multi-job plan (the Dispatcher analogue)
+* :func:`trapezoidal_spectrum` — sources/HHG/Fourier/TrapezoidalFFT.cpp:11-49
+
+Every solve runs on the GPU through the HIP kernels; there is no CPU path.
+A missing library raises :class:`HHGError` instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhhg_amd.so")
+
+# ---- enums (include/hhg_amd.h) ----
+SUCCESS, ERROR_INVALID_ARGUMENT, ERROR_HIP, ERROR_TOLERANCE, ERROR_OUT_OF_MEMORY, ERROR_UNSUPPORTED, \
+    ERROR_NO_DEVICE, ERROR_NONFINITE = range(8)
+MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB = 0, 1, 2
+RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2 = 0, 1, 2
+PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2 = 0, 1, 2, 3
+KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT = 0, 1, 2
+METHOD_RKF78, METHOD_MAGNUS4 = 0, 1
+PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
+
+_STATUS_NAMES = {0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "TOLERANCE", 4: "OUT_OF_MEMORY",
+                 5: "UNSUPPORTED", 6: "NO_DEVICE", 7: "NONFINITE"}
+
+
+class HHGError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+# ---- ABI structs ----
+class ModelDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
+                ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
+                ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double)]
+
+
+class PulseDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("amp_a", ctypes.c_double),
+                ("amp_b", ctypes.c_double), ("n_cycles", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("t0", ctypes.c_double), ("delay", ctypes.c_double), ("cep", ctypes.c_double)]
+
+
+class TimeDesc(ctypes.Structure):
+    _fields_ = [("t_begin", ctypes.c_double), ("t_end", ctypes.c_double), ("n_measurements", ctypes.c_int32),
+                ("n_subdivisions", ctypes.c_int32)]
+
+
+class KGridDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("x0", ctypes.c_double), ("x1", ctypes.c_double), ("y0", ctypes.c_double), ("y1", ctypes.c_double),
+                ("kx", ctypes.POINTER(ctypes.c_double)), ("ky", ctypes.POINTER(ctypes.c_double)),
+                ("w", ctypes.POINTER(ctypes.c_double)), ("n_points", ctypes.c_int64)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("model", ModelDesc), ("pulse", PulseDesc), ("time", TimeDesc), ("kgrid", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+class SolverDesc(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("proposal", ctypes.c_int32), ("rtol", ctypes.c_double),
+                ("atol", ctypes.c_double), ("max_fail", ctypes.c_int32), ("magnus_substeps", ctypes.c_int32),
+                ("mlp", ctypes.c_void_p)]
+
+
+class LaunchDesc(ctypes.Structure):
+    _fields_ = [("waves_per_block", ctypes.c_int32), ("queue_chunk", ctypes.c_int32),
+                ("persistent", ctypes.c_int32), ("group_by_cost", ctypes.c_int32)]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("n_kgrids", ctypes.c_int32), ("n_jobs", ctypes.c_int32), ("kgrids", ctypes.POINTER(KGridDesc)),
+                ("jobs", ctypes.POINTER(JobDesc)), ("solver", SolverDesc), ("launch", LaunchDesc),
+                ("rank", ctypes.c_int32), ("n_ranks", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("trajectories", ctypes.c_int64), ("attempts", ctypes.c_int64), ("rejects", ctypes.c_int64),
+                ("failed_trajectories", ctypes.c_int64), ("nonfinite_trajectories", ctypes.c_int64),
+                ("proposal_fallbacks", ctypes.c_int64), ("solve_ms", ctypes.c_double),
+                ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# Every symbol include/hhg_amd.h declares (checked by the ABI test).
+EXPORTED_SYMBOLS = [
+    "hhg_abi_version", "hhg_last_error_string", "hhg_plan_create", "hhg_plan_output_size", "hhg_plan_job_offsets",
+    "hhg_plan_execute", "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph",
+    "hhg_plan_destroy", "hhg_trajectories", "hhg_gauss_legendre", "hhg_kgrid_size", "hhg_kgrid_build",
+    "hhg_pulse_evaluate", "hhg_trapezoidal_spectrum", "hhg_plan_execute_accumulators", "hhg_accumulators_add",
+    "hhg_accumulators_to_double",
+]
+
+_lib: Optional[ctypes.CDLL] = None
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libhhg_amd.so.  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HHGError(ERROR_NO_DEVICE, f"{LIB_PATH} missing: run `python -m higherharmonics_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    L.hhg_abi_version.restype = ctypes.c_int32
+    L.hhg_last_error_string.restype = ctypes.c_char_p
+    L.hhg_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(P)]
+    L.hhg_plan_output_size.restype = ctypes.c_int64
+    L.hhg_plan_output_size.argtypes = [P]
+    L.hhg_plan_job_offsets.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
+    L.hhg_plan_execute.argtypes = [P, _DP, ctypes.POINTER(Stats)]
+    L.hhg_plan_execute_accumulators.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]
+    L.hhg_plan_execute_device.argtypes = [P, _DP, P]
+    L.hhg_plan_autotune.argtypes = [P, ctypes.POINTER(LaunchDesc)]
+    L.hhg_plan_capture_graph.argtypes = [P]
+    L.hhg_plan_destroy.argtypes = [P]
+    L.hhg_trajectories.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
+    L.hhg_gauss_legendre.argtypes = [ctypes.c_int32, _DP, _DP]
+    L.hhg_kgrid_size.restype = ctypes.c_int64
+    L.hhg_kgrid_size.argtypes = [ctypes.POINTER(KGridDesc)]
+    L.hhg_kgrid_build.argtypes = [ctypes.POINTER(KGridDesc), _DP, _DP, _DP]
+    L.hhg_pulse_evaluate.argtypes = [ctypes.POINTER(PulseDesc), _DP, _DP, ctypes.c_int64]
+    L.hhg_trapezoidal_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), ctypes.c_int32, _DP, _DP, _DP]
+    L.hhg_accumulators_add.restype = None
+    L.hhg_accumulators_add.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                       ctypes.c_int64]
+    L.hhg_accumulators_to_double.restype = None
+    L.hhg_accumulators_to_double.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64, _DP]
+    for name in ("hhg_plan_create", "hhg_plan_job_offsets", "hhg_plan_execute", "hhg_plan_execute_accumulators",
+                 "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
+                 "hhg_trajectories", "hhg_gauss_legendre", "hhg_kgrid_build", "hhg_pulse_evaluate",
+                 "hhg_trapezoidal_spectrum"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(status: int, allow=()) -> int:
+    if status != SUCCESS and status not in allow:
+        msg = lib().hhg_last_error_string()
+        raise HHGError(status, msg.decode() if msg else "")
+    return status
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+# --------------------------------------------------------------------------
+# Reference-shaped host objects
+# --------------------------------------------------------------------------
+
+@dataclass
+class TimeIntegrationConfig:
+    """{t_begin, t_end, n_measurements, n_subdivisions} (TimeIntegrationConfig.hpp:7-15)."""
+    t_begin: float
+    t_end: float
+    n_measurements: int
+    n_subdivisions: int = 50
+
+    def measure_every(self) -> float:
+        return (self.t_end - self.t_begin) / self.n_measurements
+
+    def dt(self) -> float:
+        return self.measure_every() / self.n_subdivisions
+
+    def times(self) -> np.ndarray:
+        return self.t_begin + np.arange(self.n_measurements + 1) * self.measure_every()
+
+    def desc(self) -> TimeDesc:
+        return TimeDesc(self.t_begin, self.t_end, self.n_measurements, self.n_subdivisions)
+
+
+class Laser:
+    """A(t) = amplitude * envelope(t) * sin(t + cep) (Laser.hpp:32-54)."""
+    t_begin: float
+    t_end: float
+
+    def desc(self) -> PulseDesc:  # pragma: no cover - overridden
+        raise NotImplementedError
+
+    def laser_function(self, t) -> np.ndarray:
+        t = np.ascontiguousarray(np.atleast_1d(np.asarray(t, dtype=np.float64)))
+        A = np.empty_like(t)
+        d = self.desc()
+        _check(lib().hhg_pulse_evaluate(ctypes.byref(d), _dp(t), _dp(A), len(t)))
+        return A
+
+
+class CosineLaser(Laser):
+    """cos^2 envelope on (t0, t0 + 2 pi n) (CosineLaser.cpp:22-26, CosineLaser.hpp:13-15);
+    ``begin_shift`` is the reference's window begin (pi * t0_offset)."""
+
+    def __init__(self, amplitude: float, n_cycles: float, begin_shift: float = 0.0, cep: float = 0.0):
+        self.amplitude, self.n_cycles, self.cep = amplitude, n_cycles, cep
+        self.t_begin = begin_shift
+        self.t_end = begin_shift + 2 * math.pi * n_cycles
+
+    def desc(self) -> PulseDesc:
+        return PulseDesc(PULSE_COS2, 0, self.amplitude, 0.0, self.n_cycles, 0.0, self.t_begin, 0.0, self.cep)
+
+
+class GaussLaser(Laser):
+    """Gaussian envelope exp(-(t-c)^2 / 2 sigma^2) (GaussLaser.cpp:26-28).  ``window``
+    is the half width of the time window in units of sigma."""
+
+    def __init__(self, amplitude: float, sigma: float, center: float = 0.0, cep: float = 0.0, window: float = 5.0):
+        self.amplitude, self.sigma, self.center, self.cep = amplitude, sigma, center, cep
+        self.t_begin = center - window * sigma
+        self.t_end = center + window * sigma
+
+    @staticmethod
+    def reference_sigma(n_cycles: float) -> float:
+        """sigma of the reference's GaussLaser(n_cycles) (GaussLaser.cpp:7)."""
+        return math.pi * n_cycles / math.sqrt(2 * math.log(1 / 0.05))
+
+    def desc(self) -> PulseDesc:
+        return PulseDesc(PULSE_GAUSS, 0, self.amplitude, 0.0, 0.0, self.sigma, self.center, 0.0, self.cep)
+
+
+class DoubleLaser(Laser):
+    """A/B pair: A(t) + B(t - delay), B the same base pulse (DoubleLaser.hpp:63-67)
+    with its own amplitude, evaluated analytically."""
+
+    def __init__(self, base: Laser, amplitude_b: float, delay: float):
+        self.base, self.amplitude_b, self.delay = base, amplitude_b, delay
+        self.t_begin = min(base.t_begin, base.t_begin + delay)
+        self.t_end = max(base.t_end, base.t_end + delay)
+
+    def desc(self) -> PulseDesc:
+        d = self.base.desc()
+        d.kind = PULSE_DOUBLE_GAUSS if d.kind == PULSE_GAUSS else PULSE_DOUBLE_COS2
+        d.amp_b = self.amplitude_b
+        d.delay = self.delay
+        return d
+
+
+@dataclass
+class KGrid:
+    kind: int
+    nx: int = 0
+    ny: int = 0
+    x0: float = 0.0
+    x1: float = 0.0
+    y0: float = 0.0
+    y1: float = 0.0
+    points: Optional[tuple] = None  # (kx, ky, w) for explicit grids
+
+    @staticmethod
+    def box(nx, ny, x0, x1, y0, y1) -> "KGrid":
+        return KGrid(KGRID_GL_BOX, nx, ny, x0, x1, y0, y1)
+
+    @staticmethod
+    def hexagonal(nx, ny) -> "KGrid":
+        return KGrid(KGRID_GL_HEX, nx, ny)
+
+    @staticmethod
+    def explicit(kx, ky, w) -> "KGrid":
+        arrs = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in (kx, ky, w))
+        return KGrid(KGRID_EXPLICIT, points=arrs)
+
+    def desc(self) -> KGridDesc:
+        d = KGridDesc(self.kind, self.nx, self.ny, 0, self.x0, self.x1, self.y0, self.y1, None, None, None, 0)
+        if self.kind == KGRID_EXPLICIT:
+            kx, ky, w = self.points
+            d.kx, d.ky, d.w, d.n_points = _dp(kx), _dp(ky), _dp(w), len(kx)
+        return d
+
+    def build(self):
+        d = self.desc()
+        n = lib().hhg_kgrid_size(ctypes.byref(d))
+        kx, ky, w = np.empty(n), np.empty(n), np.empty(n)
+        _check(lib().hhg_kgrid_build(ctypes.byref(d), _dp(kx), _dp(ky), _dp(w)))
+        return kx, ky, w
+
+
+@dataclass
+class Solver:
+    method: int = METHOD_RKF78
+    rtol: float = 1e-10
+    atol: float = 1e-10
+    proposal: int = PROPOSAL_RESTART
+    max_fail: int = 500
+    magnus_substeps: int = 1
+
+    def desc(self) -> SolverDesc:
+        return SolverDesc(self.method, self.proposal, self.rtol, self.atol, self.max_fail, self.magnus_substeps, None)
+
+
+@dataclass
+class Job:
+    model: ModelDesc
+    laser: Laser
+    time_config: TimeIntegrationConfig
+    kgrid: int = 0
+
+
+class Plan:
+    """A compiled set of jobs on one device (the reference Dispatcher's role).
+
+    ``execute()`` returns one j(t) array per job."""
+
+    def __init__(self, jobs: Sequence[Job], kgrids: Sequence[KGrid], solver: Solver = Solver(),
+                 rank: int = 0, n_ranks: int = 1, device: int = 0, waves_per_block: int = 0,
+                 queue_chunk: int = 0, persistent: int = 1):
+        L = lib()
+        self._kgrids = list(kgrids)  # keep explicit arrays alive
+        kg = (KGridDesc * len(kgrids))(*[g.desc() for g in kgrids])
+        jd = (JobDesc * len(jobs))(*[JobDesc(j.model, j.laser.desc(), j.time_config.desc(), j.kgrid, 0)
+                                     for j in jobs])
+        d = PlanDesc(len(kgrids), len(jobs), kg, jd, solver.desc(),
+                     LaunchDesc(waves_per_block, queue_chunk, persistent, 0), rank, n_ranks, device, 0)
+        h = ctypes.c_void_p()
+        _check(L.hhg_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.n_jobs = len(jobs)
+        self.n_out = L.hhg_plan_output_size(h)
+        off = (ctypes.c_int64 * (len(jobs) + 1))()
+        _check(L.hhg_plan_job_offsets(h, off))
+        self.offsets = list(off)
+        self.stats = Stats()
+
+    def execute_flat(self, allow_failures: bool = False) -> np.ndarray:
+        j = np.empty(self.n_out)
+        allow = (ERROR_TOLERANCE, ERROR_NONFINITE) if allow_failures else ()
+        self.status = _check(lib().hhg_plan_execute(self._h, _dp(j), ctypes.byref(self.stats)), allow)
+        return j
+
+    def execute(self, allow_failures: bool = False):
+        j = self.execute_flat(allow_failures)
+        return [j[self.offsets[i]:self.offsets[i + 1]] for i in range(self.n_jobs)]
+
+    def execute_accumulators(self) -> np.ndarray:
+        acc = np.empty(2 * self.n_out, dtype=np.uint64)
+        _check(lib().hhg_plan_execute_accumulators(
+            self._h, acc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(self.stats)))
+        return acc
+
+    def autotune(self) -> dict:
+        ld = LaunchDesc()
+        _check(lib().hhg_plan_autotune(self._h, ctypes.byref(ld)))
+        return {"waves_per_block": ld.waves_per_block, "queue_chunk": ld.queue_chunk}
+
+    def capture_graph(self) -> None:
+        _check(lib().hhg_plan_capture_graph(self._h))
+
+    def trajectories(self, k_index) -> np.ndarray:
+        """Bloch vectors [n_sel][N+1][3] of job 0 for the given k indices."""
+        sel = np.ascontiguousarray(k_index, dtype=np.int64)
+        n = self.offsets[1] - self.offsets[0]
+        out = np.empty((len(sel), n, 3))
+        _check(lib().hhg_trajectories(self._h, sel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(sel),
+                                      _dp(out)))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().hhg_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def accumulators_to_double(acc: np.ndarray) -> np.ndarray:
+    n = len(acc) // 2
+    j = np.empty(n)
+    lib().hhg_accumulators_to_double(np.ascontiguousarray(acc).ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
+                                     _dp(j))
+    return j
+
+
+def accumulators_add(total: np.ndarray, part: np.ndarray) -> None:
+    lib().hhg_accumulators_add(total.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                               np.ascontiguousarray(part).ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                               len(total) // 2)
+
+
+# --------------------------------------------------------------------------
+# Systems with the reference's compute_current_density shape
+# --------------------------------------------------------------------------
+
+class _System:
+    kind = MODEL_DIRAC2D
+
+    def __init__(self, fermi_energy=0.0, beta=0.0, relax=RELAX_NONE, relax_time=0.0, theta=0.0, solver=None,
+                 kgrid: Optional[KGrid] = None):
+        self.fermi_energy, self.beta, self.relax, self.relax_time, self.theta = (fermi_energy, beta, relax,
+                                                                                 relax_time, theta)
+        self.solver = solver or Solver()
+        self.kgrid = kgrid or self.default_grid()
+
+    def default_grid(self) -> KGrid:  # pragma: no cover - overridden
+        raise NotImplementedError
+
+    def model_desc(self) -> ModelDesc:  # pragma: no cover - overridden
+        raise NotImplementedError
+
+    def compute_current_density(self, laser: Laser, time_config: TimeIntegrationConfig, rank: int = 0,
+                                n_ranks: int = 1) -> np.ndarray:
+        """j(t_i), i = 0..N (sources/HHG/Systems/*::compute_current_density)."""
+        with Plan([Job(self.model_desc(), laser, time_config, 0)], [self.kgrid], self.solver, rank, n_ranks) as p:
+            out = p.execute()[0]
+            self.last_stats = p.stats.as_dict()
+        return out
+
+
+class DiracSystem2D(_System):
+    """Massive 2D Dirac cone d = (v qx, v qy, Delta), q = k - A e_theta."""
+    kind = MODEL_DIRAC2D
+
+    def __init__(self, v=1.0, mass=0.1, k_max=3.0, nx=128, ny=64, **kw):
+        self.v, self.mass, self.k_max, self.nx, self.ny = v, mass, k_max, nx, ny
+        super().__init__(**kw)
+
+    def default_grid(self) -> KGrid:
+        return KGrid.box(self.nx, self.ny, -self.k_max, self.k_max, -self.k_max, self.k_max)
+
+    def model_desc(self) -> ModelDesc:
+        return ModelDesc(MODEL_DIRAC2D, self.relax, self.v, self.mass, 0.0, self.theta, self.fermi_energy,
+                         self.beta, self.relax_time)
+
+
+class PiFlux2D(_System):
+    """PiFlux square lattice d = 2t (cos(kx - A), cos ky, 0) over the full BZ."""
+    kind = MODEL_PIFLUX2D
+
+    def __init__(self, hopping=1.0, nx=512, ny=512, **kw):
+        self.hopping, self.nx, self.ny = hopping, nx, ny
+        super().__init__(**kw)
+
+    def default_grid(self) -> KGrid:
+        return KGrid.box(self.nx, self.ny, -math.pi, math.pi, -math.pi, math.pi)
+
+    def model_desc(self) -> ModelDesc:
+        return ModelDesc(MODEL_PIFLUX2D, self.relax, 0.0, 0.0, self.hopping, self.theta, self.fermi_energy,
+                         self.beta, self.relax_time)
+
+
+class Honeycomb(_System):
+    """Honeycomb nearest-neighbour model over the hexagonal BZ (Honeycomb.cpp:225-336)."""
+    kind = MODEL_HONEYCOMB
+
+    def __init__(self, hopping=1.0, nx=128, ny=128, **kw):
+        self.hopping, self.nx, self.ny = hopping, nx, ny
+        super().__init__(**kw)
+
+    def default_grid(self) -> KGrid:
+        return KGrid.hexagonal(self.nx, self.ny)
+
+    def model_desc(self) -> ModelDesc:
+        return ModelDesc(MODEL_HONEYCOMB, self.relax, 0.0, 0.0, self.hopping, self.theta, self.fermi_energy,
+                         self.beta, self.relax_time)
+
+
+# --------------------------------------------------------------------------
+# Host utilities
+# --------------------------------------------------------------------------
+
+def gauss_legendre(n: int):
+    x, w = np.empty(n), np.empty(n)
+    _check(lib().hhg_gauss_legendre(n, _dp(x), _dp(w)))
+    return x, w
+
+
+def trapezoidal_spectrum(j: np.ndarray, time_config: TimeIntegrationConfig, zero_padding: int = 8):
+    """(frequencies, real, imag) of the end-corrected Fourier integral of j."""
+    j = np.ascontiguousarray(j, dtype=np.float64)
+    n = (zero_padding * (time_config.n_measurements + 1)) // 2 + 1
+    f, re, im = np.empty(n), np.empty(n), np.empty(n)
+    td = time_config.desc()
+    _check(lib().hhg_trapezoidal_spectrum(_dp(j), ctypes.byref(td), zero_padding, _dp(f), _dp(re), _dp(im)))
+    return f, re, im

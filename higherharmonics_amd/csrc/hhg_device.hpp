@@ -1,0 +1,489 @@
+// hhg_device.hpp — device-side building blocks of the Bloch hot path (gfx950).
+//
+// One wavefront lane owns one k-point.  Everything is FP64.  The functions
+// here are the MI355X formulation of:
+//   Laser::laser_function            sources/HHG/Laser/Laser.hpp:32-54
+//   CosineLaser / GaussLaser envelope sources/HHG/Laser/CosineLaser.cpp:22-26,
+//                                     sources/HHG/Laser/GaussLaser.cpp:26-28
+//   DoubleLaser A + B(t - delay)      sources/HHG/Laser/DoubleLaser.hpp:63-67
+//   Bloch RHS ds/dt = 2 d x s         sources/HHG/Systems/PiFlux.cpp:89-94,
+//                                     sources/HHG/Systems/Honeycomb.cpp:130-138
+//   relaxation to the initial state   sources/HHG/Systems/DiracSystem.cpp:231-233
+//   odeint RKF78 controller           Boost.Odeint controlled_runge_kutta (call
+//                                     sites sources/HHG/Systems/DiracSystem.cpp:167-173)
+//   current vertices                  sources/HHG/Systems/PiFlux.cpp:831-838,
+//                                     sources/HHG/Systems/Honeycomb.cpp:273-279
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hhg_internal.hpp"
+
+namespace hhg {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kSqrt3 = 1.73205080756887729352744634150587237;
+constexpr double kHalfSqrt3 = 0.866025403784438646763723170752936183;
+constexpr double kDblEps = 2.220446049250313080847263336181640625e-16;  // odeint's less_with_sign epsilon
+
+// Job records are read-only during a launch; referencing them through the
+// constant address space lets every field be fetched with a scalar load when
+// it is used instead of being held in registers.
+using ConstJob = const __attribute__((address_space(4))) JobDev;
+using JobRef = ConstJob&;
+
+__device__ __forceinline__ JobRef job_ref(const JobDev* p) {
+    return *reinterpret_cast<ConstJob*>(reinterpret_cast<uintptr_t>(p));
+}
+
+// ---------------------------------------------------------------------------
+// Pulses: A(t) = amp * envelope * sin(t + cep); B = same shape at t - delay.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double env_cos2(JobRef J, double t) {
+    if (t <= J.t0 || t >= J.t_env_end) return 0.0;
+    return 0.5 * (1.0 - cos((t - J.t0) * J.inv_n));
+}
+
+__device__ __forceinline__ double env_gauss(JobRef J, double t) {
+    const double x = t - J.t0;
+    return exp(-(x * x) * J.inv_2s2);
+}
+
+template <int P>
+__device__ __forceinline__ double pulse_A(JobRef J, double t) {
+    if constexpr (P == PULSE_COS2) {
+        return J.amp_a * env_cos2(J, t) * sin(t + J.cep);
+    } else if constexpr (P == PULSE_GAUSS) {
+        return J.amp_a * env_gauss(J, t) * sin(t + J.cep);
+    } else if constexpr (P == PULSE_DOUBLE_GAUSS) {
+        const double tb = t - J.delay;
+        return J.amp_a * env_gauss(J, t) * sin(t + J.cep) + J.amp_b * env_gauss(J, tb) * sin(tb + J.cep);
+    } else {
+        const double tb = t - J.delay;
+        return J.amp_a * env_cos2(J, t) * sin(t + J.cep) + J.amp_b * env_cos2(J, tb) * sin(tb + J.cep);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-step pulse cache.  At the start of a step (time t) the carrier and the
+// cos^2 envelope phase are evaluated once with full-range sincos; the stage
+// times t + c_i h then follow by exact angle addition with a small-argument
+// series.  The series needs |c_i h| <= 0.25 and |c_i h / n| <= 0.25; steps
+// never exceed the sampling interval, and the plan rejects grids with
+// Delta > 0.25, so the bound always holds.  Gaussian envelopes are evaluated
+// directly with exp at every stage.
+// ---------------------------------------------------------------------------
+constexpr double kSmallArg = 0.25;
+
+// sin x and cos x for |x| <= 0.25 (Taylor; remainder < 3e-20 relative).
+__device__ __forceinline__ void small_sincos(double x, double& sn, double& cs) {
+    const double x2 = x * x;
+    sn = x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0 +
+                                                                                          x2 * (-1.0 / 39916800.0))))));
+    cs = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 + x2 * (-1.0 / 3628800.0 +
+                                                                                                x2 * (1.0 / 479001600.0))))));
+}
+
+// One pulse component (A or B) at a step start.
+struct PulseComp {
+    double S, C;    // sin, cos of the carrier phase
+    double E0, E1;  // cos^2 only: sin, cos of the envelope phase
+};
+
+template <int P>
+struct PulseCache {
+    static constexpr bool kGauss = (P == PULSE_GAUSS || P == PULSE_DOUBLE_GAUSS);
+    static constexpr bool kDouble = (P == PULSE_DOUBLE_GAUSS || P == PULSE_DOUBLE_COS2);
+    double t;
+    PulseComp a, b;
+
+    __device__ __forceinline__ void prep_comp(JobRef J, double tc, PulseComp& c) {
+        sincos(tc + J.cep, &c.S, &c.C);
+        if constexpr (!kGauss) sincos((tc - J.t0) * J.inv_n, &c.E0, &c.E1);
+    }
+
+    __device__ __forceinline__ void prepare(JobRef J, double t_) {
+        t = t_;
+        prep_comp(J, t, a);
+        if constexpr (kDouble) prep_comp(J, t - J.delay, b);
+    }
+
+    // envelope * carrier of one component at tc + dlt, with (sd, cd) =
+    // (sin dlt, cos dlt)
+    __device__ __forceinline__ double comp_at(JobRef J, const PulseComp& c, double tc, double dlt, double sd,
+                                              double cd) const {
+        const double carrier = c.S * cd + c.C * sd;
+        const double tt = tc + dlt;
+        if constexpr (kGauss) {
+            const double x = tt - J.t0;
+            return exp(-(x * x) * J.inv_2s2) * carrier;
+        } else {
+            double sn, cs;
+            small_sincos(dlt * J.inv_n, sn, cs);
+            const double ce = c.E1 * cs - c.E0 * sn;  // cos(envelope phase + dlt / n)
+            const double v = 0.5 * (1.0 - ce) * carrier;
+            return (tt <= J.t0 || tt >= J.t_env_end) ? 0.0 : v;
+        }
+    }
+
+    // A(t + dlt)
+    __device__ __forceinline__ double at(JobRef J, double dlt) const {
+        double sd, cd;
+        small_sincos(dlt, sd, cd);
+        double r = J.amp_a * comp_at(J, a, t, dlt, sd, cd);
+        if constexpr (kDouble) r += J.amp_b * comp_at(J, b, t - J.delay, dlt, sd, cd);
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Models.  Honeycomb g(q) = e^{i qx} + 2 e^{-i qx/2} cos(sqrt3/2 qy) is the
+// nearest-neighbour sum of Honeycomb.cpp:44-50 with the two oblique bonds
+// combined, so one sincos(qx/2) replaces three complex exponentials; with the
+// field along x (theta = 0) the cos(sqrt3/2 ky) factor is a per-lane constant.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double fermi(double e, double beta) {
+    if (!(beta > 0.0)) return e != 0.0 ? (e < 0.0 ? 1.0 : 0.0) : 0.5;
+    return 1.0 / (1.0 + exp(beta * e));
+}
+
+// Lattice models take the field along x (theta = 0), as the reference's
+// Honeycomb (Honeycomb.hpp:88-90); the Dirac model accepts any theta.
+template <int M, int P, int R>
+struct Bloch {
+    JobRef J;
+    double kx, ky;
+    double s0[3];
+    double cy;  // theta = 0 constant: PiFlux 2t cos ky, honeycomb cos(sqrt3/2 ky)
+
+    __device__ __forceinline__ Bloch(JobRef job, double kx_, double ky_) : J(job), kx(kx_), ky(ky_) {
+        if constexpr (M == MODEL_PIFLUX2D) cy = 2.0 * J.hop * cos(ky);
+        else if constexpr (M == MODEL_HONEYCOMB) cy = cos(kHalfSqrt3 * ky);
+        else cy = 0.0;
+    }
+
+    // d(k - A e_theta)
+    __device__ __forceinline__ void d_of(double A, double d[3]) const {
+        if constexpr (M == MODEL_DIRAC2D) {
+            d[0] = J.v * (kx - A * J.ct);
+            d[1] = J.v * (ky - A * J.st);
+            d[2] = J.mass;
+        } else if constexpr (M == MODEL_PIFLUX2D) {
+            d[0] = 2.0 * J.hop * cos(kx - A);
+            d[1] = cy;
+            d[2] = 0.0;
+        } else {
+            double sh, ch;
+            sincos(0.5 * (kx - A), &sh, &ch);
+            const double cq = (ch - sh) * (ch + sh);  // cos qx
+            const double sq = 2.0 * sh * ch;          // sin qx
+            d[0] = J.hop * (cq + 2.0 * ch * cy);      // t Re g
+            d[1] = -J.hop * (sq - 2.0 * sh * cy);     // -t Im g
+            d[2] = 0.0;
+        }
+    }
+
+    // Initial Bloch vector: Honeycomb::ic_sigma (Honeycomb.cpp:355-361) with
+    // the occupations of Honeycomb.cpp:347-353; the 2D Dirac/PiFlux models use
+    // the band-diagonal thermal state (n_up - n_low) d^, = -d^ at T = 0, E_F = 0.
+    __device__ __forceinline__ void initial(double s[3]) {
+        double d[3];
+        d_of(0.0, d);
+        const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if constexpr (M == MODEL_HONEYCOMB) {
+            const double g = e / J.hop;
+            const double a2 = fermi(J.fermi - J.hop * g, J.beta);
+            const double b2 = fermi(J.fermi + J.hop * g, J.beta);
+            const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
+            if (e == 0.0) {
+                s[0] = s[1] = 0.0;
+            } else {
+                s[0] = -diff * (d[0] / J.hop) / g;
+                s[1] = diff * (-d[1] / J.hop) / g;
+            }
+            s[2] = -prod;
+        } else {
+            if (e == 0.0) {
+                s[0] = s[1] = s[2] = 0.0;
+            } else {
+                const double nu = fermi(e - J.fermi, J.beta);
+                const double nl = fermi(-e - J.fermi, J.beta);
+                const double f = (nu - nl) / e;
+                s[0] = f * d[0];
+                s[1] = f * d[1];
+                s[2] = f * d[2];
+            }
+        }
+        s0[0] = s[0];
+        s0[1] = s[1];
+        s0[2] = s[2];
+    }
+
+    PulseCache<P> pc;
+
+    // start of a step at time t (stage times are t + c_i h)
+    __device__ __forceinline__ void prepare(double t) { pc.prepare(J, t); }
+
+    // A(t + dlt) (t = the prepared step start)
+    __device__ __forceinline__ double pulse_at(double dlt) const { return pc.at(J, dlt); }
+
+    // ds/dt = 2 d(k - A) x s - relaxation
+    __device__ __forceinline__ void rhs_A(const double s[3], double A, double out[3]) const {
+        double d[3];
+        d_of(A, d);
+        const double wx = 2.0 * d[0], wy = 2.0 * d[1], wz = 2.0 * d[2];
+        out[0] = wy * s[2] - wz * s[1];
+        out[1] = wz * s[0] - wx * s[2];
+        out[2] = wx * s[1] - wy * s[0];
+        if constexpr (R == RELAX_TO_INITIAL) {
+            const double r = J.relax_rate;
+            out[0] -= (s[0] - s0[0]) * r;
+            out[1] -= (s[1] - s0[1]) * r;
+            out[2] -= (s[2] - s0[2]) * r;
+        } else if constexpr (R == RELAX_T2) {
+            const double r = J.relax_rate;
+            const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double proj = dd > 0.0 ? (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) / dd : 0.0;
+            out[0] -= (s[0] - proj * d[0]) * r;
+            out[1] -= (s[1] - proj * d[1]) * r;
+            out[2] -= (s[2] - proj * d[2]) * r;
+        }
+    }
+
+    // current vertex g(s, A) along the polarisation
+    __device__ __forceinline__ double vertex(double A, const double s[3]) const {
+        if constexpr (M == MODEL_DIRAC2D) {
+            return -J.v * (J.ct * s[0] + J.st * s[1]);
+        } else if constexpr (M == MODEL_PIFLUX2D) {
+            return sin(kx - A) * s[0];
+        } else {
+            // Gx = sum_delta delta_x e^{i q.delta} = e^{i qx} - e^{-i qx/2} cos(sqrt3/2 qy)
+            // (shifted_grad_gamma, Honeycomb.cpp:52-61), contracted as Honeycomb.cpp:273-279
+            double sh, ch;
+            sincos(0.5 * (kx - A), &sh, &ch);
+            const double cq = (ch - sh) * (ch + sh), sq = 2.0 * sh * ch;
+            return (cq - ch * cy) * s[0] + (sq + sh * cy) * s[1];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Fehlberg 7(8) (E. Fehlberg, NASA TR R-287, 1968).  Stages are fully
+// unrolled so every coefficient is an immediate and zero entries vanish.
+// ---------------------------------------------------------------------------
+struct F78 {
+    static constexpr double c[13] = {0.0,       2.0 / 27.0, 1.0 / 9.0, 1.0 / 6.0, 5.0 / 12.0, 1.0 / 2.0, 5.0 / 6.0,
+                                     1.0 / 6.0, 2.0 / 3.0,  1.0 / 3.0, 1.0,       0.0,        1.0};
+    static constexpr double a[13][12] = {
+        {0},
+        {2.0 / 27.0},
+        {1.0 / 36.0, 1.0 / 12.0},
+        {1.0 / 24.0, 0.0, 1.0 / 8.0},
+        {5.0 / 12.0, 0.0, -25.0 / 16.0, 25.0 / 16.0},
+        {1.0 / 20.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 5.0},
+        {-25.0 / 108.0, 0.0, 0.0, 125.0 / 108.0, -65.0 / 27.0, 125.0 / 54.0},
+        {31.0 / 300.0, 0.0, 0.0, 0.0, 61.0 / 225.0, -2.0 / 9.0, 13.0 / 900.0},
+        {2.0, 0.0, 0.0, -53.0 / 6.0, 704.0 / 45.0, -107.0 / 9.0, 67.0 / 90.0, 3.0},
+        {-91.0 / 108.0, 0.0, 0.0, 23.0 / 108.0, -976.0 / 135.0, 311.0 / 54.0, -19.0 / 60.0, 17.0 / 6.0,
+         -1.0 / 12.0},
+        {2383.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0, -301.0 / 82.0, 2133.0 / 4100.0, 45.0 / 82.0,
+         45.0 / 164.0, 18.0 / 41.0},
+        {3.0 / 205.0, 0.0, 0.0, 0.0, 0.0, -6.0 / 41.0, -3.0 / 205.0, -3.0 / 41.0, 3.0 / 41.0, 6.0 / 41.0, 0.0},
+        {-1777.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0, -289.0 / 82.0, 2193.0 / 4100.0, 51.0 / 82.0,
+         33.0 / 164.0, 12.0 / 41.0, 0.0, 1.0},
+    };
+    // 8th-order (propagated) weights; the embedded 7th-order solution differs
+    // by 41/840 (k1 + k11 - k12 - k13).
+    static constexpr double b8[13] = {0.0,         0.0,         0.0, 0.0,          0.0,         34.0 / 105.0, 9.0 / 35.0,
+                                      9.0 / 35.0,  9.0 / 280.0, 9.0 / 280.0, 0.0, 41.0 / 840.0, 41.0 / 840.0};
+};
+
+struct StepResult {
+    double err;  // max_i |err_i| / (atol + rtol (|x_i| + h |dxdt_i|))
+};
+
+// One embedded RKF78 attempt from (s, t) with step h; k1 = f(s, t) given.
+// Like odeint's try_step for a non-FSAL stepper, each attempt evaluates the
+// derivative at its start, so nothing but the state crosses attempts.
+template <class F>
+__device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double t, double h, double snew[3],
+                                                double rtol, double atol) {
+    f.prepare(t);
+    // The pulse does not depend on the state: A at the stage times is
+    // evaluated first (c_7 = c_3, c_11 = c_0 = 0, c_12 = c_10 are shared).
+    double As[13];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) {
+        if (i == 7) As[i] = As[3];
+        else if (i == 11) As[i] = As[0];
+        else if (i == 12) As[i] = As[10];
+        else As[i] = f.pulse_at(F78::c[i] * h);
+    }
+    double k[13][3];
+    f.rhs_A(s, As[0], k[0]);
+    const double k1[3] = {k[0][0], k[0][1], k[0][2]};
+#pragma unroll
+    for (int i = 1; i < 13; ++i) {
+        double x[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < i; ++j)
+                if (F78::a[i][j] != 0.0) acc = fma(F78::a[i][j], k[j][q], acc);
+            x[q] = fma(h, acc, s[q]);
+        }
+        f.rhs_A(x, As[i], k[i]);
+    }
+    double err = 0.0;
+    bool nan = false;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 13; ++j)
+            if (F78::b8[j] != 0.0) acc = fma(F78::b8[j], k[j][q], acc);
+        snew[q] = fma(h, acc, s[q]);
+        const double e = h * (41.0 / 840.0) * ((k[0][q] + k[10][q]) - (k[11][q] + k[12][q]));
+        const double sc = atol + rtol * (fabs(s[q]) + fabs(h) * fabs(k1[q]));
+        const double r = fabs(e) / sc;
+        nan |= (r != r);
+        err = r > err ? r : err;
+    }
+    return nan ? __builtin_nan("") : err;
+}
+
+// integrate_adaptive over [t0, t1] (odeint semantics: absolute-epsilon loop
+// and clip tests, reject -> dt *= max(0.9 err^(-1/6), 0.2), accept and for
+// err < 0.5 grow dt *= 0.9 max(err, 5^-8)^(-1/8)).  h is the starting step
+// and on return the controller's proposal; unclipped the step that was in
+// force before the final clip.  status: 0 ok, 1 controller gave up, 2 the
+// state became non-finite.
+template <class F>
+__device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, double t1, double& h,
+                                               double& unclipped, const SolverDev& sp, uint32_t& n_att,
+                                               uint32_t& n_rej, int& status) {
+    double t = t0;
+    bool fresh = true;
+    int tries = 0;
+    for (;;) {
+        if (fresh) {
+            if (!(t1 - t > kDblEps)) break;
+            unclipped = h;
+            if ((t + h) - t1 > kDblEps) h = t1 - t;
+            tries = 0;
+            fresh = false;
+        }
+        double snew[3];
+        const double err = rkf78_attempt(f, s, t, h, snew, sp.rtol, sp.atol);
+        ++n_att;
+        if (err > 1.0) {
+            ++n_rej;
+            double fac = 0.9 * pow(err, -1.0 / 6.0);
+            fac = fac > 0.2 ? fac : 0.2;
+            h *= fac;
+            if (++tries > sp.max_fail || !(h > 0.0)) {
+                status = 1;
+                break;
+            }
+        } else {
+            t += h;
+            s[0] = snew[0];
+            s[1] = snew[1];
+            s[2] = snew[2];
+            if (err < 0.5) {
+                const double e = err > (1.0 / 390625.0) ? err : (1.0 / 390625.0);
+                h *= 0.9 / sqrt(sqrt(sqrt(e)));
+            }
+            fresh = true;
+            if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) {
+                status = 2;
+                break;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Magnus 4 (two Gauss nodes) with the closed-form SO(3) exponential
+// (Rodrigues, cf. sources/HHG/Systems/DiracDetail/MagnusMatrix.cpp:64-74, here
+// with the small-angle series instead of dividing by theta^2).
+// ---------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ void magnus4_step(const F& f, double s[3], double A1, double A2, double h) {
+    double w1[3], w2[3];
+    f.d_of(A1, w1);
+    f.d_of(A2, w2);
+    const double a = h, b = (kSqrt3 / 12.0) * 4.0 * h * h;  // omega = 2 d -> w2 x w1 scales by 4
+    double O[3];
+    O[0] = a * (w1[0] + w2[0]) + b * (w2[1] * w1[2] - w2[2] * w1[1]);
+    O[1] = a * (w1[1] + w2[1]) + b * (w2[2] * w1[0] - w2[0] * w1[2]);
+    O[2] = a * (w1[2] + w2[2]) + b * (w2[0] * w1[1] - w2[1] * w1[0]);
+    const double th2 = O[0] * O[0] + O[1] * O[1] + O[2] * O[2];
+    double sa, sb;  // sin(th)/th, (1 - cos th)/th^2
+    if (th2 < 1e-2) {
+        sa = 1.0 + th2 * (-1.0 / 6.0 + th2 * (1.0 / 120.0 + th2 * (-1.0 / 5040.0 + th2 * (1.0 / 362880.0 +
+                                                                                            th2 * (-1.0 / 39916800.0)))));
+        sb = 0.5 + th2 * (-1.0 / 24.0 + th2 * (1.0 / 720.0 + th2 * (-1.0 / 40320.0 + th2 * (1.0 / 3628800.0 +
+                                                                                             th2 * (-1.0 / 479001600.0)))));
+    } else {
+        const double th = sqrt(th2);
+        double sn, cs;
+        sincos(0.5 * th, &sn, &cs);
+        sa = 2.0 * sn * cs / th;
+        sb = 2.0 * sn * sn / th2;
+    }
+    const double x0 = O[1] * s[2] - O[2] * s[1];
+    const double x1 = O[2] * s[0] - O[0] * s[2];
+    const double x2 = O[0] * s[1] - O[1] * s[0];
+    const double y0 = O[1] * x2 - O[2] * x1;
+    const double y1 = O[2] * x0 - O[0] * x2;
+    const double y2 = O[0] * x1 - O[1] * x0;
+    s[0] += sa * x0 + sb * y0;
+    s[1] += sa * x1 + sb * y1;
+    s[2] += sa * x2 + sb * y2;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront reduction and exact cross-wave accumulation.
+// ---------------------------------------------------------------------------
+// Butterfly over 64 lanes; IEEE addition is commutative, so every lane ends
+// with the bitwise-identical total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Signed 128-bit fixed point with 80 fractional bits: x -> round-toward-minus
+// of x 2^80 split as hi 2^64 + lo.  Integer addition is associative, so the
+// accumulated j is bitwise independent of the order in which waves finish.
+constexpr double kFixScaleHi = 65536.0;                 // 2^16
+constexpr double kFixScaleLo = 18446744073709551616.0;  // 2^64
+constexpr double kFixLimit = 4611686018427387904.0;     // 2^62 (|x 2^16| bound)
+
+__device__ __forceinline__ bool fix_accumulate(unsigned long long* acc, double x) {
+    const double y = x * kFixScaleHi;
+    if (!(fabs(y) < kFixLimit)) return false;
+    const double hi_d = floor(y);
+    const double frac = y - hi_d;  // exact, in [0, 1)
+    const unsigned long long lo = (unsigned long long)(frac * kFixScaleLo);
+    const long long hi = (long long)hi_d;
+    const unsigned long long old = atomicAdd(acc, lo);
+    const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
+    atomicAdd(acc + 1, (unsigned long long)hi + carry);
+    return true;
+}
+
+__device__ __forceinline__ double fix_to_double(const unsigned long long* acc) {
+    const long long hi = (long long)acc[1];
+    const unsigned long long lo = acc[0];
+    return (double)hi * (1.0 / kFixScaleHi) + (double)lo * (1.0 / (kFixScaleHi * kFixScaleLo));
+}
+
+}  // namespace hhg

@@ -1,0 +1,318 @@
+// hhg_host.cpp — host utilities of the C ABI (no GPU needed): Gauss-Legendre
+// nodes, Brillouin-zone grids, host pulse evaluation and the end-corrected
+// spectrum.
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hhg_amd.h"
+#include "hhg_host.hpp"
+
+namespace hhg {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+hhg_status fail(hhg_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+const char* last_error() { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre nodes: Newton iteration on P_n in long double from the
+// Tricomi initial guess, weights 2 / ((1 - x^2) P_n'(x)^2).  Replaces the
+// Boost-generated tables of sources/HHG/thread_gauss.hpp:21-311, which stop
+// at N = 960.
+// ---------------------------------------------------------------------------
+void gauss_legendre(int n, double* x, double* w) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const int half = n / 2;
+    for (int i = 1; i <= half; ++i) {
+        // Tricomi: x_i ~ (1 - (n-1)/(8 n^3)) cos(pi (4i - 1) / (4n + 2)), the i-th largest root
+        const long double th = pi * (4.0L * i - 1.0L) / (4.0L * n + 2.0L);
+        long double z = (1.0L - (n - 1.0L) / (8.0L * n * n * n)) * cosl(th);
+        long double dp = 0.0L;
+        for (int it = 0; it < 64; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) p0 = 1.0L, p1 = z;
+            dp = n * (p0 - z * p1) / (1.0L - z * z);
+            const long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L * fabsl(z) + 1e-30L) {
+                p0 = 1.0L;
+                p1 = z;
+                for (int k = 2; k <= n; ++k) {
+                    const long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                dp = n * (p0 - z * p1) / (1.0L - z * z);
+                break;
+            }
+        }
+        const long double wi = 2.0L / ((1.0L - z * z) * dp * dp);
+        x[n - i] = (double)z;
+        w[n - i] = (double)wi;
+        x[i - 1] = -(double)z;
+        w[i - 1] = (double)wi;
+    }
+    if (n % 2 == 1) {
+        // centre node: P_n'(0) from the recurrence of P_{n-1}(0)
+        long double p0 = 1.0L, p1 = 0.0L;
+        for (int k = 2; k <= n; ++k) {
+            const long double p2 = (-(k - 1.0L) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        const long double dp = n * p0;  // P_n'(0) = n P_{n-1}(0)
+        x[half] = 0.0;
+        w[half] = (double)(2.0L / (dp * dp));
+    }
+}
+
+int64_t kgrid_size(const hhg_kgrid_desc& g) {
+    if (g.kind == HHG_KGRID_EXPLICIT) return g.n_points;
+    return (int64_t)g.nx * (int64_t)g.ny;
+}
+
+hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* w) {
+    if (g.kind == HHG_KGRID_EXPLICIT) {
+        if (g.n_points < 0 || (g.n_points > 0 && (!g.kx || !g.ky || !g.w)))
+            return fail(HHG_ERROR_INVALID_ARGUMENT, "explicit k-grid needs kx, ky, w");
+        std::memcpy(kx, g.kx, sizeof(double) * g.n_points);
+        std::memcpy(ky, g.ky, sizeof(double) * g.n_points);
+        std::memcpy(w, g.w, sizeof(double) * g.n_points);
+        return HHG_SUCCESS;
+    }
+    if (g.nx <= 0 || g.ny <= 0) return fail(HHG_ERROR_INVALID_ARGUMENT, "k-grid needs nx, ny > 0");
+    std::vector<double> gx(g.nx), wx(g.nx), gy(g.ny), wy(g.ny);
+    gauss_legendre(g.nx, gx.data(), wx.data());
+    gauss_legendre(g.ny, gy.data(), wy.data());
+    if (g.kind == HHG_KGRID_GL_BOX) {
+        if (!(g.x1 > g.x0) || !(g.y1 > g.y0)) return fail(HHG_ERROR_INVALID_ARGUMENT, "k-grid box must have x1 > x0, y1 > y0");
+        const double hx = 0.5 * (g.x1 - g.x0), cx = 0.5 * (g.x1 + g.x0);
+        const double hy = 0.5 * (g.y1 - g.y0), cy = 0.5 * (g.y1 + g.y0);
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j) {
+                const int64_t k = (int64_t)i * g.ny + j;
+                kx[k] = hx * gx[i] + cx;
+                ky[k] = hy * gy[j] + cy;
+                w[k] = (wx[i] * hx) * (wy[j] * hy);
+            }
+        return HHG_SUCCESS;
+    }
+    if (g.kind == HHG_KGRID_GL_HEX) {
+        // Honeycomb.cpp:229-268: x on [-2pi/3, 2pi/3], y on +-(4pi/(3 sqrt3) - |x|/sqrt3)
+        const double pi = 3.14159265358979323846;
+        const double sqrt3 = 1.73205080756887729353;
+        const double hx = 2.0 * pi / 3.0;
+        for (int i = 0; i < g.nx; ++i) {
+            const double x = hx * gx[i];
+            const double ymax = 4.0 * pi / (3.0 * sqrt3) - std::fabs(x) / sqrt3;
+            for (int j = 0; j < g.ny; ++j) {
+                const int64_t k = (int64_t)i * g.ny + j;
+                kx[k] = x;
+                ky[k] = ymax * gy[j];
+                w[k] = (hx * wx[i]) * (ymax * wy[j]);
+            }
+        }
+        return HHG_SUCCESS;
+    }
+    return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown k-grid kind");
+}
+
+// ---------------------------------------------------------------------------
+// Host pulse evaluation (Laser::laser_function, sources/HHG/Laser/Laser.hpp:32).
+// ---------------------------------------------------------------------------
+static double env_cos2_h(const hhg_pulse_desc& p, double t) {
+    const double te = p.t0 + 2.0 * 3.14159265358979323846 * p.n_cycles;
+    if (t <= p.t0 || t >= te) return 0.0;
+    return 0.5 * (1.0 - std::cos((t - p.t0) / p.n_cycles));
+}
+
+static double env_gauss_h(const hhg_pulse_desc& p, double t) {
+    const double x = t - p.t0;
+    return std::exp(-(x * x) / (2.0 * p.sigma * p.sigma));
+}
+
+double pulse_eval(const hhg_pulse_desc& p, double t) {
+    switch (p.kind) {
+    case HHG_PULSE_COS2: return p.amp_a * env_cos2_h(p, t) * std::sin(t + p.cep);
+    case HHG_PULSE_GAUSS: return p.amp_a * env_gauss_h(p, t) * std::sin(t + p.cep);
+    case HHG_PULSE_DOUBLE_GAUSS: {
+        const double tb = t - p.delay;
+        return p.amp_a * env_gauss_h(p, t) * std::sin(t + p.cep) + p.amp_b * env_gauss_h(p, tb) * std::sin(tb + p.cep);
+    }
+    case HHG_PULSE_DOUBLE_COS2: {
+        const double tb = t - p.delay;
+        return p.amp_a * env_cos2_h(p, t) * std::sin(t + p.cep) + p.amp_b * env_cos2_h(p, tb) * std::sin(tb + p.cep);
+    }
+    }
+    return NAN;
+}
+
+// ---------------------------------------------------------------------------
+// FFT: iterative radix-2 for powers of two, Bluestein's chirp-z for any n.
+// ---------------------------------------------------------------------------
+using cplx = std::complex<double>;
+
+static void fft_pow2(std::vector<cplx>& a, bool inverse) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    const double sgn = inverse ? 1.0 : -1.0;
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t half = len / 2;
+        // twiddles computed directly per index to avoid recurrence drift
+        std::vector<cplx> tw(half);
+        for (size_t k = 0; k < half; ++k) {
+            const double ang = sgn * 2.0 * 3.14159265358979323846 * (double)k / (double)len;
+            tw[k] = cplx(std::cos(ang), std::sin(ang));
+        }
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < half; ++k) {
+                const cplx u = a[i + k], v = a[i + k + half] * tw[k];
+                a[i + k] = u + v;
+                a[i + k + half] = u - v;
+            }
+    }
+}
+
+// X_k = sum_n x_n exp(-2 pi i k n / n_total), k = 0..n_total-1
+static std::vector<cplx> dft_any(const std::vector<double>& x) {
+    const size_t n = x.size();
+    size_t m = 1;
+    while (m < 2 * n - 1) m <<= 1;
+    // chirp c_k = exp(-i pi k^2 / n), with k^2 reduced mod 2n exactly
+    std::vector<cplx> chirp(n);
+    for (size_t k = 0; k < n; ++k) {
+        const unsigned long long kk = ((unsigned long long)k * (unsigned long long)k) % (2ull * n);
+        const double ang = -3.14159265358979323846 * (double)kk / (double)n;
+        chirp[k] = cplx(std::cos(ang), std::sin(ang));
+    }
+    std::vector<cplx> a(m, 0.0), b(m, 0.0);
+    for (size_t k = 0; k < n; ++k) a[k] = x[k] * chirp[k];
+    b[0] = std::conj(chirp[0]);
+    for (size_t k = 1; k < n; ++k) b[k] = b[m - k] = std::conj(chirp[k]);
+    fft_pow2(a, false);
+    fft_pow2(b, false);
+    for (size_t k = 0; k < m; ++k) a[k] *= b[k];
+    fft_pow2(a, true);
+    std::vector<cplx> X(n);
+    for (size_t k = 0; k < n; ++k) X[k] = a[k] / (double)m * chirp[k];
+    return X;
+}
+
+// TrapezoidalFFT::compute (sources/HHG/Fourier/TrapezoidalFFT.cpp:11-49) on j
+// zero-padded to zero_padding (N+1) samples (sources/hhg.cpp:277-282).  The
+// reference's conventions are kept: the end correction reads the last element
+// of the padded input, the transform term carries delta_t twice and the phase
+// factor is exp(+i w t_begin) next to a forward exp(-i ...) transform.
+hhg_status trapezoidal_spectrum(const double* j, const hhg_time_desc& tc, int zero_padding, double* freq, double* re,
+                                double* im) {
+    if (tc.n_measurements <= 0 || zero_padding <= 0 || !(tc.t_end > tc.t_begin))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "spectrum needs n_measurements > 0, zero_padding > 0, t_end > t_begin");
+    const int64_t n_in = (int64_t)zero_padding * (tc.n_measurements + 1);
+    std::vector<double> x(n_in, 0.0);
+    for (int i = 0; i <= tc.n_measurements; ++i) x[i] = j[i];
+    const std::vector<cplx> X = dft_any(x);
+    const int64_t n_fft = n_in / 2 + 1;
+    const double delta = (tc.t_end - tc.t_begin) / tc.n_measurements;
+    const double tiny = 5.684341886080802e-14;  // is_zero threshold, GlobalDefinitions.hpp:57-71
+    for (int64_t i = 0; i < n_fft; ++i) {
+        const double f = 2.0 * (double)i * 3.14159265358979323846 / ((double)n_in * delta);
+        const double th = f * delta;
+        double W;
+        cplx a0;
+        if (std::fabs(th) < tiny) {
+            W = 1.0;
+            a0 = cplx(-0.5, 0.0);
+        } else {
+            W = 2.0 * (1.0 - std::cos(th)) / (th * th);
+            a0 = cplx(-(1.0 - std::cos(th)), th - std::sin(th)) / (th * th);
+        }
+        const cplx corr = a0 * x[0] + std::conj(a0) * std::polar(1.0, f * (tc.t_end - tc.t_begin)) * x[n_in - 1];
+        const cplx out = std::polar(delta * W, f * tc.t_begin) * X[i] + corr;
+        freq[i] = f;
+        re[i] = delta * out.real();
+        im[i] = delta * out.imag();
+    }
+    return HHG_SUCCESS;
+}
+
+}  // namespace hhg
+
+// ---------------------------------------------------------------------------
+// C ABI: host utilities
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int32_t hhg_abi_version(void) { return HHG_AMD_ABI_VERSION; }
+
+const char* hhg_last_error_string(void) { return hhg::last_error(); }
+
+hhg_status hhg_gauss_legendre(int32_t n, double* x, double* w) {
+    hhg::clear_error();
+    if (n <= 0 || !x || !w) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_gauss_legendre: n > 0 and outputs required");
+    hhg::gauss_legendre(n, x, w);
+    return HHG_SUCCESS;
+}
+
+int64_t hhg_kgrid_size(const hhg_kgrid_desc* g) { return g ? hhg::kgrid_size(*g) : -1; }
+
+hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, double* w) {
+    hhg::clear_error();
+    if (!g || !kx || !ky || !w) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_kgrid_build: null argument");
+    return hhg::kgrid_build(*g, kx, ky, w);
+}
+
+hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n) {
+    hhg::clear_error();
+    if (!p || (n > 0 && (!t || !A))) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_pulse_evaluate: null argument");
+    if (p->kind < 0 || p->kind > 3) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "unknown pulse kind");
+    for (int64_t i = 0; i < n; ++i) A[i] = hhg::pulse_eval(*p, t[i]);
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_trapezoidal_spectrum(const double* j, const hhg_time_desc* time, int32_t zero_padding, double* freq,
+                                    double* re, double* im) {
+    hhg::clear_error();
+    if (!j || !time || !freq || !re || !im)
+        return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_trapezoidal_spectrum: null argument");
+    return hhg::trapezoidal_spectrum(j, *time, zero_padding, freq, re, im);
+}
+
+void hhg_accumulators_to_double(const uint64_t* acc, int64_t n, double* j) {
+    for (int64_t i = 0; i < n; ++i) {
+        const long long hi = (long long)acc[2 * i + 1];
+        j[i] = (double)hi * (1.0 / 65536.0) + (double)acc[2 * i] * (1.0 / (65536.0 * 18446744073709551616.0));
+    }
+}
+
+void hhg_accumulators_add(uint64_t* total, const uint64_t* part, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t lo = total[2 * i] + part[2 * i];
+        const uint64_t carry = lo < total[2 * i] ? 1u : 0u;
+        total[2 * i] = lo;
+        total[2 * i + 1] += part[2 * i + 1] + carry;
+    }
+}
+
+}  // extern "C"

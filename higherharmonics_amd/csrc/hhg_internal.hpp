@@ -1,0 +1,69 @@
+// hhg_internal.hpp — layouts shared by the HIP kernels and the plan code.
+//
+// Device memory of a plan (all FP64 unless stated):
+//   kx, ky, w        : SoA k-points of every k-grid, concatenated
+//   perm (int64)     : optional lane -> k-point map per job (cost grouping)
+//   jobs (JobDev)    : one record per job
+//   items (int2)     : work items {job, chunk}; one item = 64 consecutive
+//                      k-points of one job = one wavefront
+//   At               : A(t_i), i = 0..N, per job (vertex evaluation)
+//   Am               : A at the two Gauss nodes of every Magnus substep
+//   partial          : one row of N+1 wave sums per item, per job contiguous
+//   seg_sum/seg_comp : level-1 compensated sums over item segments
+//   j                : the result, sum over jobs of N+1 samples
+#pragma once
+#include <stdint.h>
+
+namespace hhg {
+
+enum : int { MODEL_DIRAC2D = 0, MODEL_PIFLUX2D = 1, MODEL_HONEYCOMB = 2, N_MODELS = 3 };
+enum : int { PULSE_COS2 = 0, PULSE_GAUSS = 1, PULSE_DOUBLE_GAUSS = 2, PULSE_DOUBLE_COS2 = 3, N_PULSES = 4 };
+enum : int { RELAX_NONE = 0, RELAX_TO_INITIAL = 1, RELAX_T2 = 2 };
+enum : int { PROPOSAL_RESTART = 0, PROPOSAL_CARRY = 1, PROPOSAL_LEARNED = 2 };
+
+constexpr int WAVE = 64;
+
+struct JobDev {
+    // model
+    int32_t model, relax;
+    double v, mass, hop;
+    double ct, st;        // cos/sin of the polarisation angle
+    double fermi, beta;   // beta <= 0: T = 0
+    double relax_rate;    // 1 / relax_time
+    // pulse
+    int32_t pulse, theta_zero;
+    double amp_a, amp_b;
+    double inv_n;         // 1 / n_cycles (cos^2)
+    double t_env_end;     // t0 + 2 pi n_cycles (cos^2)
+    double inv_2s2;       // 1 / (2 sigma^2) (Gaussian)
+    double t0, delay, cep;
+    // time grid
+    double t_begin, delta;
+    int32_t n_meas, n_sub;
+    int32_t magnus_sub, pad0;
+    // k-grid slice and output placement
+    int64_t k_base, n_k;
+    int64_t row_base;     // first partial row double of this job
+    int64_t sample_base;  // offset of this job in At and j
+    int64_t magnus_base;  // offset of this job in Am
+    int64_t item_base;    // first item of this job (items are job-major before reordering)
+    int32_t n_items, pad1;
+};
+
+struct Counters {
+    unsigned long long attempts;
+    unsigned long long rejects;
+    unsigned long long failed;
+    unsigned long long nonfinite;
+    unsigned long long fallbacks;
+    unsigned long long queue;  // work-queue head
+    unsigned long long pad[2];
+};
+
+struct SolverDev {
+    double rtol, atol;
+    int32_t max_fail;
+    int32_t proposal;
+};
+
+}  // namespace hhg

@@ -1,0 +1,560 @@
+// hhg_plan.hip — plans: validation, k-grid/job tables in HBM, launches,
+// statistics, autotuning and graph capture behind the C ABI.
+//
+// A plan replaces the reference's Dispatcher (sources/HHG/Dispatch/Dispatcher.hpp:13-33):
+// it is built once from a description (the Dispatcher constructors,
+// DiracDispatcher.cpp:25-62, PiFluxDispatcher.cpp:46-105,
+// HoneycombDispatcher.cpp:28-56) and then computes j(t) (Dispatcher::compute).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hhg_amd.h"
+#include "hhg_host.hpp"
+#include "hhg_internal.hpp"
+#include "hhg_kernels.hpp"
+
+using hhg::fail;
+
+namespace {
+
+constexpr int kCounterWords = (int)(sizeof(hhg::Counters) / sizeof(unsigned long long));
+
+struct Group {
+    int model, pulse, relax;
+    int item_begin, n_items;
+    int waves_per_block, queue_chunk, blocks;
+};
+
+std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
+
+}  // namespace
+
+struct hhg_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hhg_solver_desc solver{};
+    hhg_launch_desc launch{};
+    int n_jobs = 0;
+    std::vector<hhg::JobDev> jobs;
+    std::vector<hhg_job_desc> job_desc;
+    std::vector<int64_t> offsets;  // n_jobs + 1
+    int64_t n_out = 0;
+    int64_t n_kpts = 0;
+    std::vector<Group> groups;
+    int num_cus = 0;
+
+    double *d_kx = nullptr, *d_ky = nullptr, *d_w = nullptr;
+    hhg::JobDev* d_jobs = nullptr;
+    int2* d_items = nullptr;
+    double *d_At = nullptr, *d_Am = nullptr;
+    unsigned long long* d_words = nullptr;  // [counters per group][accumulators]
+    int64_t n_words = 0;
+    double* d_j = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+
+    // at least one counter block, so trajectory launches have one even when
+    // this shard owns no work items
+    int64_t n_counter_blocks() const { return std::max<int64_t>(1, (int64_t)groups.size()); }
+    hhg::Counters* counters(int g) const { return reinterpret_cast<hhg::Counters*>(d_words) + g; }
+    unsigned long long* acc() const { return d_words + n_counter_blocks() * kCounterWords; }
+};
+
+namespace {
+
+hhg_status check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return HHG_SUCCESS;
+    if (e == hipErrorOutOfMemory) return fail(HHG_ERROR_OUT_OF_MEMORY, hip_msg(what, e));
+    return fail(HHG_ERROR_HIP, hip_msg(what, e));
+}
+
+#define HHG_TRY(expr, what)                                 \
+    do {                                                    \
+        const hhg_status _st = check_hip((expr), (what));   \
+        if (_st != HHG_SUCCESS) return _st;                 \
+    } while (0)
+
+hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n_kgrids, int idx) {
+    const std::string at = "job " + std::to_string(idx) + ": ";
+    const hhg_model_desc& m = jd.model;
+    const hhg_pulse_desc& p = jd.pulse;
+    const hhg_time_desc& t = jd.time;
+    if (jd.kgrid < 0 || jd.kgrid >= n_kgrids) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "k-grid index out of range");
+    if (m.kind < 0 || m.kind > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown model kind");
+    if (m.relax < 0 || m.relax > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown relaxation kind");
+    if (m.relax != HHG_RELAX_NONE && !(m.relax_time > 0.0))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "relaxation needs relax_time > 0");
+    if (m.kind != HHG_MODEL_DIRAC2D && m.theta != 0.0)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "lattice models take the field along x (theta = 0)");
+    if (m.kind != HHG_MODEL_DIRAC2D && !(m.hopping > 0.0))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "lattice models need hopping > 0");
+    if (p.kind < 0 || p.kind > 3) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown pulse kind");
+    const bool gauss = p.kind == HHG_PULSE_GAUSS || p.kind == HHG_PULSE_DOUBLE_GAUSS;
+    if (gauss && !(p.sigma > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "Gaussian pulse needs sigma > 0");
+    if (!gauss && !(p.n_cycles > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "cos^2 pulse needs n_cycles > 0");
+    if (t.n_measurements < 1 || t.n_subdivisions < 1 || !(t.t_end > t.t_begin))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "time grid needs N >= 1, n_subdivisions >= 1, t_end > t_begin");
+    const double delta = (t.t_end - t.t_begin) / t.n_measurements;
+    if (delta > 0.25 || (!gauss && delta / p.n_cycles > 0.25))
+        return fail(HHG_ERROR_UNSUPPORTED,
+                    at + "sampling interval must not exceed 0.25 (>= 26 samples per optical cycle)");
+    if (sv.method == HHG_METHOD_MAGNUS4 && m.relax != HHG_RELAX_NONE)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the Magnus step is unitary: relaxation needs RKF78");
+    return HHG_SUCCESS;
+}
+
+hhg::JobDev make_job(const hhg_job_desc& jd, const hhg_solver_desc& sv) {
+    hhg::JobDev J{};
+    const hhg_model_desc& m = jd.model;
+    const hhg_pulse_desc& p = jd.pulse;
+    J.model = m.kind;
+    J.relax = m.relax;
+    J.v = m.v;
+    J.mass = m.mass;
+    J.hop = m.hopping;
+    J.ct = std::cos(m.theta);
+    J.st = std::sin(m.theta);
+    if (m.theta == 0.0) J.ct = 1.0, J.st = 0.0;
+    J.fermi = m.fermi_energy;
+    J.beta = m.beta;
+    J.relax_rate = m.relax != HHG_RELAX_NONE ? 1.0 / m.relax_time : 0.0;
+    J.pulse = p.kind;
+    J.theta_zero = m.theta == 0.0;
+    J.amp_a = p.amp_a;
+    J.amp_b = p.amp_b;
+    J.inv_n = p.n_cycles > 0.0 ? 1.0 / p.n_cycles : 0.0;
+    J.t_env_end = p.t0 + 2.0 * 3.14159265358979323846 * p.n_cycles;
+    J.inv_2s2 = p.sigma > 0.0 ? 1.0 / (2.0 * p.sigma * p.sigma) : 0.0;
+    J.t0 = p.t0;
+    J.delay = p.delay;
+    J.cep = p.cep;
+    J.t_begin = jd.time.t_begin;
+    J.delta = (jd.time.t_end - jd.time.t_begin) / jd.time.n_measurements;
+    J.n_meas = jd.time.n_measurements;
+    J.n_sub = jd.time.n_subdivisions;
+    J.magnus_sub = sv.method == HHG_METHOD_MAGNUS4 ? std::max(1, sv.magnus_substeps) : 0;
+    return J;
+}
+
+void destroy_device(hhg_plan* p) {
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    for (auto& e : p->ev)
+        if (e) hipEventDestroy(e);
+    hipFree(p->d_kx);
+    hipFree(p->d_ky);
+    hipFree(p->d_w);
+    hipFree(p->d_jobs);
+    hipFree(p->d_items);
+    hipFree(p->d_At);
+    hipFree(p->d_Am);
+    hipFree(p->d_words);
+    hipFree(p->d_j);
+    if (p->stream) hipStreamDestroy(p->stream);
+}
+
+int max_block_threads(const hhg_plan* p, const Group& g) {
+    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax);
+    hipFuncAttributes attr{};
+    if (!fn || hipFuncGetAttributes(&attr, fn) != hipSuccess) return 256;
+    return attr.maxThreadsPerBlock;
+}
+
+int resident_blocks(const hhg_plan* p, const Group& g, int threads) {
+    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax);
+    int per_cu = 0;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return per_cu * p->num_cus;
+}
+
+void size_group(const hhg_plan* p, Group& g, int waves_per_block, int queue_chunk, int persistent) {
+    g.waves_per_block = waves_per_block;
+    g.queue_chunk = queue_chunk;
+    const int items_blocks = (g.n_items + waves_per_block - 1) / waves_per_block;
+    if (persistent) {
+        const int res = resident_blocks(p, g, 64 * waves_per_block);
+        g.blocks = std::max(1, std::min(items_blocks, res));
+    } else {
+        g.queue_chunk = 1;
+        g.blocks = std::max(1, items_blocks);
+    }
+}
+
+hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
+    HHG_TRY(hhg::launch_clear(p->d_words, p->n_words, st), "clear");
+    HHG_TRY(hipEventRecord(p->ev[0], st), "event");
+    for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+        const Group& g = p->groups[gi];
+        hhg::KernelArgs a{};
+        a.jobs = p->d_jobs;
+        a.items = p->d_items + g.item_begin;
+        a.n_items = n_items_limit > 0 ? std::min(g.n_items, n_items_limit) : g.n_items;
+        a.queue_chunk = g.queue_chunk;
+        a.kx = p->d_kx;
+        a.ky = p->d_ky;
+        a.w = p->d_w;
+        a.perm = nullptr;
+        a.At = p->d_At;
+        a.Am = p->d_Am;
+        a.acc = p->acc();
+        a.ctr = p->counters((int)gi);
+        a.sp.rtol = p->solver.rtol;
+        a.sp.atol = p->solver.atol;
+        a.sp.max_fail = p->solver.max_fail > 0 ? p->solver.max_fail : 500;
+        a.sp.proposal = p->solver.proposal;
+        int blocks = g.blocks;
+        if (n_items_limit > 0) blocks = std::min(blocks, (a.n_items + g.waves_per_block - 1) / g.waves_per_block);
+        HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, a, std::max(1, blocks),
+                                  64 * g.waves_per_block, st),
+                "solve launch");
+    }
+    HHG_TRY(hipEventRecord(p->ev[1], st), "event");
+    HHG_TRY(hhg::launch_finalize(p->acc(), p->n_out, p->d_j, st), "finalize");
+    HHG_TRY(hipEventRecord(p->ev[2], st), "event");
+    return HHG_SUCCESS;
+}
+
+hhg_status collect_stats(hhg_plan* p, hhg_stats* stats, double wall_ms) {
+    std::vector<hhg::Counters> c(p->groups.size());
+    HHG_TRY(hipMemcpy(c.data(), p->d_words, sizeof(hhg::Counters) * c.size(), hipMemcpyDeviceToHost), "counters");
+    hhg_stats s{};
+    for (const auto& j : p->jobs) s.trajectories += j.n_k;
+    for (const auto& x : c) {
+        s.attempts += (int64_t)x.attempts;
+        s.rejects += (int64_t)x.rejects;
+        s.failed_trajectories += (int64_t)x.failed;
+        s.nonfinite_trajectories += (int64_t)x.nonfinite;
+        s.proposal_fallbacks += (int64_t)x.fallbacks;
+    }
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, p->ev[0], p->ev[1]) == hipSuccess) s.solve_ms = a;
+    if (hipEventElapsedTime(&b, p->ev[1], p->ev[2]) == hipSuccess) s.reduce_ms = b;
+    s.total_ms = wall_ms;
+    if (stats) *stats = s;
+    if (s.nonfinite_trajectories > 0)
+        return fail(HHG_ERROR_NONFINITE, std::to_string(s.nonfinite_trajectories) + " trajectories became non-finite");
+    if (s.failed_trajectories > 0)
+        return fail(HHG_ERROR_TOLERANCE,
+                    std::to_string(s.failed_trajectories) + " trajectories exceeded the step controller's attempts");
+    return HHG_SUCCESS;
+}
+
+hhg_status run(hhg_plan* p, hipStream_t st, int n_items_limit = 0) {
+    if (p->graph_exec && n_items_limit == 0 && st == p->stream) {
+        HHG_TRY(hipGraphLaunch(p->graph_exec, st), "graph launch");
+        return HHG_SUCCESS;
+    }
+    return enqueue(p, st, n_items_limit);
+}
+
+}  // namespace
+
+extern "C" {
+
+hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
+    hhg::clear_error();
+    if (!d || !out) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_create: null argument");
+    *out = nullptr;
+    if (d->n_jobs < 1 || !d->jobs || d->n_kgrids < 1 || !d->kgrids)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "plan needs >= 1 job and >= 1 k-grid");
+    const int n_ranks = d->n_ranks > 0 ? d->n_ranks : 1;
+    if (d->rank < 0 || d->rank >= n_ranks) return fail(HHG_ERROR_INVALID_ARGUMENT, "rank outside [0, n_ranks)");
+    const hhg_solver_desc& sv = d->solver;
+    if (sv.method != HHG_METHOD_RKF78 && sv.method != HHG_METHOD_MAGNUS4)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown method");
+    if (sv.method == HHG_METHOD_RKF78 && (!(sv.rtol >= 0.0) || !(sv.atol >= 0.0) || !(sv.rtol + sv.atol > 0.0)))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "RKF78 needs rtol, atol >= 0, not both 0");
+    if (sv.proposal == HHG_PROPOSAL_LEARNED)
+        return fail(HHG_ERROR_UNSUPPORTED, "learned proposal needs an MLP handle (not provided by this entry point)");
+    if (sv.proposal < 0 || sv.proposal > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown proposal");
+    for (int i = 0; i < d->n_jobs; ++i) {
+        const hhg_status st = validate_job(d->jobs[i], sv, d->n_kgrids, i);
+        if (st != HHG_SUCCESS) return st;
+    }
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev < 1) return fail(HHG_ERROR_NO_DEVICE, "no HIP device");
+    if (d->device < 0 || d->device >= n_dev) return fail(HHG_ERROR_INVALID_ARGUMENT, "device ordinal out of range");
+
+    hhg_plan* p = new hhg_plan();
+    p->device = d->device;
+    p->solver = sv;
+    p->launch = d->launch;
+    p->n_jobs = d->n_jobs;
+    p->job_desc.assign(d->jobs, d->jobs + d->n_jobs);
+    auto bail = [&](hhg_status st) {
+        destroy_device(p);
+        delete p;
+        return st;
+    };
+    if (hipSetDevice(p->device) != hipSuccess) return bail(fail(HHG_ERROR_HIP, "hipSetDevice failed"));
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, p->device) != hipSuccess)
+            return bail(fail(HHG_ERROR_HIP, "hipGetDeviceProperties failed"));
+        p->num_cus = prop.multiProcessorCount;
+    }
+
+    // k-grids, concatenated
+    std::vector<int64_t> grid_base(d->n_kgrids + 1, 0), grid_size(d->n_kgrids);
+    for (int g = 0; g < d->n_kgrids; ++g) {
+        grid_size[g] = hhg::kgrid_size(d->kgrids[g]);
+        if (grid_size[g] < 0) return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "bad k-grid size"));
+        grid_base[g + 1] = grid_base[g] + grid_size[g];
+    }
+    p->n_kpts = grid_base[d->n_kgrids];
+    std::vector<double> kx(p->n_kpts), ky(p->n_kpts), w(p->n_kpts);
+    for (int g = 0; g < d->n_kgrids; ++g) {
+        const hhg_status st = hhg::kgrid_build(d->kgrids[g], kx.data() + grid_base[g], ky.data() + grid_base[g],
+                                               w.data() + grid_base[g]);
+        if (st != HHG_SUCCESS) return bail(st);
+    }
+
+    // jobs: shard = contiguous block of 64-point chunks of each job's grid
+    p->offsets.assign(d->n_jobs + 1, 0);
+    int64_t magnus_words = 0;
+    for (int i = 0; i < d->n_jobs; ++i) {
+        hhg::JobDev J = make_job(d->jobs[i], sv);
+        const int g = d->jobs[i].kgrid;
+        const int64_t n_chunks = (grid_size[g] + hhg::WAVE - 1) / hhg::WAVE;
+        const int64_t per_rank = (n_chunks + n_ranks - 1) / n_ranks;
+        const int64_t c0 = std::min(n_chunks, per_rank * d->rank), c1 = std::min(n_chunks, c0 + per_rank);
+        J.k_base = grid_base[g] + c0 * hhg::WAVE;
+        J.n_k = std::max<int64_t>(0, std::min(grid_size[g], c1 * hhg::WAVE) - c0 * hhg::WAVE);
+        J.n_items = (int32_t)(c1 - c0);
+        J.sample_base = p->offsets[i];
+        p->offsets[i + 1] = p->offsets[i] + J.n_meas + 1;
+        J.magnus_base = magnus_words;
+        magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
+        J.row_base = 0;
+        p->jobs.push_back(J);
+    }
+    p->n_out = p->offsets[d->n_jobs];
+
+    // groups by (model, pulse, relax); inside a group, longest jobs first
+    std::map<std::tuple<int, int, int>, std::vector<int>> by_key;
+    for (int i = 0; i < d->n_jobs; ++i) {
+        const auto& J = p->jobs[i];
+        const int pulse = sv.method == HHG_METHOD_MAGNUS4 ? 0 : J.pulse;
+        by_key[{J.model, pulse, J.relax}].push_back(i);
+    }
+    std::vector<int2> items;
+    for (auto& kv : by_key) {
+        auto& js = kv.second;
+        std::stable_sort(js.begin(), js.end(), [&](int a, int b) { return p->jobs[a].n_meas > p->jobs[b].n_meas; });
+        Group g{};
+        std::tie(g.model, g.pulse, g.relax) = kv.first;
+        g.item_begin = (int)items.size();
+        for (int j : js)
+            for (int c = 0; c < p->jobs[j].n_items; ++c) items.push_back(make_int2(j, c));
+        g.n_items = (int)items.size() - g.item_begin;
+        if (g.n_items > 0) p->groups.push_back(g);
+    }
+    if (items.size() > (size_t)0x7fffffff) return bail(fail(HHG_ERROR_UNSUPPORTED, "more than 2^31 work items"));
+
+    // device memory
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(HHG_ERROR_HIP, "stream creation failed"));
+    for (auto& e : p->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(HHG_ERROR_HIP, "event creation failed"));
+    p->n_words = p->n_counter_blocks() * kCounterWords + 2 * p->n_out;
+    const size_t nk = sizeof(double) * std::max<int64_t>(1, p->n_kpts);
+    hhg_status st = HHG_SUCCESS;
+    auto A = [&](void** ptr, size_t bytes, const char* what) {
+        if (st == HHG_SUCCESS) st = check_hip(hipMalloc(ptr, std::max<size_t>(bytes, 8)), what);
+    };
+    A((void**)&p->d_kx, nk, "kx");
+    A((void**)&p->d_ky, nk, "ky");
+    A((void**)&p->d_w, nk, "w");
+    A((void**)&p->d_jobs, sizeof(hhg::JobDev) * p->jobs.size(), "jobs");
+    A((void**)&p->d_items, sizeof(int2) * std::max<size_t>(1, items.size()), "items");
+    A((void**)&p->d_At, sizeof(double) * p->n_out, "pulse samples");
+    A((void**)&p->d_Am, sizeof(double) * std::max<int64_t>(1, magnus_words), "magnus table");
+    A((void**)&p->d_words, sizeof(unsigned long long) * p->n_words, "accumulators");
+    A((void**)&p->d_j, sizeof(double) * p->n_out, "j");
+    if (st != HHG_SUCCESS) return bail(st);
+    auto C = [&](void* dst, const void* src, size_t bytes, const char* what) {
+        if (st == HHG_SUCCESS && bytes) st = check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), what);
+    };
+    C(p->d_kx, kx.data(), sizeof(double) * p->n_kpts, "upload kx");
+    C(p->d_ky, ky.data(), sizeof(double) * p->n_kpts, "upload ky");
+    C(p->d_w, w.data(), sizeof(double) * p->n_kpts, "upload w");
+    C(p->d_jobs, p->jobs.data(), sizeof(hhg::JobDev) * p->jobs.size(), "upload jobs");
+    C(p->d_items, items.data(), sizeof(int2) * items.size(), "upload items");
+    if (st != HHG_SUCCESS) return bail(st);
+    for (int i = 0; i < d->n_jobs; ++i) {
+        const auto& J = p->jobs[i];
+        const int64_t n = std::max<int64_t>(J.n_meas + 1, (int64_t)J.n_meas * J.magnus_sub);
+        st = check_hip(hhg::launch_pulse_table(J.pulse, p->d_jobs, i, n, p->d_At, p->d_Am, J.magnus_sub > 0, p->stream),
+                       "pulse table");
+        if (st != HHG_SUCCESS) return bail(st);
+    }
+    if ((st = check_hip(hipStreamSynchronize(p->stream), "pulse table sync")) != HHG_SUCCESS) return bail(st);
+
+    // launch shapes
+    const int wpb = p->launch.waves_per_block > 0 ? p->launch.waves_per_block : 4;
+    const int qc = p->launch.queue_chunk > 0 ? p->launch.queue_chunk : 1;
+    if (wpb > 8 || (wpb & (wpb - 1))) return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block must be 1, 2, 4 or 8"));
+    for (auto& g : p->groups)
+        if (64 * wpb > max_block_threads(p, g))
+            return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block exceeds the kernel's block size limit"));
+    for (auto& g : p->groups) size_group(p, g, wpb, qc, p->launch.persistent);
+    *out = p;
+    return HHG_SUCCESS;
+}
+
+int64_t hhg_plan_output_size(const hhg_plan* p) { return p ? p->n_out : -1; }
+
+hhg_status hhg_plan_job_offsets(const hhg_plan* p, int64_t* offsets) {
+    hhg::clear_error();
+    if (!p || !offsets) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_job_offsets: null argument");
+    std::copy(p->offsets.begin(), p->offsets.end(), offsets);
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_plan_execute(hhg_plan* p, double* j_out, hhg_stats* stats) {
+    hhg::clear_error();
+    if (!p || !j_out) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_execute: null argument");
+    const auto t0 = std::chrono::steady_clock::now();
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    hhg_status st = run(p, p->stream);
+    if (st != HHG_SUCCESS) return st;
+    HHG_TRY(hipMemcpyAsync(j_out, p->d_j, sizeof(double) * p->n_out, hipMemcpyDeviceToHost, p->stream), "copy j");
+    HHG_TRY(hipStreamSynchronize(p->stream), "synchronize");
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return collect_stats(p, stats, ms);
+}
+
+hhg_status hhg_plan_execute_accumulators(hhg_plan* p, uint64_t* acc_out, hhg_stats* stats) {
+    hhg::clear_error();
+    if (!p || !acc_out) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_execute_accumulators: null argument");
+    const auto t0 = std::chrono::steady_clock::now();
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    hhg_status st = run(p, p->stream);
+    if (st != HHG_SUCCESS) return st;
+    HHG_TRY(hipMemcpyAsync(acc_out, p->acc(), sizeof(uint64_t) * 2 * p->n_out, hipMemcpyDeviceToHost, p->stream),
+            "copy accumulators");
+    HHG_TRY(hipStreamSynchronize(p->stream), "synchronize");
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return collect_stats(p, stats, ms);
+}
+
+hhg_status hhg_plan_execute_device(hhg_plan* p, double* j_dev, void* stream) {
+    hhg::clear_error();
+    if (!p || !j_dev) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_execute_device: null argument");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+    hhg_status r = run(p, st);
+    if (r != HHG_SUCCESS) return r;
+    HHG_TRY(hipMemcpyAsync(j_dev, p->d_j, sizeof(double) * p->n_out, hipMemcpyDeviceToDevice, st), "copy j");
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
+    hhg::clear_error();
+    if (!p) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_autotune: null argument");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    if (p->graph_exec) return fail(HHG_ERROR_UNSUPPORTED, "autotune before capturing the graph");
+    // Tune on a prefix of every group's items: enough waves to fill the chip a
+    // few times, so the measured rate is the steady-state one.
+    const int wpbs[] = {1, 2, 4, 8};
+    const int chunks[] = {1, 2, 4};
+    double best = 1e300;
+    int best_w = 4, best_c = 1;
+    const std::vector<Group> saved = p->groups;
+    for (int wpb : wpbs)
+        for (int qc : chunks) {
+            int limit = 0;
+            for (auto& g : p->groups) {
+                size_group(p, g, wpb, qc, 1);
+                limit = std::max(limit, 3 * g.blocks * wpb);
+            }
+            hhg_status st = enqueue(p, p->stream, limit);
+            if (st != HHG_SUCCESS) return st;
+            HHG_TRY(hipStreamSynchronize(p->stream), "autotune sync");
+            float ms = 0.f;
+            HHG_TRY(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]), "autotune timing");
+            // normalise by the items actually run
+            int64_t ran = 0;
+            for (auto& g : p->groups) ran += std::min(g.n_items, limit);
+            const double per_item = ms / std::max<int64_t>(1, ran);
+            if (per_item < best) best = per_item, best_w = wpb, best_c = qc;
+        }
+    p->groups = saved;
+    p->launch.waves_per_block = best_w;
+    p->launch.queue_chunk = best_c;
+    p->launch.persistent = 1;
+    for (auto& g : p->groups) size_group(p, g, best_w, best_c, 1);
+    if (chosen) *chosen = p->launch;
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_plan_capture_graph(hhg_plan* p) {
+    hhg::clear_error();
+    if (!p) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_capture_graph: null argument");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    if (p->graph_exec) return HHG_SUCCESS;
+    HHG_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    const hhg_status st = enqueue(p, p->stream, 0);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(p->stream, &g);
+    if (st != HHG_SUCCESS) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
+    HHG_TRY(e, "end capture");
+    p->graph = g;
+    HHG_TRY(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_plan_destroy(hhg_plan* p) {
+    hhg::clear_error();
+    if (!p) return HHG_SUCCESS;
+    hipSetDevice(p->device);
+    destroy_device(p);
+    delete p;
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_trajectories(hhg_plan* p, const int64_t* k_index, int64_t n_sel, double* s_out) {
+    hhg::clear_error();
+    if (!p || (n_sel > 0 && (!k_index || !s_out))) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_trajectories: null argument");
+    if (n_sel <= 0) return HHG_SUCCESS;
+    const hhg::JobDev& J = p->jobs[0];
+    for (int64_t i = 0; i < n_sel; ++i)
+        if (k_index[i] < 0 || k_index[i] >= J.n_k)
+            return fail(HHG_ERROR_INVALID_ARGUMENT, "k index outside job 0's k-points");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    int64_t* d_sel = nullptr;
+    double* d_s = nullptr;
+    const size_t out_bytes = sizeof(double) * 3 * (size_t)(J.n_meas + 1) * (size_t)n_sel;
+    hhg_status st = check_hip(hipMalloc(&d_sel, sizeof(int64_t) * n_sel), "selection");
+    if (st == HHG_SUCCESS) st = check_hip(hipMalloc(&d_s, out_bytes), "trajectory output");
+    if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(d_sel, k_index, sizeof(int64_t) * n_sel, hipMemcpyHostToDevice), "upload");
+    if (st == HHG_SUCCESS) st = check_hip(hhg::launch_clear(p->d_words, p->n_words, p->stream), "clear");
+    if (st == HHG_SUCCESS) {
+        hhg::SolverDev sp{p->solver.rtol, p->solver.atol, p->solver.max_fail > 0 ? p->solver.max_fail : 500,
+                          p->solver.proposal};
+        const int pulse = p->solver.method == HHG_METHOD_MAGNUS4 ? 0 : J.pulse;
+        st = check_hip(hhg::launch_trajectories(p->solver.method, J.model, pulse, J.relax, p->d_jobs, p->d_kx, p->d_ky,
+                                                d_sel, n_sel, p->d_Am, sp, d_s, p->counters(0), p->stream),
+                       "trajectory launch");
+    }
+    if (st == HHG_SUCCESS) st = check_hip(hipStreamSynchronize(p->stream), "synchronize");
+    if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(s_out, d_s, out_bytes, hipMemcpyDeviceToHost), "download");
+    hipFree(d_sel);
+    hipFree(d_s);
+    return st;
+}
+
+}  // extern "C"

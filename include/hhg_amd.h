@@ -1,0 +1,269 @@
+/*
+ * hhg_amd.h — C ABI of the MI355X HigherHarmonics Bloch-equation hot path.
+ *
+ * One library (libhhg_amd.so) exposes the per-k-point Bloch solve and the
+ * quadrature-weighted current reduction j(t) as plain C: descriptors in,
+ * caller-owned double buffers out, status codes instead of exceptions.
+ *
+ * Reference interfaces this ABI replaces (paths relative to the reference
+ * repository majesticrock/HigherHarmonics):
+ *   - Dispatcher::compute(rank, n_ranks, n_z) filling current_density_time
+ *       (sources/HHG/Dispatch/Dispatcher.hpp:17,29) -> hhg_plan_execute
+ *   - DiracSystem / PiFlux / Honeycomb ::compute_current_density(laser,
+ *       time_config, rank, n_ranks, ...) (sources/HHG/Systems/DiracSystem.hpp:58-59,
+ *       PiFlux.hpp:92-93, Honeycomb.hpp:70-71) -> hhg_plan_create + hhg_plan_execute
+ *   - per-k time_evolution_sigma / time_evolution_magnus /
+ *       time_evolution_diagonal_relaxation (DiracSystem.hpp:40-47,
+ *       Honeycomb.hpp:64-68, PiFlux.hpp:80-81) -> hhg_trajectories
+ *   - Laser::laser_function(t) (sources/HHG/Laser/Laser.hpp:32) -> hhg_pulse_evaluate
+ *   - TimeIntegrationConfig {t_begin, t_end, n_measurements, n_subdivisions}
+ *       (sources/HHG/TimeIntegrationConfig.hpp:7-15) -> hhg_time_desc
+ *   - gauss::container<N> tables (sources/HHG/thread_gauss.hpp:21-311) -> hhg_gauss_legendre
+ *   - TrapezoidalFFT::compute (sources/HHG/Fourier/TrapezoidalFFT.cpp:11-49) with
+ *       the x8 zero padding of sources/hhg.cpp:277-282 -> hhg_trapezoidal_spectrum
+ *
+ * Units are the reference's dimensionless ones: time in 1/omega_L, energy in
+ * hbar omega_L, A(t) as a momentum shift k -> k - A(t) e_theta.
+ * Everything is FP64.
+ *
+ * Threading: a plan is not thread-safe; distinct plans may be used
+ * concurrently from different threads.  All device memory of a plan is owned
+ * by the plan and released by hhg_plan_destroy.
+ */
+#ifndef HHG_AMD_H
+#define HHG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HHG_AMD_ABI_VERSION 1
+
+typedef enum hhg_status {
+    HHG_SUCCESS = 0,
+    HHG_ERROR_INVALID_ARGUMENT = 1, /* bad descriptor (reference: std::invalid_argument) */
+    HHG_ERROR_HIP = 2,              /* a HIP runtime call failed */
+    HHG_ERROR_TOLERANCE = 3,        /* >= 1 trajectory failed its step controller (reference: odeint
+                                       step_adjustment_error); j is still written, see hhg_stats */
+    HHG_ERROR_OUT_OF_MEMORY = 4,
+    HHG_ERROR_UNSUPPORTED = 5,      /* combination not provided (reference: std::runtime_error) */
+    HHG_ERROR_NO_DEVICE = 6,
+    HHG_ERROR_NONFINITE = 7         /* a pulse or state value became NaN/Inf */
+} hhg_status;
+
+/* ---- models: d(k - A(t) e_theta); Bloch equation ds/dt = 2 d x s [+ relaxation] ---- */
+typedef enum hhg_model_kind {
+    HHG_MODEL_DIRAC2D = 0,   /* d = (v qx, v qy, mass) */
+    HHG_MODEL_PIFLUX2D = 1,  /* d = 2t (cos qx, cos qy, 0); cf. PiFlux.cpp:89-94 */
+    HHG_MODEL_HONEYCOMB = 2  /* d = t (Re g, -Im g, 0), g = sum_delta e^{i q.delta}; Honeycomb.cpp:44-50,130-138 */
+} hhg_model_kind;
+
+typedef enum hhg_relax_kind {
+    HHG_RELAX_NONE = 0,
+    HHG_RELAX_TO_INITIAL = 1, /* -(s - s0)/tau, DiracSystem.cpp:231-233 */
+    HHG_RELAX_T2 = 2          /* -(s - (s.d^)d^)/T2 about the instantaneous d(k - A(t)) */
+} hhg_relax_kind;
+
+typedef struct hhg_model_desc {
+    int32_t kind;        /* hhg_model_kind */
+    int32_t relax;       /* hhg_relax_kind */
+    double v;            /* Dirac-2D velocity */
+    double mass;         /* Dirac-2D gap parameter */
+    double hopping;      /* PiFlux-2D / honeycomb hopping t */
+    double theta;        /* polarisation angle (Honeycomb.hpp:88-90 fixes 0) */
+    double fermi_energy; /* E_F in hbar omega_L */
+    double beta;         /* 1/T in 1/(hbar omega_L); <= 0 means T = 0 */
+    double relax_time;   /* tau or T2 in 1/omega_L */
+} hhg_model_desc;
+
+/* ---- pulses: A(t) = amplitude * envelope * sin(t + cep) ---- */
+typedef enum hhg_pulse_kind {
+    HHG_PULSE_COS2 = 0,         /* CosineLaser.cpp:22-26, window (t0, t0 + 2 pi n) */
+    HHG_PULSE_GAUSS = 1,        /* GaussLaser.cpp:26-28, centre t0, width sigma */
+    HHG_PULSE_DOUBLE_GAUSS = 2, /* DoubleLaser<GaussLaser>: A(t) + B(t - delay), DoubleLaser.hpp:63-67 */
+    HHG_PULSE_DOUBLE_COS2 = 3   /* DoubleLaser<CosineLaser> */
+} hhg_pulse_kind;
+
+typedef struct hhg_pulse_desc {
+    int32_t kind; /* hhg_pulse_kind */
+    int32_t pad;
+    double amp_a;    /* peak of pulse A */
+    double amp_b;    /* peak of pulse B (double pulses) */
+    double n_cycles; /* cos^2 envelope length 2 pi n_cycles */
+    double sigma;    /* Gaussian width */
+    double t0;       /* cos^2 window begin / Gaussian centre of pulse A */
+    double delay;    /* pulse B delay */
+    double cep;      /* carrier-envelope phase */
+} hhg_pulse_desc;
+
+/* TimeIntegrationConfig: samples t_i = t_begin + i (t_end - t_begin)/N, i = 0..N;
+ * the reference controller restarts each interval at dt = Delta/n_subdivisions. */
+typedef struct hhg_time_desc {
+    double t_begin;
+    double t_end;
+    int32_t n_measurements;
+    int32_t n_subdivisions;
+} hhg_time_desc;
+
+/* ---- Brillouin-zone quadrature ---- */
+typedef enum hhg_kgrid_kind {
+    HHG_KGRID_GL_BOX = 0,  /* nx x ny Gauss-Legendre on [x0,x1] x [y0,y1], x-major */
+    HHG_KGRID_GL_HEX = 1,  /* hexagonal BZ of Honeycomb.cpp:229-268 (nx x ny nodes) */
+    HHG_KGRID_EXPLICIT = 2 /* caller's points and weights */
+} hhg_kgrid_kind;
+
+typedef struct hhg_kgrid_desc {
+    int32_t kind; /* hhg_kgrid_kind */
+    int32_t nx;
+    int32_t ny;
+    int32_t pad;
+    double x0, x1, y0, y1;
+    const double* kx; /* explicit grids only: n_points each */
+    const double* ky;
+    const double* w;
+    int64_t n_points;
+} hhg_kgrid_desc;
+
+/* One job = one (model, pulse, time grid, k-grid) current j(t). */
+typedef struct hhg_job_desc {
+    hhg_model_desc model;
+    hhg_pulse_desc pulse;
+    hhg_time_desc time;
+    int32_t kgrid; /* index into hhg_plan_desc.kgrids */
+    int32_t pad;
+} hhg_job_desc;
+
+typedef enum hhg_method { HHG_METHOD_RKF78 = 0, HHG_METHOD_MAGNUS4 = 1 } hhg_method;
+
+typedef enum hhg_proposal {
+    HHG_PROPOSAL_RESTART = 0, /* reference: every interval starts at dt = Delta/n_sub */
+    HHG_PROPOSAL_CARRY = 1,   /* carry the controller's step across intervals (ablation) */
+    HHG_PROPOSAL_LEARNED = 2  /* MLP proposes each interval's first step; controller still judges */
+} hhg_proposal;
+
+typedef struct hhg_solver_desc {
+    int32_t method;   /* hhg_method */
+    int32_t proposal; /* hhg_proposal */
+    double rtol;
+    double atol;
+    int32_t max_fail;        /* attempts per step before a trajectory fails (odeint: 500) */
+    int32_t magnus_substeps; /* Magnus steps per sampling interval */
+    const void* mlp;         /* HHG_PROPOSAL_LEARNED: hhg_mlp handle (see hhg_mlp_*) */
+} hhg_solver_desc;
+
+typedef struct hhg_launch_desc {
+    int32_t waves_per_block; /* 0 = default (4); 1, 2, 4 or 8 waves of 64 lanes per block */
+    int32_t queue_chunk;     /* 0 = default/autotuned: work items grabbed per queue pop */
+    int32_t persistent;      /* 1 = persistent grid + device work queue, 0 = one wave per item */
+    int32_t group_by_cost;   /* 1 = order k-points inside a job by predicted step count */
+} hhg_launch_desc;
+
+typedef struct hhg_plan_desc {
+    int32_t n_kgrids;
+    int32_t n_jobs;
+    const hhg_kgrid_desc* kgrids;
+    const hhg_job_desc* jobs;
+    hhg_solver_desc solver;
+    hhg_launch_desc launch;
+    int32_t rank;    /* shard of the k-points (reference: contiguous block per MPI rank) */
+    int32_t n_ranks; /* 1 on a single GPU */
+    int32_t device;  /* HIP device ordinal */
+    int32_t pad;
+} hhg_plan_desc;
+
+typedef struct hhg_stats {
+    int64_t trajectories;
+    int64_t attempts;            /* attempted RKF78 steps (or Magnus steps) */
+    int64_t rejects;
+    int64_t failed_trajectories; /* controller gave up (HHG_ERROR_TOLERANCE) */
+    int64_t nonfinite_trajectories;
+    int64_t proposal_fallbacks;  /* learned proposals replaced by the plain controller */
+    double solve_ms;             /* device time of the solve kernel(s) */
+    double reduce_ms;            /* device time of the j reduction */
+    double total_ms;             /* host wall time of hhg_plan_execute */
+} hhg_stats;
+
+typedef struct hhg_plan hhg_plan;
+
+/* Version of this ABI (HHG_AMD_ABI_VERSION). */
+int32_t hhg_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* hhg_last_error_string(void);
+
+/* Builds the k-grids, uploads them and the job tables to the device and
+ * allocates all scratch.  Replaces constructing a Dispatcher
+ * (sources/HHG/Dispatch/DiracDispatcher.cpp:25-62 and siblings). */
+hhg_status hhg_plan_create(const hhg_plan_desc* desc, hhg_plan** plan);
+
+/* Number of output doubles: sum over jobs of (n_measurements + 1). */
+int64_t hhg_plan_output_size(const hhg_plan* plan);
+
+/* offsets[n_jobs + 1]: job j's samples are j_out[offsets[j] .. offsets[j+1]). */
+hhg_status hhg_plan_job_offsets(const hhg_plan* plan, int64_t* offsets);
+
+/* Solves every trajectory and writes j(t) of every job into the caller-owned
+ * host buffer j_out (hhg_plan_output_size doubles).  stats may be NULL.
+ * Replaces Dispatcher::compute (sources/HHG/Dispatch/Dispatcher.hpp:29). */
+hhg_status hhg_plan_execute(hhg_plan* plan, double* j_out, hhg_stats* stats);
+
+/* As hhg_plan_execute but leaves j in device memory j_dev (hhg_plan_output_size
+ * doubles, caller-allocated on the plan's device) and does not synchronise
+ * the host beyond what the plan's stream requires; stream is a hipStream_t or
+ * NULL for the plan's own stream. */
+hhg_status hhg_plan_execute_device(hhg_plan* plan, double* j_dev, void* stream);
+
+/* Tries launch shapes on the plan's own work and keeps the fastest
+ * (waves_per_block x queue_chunk); writes the choice into chosen (may be NULL). */
+hhg_status hhg_plan_autotune(hhg_plan* plan, hhg_launch_desc* chosen);
+
+/* Records the plan's kernels into a HIP graph; later executes replay it. */
+hhg_status hhg_plan_capture_graph(hhg_plan* plan);
+
+hhg_status hhg_plan_destroy(hhg_plan* plan);
+
+/* Per-k Bloch vectors at every sample of job 0 for the listed k indices of
+ * its k-grid: s_out[n_sel][N+1][3].  Replaces time_evolution_sigma /
+ * time_evolution_magnus for inspection and tests. */
+hhg_status hhg_trajectories(hhg_plan* plan, const int64_t* k_index, int64_t n_sel, double* s_out);
+
+/* ---- host utilities (no GPU needed) ---- */
+
+/* Gauss-Legendre nodes/weights on [-1, 1], nodes ascending. */
+hhg_status hhg_gauss_legendre(int32_t n, double* x, double* w);
+
+/* Number of points of a k-grid, and the points/weights themselves. */
+int64_t hhg_kgrid_size(const hhg_kgrid_desc* g);
+hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, double* w);
+
+/* A(t) at n times (host evaluation, Laser::laser_function). */
+hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n);
+
+/* End-corrected Fourier integral of j (n_measurements + 1 samples), zero
+ * padded to zero_padding * (N + 1); outputs have zero_padding*(N+1)/2 + 1
+ * entries.  Keeps the reference's conventions (see hhg_amd spectrum notes). */
+hhg_status hhg_trapezoidal_spectrum(const double* j, const hhg_time_desc* time, int32_t zero_padding,
+                                    double* freq, double* re, double* im);
+
+/* ---- exact accumulators (shard reduction) ---- */
+
+/* As hhg_plan_execute, but returns j as the exact signed 128-bit fixed-point
+ * sums the device accumulates (2 words per sample: low, high; value =
+ * (high * 2^64 + low) * 2^-80).  Shards of the same plan description added
+ * with hhg_accumulators_add equal the unsharded accumulators bit for bit,
+ * whatever the order: this is the exact replacement of the reference's
+ * MPI_Reduce(SUM) of j (sources/hhg.cpp:241-243). */
+hhg_status hhg_plan_execute_accumulators(hhg_plan* plan, uint64_t* acc_out, hhg_stats* stats);
+
+/* total += part (n samples, 128-bit two's complement with carry). */
+void hhg_accumulators_add(uint64_t* total, const uint64_t* part, int64_t n);
+
+/* acc (n samples) -> j as doubles. */
+void hhg_accumulators_to_double(const uint64_t* acc, int64_t n, double* j);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,239 @@
+"""CPU oracle for the HigherHarmonics Bloch hot path.
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker against which the HIP
+product path is compared, and the CPU baseline that bench.py times.  Only
+``tests/``, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
+import it; product code never does.
+
+* ``hhg_oracle.c`` (built by ``oracle/Makefile`` into ``libhhg_oracle.so`` and
+  the optimised ``libhhg_oracle_fast.so``) restates the reference's pulses,
+  Bloch right-hand sides, Boost.Odeint RKF78 controller with per-interval
+  restart, Magnus step and quadrature-weighted current sum.
+* :func:`trapezoidal_spectrum` restates the reference's end-corrected Fourier
+  integral (sources/HHG/Fourier/TrapezoidalFFT.cpp:11-49) with the x8 zero
+  padding of sources/hhg.cpp:36,277-282, on numpy's FFT instead of FFTW.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB = 0, 1, 2
+PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2 = 0, 1, 2, 3
+RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2 = 0, 1, 2
+METHOD_RKF78, METHOD_MAGNUS4 = 0, 1
+PROPOSAL_RESTART, PROPOSAL_CARRY = 0, 1
+
+
+class Model(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
+                ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
+                ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double)]
+
+
+class Pulse(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("amp_a", ctypes.c_double),
+                ("amp_b", ctypes.c_double), ("n_cycles", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("t0", ctypes.c_double), ("delay", ctypes.c_double), ("cep", ctypes.c_double)]
+
+
+class Time(ctypes.Structure):
+    _fields_ = [("t_begin", ctypes.c_double), ("t_end", ctypes.c_double), ("n_measurements", ctypes.c_int32),
+                ("n_subdivisions", ctypes.c_int32)]
+
+
+class Solver(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("proposal", ctypes.c_int32), ("rtol", ctypes.c_double),
+                ("atol", ctypes.c_double), ("max_fail", ctypes.c_int32), ("magnus_substeps", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("attempts", ctypes.c_int64), ("rejects", ctypes.c_int64),
+                ("failed_trajectories", ctypes.c_int64), ("rhs_evals", ctypes.c_int64)]
+
+
+_libs: dict = {}
+
+
+def build(quiet: bool = True) -> None:
+    """Compile both oracle libraries with oracle/Makefile."""
+    out = subprocess.run(["make", "-C", _HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+
+
+def lib(fast: bool = False) -> ctypes.CDLL:
+    name = "libhhg_oracle_fast.so" if fast else "libhhg_oracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.or_pulse_A.restype = ctypes.c_double
+    L.or_pulse_A.argtypes = [ctypes.POINTER(Pulse), ctypes.c_double]
+    L.or_gauss_legendre.argtypes = [ctypes.c_int, dp, dp]
+    L.or_rkf78_tableau.argtypes = [dp, dp, dp, dp]
+    L.or_initial_state.argtypes = [ctypes.POINTER(Model), ctypes.c_double, ctypes.c_double, dp]
+    L.or_rhs.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.c_double, ctypes.c_double, dp, dp,
+                         ctypes.c_double, dp]
+    L.or_vertex.restype = ctypes.c_double
+    L.or_vertex.argtypes = [ctypes.POINTER(Model), ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+    L.or_trajectory.restype = ctypes.c_int
+    L.or_trajectory.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
+                                ctypes.POINTER(Solver), ctypes.c_double, ctypes.c_double, dp, ctypes.POINTER(Stats)]
+    L.or_current_density.restype = ctypes.c_int64
+    L.or_current_density.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
+                                     ctypes.POINTER(Solver), dp, dp, dp, ctypes.c_int64, dp, ctypes.POINTER(Stats),
+                                     ctypes.c_int]
+    _libs[name] = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def model(kind=MODEL_DIRAC2D, v=1.0, mass=0.1, hopping=1.0, theta=0.0, fermi_energy=0.0, beta=0.0,
+          relax=RELAX_NONE, relax_time=0.0) -> Model:
+    return Model(kind, relax, v, mass, hopping, theta, fermi_energy, beta, relax_time)
+
+
+def pulse(kind=PULSE_COS2, amp_a=0.5, amp_b=0.0, n_cycles=8.0, sigma=0.0, t0=0.0, delay=0.0, cep=0.0) -> Pulse:
+    return Pulse(kind, 0, amp_a, amp_b, n_cycles, sigma, t0, delay, cep)
+
+
+def time_grid(t_begin, t_end, n_measurements, n_subdivisions=50) -> Time:
+    return Time(t_begin, t_end, n_measurements, n_subdivisions)
+
+
+def solver(method=METHOD_RKF78, rtol=1e-10, atol=1e-10, proposal=PROPOSAL_RESTART, max_fail=500,
+           magnus_substeps=1) -> Solver:
+    return Solver(method, proposal, rtol, atol, max_fail, magnus_substeps)
+
+
+def pulse_A(p: Pulse, t) -> np.ndarray:
+    L = lib()
+    t = np.atleast_1d(np.asarray(t, dtype=np.float64))
+    return np.array([L.or_pulse_A(ctypes.byref(p), float(x)) for x in t])
+
+
+def gauss_legendre(n: int):
+    x = np.empty(n)
+    w = np.empty(n)
+    lib().or_gauss_legendre(n, _dp(x), _dp(w))
+    return x, w
+
+
+def rkf78_tableau():
+    c, a, b8, b7 = np.empty(13), np.empty(169), np.empty(13), np.empty(13)
+    lib().or_rkf78_tableau(_dp(c), _dp(a), _dp(b8), _dp(b7))
+    return c, a.reshape(13, 13), b8, b7
+
+
+def initial_state(m: Model, kx: float, ky: float) -> np.ndarray:
+    s = np.empty(3)
+    lib().or_initial_state(ctypes.byref(m), kx, ky, _dp(s))
+    return s
+
+
+def rhs(m: Model, p: Pulse, kx, ky, s_init, s, t) -> np.ndarray:
+    out = np.empty(3)
+    si = np.ascontiguousarray(s_init, dtype=np.float64)
+    ss = np.ascontiguousarray(s, dtype=np.float64)
+    lib().or_rhs(ctypes.byref(m), ctypes.byref(p), kx, ky, _dp(si), _dp(ss), t, _dp(out))
+    return out
+
+
+def vertex(m: Model, kx, ky, A, s) -> float:
+    ss = np.ascontiguousarray(s, dtype=np.float64)
+    return lib().or_vertex(ctypes.byref(m), kx, ky, A, _dp(ss))
+
+
+def trajectory(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float):
+    out = np.empty((tc.n_measurements + 1, 3))
+    st = Stats()
+    r = lib().or_trajectory(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv), kx, ky,
+                            _dp(out), ctypes.byref(st))
+    if r < 0:
+        raise ValueError("unsupported model/solver combination")
+    return out, st, r
+
+
+def current_density(m: Model, p: Pulse, tc: Time, sv: Solver, kx, ky, w, n_threads: int = 0,
+                    fast: bool = False):
+    """j(t_i), i = 0..N, and the solver statistics.  Deterministic for any
+    thread count (per-chunk partial rows summed in chunk order)."""
+    kx = np.ascontiguousarray(kx, dtype=np.float64)
+    ky = np.ascontiguousarray(ky, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    j = np.zeros(tc.n_measurements + 1)
+    st = Stats()
+    r = lib(fast).or_current_density(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv),
+                                     _dp(kx), _dp(ky), _dp(w), len(kx), _dp(j), ctypes.byref(st), n_threads)
+    if r < 0:
+        raise ValueError("unsupported model/solver combination")
+    return j, st
+
+
+# --------------------------------------------------------------------------
+# Quadrature grids (restating the reference's GL mappings)
+# --------------------------------------------------------------------------
+
+def gl_box(nx: int, ny: int, x0: float, x1: float, y0: float, y1: float):
+    """Tensor GL grid on [x0,x1]x[y0,y1], x-major (x index slowest)."""
+    gx, wx = gauss_legendre(nx)
+    gy, wy = gauss_legendre(ny)
+    X = 0.5 * (x1 - x0) * gx + 0.5 * (x1 + x0)
+    Y = 0.5 * (y1 - y0) * gy + 0.5 * (y1 + y0)
+    kx = np.repeat(X, ny)
+    ky = np.tile(Y, nx)
+    w = np.outer(wx * 0.5 * (x1 - x0), wy * 0.5 * (y1 - y0)).ravel()
+    return kx, ky, w
+
+
+def gl_hex(nx: int, ny: int):
+    """Hexagonal-BZ GL grid of sources/HHG/Systems/Honeycomb.cpp:229-268:
+    x in [-2pi/3, 2pi/3], y in [-ymax(x), ymax(x)], ymax = 4pi/(3 sqrt 3) - |x|/sqrt 3."""
+    gx, wx = gauss_legendre(nx)
+    gy, wy = gauss_legendre(ny)
+    X = (2 * np.pi / 3) * gx
+    ymax = 4 * np.pi / (3 * np.sqrt(3)) - np.abs(X) / np.sqrt(3)
+    kx = np.repeat(X, ny)
+    ky = (ymax[:, None] * gy[None, :]).ravel()
+    w = ((2 * np.pi / 3) * wx[:, None] * ymax[:, None] * wy[None, :]).ravel()
+    return kx, ky, w
+
+
+# --------------------------------------------------------------------------
+# Spectrum
+# --------------------------------------------------------------------------
+
+def trapezoidal_spectrum(j: np.ndarray, t_begin: float, t_end: float, n_measurements: int, zero_padding: int = 8):
+    """Restates TrapezoidalFFT::compute (sources/HHG/Fourier/TrapezoidalFFT.cpp:11-49)
+    applied to j zero-padded to zero_padding*(N+1) samples (sources/hhg.cpp:277-282),
+    including the reference's conventions: the upper-end correction reads the last
+    element of the padded input (zero after padding), the DFT term is scaled by
+    delta_t twice, and the phase factor uses exp(+i w t_begin) with FFTW's
+    forward (exp(-i...)) transform.  Returns (frequencies, real, imag)."""
+    delta = (t_end - t_begin) / n_measurements
+    x = np.zeros(zero_padding * len(j))
+    x[:len(j)] = j
+    n_input = len(x)
+    Y = np.fft.rfft(x)  # FFTW r2c forward: sum x_n exp(-2 pi i k n / N)
+    freq = 2 * np.arange(len(Y)) * np.pi / (n_input * delta)
+    theta = freq * delta
+    small = np.abs(theta) < 5.684341886080802e-14
+    th = np.where(small, 1.0, theta)
+    W = np.where(small, 1.0, 2.0 * (1.0 - np.cos(th)) / (th * th))
+    a0 = np.where(small, -0.5 + 0j, (-(1.0 - np.cos(th)) + 1j * (th - np.sin(th))) / (th * th))
+    corr = a0 * x[0] + np.conj(a0) * np.exp(1j * freq * (t_end - t_begin)) * x[n_input - 1]
+    out = delta * W * np.exp(1j * freq * t_begin) * Y + corr
+    out = delta * out
+    return freq, out.real.copy(), out.imag.copy()

@@ -1,0 +1,593 @@
+/*
+ * hhg_oracle.c — CPU restatement of the HigherHarmonics per-k-point Bloch solve.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline
+ * that bench.py times.  Nothing in the product links or loads this file.
+ *
+ * What it restates (reference paths relative to the reference repository):
+ *  - pulses: Laser::laser_function = A0 * envelope(t) * sin(t)
+ *      (sources/HHG/Laser/Laser.hpp:32-54), cos^2 envelope
+ *      (sources/HHG/Laser/CosineLaser.cpp:22-26), Gaussian envelope
+ *      (sources/HHG/Laser/GaussLaser.cpp:26-28), A/B pair where B is the same
+ *      base pulse evaluated at t - delay (sources/HHG/Laser/DoubleLaser.hpp:63-67),
+ *      evaluated analytically instead of through the reference's cubic B-spline;
+ *  - Bloch equation ds/dt = 2 d(k - A(t)) x s in the lab frame
+ *      (PiFlux: sources/HHG/Systems/PiFlux.cpp:89-94, prefactor 4t = 2 * 2t;
+ *       honeycomb: sources/HHG/Systems/Honeycomb.cpp:130-138, omega = 2t(Re g, -Im g, 0));
+ *  - relaxation toward the initial state (sources/HHG/Systems/DiracSystem.cpp:231-233)
+ *      and, for the 2D configurations, transverse T2 dephasing about d(k - A(t));
+ *  - Boost.Odeint runge_kutta_fehlberg78 + make_controlled + integrate_adaptive,
+ *      restarted at dt = Delta / n_subdivisions every sampling interval
+ *      (sources/HHG/Systems/DiracSystem.cpp:159-173, Honeycomb.cpp:145-157,
+ *       PiFlux.cpp:96-110; tolerances passed by the caller);
+ *  - the quadrature-weighted current sum j(t_i) = sum_k w_k g_k(t_i)
+ *      (sources/hhg_gauss.cpp:71-133, sources/HHG/Systems/Honeycomb.cpp:225-336,
+ *       sources/HHG/Systems/PiFlux.cpp:794-863);
+ *  - a 4th-order (two-node Gauss) Magnus step with the closed-form SO(3)
+ *      exponential, the structural analogue of the reference's Magnus path
+ *      (sources/HHG/Systems/DiracSystem.cpp:176-211,
+ *       sources/HHG/Systems/DiracDetail/MagnusMatrix.cpp:64-74).
+ *
+ * Boost.Odeint is a third-party dependency absent from the reference tree
+ * (cmake/user_settings.cmake:3 points at Boost 1.78.0).  Its published
+ * controlled_runge_kutta / default_error_checker / default_step_adjuster /
+ * integrate_adaptive algorithm is restated in rkf78_* below.
+ */
+#include "hhg_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI 3.14159265358979323846264338327950288
+#define OR_SQRT3 1.73205080756887729352744634150587237
+
+/* ------------------------------------------------------------------------- */
+/* Pulses                                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* cos^2 envelope on (t0, t0 + 2 pi n): 0.5 (1 - cos((t - t0) / n)).
+ * sources/HHG/Laser/CosineLaser.cpp:22-26, duration CosineLaser.hpp:13-15. */
+static double env_cos2(double t, double t0, double n) {
+    const double te = t0 + 2.0 * OR_PI * n;
+    if (t <= t0 || t >= te) return 0.0;
+    return 0.5 * (1.0 - cos((t - t0) / n));
+}
+
+/* Gaussian envelope exp(-(t - c)^2 / (2 sigma^2)).
+ * sources/HHG/Laser/GaussLaser.cpp:26-28. */
+static double env_gauss(double t, double c, double sigma) {
+    const double x = t - c;
+    return exp(-(x * x) / (2.0 * sigma * sigma));
+}
+
+double or_pulse_A(const or_pulse* p, double t) {
+    switch (p->kind) {
+    case OR_PULSE_COS2:
+        return p->amp_a * env_cos2(t, p->t0, p->n_cycles) * sin(t + p->cep);
+    case OR_PULSE_GAUSS:
+        return p->amp_a * env_gauss(t, p->t0, p->sigma) * sin(t + p->cep);
+    case OR_PULSE_DOUBLE_GAUSS:
+        /* DoubleLaser.hpp:63-67: B = base laser at (t - delay). */
+        return p->amp_a * env_gauss(t, p->t0, p->sigma) * sin(t + p->cep) +
+               p->amp_b * env_gauss(t - p->delay, p->t0, p->sigma) * sin(t - p->delay + p->cep);
+    case OR_PULSE_DOUBLE_COS2:
+        return p->amp_a * env_cos2(t, p->t0, p->n_cycles) * sin(t + p->cep) +
+               p->amp_b * env_cos2(t - p->delay, p->t0, p->n_cycles) * sin(t - p->delay + p->cep);
+    default:
+        return NAN;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gauss-Legendre nodes (Newton on P_n in long double)                       */
+/* ------------------------------------------------------------------------- */
+
+void or_gauss_legendre(int n, double* x, double* w) {
+    /* Nodes are symmetric; compute the positive half by Newton's method on
+     * the three-term recurrence (n+1) P_{n+1} = (2n+1) x P_n - n P_{n-1}. */
+    const int m = (n + 1) / 2;
+    for (int i = 0; i < m; ++i) {
+        long double z = cosl((long double)OR_PI * ((long double)i + 0.75L) / ((long double)n + 0.5L));
+        long double pp = 0.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p1 = 1.0L, p2 = 0.0L;
+            for (int j = 1; j <= n; ++j) {
+                long double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / (long double)j;
+            }
+            pp = (long double)n * (z * p1 - p2) / (z * z - 1.0L);
+            long double z1 = z;
+            z = z1 - p1 / pp;
+            if (fabsl(z - z1) <= 4.0L * LDBL_EPSILON) {
+                /* one more evaluation at the converged node for the weight */
+                p1 = 1.0L; p2 = 0.0L;
+                for (int j = 1; j <= n; ++j) {
+                    long double p3 = p2;
+                    p2 = p1;
+                    p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / (long double)j;
+                }
+                pp = (long double)n * (z * p1 - p2) / (z * z - 1.0L);
+                break;
+            }
+        }
+        const long double wi = 2.0L / ((1.0L - z * z) * pp * pp);
+        /* ascending order: negative node first */
+        x[i] = (double)(-z);
+        w[i] = (double)wi;
+        x[n - 1 - i] = (double)z;
+        w[n - 1 - i] = (double)wi;
+    }
+    if (n & 1) x[n / 2] = 0.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Fehlberg 7(8) tableau (E. Fehlberg, NASA TR R-287, 1968, Table X)          */
+/* ------------------------------------------------------------------------- */
+
+static const double FC[13] = {0.0,       2.0 / 27.0, 1.0 / 9.0, 1.0 / 6.0, 5.0 / 12.0, 1.0 / 2.0, 5.0 / 6.0,
+                              1.0 / 6.0, 2.0 / 3.0,  1.0 / 3.0, 1.0,       0.0,        1.0};
+
+static const double FA[13][12] = {
+    {0},
+    {2.0 / 27.0},
+    {1.0 / 36.0, 1.0 / 12.0},
+    {1.0 / 24.0, 0.0, 1.0 / 8.0},
+    {5.0 / 12.0, 0.0, -25.0 / 16.0, 25.0 / 16.0},
+    {1.0 / 20.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 5.0},
+    {-25.0 / 108.0, 0.0, 0.0, 125.0 / 108.0, -65.0 / 27.0, 125.0 / 54.0},
+    {31.0 / 300.0, 0.0, 0.0, 0.0, 61.0 / 225.0, -2.0 / 9.0, 13.0 / 900.0},
+    {2.0, 0.0, 0.0, -53.0 / 6.0, 704.0 / 45.0, -107.0 / 9.0, 67.0 / 90.0, 3.0},
+    {-91.0 / 108.0, 0.0, 0.0, 23.0 / 108.0, -976.0 / 135.0, 311.0 / 54.0, -19.0 / 60.0, 17.0 / 6.0, -1.0 / 12.0},
+    {2383.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0, -301.0 / 82.0, 2133.0 / 4100.0, 45.0 / 82.0,
+     45.0 / 164.0, 18.0 / 41.0},
+    {3.0 / 205.0, 0.0, 0.0, 0.0, 0.0, -6.0 / 41.0, -3.0 / 205.0, -3.0 / 41.0, 3.0 / 41.0, 6.0 / 41.0, 0.0},
+    {-1777.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0, -289.0 / 82.0, 2193.0 / 4100.0, 51.0 / 82.0,
+     33.0 / 164.0, 12.0 / 41.0, 0.0, 1.0},
+};
+
+/* 8th-order weights (propagated: odeint's runge_kutta_fehlberg78 has order 8,
+ * error order 7) and the embedded 7th-order weights. */
+static const double FB8[13] = {0.0, 0.0, 0.0, 0.0, 0.0, 34.0 / 105.0, 9.0 / 35.0, 9.0 / 35.0, 9.0 / 280.0,
+                               9.0 / 280.0, 0.0, 41.0 / 840.0, 41.0 / 840.0};
+static const double FB7[13] = {41.0 / 840.0, 0.0, 0.0, 0.0, 0.0, 34.0 / 105.0, 9.0 / 35.0, 9.0 / 35.0,
+                               9.0 / 280.0, 9.0 / 280.0, 41.0 / 840.0, 0.0, 0.0};
+
+void or_rkf78_tableau(double* c, double* a, double* b8, double* b7) {
+    for (int i = 0; i < 13; ++i) {
+        c[i] = FC[i];
+        b8[i] = FB8[i];
+        b7[i] = FB7[i];
+        for (int j = 0; j < 13; ++j) a[i * 13 + j] = (j < 12 && j < i) ? FA[i][j] : 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Models                                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* Honeycomb nearest-neighbour vectors, sources/HHG/Systems/Honeycomb.hpp:23-27. */
+static const double NN[3][2] = {{1.0, 0.0}, {-0.5, 0.5 * OR_SQRT3}, {-0.5, -0.5 * OR_SQRT3}};
+
+/* d(q) with q = k - A e_theta.  2D Dirac: (v qx, v qy, Delta);
+ * PiFlux-2D: 2t (cos qx, cos qy, 0); honeycomb: t (Re g, -Im g, 0),
+ * g(q) = sum_delta exp(i q.delta) (Honeycomb.cpp:44-50). */
+static void model_d(const or_model* m, double qx, double qy, double d[3]) {
+    switch (m->kind) {
+    case OR_MODEL_DIRAC2D:
+        d[0] = m->v * qx;
+        d[1] = m->v * qy;
+        d[2] = m->mass;
+        break;
+    case OR_MODEL_PIFLUX2D:
+        d[0] = 2.0 * m->hopping * cos(qx);
+        d[1] = 2.0 * m->hopping * cos(qy);
+        d[2] = 0.0;
+        break;
+    case OR_MODEL_HONEYCOMB: {
+        double gr = 0.0, gi = 0.0;
+        for (int n = 0; n < 3; ++n) {
+            const double ph = NN[n][0] * qx + NN[n][1] * qy;
+            gr += cos(ph);
+            gi += sin(ph);
+        }
+        d[0] = m->hopping * gr;
+        d[1] = -m->hopping * gi;
+        d[2] = 0.0;
+        break;
+    }
+    default:
+        d[0] = d[1] = d[2] = NAN;
+    }
+}
+
+/* Fermi function, sources/HHG/GlobalDefinitions.hpp:36-41 (beta <= 0: T = 0). */
+static double fermi(double e, double beta) {
+    if (!(beta > 0.0)) return e != 0.0 ? (e < 0.0 ? 1.0 : 0.0) : 0.5;
+    return 1.0 / (1.0 + exp(beta * e));
+}
+
+void or_initial_state(const or_model* m, double kx, double ky, double s[3]) {
+    double d[3];
+    model_d(m, kx, ky, d);
+    const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (m->kind == OR_MODEL_HONEYCOMB) {
+        /* Honeycomb::ic_sigma (Honeycomb.cpp:355-361) with the occupations of
+         * Honeycomb.cpp:347-353: alpha2 = f(E_F - t|g|), beta2 = f(E_F + t|g|). */
+        const double g = e / m->hopping; /* |g| */
+        const double a2 = fermi(m->fermi_energy - m->hopping * g, m->beta);
+        const double b2 = fermi(m->fermi_energy + m->hopping * g, m->beta);
+        const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
+        if (e == 0.0) { s[0] = s[1] = 0.0; s[2] = -prod; return; }
+        /* Re g = d0 / t, Im g = -d1 / t */
+        s[0] = -diff * (d[0] / m->hopping) / g;
+        s[1] = diff * (-d[1] / m->hopping) / g;
+        s[2] = -prod;
+        return;
+    }
+    /* 2D Dirac / PiFlux: band-diagonal thermal state (n_upper - n_lower) d_hat,
+     * equal to -d_hat at T = 0, E_F = 0 as in DiracSystem.cpp:155-158. */
+    if (e == 0.0) { s[0] = s[1] = s[2] = 0.0; return; }
+    const double nu = fermi(e - m->fermi_energy, m->beta);
+    const double nl = fermi(-e - m->fermi_energy, m->beta);
+    const double f = (nu - nl) / e;
+    s[0] = f * d[0];
+    s[1] = f * d[1];
+    s[2] = f * d[2];
+}
+
+void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const double s_init[3], const double s[3],
+            double t, double out[3]) {
+    const double A = or_pulse_A(p, t);
+    const double qx = kx - A * cos(m->theta), qy = ky - A * sin(m->theta);
+    double d[3];
+    model_d(m, qx, qy, d);
+    const double wx = 2.0 * d[0], wy = 2.0 * d[1], wz = 2.0 * d[2];
+    out[0] = wy * s[2] - wz * s[1];
+    out[1] = wz * s[0] - wx * s[2];
+    out[2] = wx * s[1] - wy * s[0];
+    if (m->relax == OR_RELAX_TO_INITIAL) {
+        const double r = 1.0 / m->relax_time;
+        for (int c = 0; c < 3; ++c) out[c] -= (s[c] - s_init[c]) * r;
+    } else if (m->relax == OR_RELAX_T2) {
+        const double r = 1.0 / m->relax_time;
+        const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double proj = dd > 0.0 ? (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) / dd : 0.0;
+        for (int c = 0; c < 3; ++c) out[c] -= (s[c] - proj * d[c]) * r;
+    }
+}
+
+double or_vertex(const or_model* m, double kx, double ky, double A, const double s[3]) {
+    const double ct = cos(m->theta), st = sin(m->theta);
+    const double qx = kx - A * ct, qy = ky - A * st;
+    switch (m->kind) {
+    case OR_MODEL_DIRAC2D:
+        /* -(d d / d k_par) . s */
+        return -m->v * (ct * s[0] + st * s[1]);
+    case OR_MODEL_PIFLUX2D:
+        /* reference vertex sin(k - A) s (PiFlux.cpp:831-838) along the field */
+        return ct * sin(qx) * s[0] + st * sin(qy) * s[1];
+    case OR_MODEL_HONEYCOMB: {
+        /* shifted_grad_gamma (Honeycomb.cpp:52-61), contraction Honeycomb.cpp:273-279,
+         * projection on the polarisation Honeycomb.cpp:328-334. */
+        double gxr = 0, gxi = 0, gyr = 0, gyi = 0;
+        for (int n = 0; n < 3; ++n) {
+            const double ph = NN[n][0] * qx + NN[n][1] * qy;
+            const double c = cos(ph), sn = sin(ph);
+            gxr += NN[n][0] * c;
+            gxi += NN[n][0] * sn;
+            gyr += NN[n][1] * c;
+            gyi += NN[n][1] * sn;
+        }
+        const double jx = gxr * s[0] + gxi * s[1];
+        const double jy = gyr * s[0] + gyi * s[1];
+        return ct * jx + st * jy;
+    }
+    default:
+        return NAN;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* RKF78 with the odeint controller                                           */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    const or_model* m;
+    const or_pulse* p;
+    double kx, ky;
+    double s_init[3];
+    double rtol, atol;
+    int max_fail;
+    or_stats* st;
+} or_ctx;
+
+static void ctx_rhs(or_ctx* c, const double s[3], double t, double out[3]) {
+    or_rhs(c->m, c->p, c->kx, c->ky, c->s_init, s, t, out);
+    c->st->rhs_evals++;
+}
+
+/* controlled_runge_kutta::try_step for an explicit error stepper: derivative at
+ * the start, one embedded step, error = max_i |err_i| / (atol + rtol (|x_i| +
+ * dt |dxdt_i|)), reject with dt *= max(0.9 err^(-1/(q_err - 1)), 0.2),
+ * q_err = 7; accept, and for err < 0.5 grow dt *= 0.9 max(err, 5^-q)^(-1/q),
+ * q = 8 (at most 4.5x).  Returns 1 on acceptance. */
+static int rkf78_try_step(or_ctx* c, double s[3], double* t, double* dt) {
+    double k[13][3];
+    const double h = *dt;
+    ctx_rhs(c, s, *t, k[0]);
+    for (int i = 1; i < 13; ++i) {
+        double x[3];
+        for (int q = 0; q < 3; ++q) {
+            double acc = 0.0;
+            for (int j = 0; j < i; ++j) acc += FA[i][j] * k[j][q];
+            x[q] = s[q] + h * acc;
+        }
+        ctx_rhs(c, x, *t + FC[i] * h, k[i]);
+    }
+    double snew[3], err = 0.0;
+    int has_nan = 0;
+    for (int q = 0; q < 3; ++q) {
+        double acc = 0.0;
+        for (int j = 0; j < 13; ++j) acc += FB8[j] * k[j][q];
+        snew[q] = s[q] + h * acc;
+        const double e = h * (41.0 / 840.0) * (k[0][q] + k[10][q] - k[11][q] - k[12][q]);
+        const double sc = c->atol + c->rtol * (fabs(s[q]) + fabs(h) * fabs(k[0][q]));
+        const double r = fabs(e) / sc;
+        if (isnan(r))
+            has_nan = 1;
+        else if (r > err)
+            err = r;
+    }
+    /* a NaN error is not > 1 and is accepted, as in odeint; the caller's
+     * finiteness check then marks the trajectory failed */
+    if (has_nan) err = NAN;
+    c->st->attempts++;
+    if (err > 1.0) {
+        double f = 0.9 * pow(err, -1.0 / 6.0);
+        if (f < 0.2) f = 0.2;
+        *dt = h * f;
+        c->st->rejects++;
+        return 0;
+    }
+    *t += h;
+    s[0] = snew[0];
+    s[1] = snew[1];
+    s[2] = snew[2];
+    if (err < 0.5) {
+        const double e8 = pow(5.0, -8.0);
+        if (err < e8) err = e8;
+        *dt = h * 0.9 * pow(err, -1.0 / 8.0);
+    }
+    return 1;
+}
+
+/* integrate_adaptive over [t0, t1] starting from step dt (by reference: on
+ * return dt holds the controller's proposal; last_unclipped the proposal that
+ * was in force before the final step was clipped to t1).  Loop and clip
+ * tests use the absolute DBL_EPSILON exactly like odeint's less_with_sign.
+ * Returns 0, or 1 when a step needed more than max_fail attempts or made no
+ * progress. */
+static int rkf78_integrate(or_ctx* c, double s[3], double t0, double t1, double* dt, double* last_unclipped) {
+    double t = t0;
+    double h = *dt;
+    double unclipped = h;
+    long guard = 0;
+    while (t1 - t > DBL_EPSILON) {
+        unclipped = h;
+        if ((t + h) - t1 > DBL_EPSILON) h = t1 - t;
+        int tries = 0;
+        for (;;) {
+            if (!(h > 0.0) || !isfinite(h)) return 1;
+            const int ok = rkf78_try_step(c, s, &t, &h);
+            ++tries;
+            if (ok) break;
+            if (tries > c->max_fail) return 1;
+        }
+        if (++guard > 100000000L) return 1;
+        if (!isfinite(s[0]) || !isfinite(s[1]) || !isfinite(s[2])) return 1;
+    }
+    *dt = h;
+    if (last_unclipped) *last_unclipped = unclipped;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Magnus 4 (two-node Gauss) with the closed-form rotation                   */
+/* ------------------------------------------------------------------------- */
+
+static void magnus4_step(const or_model* m, const or_pulse* p, double kx, double ky, double s[3], double t,
+                         double h) {
+    const double c1 = 0.5 - OR_SQRT3 / 6.0, c2 = 0.5 + OR_SQRT3 / 6.0;
+    double w1[3], w2[3];
+    const double ct = cos(m->theta), st = sin(m->theta);
+    const double A1 = or_pulse_A(p, t + c1 * h), A2 = or_pulse_A(p, t + c2 * h);
+    model_d(m, kx - A1 * ct, ky - A1 * st, w1);
+    model_d(m, kx - A2 * ct, ky - A2 * st, w2);
+    for (int q = 0; q < 3; ++q) { w1[q] *= 2.0; w2[q] *= 2.0; }
+    /* Omega = h/2 (w1 + w2) + sqrt(3)/12 h^2 (w2 x w1) */
+    const double f = OR_SQRT3 / 12.0 * h * h;
+    double O[3];
+    O[0] = 0.5 * h * (w1[0] + w2[0]) + f * (w2[1] * w1[2] - w2[2] * w1[1]);
+    O[1] = 0.5 * h * (w1[1] + w2[1]) + f * (w2[2] * w1[0] - w2[0] * w1[2]);
+    O[2] = 0.5 * h * (w1[2] + w2[2]) + f * (w2[0] * w1[1] - w2[1] * w1[0]);
+    const double th2 = O[0] * O[0] + O[1] * O[1] + O[2] * O[2];
+    const double th = sqrt(th2);
+    double sa, sb; /* sin(th)/th, (1 - cos th)/th^2 */
+    if (th < 1e-4) {
+        sa = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+        sb = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+    } else {
+        sa = sin(th) / th;
+        const double hs = sin(0.5 * th);
+        sb = 2.0 * hs * hs / th2;
+    }
+    double oxs[3] = {O[1] * s[2] - O[2] * s[1], O[2] * s[0] - O[0] * s[2], O[0] * s[1] - O[1] * s[0]};
+    double oxoxs[3] = {O[1] * oxs[2] - O[2] * oxs[1], O[2] * oxs[0] - O[0] * oxs[2], O[0] * oxs[1] - O[1] * oxs[0]};
+    for (int q = 0; q < 3; ++q) s[q] += sa * oxs[q] + sb * oxoxs[q];
+}
+
+/* ------------------------------------------------------------------------- */
+/* Trajectories and the current sum                                           */
+/* ------------------------------------------------------------------------- */
+
+static double sample_time(const or_time* tc, int i) {
+    const double delta = (tc->t_end - tc->t_begin) / tc->n_measurements;
+    return tc->t_begin + (double)i * delta;
+}
+
+/* Evolves one k-point and calls emit(i, s) at every sample.  Returns 1 on a
+ * controller failure (the remaining samples are emitted from the frozen
+ * state), 0 otherwise, or -1 for an unsupported combination. */
+typedef void (*emit_fn)(void* user, int i, const double s[3]);
+
+static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv, double kx, double ky,
+                  or_stats* st, emit_fn emit, void* user) {
+    or_ctx c;
+    c.m = m;
+    c.p = p;
+    c.kx = kx;
+    c.ky = ky;
+    c.rtol = sv->rtol;
+    c.atol = sv->atol;
+    c.max_fail = sv->max_fail > 0 ? sv->max_fail : 500;
+    c.st = st;
+    double s[3];
+    or_initial_state(m, kx, ky, s);
+    c.s_init[0] = s[0];
+    c.s_init[1] = s[1];
+    c.s_init[2] = s[2];
+    emit(user, 0, s);
+    const int N = tc->n_measurements;
+    const double delta = (tc->t_end - tc->t_begin) / N;
+    if (sv->method == OR_METHOD_MAGNUS4) {
+        if (m->relax != OR_RELAX_NONE) return -1;
+        const int ms = sv->magnus_substeps > 0 ? sv->magnus_substeps : 1;
+        const double h = delta / ms;
+        for (int i = 1; i <= N; ++i) {
+            const double t0 = sample_time(tc, i - 1);
+            for (int q = 0; q < ms; ++q) magnus4_step(m, p, kx, ky, s, t0 + q * h, h);
+            st->attempts += ms;
+            emit(user, i, s);
+        }
+        return 0;
+    }
+    int failed = 0;
+    const double dt0 = delta / tc->n_subdivisions;
+    double dt = dt0;
+    for (int i = 1; i <= N; ++i) {
+        if (!failed) {
+            double h = (sv->proposal == OR_PROPOSAL_CARRY) ? dt : dt0;
+            double unclipped = h;
+            if (rkf78_integrate(&c, s, sample_time(tc, i - 1), sample_time(tc, i), &h, &unclipped)) {
+                failed = 1;
+            } else {
+                /* carry mode continues from the proposal of the last
+                 * unclipped step when the final step was clipped */
+                dt = h > unclipped ? h : unclipped;
+            }
+        }
+        emit(user, i, s);
+    }
+    return failed;
+}
+
+typedef struct {
+    double* out;
+} traj_user;
+
+static void emit_traj(void* user, int i, const double s[3]) {
+    traj_user* u = (traj_user*)user;
+    u->out[3 * i + 0] = s[0];
+    u->out[3 * i + 1] = s[1];
+    u->out[3 * i + 2] = s[2];
+}
+
+int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv, double kx, double ky,
+                  double* s_out, or_stats* st) {
+    traj_user u = {s_out};
+    or_stats local = {0, 0, 0, 0};
+    const int r = evolve(m, p, tc, sv, kx, ky, st ? st : &local, emit_traj, &u);
+    if (r > 0 && st) st->failed_trajectories++;
+    return r;
+}
+
+typedef struct {
+    const or_model* m;
+    const double* A;  /* A(t_i) */
+    double kx, ky, w;
+    double* row;
+} acc_user;
+
+static void emit_acc(void* user, int i, const double s[3]) {
+    acc_user* u = (acc_user*)user;
+    u->row[i] += u->w * or_vertex(u->m, u->kx, u->ky, u->A[i], s);
+}
+
+#define OR_CHUNK 64
+
+int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
+                           const double* kx, const double* ky, const double* w, int64_t n_k, double* j_out,
+                           or_stats* st, int n_threads) {
+    const int N = tc->n_measurements;
+    const int64_t n_chunks = (n_k + OR_CHUNK - 1) / OR_CHUNK;
+    double* A = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+    double* partial = (double*)calloc((size_t)n_chunks * (size_t)(N + 1), sizeof(double));
+    if (!A || !partial) {
+        free(A);
+        free(partial);
+        return -1;
+    }
+    for (int i = 0; i <= N; ++i) A[i] = or_pulse_A(p, sample_time(tc, i));
+    int64_t failed = 0, attempts = 0, rejects = 0, rhs = 0;
+    int unsupported = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : failed, attempts, rejects, rhs, unsupported)
+    for (int64_t c = 0; c < n_chunks; ++c) {
+        or_stats ls = {0, 0, 0, 0};
+        double* row = partial + (size_t)c * (size_t)(N + 1);
+        const int64_t k_end = (c + 1) * OR_CHUNK < n_k ? (c + 1) * OR_CHUNK : n_k;
+        for (int64_t k = c * OR_CHUNK; k < k_end; ++k) {
+            acc_user u = {m, A, kx[k], ky[k], w[k], row};
+            const int r = evolve(m, p, tc, sv, kx[k], ky[k], &ls, emit_acc, &u);
+            if (r > 0) failed++;
+            if (r < 0) unsupported++;
+        }
+        attempts += ls.attempts;
+        rejects += ls.rejects;
+        rhs += ls.rhs_evals;
+    }
+    /* deterministic compensated (Neumaier) sum over chunks in index order */
+    for (int i = 0; i <= N; ++i) {
+        double sum = 0.0, comp = 0.0;
+        for (int64_t c = 0; c < n_chunks; ++c) {
+            const double x = partial[(size_t)c * (size_t)(N + 1) + i];
+            const double t = sum + x;
+            if (fabs(sum) >= fabs(x))
+                comp += (sum - t) + x;
+            else
+                comp += (x - t) + sum;
+            sum = t;
+        }
+        j_out[i] = sum + comp;
+    }
+    free(A);
+    free(partial);
+    if (st) {
+        st->attempts += attempts;
+        st->rejects += rejects;
+        st->rhs_evals += rhs;
+        st->failed_trajectories += failed;
+    }
+    if (unsupported) return -1;
+    return failed;
+}

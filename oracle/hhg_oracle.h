@@ -1,0 +1,108 @@
+/*
+ * hhg_oracle.h — CPU restatement of the HigherHarmonics per-k-point Bloch
+ * solve and current reduction.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path and the CPU baseline that bench.py times.  Only tests/, the smoke
+ * entry point and bench.py's CPU-baseline leg load it.  Product code never
+ * links, loads or calls it.
+ *
+ * Everything is FP64 (reference: sources/HHG/GlobalDefinitions.hpp:14).
+ */
+#ifndef HHG_ORACLE_H
+#define HHG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_MODEL_DIRAC2D = 0, OR_MODEL_PIFLUX2D = 1, OR_MODEL_HONEYCOMB = 2 };
+enum { OR_PULSE_COS2 = 0, OR_PULSE_GAUSS = 1, OR_PULSE_DOUBLE_GAUSS = 2, OR_PULSE_DOUBLE_COS2 = 3 };
+enum { OR_RELAX_NONE = 0, OR_RELAX_TO_INITIAL = 1, OR_RELAX_T2 = 2 };
+enum { OR_METHOD_RKF78 = 0, OR_METHOD_MAGNUS4 = 1 };
+enum { OR_PROPOSAL_RESTART = 0, OR_PROPOSAL_CARRY = 1 };
+
+typedef struct {
+    int32_t kind;
+    int32_t relax;
+    double v;          /* Dirac-2D velocity */
+    double mass;       /* Dirac-2D gap parameter Delta */
+    double hopping;    /* PiFlux / honeycomb t */
+    double theta;      /* polarisation angle of A(t) in the kx-ky plane */
+    double fermi_energy;
+    double beta;       /* inverse temperature; <= 0 means T = 0 */
+    double relax_time;
+} or_model;
+
+typedef struct {
+    int32_t kind;
+    int32_t pad;
+    double amp_a;      /* peak vector-potential amplitude of pulse A */
+    double amp_b;      /* of pulse B (double pulses) */
+    double n_cycles;   /* cos^2: envelope lasts 2*pi*n_cycles */
+    double sigma;      /* Gaussian: envelope width */
+    double t0;         /* cos^2: window begin; Gaussian: centre of pulse A */
+    double delay;      /* B pulse delay relative to A */
+    double cep;        /* carrier-envelope phase added to the carrier */
+} or_pulse;
+
+typedef struct {
+    double t_begin;
+    double t_end;
+    int32_t n_measurements;
+    int32_t n_subdivisions;
+} or_time;
+
+typedef struct {
+    int32_t method;
+    int32_t proposal;
+    double rtol;
+    double atol;
+    int32_t max_fail;        /* consecutive rejections before a trajectory is marked failed */
+    int32_t magnus_substeps; /* Magnus steps per sampling interval */
+} or_solver;
+
+typedef struct {
+    int64_t attempts;
+    int64_t rejects;
+    int64_t failed_trajectories;
+    int64_t rhs_evals;
+} or_stats;
+
+/* Pulse A(t) (vector potential including its amplitude). */
+double or_pulse_A(const or_pulse* p, double t);
+
+/* Gauss-Legendre nodes/weights on [-1,1], ascending nodes. */
+void or_gauss_legendre(int n, double* x, double* w);
+
+/* Fehlberg 7(8) tableau as used by the oracle: c[13], a[13*13] row-major,
+ * b8[13] (propagated), b7[13] (embedded). */
+void or_rkf78_tableau(double* c, double* a, double* b8, double* b7);
+
+/* Initial Bloch vector of one k-point. */
+void or_initial_state(const or_model* m, double kx, double ky, double s[3]);
+
+/* Bloch right-hand side ds/dt at time t (used by tests against scipy). */
+void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const double s_init[3],
+            const double s[3], double t, double out[3]);
+
+/* Current vertex g(s, A) of one k-point (unweighted). */
+double or_vertex(const or_model* m, double kx, double ky, double A, const double s[3]);
+
+/* One k-point: Bloch vector at every sample t_i, s_out[(N+1)*3]. Returns 0 or
+ * 1 if the controller failed. */
+int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
+                  double kx, double ky, double* s_out, or_stats* st);
+
+/* j(t_i) = sum_k w_k g_k(t_i), i = 0..N, deterministic for any thread count.
+ * Returns the number of failed trajectories. */
+int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
+                           const double* kx, const double* ky, const double* w, int64_t n_k, double* j_out,
+                           or_stats* st, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

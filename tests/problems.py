@@ -1,0 +1,152 @@
+"""Shared problem definitions for the parity tests: one description feeds
+both the oracle (oracle.*) and the HIP path (higherharmonics_amd.*)."""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import oracle as O
+
+
+@dataclass
+class Problem:
+    name: str
+    model: dict
+    pulse: dict
+    time: tuple            # (t_begin, t_end, N, n_sub)
+    grid: tuple            # ("box", nx, ny, x0, x1, y0, y1) or ("hex", nx, ny)
+    solver: dict = field(default_factory=lambda: dict(method=0, rtol=1e-10, atol=1e-10, proposal=0))
+
+    # ---- oracle side ----
+    def o_model(self):
+        return O.model(**self.model)
+
+    def o_pulse(self):
+        return O.pulse(**self.pulse)
+
+    def o_time(self):
+        return O.time_grid(*self.time)
+
+    def o_solver(self, **over):
+        d = dict(self.solver)
+        d.update(over)
+        return O.solver(**d)
+
+    def o_grid(self):
+        if self.grid[0] == "box":
+            return O.gl_box(*self.grid[1:])
+        return O.gl_hex(*self.grid[1:])
+
+    # ---- HIP side ----
+    def h_model(self, H):
+        m = self.model
+        return H.ModelDesc(m.get("kind", 0), m.get("relax", 0), m.get("v", 1.0), m.get("mass", 0.1),
+                           m.get("hopping", 1.0), m.get("theta", 0.0), m.get("fermi_energy", 0.0),
+                           m.get("beta", 0.0), m.get("relax_time", 0.0))
+
+    def h_laser(self, H):
+        p = self.pulse
+        kind = p.get("kind", 0)
+
+        class _L(H.Laser):
+            def desc(_self):
+                return H.PulseDesc(kind, 0, p.get("amp_a", 0.5), p.get("amp_b", 0.0), p.get("n_cycles", 8.0),
+                                   p.get("sigma", 0.0), p.get("t0", 0.0), p.get("delay", 0.0), p.get("cep", 0.0))
+        return _L()
+
+    def h_time(self, H):
+        return H.TimeIntegrationConfig(*self.time)
+
+    def h_grid(self, H):
+        if self.grid[0] == "box":
+            return H.KGrid.box(*self.grid[1:])
+        return H.KGrid.hexagonal(*self.grid[1:])
+
+    def h_solver(self, H, **over):
+        d = dict(self.solver)
+        d.update(over)
+        return H.Solver(method=d.get("method", 0), rtol=d.get("rtol", 1e-10), atol=d.get("atol", 1e-10),
+                        proposal=d.get("proposal", 0), max_fail=d.get("max_fail", 500),
+                        magnus_substeps=d.get("magnus_substeps", 1))
+
+    def h_plan(self, H, **kw):
+        solver_over = kw.pop("solver", {})
+        return H.Plan([H.Job(self.h_model(H), self.h_laser(H), self.h_time(H), 0)], [self.h_grid(H)],
+                      self.h_solver(H, **solver_over), **kw)
+
+
+def oracle_j(pr: Problem, n_threads=0, **solver_over):
+    kx, ky, w = pr.o_grid()
+    return O.current_density(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(**solver_over), kx, ky, w,
+                             n_threads=n_threads)
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+PI = math.pi
+T1 = 16 * PI  # 8 cycles
+
+# BASELINE configuration 1 (full): Dirac-2D, v=1, Delta=0.1, cos^2 A0=0.5, 8 cycles,
+# T2 = 10 cycles, GL 128 x 64 on |k| <= 3, RKF78 1e-10, 4096 samples.
+CONFIG1 = Problem(
+    "config1_dirac_128x64",
+    model=dict(kind=O.MODEL_DIRAC2D, v=1.0, mass=0.1, relax=O.RELAX_T2, relax_time=20 * PI),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=0.5, n_cycles=8.0, t0=-T1 / 2),
+    time=(-T1 / 2, T1 / 2, 4096, 50),
+    grid=("box", 128, 64, -3.0, 3.0, -3.0, 3.0),
+)
+
+# Reduced cases (seconds on the oracle)
+DIRAC_SMALL = Problem(
+    "dirac_t2_small",
+    model=dict(kind=O.MODEL_DIRAC2D, v=1.0, mass=0.1, relax=O.RELAX_T2, relax_time=20 * PI),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=0.5, n_cycles=2.0, t0=-2 * PI),
+    time=(-2 * PI, 2 * PI, 512, 50),
+    grid=("box", 24, 16, -3.0, 3.0, -3.0, 3.0),
+)
+
+DIRAC_COHERENT = Problem(
+    "dirac_coherent_small",
+    model=dict(kind=O.MODEL_DIRAC2D, v=1.0, mass=0.1),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=0.5, n_cycles=2.0, t0=-2 * PI, cep=0.3),
+    time=(-2 * PI, 2 * PI, 512, 50),
+    grid=("box", 24, 16, -3.0, 3.0, -3.0, 3.0),
+)
+
+DIRAC_RELAX_INIT = Problem(
+    "dirac_relax_to_initial",
+    model=dict(kind=O.MODEL_DIRAC2D, v=1.0, mass=0.2, relax=O.RELAX_TO_INITIAL, relax_time=5.0, theta=0.4,
+               fermi_energy=0.05, beta=20.0),
+    pulse=dict(kind=O.PULSE_GAUSS, amp_a=0.4, sigma=PI, t0=0.0),
+    time=(-5 * PI, 5 * PI, 640, 50),
+    grid=("box", 16, 16, -2.5, 2.5, -2.5, 2.5),
+)
+
+PIFLUX_SMALL = Problem(
+    "piflux_double_gauss",
+    model=dict(kind=O.MODEL_PIFLUX2D, hopping=1.0),
+    pulse=dict(kind=O.PULSE_DOUBLE_GAUSS, amp_a=0.6, amp_b=0.6, sigma=PI, t0=0.0, delay=2 * PI, cep=0.7),
+    time=(-5 * PI, 2 * PI + 5 * PI, 700, 50),
+    grid=("box", 24, 24, -PI, PI, -PI, PI),
+)
+
+HONEYCOMB_SMALL = Problem(
+    "honeycomb_cos2",
+    model=dict(kind=O.MODEL_HONEYCOMB, hopping=1.0),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=0.8, n_cycles=2.0, t0=0.0, cep=1.1),
+    time=(0.0, 4 * PI + 2 * PI, 600, 50),
+    grid=("hex", 20, 20),
+)
+
+HONEYCOMB_DOUBLE_COS2 = Problem(
+    "honeycomb_double_cos2",
+    model=dict(kind=O.MODEL_HONEYCOMB, hopping=1.0, fermi_energy=0.1, beta=8.0),
+    pulse=dict(kind=O.PULSE_DOUBLE_COS2, amp_a=0.5, amp_b=0.3, n_cycles=1.5, t0=0.0, delay=PI),
+    time=(0.0, 3 * PI + PI + 2 * PI, 600, 50),
+    grid=("hex", 16, 16),
+)
+
+SMALL_CASES = [DIRAC_SMALL, DIRAC_COHERENT, DIRAC_RELAX_INIT, PIFLUX_SMALL, HONEYCOMB_SMALL, HONEYCOMB_DOUBLE_COS2]

@@ -1,0 +1,146 @@
+"""HIP path vs the CPU oracle, through the C ABI (libhhg_amd.so).
+
+Parity bar (FP64 floating point): relative L2 error of j(t) <= 1e-9 against
+the oracle on the same grid, pulse, time grid and controller semantics; the
+per-harmonic spectrum within 1e-6 relative down to 1e-12 of its peak.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_SMALL, HONEYCOMB_SMALL, PIFLUX_SMALL, SMALL_CASES,
+                            oracle_j, rel_l2)
+
+pytestmark = pytest.mark.gpu
+
+J_TOL = 1e-9
+
+
+def _print_case(pr, **extra):
+    print(f"[case] {pr.name} grid={pr.grid} N={pr.time[2]} {extra}", flush=True)
+
+
+@pytest.mark.parametrize("pr", SMALL_CASES, ids=lambda p: p.name)
+def test_current_density_matches_oracle(hhg, pr):
+    _print_case(pr)
+    j_ref, st_ref = oracle_j(pr)
+    with pr.h_plan(hhg) as plan:
+        j = plan.execute()[0]
+        stats = plan.stats
+    err = rel_l2(j, j_ref)
+    print(f"  rel_l2={err:.3e} attempts gpu={stats.attempts} oracle={st_ref.attempts} "
+          f"rejects gpu={stats.rejects} oracle={st_ref.rejects}", flush=True)
+    assert err <= J_TOL
+    assert stats.failed_trajectories == 0 and stats.nonfinite_trajectories == 0
+    # same controller: the attempted-step totals agree to rounding-level flips
+    assert abs(stats.attempts - st_ref.attempts) <= 1e-3 * st_ref.attempts
+
+
+@pytest.mark.parametrize("pr", [DIRAC_SMALL, PIFLUX_SMALL, HONEYCOMB_SMALL], ids=lambda p: p.name)
+def test_trajectories_match_oracle(hhg, pr):
+    _print_case(pr)
+    kx, ky, w = pr.o_grid()
+    sel = np.array([0, 5, len(kx) // 2, len(kx) - 1])
+    with pr.h_plan(hhg) as plan:
+        s_gpu = plan.trajectories(sel)
+    for n, k in enumerate(sel):
+        s_ref, _, r = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), kx[k], ky[k])
+        assert r == 0
+        assert np.max(np.abs(s_gpu[n] - s_ref)) < 1e-10
+
+
+def test_config1_full_grid(hhg):
+    """BASELINE configuration 1 in full (8192 k-points, 4096 samples)."""
+    pr = CONFIG1
+    _print_case(pr)
+    j_ref, st_ref = oracle_j(pr, n_threads=16)
+    with pr.h_plan(hhg) as plan:
+        j = plan.execute()[0]
+    err = rel_l2(j, j_ref)
+    print(f"  config1 rel_l2={err:.3e}", flush=True)
+    assert err <= J_TOL
+    # spectrum criterion on the product's own spectrum routine vs the oracle's
+    tc = pr.h_time(hhg)
+    f, re, im = hhg.trapezoidal_spectrum(j, tc)
+    f0, re0, im0 = O.trapezoidal_spectrum(j_ref, *pr.time[:3])
+    I, I0 = re ** 2 + im ** 2, re0 ** 2 + im0 ** 2
+    keep = I0 >= 1e-12 * I0.max()
+    assert np.max(np.abs(I[keep] - I0[keep]) / I0[keep]) <= 1e-6
+
+
+def test_deterministic_and_launch_shape_independent(hhg):
+    pr = DIRAC_COHERENT
+    _print_case(pr)
+    outs = []
+    for wpb, qc, pers in [(4, 1, 1), (1, 1, 0), (8, 2, 1), (2, 4, 1)]:
+        with pr.h_plan(hhg, waves_per_block=wpb, queue_chunk=qc, persistent=pers) as plan:
+            outs.append(plan.execute()[0])
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])  # exact fixed-point accumulation
+
+
+def test_shards_sum_to_unsharded_bitwise(hhg):
+    pr = PIFLUX_SMALL
+    _print_case(pr)
+    with pr.h_plan(hhg) as plan:
+        acc_all = plan.execute_accumulators()
+    for n_ranks in (2, 3):
+        total = np.zeros_like(acc_all)
+        for r in range(n_ranks):
+            with pr.h_plan(hhg, rank=r, n_ranks=n_ranks) as plan:
+                hhg.accumulators_add(total, plan.execute_accumulators())
+        assert np.array_equal(total, acc_all)
+
+
+def test_magnus_matches_oracle(hhg):
+    pr = DIRAC_COHERENT
+    _print_case(pr, method="magnus4")
+    j_ref, _ = oracle_j(pr, method=O.METHOD_MAGNUS4, magnus_substeps=5)
+    with pr.h_plan(hhg, solver=dict(method=1, magnus_substeps=5)) as plan:
+        j = plan.execute()[0]
+    assert rel_l2(j, j_ref) <= J_TOL
+
+
+def test_carry_proposal_matches_oracle(hhg):
+    pr = HONEYCOMB_SMALL
+    _print_case(pr, proposal="carry")
+    j_ref, st_ref = oracle_j(pr, proposal=O.PROPOSAL_CARRY)
+    with pr.h_plan(hhg, solver=dict(proposal=1)) as plan:
+        j = plan.execute()[0]
+        att = plan.stats.attempts
+    assert rel_l2(j, j_ref) <= J_TOL
+    assert abs(att - st_ref.attempts) <= 1e-3 * st_ref.attempts
+
+
+def test_graph_replay_matches_eager(hhg):
+    pr = DIRAC_SMALL
+    _print_case(pr, graph=True)
+    with pr.h_plan(hhg) as plan:
+        eager = plan.execute()[0]
+        plan.capture_graph()
+        for _ in range(3):
+            assert np.array_equal(plan.execute()[0], eager)
+            _ = np.sum(eager)  # host work between replays
+
+
+def test_controller_failure_is_reported(hhg):
+    """A controller that cannot meet its tolerance (rtol = atol = 1e-300) and may
+    reject only once per step (max_fail = 1) fails every trajectory; the kernel
+    still finishes (bounded attempts) and the call reports TOLERANCE with the
+    oracle's failure count.  The window starts mid-pulse so no trajectory sits
+    in a stationary state whose error estimate cancels to zero."""
+    import math
+    from tests.problems import Problem
+    pr = Problem("forced_failure", model=dict(kind=O.MODEL_DIRAC2D, v=1.0, mass=0.1),
+                 pulse=dict(kind=O.PULSE_COS2, amp_a=0.5, n_cycles=2.0, t0=-2 * math.pi),
+                 time=(0.3, 4.3, 64, 50), grid=("box", 24, 16, -3.0, 3.0, -3.0, 3.0),
+                 solver=dict(method=0, rtol=1e-300, atol=1e-300, proposal=0, max_fail=1))
+    _print_case(pr, rtol=1e-300, max_fail=1)
+    _, st_ref = oracle_j(pr)
+    assert st_ref.failed_trajectories == 24 * 16
+    with pr.h_plan(hhg) as plan:
+        with pytest.raises(hhg.HHGError) as ei:
+            plan.execute()
+        assert ei.value.status == hhg.ERROR_TOLERANCE
+        assert plan.stats.failed_trajectories == 24 * 16
+        assert plan.stats.nonfinite_trajectories == 0

@@ -1,0 +1,142 @@
+"""The C ABI library on the CPU: it loads, exports every declared symbol, and
+its host utilities (Gauss-Legendre nodes, BZ grids, pulses, spectrum, input
+validation) agree with the oracle and the reference's tables.  No kernel runs."""
+import ctypes
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.problems import SMALL_CASES
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_exports_every_declared_symbol(hhg):
+    header = open(os.path.join(ROOT, "include", "hhg_amd.h")).read()
+    declared = set(re.findall(r"\b(hhg_[a-z0-9_]+)\s*\(", header))
+    assert declared == set(hhg.EXPORTED_SYMBOLS)
+    L = hhg.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.hhg_abi_version() == 1
+
+
+def test_gauss_legendre_matches_reference_tables(hhg):
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "thread_gauss_subset.json")))
+    for n_str, tab in fx["orders"].items():
+        n = int(n_str)
+        x, w = hhg.gauss_legendre(n)
+        ref_x = np.array([float(v) for v in tab["abscissa"]])
+        ref_w = np.array([float(v) for v in tab["weights"]])
+        # container<N> holds the N/2 positive nodes in ascending order
+        assert np.max(np.abs(x[n // 2:] - ref_x)) < 4e-16
+        assert np.max(np.abs(w[n // 2:] - ref_w) / ref_w) < 1e-14
+        assert np.allclose(x[:n // 2], -ref_x[::-1], rtol=0, atol=4e-16)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 30, 64, 255, 1024])
+def test_gauss_legendre_exactness_and_numpy(hhg, n):
+    x, w = hhg.gauss_legendre(n)
+    xn, wn = np.polynomial.legendre.leggauss(n)
+    assert np.max(np.abs(x - xn)) < 1e-14
+    # numpy's weights lose up to ~1e-9 relative near the ends at large n
+    assert np.max(np.abs(w - wn) / wn) < 1e-8
+    ox, ow = O.gauss_legendre(n)
+    assert np.max(np.abs(x - ox)) < 4e-16 and np.max(np.abs(w - ow) / ow) < 1e-13
+    assert abs(w.sum() - 2.0) < 1e-13
+    # exact for polynomials of degree 2n-1: int_{-1}^{1} x^{2m} = 2/(2m+1)
+    for m in range(0, n):
+        assert abs(np.dot(w, x ** (2 * m)) - 2.0 / (2 * m + 1)) < 1e-13
+
+
+def test_kgrids_match_oracle_and_areas(hhg):
+    kx, ky, w = hhg.KGrid.box(32, 16, -3, 3, -2, 2).build()
+    okx, oky, ow = O.gl_box(32, 16, -3, 3, -2, 2)
+    assert np.allclose(kx, okx, atol=1e-15) and np.allclose(ky, oky, atol=1e-15)
+    assert np.allclose(w, ow, rtol=1e-14)
+    assert abs(w.sum() - 24.0) < 1e-12
+    hx, hy, hw = hhg.KGrid.hexagonal(24, 24).build()
+    ox, oy, owh = O.gl_hex(24, 24)
+    assert np.allclose(hx, ox, atol=1e-15) and np.allclose(hy, oy, atol=1e-15) and np.allclose(hw, owh, rtol=1e-14)
+    # hexagonal BZ of the unit honeycomb lattice: area 8 pi^2 / (3 sqrt 3).  The
+    # reference's x-GL crosses the |x| kink of y_max at x = 0 (Honeycomb.cpp:231),
+    # so the area converges as 1/n^2 rather than exponentially.
+    area = 8 * math.pi ** 2 / (3 * math.sqrt(3))
+    errs = [abs(hhg.KGrid.hexagonal(n, n).build()[2].sum() - area) for n in (24, 48, 96)]
+    assert errs[0] < 1e-2 and 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5
+
+
+@pytest.mark.parametrize("pr", SMALL_CASES, ids=lambda p: p.name)
+def test_pulse_matches_oracle(hhg, pr):
+    t = np.linspace(pr.time[0] - 1.0, pr.time[1] + 1.0, 997)
+    A = pr.h_laser(hhg).laser_function(t)
+    A0 = O.pulse_A(pr.o_pulse(), t)
+    assert np.max(np.abs(A - A0)) < 1e-15
+
+
+def test_cos2_pulse_is_the_configured_cos2_form(hhg):
+    """The reference's cos^2 window (CosineLaser.cpp:22-26) with begin shift -T/2
+    is A0 cos^2(pi t/T) sin t on [-T/2, T/2], T = 2 pi n (configuration 1)."""
+    T = 16 * math.pi
+    las = hhg.CosineLaser(0.5, 8.0, -T / 2)
+    t = np.linspace(-T / 2 + 1e-9, T / 2 - 1e-9, 4001)
+    assert np.max(np.abs(las.laser_function(t) - 0.5 * np.cos(np.pi * t / T) ** 2 * np.sin(t))) < 1e-15
+    assert np.all(las.laser_function(np.array([-T / 2 - 1, T / 2 + 1])) == 0.0)
+
+
+def test_spectrum_matches_oracle(hhg):
+    rng = np.random.default_rng(7)
+    tc = hhg.TimeIntegrationConfig(-3.0, 5.0, 255, 50)
+    j = rng.standard_normal(256)
+    f, re, im = hhg.trapezoidal_spectrum(j, tc)
+    f0, re0, im0 = O.trapezoidal_spectrum(j, -3.0, 5.0, 255)
+    assert np.allclose(f, f0, rtol=1e-15)
+    scale = np.max(np.hypot(re0, im0))
+    assert np.max(np.abs(re - re0)) < 1e-13 * scale
+    assert np.max(np.abs(im - im0)) < 1e-13 * scale
+
+
+@pytest.mark.parametrize("mutate,status", [
+    (lambda j, s: setattr(j.model, "kind", 7), 1),
+    (lambda j, s: setattr(j.time, "n_measurements", 0), 1),
+    (lambda j, s: setattr(j.time, "n_measurements", 10), 5),        # Delta > 0.25
+    (lambda j, s: setattr(j.model, "theta", 0.3), 5),               # lattice with theta != 0
+    (lambda j, s: setattr(s, "rtol", -1.0), 1),
+    (lambda j, s: (setattr(s, "method", 1), setattr(j.model, "relax", 2), setattr(j.model, "relax_time", 3.0)), 5),
+    (lambda j, s: setattr(j, "kgrid", 3), 1),
+])
+def test_plan_rejects_invalid_descriptions(hhg, mutate, status):
+    from tests.problems import HONEYCOMB_SMALL as pr
+    jd = hhg.JobDesc(pr.h_model(hhg), pr.h_laser(hhg).desc(), pr.h_time(hhg).desc(), 0, 0)
+    sv = pr.h_solver(hhg).desc()
+    mutate(jd, sv)
+    kg = (hhg.KGridDesc * 1)(pr.h_grid(hhg).desc())
+    jobs = (hhg.JobDesc * 1)(jd)
+    d = hhg.PlanDesc(1, 1, kg, jobs, sv, hhg.LaunchDesc(0, 0, 1, 0), 0, 1, 0, 0)
+    h = ctypes.c_void_p()
+    st = hhg.lib().hhg_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert st == status
+    assert hhg.lib().hhg_last_error_string()
+
+
+def test_gauss_legendre_1024_against_mpmath(hhg):
+    """N = 1024 is beyond the reference tables (they stop at 960)."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 40
+    n = 1024
+    x, w = hhg.gauss_legendre(n)
+    for i in (0, 1, 100, 511):
+        z = mp.mpf(x[i])
+        for _ in range(4):
+            p = mp.legendre(n, z)
+            dp = n * (mp.legendre(n - 1, z) - z * p) / (1 - z * z)
+            z = z - p / dp
+        dp = n * (mp.legendre(n - 1, z) - z * mp.legendre(n, z)) / (1 - z * z)
+        wt = 2 / ((1 - z * z) * dp * dp)
+        assert abs(float(mp.mpf(x[i]) - z)) < 1e-16
+        assert abs(float((mp.mpf(w[i]) - wt) / wt)) < 5e-14
