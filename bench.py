@@ -87,11 +87,12 @@ def cpu_baseline(wl, target_s=10.0):
     sv = O.solver(rtol=wl["rtol"], atol=wl["atol"])
     threads = max(1, min(16, os.cpu_count() or 1))
     rng = np.random.default_rng(20261203)
-    pilot = rng.choice(len(kx), size=threads * 2, replace=False)
+    # pilot large enough that every thread runs several k-chunks of 64
+    pilot = rng.choice(len(kx), size=threads * 256, replace=False)
     t0 = time.perf_counter()
     O.current_density(m, p, tc, sv, kx[pilot], ky[pilot], w[pilot], n_threads=threads, fast=True)
     per = (time.perf_counter() - t0) / len(pilot)
-    n = int(min(len(kx), max(threads * 4, target_s / max(per, 1e-9))))
+    n = int(min(len(kx), max(len(pilot), target_s / max(per, 1e-9))))
     sel = rng.choice(len(kx), size=n, replace=False)
     t0 = time.perf_counter()
     O.current_density(m, p, tc, sv, kx[sel], ky[sel], w[sel], n_threads=threads, fast=True)
@@ -110,7 +111,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--extras", action="store_true", help="also time Magnus-4 and the carry ablation")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 8 * n}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it shards)",
               file=sys.stderr)

@@ -242,7 +242,12 @@ struct Bloch {
         } else if constexpr (R == RELAX_T2) {
             const double r = J.relax_rate;
             const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            const double proj = dd > 0.0 ? (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) / dd : 0.0;
+            // 1/dd: hardware reciprocal refined by two Newton steps (within an
+            // ulp of the quotient, fewer instructions than an IEEE division)
+            double y = __builtin_amdgcn_rcp(dd);
+            y = fma(fma(-dd, y, 1.0), y, y);
+            y = fma(fma(-dd, y, 1.0), y, y);
+            const double proj = dd > 0.0 ? (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) * y : 0.0;
             out[0] -= (s[0] - proj * d[0]) * r;
             out[1] -= (s[1] - proj * d[1]) * r;
             out[2] -= (s[2] - proj * d[2]) * r;
@@ -302,22 +307,47 @@ struct StepResult {
 };
 
 // One embedded RKF78 attempt from (s, t) with step h; k1 = f(s, t) given.
+// Largest step growth of the controller: 0.9 * (5^-8)^(-1/8), evaluated with
+// the same operations as the accept branch of rkf78_interval so that replayed
+// nominal steps are bitwise equal to the integrator's.
+__device__ __forceinline__ double max_growth() {
+    const double e = 1.0 / 390625.0;
+    return 0.9 / sqrt(sqrt(sqrt(e)));
+}
+
+// The pulse does not depend on the state, so A at the 13 stage times of a
+// step is produced first (c_7 = c_3, c_11 = c_0 = 0, c_12 = c_10 are shared):
+// from the nominal-step table row `cand` when its (t, h) equal the step's,
+// else from the per-step pulse cache.  All 15 table words are loaded at once
+// so the match test and the values arrive together.
+template <class F>
+__device__ __forceinline__ void stage_pulses(F& f, double t, double h, const double* cand, double As[13]) {
+    bool hit = false;
+    if (cand) {
+        double cv[kNomStride];
+#pragma unroll
+        for (int j = 0; j < kNomStride; ++j) cv[j] = cand[j];
+        hit = (cv[0] == t) && (cv[1] == h);
+#pragma unroll
+        for (int j = 0; j < 13; ++j) As[j] = cv[2 + j];
+    }
+    if (!hit) {
+        f.prepare(t);
+#pragma unroll
+        for (int i = 0; i < 13; ++i) {
+            if (i == 7) As[i] = As[3];
+            else if (i == 11) As[i] = As[0];
+            else if (i == 12) As[i] = As[10];
+            else As[i] = f.pulse_at(F78::c[i] * h);
+        }
+    }
+}
+
 // Like odeint's try_step for a non-FSAL stepper, each attempt evaluates the
 // derivative at its start, so nothing but the state crosses attempts.
 template <class F>
-__device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double t, double h, double snew[3],
-                                                double rtol, double atol) {
-    f.prepare(t);
-    // The pulse does not depend on the state: A at the stage times is
-    // evaluated first (c_7 = c_3, c_11 = c_0 = 0, c_12 = c_10 are shared).
-    double As[13];
-#pragma unroll
-    for (int i = 0; i < 13; ++i) {
-        if (i == 7) As[i] = As[3];
-        else if (i == 11) As[i] = As[0];
-        else if (i == 12) As[i] = As[10];
-        else As[i] = f.pulse_at(F78::c[i] * h);
-    }
+__device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double h, const double As[13],
+                                                double snew[3], double rtol, double atol) {
     double k[13][3];
     f.rhs_A(s, As[0], k[0]);
     const double k1[3] = {k[0][0], k[0][1], k[0][2]};
@@ -358,13 +388,16 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
 // and on return the controller's proposal; unclipped the step that was in
 // force before the final clip.  status: 0 ok, 1 controller gave up, 2 the
 // state became non-finite.
+//
+// nom: this interval's row of the nominal-step table (nullptr: none).
 template <class F>
 __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, double t1, double& h,
                                                double& unclipped, const SolverDev& sp, uint32_t& n_att,
-                                               uint32_t& n_rej, int& status) {
+                                               uint32_t& n_rej, int& status, const double* nom) {
     double t = t0;
     bool fresh = true;
     int tries = 0;
+    int step = -1;  // index of the current step inside the interval
     for (;;) {
         if (fresh) {
             if (!(t1 - t > kDblEps)) break;
@@ -372,9 +405,22 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
             if ((t + h) - t1 > kDblEps) h = t1 - t;
             tries = 0;
             fresh = false;
+            ++step;
         }
+        // candidate slot: the restart sequence's step, or the whole-interval
+        // step for a carried or proposed first step
+        const double* cand = nullptr;
+        if (nom) {
+            if (sp.proposal == PROPOSAL_RESTART) {
+                if (step < kNomRestartSlots) cand = nom + step * kNomStride;
+            } else if (step == 0) {
+                cand = nom + kNomRestartSlots * kNomStride;
+            }
+        }
+        double As[13];
+        stage_pulses(f, t, h, cand, As);
         double snew[3];
-        const double err = rkf78_attempt(f, s, t, h, snew, sp.rtol, sp.atol);
+        const double err = rkf78_attempt(f, s, h, As, snew, sp.rtol, sp.atol);
         ++n_att;
         if (err > 1.0) {
             ++n_rej;
@@ -391,8 +437,8 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
             s[1] = snew[1];
             s[2] = snew[2];
             if (err < 0.5) {
-                const double e = err > (1.0 / 390625.0) ? err : (1.0 / 390625.0);
-                h *= 0.9 / sqrt(sqrt(sqrt(e)));
+                if (err > (1.0 / 390625.0)) h *= 0.9 / sqrt(sqrt(sqrt(err)));
+                else h *= max_growth();
             }
             fresh = true;
             if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) {

@@ -23,6 +23,16 @@ enum : int { PROPOSAL_RESTART = 0, PROPOSAL_CARRY = 1, PROPOSAL_LEARNED = 2 };
 
 constexpr int WAVE = 64;
 
+// Nominal-step pulse table: per sampling interval, kNomSlots candidate steps
+// (t, h, A(t + c_j h) for the 13 Fehlberg stages).  Slots 0..5 replay the
+// reference controller from its restart step Delta/n_sub growing at the
+// maximal factor until the clipped last step; slot 6 is the single step
+// covering the whole interval.
+constexpr int kNomSlots = 7;
+constexpr int kNomRestartSlots = 6;
+constexpr int kNomStride = 15;  // t, h, 13 stage values
+constexpr int kNomRow = kNomSlots * kNomStride;
+
 struct JobDev {
     // model
     int32_t model, relax;
@@ -46,6 +56,7 @@ struct JobDev {
     int64_t row_base;     // first partial row double of this job
     int64_t sample_base;  // offset of this job in At and j
     int64_t magnus_base;  // offset of this job in Am
+    int64_t nominal_base; // offset of this job in the nominal-step table (N rows of kNomRow)
     int64_t item_base;    // first item of this job (items are job-major before reordering)
     int32_t n_items, pad1;
 };

@@ -101,7 +101,8 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
             const double t1 = J.t_begin + (double)i * J.delta;
             double h = restart ? h_restart : h_carry;
             double unclipped = h;
-            rkf78_interval(f, s, t0, t1, h, unclipped, a.sp, lc.att, lc.rej, status);
+            const double* nom = a.nom ? a.nom + J.nominal_base + (int64_t)(i - 1) * kNomRow : nullptr;
+            rkf78_interval(f, s, t0, t1, h, unclipped, a.sp, lc.att, lc.rej, status, nom);
             h_carry = h > unclipped ? h : unclipped;
         }
         const double g = (status != 3) ? w * f.vertex(At[i], s) : 0.0;
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
                 const double t1 = J.t_begin + (double)i * J.delta;
                 double h = (sp.proposal == PROPOSAL_RESTART) ? h_restart : h_carry;
                 double unclipped = h;
-                rkf78_interval(f, s, t0, t1, h, unclipped, sp, att, rej, status);
+                rkf78_interval(f, s, t0, t1, h, unclipped, sp, att, rej, status, nullptr);
                 h_carry = h > unclipped ? h : unclipped;
             }
         } else {
@@ -249,6 +250,43 @@ __global__ void pulse_table_kernel(const JobDev* jobs, int job, double* At, doub
             Am[J.magnus_base + 2 * i + 1] = pulse_A<P>(J, t + c2 * h);
         }
     }
+}
+
+// Nominal-step table of one job: one thread per sampling interval replays the
+// controller's maximal-growth sequence from the restart step (the same
+// floating-point operations as rkf78_interval) and the whole-interval step,
+// and stores A at the 13 stage times of each candidate.  Unused slots hold
+// t = NaN so they never match.
+template <int P>
+__global__ void nominal_table_kernel(const JobDev* jobs, int job, double* tab) {
+    JobRef J = job_ref(jobs + job);
+    const int64_t ii = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= J.n_meas) return;
+    double* row = tab + J.nominal_base + ii * kNomRow;
+    const double t0 = J.t_begin + (double)ii * J.delta;
+    const double t1 = J.t_begin + (double)(ii + 1) * J.delta;
+    auto put = [&](int slot, double t, double h) {
+        double* c = row + slot * kNomStride;
+        c[0] = t;
+        c[1] = h;
+        for (int j = 0; j < 13; ++j) c[2 + j] = pulse_A<P>(J, t + F78::c[j] * h);
+    };
+    auto put_empty = [&](int slot) {
+        double* c = row + slot * kNomStride;
+        for (int j = 0; j < kNomStride; ++j) c[j] = __builtin_nan("");
+    };
+    double t = t0, h = J.delta / J.n_sub;
+    for (int slot = 0; slot < kNomRestartSlots; ++slot) {
+        if (!(t1 - t > kDblEps)) {
+            put_empty(slot);
+            continue;
+        }
+        if ((t + h) - t1 > kDblEps) h = t1 - t;
+        put(slot, t, h);
+        t += h;
+        h *= max_growth();
+    }
+    put(kNomRestartSlots, t0, t1 - t0);
 }
 
 // Zeroes the counters and accumulators (a fill kernel rather than a memset so
@@ -386,6 +424,29 @@ hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_
     case PULSE_DOUBLE_COS2:
         hipLaunchKernelGGL((pulse_table_kernel<PULSE_DOUBLE_COS2>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Am,
                            with_magnus);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_nominal_table(int pulse, const JobDev* jobs, int job, int64_t n_meas, double* tab,
+                                hipStream_t st) {
+    const int blocks = (int)((n_meas + 127) / 128);
+    switch (pulse) {
+    case PULSE_COS2:
+        hipLaunchKernelGGL((nominal_table_kernel<PULSE_COS2>), dim3(blocks), dim3(128), 0, st, jobs, job, tab);
+        break;
+    case PULSE_GAUSS:
+        hipLaunchKernelGGL((nominal_table_kernel<PULSE_GAUSS>), dim3(blocks), dim3(128), 0, st, jobs, job, tab);
+        break;
+    case PULSE_DOUBLE_GAUSS:
+        hipLaunchKernelGGL((nominal_table_kernel<PULSE_DOUBLE_GAUSS>), dim3(blocks), dim3(128), 0, st, jobs, job,
+                           tab);
+        break;
+    case PULSE_DOUBLE_COS2:
+        hipLaunchKernelGGL((nominal_table_kernel<PULSE_DOUBLE_COS2>), dim3(blocks), dim3(128), 0, st, jobs, job, tab);
         break;
     default:
         return hipErrorInvalidValue;

@@ -18,6 +18,7 @@ struct KernelArgs {
     const int64_t* perm;  // nullptr: identity lane -> k map
     const double* At;
     const double* Am;
+    const double* nom;  // nominal-step pulse table (nullptr: evaluate every stage)
     unsigned long long* acc;  // 2 words per output sample
     Counters* ctr;
     SolverDev sp;
@@ -31,6 +32,8 @@ hipError_t launch_trajectories(int method, int model, int pulse, int relax, cons
                                double* s_out, Counters* ctr, hipStream_t st);
 hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Am,
                               int with_magnus, hipStream_t st);
+hipError_t launch_nominal_table(int pulse, const JobDev* jobs, int job, int64_t n_meas, double* tab,
+                                hipStream_t st);
 hipError_t launch_clear(unsigned long long* p, int64_t n_words, hipStream_t st);
 hipError_t launch_finalize(const unsigned long long* acc, int64_t n, double* j, hipStream_t st);
 

@@ -54,7 +54,7 @@ struct hhg_plan {
     double *d_kx = nullptr, *d_ky = nullptr, *d_w = nullptr;
     hhg::JobDev* d_jobs = nullptr;
     int2* d_items = nullptr;
-    double *d_At = nullptr, *d_Am = nullptr;
+    double *d_At = nullptr, *d_Am = nullptr, *d_nom = nullptr;
     unsigned long long* d_words = nullptr;  // [counters per group][accumulators]
     int64_t n_words = 0;
     double* d_j = nullptr;
@@ -157,6 +157,7 @@ void destroy_device(hhg_plan* p) {
     hipFree(p->d_items);
     hipFree(p->d_At);
     hipFree(p->d_Am);
+    hipFree(p->d_nom);
     hipFree(p->d_words);
     hipFree(p->d_j);
     if (p->stream) hipStreamDestroy(p->stream);
@@ -206,6 +207,7 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.perm = nullptr;
         a.At = p->d_At;
         a.Am = p->d_Am;
+        a.nom = p->d_nom;
         a.acc = p->acc();
         a.ctr = p->counters((int)gi);
         a.sp.rtol = p->solver.rtol;
@@ -321,7 +323,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
 
     // jobs: shard = contiguous block of 64-point chunks of each job's grid
     p->offsets.assign(d->n_jobs + 1, 0);
-    int64_t magnus_words = 0;
+    int64_t magnus_words = 0, nominal_words = 0;
     for (int i = 0; i < d->n_jobs; ++i) {
         hhg::JobDev J = make_job(d->jobs[i], sv);
         const int g = d->jobs[i].kgrid;
@@ -335,6 +337,8 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         p->offsets[i + 1] = p->offsets[i] + J.n_meas + 1;
         J.magnus_base = magnus_words;
         magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
+        J.nominal_base = nominal_words;
+        if (sv.method == HHG_METHOD_RKF78) nominal_words += (int64_t)J.n_meas * hhg::kNomRow;
         J.row_base = 0;
         p->jobs.push_back(J);
     }
@@ -379,6 +383,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     A((void**)&p->d_items, sizeof(int2) * std::max<size_t>(1, items.size()), "items");
     A((void**)&p->d_At, sizeof(double) * p->n_out, "pulse samples");
     A((void**)&p->d_Am, sizeof(double) * std::max<int64_t>(1, magnus_words), "magnus table");
+    if (nominal_words > 0) A((void**)&p->d_nom, sizeof(double) * nominal_words, "nominal-step table");
     A((void**)&p->d_words, sizeof(unsigned long long) * p->n_words, "accumulators");
     A((void**)&p->d_j, sizeof(double) * p->n_out, "j");
     if (st != HHG_SUCCESS) return bail(st);
@@ -397,6 +402,11 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         st = check_hip(hhg::launch_pulse_table(J.pulse, p->d_jobs, i, n, p->d_At, p->d_Am, J.magnus_sub > 0, p->stream),
                        "pulse table");
         if (st != HHG_SUCCESS) return bail(st);
+        if (p->d_nom) {
+            st = check_hip(hhg::launch_nominal_table(J.pulse, p->d_jobs, i, J.n_meas, p->d_nom, p->stream),
+                           "nominal-step table");
+            if (st != HHG_SUCCESS) return bail(st);
+        }
     }
     if ((st = check_hip(hipStreamSynchronize(p->stream), "pulse table sync")) != HHG_SUCCESS) return bail(st);
 
