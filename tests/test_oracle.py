@@ -1,0 +1,424 @@
+"""Pins the CPU oracle (oracle/hhg_oracle.c) against mathematics and
+independent solvers, so that the oracle can serve as the parity reference.
+
+* Fehlberg 7(8) tableau: every Butcher order condition through order 8 for
+  the propagated weights and through order 7 for the embedded ones, in exact
+  rational arithmetic over all 200 rooted trees (a wrong coefficient, sign or
+  index breaks at least one condition).
+* Bloch right-hand side and integrator: trajectories against scipy's DOP853
+  at rtol = 1e-13 integrating the von Neumann equation i rho' = [H, rho] of
+  the 2x2 Hamiltonian H = d(k - A(t)).sigma, built independently in numpy
+  from the model definitions and the configured cos^2(pi t/T) pulse form.
+* Invariants: |s| conservation without relaxation, zero current without a
+  field on symmetric grids, mirror symmetries of the lattice models, thermal
+  initial states in closed form, decay rates of the relaxation terms.
+* Controller: the restart step sequence Delta/n_sub, x4.5, ... (growth cap
+  0.9 * 5 of the published odeint step adjuster) fixes the attempt count per
+  interval; forced failures are reported, not hung.
+* Error budget: RKF78 at 1e-10 against the same oracle at 1e-13 stays
+  within the 1e-9 relative j(t) criterion; Magnus-4 converges at order 4.
+"""
+import math
+from fractions import Fraction
+from functools import lru_cache
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_RELAX_INIT, DIRAC_SMALL, HONEYCOMB_SMALL, PIFLUX_SMALL,
+                            Problem, oracle_j, rel_l2)
+
+# --------------------------------------------------------------------------
+# Butcher trees and order conditions
+# --------------------------------------------------------------------------
+
+
+def _order(t):
+    return 1 + sum(_order(c) for c in t)
+
+
+def _key(t):
+    return (_order(t), t)
+
+
+@lru_cache(None)
+def _trees(n):
+    if n == 1:
+        return ((),)
+    out = set()
+    for forest in _forests(n - 1, None):
+        out.add(tuple(sorted(forest, key=_key)))
+    return tuple(sorted(out, key=_key))
+
+
+@lru_cache(None)
+def _forests(m, bound):
+    if m == 0:
+        return ((),)
+    res = []
+    for k in range(1, m + 1):
+        for t in _trees(k):
+            if bound is not None and _key(t) > bound:
+                continue
+            for rest in _forests(m - k, _key(t)):
+                res.append((t,) + rest)
+    return tuple(res)
+
+
+def _gamma(t):
+    g = _order(t)
+    for c in t:
+        g *= _gamma(c)
+    return g
+
+
+def _tableau_fractions():
+    c, a, b8, b7 = O.rkf78_tableau()
+    F = lambda x: Fraction(x).limit_denominator(10000)
+    return [F(x) for x in c], [[F(x) for x in row] for row in a], [F(x) for x in b8], [F(x) for x in b7]
+
+
+def _phi(t, A, s):
+    """Elementary weight vector: 1 for the single node, prod_children (A phi)."""
+    v = [Fraction(1)] * s
+    for ch in t:
+        pc = _phi(ch, A, s)
+        Ap = [sum(A[i][j] * pc[j] for j in range(s)) for i in range(s)]
+        v = [v[i] * Ap[i] for i in range(s)]
+    return v
+
+
+def test_tree_counts():
+    assert [len(_trees(n)) for n in range(1, 9)] == [1, 1, 2, 4, 9, 20, 48, 115]
+
+
+def test_fehlberg_tableau_is_exactly_rational():
+    c, a, b8, b7 = O.rkf78_tableau()
+    cf, af, b8f, b7f = _tableau_fractions()
+    assert all(float(x) == y for x, y in zip(cf, c))
+    assert all(float(af[i][j]) == a[i][j] for i in range(13) for j in range(13))
+    # explicit method, row sums equal the nodes
+    for i in range(13):
+        assert all(af[i][j] == 0 for j in range(i, 13))
+        assert sum(af[i]) == cf[i]
+    # embedded difference 41/840 (1, 0..0, 1, -1, -1)
+    d = [x - y for x, y in zip(b7f, b8f)]
+    assert d == [Fraction(41, 840)] + [0] * 9 + [Fraction(41, 840), -Fraction(41, 840), -Fraction(41, 840)]
+
+
+@pytest.mark.parametrize("weights,order", [("b8", 8), ("b7", 7)])
+def test_fehlberg_order_conditions(weights, order):
+    cf, af, b8f, b7f = _tableau_fractions()
+    b = b8f if weights == "b8" else b7f
+    n_checked = 0
+    for p in range(1, order + 1):
+        for t in _trees(p):
+            phi = _phi(t, af, 13)
+            assert sum(bi * x for bi, x in zip(b, phi)) == Fraction(1, _gamma(t)), (weights, t)
+            n_checked += 1
+    assert n_checked == sum(len(_trees(p)) for p in range(1, order + 1))
+    if weights == "b8":
+        # and it is not of order 9: some order-9 condition fails
+        fails = sum(1 for t in _trees(9) if sum(bi * x for bi, x in zip(b, _phi(t, af, 13))) != Fraction(1, _gamma(t)))
+        assert fails > 0
+
+
+# --------------------------------------------------------------------------
+# Independent reference solver: von Neumann equation with scipy DOP853
+# --------------------------------------------------------------------------
+
+SX = np.array([[0, 1], [1, 0]], dtype=complex)
+SY = np.array([[0, -1j], [1j, 0]], dtype=complex)
+SZ = np.array([[1, 0], [0, -1]], dtype=complex)
+NN = np.array([[1.0, 0.0], [-0.5, 0.5 * math.sqrt(3)], [-0.5, -0.5 * math.sqrt(3)]])
+
+
+def _d_numpy(kind, qx, qy, v=1.0, mass=0.1, hop=1.0):
+    if kind == O.MODEL_DIRAC2D:
+        return np.array([v * qx, v * qy, mass])
+    if kind == O.MODEL_PIFLUX2D:
+        return 2 * hop * np.array([math.cos(qx), math.cos(qy), 0.0])
+    g = np.sum(np.exp(1j * (NN[:, 0] * qx + NN[:, 1] * qy)))
+    return hop * np.array([g.real, -g.imag, 0.0])
+
+
+def _neumann_trajectory(kind, kx, ky, A_of_t, s0, times, **mp):
+    from scipy.integrate import solve_ivp
+
+    def H(t):
+        A = A_of_t(t)
+        d = _d_numpy(kind, kx - A, ky, **mp)
+        return d[0] * SX + d[1] * SY + d[2] * SZ
+
+    rho0 = 0.5 * (np.eye(2) + s0[0] * SX + s0[1] * SY + s0[2] * SZ)
+
+    def f(t, y):
+        rho = y[:4].view(complex).reshape(2, 2) if False else (y[0::2] + 1j * y[1::2]).reshape(2, 2)
+        h = H(t)
+        dr = -1j * (h @ rho - rho @ h)
+        out = np.empty(8)
+        out[0::2] = dr.ravel().real
+        out[1::2] = dr.ravel().imag
+        return out
+
+    y0 = np.empty(8)
+    y0[0::2] = rho0.ravel().real
+    y0[1::2] = rho0.ravel().imag
+    sol = solve_ivp(f, (times[0], times[-1]), y0, method="DOP853", t_eval=times, rtol=1e-13, atol=1e-14)
+    assert sol.success
+    S = []
+    for k in range(len(times)):
+        rho = (sol.y[0::2, k] + 1j * sol.y[1::2, k]).reshape(2, 2)
+        S.append([np.trace(rho @ M).real for M in (SX, SY, SZ)])
+    return np.array(S)
+
+
+@pytest.mark.parametrize("kind,k", [(O.MODEL_DIRAC2D, (0.7, -0.4)), (O.MODEL_PIFLUX2D, (0.9, 2.1)),
+                                    (O.MODEL_HONEYCOMB, (1.1, 0.6))])
+def test_trajectory_matches_von_neumann_dop853(kind, k):
+    T = 4 * math.pi  # 2 cycles, configured form A0 cos^2(pi t / T) sin t on [-T/2, T/2]
+    A0 = 0.6
+    A_cfg = lambda t: A0 * math.cos(math.pi * t / T) ** 2 * math.sin(t) if abs(t) < T / 2 else 0.0
+    m = O.model(kind, v=1.0, mass=0.1, hopping=1.0)
+    p = O.pulse(O.PULSE_COS2, A0, n_cycles=2.0, t0=-T / 2)
+    tc = O.time_grid(-T / 2, T / 2, 200, 50)
+    s_or, st, r = O.trajectory(m, p, tc, O.solver(rtol=1e-12, atol=1e-12), *k)
+    assert r == 0
+    s0 = O.initial_state(m, *k)
+    d0 = _d_numpy(kind, *k)
+    assert np.allclose(s0, -d0 / np.linalg.norm(d0), atol=1e-15)  # lower band filled
+    times = -T / 2 + np.arange(201) * (T / 200)
+    s_ref = _neumann_trajectory(kind, k[0], k[1], A_cfg, s0, times)
+    assert np.max(np.abs(s_or - s_ref)) < 2e-10
+
+
+# --------------------------------------------------------------------------
+# Invariants and symmetries
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("pr", [DIRAC_COHERENT, PIFLUX_SMALL, HONEYCOMB_SMALL], ids=lambda p: p.name)
+def test_bloch_norm_conserved_without_relaxation(pr):
+    kx, ky, _ = pr.o_grid()
+    for k in (0, len(kx) // 3, len(kx) - 1):
+        s, _, r = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), kx[k], ky[k])
+        assert r == 0
+        n = np.linalg.norm(s, axis=1)
+        assert np.max(np.abs(n - n[0])) < 1e-11
+
+
+@pytest.mark.parametrize("pr", [DIRAC_COHERENT, PIFLUX_SMALL, HONEYCOMB_SMALL], ids=lambda p: p.name)
+def test_no_field_stationary_current(pr):
+    """A = 0: the filled lower band is stationary, so j(t) is constant.  For
+    the Dirac and PiFlux vertices (odd under k -> -k on a symmetric grid) it
+    vanishes.  The reference's honeycomb vertex Re(Gx) s_x + Im(Gx) s_y
+    (Honeycomb.cpp:273-279) is even under k -> -k at equilibrium and leaves a
+    static offset, which the oracle reproduces."""
+    p = pr.o_pulse()
+    p.amp_a = 0.0
+    p.amp_b = 0.0
+    kx, ky, w = pr.o_grid()
+    j, st = O.current_density(pr.o_model(), p, pr.o_time(), pr.o_solver(), kx, ky, w)
+    scale = np.sum(np.abs(w))
+    assert np.max(np.abs(j - j[0])) < 1e-14 * scale
+    if pr.model["kind"] != O.MODEL_HONEYCOMB:
+        assert np.max(np.abs(j)) < 1e-14 * scale
+    assert st.rejects == 0
+
+
+def test_lattice_mirror_symmetry_ky():
+    """d(qx, -qy) = d(qx, qy) for PiFlux-2D and honeycomb (the honeycomb bond set
+    is mirror symmetric), so trajectories at (kx, ky) and (kx, -ky) coincide;
+    with the field along x the transverse honeycomb current therefore cancels
+    (cf. the j_perp check of Honeycomb.cpp:328-334)."""
+    for pr in (PIFLUX_SMALL, HONEYCOMB_SMALL):
+        a, _, _ = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), 0.8, 0.35)
+        b, _, _ = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), 0.8, -0.35)
+        assert np.max(np.abs(a - b)) < 1e-13
+
+
+def test_thermal_initial_states():
+    beta, d = 3.0, None
+    m = O.model(O.MODEL_DIRAC2D, v=1.0, mass=0.3, beta=beta)
+    s = O.initial_state(m, 0.4, -0.2)
+    dv = np.array([0.4, -0.2, 0.3])
+    e = np.linalg.norm(dv)
+    assert np.allclose(s, -math.tanh(beta * e / 2) * dv / e, atol=1e-15)
+    # honeycomb, reference occupation formulas at E_F = 0: a pure state with
+    # in-plane part tanh and s_z = -sech of beta t |g| / 2
+    mh = O.model(O.MODEL_HONEYCOMB, hopping=1.0, beta=beta)
+    sh = O.initial_state(mh, 0.7, 0.2)
+    g = np.sum(np.exp(1j * (NN[:, 0] * 0.7 + NN[:, 1] * 0.2)))
+    x = beta * abs(g) / 2
+    assert np.allclose(sh, [-math.tanh(x) * g.real / abs(g), math.tanh(x) * g.imag / abs(g), -1 / math.cosh(x)],
+                       atol=1e-15)
+    assert abs(np.linalg.norm(sh) - 1.0) < 1e-15
+
+
+def test_relaxation_terms():
+    """Without a field: relaxation toward the initial state leaves s0 fixed;
+    T2 dephasing removes the component of s transverse to d at rate 1/T2."""
+    m = O.model(O.MODEL_DIRAC2D, v=1.0, mass=0.2, relax=O.RELAX_T2, relax_time=7.0)
+    p = O.pulse(O.PULSE_COS2, 0.0, n_cycles=2.0, t0=0.0)
+    s = np.array([0.3, -0.5, 0.4])
+    d = np.array([0.5, 0.25, 0.2])
+    out = O.rhs(m, p, 0.5, 0.25, s, s, 1.0)
+    perp = s - np.dot(s, d) / np.dot(d, d) * d
+    assert np.allclose(out, 2 * np.cross(d, s) - perp / 7.0, atol=1e-15)
+    m2 = O.model(O.MODEL_DIRAC2D, v=1.0, mass=0.2, relax=O.RELAX_TO_INITIAL, relax_time=3.0)
+    s0 = O.initial_state(m2, 0.5, 0.25)
+    assert np.allclose(O.rhs(m2, p, 0.5, 0.25, s0, s0, 1.0), 0.0, atol=1e-16)
+
+
+# --------------------------------------------------------------------------
+# Controller semantics (published Boost.Odeint algorithm)
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("n_sub,expected", [(50, 4), (600, 6), (3000, 7)])
+def test_restart_step_sequence(n_sub, expected):
+    """Each interval restarts at Delta/n_sub; with negligible error every
+    accepted step grows by exactly 0.9 * (5^-8)^(-1/8) = 4.5 until the step is
+    clipped at the interval end, so the number of attempts per interval is
+    the smallest k with (4.5^k - 1)/3.5 >= n_sub (a growth cap of 5 would give
+    5 at n_sub = 600 and 6 at 3000)."""
+    m = O.model(O.MODEL_DIRAC2D, v=1.0, mass=0.1)
+    p = O.pulse(O.PULSE_COS2, 0.2, n_cycles=2.0, t0=-2 * math.pi)
+    tc = O.time_grid(-2 * math.pi, 2 * math.pi, 400, n_sub)
+    _, st, r = O.trajectory(m, p, tc, O.solver(rtol=1e-6, atol=1e-6), 0.3, 0.2)
+    assert r == 0 and st.rejects == 0
+    k = next(k for k in range(1, 20) if (4.5 ** k - 1) / 3.5 >= n_sub)
+    assert k == expected
+    assert st.attempts == expected * 400
+
+
+def test_reject_shrinks_and_failure_is_reported():
+    """An unreachable tolerance rejects repeatedly; with max_fail = 1 the
+    trajectory is reported failed (odeint throws) and the call returns."""
+    m = O.model(O.MODEL_DIRAC2D, v=1.0, mass=0.1)
+    p = O.pulse(O.PULSE_COS2, 0.5, n_cycles=2.0, t0=-2 * math.pi)
+    tc = O.time_grid(0.3, 4.3, 64, 50)
+    _, st, r = O.trajectory(m, p, tc, O.solver(rtol=1e-300, atol=1e-300, max_fail=1), 0.4, 0.1)
+    assert r == 1 and st.rejects >= 2
+
+
+# --------------------------------------------------------------------------
+# Error budget and Magnus order
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("pr", [DIRAC_SMALL, PIFLUX_SMALL, HONEYCOMB_SMALL, DIRAC_RELAX_INIT], ids=lambda p: p.name)
+def test_rtol_1e10_within_criterion_of_tight_oracle(pr):
+    j, _ = oracle_j(pr)
+    j_tight, _ = oracle_j(pr, rtol=1e-13, atol=1e-13)
+    assert rel_l2(j, j_tight) <= 1e-9
+
+
+def test_carry_proposal_within_criterion_of_tight_oracle():
+    pr = DIRAC_SMALL
+    j, st = oracle_j(pr, proposal=O.PROPOSAL_CARRY)
+    j_tight, _ = oracle_j(pr, rtol=1e-13, atol=1e-13)
+    assert rel_l2(j, j_tight) <= 1e-9
+
+
+def test_magnus4_converges_at_order_4():
+    pr = DIRAC_COHERENT
+    kx, ky, _ = pr.o_grid()
+    k = 7
+    ref, _, _ = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(rtol=1e-13, atol=1e-13), kx[k],
+                             ky[k])
+    errs = []
+    for ms in (2, 4, 8):
+        s, _, _ = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(),
+                               pr.o_solver(method=O.METHOD_MAGNUS4, magnus_substeps=ms), kx[k], ky[k])
+        errs.append(np.max(np.abs(s - ref)))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 12 < r1 < 20 and 12 < r2 < 20, errs
+
+
+def test_magnus_exact_without_field():
+    pr = DIRAC_COHERENT
+    p = pr.o_pulse()
+    p.amp_a = 0.0
+    s, _, _ = O.trajectory(pr.o_model(), p, pr.o_time(), pr.o_solver(method=O.METHOD_MAGNUS4), 0.3, 0.2)
+    assert np.max(np.abs(s - s[0])) < 1e-15
+
+
+def test_magnus_with_relaxation_is_unsupported():
+    with pytest.raises(ValueError):
+        O.trajectory(DIRAC_SMALL.o_model(), DIRAC_SMALL.o_pulse(), DIRAC_SMALL.o_time(),
+                     O.solver(method=O.METHOD_MAGNUS4), 0.1, 0.1)
+
+
+# --------------------------------------------------------------------------
+# Reduction and spectrum
+# --------------------------------------------------------------------------
+
+
+def test_current_density_thread_count_independent():
+    pr = HONEYCOMB_SMALL
+    kx, ky, w = pr.o_grid()
+    j1, _ = O.current_density(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), kx, ky, w, n_threads=1)
+    j4, _ = O.current_density(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), kx, ky, w, n_threads=4)
+    assert np.array_equal(j1, j4)
+
+
+def test_current_density_is_weighted_vertex_sum():
+    """j(t_i) = sum_k w_k g(s_k(t_i), A(t_i)) recomputed from trajectories."""
+    pr = PIFLUX_SMALL
+    kx, ky, w = pr.o_grid()
+    sel = np.arange(0, len(kx), 37)
+    j, _ = O.current_density(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), kx[sel], ky[sel], w[sel])
+    t = pr.time[0] + np.arange(pr.time[2] + 1) * ((pr.time[1] - pr.time[0]) / pr.time[2])
+    A = O.pulse_A(pr.o_pulse(), t)
+    acc = np.zeros_like(j)
+    for k in sel:
+        s, _, _ = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), kx[k], ky[k])
+        acc += w[k] * np.sin(kx[k] - A) * s[:, 0]  # PiFlux vertex along x
+    assert np.max(np.abs(acc - j)) < 1e-15 * np.sum(np.abs(w))
+
+
+def test_spectrum_fft_equals_direct_sum():
+    """The oracle's spectrum uses numpy's FFT on the zero-padded signal; a
+    direct O(N^2) exponential sum over the padded signal gives the same."""
+    rng = np.random.default_rng(3)
+    N, tb, te = 40, -2.0, 3.0
+    j = rng.standard_normal(N + 1)
+    f, re, im = O.trapezoidal_spectrum(j, tb, te, N)
+    x = np.zeros(8 * (N + 1))
+    x[:N + 1] = j
+    n = np.arange(len(x))
+    Y = np.array([np.sum(x * np.exp(-2j * np.pi * k * n / len(x))) for k in range(len(f))])
+    delta = (te - tb) / N
+    th = f * delta
+    with np.errstate(invalid="ignore", divide="ignore"):
+        W = np.where(th == 0, 1.0, 2 * (1 - np.cos(th)) / th ** 2)
+        a0 = np.where(th == 0, -0.5, (-(1 - np.cos(th)) + 1j * (th - np.sin(th))) / th ** 2)
+    out = delta * (delta * W * np.exp(1j * f * tb) * Y + a0 * x[0] + np.conj(a0) * np.exp(1j * f * (te - tb)) * x[-1])
+    assert np.allclose(re, out.real, atol=1e-12) and np.allclose(im, out.imag, atol=1e-12)
+
+
+def test_trapezoidal_weights_reproduce_analytic_fourier_integral():
+    """W(theta) and alpha_0(theta) are the weights of the exact Fourier integral
+    of the piecewise-linear interpolant (Numerical Recipes 13.9, trapezoidal
+    order): applied with the real last sample, one factor delta and a
+    consistent exp(+i w t) convention they approximate a smooth pulse's
+    Fourier integral with second-order error, also for w delta far from 0
+    where the plain trapezoidal sum would alias."""
+    tb, te = -20.0, 20.0
+
+    def integral(N, w):
+        delta = (te - tb) / N
+        t = tb + delta * np.arange(N + 1)
+        h = np.exp(-t ** 2 / 8) * np.cos(3 * t)
+        th = w * delta
+        W = 2 * (1 - math.cos(th)) / th ** 2
+        a0 = (-(1 - math.cos(th)) + 1j * (th - math.sin(th))) / th ** 2
+        S = np.sum(h * np.exp(1j * w * (t - tb)))
+        return delta * np.exp(1j * w * tb) * (W * S + a0 * h[0] + np.conj(a0) * np.exp(1j * w * (te - tb)) * h[-1])
+
+    for w in (0.5, 2.0, 3.0, 4.5):
+        exact = math.sqrt(8 * math.pi) / 2 * (math.exp(-2 * (w - 3) ** 2) + math.exp(-2 * (w + 3) ** 2))
+        e1, e2 = abs(integral(400, w) - exact), abs(integral(800, w) - exact)
+        assert e1 < 1e-2 * exact + 1e-8 and 3.5 < e1 / e2 < 4.5, (w, e1, e2)
