@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhhg_amd.so")
+RUN = os.path.join(HERE, "hhg_run")
 
 SOURCES = ["hhg_kernels.hip", "hhg_plan.hip", "hhg_host.cpp"]
 HEADERS = ["hhg_device.hpp", "hhg_internal.hpp", "hhg_kernels.hpp", "hhg_host.hpp"]
@@ -56,8 +57,16 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C++ driver over the host interface (cpp/hhg.hpp): a host-only program
+    cxx = shutil.which("g++") or shutil.which("c++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "cpp", "hhg_run.cpp"), "-o", RUN + ".tmp",
+           "-L" + HERE, "-lhhg_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(RUN + ".tmp", RUN)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", RUN)
     return LIB
 
 

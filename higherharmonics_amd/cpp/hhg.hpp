@@ -1,0 +1,296 @@
+// hhg.hpp — C++ host interface over the C ABI (include/hhg_amd.h), shaped
+// like the reference's host classes so a caller of the reference finds the
+// same names and argument meanings:
+//
+//   HHG::TimeIntegrationConfig      sources/HHG/TimeIntegrationConfig.hpp:7-15
+//   HHG::Laser::CosineLaser         sources/HHG/Laser/CosineLaser.hpp:6-19
+//   HHG::Laser::GaussLaser          sources/HHG/Laser/GaussLaser.hpp:6-21
+//   HHG::Laser::DoubleLaser         sources/HHG/Laser/DoubleLaser.hpp:8-74
+//   Laser::laser_function(t)        sources/HHG/Laser/Laser.hpp:32
+//   HHG::Systems::*::compute_current_density(laser, time_config, rank, n_ranks)
+//                                   sources/HHG/Systems/DiracSystem.hpp:58-59,
+//                                   PiFlux.hpp:92-93, Honeycomb.hpp:70-71
+//   HHG::Dispatch::Plan             the role of Dispatcher::compute
+//                                   (sources/HHG/Dispatch/Dispatcher.hpp:29)
+//
+// Errors: the C ABI returns status codes; this layer turns them into the
+// exceptions the reference throws (std::invalid_argument for bad input,
+// std::runtime_error otherwise).  Every solve runs on the GPU.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hhg_amd.h"
+
+namespace HHG {
+
+using h_float = double;
+constexpr h_float pi = 3.14159265358979323846;
+
+inline void check(hhg_status st) {
+    if (st == HHG_SUCCESS) return;
+    const std::string msg = hhg_last_error_string();
+    if (st == HHG_ERROR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error("hhg status " + std::to_string((int)st) + ": " + msg);
+}
+
+struct TimeIntegrationConfig {
+    h_float t_begin{};
+    h_float t_end{};
+    int n_measurements{};
+    int n_subdivisions{};
+
+    h_float measure_every() const { return (t_end - t_begin) / n_measurements; }
+    h_float dt() const { return measure_every() / n_subdivisions; }
+    hhg_time_desc desc() const { return hhg_time_desc{t_begin, t_end, n_measurements, n_subdivisions}; }
+};
+
+namespace Laser {
+
+// A(t) = amplitude * envelope(t) * sin(t + cep); amplitude is the reference's
+// momentum_amplitude (Laser.hpp:17) already in dimensionless units.
+struct Laser {
+    h_float momentum_amplitude{};
+    h_float t_begin{};
+    h_float t_end{};
+    virtual ~Laser() = default;
+    virtual hhg_pulse_desc desc() const = 0;
+
+    h_float laser_function(h_float t) const {
+        const hhg_pulse_desc d = desc();
+        h_float A = 0.0;
+        check(hhg_pulse_evaluate(&d, &t, &A, 1));
+        return A;
+    }
+};
+
+// cos^2 envelope on (t_begin, t_begin + 2 pi n_cycles); begin_shift = pi * t0_offset
+struct CosineLaser : Laser {
+    h_float n_cycles{}, cep{};
+    CosineLaser(h_float amplitude, h_float n_cycles_, h_float begin_shift = 0.0, h_float cep_ = 0.0)
+        : n_cycles(n_cycles_), cep(cep_) {
+        momentum_amplitude = amplitude;
+        t_begin = begin_shift;
+        t_end = begin_shift + 2 * pi * n_cycles_;
+    }
+    hhg_pulse_desc desc() const override {
+        hhg_pulse_desc d{};
+        d.kind = HHG_PULSE_COS2;
+        d.amp_a = momentum_amplitude;
+        d.n_cycles = n_cycles;
+        d.t0 = t_begin;
+        d.cep = cep;
+        return d;
+    }
+};
+
+// Gaussian envelope exp(-(t - center)^2 / 2 sigma^2), window center +- window * sigma
+struct GaussLaser : Laser {
+    h_float sigma{}, center{}, cep{};
+    GaussLaser(h_float amplitude, h_float sigma_, h_float center_ = 0.0, h_float cep_ = 0.0, h_float window = 5.0)
+        : sigma(sigma_), center(center_), cep(cep_) {
+        momentum_amplitude = amplitude;
+        t_begin = center_ - window * sigma_;
+        t_end = center_ + window * sigma_;
+    }
+    // sigma of the reference's GaussLaser(n_cycles) (GaussLaser.cpp:7)
+    static h_float reference_sigma(h_float n_cycles) { return pi * n_cycles / std::sqrt(2.0 * std::log(1.0 / 0.05)); }
+    hhg_pulse_desc desc() const override {
+        hhg_pulse_desc d{};
+        d.kind = HHG_PULSE_GAUSS;
+        d.amp_a = momentum_amplitude;
+        d.sigma = sigma;
+        d.t0 = center;
+        d.cep = cep;
+        return d;
+    }
+};
+
+// A(t) + B(t - delay), B the same base pulse with amplitude amplitude_b
+template <class BaseLaser>
+struct DoubleLaser : Laser {
+    BaseLaser base;
+    h_float amplitude_b{}, delay{};
+    DoubleLaser(BaseLaser base_, h_float amplitude_b_, h_float delay_)
+        : base(std::move(base_)), amplitude_b(amplitude_b_), delay(delay_) {
+        momentum_amplitude = base.momentum_amplitude;
+        t_begin = std::min(base.t_begin, base.t_begin + delay_);
+        t_end = std::max(base.t_end, base.t_end + delay_);
+    }
+    hhg_pulse_desc desc() const override {
+        hhg_pulse_desc d = base.desc();
+        d.kind = d.kind == HHG_PULSE_GAUSS ? HHG_PULSE_DOUBLE_GAUSS : HHG_PULSE_DOUBLE_COS2;
+        d.amp_b = amplitude_b;
+        d.delay = delay;
+        return d;
+    }
+};
+
+}  // namespace Laser
+
+namespace Dispatch {
+
+struct Solver {
+    int method = HHG_METHOD_RKF78;
+    int proposal = HHG_PROPOSAL_RESTART;
+    h_float rtol = 1e-10, atol = 1e-10;
+    int max_fail = 500;
+    int magnus_substeps = 1;
+    hhg_solver_desc desc() const {
+        hhg_solver_desc d{};
+        d.method = method;
+        d.proposal = proposal;
+        d.rtol = rtol;
+        d.atol = atol;
+        d.max_fail = max_fail;
+        d.magnus_substeps = magnus_substeps;
+        return d;
+    }
+};
+
+// A compiled set of jobs on one device; compute() returns one j(t) per job.
+class Plan {
+   public:
+    Plan(const std::vector<hhg_kgrid_desc>& kgrids, const std::vector<hhg_job_desc>& jobs, const Solver& solver,
+         int rank = 0, int n_ranks = 1, int device = 0) {
+        hhg_plan_desc d{};
+        d.n_kgrids = (int32_t)kgrids.size();
+        d.kgrids = kgrids.data();
+        d.n_jobs = (int32_t)jobs.size();
+        d.jobs = jobs.data();
+        d.solver = solver.desc();
+        d.launch.persistent = 1;
+        d.rank = rank;
+        d.n_ranks = n_ranks;
+        d.device = device;
+        hhg_plan* p = nullptr;
+        check(hhg_plan_create(&d, &p));
+        plan_.reset(p);
+        offsets_.resize(jobs.size() + 1);
+        check(hhg_plan_job_offsets(p, offsets_.data()));
+    }
+
+    std::vector<std::vector<h_float>> compute(hhg_stats* stats = nullptr) {
+        std::vector<h_float> flat((size_t)hhg_plan_output_size(plan_.get()));
+        check(hhg_plan_execute(plan_.get(), flat.data(), stats));
+        std::vector<std::vector<h_float>> out;
+        for (size_t j = 0; j + 1 < offsets_.size(); ++j)
+            out.emplace_back(flat.begin() + offsets_[j], flat.begin() + offsets_[j + 1]);
+        return out;
+    }
+
+    void autotune() { check(hhg_plan_autotune(plan_.get(), nullptr)); }
+    void capture_graph() { check(hhg_plan_capture_graph(plan_.get())); }
+
+   private:
+    struct Del {
+        void operator()(hhg_plan* p) const { hhg_plan_destroy(p); }
+    };
+    std::unique_ptr<hhg_plan, Del> plan_;
+    std::vector<int64_t> offsets_;
+};
+
+}  // namespace Dispatch
+
+namespace Systems {
+
+// Common part of the three systems: a model descriptor plus its k-grid.
+class System {
+   public:
+    hhg_model_desc model{};
+    hhg_kgrid_desc kgrid{};
+    Dispatch::Solver solver{};
+
+    // j(t_i), i = 0..N (the reference's compute_current_density)
+    std::vector<h_float> compute_current_density(const Laser::Laser* laser, const TimeIntegrationConfig& time_config,
+                                                 int rank = 0, int n_ranks = 1, hhg_stats* stats = nullptr) const {
+        hhg_job_desc job{};
+        job.model = model;
+        job.pulse = laser->desc();
+        job.time = time_config.desc();
+        job.kgrid = 0;
+        Dispatch::Plan plan({kgrid}, {job}, solver, rank, n_ranks);
+        return plan.compute(stats)[0];
+    }
+
+   protected:
+    System() = default;
+};
+
+// 2D massive Dirac cone d = (v qx, v qy, Delta) on a GL box |kx|, |ky| <= k_max
+class DiracSystem2D : public System {
+   public:
+    DiracSystem2D(h_float v, h_float mass, h_float k_max, int nx, int ny) {
+        model.kind = HHG_MODEL_DIRAC2D;
+        model.v = v;
+        model.mass = mass;
+        kgrid.kind = HHG_KGRID_GL_BOX;
+        kgrid.nx = nx;
+        kgrid.ny = ny;
+        kgrid.x0 = kgrid.y0 = -k_max;
+        kgrid.x1 = kgrid.y1 = k_max;
+    }
+    DiracSystem2D& dephasing(h_float T2) {
+        model.relax = HHG_RELAX_T2;
+        model.relax_time = T2;
+        return *this;
+    }
+    DiracSystem2D& diagonal_relaxation(h_float tau) {
+        model.relax = HHG_RELAX_TO_INITIAL;
+        model.relax_time = tau;
+        return *this;
+    }
+};
+
+// PiFlux square lattice d = 2t (cos(kx - A), cos ky, 0) over [-pi, pi]^2
+class PiFlux2D : public System {
+   public:
+    PiFlux2D(h_float hopping, int nx, int ny) {
+        model.kind = HHG_MODEL_PIFLUX2D;
+        model.hopping = hopping;
+        kgrid.kind = HHG_KGRID_GL_BOX;
+        kgrid.nx = nx;
+        kgrid.ny = ny;
+        kgrid.x0 = kgrid.y0 = -pi;
+        kgrid.x1 = kgrid.y1 = pi;
+    }
+};
+
+// Honeycomb nearest-neighbour model over the hexagonal BZ (Honeycomb.cpp:225-336)
+class Honeycomb : public System {
+   public:
+    Honeycomb(h_float hopping, int nx, int ny) {
+        model.kind = HHG_MODEL_HONEYCOMB;
+        model.hopping = hopping;
+        kgrid.kind = HHG_KGRID_GL_HEX;
+        kgrid.nx = nx;
+        kgrid.ny = ny;
+    }
+};
+
+}  // namespace Systems
+
+namespace Fourier {
+
+// TrapezoidalFFT::compute with the x8 zero padding of sources/hhg.cpp:277-282
+struct Spectrum {
+    std::vector<h_float> frequencies, real, imag;
+};
+
+inline Spectrum trapezoidal_spectrum(const std::vector<h_float>& j, const TimeIntegrationConfig& tc,
+                                     int zero_padding = 8) {
+    const size_t n = (size_t)(zero_padding * (tc.n_measurements + 1)) / 2 + 1;
+    Spectrum s{std::vector<h_float>(n), std::vector<h_float>(n), std::vector<h_float>(n)};
+    const hhg_time_desc td = tc.desc();
+    check(hhg_trapezoidal_spectrum(j.data(), &td, zero_padding, s.frequencies.data(), s.real.data(), s.imag.data()));
+    return s;
+}
+
+}  // namespace Fourier
+}  // namespace HHG

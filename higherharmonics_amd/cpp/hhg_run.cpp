@@ -1,0 +1,168 @@
+// hhg_run — command-line driver over the C++ host interface, the role of the
+// reference's sources/hhg.cpp main(): read a key-value configuration, build
+// the system, laser and time grid (the Dispatcher constructors,
+// sources/HHG/Dispatch/*Dispatcher.cpp), compute j(t) on the GPU, take the
+// end-corrected spectrum and write one JSON document.
+//
+//   hhg_run <config> [output.json]
+//
+// Configuration: whitespace-separated "key value" lines, '#' starts a comment
+// (the reference's InputFileReader format, sources/hhg.cpp:58-80), in the
+// reference's dimensionless units (time 1/omega_L, energy hbar omega_L).
+//   system_type       Dirac2D | PiFlux2D | Honeycomb
+//   laser_type        cosine | gauss | double_gauss | double_cosine
+//   field_amplitude   A0 (momentum amplitude)
+//   n_laser_cycles    cos^2 length in cycles (default 8)
+//   t0_offset         window begin of the cos^2 pulse in units of pi (default -n_laser_cycles)
+//   sigma, center     Gaussian width and centre (default 4 pi, 0)
+//   amplitude_b, delay   second pulse of a double pulse
+//   cep               carrier-envelope phase
+//   v, mass, k_max    Dirac-2D parameters (1, 0.1, 3)
+//   hopping           lattice hopping t (1)
+//   nx, ny            k-grid size
+//   T2, tau           transverse dephasing / relaxation to the initial state (0 = off)
+//   n_measurements    sampling intervals (default n_laser_cycles * 1024, sources/hhg.cpp:87-88)
+//   n_subdivisions    controller restart divisor (50)
+//   extra_time        time appended after a cos^2 pulse (2 pi for the lattices,
+//                     HoneycombDispatcher.cpp:35, PiFluxDispatcher.cpp:53)
+//   method            rkf78 | magnus4;  rtol, atol, magnus_substeps
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+
+#include "hhg.hpp"
+
+namespace {
+
+std::map<std::string, std::string> read_config(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw std::invalid_argument("cannot open " + path);
+    std::map<std::string, std::string> kv;
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        std::istringstream ls(line);
+        std::string k, v;
+        if (ls >> k >> v) kv[k] = v;
+    }
+    return kv;
+}
+
+double num(const std::map<std::string, std::string>& kv, const std::string& k, double def) {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : std::stod(it->second);
+}
+
+std::string str(const std::map<std::string, std::string>& kv, const std::string& k, const std::string& def) {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : it->second;
+}
+
+void write_array(std::ostream& o, const char* name, const std::vector<double>& v, bool last = false) {
+    o << "  \"" << name << "\": [";
+    char buf[40];
+    for (size_t i = 0; i < v.size(); ++i) {
+        std::snprintf(buf, sizeof buf, "%.17g", v[i]);
+        o << (i ? ", " : "") << buf;
+    }
+    o << "]" << (last ? "\n" : ",\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    using namespace HHG;
+    if (argc < 2) {
+        std::cerr << "usage: hhg_run <config> [output.json]\n";
+        return 2;
+    }
+    try {
+        const auto kv = read_config(argv[1]);
+        const std::string system = str(kv, "system_type", "Dirac2D");
+        const std::string laser_type = str(kv, "laser_type", "cosine");
+        const double A0 = num(kv, "field_amplitude", 0.5);
+        const double n_cycles = num(kv, "n_laser_cycles", 8);
+        const bool lattice = system != "Dirac2D";
+
+        std::unique_ptr<Laser::Laser> laser;
+        if (laser_type == "cosine" || laser_type == "double_cosine") {
+            Laser::CosineLaser base(A0, n_cycles, pi * num(kv, "t0_offset", -n_cycles), num(kv, "cep", 0.0));
+            if (laser_type == "cosine")
+                laser = std::make_unique<Laser::CosineLaser>(base);
+            else
+                laser = std::make_unique<Laser::DoubleLaser<Laser::CosineLaser>>(base, num(kv, "amplitude_b", A0),
+                                                                                  num(kv, "delay", 0.0));
+        } else if (laser_type == "gauss" || laser_type == "double_gauss") {
+            Laser::GaussLaser base(A0, num(kv, "sigma", 4 * pi), num(kv, "center", 0.0), num(kv, "cep", 0.0));
+            if (laser_type == "gauss")
+                laser = std::make_unique<Laser::GaussLaser>(base);
+            else
+                laser = std::make_unique<Laser::DoubleLaser<Laser::GaussLaser>>(base, num(kv, "amplitude_b", A0),
+                                                                                 num(kv, "delay", 0.0));
+        } else {
+            throw std::invalid_argument("Laser type '" + laser_type + "' is not recognized!");
+        }
+
+        const bool cos2 = laser_type == "cosine" || laser_type == "double_cosine";
+        const double extra = num(kv, "extra_time", (cos2 && lattice) ? 2 * pi : 0.0);
+        TimeIntegrationConfig tc{laser->t_begin, laser->t_end + extra,
+                                 (int)num(kv, "n_measurements", n_cycles * 1024), (int)num(kv, "n_subdivisions", 50)};
+
+        std::unique_ptr<Systems::System> sys;
+        const int nx = (int)num(kv, "nx", 128), ny = (int)num(kv, "ny", 64);
+        if (system == "Dirac2D") {
+            auto d = std::make_unique<Systems::DiracSystem2D>(num(kv, "v", 1.0), num(kv, "mass", 0.1),
+                                                              num(kv, "k_max", 3.0), nx, ny);
+            if (num(kv, "T2", 0.0) > 0) d->dephasing(num(kv, "T2", 0.0));
+            if (num(kv, "tau", 0.0) > 0) d->diagonal_relaxation(num(kv, "tau", 0.0));
+            sys = std::move(d);
+        } else if (system == "PiFlux2D") {
+            sys = std::make_unique<Systems::PiFlux2D>(num(kv, "hopping", 1.0), nx, ny);
+        } else if (system == "Honeycomb") {
+            sys = std::make_unique<Systems::Honeycomb>(num(kv, "hopping", 1.0), nx, ny);
+        } else {
+            throw std::invalid_argument("System type '" + system + "' is not recognized!");
+        }
+        if (lattice && num(kv, "tau", 0.0) > 0) {
+            sys->model.relax = HHG_RELAX_TO_INITIAL;
+            sys->model.relax_time = num(kv, "tau", 0.0);
+        }
+        sys->solver.method = str(kv, "method", "rkf78") == "magnus4" ? HHG_METHOD_MAGNUS4 : HHG_METHOD_RKF78;
+        sys->solver.rtol = num(kv, "rtol", 1e-10);
+        sys->solver.atol = num(kv, "atol", 1e-10);
+        sys->solver.magnus_substeps = (int)num(kv, "magnus_substeps", 1);
+
+        hhg_stats stats{};
+        const std::vector<double> j = sys->compute_current_density(laser.get(), tc, 0, 1, &stats);
+        const Fourier::Spectrum sp = Fourier::trapezoidal_spectrum(j, tc);
+        std::vector<double> times(j.size()), A(j.size());
+        for (size_t i = 0; i < j.size(); ++i) {
+            times[i] = tc.t_begin + (double)i * tc.measure_every();
+            A[i] = laser->laser_function(times[i]);
+        }
+
+        std::ofstream fout;
+        std::ostream& o = argc > 2 ? (fout.open(argv[2]), static_cast<std::ostream&>(fout)) : std::cout;
+        o << "{\n";
+        o << "  \"system_type\": \"" << system << "\", \"laser_type\": \"" << laser_type << "\",\n";
+        o << "  \"t_begin\": " << tc.t_begin << ", \"t_end\": " << tc.t_end << ", \"n_measurements\": "
+          << tc.n_measurements << ", \"zero_padding\": 8,\n";
+        o << "  \"trajectories\": " << stats.trajectories << ", \"attempted_steps\": " << stats.attempts
+          << ", \"rejects\": " << stats.rejects << ", \"solve_ms\": " << stats.solve_ms << ",\n";
+        write_array(o, "time", times);
+        write_array(o, "laser_function", A);
+        write_array(o, "current_density_time", j);
+        write_array(o, "frequencies", sp.frequencies);
+        write_array(o, "current_density_frequency_real", sp.real);
+        write_array(o, "current_density_frequency_imag", sp.imag, true);
+        o << "}\n";
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "hhg_run: " << e.what() << "\n";
+        return 1;
+    }
+}

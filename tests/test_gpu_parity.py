@@ -144,3 +144,34 @@ def test_controller_failure_is_reported(hhg):
         assert ei.value.status == hhg.ERROR_TOLERANCE
         assert plan.stats.failed_trajectories == 24 * 16
         assert plan.stats.nonfinite_trajectories == 0
+
+
+def test_cpp_driver_matches_python_path(hhg, tmp_path):
+    """hhg_run (C++ host interface over the C ABI) on a small Dirac run gives
+    the same j(t) as the Python binding and the oracle."""
+    import json
+    import math
+    import os
+    import subprocess
+    cfg = tmp_path / "run.cfg"
+    cfg.write_text("system_type Dirac2D\nlaser_type cosine\nfield_amplitude 0.5\nn_laser_cycles 2\n"
+                   "nx 16\nny 8\nn_measurements 256\nT2 62.83185307179586\n")
+    out = tmp_path / "out.json"
+    exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
+    print("[case] hhg_run Dirac2D 16x8 N=256", flush=True)
+    r = subprocess.run([exe, str(cfg), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(out.read_text())
+    T = 4 * math.pi
+    md = hhg.ModelDesc(0, 2, 1.0, 0.1, 0.0, 0.0, 0.0, 0.0, 62.83185307179586)
+    with hhg.Plan([hhg.Job(md, hhg.CosineLaser(0.5, 2.0, -T / 2), hhg.TimeIntegrationConfig(-T / 2, T / 2, 256, 50))],
+                  [hhg.KGrid.box(16, 8, -3.0, 3.0, -3.0, 3.0)]) as plan:
+        j = plan.execute()[0]
+    assert np.array_equal(np.array(d["current_density_time"]), j)
+    kx, ky, w = O.gl_box(16, 8, -3.0, 3.0, -3.0, 3.0)
+    j_ref, _ = O.current_density(O.model(0, v=1.0, mass=0.1, relax=2, relax_time=62.83185307179586),
+                                 O.pulse(0, 0.5, n_cycles=2.0, t0=-T / 2), O.time_grid(-T / 2, T / 2, 256, 50),
+                                 O.solver(), kx, ky, w)
+    assert rel_l2(j, j_ref) <= J_TOL
+    f, re, im = hhg.trapezoidal_spectrum(j, hhg.TimeIntegrationConfig(-T / 2, T / 2, 256, 50))
+    assert np.array_equal(np.array(d["current_density_frequency_real"]), re)

@@ -140,3 +140,12 @@ def test_gauss_legendre_1024_against_mpmath(hhg):
         wt = 2 / ((1 - z * z) * dp * dp)
         assert abs(float(mp.mpf(x[i]) - z)) < 1e-16
         assert abs(float((mp.mpf(w[i]) - wt) / wt)) < 5e-14
+
+
+def test_cpp_driver_rejects_unknown_laser(hhg, tmp_path):
+    import subprocess
+    cfg = tmp_path / "bad.cfg"
+    cfg.write_text("system_type Dirac2D\nlaser_type sawtooth\n")
+    exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
+    r = subprocess.run([exe, str(cfg)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "not recognized" in r.stderr
