@@ -37,6 +37,11 @@ F_RHS = {"dirac_t2": 47.0, "dirac": 23.0, "piflux": 26.0, "honeycomb": 34.0}
 F_VERTEX = {"dirac": 3.0, "piflux": 4.0, "honeycomb": 20.0}
 F_MAGNUS = 85.0
 
+# HBM bytes per solve launch of this workload from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (28,239.1 KiB x 2.0 + 1,050,496 KiB x 1.0, the factors a plain 1 GiB
+# copy reads); profiles/solve_kernel_dirac_t2_1024x512_counters.md.
+TRAFFIC_BYTES = (28239.1 * 2.0 + 1050496.0 * 1.0) * 1024
+
 
 def workload(small: bool):
     T = 16 * math.pi
@@ -160,7 +165,7 @@ def main():
                    "attempted_steps": attempts_per_step, "rejects": stats["rejects"],
                    "launch": tuned},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES if not args.small else None,
                      "kernel_ms": solve_ms, "algorithmic_flop_per_launch": flops},
     }
     if args.extras:
