@@ -107,6 +107,34 @@ def cpu_baseline(wl, target_s=10.0):
                       f"oracle -O3 -ffast-math build, OpenMP {threads} threads, {wall:.1f} s"}
 
 
+def sweep_extras(H):
+    """Reduced-size sweeps of configurations 3-5 (same structure, fewer
+    k-points per job): one plan each, autotuned, graph-captured, timed once
+    after one warmup."""
+    from higherharmonics_amd import workloads as W
+    res = {}
+    cases = [("config3_piflux_delay_sweep_128x128", W.piflux_sweep(n=128)),
+             ("config4_honeycomb_16_pulses_192x192", W.honeycomb_cos2(n=192)),
+             ("config5_mixed_64_honeycomb_64_piflux_64x64", W.mixed_sweep(n=64))]
+    for name, jobs in cases:
+        print(f"[bench] sweep {name}: {len(jobs)} jobs, {W.trajectories(jobs)} trajectories", file=sys.stderr,
+              flush=True)
+        plan = W.build_plan(H, jobs, persistent=1)
+        tuned = plan.autotune()
+        plan.capture_graph()
+        plan.execute()
+        t0 = time.perf_counter()
+        plan.execute()
+        wall = time.perf_counter() - t0
+        st = plan.stats.as_dict()
+        plan.close()
+        res[name] = {"jobs": len(jobs), "trajectories": W.trajectories(jobs),
+                     "samples_per_job": [min(j["time"][2] for j in jobs), max(j["time"][2] for j in jobs)],
+                     "trajectories_per_s": W.trajectories(jobs) / wall, "wall_s": wall, "solve_ms": st["solve_ms"],
+                     "attempts": st["attempts"], "rejects": st["rejects"], "launch": tuned}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +144,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--extras", action="store_true", help="also time Magnus-4 and the carry ablation")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--sweeps", action="store_true", help="also time reduced configurations 3-5")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -179,6 +208,8 @@ def main():
             extras[name] = {"trajectories_per_s": max(1, args.steps // 2) * K / w2, "kernel_ms": sm,
                             "attempts": st["attempts"]}
         out["extras"] = extras
+    if args.sweeps:
+        out["sweeps"] = sweep_extras(H)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out))

@@ -1,0 +1,127 @@
+"""Job specifications of the measured configurations.
+
+Each job is a plain dict (model, pulse, time, grid) that can be turned into a
+:class:`higherharmonics_amd.Plan` job here, or into any other solver's input.
+Units are the reference's dimensionless ones (omega = 1).
+
+* ``dirac2d``   — 2D massive Dirac, cos^2 A0 = 0.5, 8 cycles, T2 = 10 cycles,
+  GL nx x ny on |k| <= 3, 4096 samples (configurations 1 and 2).
+* ``piflux_sweep`` — PiFlux-2D over the full BZ, A/B double Gaussian
+  (sigma = 2 cycles), delays x amplitudes, window [min(0, tau) - 5 sigma,
+  max(0, tau) + 5 sigma] at fixed sample density, so N differs per job
+  (configuration 3).
+* ``honeycomb_cos2`` — honeycomb over the hexagonal BZ, cos^2 10 cycles plus
+  one cycle (HoneycombDispatcher.cpp:35), seeded A0 ~ U[0.1, 1.5] and CEP ~
+  U[0, 2 pi] (configuration 4).
+* ``mixed_sweep`` — honeycomb (double cos^2) and PiFlux (double Gaussian)
+  jobs with seeded A0 ~ U[0.1, 2.0], CEP ~ U[0, 2 pi], delay ~ U[-10, 10]
+  cycles (configuration 5).
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, List
+
+import numpy as np
+
+TWO_PI = 2 * math.pi
+SEED = 20261203
+
+
+def _job(model, pulse, time, grid):
+    return {"model": model, "pulse": pulse, "time": time, "grid": grid}
+
+
+def dirac2d(nx=1024, ny=512, n_meas=4096, T2=True):
+    T = 16 * math.pi
+    model = dict(kind=0, v=1.0, mass=0.1, relax=2 if T2 else 0, relax_time=20 * math.pi if T2 else 0.0)
+    pulse = dict(kind=0, amp_a=0.5, n_cycles=8.0, t0=-T / 2)
+    return [_job(model, pulse, (-T / 2, T / 2, n_meas, 50), ("box", nx, ny, -3.0, 3.0, -3.0, 3.0))]
+
+
+def piflux_sweep(n=512, n_delays=32, amps=(0.2, 0.4667, 0.7333, 1.0), per_cycle=512, delays=None):
+    sigma = 2 * TWO_PI
+    if delays is None:
+        delays = np.linspace(-10, 10, n_delays) * TWO_PI
+    jobs = []
+    for a in amps:
+        for tau in delays:
+            t_b = min(0.0, tau) - 5 * sigma
+            t_e = max(0.0, tau) + 5 * sigma
+            N = int(round((t_e - t_b) / TWO_PI * per_cycle))
+            jobs.append(_job(dict(kind=1, hopping=1.0),
+                             dict(kind=2, amp_a=float(a), amp_b=float(a), sigma=sigma, t0=0.0, delay=float(tau)),
+                             (t_b, t_e, N, 50), ("box", n, n, -math.pi, math.pi, -math.pi, math.pi)))
+    return jobs
+
+
+def honeycomb_cos2(n=768, n_pulses=16, n_cycles=10, per_cycle=512, seed=SEED):
+    rng = np.random.default_rng(seed)
+    A0 = rng.uniform(0.1, 1.5, n_pulses)
+    cep = rng.uniform(0.0, TWO_PI, n_pulses)
+    t_end = TWO_PI * n_cycles + TWO_PI
+    N = (n_cycles + 1) * per_cycle
+    return [_job(dict(kind=2, hopping=1.0), dict(kind=0, amp_a=float(a), n_cycles=float(n_cycles), t0=0.0, cep=float(c)),
+                 (0.0, t_end, N, 50), ("hex", n, n)) for a, c in zip(A0, cep)]
+
+
+def mixed_sweep(n=256, n_each=64, per_cycle=256, seed=SEED, cos2_cycles=4, sigma_cycles=2):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    sigma = sigma_cycles * TWO_PI
+    for i in range(n_each):
+        a, b = rng.uniform(0.1, 2.0, 2)
+        cep = rng.uniform(0.0, TWO_PI)
+        tau = rng.uniform(-10, 10) * TWO_PI
+        # honeycomb: double cos^2, window covering both pulses plus one cycle
+        t_b = min(0.0, tau)
+        t_e = max(0.0, tau) + TWO_PI * cos2_cycles + TWO_PI
+        N = int(round((t_e - t_b) / TWO_PI * per_cycle))
+        jobs.append(_job(dict(kind=2, hopping=1.0),
+                         dict(kind=3, amp_a=float(a), amp_b=float(b), n_cycles=float(cos2_cycles), t0=0.0,
+                              delay=float(tau), cep=float(cep)), (t_b, t_e, N, 50), ("hex", n, n)))
+    for i in range(n_each):
+        a, b = rng.uniform(0.1, 2.0, 2)
+        cep = rng.uniform(0.0, TWO_PI)
+        tau = rng.uniform(-10, 10) * TWO_PI
+        t_b = min(0.0, tau) - 5 * sigma
+        t_e = max(0.0, tau) + 5 * sigma
+        N = int(round((t_e - t_b) / TWO_PI * per_cycle))
+        jobs.append(_job(dict(kind=1, hopping=1.0),
+                         dict(kind=2, amp_a=float(a), amp_b=float(b), sigma=sigma, t0=0.0, delay=float(tau),
+                              cep=float(cep)), (t_b, t_e, N, 50), ("box", n, n, -math.pi, math.pi, -math.pi, math.pi)))
+    return jobs
+
+
+def trajectories(jobs: List[Dict]) -> int:
+    n = 0
+    for j in jobs:
+        g = j["grid"]
+        n += g[1] * g[2]
+    return n
+
+
+def build_plan(H, jobs: List[Dict], solver=None, **launch):
+    """A Plan for the jobs; identical grids are shared between jobs."""
+    grids, grid_index, hjobs = [], {}, []
+    for j in jobs:
+        key = tuple(j["grid"])
+        if key not in grid_index:
+            grid_index[key] = len(grids)
+            grids.append(H.KGrid.box(*key[1:]) if key[0] == "box" else H.KGrid.hexagonal(*key[1:]))
+        m, p = j["model"], j["pulse"]
+        md = H.ModelDesc(m.get("kind", 0), m.get("relax", 0), m.get("v", 1.0), m.get("mass", 0.1),
+                         m.get("hopping", 1.0), m.get("theta", 0.0), m.get("fermi_energy", 0.0), m.get("beta", 0.0),
+                         m.get("relax_time", 0.0))
+        pd = H.PulseDesc(p.get("kind", 0), 0, p.get("amp_a", 0.5), p.get("amp_b", 0.0), p.get("n_cycles", 8.0),
+                         p.get("sigma", 0.0), p.get("t0", 0.0), p.get("delay", 0.0), p.get("cep", 0.0))
+
+        class _L(H.Laser):
+            def __init__(self, d):
+                self._d = d
+
+            def desc(self):
+                return self._d
+
+        hjobs.append(H.Job(md, _L(pd), H.TimeIntegrationConfig(*j["time"]), grid_index[key]))
+    return H.Plan(hjobs, grids, solver or H.Solver(), **launch)

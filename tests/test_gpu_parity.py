@@ -175,3 +175,46 @@ def test_cpp_driver_matches_python_path(hhg, tmp_path):
     assert rel_l2(j, j_ref) <= J_TOL
     f, re, im = hhg.trapezoidal_spectrum(j, hhg.TimeIntegrationConfig(-T / 2, T / 2, 256, 50))
     assert np.array_equal(np.array(d["current_density_frequency_real"]), re)
+
+
+def _oracle_job(job):
+    from tests.problems import Problem
+    g = job["grid"]
+    pr = Problem("job", model=dict(job["model"]), pulse=dict(job["pulse"]), time=tuple(job["time"]), grid=tuple(g))
+    return oracle_j(pr)[0]
+
+
+def test_mixed_sweep_one_plan_matches_oracle(hhg):
+    """Configuration-5 structure at small size: honeycomb (double cos^2) and
+    PiFlux (double Gaussian) jobs with seeded amplitudes, CEPs and delays,
+    different N per job, in one plan (one persistent launch per model/pulse
+    group), autotuned and graph-captured."""
+    from higherharmonics_amd import workloads as W
+    jobs = W.mixed_sweep(n=12, n_each=3, per_cycle=64, cos2_cycles=2, sigma_cycles=0.5)
+    print(f"[case] mixed sweep {len(jobs)} jobs, N = {[j['time'][2] for j in jobs]}", flush=True)
+    refs = [_oracle_job(j) for j in jobs]
+    with W.build_plan(hhg, jobs) as plan:
+        out = plan.execute()
+        for j, r in zip(out, refs):
+            assert rel_l2(j, r) <= J_TOL
+        plan.autotune()
+        tuned = plan.execute()
+        plan.capture_graph()
+        for _ in range(2):
+            again = plan.execute()
+            for a, b in zip(again, tuned):
+                assert np.array_equal(a, b)
+    for a, b in zip(tuned, out):
+        assert np.array_equal(a, b)
+
+
+def test_piflux_delay_sweep_matches_oracle(hhg):
+    """Configuration-3 structure at small size: double-Gaussian delays x
+    amplitudes, uneven windows and step counts, one plan."""
+    from higherharmonics_amd import workloads as W
+    jobs = W.piflux_sweep(n=10, delays=np.array([-3.0, 0.0, 2.5]) * 2 * np.pi, amps=(0.3, 1.0), per_cycle=64)
+    print(f"[case] piflux sweep {len(jobs)} jobs", flush=True)
+    refs = [_oracle_job(j) for j in jobs]
+    with W.build_plan(hhg, jobs) as plan:
+        for j, r in zip(plan.execute(), refs):
+            assert rel_l2(j, r) <= J_TOL
