@@ -82,7 +82,7 @@ class JobDesc(ctypes.Structure):
 class SolverDesc(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("proposal", ctypes.c_int32), ("rtol", ctypes.c_double),
                 ("atol", ctypes.c_double), ("max_fail", ctypes.c_int32), ("magnus_substeps", ctypes.c_int32),
-                ("mlp", ctypes.c_void_p)]
+                ("mlp", ctypes.c_void_p), ("mlp_infer_every", ctypes.c_int32), ("fallback_after", ctypes.c_int32)]
 
 
 class LaunchDesc(ctypes.Structure):
@@ -113,7 +113,9 @@ EXPORTED_SYMBOLS = [
     "hhg_plan_execute", "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph",
     "hhg_plan_destroy", "hhg_trajectories", "hhg_gauss_legendre", "hhg_kgrid_size", "hhg_kgrid_build",
     "hhg_pulse_evaluate", "hhg_trapezoidal_spectrum", "hhg_plan_execute_accumulators", "hhg_accumulators_add",
-    "hhg_accumulators_to_double",
+    "hhg_accumulators_to_double", "hhg_plan_set_step_log", "hhg_plan_get_step_log", "hhg_mlp_param_count",
+    "hhg_mlp_create", "hhg_mlp_destroy", "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params",
+    "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -153,6 +155,24 @@ def lib() -> ctypes.CDLL:
                                        ctypes.c_int64]
     L.hhg_accumulators_to_double.restype = None
     L.hhg_accumulators_to_double.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64, _DP]
+    FP = ctypes.POINTER(ctypes.c_float)
+    L.hhg_plan_set_step_log.argtypes = [P, ctypes.c_int64, ctypes.c_int32]
+    L.hhg_plan_get_step_log.argtypes = [P, FP, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    L.hhg_mlp_param_count.restype = ctypes.c_int64
+    L.hhg_mlp_create.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(P)]
+    L.hhg_mlp_destroy.argtypes = [P]
+    L.hhg_mlp_set_normalization.argtypes = [P, FP, FP, FP, FP]
+    L.hhg_mlp_get_params.argtypes = [P, FP]
+    L.hhg_mlp_set_params.argtypes = [P, FP]
+    L.hhg_mlp_forward.argtypes = [P, FP, ctypes.c_int64, FP]
+    L.hhg_mlp_forward_mfma.argtypes = [P, FP, ctypes.c_int64, FP]
+    L.hhg_mlp_gradients.argtypes = [P, FP, FP, ctypes.c_int64, FP, FP]
+    L.hhg_mlp_train.argtypes = [P, FP, FP, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                ctypes.c_uint64, FP]
+    for name in ("hhg_plan_set_step_log", "hhg_plan_get_step_log", "hhg_mlp_create", "hhg_mlp_destroy",
+                 "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params", "hhg_mlp_forward",
+                 "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train"):
+        getattr(L, name).restype = ctypes.c_int
     for name in ("hhg_plan_create", "hhg_plan_job_offsets", "hhg_plan_execute", "hhg_plan_execute_accumulators",
                  "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
                  "hhg_trajectories", "hhg_gauss_legendre", "hhg_kgrid_build", "hhg_pulse_evaluate",
@@ -301,6 +321,73 @@ class KGrid:
         return kx, ky, w
 
 
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class MLP:
+    """The step-proposal network on one device (8 -> 64 -> 64 -> 64 -> 2,
+    ReLU; BF16 weights and activations, FP32 accumulation, FP32 masters)."""
+    N_IN, N_OUT = 8, 2
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        h = ctypes.c_void_p()
+        _check(lib().hhg_mlp_create(device, seed, ctypes.byref(h)))
+        self._h = h
+        self.n_params = lib().hhg_mlp_param_count()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_normalization(self, in_mean, in_std, out_mean, out_std) -> None:
+        arr = [np.ascontiguousarray(a, dtype=np.float32) for a in (in_mean, in_std, out_mean, out_std)]
+        self._norm = arr
+        _check(lib().hhg_mlp_set_normalization(self._h, *[_fp(a) for a in arr]))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        _check(lib().hhg_mlp_get_params(self._h, _fp(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().hhg_mlp_set_params(self._h, _fp(p)))
+
+    def forward(self, x, mfma: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((len(x), self.N_OUT), dtype=np.float32)
+        fn = lib().hhg_mlp_forward_mfma if mfma else lib().hhg_mlp_forward
+        _check(fn(self._h, _fp(x), len(x), _fp(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(lib().hhg_mlp_gradients(self._h, _fp(x), _fp(t), len(x), _fp(g), _fp(loss)))
+        return g, float(loss[0])
+
+    def train(self, x, t, epochs=4, batch=1024, lr=2e-3, seed=7) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(lib().hhg_mlp_train(self._h, _fp(x), _fp(t), len(x), epochs, batch, lr, seed, _fp(losses)))
+        return losses
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().hhg_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 @dataclass
 class Solver:
     method: int = METHOD_RKF78
@@ -309,9 +396,13 @@ class Solver:
     proposal: int = PROPOSAL_RESTART
     max_fail: int = 500
     magnus_substeps: int = 1
+    mlp: Optional["MLP"] = None
+    infer_every: int = 1
+    fallback_after: int = 0
 
     def desc(self) -> SolverDesc:
-        return SolverDesc(self.method, self.proposal, self.rtol, self.atol, self.max_fail, self.magnus_substeps, None)
+        return SolverDesc(self.method, self.proposal, self.rtol, self.atol, self.max_fail, self.magnus_substeps,
+                          self.mlp.handle if self.mlp is not None else None, self.infer_every, self.fallback_after)
 
 
 @dataclass
@@ -329,14 +420,15 @@ class Plan:
 
     def __init__(self, jobs: Sequence[Job], kgrids: Sequence[KGrid], solver: Solver = Solver(),
                  rank: int = 0, n_ranks: int = 1, device: int = 0, waves_per_block: int = 0,
-                 queue_chunk: int = 0, persistent: int = 1):
+                 queue_chunk: int = 0, persistent: int = 1, group_by_cost: int = 0):
         L = lib()
+        self._solver = solver  # keeps the network alive as long as the plan
         self._kgrids = list(kgrids)  # keep explicit arrays alive
         kg = (KGridDesc * len(kgrids))(*[g.desc() for g in kgrids])
         jd = (JobDesc * len(jobs))(*[JobDesc(j.model, j.laser.desc(), j.time_config.desc(), j.kgrid, 0)
                                      for j in jobs])
         d = PlanDesc(len(kgrids), len(jobs), kg, jd, solver.desc(),
-                     LaunchDesc(waves_per_block, queue_chunk, persistent, 0), rank, n_ranks, device, 0)
+                     LaunchDesc(waves_per_block, queue_chunk, persistent, group_by_cost), rank, n_ranks, device, 0)
         h = ctypes.c_void_p()
         _check(L.hhg_plan_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -362,6 +454,18 @@ class Plan:
         _check(lib().hhg_plan_execute_accumulators(
             self._h, acc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(self.stats)))
         return acc
+
+    def set_step_log(self, capacity: int, stride: int = 16) -> None:
+        _check(lib().hhg_plan_set_step_log(self._h, capacity, stride))
+        self._log_cap = capacity
+
+    def step_log(self) -> np.ndarray:
+        """Records of the last execute: [n][10] = 8 features, log(h/Delta), log(attempts)."""
+        cap = getattr(self, "_log_cap", 0)
+        rec = np.empty((max(cap, 1), 10), dtype=np.float32)
+        n = ctypes.c_int64()
+        _check(lib().hhg_plan_get_step_log(self._h, _fp(rec), cap, ctypes.byref(n)))
+        return rec[:n.value].copy()
 
     def autotune(self) -> dict:
         ld = LaunchDesc()

@@ -142,6 +142,9 @@ struct Solver {
     h_float rtol = 1e-10, atol = 1e-10;
     int max_fail = 500;
     int magnus_substeps = 1;
+    const hhg_mlp* mlp = nullptr;
+    int mlp_infer_every = 1;
+    int fallback_after = 0;
     hhg_solver_desc desc() const {
         hhg_solver_desc d{};
         d.method = method;
@@ -150,6 +153,9 @@ struct Solver {
         d.atol = atol;
         d.max_fail = max_fail;
         d.magnus_substeps = magnus_substeps;
+        d.mlp = mlp;
+        d.mlp_infer_every = mlp_infer_every;
+        d.fallback_after = fallback_after;
         return d;
     }
 };

@@ -382,6 +382,18 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
     return nan ? __builtin_nan("") : err;
 }
 
+// Options and results of one sampling interval beyond the odeint semantics.
+struct IntervalCtl {
+    int fallback_after;  // rejections of the first step after which it restarts at h_fallback (0: never)
+    double h_fallback;   // the plain controller's restart step Delta / n_sub
+};
+struct IntervalOut {
+    double h_first;      // the controller's proposal after the first accepted step
+    double last_err;     // error ratio of the last accepted step
+    uint32_t attempts;   // attempts in this interval
+    bool fell_back;      // the first step was handed back to the plain controller
+};
+
 // integrate_adaptive over [t0, t1] (odeint semantics: absolute-epsilon loop
 // and clip tests, reject -> dt *= max(0.9 err^(-1/6), 0.2), accept and for
 // err < 0.5 grow dt *= 0.9 max(err, 5^-8)^(-1/8)).  h is the starting step
@@ -393,11 +405,15 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
 template <class F>
 __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, double t1, double& h,
                                                double& unclipped, const SolverDev& sp, uint32_t& n_att,
-                                               uint32_t& n_rej, int& status, const double* nom) {
+                                               uint32_t& n_rej, int& status, const double* nom,
+                                               const IntervalCtl& ctl, IntervalOut& io) {
     double t = t0;
     bool fresh = true;
     int tries = 0;
     int step = -1;  // index of the current step inside the interval
+    io.attempts = 0;
+    io.fell_back = false;
+    io.h_first = 0.0;
     for (;;) {
         if (fresh) {
             if (!(t1 - t > kDblEps)) break;
@@ -411,7 +427,7 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
         // step for a carried or proposed first step
         const double* cand = nullptr;
         if (nom) {
-            if (sp.proposal == PROPOSAL_RESTART) {
+            if (sp.proposal == PROPOSAL_RESTART || io.fell_back) {
                 if (step < kNomRestartSlots) cand = nom + step * kNomStride;
             } else if (step == 0) {
                 cand = nom + kNomRestartSlots * kNomStride;
@@ -422,12 +438,18 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
         double snew[3];
         const double err = rkf78_attempt(f, s, h, As, snew, sp.rtol, sp.atol);
         ++n_att;
+        ++io.attempts;
         if (err > 1.0) {
             ++n_rej;
             double fac = 0.9 * pow(err, -1.0 / 6.0);
             fac = fac > 0.2 ? fac : 0.2;
             h *= fac;
-            if (++tries > sp.max_fail || !(h > 0.0)) {
+            ++tries;
+            if (ctl.fallback_after > 0 && step == 0 && !io.fell_back && tries >= ctl.fallback_after) {
+                h = ctl.h_fallback;
+                io.fell_back = true;
+            }
+            if (tries > sp.max_fail || !(h > 0.0)) {
                 status = 1;
                 break;
             }
@@ -436,10 +458,12 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
             s[0] = snew[0];
             s[1] = snew[1];
             s[2] = snew[2];
+            io.last_err = err;
             if (err < 0.5) {
                 if (err > (1.0 / 390625.0)) h *= 0.9 / sqrt(sqrt(sqrt(err)));
                 else h *= max_growth();
             }
+            if (io.h_first == 0.0) io.h_first = h;
             fresh = true;
             if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) {
                 status = 2;

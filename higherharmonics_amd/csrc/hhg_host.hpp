@@ -7,6 +7,10 @@
 
 namespace hhg {
 
+struct MlpDev;
+const MlpDev* mlp_device_record(const hhg_mlp* m);
+int mlp_device(const hhg_mlp* m);
+
 hhg_status fail(hhg_status st, const std::string& msg);
 void clear_error();
 const char* last_error();

@@ -68,7 +68,8 @@ struct Counters {
     unsigned long long nonfinite;
     unsigned long long fallbacks;
     unsigned long long queue;  // work-queue head
-    unsigned long long pad[2];
+    unsigned long long log_count;
+    unsigned long long pad;
 };
 
 struct SolverDev {

@@ -6,30 +6,61 @@
 
 namespace hhg {
 
+struct MlpDev;
+
 struct KernelArgs {
     const JobDev* jobs;
     const int2* items;  // {job, chunk}
     int n_items;
     int queue_chunk;
-    int pad0, pad1;
+    int infer_every;     // learned proposal: intervals between network evaluations
+    int fallback_after;  // learned proposal: rejections before the plain controller takes over
     const double* kx;
     const double* ky;
     const double* w;
-    const int64_t* perm;  // nullptr: identity lane -> k map
+    const int64_t* perm;  // nullptr: identity lane -> k map; else per job relative k index
     const double* At;
     const double* Am;
     const double* nom;  // nominal-step pulse table (nullptr: evaluate every stage)
     unsigned long long* acc;  // 2 words per output sample
     Counters* ctr;
+    const MlpDev* mlp;        // learned proposal / cost probe network
+    float* log;               // step log records (nullptr: no logging)
+    unsigned long long* log_count;
+    long long log_cap;
+    int log_stride;
+    int pad;
     SolverDev sp;
 };
 
-hipError_t launch_solve(int method, int model, int pulse, int relax, const KernelArgs& a, int blocks, int threads,
-                        hipStream_t st);
-const void* solve_kernel_fn(int method, int model, int pulse, int relax);
-hipError_t launch_trajectories(int method, int model, int pulse, int relax, const JobDev* jobs, const double* kx,
-                               const double* ky, const int64_t* sel, int64_t n_sel, const double* Am, SolverDev sp,
-                               double* s_out, Counters* ctr, hipStream_t st);
+struct TrajArgs {
+    const JobDev* jobs;
+    const double* kx;
+    const double* ky;
+    const int64_t* sel;
+    int64_t n_sel;
+    const double* Am;
+    SolverDev sp;
+    double* s_out;
+    Counters* ctr;
+    hipStream_t st;
+};
+
+// per-model entry points (explicitly instantiated in hhg_solve_<model>.hip)
+template <int M>
+hipError_t solve_model(int method, int pulse, int relax, bool learned, const KernelArgs& a, int blocks, int threads,
+                       size_t smem, hipStream_t st, const void** fn);
+template <int M>
+hipError_t traj_model(int method, int pulse, int relax, const TrajArgs& x);
+template <int M>
+hipError_t cost_model(int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem, hipStream_t st);
+
+hipError_t launch_solve(int method, int model, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
+                        int threads, size_t smem, hipStream_t st);
+const void* solve_kernel_fn(int method, int model, int pulse, int relax, bool learned);
+hipError_t launch_trajectories(int method, int model, int pulse, int relax, const TrajArgs& x);
+hipError_t launch_cost_probe(int model, int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem,
+                             hipStream_t st);
 hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Am,
                               int with_magnus, hipStream_t st);
 hipError_t launch_nominal_table(int pulse, const JobDev* jobs, int job, int64_t n_meas, double* tab,

@@ -1,0 +1,475 @@
+// hhg_mlp.hip — training and evaluation of the step-proposal network on the
+// GPU (FP32 master weights, BF16 weights and activations in every forward
+// pass, FP32 accumulation), and its C ABI.
+//
+// A training step on a batch of n samples:
+//   fwd_bwd_kernel   one thread per sample: forward pass with the BF16-rounded
+//                    weights (staged in LDS) and BF16-rounded activations, loss
+//                    0.5 |y - t|^2 averaged over the batch, backward pass with
+//                    the straight-through derivative of the BF16 rounding;
+//                    activations and deltas of every sample go to a work array
+//   grad_kernel      one thread per parameter: the batch sum of delta x
+//                    activation in sample order (deterministic, no atomics)
+//   adam_kernel      Adam update of the FP32 masters
+//   refresh_kernel   BF16 roundings (FP32 copy) and the MFMA operand packing
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/hhg_amd.h"
+#include "hhg_host.hpp"
+#include "hhg_mlp.hpp"
+
+using hhg::fail;
+
+namespace hhg {
+namespace {
+
+constexpr int kRec = 400;  // floats per sample in the work array
+constexpr int kA0 = 0, kA1 = 8, kA2 = 72, kA3 = 136, kD1 = 200, kD2 = 264, kD3 = 328, kDY = 392, kLoss = 394;
+
+__device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
+
+__global__ void refresh_kernel(const float* __restrict__ params, float* __restrict__ params_bf,
+                               __bf16* __restrict__ pack) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < kMlpParams) params_bf[idx] = bf16r(params[idx]);
+    if (idx < kPackTotal) {
+        int layer, off;
+        if (idx < kPackL1) layer = 1, off = idx;
+        else if (idx < kPackL1 + kPackL23) layer = 2, off = idx - kPackL1;
+        else layer = 3, off = idx - kPackL1 - kPackL23;
+        const int nks = layer == 1 ? 1 : 4;
+        const int tile = off / kPackTile, rem = off % kPackTile;
+        const int mt = tile / nks, ks = tile % nks;
+        const int lane = rem / 8, i = rem % 8;
+        const int row = 32 * mt + (lane & 31);
+        const int k = 16 * ks + 8 * (lane >> 5) + i;
+        float w = 0.0f;
+        if (layer == 1) w = k < kMlpIn ? params[kOffW1 + row * kMlpIn + k] : 0.0f;
+        else w = params[(layer == 2 ? kOffW2 : kOffW3) + row * kMlpHidden + k];
+        pack[idx] = (__bf16)w;
+    }
+}
+
+// thread per sample; W: the BF16-rounded parameters.  x, t raw (t may be null:
+// forward only, y written de-normalised to y_out).
+__global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ Wg, const float* __restrict__ x,
+                                                      const float* __restrict__ t, const int* __restrict__ idx,
+                                                      int n, MlpDev norm, float inv_n, float* __restrict__ work,
+                                                      float* __restrict__ y_out) {
+    __shared__ float W[kMlpParams];
+    for (int p = threadIdx.x; p < kMlpParams; p += blockDim.x) W[p] = Wg[p];
+    __syncthreads();
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const int s = idx ? idx[b] : b;
+    float* R = work + (size_t)b * kRec;
+    float a0[kMlpIn];
+    for (int i = 0; i < kMlpIn; ++i) {
+        a0[i] = bf16r((x[(size_t)s * kMlpIn + i] - norm.in_mean[i]) * norm.in_istd[i]);
+        R[kA0 + i] = a0[i];
+    }
+    // forward
+    for (int u = 0; u < kMlpHidden; ++u) {
+        float z = W[kOffB1 + u];
+        for (int i = 0; i < kMlpIn; ++i) z = fmaf(W[kOffW1 + u * kMlpIn + i], a0[i], z);
+        R[kA1 + u] = bf16r(z > 0.0f ? z : 0.0f);
+    }
+    for (int layer = 0; layer < 2; ++layer) {
+        const int w0 = layer == 0 ? kOffW2 : kOffW3, b0 = layer == 0 ? kOffB2 : kOffB3;
+        const int in = layer == 0 ? kA1 : kA2, out = layer == 0 ? kA2 : kA3;
+        for (int u = 0; u < kMlpHidden; ++u) {
+            float z = W[b0 + u];
+            for (int v = 0; v < kMlpHidden; ++v) z = fmaf(W[w0 + u * kMlpHidden + v], R[in + v], z);
+            R[out + u] = bf16r(z > 0.0f ? z : 0.0f);
+        }
+    }
+    float y[kMlpOut];
+    for (int o = 0; o < kMlpOut; ++o) {
+        float z = W[kOffB4 + o];
+        for (int u = 0; u < kMlpHidden; ++u) z = fmaf(W[kOffW4 + o * kMlpHidden + u], R[kA3 + u], z);
+        y[o] = z;
+    }
+    if (!t) {
+        for (int o = 0; o < kMlpOut; ++o) y_out[(size_t)b * kMlpOut + o] = y[o] * norm.out_std[o] + norm.out_mean[o];
+        return;
+    }
+    // loss and backward
+    float loss = 0.0f, dy[kMlpOut];
+    for (int o = 0; o < kMlpOut; ++o) {
+        const float tn = (t[(size_t)s * kMlpOut + o] - norm.out_mean[o]) / norm.out_std[o];
+        const float e = y[o] - tn;
+        loss += 0.5f * e * e;
+        dy[o] = e * inv_n;
+        R[kDY + o] = dy[o];
+    }
+    R[kLoss] = loss;
+    for (int u = 0; u < kMlpHidden; ++u) {
+        float g = 0.0f;
+        for (int o = 0; o < kMlpOut; ++o) g = fmaf(W[kOffW4 + o * kMlpHidden + u], dy[o], g);
+        R[kD3 + u] = R[kA3 + u] > 0.0f ? g : 0.0f;
+    }
+    for (int layer = 1; layer >= 0; --layer) {
+        const int w0 = layer == 1 ? kOffW3 : kOffW2;
+        const int dout = layer == 1 ? kD3 : kD2, din = layer == 1 ? kD2 : kD1, act = layer == 1 ? kA2 : kA1;
+        for (int v = 0; v < kMlpHidden; ++v) {
+            float g = 0.0f;
+            for (int u = 0; u < kMlpHidden; ++u) g = fmaf(W[w0 + u * kMlpHidden + v], R[dout + u], g);
+            R[din + v] = R[act + v] > 0.0f ? g : 0.0f;
+        }
+    }
+}
+
+// thread per parameter, batch sum in sample order
+__global__ void grad_kernel(const float* __restrict__ work, int n, float* __restrict__ grad,
+                            float* __restrict__ loss_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kMlpParams) return;
+    float acc = 0.0f;
+    if (p == kMlpParams) {
+        for (int b = 0; b < n; ++b) acc += work[(size_t)b * kRec + kLoss];
+        *loss_out = acc / (float)n;
+        return;
+    }
+    int dofs, aofs = -1;
+    if (p < kOffB1) dofs = kD1 + (p - kOffW1) / kMlpIn, aofs = kA0 + (p - kOffW1) % kMlpIn;
+    else if (p < kOffW2) dofs = kD1 + (p - kOffB1);
+    else if (p < kOffB2) dofs = kD2 + (p - kOffW2) / kMlpHidden, aofs = kA1 + (p - kOffW2) % kMlpHidden;
+    else if (p < kOffW3) dofs = kD2 + (p - kOffB2);
+    else if (p < kOffB3) dofs = kD3 + (p - kOffW3) / kMlpHidden, aofs = kA2 + (p - kOffW3) % kMlpHidden;
+    else if (p < kOffW4) dofs = kD3 + (p - kOffB3);
+    else if (p < kOffB4) dofs = kDY + (p - kOffW4) / kMlpHidden, aofs = kA3 + (p - kOffW4) % kMlpHidden;
+    else dofs = kDY + (p - kOffB4);
+    for (int b = 0; b < n; ++b) {
+        const float* R = work + (size_t)b * kRec;
+        acc = aofs >= 0 ? fmaf(R[dofs], R[aofs], acc) : acc + R[dofs];
+    }
+    grad[p] = acc;
+}
+
+__global__ void adam_kernel(float* __restrict__ params, const float* __restrict__ grad, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const float g = grad[p];
+    m[p] = b1 * m[p] + (1.0f - b1) * g;
+    v[p] = b2 * v[p] + (1.0f - b2) * g * g;
+    params[p] -= lr * (m[p] * c1) / (sqrtf(v[p] * c2) + eps);
+}
+
+extern __shared__ __bf16 mfma_fwd_lds[];
+
+// The solve kernel's wave-level MFMA inference on a feature array: one wave
+// per 64 samples (padding lanes take zero features and write nothing).
+__global__ __launch_bounds__(256) void mfma_forward_kernel(const MlpDev* M, const float* __restrict__ x, int64_t n,
+                                                           float* __restrict__ y) {
+    const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
+    const int64_t s = ((int64_t)blockIdx.x * 4 + w) * 64 + lane;
+    float feat[kMlpIn], out[kMlpOut];
+    for (int i = 0; i < kMlpIn; ++i) feat[i] = s < n ? x[s * kMlpIn + i] : 0.0f;
+    mlp_infer_wave(M, feat, out, mfma_fwd_lds + w * kMlpLdsPerWave);
+    if (s < n)
+        for (int o = 0; o < kMlpOut; ++o) y[s * kMlpOut + o] = out[o];
+}
+
+}  // namespace
+}  // namespace hhg
+
+struct hhg_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float *d_params = nullptr, *d_params_bf = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;
+    __bf16* d_pack = nullptr;
+    hhg::MlpDev* d_dev = nullptr;
+    hhg::MlpDev host_dev{};
+    float* d_loss = nullptr;
+    int64_t adam_t = 0;
+    // training scratch
+    float *d_x = nullptr, *d_t = nullptr, *d_work = nullptr, *d_y = nullptr;
+    int* d_idx = nullptr;
+    int64_t cap_data = 0, cap_batch = 0;
+};
+
+namespace {
+
+hhg_status chk(hipError_t e, const char* what) {
+    if (e == hipSuccess) return HHG_SUCCESS;
+    return fail(e == hipErrorOutOfMemory ? HHG_ERROR_OUT_OF_MEMORY : HHG_ERROR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define MLP_TRY(expr, what)                               \
+    do {                                                  \
+        const hhg_status _s = chk((expr), (what));        \
+        if (_s != HHG_SUCCESS) return _s;                 \
+    } while (0)
+
+hhg_status refresh(hhg_mlp* m) {
+    const int n = std::max(hhg::kMlpParams, hhg::kPackTotal);
+    hipLaunchKernelGGL(hhg::refresh_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_params,
+                       m->d_params_bf, m->d_pack);
+    MLP_TRY(hipGetLastError(), "refresh");
+    m->host_dev.pack = m->d_pack;
+    m->host_dev.params = m->d_params_bf;
+    MLP_TRY(hipMemcpyAsync(m->d_dev, &m->host_dev, sizeof(hhg::MlpDev), hipMemcpyHostToDevice, m->stream),
+            "upload network record");
+    MLP_TRY(hipStreamSynchronize(m->stream), "refresh sync");
+    return HHG_SUCCESS;
+}
+
+hhg_status ensure_data(hhg_mlp* m, int64_t n) {
+    if (n <= m->cap_data) return HHG_SUCCESS;
+    hipFree(m->d_x);
+    hipFree(m->d_t);
+    hipFree(m->d_idx);
+    hipFree(m->d_y);
+    m->d_x = m->d_t = m->d_y = nullptr;
+    m->d_idx = nullptr;
+    MLP_TRY(hipMalloc(&m->d_x, sizeof(float) * hhg::kMlpIn * n), "x");
+    MLP_TRY(hipMalloc(&m->d_t, sizeof(float) * hhg::kMlpOut * n), "t");
+    MLP_TRY(hipMalloc(&m->d_y, sizeof(float) * hhg::kMlpOut * n), "y");
+    MLP_TRY(hipMalloc(&m->d_idx, sizeof(int) * n), "index");
+    m->cap_data = n;
+    return HHG_SUCCESS;
+}
+
+hhg_status ensure_batch(hhg_mlp* m, int64_t n) {
+    if (n <= m->cap_batch) return HHG_SUCCESS;
+    hipFree(m->d_work);
+    m->d_work = nullptr;
+    MLP_TRY(hipMalloc(&m->d_work, sizeof(float) * hhg::kRec * n), "work");
+    m->cap_batch = n;
+    return HHG_SUCCESS;
+}
+
+// gradient of the mean loss over samples idx[0..n) (or 0..n) of the uploaded data
+hhg_status gradient_step(hhg_mlp* m, const int* d_idx, int n) {
+    const hhg_status eb = ensure_batch(m, n);
+    if (eb != HHG_SUCCESS) return eb;
+    hipLaunchKernelGGL(hhg::fwd_bwd_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_params_bf, m->d_x,
+                       m->d_t, d_idx, n, m->host_dev, 1.0f / (float)n, m->d_work, nullptr);
+    MLP_TRY(hipGetLastError(), "forward/backward");
+    hipLaunchKernelGGL(hhg::grad_kernel, dim3((hhg::kMlpParams + 1 + 255) / 256), dim3(256), 0, m->stream, m->d_work, n,
+                       m->d_grad, m->d_loss);
+    MLP_TRY(hipGetLastError(), "gradient");
+    return HHG_SUCCESS;
+}
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+namespace hhg {
+const MlpDev* mlp_device_record(const hhg_mlp* m) { return m ? m->d_dev : nullptr; }
+int mlp_device(const hhg_mlp* m) { return m ? m->device : -1; }
+}  // namespace hhg
+
+extern "C" {
+
+int64_t hhg_mlp_param_count(void) { return hhg::kMlpParams; }
+
+hhg_status hhg_mlp_create(int32_t device, uint64_t seed, hhg_mlp** out) {
+    hhg::clear_error();
+    if (!out) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_create: null argument");
+    *out = nullptr;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev < 1) return fail(HHG_ERROR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n_dev) return fail(HHG_ERROR_INVALID_ARGUMENT, "device ordinal out of range");
+    hhg_mlp* m = new hhg_mlp();
+    m->device = device;
+    auto bail = [&](hhg_status st) {
+        hhg_mlp_destroy(m);
+        return st;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail(fail(HHG_ERROR_HIP, "hipSetDevice"));
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(HHG_ERROR_HIP, "stream"));
+    const size_t P = sizeof(float) * hhg::kMlpParams;
+    hhg_status st = HHG_SUCCESS;
+    for (float** p : {&m->d_params, &m->d_params_bf, &m->d_grad, &m->d_m, &m->d_v})
+        if (st == HHG_SUCCESS) st = chk(hipMalloc(p, P), "parameters");
+    if (st == HHG_SUCCESS) st = chk(hipMalloc(&m->d_pack, sizeof(__bf16) * hhg::kPackTotal), "packed weights");
+    if (st == HHG_SUCCESS) st = chk(hipMalloc(&m->d_dev, sizeof(hhg::MlpDev)), "network record");
+    if (st == HHG_SUCCESS) st = chk(hipMalloc(&m->d_loss, sizeof(float)), "loss");
+    if (st != HHG_SUCCESS) return bail(st);
+    // He-uniform initialisation from a splitmix64 stream; zero biases
+    std::vector<float> w(hhg::kMlpParams, 0.0f);
+    uint64_t s = seed;
+    auto fill = [&](int off, int rows, int cols) {
+        const float bound = std::sqrt(6.0f / (float)cols);
+        for (int i = 0; i < rows * cols; ++i) {
+            const double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+            w[off + i] = (float)((2.0 * u - 1.0) * bound);
+        }
+    };
+    fill(hhg::kOffW1, hhg::kMlpHidden, hhg::kMlpIn);
+    fill(hhg::kOffW2, hhg::kMlpHidden, hhg::kMlpHidden);
+    fill(hhg::kOffW3, hhg::kMlpHidden, hhg::kMlpHidden);
+    fill(hhg::kOffW4, hhg::kMlpOut, hhg::kMlpHidden);
+    for (int i = 0; i < hhg::kMlpIn; ++i) m->host_dev.in_mean[i] = 0.0f, m->host_dev.in_istd[i] = 1.0f;
+    for (int i = 0; i < hhg::kMlpOut; ++i) m->host_dev.out_mean[i] = 0.0f, m->host_dev.out_std[i] = 1.0f;
+    std::vector<float> zeros(hhg::kMlpParams, 0.0f);
+    if ((st = chk(hipMemcpy(m->d_params, w.data(), P, hipMemcpyHostToDevice), "init")) != HHG_SUCCESS ||
+        (st = chk(hipMemcpy(m->d_m, zeros.data(), P, hipMemcpyHostToDevice), "init")) != HHG_SUCCESS ||
+        (st = chk(hipMemcpy(m->d_v, zeros.data(), P, hipMemcpyHostToDevice), "init")) != HHG_SUCCESS)
+        return bail(st);
+    if ((st = refresh(m)) != HHG_SUCCESS) return bail(st);
+    *out = m;
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_mlp_destroy(hhg_mlp* m) {
+    if (!m) return HHG_SUCCESS;
+    hipSetDevice(m->device);
+    for (void* p : {(void*)m->d_params, (void*)m->d_params_bf, (void*)m->d_grad, (void*)m->d_m, (void*)m->d_v,
+                    (void*)m->d_pack, (void*)m->d_dev, (void*)m->d_loss, (void*)m->d_x, (void*)m->d_t,
+                    (void*)m->d_work, (void*)m->d_y, (void*)m->d_idx})
+        hipFree(p);
+    if (m->stream) hipStreamDestroy(m->stream);
+    delete m;
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_mlp_set_normalization(hhg_mlp* m, const float* in_mean, const float* in_std, const float* out_mean,
+                                     const float* out_std) {
+    hhg::clear_error();
+    if (!m || !in_mean || !in_std || !out_mean || !out_std)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_set_normalization: null argument");
+    for (int i = 0; i < hhg::kMlpIn; ++i) {
+        if (!(in_std[i] > 0.0f)) return fail(HHG_ERROR_INVALID_ARGUMENT, "input std must be > 0");
+        m->host_dev.in_mean[i] = in_mean[i];
+        m->host_dev.in_istd[i] = 1.0f / in_std[i];
+    }
+    for (int i = 0; i < hhg::kMlpOut; ++i) {
+        if (!(out_std[i] > 0.0f)) return fail(HHG_ERROR_INVALID_ARGUMENT, "output std must be > 0");
+        m->host_dev.out_mean[i] = out_mean[i];
+        m->host_dev.out_std[i] = out_std[i];
+    }
+    MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
+    return refresh(m);
+}
+
+hhg_status hhg_mlp_get_params(const hhg_mlp* m, float* params) {
+    hhg::clear_error();
+    if (!m || !params) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_get_params: null argument");
+    MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
+    MLP_TRY(hipMemcpy(params, m->d_params, sizeof(float) * hhg::kMlpParams, hipMemcpyDeviceToHost), "download");
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_mlp_set_params(hhg_mlp* m, const float* params) {
+    hhg::clear_error();
+    if (!m || !params) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_set_params: null argument");
+    MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
+    MLP_TRY(hipMemcpy(m->d_params, params, sizeof(float) * hhg::kMlpParams, hipMemcpyHostToDevice), "upload");
+    return refresh(m);
+}
+
+hhg_status hhg_mlp_forward(hhg_mlp* m, const float* x, int64_t n, float* y) {
+    hhg::clear_error();
+    if (!m || (n > 0 && (!x || !y))) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_forward: null argument");
+    if (n <= 0) return HHG_SUCCESS;
+    if (n > (1 << 26)) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_forward: n too large");
+    MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
+    hhg_status st = ensure_data(m, n);
+    if (st == HHG_SUCCESS) st = ensure_batch(m, n);
+    if (st != HHG_SUCCESS) return st;
+    MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
+    hipLaunchKernelGGL(hhg::fwd_bwd_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->d_params_bf,
+                       m->d_x, nullptr, nullptr, (int)n, m->host_dev, 1.0f, m->d_work, m->d_y);
+    MLP_TRY(hipGetLastError(), "forward");
+    MLP_TRY(hipStreamSynchronize(m->stream), "forward sync");
+    MLP_TRY(hipMemcpy(y, m->d_y, sizeof(float) * hhg::kMlpOut * n, hipMemcpyDeviceToHost), "download y");
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_mlp_forward_mfma(hhg_mlp* m, const float* x, int64_t n, float* y) {
+    hhg::clear_error();
+    if (!m || (n > 0 && (!x || !y))) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_forward_mfma: null argument");
+    if (n <= 0) return HHG_SUCCESS;
+    if (n > (1 << 26)) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_forward_mfma: n too large");
+    MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
+    hhg_status st = ensure_data(m, n);
+    if (st != HHG_SUCCESS) return st;
+    MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
+    const int blocks = (int)((n + 255) / 256);
+    const size_t smem = 4 * hhg::kMlpLdsPerWave * sizeof(__bf16);
+    hipLaunchKernelGGL(hhg::mfma_forward_kernel, dim3(blocks), dim3(256), smem, m->stream, m->d_dev, m->d_x, n, m->d_y);
+    MLP_TRY(hipGetLastError(), "mfma forward");
+    MLP_TRY(hipStreamSynchronize(m->stream), "mfma forward sync");
+    MLP_TRY(hipMemcpy(y, m->d_y, sizeof(float) * hhg::kMlpOut * n, hipMemcpyDeviceToHost), "download y");
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_mlp_gradients(hhg_mlp* m, const float* x, const float* t, int64_t n, float* grad, float* loss) {
+    hhg::clear_error();
+    if (!m || n <= 0 || !x || !t || !grad) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_gradients: bad argument");
+    if (n > (1 << 20)) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_gradients: n too large");
+    MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
+    hhg_status st = ensure_data(m, n);
+    if (st != HHG_SUCCESS) return st;
+    MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
+    MLP_TRY(hipMemcpy(m->d_t, t, sizeof(float) * hhg::kMlpOut * n, hipMemcpyHostToDevice), "upload t");
+    if ((st = gradient_step(m, nullptr, (int)n)) != HHG_SUCCESS) return st;
+    MLP_TRY(hipStreamSynchronize(m->stream), "gradient sync");
+    MLP_TRY(hipMemcpy(grad, m->d_grad, sizeof(float) * hhg::kMlpParams, hipMemcpyDeviceToHost), "download");
+    if (loss) MLP_TRY(hipMemcpy(loss, m->d_loss, sizeof(float), hipMemcpyDeviceToHost), "download loss");
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_mlp_train(hhg_mlp* m, const float* x, const float* t, int64_t n, int32_t epochs, int32_t batch,
+                         float lr, uint64_t seed, float* epoch_loss) {
+    hhg::clear_error();
+    if (!m || n <= 0 || !x || !t || epochs < 1 || batch < 1 || !(lr > 0.0f))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_train: bad argument");
+    if (n > (1 << 26) || batch > (1 << 16)) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_train: too large");
+    MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
+    hhg_status st = ensure_data(m, n);
+    if (st != HHG_SUCCESS) return st;
+    MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
+    MLP_TRY(hipMemcpy(m->d_t, t, sizeof(float) * hhg::kMlpOut * n, hipMemcpyHostToDevice), "upload t");
+    std::vector<int> order(n);
+    uint64_t s = seed;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int e = 0; e < epochs; ++e) {
+        std::iota(order.begin(), order.end(), 0);
+        for (int64_t i = n - 1; i > 0; --i) std::swap(order[i], order[splitmix(s) % (uint64_t)(i + 1)]);
+        MLP_TRY(hipMemcpyAsync(m->d_idx, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, m->stream), "order");
+        double acc = 0.0;
+        int steps = 0;
+        for (int64_t b0 = 0; b0 < n; b0 += batch) {
+            const int nb = (int)std::min<int64_t>(batch, n - b0);
+            if ((st = gradient_step(m, m->d_idx + b0, nb)) != HHG_SUCCESS) return st;
+            ++m->adam_t;
+            const float c1 = 1.0f / (1.0f - std::pow(b1, (float)m->adam_t));
+            const float c2 = 1.0f / (1.0f - std::pow(b2, (float)m->adam_t));
+            hipLaunchKernelGGL(hhg::adam_kernel, dim3((hhg::kMlpParams + 255) / 256), dim3(256), 0, m->stream,
+                               m->d_params, m->d_grad, m->d_m, m->d_v, lr, b1, b2, eps, c1, c2);
+            MLP_TRY(hipGetLastError(), "adam");
+            const int nr = std::max(hhg::kMlpParams, hhg::kPackTotal);
+            hipLaunchKernelGGL(hhg::refresh_kernel, dim3((nr + 255) / 256), dim3(256), 0, m->stream, m->d_params,
+                               m->d_params_bf, m->d_pack);
+            MLP_TRY(hipGetLastError(), "refresh");
+            float l = 0.0f;
+            MLP_TRY(hipMemcpyAsync(&l, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, m->stream), "loss");
+            MLP_TRY(hipStreamSynchronize(m->stream), "train sync");
+            acc += l;
+            ++steps;
+        }
+        if (epoch_loss) epoch_loss[e] = (float)(acc / std::max(1, steps));
+    }
+    return refresh(m);
+}
+
+}  // extern "C"

@@ -20,6 +20,7 @@
 #include "hhg_host.hpp"
 #include "hhg_internal.hpp"
 #include "hhg_kernels.hpp"
+#include "hhg_mlp.hpp"
 
 using hhg::fail;
 
@@ -61,6 +62,16 @@ struct hhg_plan {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
+    const hhg::MlpDev* d_mlp = nullptr;  // owned by the hhg_mlp handle
+    int64_t* d_perm = nullptr;
+    float* d_log = nullptr;
+    int64_t log_cap = 0;
+    int log_stride = 16;
+
+    bool learned() const { return solver.proposal == HHG_PROPOSAL_LEARNED; }
+    size_t smem(int waves_per_block) const {
+        return learned() ? (size_t)waves_per_block * hhg::kMlpLdsPerWave * sizeof(__bf16) : 0;
+    }
 
     // at least one counter block, so trajectory launches have one even when
     // this shard owns no work items
@@ -158,22 +169,25 @@ void destroy_device(hhg_plan* p) {
     hipFree(p->d_At);
     hipFree(p->d_Am);
     hipFree(p->d_nom);
+    hipFree(p->d_perm);
+    hipFree(p->d_log);
     hipFree(p->d_words);
     hipFree(p->d_j);
     if (p->stream) hipStreamDestroy(p->stream);
 }
 
 int max_block_threads(const hhg_plan* p, const Group& g) {
-    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax);
+    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->learned());
     hipFuncAttributes attr{};
     if (!fn || hipFuncGetAttributes(&attr, fn) != hipSuccess) return 256;
     return attr.maxThreadsPerBlock;
 }
 
 int resident_blocks(const hhg_plan* p, const Group& g, int threads) {
-    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax);
+    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->learned());
     int per_cu = 0;
-    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess || per_cu < 1)
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, p->smem(threads / 64)) != hipSuccess ||
+        per_cu < 1)
         per_cu = 1;
     return per_cu * p->num_cus;
 }
@@ -210,14 +224,22 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.nom = p->d_nom;
         a.acc = p->acc();
         a.ctr = p->counters((int)gi);
+        a.perm = p->d_perm;
+        a.mlp = p->d_mlp;
+        a.infer_every = p->solver.mlp_infer_every > 0 ? p->solver.mlp_infer_every : 1;
+        a.fallback_after = p->solver.fallback_after > 0 ? p->solver.fallback_after : 2;
+        a.log = p->d_log;
+        a.log_count = &p->counters(0)->log_count;
+        a.log_cap = p->log_cap;
+        a.log_stride = p->log_stride;
         a.sp.rtol = p->solver.rtol;
         a.sp.atol = p->solver.atol;
         a.sp.max_fail = p->solver.max_fail > 0 ? p->solver.max_fail : 500;
         a.sp.proposal = p->solver.proposal;
         int blocks = g.blocks;
         if (n_items_limit > 0) blocks = std::min(blocks, (a.n_items + g.waves_per_block - 1) / g.waves_per_block);
-        HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, a, std::max(1, blocks),
-                                  64 * g.waves_per_block, st),
+        HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, p->learned(), a, std::max(1, blocks),
+                                  64 * g.waves_per_block, p->smem(g.waves_per_block), st),
                 "solve launch");
     }
     HHG_TRY(hipEventRecord(p->ev[1], st), "event");
@@ -276,9 +298,15 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown method");
     if (sv.method == HHG_METHOD_RKF78 && (!(sv.rtol >= 0.0) || !(sv.atol >= 0.0) || !(sv.rtol + sv.atol > 0.0)))
         return fail(HHG_ERROR_INVALID_ARGUMENT, "RKF78 needs rtol, atol >= 0, not both 0");
-    if (sv.proposal == HHG_PROPOSAL_LEARNED)
-        return fail(HHG_ERROR_UNSUPPORTED, "learned proposal needs an MLP handle (not provided by this entry point)");
     if (sv.proposal < 0 || sv.proposal > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown proposal");
+    if (sv.proposal == HHG_PROPOSAL_LEARNED && (!sv.mlp || sv.method != HHG_METHOD_RKF78))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "the learned proposal needs RKF78 and a network (solver.mlp)");
+    if (d->launch.group_by_cost && !sv.mlp)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "grouping by predicted cost needs a network (solver.mlp)");
+    if (sv.mlp && hhg::mlp_device(sv.mlp) != d->device)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "the network lives on another device than the plan");
+    if (sv.mlp_infer_every < 0 || sv.fallback_after < 0)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "mlp_infer_every and fallback_after must be >= 0");
     for (int i = 0; i < d->n_jobs; ++i) {
         const hhg_status st = validate_job(d->jobs[i], sv, d->n_kgrids, i);
         if (st != HHG_SUCCESS) return st;
@@ -410,6 +438,52 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     }
     if ((st = check_hip(hipStreamSynchronize(p->stream), "pulse table sync")) != HHG_SUCCESS) return bail(st);
 
+    if (sv.mlp) p->d_mlp = hhg::mlp_device_record(sv.mlp);
+
+    // lockstep grouping: order each job's k-points by the network's predicted
+    // attempts so that the lanes of a wave need similar work
+    if (p->launch.group_by_cost) {
+        float* d_cost = nullptr;
+        if ((st = check_hip(hipMalloc(&d_cost, sizeof(float) * std::max<int64_t>(1, p->n_kpts)), "cost")) != HHG_SUCCESS)
+            return bail(st);
+        if ((st = check_hip(hipMalloc(&p->d_perm, sizeof(int64_t) * std::max<int64_t>(1, p->n_kpts)), "perm")) !=
+            HHG_SUCCESS) {
+            hipFree(d_cost);
+            return bail(st);
+        }
+        for (const auto& g : p->groups) {
+            hhg::KernelArgs a{};
+            a.jobs = p->d_jobs;
+            a.items = p->d_items + g.item_begin;
+            a.n_items = g.n_items;
+            a.kx = p->d_kx;
+            a.ky = p->d_ky;
+            a.At = p->d_At;
+            a.mlp = p->d_mlp;
+            if (st == HHG_SUCCESS)
+                st = check_hip(hhg::launch_cost_probe(g.model, g.pulse, a, 16, d_cost,
+                                                      4 * hhg::kMlpLdsPerWave * sizeof(__bf16), p->stream),
+                               "cost probe");
+        }
+        std::vector<float> cost(p->n_kpts);
+        if (st == HHG_SUCCESS) st = check_hip(hipStreamSynchronize(p->stream), "cost probe sync");
+        if (st == HHG_SUCCESS)
+            st = check_hip(hipMemcpy(cost.data(), d_cost, sizeof(float) * p->n_kpts, hipMemcpyDeviceToHost), "cost");
+        hipFree(d_cost);
+        if (st != HHG_SUCCESS) return bail(st);
+        std::vector<int64_t> perm(p->n_kpts, 0);
+        for (const auto& J : p->jobs) {
+            std::vector<int64_t> order(J.n_k);
+            for (int64_t r = 0; r < J.n_k; ++r) order[r] = r;
+            const float* c = cost.data() + J.k_base;
+            std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return c[x] > c[y]; });
+            for (int64_t r = 0; r < J.n_k; ++r) perm[J.k_base + r] = order[r];
+        }
+        if ((st = check_hip(hipMemcpy(p->d_perm, perm.data(), sizeof(int64_t) * p->n_kpts, hipMemcpyHostToDevice),
+                            "upload perm")) != HHG_SUCCESS)
+            return bail(st);
+    }
+
     // launch shapes
     const int wpb = p->launch.waves_per_block > 0 ? p->launch.waves_per_block : 4;
     const int qc = p->launch.queue_chunk > 0 ? p->launch.queue_chunk : 1;
@@ -527,6 +601,41 @@ hhg_status hhg_plan_capture_graph(hhg_plan* p) {
     return HHG_SUCCESS;
 }
 
+hhg_status hhg_plan_set_step_log(hhg_plan* p, int64_t capacity, int32_t stride) {
+    hhg::clear_error();
+    if (!p || capacity < 0 || stride < 1) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_set_step_log: bad argument");
+    if (p->solver.method != HHG_METHOD_RKF78) return fail(HHG_ERROR_UNSUPPORTED, "step logs need RKF78");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    hipFree(p->d_log);
+    p->d_log = nullptr;
+    p->log_cap = 0;
+    if (capacity > 0) {
+        HHG_TRY(hipMalloc(&p->d_log, sizeof(float) * (hhg::kMlpIn + hhg::kMlpOut) * capacity), "step log");
+        p->log_cap = capacity;
+    }
+    p->log_stride = stride;
+    // a captured graph holds the old arguments
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec), p->graph_exec = nullptr;
+    if (p->graph) hipGraphDestroy(p->graph), p->graph = nullptr;
+    return HHG_SUCCESS;
+}
+
+hhg_status hhg_plan_get_step_log(hhg_plan* p, float* records, int64_t capacity, int64_t* n_records) {
+    hhg::clear_error();
+    if (!p || !n_records || (capacity > 0 && !records))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_get_step_log: null argument");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    hhg::Counters c{};
+    HHG_TRY(hipMemcpy(&c, p->d_words, sizeof(c), hipMemcpyDeviceToHost), "log count");
+    const int64_t n = std::min<int64_t>((int64_t)c.log_count, p->log_cap);
+    *n_records = n;
+    const int64_t m = std::min(n, capacity);
+    if (m > 0)
+        HHG_TRY(hipMemcpy(records, p->d_log, sizeof(float) * (hhg::kMlpIn + hhg::kMlpOut) * m, hipMemcpyDeviceToHost),
+                "step log");
+    return HHG_SUCCESS;
+}
+
 hhg_status hhg_plan_destroy(hhg_plan* p) {
     hhg::clear_error();
     if (!p) return HHG_SUCCESS;
@@ -556,9 +665,8 @@ hhg_status hhg_trajectories(hhg_plan* p, const int64_t* k_index, int64_t n_sel, 
         hhg::SolverDev sp{p->solver.rtol, p->solver.atol, p->solver.max_fail > 0 ? p->solver.max_fail : 500,
                           p->solver.proposal};
         const int pulse = p->solver.method == HHG_METHOD_MAGNUS4 ? 0 : J.pulse;
-        st = check_hip(hhg::launch_trajectories(p->solver.method, J.model, pulse, J.relax, p->d_jobs, p->d_kx, p->d_ky,
-                                                d_sel, n_sel, p->d_Am, sp, d_s, p->counters(0), p->stream),
-                       "trajectory launch");
+        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, d_sel, n_sel, p->d_Am, sp, d_s, p->counters(0), p->stream};
+        st = check_hip(hhg::launch_trajectories(p->solver.method, J.model, pulse, J.relax, x), "trajectory launch");
     }
     if (st == HHG_SUCCESS) st = check_hip(hipStreamSynchronize(p->stream), "synchronize");
     if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(s_out, d_s, out_bytes, hipMemcpyDeviceToHost), "download");

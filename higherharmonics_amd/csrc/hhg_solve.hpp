@@ -1,0 +1,465 @@
+// hhg_solve.hpp — the solve kernels (RKF78 plain/carried/learned, Magnus),
+// the per-k trajectory kernel and the cost probe, as templates instantiated
+// once per model in hhg_solve_<model>.hip.
+//
+// Work decomposition: a work item is 64 consecutive k-points of one job, one
+// per lane of a wavefront.  Lanes advance sampling interval by sampling
+// interval (reference loop: sources/HHG/Systems/DiracSystem.cpp:166-173); at
+// every sample t_i the wave sums w_k g_k(t_i) with a shuffle butterfly, lane
+// (i mod 64) keeps the sum, and every 64 samples the wave adds its 64 sums to
+// the job's exact 128-bit fixed-point accumulators (one vector atomic pair).
+// Waves pop items from a device counter until the list is exhausted.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hhg_device.hpp"
+#include "hhg_kernels.hpp"
+#include "hhg_mlp.hpp"
+
+namespace hhg {
+
+#ifndef HHG_WAVES_PER_EU
+#define HHG_WAVES_PER_EU 2
+#endif
+
+namespace solve_detail {
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Copies a wave-uniform record into scalar registers.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "record must be dword sized");
+    T out;
+    const int* src = reinterpret_cast<const int*>(p);
+    int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return out;
+}
+
+// Per-lane buffering of one sample sum and the 64-sample flush.
+struct SampleSink {
+    unsigned long long* acc;  // job accumulators, 2 words per sample
+    int N;
+    double buf;
+    bool overflow;
+    __device__ __forceinline__ void put(int i, double total, int lane) {
+        const int slot = i & (WAVE - 1);
+        if (lane == slot) buf = total;
+        if (slot == WAVE - 1 || i == N) {
+            const int base = i - slot;
+            if (lane <= slot) overflow |= !fix_accumulate(acc + 2 * (int64_t)(base + lane), buf);
+        }
+    }
+};
+
+struct LaneCounters {
+    uint32_t att = 0, rej = 0, fail = 0, nonfin = 0, fallback = 0;
+};
+
+__device__ __forceinline__ void flush_counters(Counters* ctr, LaneCounters& lc) {
+    const uint32_t a = wave_sum_u32(lc.att), r = wave_sum_u32(lc.rej), f = wave_sum_u32(lc.fail),
+                   n = wave_sum_u32(lc.nonfin), b = wave_sum_u32(lc.fallback);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        if (a) atomicAdd(&ctr->attempts, (unsigned long long)a);
+        if (r) atomicAdd(&ctr->rejects, (unsigned long long)r);
+        if (f) atomicAdd(&ctr->failed, (unsigned long long)f);
+        if (n) atomicAdd(&ctr->nonfinite, (unsigned long long)n);
+        if (b) atomicAdd(&ctr->fallbacks, (unsigned long long)b);
+    }
+    lc = LaneCounters{};
+}
+
+// Network features of one k-point at the start of interval i (times
+// t_{i-1}..t_i): k, |d(k - A)|, gap at k, A, E = -dA/dt (central difference
+// of the sample table), log(h_prev / Delta), log10 of the last error ratio.
+template <class F>
+__device__ __forceinline__ void interval_features(const F& f, JobRef J, const double* At, int i, double gap_k,
+                                                  double h_prev, double last_err, float feat[kMlpIn]) {
+    const double A0 = At[i - 1];
+    const double dA = (i >= 2 && i <= J.n_meas) ? (At[i] - At[i - 2]) / (2.0 * J.delta) : (At[i] - At[i - 1]) / J.delta;
+    double d[3];
+    f.d_of(A0, d);
+    feat[0] = (float)f.kx;
+    feat[1] = (float)f.ky;
+    feat[2] = (float)sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    feat[3] = (float)gap_k;
+    feat[4] = (float)A0;
+    feat[5] = (float)(-dA);
+    feat[6] = (float)log(h_prev / J.delta);
+    feat[7] = (float)log10(last_err > 1e-16 ? last_err : 1e-16);
+}
+
+extern __shared__ __bf16 solve_dyn_lds[];
+
+template <int M, int P, int R, bool LEARNED>
+__device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t kk = (int64_t)chunk * WAVE + lane;
+    const bool active = kk < J.n_k;
+    double kx = 0.0, ky = 0.0, w = 0.0;
+    if (active) {
+        const int64_t idx = J.k_base + (a.perm ? a.perm[J.k_base + kk] : kk);
+        kx = a.kx[idx];
+        ky = a.ky[idx];
+        w = a.w[idx];
+    }
+    Bloch<M, P, R> f(J, kx, ky);
+    double s[3];
+    f.initial(s);
+    double gap_k;
+    {
+        double d[3];
+        f.d_of(0.0, d);
+        gap_k = 2.0 * sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    }
+    const int N = J.n_meas;
+    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
+    const double* At = a.At + J.sample_base;
+    sink.put(0, wave_sum(active ? w * f.vertex(At[0], s) : 0.0), lane);
+
+    int status = active ? 0 : 3;
+    const bool restart = a.sp.proposal == PROPOSAL_RESTART;
+    const double h_restart = J.delta / J.n_sub;
+    double h_carry = h_restart;
+    double h_ratio = 1.0;  // learned proposal h / Delta
+    IntervalCtl ctl{LEARNED ? a.fallback_after : 0, h_restart};
+    IntervalOut io{};
+    io.last_err = 1e-3;
+    const bool logging = a.log != nullptr;
+    __bf16* lds = LEARNED ? solve_dyn_lds + (threadIdx.x / WAVE) * kMlpLdsPerWave : nullptr;
+    for (int i = 1; i <= N; ++i) {
+        float feat[kMlpIn];
+        const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
+        if (LEARNED && ((i - 1) % a.infer_every) == 0) {
+            // every lane of the wave takes part in the inference
+            interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
+            float out[kMlpOut];
+            mlp_infer_wave(a.mlp, feat, out, lds);
+            const double r = exp((double)out[0]);
+            if (r == r && r > 0.0) {
+                h_ratio = r < 1e-4 ? 1e-4 : (r > 4.5 ? 4.5 : r);
+            } else {
+                h_ratio = 0.0;  // non-finite proposal: the plain controller takes the interval
+            }
+        } else if (log_this) {
+            interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
+        }
+        if (status == 0) {
+            const double t0 = J.t_begin + (double)(i - 1) * J.delta;
+            const double t1 = J.t_begin + (double)i * J.delta;
+            double h;
+            if (LEARNED) {
+                h = h_ratio > 0.0 ? h_ratio * J.delta : h_restart;
+                if (!(h_ratio > 0.0)) lc.fallback++;
+            } else {
+                h = restart ? h_restart : h_carry;
+            }
+            double unclipped = h;
+            const double* nom = a.nom ? a.nom + J.nominal_base + (int64_t)(i - 1) * kNomRow : nullptr;
+            rkf78_interval(f, s, t0, t1, h, unclipped, a.sp, lc.att, lc.rej, status, nom, ctl, io);
+            h_carry = h > unclipped ? h : unclipped;
+            if (io.fell_back) lc.fallback++;
+            if (log_this && status == 0) {
+                const unsigned long long slot = atomicAdd(a.log_count, 1ull);
+                if (slot < (unsigned long long)a.log_cap) {
+                    float* rec = a.log + slot * (kMlpIn + kMlpOut);
+#pragma unroll
+                    for (int q = 0; q < kMlpIn; ++q) rec[q] = feat[q];
+                    rec[kMlpIn] = (float)log(io.h_first / J.delta);
+                    rec[kMlpIn + 1] = (float)log((double)io.attempts);
+                }
+            }
+        }
+        const double g = (status != 3) ? w * f.vertex(At[i], s) : 0.0;
+        sink.put(i, wave_sum(g), lane);
+    }
+    if (status == 1) lc.fail++;
+    if (status == 2) lc.nonfin++;
+    if (sink.overflow) lc.nonfin++;
+}
+
+template <int M>
+__device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t kk = (int64_t)chunk * WAVE + lane;
+    const bool active = kk < J.n_k;
+    double kx = 0.0, ky = 0.0, w = 0.0;
+    if (active) {
+        const int64_t idx = J.k_base + (a.perm ? a.perm[J.k_base + kk] : kk);
+        kx = a.kx[idx];
+        ky = a.ky[idx];
+        w = a.w[idx];
+    }
+    Bloch<M, PULSE_COS2, RELAX_NONE> f(J, kx, ky);  // the pulse enters only through the Am table
+    double s[3];
+    f.initial(s);
+    const int N = J.n_meas;
+    const int ms = J.magnus_sub;
+    const double h = J.delta / ms;
+    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
+    const double* At = a.At + J.sample_base;
+    const double* Am = a.Am + J.magnus_base;
+    sink.put(0, wave_sum(active ? w * f.vertex(At[0], s) : 0.0), lane);
+    for (int i = 1; i <= N; ++i) {
+        for (int q = 0; q < ms; ++q) {
+            const int64_t m = 2 * ((int64_t)(i - 1) * ms + q);
+            magnus4_step(f, s, Am[m], Am[m + 1], h);
+        }
+        const double g = active ? w * f.vertex(At[i], s) : 0.0;
+        sink.put(i, wave_sum(g), lane);
+    }
+    if (active) lc.att += (uint32_t)N * (uint32_t)ms;
+    if (active && !(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) lc.nonfin++;
+    if (sink.overflow) lc.nonfin++;
+}
+
+}  // namespace solve_detail
+
+// One kernel per (method, model, pulse, relaxation, learned); a launch only
+// carries items of its own group.  Every wave pops items from the group's
+// queue counter until the list is exhausted: a grid of one wave per item
+// makes each wave solve one item, a grid of the resident wave count is the
+// persistent schedule.
+template <int METHOD, int M, int P, int R, bool LEARNED>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(HHG_WAVES_PER_EU, HHG_WAVES_PER_EU))) void
+solve_kernel(KernelArgs a) {
+    using namespace solve_detail;
+    const int lane = threadIdx.x & (WAVE - 1);
+    LaneCounters lc;
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = (int)atomicAdd(&a.ctr->queue, (unsigned long long)a.queue_chunk);
+        base = uniform(base);
+        if (base >= a.n_items) break;
+        const int end = min(base + a.queue_chunk, a.n_items);
+        for (int it = base; it < end; ++it) {
+            const int2 item = load_uniform(a.items + it);
+            if constexpr (METHOD == 0)
+                solve_item_rkf78<M, P, R, LEARNED>(job_ref(a.jobs + item.x), item.y, a, lc);
+            else
+                solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc);
+        }
+    }
+    flush_counters(a.ctr, lc);
+}
+
+// Per-k trajectories of one job (tests / inspection): one lane per selected
+// k-point, Bloch vector at every sample.
+template <int METHOD, int M, int P, int R>
+__global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, const double* kx, const double* ky,
+                                                        const int64_t* sel, int64_t n_sel, const double* Am,
+                                                        SolverDev sp, double* s_out, Counters* ctr) {
+    const int64_t t_idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t_idx >= n_sel) return;
+    JobRef J = job_ref(jobs);
+    const int64_t idx = J.k_base + sel[t_idx];
+    Bloch<M, P, R> f(J, kx[idx], ky[idx]);
+    double s[3];
+    f.initial(s);
+    const int N = J.n_meas;
+    double* out = s_out + t_idx * (int64_t)(N + 1) * 3;
+    out[0] = s[0];
+    out[1] = s[1];
+    out[2] = s[2];
+    uint32_t att = 0, rej = 0;
+    int status = 0;
+    const double h_restart = J.delta / J.n_sub;
+    double h_carry = h_restart;
+    IntervalCtl ctl{0, h_restart};
+    IntervalOut io{};
+    for (int i = 1; i <= N; ++i) {
+        if constexpr (METHOD == 0) {
+            if (status == 0) {
+                const double t0 = J.t_begin + (double)(i - 1) * J.delta;
+                const double t1 = J.t_begin + (double)i * J.delta;
+                double h = (sp.proposal == PROPOSAL_RESTART) ? h_restart : h_carry;
+                double unclipped = h;
+                rkf78_interval(f, s, t0, t1, h, unclipped, sp, att, rej, status, nullptr, ctl, io);
+                h_carry = h > unclipped ? h : unclipped;
+            }
+        } else {
+            const int ms = J.magnus_sub;
+            const double h = J.delta / ms;
+            for (int q = 0; q < ms; ++q) {
+                const int64_t m = J.magnus_base + 2 * ((int64_t)(i - 1) * ms + q);
+                magnus4_step(f, s, Am[m], Am[m + 1], h);
+            }
+            att += ms;
+        }
+        out[3 * i + 0] = s[0];
+        out[3 * i + 1] = s[1];
+        out[3 * i + 2] = s[2];
+    }
+    atomicAdd(&ctr->attempts, (unsigned long long)att);
+    atomicAdd(&ctr->rejects, (unsigned long long)rej);
+    if (status == 1) atomicAdd(&ctr->failed, 1ull);
+    if (status == 2) atomicAdd(&ctr->nonfinite, 1ull);
+}
+
+// Predicted cost of every k-point of a group's items for lockstep grouping:
+// the network's attempts-per-interval output summed over n_probe evenly
+// spaced intervals (features with h_prev = Delta and a 1e-3 error ratio).
+template <int M, int P>
+__global__ __launch_bounds__(256) void cost_probe_kernel(KernelArgs a, int n_probe, float* cost) {
+    using namespace solve_detail;
+    const int lane = threadIdx.x & (WAVE - 1);
+    __bf16* lds = solve_dyn_lds + (threadIdx.x / WAVE) * kMlpLdsPerWave;
+    const int it = uniform((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
+    if (it >= a.n_items) return;
+    const int2 item = load_uniform(a.items + it);
+    JobRef J = job_ref(a.jobs + item.x);
+    const int64_t kk = (int64_t)item.y * WAVE + lane;
+    const bool active = kk < J.n_k;
+    const int64_t idx = J.k_base + (active ? kk : 0);
+    Bloch<M, P, RELAX_NONE> f(J, a.kx[idx], a.ky[idx]);
+    double d[3];
+    f.d_of(0.0, d);
+    const double gap_k = 2.0 * sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double* At = a.At + J.sample_base;
+    float total = 0.0f;
+    for (int q = 0; q < n_probe; ++q) {
+        const int i = 1 + (int)(((int64_t)q * J.n_meas) / n_probe);
+        float feat[kMlpIn], out[kMlpOut];
+        interval_features(f, J, At, i, gap_k, J.delta, 1e-3, feat);
+        mlp_infer_wave(a.mlp, feat, out, lds);
+        total += expf(out[1]);
+    }
+    if (active) cost[J.k_base + kk] = total;
+}
+
+// Run-time (method, pulse, relax, learned) -> template, for one model.
+template <int M>
+struct ModelLaunch {
+    template <int P, int R, bool L>
+    static hipError_t solve_t(const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,
+                              const void** fn) {
+        if (fn) {
+            *fn = reinterpret_cast<const void*>(&solve_kernel<0, M, P, R, L>);
+            return hipSuccess;
+        }
+        hipLaunchKernelGGL((solve_kernel<0, M, P, R, L>), dim3(blocks), dim3(threads), smem, st, a);
+        return hipGetLastError();
+    }
+    template <int P, int R>
+    static hipError_t solve_l(bool learned, const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,
+                              const void** fn) {
+        return learned ? solve_t<P, R, true>(a, blocks, threads, smem, st, fn)
+                       : solve_t<P, R, false>(a, blocks, threads, smem, st, fn);
+    }
+    template <int P>
+    static hipError_t solve_r(int relax, bool learned, const KernelArgs& a, int blocks, int threads, size_t smem,
+                              hipStream_t st, const void** fn) {
+        switch (relax) {
+        case RELAX_NONE: return solve_l<P, RELAX_NONE>(learned, a, blocks, threads, smem, st, fn);
+        case RELAX_TO_INITIAL: return solve_l<P, RELAX_TO_INITIAL>(learned, a, blocks, threads, smem, st, fn);
+        case RELAX_T2: return solve_l<P, RELAX_T2>(learned, a, blocks, threads, smem, st, fn);
+        }
+        return hipErrorInvalidValue;
+    }
+    static hipError_t solve(int method, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
+                            int threads, size_t smem, hipStream_t st, const void** fn) {
+        if (method == 1) {
+            if (fn) {
+                *fn = reinterpret_cast<const void*>(&solve_kernel<1, M, PULSE_COS2, RELAX_NONE, false>);
+                return hipSuccess;
+            }
+            hipLaunchKernelGGL((solve_kernel<1, M, PULSE_COS2, RELAX_NONE, false>), dim3(blocks), dim3(threads), 0, st,
+                               a);
+            return hipGetLastError();
+        }
+        switch (pulse) {
+        case PULSE_COS2: return solve_r<PULSE_COS2>(relax, learned, a, blocks, threads, smem, st, fn);
+        case PULSE_GAUSS: return solve_r<PULSE_GAUSS>(relax, learned, a, blocks, threads, smem, st, fn);
+        case PULSE_DOUBLE_GAUSS: return solve_r<PULSE_DOUBLE_GAUSS>(relax, learned, a, blocks, threads, smem, st, fn);
+        case PULSE_DOUBLE_COS2: return solve_r<PULSE_DOUBLE_COS2>(relax, learned, a, blocks, threads, smem, st, fn);
+        }
+        return hipErrorInvalidValue;
+    }
+
+    template <int P, int R>
+    static hipError_t traj_t(int method, const TrajArgs& x) {
+        const int blocks = (int)((x.n_sel + 63) / 64);
+        if (method == 1)
+            hipLaunchKernelGGL((trajectory_kernel<1, M, PULSE_COS2, RELAX_NONE>), dim3(blocks), dim3(64), 0, x.st,
+                               x.jobs, x.kx, x.ky, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
+        else
+            hipLaunchKernelGGL((trajectory_kernel<0, M, P, R>), dim3(blocks), dim3(64), 0, x.st, x.jobs, x.kx, x.ky,
+                               x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
+        return hipGetLastError();
+    }
+    template <int P>
+    static hipError_t traj_r(int method, int relax, const TrajArgs& x) {
+        switch (relax) {
+        case RELAX_NONE: return traj_t<P, RELAX_NONE>(method, x);
+        case RELAX_TO_INITIAL: return traj_t<P, RELAX_TO_INITIAL>(method, x);
+        case RELAX_T2: return traj_t<P, RELAX_T2>(method, x);
+        }
+        return hipErrorInvalidValue;
+    }
+    static hipError_t traj(int method, int pulse, int relax, const TrajArgs& x) {
+        switch (pulse) {
+        case PULSE_COS2: return traj_r<PULSE_COS2>(method, relax, x);
+        case PULSE_GAUSS: return traj_r<PULSE_GAUSS>(method, relax, x);
+        case PULSE_DOUBLE_GAUSS: return traj_r<PULSE_DOUBLE_GAUSS>(method, relax, x);
+        case PULSE_DOUBLE_COS2: return traj_r<PULSE_DOUBLE_COS2>(method, relax, x);
+        }
+        return hipErrorInvalidValue;
+    }
+
+    static hipError_t cost(int pulse, const KernelArgs& a, int n_probe, float* cost_out, size_t smem, hipStream_t st) {
+        const int blocks = (a.n_items + 3) / 4;
+        switch (pulse) {
+        case PULSE_COS2:
+            hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_COS2>), dim3(blocks), dim3(256), smem, st, a, n_probe,
+                               cost_out);
+            break;
+        case PULSE_GAUSS:
+            hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_GAUSS>), dim3(blocks), dim3(256), smem, st, a, n_probe,
+                               cost_out);
+            break;
+        case PULSE_DOUBLE_GAUSS:
+            hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_DOUBLE_GAUSS>), dim3(blocks), dim3(256), smem, st, a,
+                               n_probe, cost_out);
+            break;
+        case PULSE_DOUBLE_COS2:
+            hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_DOUBLE_COS2>), dim3(blocks), dim3(256), smem, st, a,
+                               n_probe, cost_out);
+            break;
+        default:
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+};
+
+}  // namespace hhg
+
+namespace hhg {
+
+template <int M>
+hipError_t solve_model(int method, int pulse, int relax, bool learned, const KernelArgs& a, int blocks, int threads,
+                       size_t smem, hipStream_t st, const void** fn) {
+    return ModelLaunch<M>::solve(method, pulse, relax, learned, a, blocks, threads, smem, st, fn);
+}
+
+template <int M>
+hipError_t traj_model(int method, int pulse, int relax, const TrajArgs& x) {
+    return ModelLaunch<M>::traj(method, pulse, relax, x);
+}
+
+template <int M>
+hipError_t cost_model(int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem, hipStream_t st) {
+    return ModelLaunch<M>::cost(pulse, a, n_probe, cost, smem, st);
+}
+
+}  // namespace hhg
+
+#define HHG_INSTANTIATE_MODEL(M)                                                                                 \
+    namespace hhg {                                                                                              \
+    template hipError_t solve_model<M>(int, int, int, bool, const KernelArgs&, int, int, size_t, hipStream_t,    \
+                                       const void**);                                                            \
+    template hipError_t traj_model<M>(int, int, int, const TrajArgs&);                                           \
+    template hipError_t cost_model<M>(int, const KernelArgs&, int, float*, size_t, hipStream_t);                 \
+    }

@@ -1,0 +1,6 @@
+// hhg_solve_dirac.hip — solve, trajectory and cost-probe kernels of the dirac model
+// (templates in hhg_solve.hpp), one translation unit per model so the
+// instantiations compile in parallel.
+#include "hhg_solve.hpp"
+
+HHG_INSTANTIATE_MODEL(hhg::MODEL_DIRAC2D)

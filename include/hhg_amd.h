@@ -143,6 +143,8 @@ typedef enum hhg_proposal {
     HHG_PROPOSAL_LEARNED = 2  /* MLP proposes each interval's first step; controller still judges */
 } hhg_proposal;
 
+typedef struct hhg_mlp hhg_mlp;
+
 typedef struct hhg_solver_desc {
     int32_t method;   /* hhg_method */
     int32_t proposal; /* hhg_proposal */
@@ -150,14 +152,18 @@ typedef struct hhg_solver_desc {
     double atol;
     int32_t max_fail;        /* attempts per step before a trajectory fails (odeint: 500) */
     int32_t magnus_substeps; /* Magnus steps per sampling interval */
-    const void* mlp;         /* HHG_PROPOSAL_LEARNED: hhg_mlp handle (see hhg_mlp_*) */
+    const hhg_mlp* mlp;      /* HHG_PROPOSAL_LEARNED and cost grouping: the step-proposal network */
+    int32_t mlp_infer_every; /* intervals between network evaluations (0: every interval) */
+    int32_t fallback_after;  /* rejections of a proposed first step before the plain controller
+                                restarts the interval at Delta/n_sub (0: 2) */
 } hhg_solver_desc;
 
 typedef struct hhg_launch_desc {
     int32_t waves_per_block; /* 0 = default (4); 1, 2, 4 or 8 waves of 64 lanes per block */
     int32_t queue_chunk;     /* 0 = default/autotuned: work items grabbed per queue pop */
     int32_t persistent;      /* 1 = persistent grid + device work queue, 0 = one wave per item */
-    int32_t group_by_cost;   /* 1 = order k-points inside a job by predicted step count */
+    int32_t group_by_cost;   /* 1 = order k-points inside a job by the network's predicted step count
+                                (needs solver.mlp) so that lanes of a wave need similar work */
 } hhg_launch_desc;
 
 typedef struct hhg_plan_desc {
@@ -246,6 +252,41 @@ hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* 
  * entries.  Keeps the reference's conventions (see hhg_amd spectrum notes). */
 hhg_status hhg_trapezoidal_spectrum(const double* j, const hhg_time_desc* time, int32_t zero_padding,
                                     double* freq, double* re, double* im);
+
+/* ---- step logs (training data for the proposal network) ---- */
+
+/* Enables logging in later executes: for every stride-th interval of every
+ * trajectory (staggered by lane) one record of 10 floats: the 8 network
+ * features at the interval start, log(h / Delta) of the controller's proposal
+ * after the interval's first accepted step, and log(attempts in the interval).
+ * At most capacity records are kept.  capacity = 0 disables logging.  No
+ * counterpart in the reference (its solver keeps no step history). */
+hhg_status hhg_plan_set_step_log(hhg_plan* plan, int64_t capacity, int32_t stride);
+
+/* Copies the records of the last execute (at most capacity) and their count. */
+hhg_status hhg_plan_get_step_log(hhg_plan* plan, float* records, int64_t capacity, int64_t* n_records);
+
+/* ---- step-proposal network: 8 -> 64 -> 64 -> 64 -> 2, ReLU, BF16 weights and
+ * activations with FP32 accumulation; FP32 master weights (no counterpart in
+ * the reference) ---- */
+
+int64_t hhg_mlp_param_count(void);
+hhg_status hhg_mlp_create(int32_t device, uint64_t seed, hhg_mlp** mlp);
+hhg_status hhg_mlp_destroy(hhg_mlp* mlp);
+/* affine normalisation: x_n = (x - in_mean) / in_std, y = y_n * out_std + out_mean */
+hhg_status hhg_mlp_set_normalization(hhg_mlp* mlp, const float* in_mean, const float* in_std, const float* out_mean,
+                                     const float* out_std);
+hhg_status hhg_mlp_get_params(const hhg_mlp* mlp, float* params);
+hhg_status hhg_mlp_set_params(hhg_mlp* mlp, const float* params);
+/* forward pass of n samples (x: n x 8, y: n x 2), one thread per sample */
+hhg_status hhg_mlp_forward(hhg_mlp* mlp, const float* x, int64_t n, float* y);
+/* the same network evaluated by the solve kernel's wave-level MFMA path */
+hhg_status hhg_mlp_forward_mfma(hhg_mlp* mlp, const float* x, int64_t n, float* y);
+/* gradient of the mean loss 0.5 |y_n - t_n|^2 over n samples w.r.t. the FP32 parameters */
+hhg_status hhg_mlp_gradients(hhg_mlp* mlp, const float* x, const float* t, int64_t n, float* grad, float* loss);
+/* Adam on shuffled batches; epoch_loss[epochs] receives the mean batch loss per epoch (may be NULL) */
+hhg_status hhg_mlp_train(hhg_mlp* mlp, const float* x, const float* t, int64_t n, int32_t epochs, int32_t batch,
+                         float lr, uint64_t seed, float* epoch_loss);
 
 /* ---- exact accumulators (shard reduction) ---- */
 
