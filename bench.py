@@ -8,6 +8,15 @@ Workload (one "step" = one full solve of the configuration):
   dephasing; 4096 sampling intervals; FP64 RKF78 at rtol = atol = 1e-10 with
   the reference's controller restart at Delta/50 every interval.
 
+Two legs are timed on the same workload: the plain controller (restart at
+Delta/50 every interval, as the reference) and the learned proposal (a BF16
+network, trained on the GPU from step logs of a carried-controller solve on a
+coarser grid, proposes each interval's first step; the embedded-error
+controller still judges every step).  Both are checked against a
+tight-tolerance solve (rtol = atol = 1e-13, first step Delta/500 every
+interval) of the same grid; ``value`` is the learned leg's trajectories/s
+when its j(t) error is within 1e-9, else the plain leg's.
+
 Prints one JSON line: trajectories/s, the FP64 roofline of the solve kernel
 (algorithmic FLOPs from the attempted-step counter over the HIP-event time of
 the kernel on its own stream), and the CPU oracle timed on a bounded sample
@@ -36,11 +45,16 @@ F_STEP = 450.0
 F_RHS = {"dirac_t2": 47.0, "dirac": 23.0, "piflux": 26.0, "honeycomb": 34.0}
 F_VERTEX = {"dirac": 3.0, "piflux": 4.0, "honeycomb": 20.0}
 F_MAGNUS = 85.0
+# One network inference per lane and interval: 2 x (8*64 + 2*64*64 + 64*2) FLOPs
+# (BF16 MFMA hidden layers, FP32 output layer), not counted as FP64 work.
+F_MLP = 2.0 * (8 * 64 + 2 * 64 * 64 + 64 * 2)
 
 # HBM bytes per solve launch of this workload from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (28,239.1 KiB x 2.0 + 1,050,496 KiB x 1.0, the factors a plain 1 GiB
 # copy reads); profiles/solve_kernel_dirac_t2_1024x512_counters.md.
 TRAFFIC_BYTES = (28239.1 * 2.0 + 1050496.0 * 1.0) * 1024
+# The learned-proposal solve kernel has not had its own counter runs yet.
+TRAFFIC_BYTES_LEARNED = None
 
 
 def workload(small: bool):
@@ -50,15 +64,38 @@ def workload(small: bool):
                 rtol=1e-10, atol=1e-10)
 
 
-def make_plan(H, wl, method=0, relax=2, proposal=0, magnus_substeps=5, **launch):
+def make_plan(H, wl, method=0, relax=2, proposal=0, magnus_substeps=5, mlp=None, rtol=None, infer_every=1, n_sub=None,
+              **launch):
     T = wl["T"]
     md = H.ModelDesc(H.MODEL_DIRAC2D, relax, wl["v"], wl["mass"], 0.0, 0.0, 0.0, 0.0, wl["T2"] if relax else 0.0)
     laser = H.CosineLaser(wl["A0"], wl["n_cycles"], -T / 2)
-    tc = H.TimeIntegrationConfig(-T / 2, T / 2, wl["N"], wl["n_sub"])
+    tc = H.TimeIntegrationConfig(-T / 2, T / 2, wl["N"], n_sub or wl["n_sub"])
     grid = H.KGrid.box(wl["nx"], wl["ny"], -3.0, 3.0, -3.0, 3.0)
-    solver = H.Solver(method=method, rtol=wl["rtol"], atol=wl["atol"], proposal=proposal,
-                      magnus_substeps=magnus_substeps)
+    tol = rtol or wl["rtol"]
+    solver = H.Solver(method=method, rtol=tol, atol=tol if rtol else wl["atol"], proposal=proposal,
+                      magnus_substeps=magnus_substeps, mlp=mlp, infer_every=infer_every)
     return H.Plan([H.Job(md, laser, tc, 0)], [grid], solver, **launch)
+
+
+def train_proposal(H, wl):
+    """The step-proposal network trained on accepted-step logs of a
+    carried-controller solve of the same configuration on a 256 x 128 grid of
+    the same box (every 64th interval of every k-point, about 2.1M records)."""
+    from higherharmonics_amd import learn
+    coarse = dict(wl, nx=256, ny=128)
+    t0 = time.perf_counter()
+    rec = learn.collect_step_log(lambda solver: make_plan(H, coarse, proposal=solver.proposal, persistent=1),
+                                 capacity=1 << 21, stride=64)
+    t1 = time.perf_counter()
+    net, losses = learn.fit_proposal(rec, epochs=6, batch=1024)
+    t2 = time.perf_counter()
+    return net, {"records": int(len(rec)), "log_s": t1 - t0, "train_s": t2 - t1, "epochs": len(losses),
+                 "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "params": net.n_params,
+                 "log_source": "carried-controller solve, 256x128 grid of the same box, stride 64"}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
 
 
 def time_plan(plan, steps, warmup):
@@ -144,6 +181,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--extras", action="store_true", help="also time Magnus-4 and the carry ablation")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--infer-every", type=int, default=1,
+                    help="learned leg: network proposal every this many sampling intervals")
     ap.add_argument("--sweeps", action="store_true", help="also time reduced configurations 3-5")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
@@ -166,24 +205,64 @@ def main():
     import higherharmonics_amd as H
     wl = workload(args.small)
     K = wl["nx"] * wl["ny"]
-    print(f"[bench] Dirac-2D {wl['nx']}x{wl['ny']} N={wl['N']} RKF78 1e-10 restart, T2 dephasing", file=sys.stderr,
-          flush=True)
+    f_traj_rhs = F_STEP + 13 * F_RHS["dirac_t2"]
+    vertex = K * (wl["N"] + 1) * F_VERTEX["dirac"]
+
+    print(f"[bench] Dirac-2D {wl['nx']}x{wl['ny']} N={wl['N']} RKF78 1e-10 restart (plain), T2 dephasing",
+          file=sys.stderr, flush=True)
     plan = make_plan(H, wl, persistent=1)
     tuned = plan.autotune() if args.autotune else None
-    wall, solve_ms, stats, j = time_plan(plan, args.steps, args.warmup)
+    wall_p, ms_p, st_p, j_plain = time_plan(plan, args.steps, args.warmup)
     plan.close()
-    value = args.steps * K / wall
-    attempts_per_step = stats["attempts"]
-    flops = attempts_per_step * (F_STEP + 13 * F_RHS["dirac_t2"]) + K * (wl["N"] + 1) * F_VERTEX["dirac"]
+    flops_p = st_p["attempts"] * f_traj_rhs + vertex
+
+    print("[bench] step logs + proposal network training", file=sys.stderr, flush=True)
+    net, train = train_proposal(H, wl)
+    print(f"[bench] learned proposal solve ({train['records']} records, loss {train['loss_first']:.3g} -> "
+          f"{train['loss_last']:.3g})", file=sys.stderr, flush=True)
+    plan = make_plan(H, wl, proposal=H.PROPOSAL_LEARNED, mlp=net, infer_every=args.infer_every, persistent=1)
+    wall_l, ms_l, st_l, j_learned = time_plan(plan, args.steps, args.warmup)
+    plan.close()
+    flops_l = st_l["attempts"] * f_traj_rhs + vertex
+    mlp_flops = K * ((wl["N"] + args.infer_every - 1) // args.infer_every) * F_MLP
+    infer_sweep = {}
+    if args.extras:
+        for m in (1, 2, 4, 8, 16):
+            if m == args.infer_every:
+                continue
+            p = make_plan(H, wl, proposal=H.PROPOSAL_LEARNED, mlp=net, infer_every=m, persistent=1)
+            w2, sm, st, jm = time_plan(p, max(1, args.steps // 2), 1)
+            p.close()
+            infer_sweep[m] = {"trajectories_per_s": max(1, args.steps // 2) * K / w2, "kernel_ms": sm,
+                              "attempts": st["attempts"], "j": jm}
+
+    # the reference: rtol = atol = 1e-13 and a first step of Delta/500 every
+    # interval, a step sequence of its own (at 1e-13 the Delta/50 restart
+    # sequence would repeat the plain leg's steps exactly on this workload)
+    print("[bench] tight-tolerance reference solve (rtol = atol = 1e-13, restart at Delta/500)", file=sys.stderr,
+          flush=True)
+    plan = make_plan(H, wl, rtol=1e-13, n_sub=500, persistent=1)
+    j_tight = plan.execute()[0]
+    plan.close()
+    err_p, err_l = rel_l2(j_plain, j_tight), rel_l2(j_learned, j_tight)
+    for m, r in infer_sweep.items():
+        r["rel_l2_err_vs_tight"] = rel_l2(r.pop("j"), j_tight)
+    net.close()
+
+    v_plain = args.steps * K / wall_p
+    v_learned = args.steps * K / wall_l
+    use_learned = err_l <= 1e-9
+    value = v_learned if use_learned else v_plain
+    solve_ms, flops = (ms_l, flops_l) if use_learned else (ms_p, flops_p)
     achieved = flops / (solve_ms * 1e-3) / 1e12
     out = {
-        "metric": "k-point trajectories/s at <=1e-9 rel j(t) err (plain GPU RKF78, reference restart controller)",
+        "metric": "k-point trajectories/s at <=1e-9 rel j(t) err; learned-proposal vs plain RKF78",
         "value": value,
         "unit": "trajectories/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": 1e3 * (wall_l if use_learned else wall_p) / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -191,11 +270,26 @@ def main():
         "data": "synthetic (analytic pulse, Gauss-Legendre k-grid)",
         "config": {"workload": f"Dirac-2D {wl['nx']}x{wl['ny']} GL, cos^2 A0=0.5 8 cycles, T2=10 cycles, "
                                f"4096 samples, RKF78 rtol=atol=1e-10", "trajectories": K,
-                   "attempted_steps": attempts_per_step, "rejects": stats["rejects"],
+                   "value_leg": "learned" if use_learned else "plain (learned leg missed 1e-9)",
                    "launch": tuned},
+        "legs": {
+            "plain": {"trajectories_per_s": v_plain, "kernel_ms": ms_p, "attempts": st_p["attempts"],
+                      "rejects": st_p["rejects"], "rel_l2_err_vs_tight": err_p,
+                      "fp64_tflops": flops_p / (ms_p * 1e-3) / 1e12},
+            "learned": {"trajectories_per_s": v_learned, "kernel_ms": ms_l, "attempts": st_l["attempts"],
+                        "rejects": st_l["rejects"], "fallbacks": st_l["proposal_fallbacks"],
+                        "rel_l2_err_vs_tight": err_l, "fp64_tflops": flops_l / (ms_l * 1e-3) / 1e12,
+                        "mlp_bf16_tflops": mlp_flops / (ms_l * 1e-3) / 1e12, "infer_every": args.infer_every,
+                        "training": train},
+            "learned_infer_every_sweep": infer_sweep or None,
+            "learned_vs_plain": v_learned / v_plain,
+            "tight_reference": "same kernel, restart controller at Delta/500, rtol=atol=1e-13",
+        },
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES if not args.small else None,
-                     "kernel_ms": solve_ms, "algorithmic_flop_per_launch": flops},
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": (TRAFFIC_BYTES_LEARNED if use_learned else TRAFFIC_BYTES) if not args.small else None,
+                     "kernel_ms": solve_ms, "algorithmic_flop_per_launch": flops,
+                     "kernel": "learned-proposal solve" if use_learned else "plain solve"},
     }
     if args.extras:
         extras = {}

@@ -8,8 +8,10 @@
 //                    0.5 |y - t|^2 averaged over the batch, backward pass with
 //                    the straight-through derivative of the BF16 rounding;
 //                    activations and deltas of every sample go to a work array
-//   grad_kernel      one thread per parameter: the batch sum of delta x
-//                    activation in sample order (deterministic, no atomics)
+//   grad_partial_kernel / grad_reduce_kernel
+//                    the batch sum of delta x activation per parameter, in
+//                    fixed 64-sample segments then in segment order
+//                    (deterministic, no atomics)
 //   adam_kernel      Adam update of the FP32 masters
 //   refresh_kernel   BF16 roundings (FP32 copy) and the MFMA operand packing
 #include <hip/hip_runtime.h>
@@ -40,19 +42,24 @@ __global__ void refresh_kernel(const float* __restrict__ params, float* __restri
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < kMlpParams) params_bf[idx] = bf16r(params[idx]);
     if (idx < kPackTotal) {
-        int layer, off;
-        if (idx < kPackL1) layer = 1, off = idx;
-        else if (idx < kPackL1 + kPackL23) layer = 2, off = idx - kPackL1;
-        else layer = 3, off = idx - kPackL1 - kPackL23;
-        const int nks = layer == 1 ? 1 : 4;
-        const int tile = off / kPackTile, rem = off % kPackTile;
-        const int mt = tile / nks, ks = tile % nks;
+        const int tile = idx / kPackTile, rem = idx % kPackTile;
         const int lane = rem / 8, i = rem % 8;
-        const int row = 32 * mt + (lane & 31);
-        const int k = 16 * ks + 8 * (lane >> 5) + i;
+        const int m = lane & 31, h = lane >> 5;
         float w = 0.0f;
-        if (layer == 1) w = k < kMlpIn ? params[kOffW1 + row * kMlpIn + k] : 0.0f;
-        else w = params[(layer == 2 ? kOffW2 : kOffW3) + row * kMlpHidden + k];
+        if (tile < kTileL2) {
+            // layer 1 [nt][mt]: features in lane half nt, bias at k = 0 of the other half
+            const int nt = tile / 2, row = 32 * (tile % 2) + m;
+            w = h == nt ? params[kOffW1 + row * kMlpIn + i] : (i == 0 ? params[kOffB1 + row] : 0.0f);
+        } else if (tile < kTileL4) {
+            const int t = (tile - kTileL2) % 8, mt = t / 4, ks = t % 4;
+            const int off = tile < kTileL3 ? kOffW2 : kOffW3;
+            w = params[off + (32 * mt + m) * kMlpHidden + hidden_unit(ks, h, i)];
+        } else {
+            // output layer [nt][ks]: outputs on rows 4 nt and 4 nt + 1
+            const int nt = (tile - kTileL4) / 4, ks = (tile - kTileL4) % 4;
+            const int o = m - 4 * nt;
+            if (o == 0 || o == 1) w = params[kOffW4 + o * kMlpHidden + hidden_unit(ks, h, i)];
+        }
         pack[idx] = (__bf16)w;
     }
 }
@@ -126,31 +133,45 @@ __global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ 
     }
 }
 
-// thread per parameter, batch sum in sample order
-__global__ void grad_kernel(const float* __restrict__ work, int n, float* __restrict__ grad,
-                            float* __restrict__ loss_out) {
+constexpr int kGradSeg = 64;  // samples per partial sum of the gradient
+
+// thread per (parameter, segment of kGradSeg samples): the segment's sum in
+// sample order; index kMlpParams carries the loss
+__global__ void grad_partial_kernel(const float* __restrict__ work, int n, float* __restrict__ partial) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kMlpParams) return;
+    const int b0 = blockIdx.y * kGradSeg, b1 = min(n, b0 + kGradSeg);
+    float acc = 0.0f;
+    if (p == kMlpParams) {
+        for (int b = b0; b < b1; ++b) acc += work[(size_t)b * kRec + kLoss];
+    } else {
+        int dofs, aofs = -1;
+        if (p < kOffB1) dofs = kD1 + (p - kOffW1) / kMlpIn, aofs = kA0 + (p - kOffW1) % kMlpIn;
+        else if (p < kOffW2) dofs = kD1 + (p - kOffB1);
+        else if (p < kOffB2) dofs = kD2 + (p - kOffW2) / kMlpHidden, aofs = kA1 + (p - kOffW2) % kMlpHidden;
+        else if (p < kOffW3) dofs = kD2 + (p - kOffB2);
+        else if (p < kOffB3) dofs = kD3 + (p - kOffW3) / kMlpHidden, aofs = kA2 + (p - kOffW3) % kMlpHidden;
+        else if (p < kOffW4) dofs = kD3 + (p - kOffB3);
+        else if (p < kOffB4) dofs = kDY + (p - kOffW4) / kMlpHidden, aofs = kA3 + (p - kOffW4) % kMlpHidden;
+        else dofs = kDY + (p - kOffB4);
+        for (int b = b0; b < b1; ++b) {
+            const float* R = work + (size_t)b * kRec;
+            acc = aofs >= 0 ? fmaf(R[dofs], R[aofs], acc) : acc + R[dofs];
+        }
+    }
+    partial[(size_t)blockIdx.y * (kMlpParams + 1) + p] = acc;
+}
+
+// thread per parameter: the segment sums in segment order (deterministic for
+// a given batch size); the loss goes to loss_out as the batch mean
+__global__ void grad_reduce_kernel(const float* __restrict__ partial, int n_seg, int n, float* __restrict__ grad,
+                                   float* __restrict__ loss_out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > kMlpParams) return;
     float acc = 0.0f;
-    if (p == kMlpParams) {
-        for (int b = 0; b < n; ++b) acc += work[(size_t)b * kRec + kLoss];
-        *loss_out = acc / (float)n;
-        return;
-    }
-    int dofs, aofs = -1;
-    if (p < kOffB1) dofs = kD1 + (p - kOffW1) / kMlpIn, aofs = kA0 + (p - kOffW1) % kMlpIn;
-    else if (p < kOffW2) dofs = kD1 + (p - kOffB1);
-    else if (p < kOffB2) dofs = kD2 + (p - kOffW2) / kMlpHidden, aofs = kA1 + (p - kOffW2) % kMlpHidden;
-    else if (p < kOffW3) dofs = kD2 + (p - kOffB2);
-    else if (p < kOffB3) dofs = kD3 + (p - kOffW3) / kMlpHidden, aofs = kA2 + (p - kOffW3) % kMlpHidden;
-    else if (p < kOffW4) dofs = kD3 + (p - kOffB3);
-    else if (p < kOffB4) dofs = kDY + (p - kOffW4) / kMlpHidden, aofs = kA3 + (p - kOffW4) % kMlpHidden;
-    else dofs = kDY + (p - kOffB4);
-    for (int b = 0; b < n; ++b) {
-        const float* R = work + (size_t)b * kRec;
-        acc = aofs >= 0 ? fmaf(R[dofs], R[aofs], acc) : acc + R[dofs];
-    }
-    grad[p] = acc;
+    for (int q = 0; q < n_seg; ++q) acc += partial[(size_t)q * (kMlpParams + 1) + p];
+    if (p == kMlpParams) *loss_out = acc / (float)n;
+    else grad[p] = acc;
 }
 
 __global__ void adam_kernel(float* __restrict__ params, const float* __restrict__ grad, float* __restrict__ m,
@@ -163,17 +184,19 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
     params[p] -= lr * (m[p] * c1) / (sqrtf(v[p] * c2) + eps);
 }
 
-extern __shared__ __bf16 mfma_fwd_lds[];
+extern __shared__ __align__(16) unsigned char mfma_fwd_lds[];
 
 // The solve kernel's wave-level MFMA inference on a feature array: one wave
 // per 64 samples (padding lanes take zero features and write nothing).
 __global__ __launch_bounds__(256) void mfma_forward_kernel(const MlpDev* M, const float* __restrict__ x, int64_t n,
                                                            float* __restrict__ y) {
+    mlp_stage_lds(M, mfma_fwd_lds);
+    __syncthreads();
     const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
     const int64_t s = ((int64_t)blockIdx.x * 4 + w) * 64 + lane;
     float feat[kMlpIn], out[kMlpOut];
     for (int i = 0; i < kMlpIn; ++i) feat[i] = s < n ? x[s * kMlpIn + i] : 0.0f;
-    mlp_infer_wave(M, feat, out, mfma_fwd_lds + w * kMlpLdsPerWave);
+    mlp_infer_wave(M, mfma_fwd_lds, feat, out);
     if (s < n)
         for (int o = 0; o < kMlpOut; ++o) y[s * kMlpOut + o] = out[o];
 }
@@ -191,7 +214,9 @@ struct hhg_mlp {
     float* d_loss = nullptr;
     int64_t adam_t = 0;
     // training scratch
-    float *d_x = nullptr, *d_t = nullptr, *d_work = nullptr, *d_y = nullptr;
+    float *d_x = nullptr, *d_t = nullptr, *d_work = nullptr, *d_y = nullptr, *d_partial = nullptr;
+    float* d_loss_hist = nullptr;  // per-step losses of one epoch
+    int64_t cap_hist = 0;
     int* d_idx = nullptr;
     int64_t cap_data = 0, cap_batch = 0;
 };
@@ -242,21 +267,32 @@ hhg_status ensure_data(hhg_mlp* m, int64_t n) {
 hhg_status ensure_batch(hhg_mlp* m, int64_t n) {
     if (n <= m->cap_batch) return HHG_SUCCESS;
     hipFree(m->d_work);
+    hipFree(m->d_partial);
     m->d_work = nullptr;
+    m->d_partial = nullptr;
+    m->cap_batch = 0;
     MLP_TRY(hipMalloc(&m->d_work, sizeof(float) * hhg::kRec * n), "work");
+    const int64_t n_seg = (n + hhg::kGradSeg - 1) / hhg::kGradSeg;
+    MLP_TRY(hipMalloc(&m->d_partial, sizeof(float) * (hhg::kMlpParams + 1) * n_seg), "gradient partial sums");
     m->cap_batch = n;
     return HHG_SUCCESS;
 }
 
 // gradient of the mean loss over samples idx[0..n) (or 0..n) of the uploaded data
-hhg_status gradient_step(hhg_mlp* m, const int* d_idx, int n) {
+// One forward/backward pass over a batch: gradient to d_grad, mean loss to *loss_dst.
+hhg_status gradient_step(hhg_mlp* m, const int* d_idx, int n, float* loss_dst) {
     const hhg_status eb = ensure_batch(m, n);
     if (eb != HHG_SUCCESS) return eb;
     hipLaunchKernelGGL(hhg::fwd_bwd_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_params_bf, m->d_x,
                        m->d_t, d_idx, n, m->host_dev, 1.0f / (float)n, m->d_work, nullptr);
     MLP_TRY(hipGetLastError(), "forward/backward");
-    hipLaunchKernelGGL(hhg::grad_kernel, dim3((hhg::kMlpParams + 1 + 255) / 256), dim3(256), 0, m->stream, m->d_work, n,
-                       m->d_grad, m->d_loss);
+    const int n_seg = (n + hhg::kGradSeg - 1) / hhg::kGradSeg;
+    const int blocks = (hhg::kMlpParams + 1 + 255) / 256;
+    hipLaunchKernelGGL(hhg::grad_partial_kernel, dim3(blocks, n_seg), dim3(256), 0, m->stream, m->d_work, n,
+                       m->d_partial);
+    MLP_TRY(hipGetLastError(), "gradient partial sums");
+    hipLaunchKernelGGL(hhg::grad_reduce_kernel, dim3(blocks), dim3(256), 0, m->stream, m->d_partial, n_seg, n,
+                       m->d_grad, loss_dst);
     MLP_TRY(hipGetLastError(), "gradient");
     return HHG_SUCCESS;
 }
@@ -334,7 +370,7 @@ hhg_status hhg_mlp_destroy(hhg_mlp* m) {
     hipSetDevice(m->device);
     for (void* p : {(void*)m->d_params, (void*)m->d_params_bf, (void*)m->d_grad, (void*)m->d_m, (void*)m->d_v,
                     (void*)m->d_pack, (void*)m->d_dev, (void*)m->d_loss, (void*)m->d_x, (void*)m->d_t,
-                    (void*)m->d_work, (void*)m->d_y, (void*)m->d_idx})
+                    (void*)m->d_work, (void*)m->d_y, (void*)m->d_idx, (void*)m->d_partial, (void*)m->d_loss_hist})
         hipFree(p);
     if (m->stream) hipStreamDestroy(m->stream);
     delete m;
@@ -404,7 +440,7 @@ hhg_status hhg_mlp_forward_mfma(hhg_mlp* m, const float* x, int64_t n, float* y)
     if (st != HHG_SUCCESS) return st;
     MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
     const int blocks = (int)((n + 255) / 256);
-    const size_t smem = 4 * hhg::kMlpLdsPerWave * sizeof(__bf16);
+    const size_t smem = hhg::kMlpLdsBytes;
     hipLaunchKernelGGL(hhg::mfma_forward_kernel, dim3(blocks), dim3(256), smem, m->stream, m->d_dev, m->d_x, n, m->d_y);
     MLP_TRY(hipGetLastError(), "mfma forward");
     MLP_TRY(hipStreamSynchronize(m->stream), "mfma forward sync");
@@ -421,7 +457,7 @@ hhg_status hhg_mlp_gradients(hhg_mlp* m, const float* x, const float* t, int64_t
     if (st != HHG_SUCCESS) return st;
     MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
     MLP_TRY(hipMemcpy(m->d_t, t, sizeof(float) * hhg::kMlpOut * n, hipMemcpyHostToDevice), "upload t");
-    if ((st = gradient_step(m, nullptr, (int)n)) != HHG_SUCCESS) return st;
+    if ((st = gradient_step(m, nullptr, (int)n, m->d_loss)) != HHG_SUCCESS) return st;
     MLP_TRY(hipStreamSynchronize(m->stream), "gradient sync");
     MLP_TRY(hipMemcpy(grad, m->d_grad, sizeof(float) * hhg::kMlpParams, hipMemcpyDeviceToHost), "download");
     if (loss) MLP_TRY(hipMemcpy(loss, m->d_loss, sizeof(float), hipMemcpyDeviceToHost), "download loss");
@@ -439,35 +475,44 @@ hhg_status hhg_mlp_train(hhg_mlp* m, const float* x, const float* t, int64_t n, 
     if (st != HHG_SUCCESS) return st;
     MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
     MLP_TRY(hipMemcpy(m->d_t, t, sizeof(float) * hhg::kMlpOut * n, hipMemcpyHostToDevice), "upload t");
+    const int64_t steps_per_epoch = (n + batch - 1) / batch;
+    if (steps_per_epoch > m->cap_hist) {
+        hipFree(m->d_loss_hist);
+        m->d_loss_hist = nullptr;
+        m->cap_hist = 0;
+        MLP_TRY(hipMalloc(&m->d_loss_hist, sizeof(float) * steps_per_epoch), "loss history");
+        m->cap_hist = steps_per_epoch;
+    }
     std::vector<int> order(n);
+    std::vector<float> hist(steps_per_epoch);
     uint64_t s = seed;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const int nr = std::max(hhg::kMlpParams, hhg::kPackTotal);
     for (int e = 0; e < epochs; ++e) {
         std::iota(order.begin(), order.end(), 0);
         for (int64_t i = n - 1; i > 0; --i) std::swap(order[i], order[splitmix(s) % (uint64_t)(i + 1)]);
         MLP_TRY(hipMemcpyAsync(m->d_idx, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, m->stream), "order");
-        double acc = 0.0;
-        int steps = 0;
-        for (int64_t b0 = 0; b0 < n; b0 += batch) {
+        int64_t step = 0;
+        for (int64_t b0 = 0; b0 < n; b0 += batch, ++step) {
             const int nb = (int)std::min<int64_t>(batch, n - b0);
-            if ((st = gradient_step(m, m->d_idx + b0, nb)) != HHG_SUCCESS) return st;
+            if ((st = gradient_step(m, m->d_idx + b0, nb, m->d_loss_hist + step)) != HHG_SUCCESS) return st;
             ++m->adam_t;
             const float c1 = 1.0f / (1.0f - std::pow(b1, (float)m->adam_t));
             const float c2 = 1.0f / (1.0f - std::pow(b2, (float)m->adam_t));
             hipLaunchKernelGGL(hhg::adam_kernel, dim3((hhg::kMlpParams + 255) / 256), dim3(256), 0, m->stream,
                                m->d_params, m->d_grad, m->d_m, m->d_v, lr, b1, b2, eps, c1, c2);
             MLP_TRY(hipGetLastError(), "adam");
-            const int nr = std::max(hhg::kMlpParams, hhg::kPackTotal);
             hipLaunchKernelGGL(hhg::refresh_kernel, dim3((nr + 255) / 256), dim3(256), 0, m->stream, m->d_params,
                                m->d_params_bf, m->d_pack);
             MLP_TRY(hipGetLastError(), "refresh");
-            float l = 0.0f;
-            MLP_TRY(hipMemcpyAsync(&l, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, m->stream), "loss");
-            MLP_TRY(hipStreamSynchronize(m->stream), "train sync");
-            acc += l;
-            ++steps;
         }
-        if (epoch_loss) epoch_loss[e] = (float)(acc / std::max(1, steps));
+        // one sync per epoch; it also ends the upload of the host order vector before its reuse
+        MLP_TRY(hipMemcpyAsync(hist.data(), m->d_loss_hist, sizeof(float) * steps_per_epoch, hipMemcpyDeviceToHost,
+                               m->stream), "loss history");
+        MLP_TRY(hipStreamSynchronize(m->stream), "train sync");
+        double acc = 0.0;
+        for (int64_t q = 0; q < steps_per_epoch; ++q) acc += hist[q];
+        if (epoch_loss) epoch_loss[e] = (float)(acc / (double)steps_per_epoch);
     }
     return refresh(m);
 }

@@ -31,91 +31,136 @@ constexpr int kOffW4 = kOffB3 + kMlpHidden;
 constexpr int kOffB4 = kOffW4 + kMlpOut * kMlpHidden;
 constexpr int kMlpParams = kOffB4 + kMlpOut;  // 9026
 
-// MFMA operand packing of the hidden layers for v_mfma_f32_32x32x16_bf16:
-// W (out x in) is the A operand; lane l of k-step ks in row tile mt holds
-// W[32 mt + l % 32][16 ks + 8 (l / 32) + i], i = 0..7.
-constexpr int kPackTile = 64 * 8;                 // bf16 per (mt, ks)
-constexpr int kPackL1 = 2 * 1 * kPackTile;        // layer 1: K = 8 padded to one k-step of 16
-constexpr int kPackL23 = 2 * 4 * kPackTile;       // layers 2, 3: K = 64
-constexpr int kPackTotal = kPackL1 + 2 * kPackL23;
+// MFMA operand packing for v_mfma_f32_32x32x16_bf16 (A = weights, out x in;
+// B = activations, in x samples; C = out x samples).  Operand layouts of the
+// instruction: A lane l holds row l % 32, k = 8 (l / 32) + i in element i;
+// B lane l holds column l % 32 with the same k; C lane l register j holds
+// row (j / 4) * 8 + 4 (l / 32) + j % 4, column l % 32.
+//
+// Activations never leave registers.  A wave's 64 samples form two column
+// tiles nt = 0, 1 (nt = 1: lanes 32..63).  The C registers of a hidden layer
+// become the next layer's B operand unchanged, element i of k-step ks being
+// c[8 (ks % 2) + i] of row tile ks / 2; the next layer's weight columns are
+// stored in that order (hidden_unit).  Layer 1 folds its bias into the
+// padding half of K (a constant-one input) and takes the features from the
+// lanes of its own column tile; the output layer writes column tile nt's
+// outputs to rows 4 nt, 4 nt + 1, so every lane finds its own sample's outputs
+// in c[0], c[1] of tile l / 32.
+//
+// One A tile = 64 lanes x 8 bf16.  Tiles: layer 1 [nt][mt], layers 2 and 3
+// [mt][ks], output layer [nt][ks].
+constexpr int kPackTile = 64 * 8;
+constexpr int kTileL1 = 0, kTileL2 = 4, kTileL3 = 12, kTileL4 = 20, kPackTiles = 28;
+constexpr int kPackTotal = kPackTiles * kPackTile;  // bf16 elements
+
+// input unit of element i of k-step ks in lane half h (hidden-layer B operand)
+__host__ __device__ constexpr int hidden_unit(int ks, int h, int i) { return 16 * ks + 8 * (i / 4) + 4 * h + (i % 4); }
 
 struct MlpDev {
-    const __bf16* pack;  // kPackTotal bf16: layer 1, layer 2, layer 3
-    const float* params; // FP32 copy of the BF16-rounded parameters (biases and W4 read here)
+    const __bf16* pack;  // kPackTotal bf16
+    const float* params; // FP32 copy of the BF16-rounded parameters
     float in_mean[kMlpIn];
     float in_istd[kMlpIn];
     float out_mean[kMlpOut];
     float out_std[kMlpOut];
 };
 
-// LDS per wave for inference: two activation buffers of 64 samples x 64 units.
-constexpr int kMlpLdsPerWave = 2 * 64 * 64;  // bf16 elements
+// Block-shared LDS of the inference: the packed weights, then the layer 2
+// and 3 biases (FP32).
+constexpr int kMlpLdsBytes = kPackTotal * 2 + 2 * kMlpHidden * 4;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Every thread of the block calls this, then the block synchronises.
+__device__ __forceinline__ void mlp_stage_lds(const MlpDev* __restrict__ M, unsigned char* lds) {
+    const uint4* src = reinterpret_cast<const uint4*>(M->pack);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (int q = threadIdx.x; q < kPackTotal / 8; q += blockDim.x) dst[q] = src[q];
+    float* bias = reinterpret_cast<float*>(lds + kPackTotal * 2);
+    for (int q = threadIdx.x; q < 2 * kMlpHidden; q += blockDim.x)
+        bias[q] = M->params[q < kMlpHidden ? kOffB2 + q : kOffB3 + q - kMlpHidden];
+}
+
+__device__ __forceinline__ bf16x8 relu_bf16(const floatx16& c, int g) {
+    bf16x8 r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float v = c[8 * g + i];
+        r[i] = (__bf16)(v > 0.0f ? v : 0.0f);
+    }
+    return r;
+}
 
 // Inference for the 64 lanes of a wave (batch = the wave's k-points).  Every
 // lane of the wave must call it.  feat: this lane's raw features; out: this
-// lane's de-normalised outputs.  lds: this wave's kMlpLdsPerWave bf16.
-__device__ __forceinline__ void mlp_infer_wave(const MlpDev* __restrict__ M, const float feat[kMlpIn],
-                                               float out[kMlpOut], __bf16* lds) {
+// lane's de-normalised outputs.  lds: the block's staged weights.
+__device__ __forceinline__ void mlp_infer_wave(const MlpDev* __restrict__ M, const unsigned char* lds,
+                                               const float feat[kMlpIn], float out[kMlpOut]) {
     const int l = threadIdx.x & 63;
-    const int half = l >> 5, col = l & 31;
-    __bf16* buf0 = lds;
-    __bf16* buf1 = lds + 64 * 64;
-    // normalised features -> buf0[sample l][0..7], zeros in [8..15]
+    const int h = l >> 5;
+    const bf16x8* A = reinterpret_cast<const bf16x8*>(lds) + l;  // tile t: A[64 t]
+    const float* bias = reinterpret_cast<const float*>(lds + kPackTotal * 2);
+    bf16x8 x, one = {};
 #pragma unroll
-    for (int i = 0; i < kMlpIn; ++i) buf0[l * 64 + i] = (__bf16)((feat[i] - M->in_mean[i]) * M->in_istd[i]);
-#pragma unroll
-    for (int i = kMlpIn; i < 16; ++i) buf0[l * 64 + i] = (__bf16)0.0f;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    for (int i = 0; i < kMlpIn; ++i) x[i] = (__bf16)((feat[i] - M->in_mean[i]) * M->in_istd[i]);
+    one[0] = (__bf16)1.0f;
+    const bf16x8 b1[2] = {h == 0 ? x : one, h == 1 ? x : one};
 
-    const bf16x8* pk = reinterpret_cast<const bf16x8*>(M->pack);
-    const float* P = M->params;
-    // one hidden layer: in -> out buffer, K = 16 * nks
-    auto layer = [&](const bf16x8* wp, int nks, const float* bias, const __bf16* in, __bf16* outb) {
+    bf16x8 act[2][4];  // [nt][ks]
+    const floatx16 zero = {};
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
+    for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
-            for (int nt = 0; nt < 2; ++nt) {
-                floatx16 c = {};
-                for (int ks = 0; ks < nks; ++ks) {
-                    const bf16x8 a = wp[(mt * nks + ks) * 64 + l];
-                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(in + (32 * nt + col) * 64 + 16 * ks + 8 * half);
-                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-                }
-                const int sample = 32 * nt + col;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int unit = 32 * mt + (j >> 2) * 8 + half * 4 + (j & 3);
-                    const float v = c[j] + bias[unit];
-                    outb[sample * 64 + unit] = (__bf16)(v > 0.0f ? v : 0.0f);
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-    };
-    layer(pk, 1, P + kOffB1, buf0, buf1);
-    layer(pk + kPackL1 / 8, 4, P + kOffB2, buf1, buf0);
-    layer(pk + (kPackL1 + kPackL23) / 8, 4, P + kOffB3, buf0, buf1);
-    // output layer on the VALU: this lane's own sample row
-    float acc0 = P[kOffB4], acc1 = P[kOffB4 + 1];
-    const bf16x8* row = reinterpret_cast<const bf16x8*>(buf1 + l * 64);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const bf16x8 v = row[q];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const float x = (float)v[i];
-            acc0 = fmaf(P[kOffW4 + 8 * q + i], x, acc0);
-            acc1 = fmaf(P[kOffW4 + kMlpHidden + 8 * q + i], x, acc1);
+        for (int nt = 0; nt < 2; ++nt) {
+            const floatx16 c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[64 * (kTileL1 + 2 * nt + mt)], b1[nt], zero,
+                                                                       0, 0, 0);
+            act[nt][2 * mt] = relu_bf16(c, 0);
+            act[nt][2 * mt + 1] = relu_bf16(c, 1);
         }
     }
-    out[0] = acc0 * M->out_std[0] + M->out_mean[0];
-    out[1] = acc1 * M->out_std[1] + M->out_mean[1];
-    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+        const int t0 = layer == 0 ? kTileL2 : kTileL3;
+        const float* bl = bias + layer * kMlpHidden;
+        bf16x8 nxt[2][4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            floatx16 c0, c1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const floatx4 b = *reinterpret_cast<const floatx4*>(bl + 32 * mt + 8 * q + 4 * h);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) c0[4 * q + r] = c1[4 * q + r] = b[r];
+            }
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const bf16x8 a = A[64 * (t0 + 4 * mt + ks)];
+                c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][ks], c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][ks], c1, 0, 0, 0);
+            }
+            nxt[0][2 * mt] = relu_bf16(c0, 0);
+            nxt[0][2 * mt + 1] = relu_bf16(c0, 1);
+            nxt[1][2 * mt] = relu_bf16(c1, 0);
+            nxt[1][2 * mt + 1] = relu_bf16(c1, 1);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) act[nt][ks] = nxt[nt][ks];
+    }
+    floatx16 o0 = {}, o1 = {};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        o0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[64 * (kTileL4 + ks)], act[0][ks], o0, 0, 0, 0);
+        o1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[64 * (kTileL4 + 4 + ks)], act[1][ks], o1, 0, 0, 0);
+    }
+    const float* P = M->params;
+    const float y0 = (h ? o1[0] : o0[0]) + P[kOffB4];
+    const float y1 = (h ? o1[1] : o0[1]) + P[kOffB4 + 1];
+    out[0] = y0 * M->out_std[0] + M->out_mean[0];
+    out[1] = y1 * M->out_std[1] + M->out_mean[1];
 }
 
 }  // namespace hhg

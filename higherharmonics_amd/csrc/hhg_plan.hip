@@ -69,8 +69,8 @@ struct hhg_plan {
     int log_stride = 16;
 
     bool learned() const { return solver.proposal == HHG_PROPOSAL_LEARNED; }
-    size_t smem(int waves_per_block) const {
-        return learned() ? (size_t)waves_per_block * hhg::kMlpLdsPerWave * sizeof(__bf16) : 0;
+    size_t smem(int /*waves_per_block*/) const {
+        return learned() ? (size_t)hhg::kMlpLdsBytes : 0;
     }
 
     // at least one counter block, so trajectory launches have one even when
@@ -462,7 +462,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
             a.mlp = p->d_mlp;
             if (st == HHG_SUCCESS)
                 st = check_hip(hhg::launch_cost_probe(g.model, g.pulse, a, 16, d_cost,
-                                                      4 * hhg::kMlpLdsPerWave * sizeof(__bf16), p->stream),
+                                                      hhg::kMlpLdsBytes, p->stream),
                                "cost probe");
         }
         std::vector<float> cost(p->n_kpts);

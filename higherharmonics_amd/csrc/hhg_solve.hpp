@@ -87,11 +87,11 @@ __device__ __forceinline__ void interval_features(const F& f, JobRef J, const do
     feat[3] = (float)gap_k;
     feat[4] = (float)A0;
     feat[5] = (float)(-dA);
-    feat[6] = (float)log(h_prev / J.delta);
-    feat[7] = (float)log10(last_err > 1e-16 ? last_err : 1e-16);
+    feat[6] = __logf((float)(h_prev / J.delta));
+    feat[7] = __log10f((float)(last_err > 1e-16 ? last_err : 1e-16));
 }
 
-extern __shared__ __bf16 solve_dyn_lds[];
+extern __shared__ __align__(16) unsigned char solve_dyn_lds[];
 
 template <int M, int P, int R, bool LEARNED>
 __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
@@ -123,26 +123,26 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const bool restart = a.sp.proposal == PROPOSAL_RESTART;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    double h_ratio = 1.0;  // learned proposal h / Delta
     IntervalCtl ctl{LEARNED ? a.fallback_after : 0, h_restart};
     IntervalOut io{};
     io.last_err = 1e-3;
     const bool logging = a.log != nullptr;
-    __bf16* lds = LEARNED ? solve_dyn_lds + (threadIdx.x / WAVE) * kMlpLdsPerWave : nullptr;
+    int next_infer = 1;  // interval of the next network proposal
     for (int i = 1; i <= N; ++i) {
         float feat[kMlpIn];
         const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
-        if (LEARNED && ((i - 1) % a.infer_every) == 0) {
+        // learned: the network proposes the first step of every infer_every-th
+        // interval, the controller's carried proposal starts the others
+        double h_ratio = -1.0;
+        if (LEARNED && i == next_infer) {
+            next_infer += a.infer_every;
             // every lane of the wave takes part in the inference
             interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
             float out[kMlpOut];
-            mlp_infer_wave(a.mlp, feat, out, lds);
-            const double r = exp((double)out[0]);
-            if (r == r && r > 0.0) {
-                h_ratio = r < 1e-4 ? 1e-4 : (r > 4.5 ? 4.5 : r);
-            } else {
-                h_ratio = 0.0;  // non-finite proposal: the plain controller takes the interval
-            }
+            mlp_infer_wave(a.mlp, solve_dyn_lds, feat, out);
+            const float r = __expf(out[0]);
+            // a non-finite proposal hands the interval to the plain controller
+            h_ratio = (r == r && r > 0.0f) ? (double)fminf(fmaxf(r, 1e-4f), 4.5f) : 0.0;
         } else if (log_this) {
             interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
         }
@@ -151,8 +151,9 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
             const double t1 = J.t_begin + (double)i * J.delta;
             double h;
             if (LEARNED) {
-                h = h_ratio > 0.0 ? h_ratio * J.delta : h_restart;
-                if (!(h_ratio > 0.0)) lc.fallback++;
+                if (h_ratio > 0.0) h = h_ratio * J.delta;
+                else if (h_ratio == 0.0) h = h_restart, lc.fallback++;
+                else h = h_carry;
             } else {
                 h = restart ? h_restart : h_carry;
             }
@@ -227,6 +228,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(HHG_WAVES_P
 solve_kernel(KernelArgs a) {
     using namespace solve_detail;
     const int lane = threadIdx.x & (WAVE - 1);
+    if constexpr (LEARNED) {
+        mlp_stage_lds(a.mlp, solve_dyn_lds);
+        __syncthreads();
+    }
     LaneCounters lc;
     for (;;) {
         int base = 0;
@@ -305,7 +310,8 @@ template <int M, int P>
 __global__ __launch_bounds__(256) void cost_probe_kernel(KernelArgs a, int n_probe, float* cost) {
     using namespace solve_detail;
     const int lane = threadIdx.x & (WAVE - 1);
-    __bf16* lds = solve_dyn_lds + (threadIdx.x / WAVE) * kMlpLdsPerWave;
+    mlp_stage_lds(a.mlp, solve_dyn_lds);
+    __syncthreads();
     const int it = uniform((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
     if (it >= a.n_items) return;
     const int2 item = load_uniform(a.items + it);
@@ -323,8 +329,8 @@ __global__ __launch_bounds__(256) void cost_probe_kernel(KernelArgs a, int n_pro
         const int i = 1 + (int)(((int64_t)q * J.n_meas) / n_probe);
         float feat[kMlpIn], out[kMlpOut];
         interval_features(f, J, At, i, gap_k, J.delta, 1e-3, feat);
-        mlp_infer_wave(a.mlp, feat, out, lds);
-        total += expf(out[1]);
+        mlp_infer_wave(a.mlp, solve_dyn_lds, feat, out);
+        total += __expf(out[1]);
     }
     if (active) cost[J.k_base + kk] = total;
 }
