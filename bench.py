@@ -50,11 +50,12 @@ F_MAGNUS = 85.0
 F_MLP = 2.0 * (8 * 64 + 2 * 64 * 64 + 64 * 2)
 
 # HBM bytes per solve launch of this workload from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (28,239.1 KiB x 2.0 + 1,050,496 KiB x 1.0, the factors a plain 1 GiB
-# copy reads); profiles/solve_kernel_dirac_t2_1024x512_counters.md.
-TRAFFIC_BYTES = (28239.1 * 2.0 + 1050496.0 * 1.0) * 1024
-# The learned-proposal solve kernel has not had its own counter runs yet.
-TRAFFIC_BYTES_LEARNED = None
+# WRITE_SIZE (KiB per dispatch, scaled by 2.0 and 1.0, the factors a plain 1 GiB copy
+# reads): plain 28,089.9 x 2.0 + 1,050,496 x 1.0, learned 14,527.3 x 2.0 + 1,050,496 x 1.0;
+# profiles/solve_kernel_dirac_t2_1024x512_counters.md and
+# profiles/solve_kernel_learned_dirac_t2_1024x512_counters.md.
+TRAFFIC_BYTES = (28089.9 * 2.0 + 1050496.0 * 1.0) * 1024
+TRAFFIC_BYTES_LEARNED = (14527.3 * 2.0 + 1050496.0 * 1.0) * 1024
 
 
 def workload(small: bool):
@@ -172,6 +173,61 @@ def sweep_extras(H):
     return res
 
 
+def config4_leg(H, n=768, n_pulses=16, log_n=96):
+    """Honeycomb over the hexagonal BZ (GL n x n), n_pulses seeded cos^2 pulses
+    (10 cycles, A0 ~ U[0.1, 1.5], CEP ~ U[0, 2 pi]), 512 samples per cycle:
+    plain vs learned proposal (with and without step-count grouping), each
+    against a tight-tolerance solve of the same jobs.  The network is trained
+    on step logs of a carried-controller solve of the same pulses on a
+    log_n x log_n grid."""
+    from higherharmonics_amd import learn
+    from higherharmonics_amd import workloads as W
+    jobs = W.honeycomb_cos2(n=n, n_pulses=n_pulses)
+    K = W.trajectories(jobs)
+    small = W.honeycomb_cos2(n=log_n, n_pulses=n_pulses)
+
+    def solver(**kw):
+        return H.Solver(rtol=1e-10, atol=1e-10, **kw)
+
+    # log stride so that the records cover the whole time window of every job
+    cap = 1 << 21
+    stride = max(1, -(-sum(j["grid"][1] * j["grid"][2] * j["time"][2] for j in small) // cap))
+    t0 = time.perf_counter()
+    rec = learn.collect_step_log(lambda sv: W.build_plan(H, small, solver(proposal=sv.proposal), persistent=1),
+                                 capacity=cap, stride=stride)
+    net, losses = learn.fit_proposal(rec, epochs=6, batch=1024)
+    train_s = time.perf_counter() - t0
+    res = {"workload": f"honeycomb hex BZ GL {n}x{n}, {n_pulses} cos^2 pulses (10 cycles + 1), 512 samples/cycle",
+           "trajectories": K, "training": {"records": int(len(rec)), "log_stride": stride, "seconds": train_s,
+                                            "loss_first": float(losses[0]), "loss_last": float(losses[-1])}}
+    js = {}
+    for name, sv, kw in [("plain", solver(), {}),
+                         ("learned", solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {}),
+                         ("learned_grouped", solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {"group_by_cost": 1}),
+                         ("carry", solver(proposal=H.PROPOSAL_CARRY), {})]:
+        print(f"[bench] config 4 {name}", file=sys.stderr, flush=True)
+        plan = W.build_plan(H, jobs, sv, persistent=1, **kw)
+        plan.execute()
+        t1 = time.perf_counter()
+        js[name] = plan.execute()
+        wall = time.perf_counter() - t1
+        st = plan.stats.as_dict()
+        plan.close()
+        res[name] = {"trajectories_per_s": K / wall, "wall_s": wall, "kernel_ms": st["solve_ms"],
+                     "attempts": st["attempts"], "rejects": st["rejects"], "fallbacks": st["proposal_fallbacks"]}
+    print("[bench] config 4 tight reference", file=sys.stderr, flush=True)
+    tight_jobs = [dict(j, time=(j["time"][0], j["time"][1], j["time"][2], 500)) for j in jobs]
+    plan = W.build_plan(H, tight_jobs, H.Solver(rtol=1e-13, atol=1e-13), persistent=1)
+    jt = plan.execute()
+    plan.close()
+    for name in js:
+        res[name]["rel_l2_err_vs_tight_max_over_pulses"] = max(rel_l2(a, b) for a, b in zip(js[name], jt))
+    res["learned_vs_plain"] = res["learned"]["trajectories_per_s"] / res["plain"]["trajectories_per_s"]
+    res["tight_reference"] = "same kernel, restart controller at Delta/500, rtol=atol=1e-13"
+    net.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,9 +237,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--extras", action="store_true", help="also time Magnus-4 and the carry ablation")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--profile-leg", choices=["plain", "learned"], default=None,
+                    help="time only this leg (profiler runs)")
+    ap.add_argument("--net-out", default=None, help="save the trained network (.npz)")
+    ap.add_argument("--net-in", default=None, help="--profile-leg learned: load the network instead of training")
     ap.add_argument("--infer-every", type=int, default=1,
                     help="learned leg: network proposal every this many sampling intervals")
     ap.add_argument("--sweeps", action="store_true", help="also time reduced configurations 3-5")
+    ap.add_argument("--config4", action="store_true",
+                    help="also run the honeycomb learned-vs-plain configuration (16 pulses on 768 x 768)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -208,6 +270,19 @@ def main():
     f_traj_rhs = F_STEP + 13 * F_RHS["dirac_t2"]
     vertex = K * (wl["N"] + 1) * F_VERTEX["dirac"]
 
+    if args.profile_leg:
+        # one leg only, for profiler runs: no tight reference, no CPU baseline
+        net = None
+        if args.profile_leg == "learned":
+            net = H.MLP.load(args.net_in) if args.net_in else train_proposal(H, wl)[0]
+        plan = make_plan(H, wl, proposal=H.PROPOSAL_LEARNED if net else 0, mlp=net, infer_every=args.infer_every,
+                         persistent=1)
+        wall, ms, st, _ = time_plan(plan, args.steps, args.warmup)
+        plan.close()
+        print(json.dumps({"profile_leg": args.profile_leg, "trajectories_per_s": args.steps * K / wall,
+                          "kernel_ms": ms, "attempts": st["attempts"]}))
+        return
+
     print(f"[bench] Dirac-2D {wl['nx']}x{wl['ny']} N={wl['N']} RKF78 1e-10 restart (plain), T2 dephasing",
           file=sys.stderr, flush=True)
     plan = make_plan(H, wl, persistent=1)
@@ -218,6 +293,8 @@ def main():
 
     print("[bench] step logs + proposal network training", file=sys.stderr, flush=True)
     net, train = train_proposal(H, wl)
+    if args.net_out:
+        net.save(args.net_out)
     print(f"[bench] learned proposal solve ({train['records']} records, loss {train['loss_first']:.3g} -> "
           f"{train['loss_last']:.3g})", file=sys.stderr, flush=True)
     plan = make_plan(H, wl, proposal=H.PROPOSAL_LEARNED, mlp=net, infer_every=args.infer_every, persistent=1)
@@ -304,6 +381,8 @@ def main():
         out["extras"] = extras
     if args.sweeps:
         out["sweeps"] = sweep_extras(H)
+    if args.config4:
+        out["config4_honeycomb_learned"] = config4_leg(H)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out))

@@ -376,6 +376,21 @@ class MLP:
         _check(lib().hhg_mlp_train(self._h, _fp(x), _fp(t), len(x), epochs, batch, lr, seed, _fp(losses)))
         return losses
 
+    def save(self, path) -> None:
+        """Parameters and normalisation to an .npz file (plain arrays)."""
+        norm = getattr(self, "_norm", None)
+        if norm is None:
+            raise ValueError("MLP.save: normalisation not set")
+        np.savez(path, params=self.params(), in_mean=norm[0], in_std=norm[1], out_mean=norm[2], out_std=norm[3])
+
+    @classmethod
+    def load(cls, path, device: int = 0) -> "MLP":
+        with np.load(path, allow_pickle=False) as z:
+            net = cls(device=device)
+            net.set_normalization(z["in_mean"], z["in_std"], z["out_mean"], z["out_std"])
+            net.set_params(z["params"])
+        return net
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().hhg_mlp_destroy(self._h)

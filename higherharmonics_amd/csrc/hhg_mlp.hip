@@ -3,8 +3,9 @@
 // pass, FP32 accumulation), and its C ABI.
 //
 // A training step on a batch of n samples:
-//   fwd_bwd_kernel   one thread per sample: forward pass with the BF16-rounded
-//                    weights (staged in LDS) and BF16-rounded activations, loss
+//   fwd_bwd_kernel   one wave per sample, one lane per hidden unit: forward pass
+//                    with the BF16-rounded weights (staged in LDS) and
+//                    BF16-rounded activations, loss
 //                    0.5 |y - t|^2 averaged over the batch, backward pass with
 //                    the straight-through derivative of the BF16 rounding;
 //                    activations and deltas of every sample go to a work array
@@ -64,74 +65,119 @@ __global__ void refresh_kernel(const float* __restrict__ params, float* __restri
     }
 }
 
-// thread per sample; W: the BF16-rounded parameters.  x, t raw (t may be null:
-// forward only, y written de-normalised to y_out).
+// LDS copy of the BF16-rounded parameters for the training passes; the rows
+// of W1, W2 and W3 are padded by one float so that both the forward (lane =
+// output unit) and the backward (lane = input unit) reads are free of bank
+// conflicts.
+constexpr int kS1 = kMlpIn + 1, kS2 = kMlpHidden + 1;
+struct TrainLds {
+    float W1[kMlpHidden * kS1], b1[kMlpHidden];
+    float W2[kMlpHidden * kS2], b2[kMlpHidden];
+    float W3[kMlpHidden * kS2], b3[kMlpHidden];
+    float W4[kMlpOut * kMlpHidden], b4[kMlpOut];
+    float act[4][4][kMlpHidden];  // per wave: a0 (first 8), a1, a2, a3 / deltas
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One wave per sample, lane u = hidden unit u; W: the BF16-rounded parameters.
+// x, t raw (t may be null: forward only, y written de-normalised to y_out).
+// Every sample's activations and deltas go to its work row for the gradient sums.
 __global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ Wg, const float* __restrict__ x,
                                                       const float* __restrict__ t, const int* __restrict__ idx,
                                                       int n, MlpDev norm, float inv_n, float* __restrict__ work,
                                                       float* __restrict__ y_out) {
-    __shared__ float W[kMlpParams];
-    for (int p = threadIdx.x; p < kMlpParams; p += blockDim.x) W[p] = Wg[p];
+    __shared__ TrainLds L;
+    for (int p = threadIdx.x; p < kMlpHidden * kMlpIn; p += blockDim.x)
+        L.W1[(p / kMlpIn) * kS1 + p % kMlpIn] = Wg[kOffW1 + p];
+    for (int p = threadIdx.x; p < kMlpHidden * kMlpHidden; p += blockDim.x) {
+        L.W2[(p / kMlpHidden) * kS2 + p % kMlpHidden] = Wg[kOffW2 + p];
+        L.W3[(p / kMlpHidden) * kS2 + p % kMlpHidden] = Wg[kOffW3 + p];
+    }
+    for (int p = threadIdx.x; p < kMlpHidden; p += blockDim.x) {
+        L.b1[p] = Wg[kOffB1 + p];
+        L.b2[p] = Wg[kOffB2 + p];
+        L.b3[p] = Wg[kOffB3 + p];
+    }
+    for (int p = threadIdx.x; p < kMlpOut * kMlpHidden; p += blockDim.x) L.W4[p] = Wg[kOffW4 + p];
+    if (threadIdx.x < kMlpOut) L.b4[threadIdx.x] = Wg[kOffB4 + threadIdx.x];
     __syncthreads();
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const int s = idx ? idx[b] : b;
-    float* R = work + (size_t)b * kRec;
-    float a0[kMlpIn];
-    for (int i = 0; i < kMlpIn; ++i) {
-        a0[i] = bf16r((x[(size_t)s * kMlpIn + i] - norm.in_mean[i]) * norm.in_istd[i]);
-        R[kA0 + i] = a0[i];
-    }
-    // forward
-    for (int u = 0; u < kMlpHidden; ++u) {
-        float z = W[kOffB1 + u];
-        for (int i = 0; i < kMlpIn; ++i) z = fmaf(W[kOffW1 + u * kMlpIn + i], a0[i], z);
-        R[kA1 + u] = bf16r(z > 0.0f ? z : 0.0f);
-    }
-    for (int layer = 0; layer < 2; ++layer) {
-        const int w0 = layer == 0 ? kOffW2 : kOffW3, b0 = layer == 0 ? kOffB2 : kOffB3;
-        const int in = layer == 0 ? kA1 : kA2, out = layer == 0 ? kA2 : kA3;
-        for (int u = 0; u < kMlpHidden; ++u) {
-            float z = W[b0 + u];
-            for (int v = 0; v < kMlpHidden; ++v) z = fmaf(W[w0 + u * kMlpHidden + v], R[in + v], z);
-            R[out + u] = bf16r(z > 0.0f ? z : 0.0f);
+
+    const int wv = threadIdx.x / 64, u = threadIdx.x & 63;
+    float(*A)[kMlpHidden] = L.act[wv];
+    for (int b = blockIdx.x * 4 + wv; b < n; b += gridDim.x * 4) {
+        const int s = idx ? idx[b] : b;
+        float* R = work + (size_t)b * kRec;  // used only with targets
+        if (u < kMlpIn) {
+            const float a0 = bf16r((x[(size_t)s * kMlpIn + u] - norm.in_mean[u]) * norm.in_istd[u]);
+            A[0][u] = a0;
+            if (t) R[kA0 + u] = a0;
         }
-    }
-    float y[kMlpOut];
-    for (int o = 0; o < kMlpOut; ++o) {
-        float z = W[kOffB4 + o];
-        for (int u = 0; u < kMlpHidden; ++u) z = fmaf(W[kOffW4 + o * kMlpHidden + u], R[kA3 + u], z);
-        y[o] = z;
-    }
-    if (!t) {
-        for (int o = 0; o < kMlpOut; ++o) y_out[(size_t)b * kMlpOut + o] = y[o] * norm.out_std[o] + norm.out_mean[o];
-        return;
-    }
-    // loss and backward
-    float loss = 0.0f, dy[kMlpOut];
-    for (int o = 0; o < kMlpOut; ++o) {
-        const float tn = (t[(size_t)s * kMlpOut + o] - norm.out_mean[o]) / norm.out_std[o];
-        const float e = y[o] - tn;
-        loss += 0.5f * e * e;
-        dy[o] = e * inv_n;
-        R[kDY + o] = dy[o];
-    }
-    R[kLoss] = loss;
-    for (int u = 0; u < kMlpHidden; ++u) {
+        wave_lds_sync();
+        // forward
+        float z = L.b1[u];
+        for (int i = 0; i < kMlpIn; ++i) z = fmaf(L.W1[u * kS1 + i], A[0][i], z);
+        const float a1 = bf16r(z > 0.0f ? z : 0.0f);
+        A[1][u] = a1;
+        wave_lds_sync();
+        z = L.b2[u];
+        for (int v = 0; v < kMlpHidden; ++v) z = fmaf(L.W2[u * kS2 + v], A[1][v], z);
+        const float a2 = bf16r(z > 0.0f ? z : 0.0f);
+        A[2][u] = a2;
+        wave_lds_sync();
+        z = L.b3[u];
+        for (int v = 0; v < kMlpHidden; ++v) z = fmaf(L.W3[u * kS2 + v], A[2][v], z);
+        const float a3 = bf16r(z > 0.0f ? z : 0.0f);
+        float y[kMlpOut];
+        for (int o = 0; o < kMlpOut; ++o) y[o] = wave_sum_f32(L.W4[o * kMlpHidden + u] * a3) + L.b4[o];
+        if (!t) {
+            if (u < kMlpOut) y_out[(size_t)b * kMlpOut + u] = y[u] * norm.out_std[u] + norm.out_mean[u];
+            wave_lds_sync();
+            continue;
+        }
+        R[kA1 + u] = a1;
+        R[kA2 + u] = a2;
+        R[kA3 + u] = a3;
+        // loss and backward
+        float loss = 0.0f, dy[kMlpOut];
+        for (int o = 0; o < kMlpOut; ++o) {
+            const float tn = (t[(size_t)s * kMlpOut + o] - norm.out_mean[o]) / norm.out_std[o];
+            const float e = y[o] - tn;
+            loss += 0.5f * e * e;
+            dy[o] = e * inv_n;
+        }
+        if (u < kMlpOut) R[kDY + u] = dy[u];
+        if (u == 0) R[kLoss] = loss;
         float g = 0.0f;
-        for (int o = 0; o < kMlpOut; ++o) g = fmaf(W[kOffW4 + o * kMlpHidden + u], dy[o], g);
-        R[kD3 + u] = R[kA3 + u] > 0.0f ? g : 0.0f;
-    }
-    for (int layer = 1; layer >= 0; --layer) {
-        const int w0 = layer == 1 ? kOffW3 : kOffW2;
-        const int dout = layer == 1 ? kD3 : kD2, din = layer == 1 ? kD2 : kD1, act = layer == 1 ? kA2 : kA1;
-        for (int v = 0; v < kMlpHidden; ++v) {
-            float g = 0.0f;
-            for (int u = 0; u < kMlpHidden; ++u) g = fmaf(W[w0 + u * kMlpHidden + v], R[dout + u], g);
-            R[din + v] = R[act + v] > 0.0f ? g : 0.0f;
-        }
+        for (int o = 0; o < kMlpOut; ++o) g = fmaf(L.W4[o * kMlpHidden + u], dy[o], g);
+        const float d3 = a3 > 0.0f ? g : 0.0f;
+        R[kD3 + u] = d3;
+        A[3][u] = d3;
+        wave_lds_sync();
+        g = 0.0f;
+        for (int w = 0; w < kMlpHidden; ++w) g = fmaf(L.W3[w * kS2 + u], A[3][w], g);
+        const float d2 = a2 > 0.0f ? g : 0.0f;
+        R[kD2 + u] = d2;
+        A[0][u] = d2;  // a0 is no longer needed
+        wave_lds_sync();
+        g = 0.0f;
+        for (int w = 0; w < kMlpHidden; ++w) g = fmaf(L.W2[w * kS2 + u], A[0][w], g);
+        R[kD1 + u] = a1 > 0.0f ? g : 0.0f;
+        wave_lds_sync();
     }
 }
+
+// blocks of fwd_bwd_kernel for n samples (4 per block pass, grid-stride beyond)
+inline unsigned fwd_bwd_blocks(int64_t n) { return (unsigned)std::min<int64_t>((n + 3) / 4, 4096); }
 
 constexpr int kGradSeg = 64;  // samples per partial sum of the gradient
 
@@ -256,6 +302,7 @@ hhg_status ensure_data(hhg_mlp* m, int64_t n) {
     hipFree(m->d_y);
     m->d_x = m->d_t = m->d_y = nullptr;
     m->d_idx = nullptr;
+    m->cap_data = 0;
     MLP_TRY(hipMalloc(&m->d_x, sizeof(float) * hhg::kMlpIn * n), "x");
     MLP_TRY(hipMalloc(&m->d_t, sizeof(float) * hhg::kMlpOut * n), "t");
     MLP_TRY(hipMalloc(&m->d_y, sizeof(float) * hhg::kMlpOut * n), "y");
@@ -283,7 +330,7 @@ hhg_status ensure_batch(hhg_mlp* m, int64_t n) {
 hhg_status gradient_step(hhg_mlp* m, const int* d_idx, int n, float* loss_dst) {
     const hhg_status eb = ensure_batch(m, n);
     if (eb != HHG_SUCCESS) return eb;
-    hipLaunchKernelGGL(hhg::fwd_bwd_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_params_bf, m->d_x,
+    hipLaunchKernelGGL(hhg::fwd_bwd_kernel, dim3(hhg::fwd_bwd_blocks(n)), dim3(256), 0, m->stream, m->d_params_bf, m->d_x,
                        m->d_t, d_idx, n, m->host_dev, 1.0f / (float)n, m->d_work, nullptr);
     MLP_TRY(hipGetLastError(), "forward/backward");
     const int n_seg = (n + hhg::kGradSeg - 1) / hhg::kGradSeg;
@@ -419,11 +466,10 @@ hhg_status hhg_mlp_forward(hhg_mlp* m, const float* x, int64_t n, float* y) {
     if (n > (1 << 26)) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_mlp_forward: n too large");
     MLP_TRY(hipSetDevice(m->device), "hipSetDevice");
     hhg_status st = ensure_data(m, n);
-    if (st == HHG_SUCCESS) st = ensure_batch(m, n);
     if (st != HHG_SUCCESS) return st;
     MLP_TRY(hipMemcpy(m->d_x, x, sizeof(float) * hhg::kMlpIn * n, hipMemcpyHostToDevice), "upload x");
-    hipLaunchKernelGGL(hhg::fwd_bwd_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->d_params_bf,
-                       m->d_x, nullptr, nullptr, (int)n, m->host_dev, 1.0f, m->d_work, m->d_y);
+    hipLaunchKernelGGL(hhg::fwd_bwd_kernel, dim3(hhg::fwd_bwd_blocks(n)), dim3(256), 0, m->stream, m->d_params_bf,
+                       m->d_x, nullptr, nullptr, (int)n, m->host_dev, 1.0f, nullptr, m->d_y);
     MLP_TRY(hipGetLastError(), "forward");
     MLP_TRY(hipStreamSynchronize(m->stream), "forward sync");
     MLP_TRY(hipMemcpy(y, m->d_y, sizeof(float) * hhg::kMlpOut * n, hipMemcpyDeviceToHost), "download y");

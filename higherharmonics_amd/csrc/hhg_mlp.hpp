@@ -83,14 +83,28 @@ __device__ __forceinline__ void mlp_stage_lds(const MlpDev* __restrict__ M, unsi
         bias[q] = M->params[q < kMlpHidden ? kOffB2 + q : kOffB3 + q - kMlpHidden];
 }
 
-__device__ __forceinline__ bf16x8 relu_bf16(const floatx16& c, int g) {
-    bf16x8 r;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// BF16 rounding (round to nearest even) of accumulator elements 8 G .. 8 G + 7,
+// then ReLU as a signed 16-bit max of each packed pair (negative values, -0
+// included, have a negative pattern, and rounding never changes a sign, so
+// this equals rounding the ReLU): one conversion and one max per pair.
+template <int G>
+__device__ __forceinline__ bf16x8 relu_bf16(const floatx16& c) {
+    union {
+        bf16x2 h[4];
+        unsigned u[4];
+        bf16x8 v;
+    } t;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const float v = c[8 * g + i];
-        r[i] = (__bf16)(v > 0.0f ? v : 0.0f);
+    for (int q = 0; q < 4; ++q) {
+        bf16x2 p;
+        p[0] = (__bf16)c[8 * G + 2 * q];
+        p[1] = (__bf16)c[8 * G + 2 * q + 1];
+        t.h[q] = p;
+        asm("v_pk_max_i16 %0, %1, 0" : "=v"(t.u[q]) : "v"(t.u[q]));
     }
-    return r;
+    return t.v;
 }
 
 // Inference for the 64 lanes of a wave (batch = the wave's k-points).  Every
@@ -116,8 +130,8 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpDev* __restrict__ M, con
         for (int nt = 0; nt < 2; ++nt) {
             const floatx16 c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[64 * (kTileL1 + 2 * nt + mt)], b1[nt], zero,
                                                                        0, 0, 0);
-            act[nt][2 * mt] = relu_bf16(c, 0);
-            act[nt][2 * mt + 1] = relu_bf16(c, 1);
+            act[nt][2 * mt] = relu_bf16<0>(c);
+            act[nt][2 * mt + 1] = relu_bf16<1>(c);
         }
     }
 #pragma unroll
@@ -140,10 +154,10 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpDev* __restrict__ M, con
                 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][ks], c0, 0, 0, 0);
                 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][ks], c1, 0, 0, 0);
             }
-            nxt[0][2 * mt] = relu_bf16(c0, 0);
-            nxt[0][2 * mt + 1] = relu_bf16(c0, 1);
-            nxt[1][2 * mt] = relu_bf16(c1, 0);
-            nxt[1][2 * mt + 1] = relu_bf16(c1, 1);
+            nxt[0][2 * mt] = relu_bf16<0>(c0);
+            nxt[0][2 * mt + 1] = relu_bf16<1>(c0);
+            nxt[1][2 * mt] = relu_bf16<0>(c1);
+            nxt[1][2 * mt + 1] = relu_bf16<1>(c1);
         }
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)

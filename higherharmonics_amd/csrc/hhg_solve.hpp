@@ -77,17 +77,19 @@ __device__ __forceinline__ void flush_counters(Counters* ctr, LaneCounters& lc) 
 template <class F>
 __device__ __forceinline__ void interval_features(const F& f, JobRef J, const double* At, int i, double gap_k,
                                                   double h_prev, double last_err, float feat[kMlpIn]) {
+    // FP32 is enough for network inputs; only the differences stay FP64
+    const float inv_delta = __fdividef(1.0f, (float)J.delta);
     const double A0 = At[i - 1];
-    const double dA = (i >= 2 && i <= J.n_meas) ? (At[i] - At[i - 2]) / (2.0 * J.delta) : (At[i] - At[i - 1]) / J.delta;
+    const float dA = i >= 2 ? (float)(At[i] - At[i - 2]) * (0.5f * inv_delta) : (float)(At[i] - At[i - 1]) * inv_delta;
     double d[3];
     f.d_of(A0, d);
     feat[0] = (float)f.kx;
     feat[1] = (float)f.ky;
-    feat[2] = (float)sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    feat[2] = __fsqrt_rn((float)(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
     feat[3] = (float)gap_k;
     feat[4] = (float)A0;
-    feat[5] = (float)(-dA);
-    feat[6] = __logf((float)(h_prev / J.delta));
+    feat[5] = -dA;
+    feat[6] = __logf((float)h_prev * inv_delta);
     feat[7] = __log10f((float)(last_err > 1e-16 ? last_err : 1e-16));
 }
 
