@@ -79,20 +79,22 @@ def make_plan(H, wl, method=0, relax=2, proposal=0, magnus_substeps=5, mlp=None,
 
 
 def train_proposal(H, wl):
-    """The step-proposal network trained on accepted-step logs of a
-    carried-controller solve of the same configuration on a 256 x 128 grid of
-    the same box (every 64th interval of every k-point, about 2.1M records)."""
+    """The step-proposal network trained on accepted-step logs of the same
+    configuration on a 256 x 128 grid of the same box (every 64th interval of
+    every k-point, about 2.1M records per round): round 1 from a
+    carried-controller solve, round 2 adds a learned-proposal solve with the
+    round-1 network."""
     from higherharmonics_amd import learn
     coarse = dict(wl, nx=256, ny=128)
     t0 = time.perf_counter()
-    rec = learn.collect_step_log(lambda solver: make_plan(H, coarse, proposal=solver.proposal, persistent=1),
-                                 capacity=1 << 21, stride=64)
-    t1 = time.perf_counter()
-    net, losses = learn.fit_proposal(rec, epochs=6, batch=1024)
-    t2 = time.perf_counter()
-    return net, {"records": int(len(rec)), "log_s": t1 - t0, "train_s": t2 - t1, "epochs": len(losses),
-                 "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "params": net.n_params,
-                 "log_source": "carried-controller solve, 256x128 grid of the same box, stride 64"}
+    net, rec, hist = learn.train_on_solver_logs(
+        lambda sv: make_plan(H, coarse, proposal=sv.proposal, mlp=sv.mlp, persistent=1),
+        capacity=1 << 21, stride=64, rounds=2, epochs=6, batch=1024)
+    return net, {"records": int(len(rec)), "seconds": time.perf_counter() - t0, "rounds": len(hist),
+                 "epochs": len(hist[-1]), "loss_first": float(hist[-1][0]), "loss_last": float(hist[-1][-1]),
+                 "params": net.n_params,
+                 "log_source": "carried-controller then learned-proposal solve, 256x128 grid of the same box, "
+                               "stride 64"}
 
 
 def rel_l2(a, b):
@@ -193,9 +195,9 @@ def config4_leg(H, n=768, n_pulses=16, log_n=96):
     cap = 1 << 21
     stride = max(1, -(-sum(j["grid"][1] * j["grid"][2] * j["time"][2] for j in small) // cap))
     t0 = time.perf_counter()
-    rec = learn.collect_step_log(lambda sv: W.build_plan(H, small, solver(proposal=sv.proposal), persistent=1),
-                                 capacity=cap, stride=stride)
-    net, losses = learn.fit_proposal(rec, epochs=6, batch=1024)
+    net, rec, hist = learn.train_on_solver_logs(lambda sv: W.build_plan(H, small, sv, persistent=1), capacity=cap,
+                                                stride=stride, rounds=2)
+    losses = hist[-1]
     train_s = time.perf_counter() - t0
     res = {"workload": f"honeycomb hex BZ GL {n}x{n}, {n_pulses} cos^2 pulses (10 cycles + 1), 512 samples/cycle",
            "trajectories": K, "training": {"records": int(len(rec)), "log_stride": stride, "seconds": train_s,
@@ -228,6 +230,60 @@ def config4_leg(H, n=768, n_pulses=16, log_n=96):
     return res
 
 
+def config5_leg(H, n=256, log_n=32):
+    """64 honeycomb (double cos^2) + 64 PiFlux (double Gaussian) seeded pulse
+    configurations on n x n grids in one plan: plain vs learned proposal with
+    step-count grouping, both autotuned and graph-captured.  The network is
+    trained on step logs of the same jobs on log_n x log_n grids.  Audit: a
+    seeded random 1 % of every job's k-points solved with the learned proposal,
+    the plain controller and a tight tolerance."""
+    from higherharmonics_amd import learn
+    from higherharmonics_amd import workloads as W
+    jobs = W.mixed_sweep(n=n)
+    K = W.trajectories(jobs)
+    small = W.mixed_sweep(n=log_n)
+    cap = 1 << 21
+    stride = max(1, -(-sum(j["grid"][1] * j["grid"][2] * j["time"][2] for j in small) // cap))
+    t0 = time.perf_counter()
+    net, rec, hist = learn.train_on_solver_logs(lambda sv: W.build_plan(H, small, sv, persistent=1), capacity=cap,
+                                                stride=stride, rounds=2)
+    losses = hist[-1]
+    res = {"workload": f"64 honeycomb double-cos^2 + 64 PiFlux double-Gaussian pulses, GL {n}x{n} each, "
+                       f"256 samples/cycle", "jobs": len(jobs), "trajectories": K,
+           "training": {"records": int(len(rec)), "log_stride": stride, "seconds": time.perf_counter() - t0,
+                        "loss_first": float(losses[0]), "loss_last": float(losses[-1])}}
+    for name, sv, kw in [("plain", H.Solver(), {}),
+                         ("learned_grouped", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {"group_by_cost": 1})]:
+        print(f"[bench] config 5 {name}", file=sys.stderr, flush=True)
+        plan = W.build_plan(H, jobs, sv, persistent=1, **kw)
+        tuned = plan.autotune()
+        plan.capture_graph()
+        plan.execute()
+        t1 = time.perf_counter()
+        plan.execute()
+        wall = time.perf_counter() - t1
+        st = plan.stats.as_dict()
+        plan.close()
+        res[name] = {"trajectories_per_s": K / wall, "wall_s": wall, "attempts": st["attempts"],
+                     "rejects": st["rejects"], "fallbacks": st["proposal_fallbacks"], "launch": tuned}
+    res["learned_vs_plain"] = res["learned_grouped"]["trajectories_per_s"] / res["plain"]["trajectories_per_s"]
+    print("[bench] config 5 audit", file=sys.stderr, flush=True)
+    sub = W.audit_subset(H, jobs, 0.01)
+    tight = [dict(j, time=(j["time"][0], j["time"][1], j["time"][2], 500)) for j in sub]
+    out = {}
+    for name, js, sv in [("learned", sub, H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net)), ("plain", sub, H.Solver()),
+                         ("tight", tight, H.Solver(rtol=1e-13, atol=1e-13))]:
+        plan = W.build_plan(H, js, sv, persistent=1)
+        out[name] = plan.execute()
+        plan.close()
+    res["audit"] = {"k_points": W.trajectories(sub), "fraction": 0.01,
+                    "max_rel_l2_learned_vs_plain": max(rel_l2(a, b) for a, b in zip(out["learned"], out["plain"])),
+                    "max_rel_l2_learned_vs_tight": max(rel_l2(a, b) for a, b in zip(out["learned"], out["tight"])),
+                    "max_rel_l2_plain_vs_tight": max(rel_l2(a, b) for a, b in zip(out["plain"], out["tight"]))}
+    net.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +302,8 @@ def main():
     ap.add_argument("--sweeps", action="store_true", help="also time reduced configurations 3-5")
     ap.add_argument("--config4", action="store_true",
                     help="also run the honeycomb learned-vs-plain configuration (16 pulses on 768 x 768)")
+    ap.add_argument("--config5", action="store_true",
+                    help="also run the mixed 128-pulse sweep, plain vs learned + grouping, with a 1%% audit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -383,6 +441,8 @@ def main():
         out["sweeps"] = sweep_extras(H)
     if args.config4:
         out["config4_honeycomb_learned"] = config4_leg(H)
+    if args.config5:
+        out["config5_mixed_learned"] = config5_leg(H)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out))

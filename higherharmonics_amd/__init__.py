@@ -475,7 +475,8 @@ class Plan:
         self._log_cap = capacity
 
     def step_log(self) -> np.ndarray:
-        """Records of the last execute: [n][10] = 8 features, log(h/Delta), log(attempts)."""
+        """Records of the last execute: [n][10] = 8 features, log(h_opt/Delta) (the step the
+        error model of the interval's first accepted step allows), log(attempts)."""
         cap = getattr(self, "_log_cap", 0)
         rec = np.empty((max(cap, 1), 10), dtype=np.float32)
         n = ctypes.c_int64()

@@ -386,9 +386,11 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
 struct IntervalCtl {
     int fallback_after;  // rejections of the first step after which it restarts at h_fallback (0: never)
     double h_fallback;   // the plain controller's restart step Delta / n_sub
+    bool plain_start;    // this interval starts at h_fallback as the plain controller
 };
 struct IntervalOut {
-    double h_first;      // the controller's proposal after the first accepted step
+    double h_acc;        // size of the first accepted step of the interval (0: none yet)
+    double err_acc;      // its error ratio
     double last_err;     // error ratio of the last accepted step
     uint32_t attempts;   // attempts in this interval
     bool fell_back;      // the first step was handed back to the plain controller
@@ -413,7 +415,8 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
     int step = -1;  // index of the current step inside the interval
     io.attempts = 0;
     io.fell_back = false;
-    io.h_first = 0.0;
+    io.h_acc = 0.0;
+    io.err_acc = 0.0;
     for (;;) {
         if (fresh) {
             if (!(t1 - t > kDblEps)) break;
@@ -427,7 +430,7 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
         // step for a carried or proposed first step
         const double* cand = nullptr;
         if (nom) {
-            if (sp.proposal == PROPOSAL_RESTART || io.fell_back) {
+            if (sp.proposal == PROPOSAL_RESTART || io.fell_back || ctl.plain_start) {
                 if (step < kNomRestartSlots) cand = nom + step * kNomStride;
             } else if (step == 0) {
                 cand = nom + kNomRestartSlots * kNomStride;
@@ -459,11 +462,11 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
             s[1] = snew[1];
             s[2] = snew[2];
             io.last_err = err;
+            if (io.h_acc == 0.0) io.h_acc = h, io.err_acc = err;
             if (err < 0.5) {
                 if (err > (1.0 / 390625.0)) h *= 0.9 / sqrt(sqrt(sqrt(err)));
                 else h *= max_growth();
             }
-            if (io.h_first == 0.0) io.h_first = h;
             fresh = true;
             if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) {
                 status = 2;

@@ -59,6 +59,11 @@ struct JobDev {
     int64_t nominal_base; // offset of this job in the nominal-step table (N rows of kNomRow)
     int64_t item_base;    // first item of this job (items are job-major before reordering)
     int32_t n_items, pad1;
+    // corners of the cos^2 envelopes inside sampling intervals (carried and
+    // learned proposals stop there): times, interval index i (t_{i-1} < t < t_i)
+    double kink_t[4];
+    int32_t kink_i[4];
+    int32_t n_kinks, pad2;
 };
 
 struct Counters {

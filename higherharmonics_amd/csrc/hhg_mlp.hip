@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ 
         const int s = idx ? idx[b] : b;
         float* R = work + (size_t)b * kRec;  // used only with targets
         if (u < kMlpIn) {
-            const float a0 = bf16r((x[(size_t)s * kMlpIn + u] - norm.in_mean[u]) * norm.in_istd[u]);
+            const float a0 = bf16r(mlp_norm_input(x[(size_t)s * kMlpIn + u], norm.in_mean[u], norm.in_istd[u]));
             A[0][u] = a0;
             if (t) R[kA0 + u] = a0;
         }

@@ -5,12 +5,13 @@
 // Parameters are FP32 master copies; every forward pass (training and
 // inference) uses their BF16 roundings and BF16 activations between layers,
 // with FP32 accumulation.  Inputs and outputs are normalised by fixed affine
-// maps stored with the network.
+// maps stored with the network; normalised inputs are clamped (kMlpInClamp).
 //
 // Features of one k-point at the start of a sampling interval t0:
 //   0 kx, 1 ky, 2 |d(k - A(t0))|, 3 2|d(k)| (gap at k), 4 A(t0), 5 E(t0) = -dA/dt,
 //   6 log(h_prev / Delta) (the controller's carried step), 7 log10 of the last error ratio
-// Outputs: 0 log(h / Delta) of the first step to attempt, 1 log(attempts in the interval).
+// Outputs: 0 log(h / Delta) of the first step to attempt (trained on the step the
+// error model of the first accepted step allows), 1 log(attempts in the interval).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -30,6 +31,15 @@ constexpr int kOffB3 = kOffW3 + kMlpHidden * kMlpHidden;
 constexpr int kOffW4 = kOffB3 + kMlpHidden;
 constexpr int kOffB4 = kOffW4 + kMlpOut * kMlpHidden;
 constexpr int kMlpParams = kOffB4 + kMlpOut;  // 9026
+// Normalised inputs are clamped to +-kMlpInClamp standard deviations in every
+// forward pass: a feature far outside the training distribution (a step size
+// the logs never showed) cannot drive the output arbitrarily far.
+constexpr float kMlpInClamp = 8.0f;
+
+__host__ __device__ inline float mlp_norm_input(float x, float mean, float istd) {
+    const float z = (x - mean) * istd;
+    return fminf(fmaxf(z, -kMlpInClamp), kMlpInClamp);
+}
 
 // MFMA operand packing for v_mfma_f32_32x32x16_bf16 (A = weights, out x in;
 // B = activations, in x samples; C = out x samples).  Operand layouts of the
@@ -118,7 +128,7 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpDev* __restrict__ M, con
     const float* bias = reinterpret_cast<const float*>(lds + kPackTotal * 2);
     bf16x8 x, one = {};
 #pragma unroll
-    for (int i = 0; i < kMlpIn; ++i) x[i] = (__bf16)((feat[i] - M->in_mean[i]) * M->in_istd[i]);
+    for (int i = 0; i < kMlpIn; ++i) x[i] = (__bf16)mlp_norm_input(feat[i], M->in_mean[i], M->in_istd[i]);
     one[0] = (__bf16)1.0f;
     const bf16x8 b1[2] = {h == 0 ? x : one, h == 1 ? x : one};
 

@@ -123,6 +123,41 @@ hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n
     return HHG_SUCCESS;
 }
 
+// Corners of the cos^2 envelopes (A'' jumps at the support edges) strictly
+// inside a sampling interval, ascending.  An RKF78 step across a corner has an
+// error far above its embedded estimate; the carried and learned proposals
+// take whole-interval steps, so they integrate up to each corner and continue
+// from there.  The plain controller keeps the reference's per-interval restart
+// sequence unchanged.  Edges within 1e-9 Delta of a sample time and edges of a
+// zero-amplitude component are skipped (the oracle's envelope_corners makes
+// the same list with the same operations).
+void set_envelope_corners(hhg::JobDev& J, const hhg_pulse_desc& p, const hhg_solver_desc& sv) {
+    J.n_kinks = 0;
+    if (sv.method != HHG_METHOD_RKF78 || sv.proposal == HHG_PROPOSAL_RESTART) return;
+    if (p.kind != HHG_PULSE_COS2 && p.kind != HHG_PULSE_DOUBLE_COS2) return;
+    std::vector<double> e;
+    if (p.amp_a != 0.0) {
+        e.push_back(p.t0);
+        e.push_back(p.t0 + 2.0 * 3.14159265358979323846 * p.n_cycles);
+    }
+    if (p.kind == HHG_PULSE_DOUBLE_COS2 && p.amp_b != 0.0) {
+        const double b = p.t0 + p.delay;
+        e.push_back(b);
+        e.push_back(b + 2.0 * 3.14159265358979323846 * p.n_cycles);
+    }
+    std::sort(e.begin(), e.end());
+    for (double t : e) {
+        const double x = (t - J.t_begin) / J.delta;
+        if (!(x > 0.0 && x < (double)J.n_meas)) continue;
+        const double fl = std::floor(x);
+        if (x - fl < 1e-9 || fl + 1.0 - x < 1e-9) continue;
+        if (J.n_kinks > 0 && J.kink_t[J.n_kinks - 1] == t) continue;
+        J.kink_t[J.n_kinks] = t;
+        J.kink_i[J.n_kinks] = (int32_t)fl + 1;
+        ++J.n_kinks;
+    }
+}
+
 hhg::JobDev make_job(const hhg_job_desc& jd, const hhg_solver_desc& sv) {
     hhg::JobDev J{};
     const hhg_model_desc& m = jd.model;
@@ -153,6 +188,7 @@ hhg::JobDev make_job(const hhg_job_desc& jd, const hhg_solver_desc& sv) {
     J.n_meas = jd.time.n_measurements;
     J.n_sub = jd.time.n_subdivisions;
     J.magnus_sub = sv.method == HHG_METHOD_MAGNUS4 ? std::max(1, sv.magnus_substeps) : 0;
+    set_envelope_corners(J, p, sv);
     return J;
 }
 

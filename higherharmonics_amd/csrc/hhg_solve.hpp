@@ -125,11 +125,12 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const bool restart = a.sp.proposal == PROPOSAL_RESTART;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{LEARNED ? a.fallback_after : 0, h_restart};
+    IntervalCtl ctl{LEARNED ? a.fallback_after : 0, h_restart, false};
     IntervalOut io{};
     io.last_err = 1e-3;
     const bool logging = a.log != nullptr;
     int next_infer = 1;  // interval of the next network proposal
+    int kq = 0;          // next envelope corner
     for (int i = 1; i <= N; ++i) {
         float feat[kMlpIn];
         const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
@@ -153,6 +154,7 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
             const double t1 = J.t_begin + (double)i * J.delta;
             double h;
             if (LEARNED) {
+                ctl.plain_start = h_ratio == 0.0;
                 if (h_ratio > 0.0) h = h_ratio * J.delta;
                 else if (h_ratio == 0.0) h = h_restart, lc.fallback++;
                 else h = h_carry;
@@ -160,21 +162,42 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
                 h = restart ? h_restart : h_carry;
             }
             double unclipped = h;
-            const double* nom = a.nom ? a.nom + J.nominal_base + (int64_t)(i - 1) * kNomRow : nullptr;
-            rkf78_interval(f, s, t0, t1, h, unclipped, a.sp, lc.att, lc.rej, status, nom, ctl, io);
+            // an envelope corner inside the interval (listed only for carried and
+            // learned proposals): integrate up to it, continue from there with the
+            // controller's carried step; the table's whole-interval step does not apply
+            const bool corner = kq < J.n_kinks && J.kink_i[kq] == i;
+            const double* nom = (a.nom && !corner) ? a.nom + J.nominal_base + (int64_t)(i - 1) * kNomRow : nullptr;
+            IntervalCtl c = ctl;
+            double ta = t0;
+            for (int q = kq;; ++q) {
+                const bool stop = q < J.n_kinks && J.kink_i[q] == i;
+                const double tb = stop ? J.kink_t[q] : t1;
+                if (status == 0) {
+                    rkf78_interval(f, s, ta, tb, h, unclipped, a.sp, lc.att, lc.rej, status, nom, c, io);
+                    if (io.fell_back) lc.fallback++;
+                }
+                if (!stop) break;
+                h = h > unclipped ? h : unclipped;
+                ta = tb;
+                c.fallback_after = 0;  // the proposal concerned the interval's first step only
+            }
             h_carry = h > unclipped ? h : unclipped;
-            if (io.fell_back) lc.fallback++;
-            if (log_this && status == 0) {
+            if (log_this && status == 0 && !corner) {
                 const unsigned long long slot = atomicAdd(a.log_count, 1ull);
                 if (slot < (unsigned long long)a.log_cap) {
                     float* rec = a.log + slot * (kMlpIn + kMlpOut);
 #pragma unroll
                     for (int q = 0; q < kMlpIn; ++q) rec[q] = feat[q];
-                    rec[kMlpIn] = (float)log(io.h_first / J.delta);
+                    // target: the step the error model of the first accepted step
+                    // allows, h 0.9 err^(-1/8) without the growth cap (near-independent
+                    // of the step that was tried: err scales as h^8)
+                    const double e = io.err_acc > 1e-16 ? io.err_acc : 1e-16;
+                    rec[kMlpIn] = (float)(log(io.h_acc / J.delta) + log(0.9) - 0.125 * log(e));
                     rec[kMlpIn + 1] = (float)log((double)io.attempts);
                 }
             }
         }
+        while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
         const double g = (status != 3) ? w * f.vertex(At[i], s) : 0.0;
         sink.put(i, wave_sum(g), lane);
     }
@@ -274,8 +297,9 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
     int status = 0;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{0, h_restart};
+    IntervalCtl ctl{0, h_restart, false};
     IntervalOut io{};
+    int kq = 0;
     for (int i = 1; i <= N; ++i) {
         if constexpr (METHOD == 0) {
             if (status == 0) {
@@ -283,9 +307,18 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
                 const double t1 = J.t_begin + (double)i * J.delta;
                 double h = (sp.proposal == PROPOSAL_RESTART) ? h_restart : h_carry;
                 double unclipped = h;
-                rkf78_interval(f, s, t0, t1, h, unclipped, sp, att, rej, status, nullptr, ctl, io);
+                double ta = t0;
+                for (int q = kq;; ++q) {  // envelope corners, as in the solve kernel
+                    const bool stop = q < J.n_kinks && J.kink_i[q] == i;
+                    const double tb = stop ? J.kink_t[q] : t1;
+                    if (status == 0) rkf78_interval(f, s, ta, tb, h, unclipped, sp, att, rej, status, nullptr, ctl, io);
+                    if (!stop) break;
+                    h = h > unclipped ? h : unclipped;
+                    ta = tb;
+                }
                 h_carry = h > unclipped ? h : unclipped;
             }
+            while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
         } else {
             const int ms = J.magnus_sub;
             const double h = J.delta / ms;

@@ -16,6 +16,8 @@ Units are the reference's dimensionless ones (omega = 1).
 * ``mixed_sweep`` — honeycomb (double cos^2) and PiFlux (double Gaussian)
   jobs with seeded A0 ~ U[0.1, 2.0], CEP ~ U[0, 2 pi], delay ~ U[-10, 10]
   cycles (configuration 5).
+* ``audit_subset`` — the same jobs on a seeded random 1 % of their k-points
+  (explicit grids), for checking one solver mode against another.
 """
 from __future__ import annotations
 
@@ -97,18 +99,40 @@ def trajectories(jobs: List[Dict]) -> int:
     n = 0
     for j in jobs:
         g = j["grid"]
-        n += g[1] * g[2]
+        n += len(g[1]) if g[0] == "explicit" else g[1] * g[2]
     return n
+
+
+def audit_subset(H, jobs: List[Dict], frac: float = 0.01, seed: int = SEED) -> List[Dict]:
+    """The jobs restricted to a seeded random fraction of each job's k-points
+    (at least 64), as explicit grids with the original quadrature weights."""
+    rng = np.random.default_rng(seed)
+    built = {}
+    out = []
+    for j in jobs:
+        key = tuple(j["grid"])
+        if key not in built:
+            g = H.KGrid.box(*key[1:]) if key[0] == "box" else H.KGrid.hexagonal(*key[1:])
+            built[key] = g.build()
+        kx, ky, w = built[key]
+        m = min(len(kx), max(64, int(round(frac * len(kx)))))
+        sel = np.sort(rng.choice(len(kx), size=m, replace=False))
+        out.append(dict(j, grid=("explicit", kx[sel], ky[sel], w[sel])))
+    return out
 
 
 def build_plan(H, jobs: List[Dict], solver=None, **launch):
     """A Plan for the jobs; identical grids are shared between jobs."""
     grids, grid_index, hjobs = [], {}, []
     for j in jobs:
-        key = tuple(j["grid"])
+        g = j["grid"]
+        key = ("explicit", id(g)) if g[0] == "explicit" else tuple(g)
         if key not in grid_index:
             grid_index[key] = len(grids)
-            grids.append(H.KGrid.box(*key[1:]) if key[0] == "box" else H.KGrid.hexagonal(*key[1:]))
+            if g[0] == "explicit":
+                grids.append(H.KGrid.explicit(*g[1:]))
+            else:
+                grids.append(H.KGrid.box(*g[1:]) if g[0] == "box" else H.KGrid.hexagonal(*g[1:]))
         m, p = j["model"], j["pulse"]
         md = H.ModelDesc(m.get("kind", 0), m.get("relax", 0), m.get("v", 1.0), m.get("mass", 0.1),
                          m.get("hopping", 1.0), m.get("theta", 0.0), m.get("fermi_energy", 0.0), m.get("beta", 0.0),

@@ -257,8 +257,10 @@ hhg_status hhg_trapezoidal_spectrum(const double* j, const hhg_time_desc* time, 
 
 /* Enables logging in later executes: for every stride-th interval of every
  * trajectory (staggered by lane) one record of 10 floats: the 8 network
- * features at the interval start, log(h / Delta) of the controller's proposal
- * after the interval's first accepted step, and log(attempts in the interval).
+ * features at the interval start, log(h_opt / Delta) with h_opt = h 0.9
+ * max(err, 1e-16)^(-1/8) of the interval's first accepted step h (error ratio
+ * err; the controller's growth rule without its 4.5 cap), and log(attempts in
+ * the interval).  Intervals split at an envelope corner are not logged.
  * At most capacity records are kept.  capacity = 0 disables logging.  No
  * counterpart in the reference (its solver keeps no step history). */
 hhg_status hhg_plan_set_step_log(hhg_plan* plan, int64_t capacity, int32_t stride);

@@ -441,6 +441,48 @@ static double sample_time(const or_time* tc, int i) {
     return tc->t_begin + (double)i * delta;
 }
 
+/* Corners of the cos^2 envelopes (A'' jumps at the support edges) that lie
+ * strictly inside a sampling interval, ascending, with the interval index i
+ * (t_{i-1} < t < t_i).  The carried controller integrates up to each corner
+ * and continues from there; the reference's restart controller does not
+ * (it has no such corners to meet with its short restart steps), so this is
+ * used only with OR_PROPOSAL_CARRY.  Edges within 1e-9 Delta of a sample
+ * time and edges of a zero-amplitude component are skipped.  The product
+ * computes the same list with the same double operations. */
+static int envelope_corners(const or_pulse* p, const or_time* tc, double* t_k, int* i_k) {
+    double e[4];
+    int ne = 0;
+    if (p->kind != OR_PULSE_COS2 && p->kind != OR_PULSE_DOUBLE_COS2) return 0;
+    if (p->amp_a != 0.0) {
+        e[ne++] = p->t0;
+        e[ne++] = p->t0 + 2.0 * OR_PI * p->n_cycles;
+    }
+    if (p->kind == OR_PULSE_DOUBLE_COS2 && p->amp_b != 0.0) {
+        const double b = p->t0 + p->delay;
+        e[ne++] = b;
+        e[ne++] = b + 2.0 * OR_PI * p->n_cycles;
+    }
+    for (int a = 1; a < ne; ++a) /* insertion sort */
+        for (int b = a; b > 0 && e[b] < e[b - 1]; --b) {
+            const double tmp = e[b];
+            e[b] = e[b - 1];
+            e[b - 1] = tmp;
+        }
+    const double delta = (tc->t_end - tc->t_begin) / tc->n_measurements;
+    int n = 0;
+    for (int q = 0; q < ne; ++q) {
+        const double x = (e[q] - tc->t_begin) / delta;
+        if (!(x > 0.0 && x < (double)tc->n_measurements)) continue;
+        const double fl = floor(x);
+        if (x - fl < 1e-9 || fl + 1.0 - x < 1e-9) continue;
+        if (n > 0 && t_k[n - 1] == e[q]) continue;
+        t_k[n] = e[q];
+        i_k[n] = (int)fl + 1;
+        ++n;
+    }
+    return n;
+}
+
 /* Evolves one k-point and calls emit(i, s) at every sample.  Returns 1 on a
  * controller failure (the remaining samples are emitted from the frozen
  * state), 0 otherwise, or -1 for an unsupported combination. */
@@ -480,17 +522,25 @@ static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const
     int failed = 0;
     const double dt0 = delta / tc->n_subdivisions;
     double dt = dt0;
+    const int carry = sv->proposal == OR_PROPOSAL_CARRY;
+    double t_k[4];
+    int i_k[4];
+    const int n_k = carry ? envelope_corners(p, tc, t_k, i_k) : 0;
+    int q = 0;
     for (int i = 1; i <= N; ++i) {
         if (!failed) {
-            double h = (sv->proposal == OR_PROPOSAL_CARRY) ? dt : dt0;
+            double h = carry ? dt : dt0;
             double unclipped = h;
-            if (rkf78_integrate(&c, s, sample_time(tc, i - 1), sample_time(tc, i), &h, &unclipped)) {
-                failed = 1;
-            } else {
-                /* carry mode continues from the proposal of the last
-                 * unclipped step when the final step was clipped */
-                dt = h > unclipped ? h : unclipped;
+            double ta = sample_time(tc, i - 1);
+            /* carry mode continues from the proposal of the last unclipped
+             * step when the final step was clipped, also across corners */
+            for (; q < n_k && i_k[q] == i && !failed; ++q) {
+                if (rkf78_integrate(&c, s, ta, t_k[q], &h, &unclipped)) failed = 1;
+                h = h > unclipped ? h : unclipped;
+                ta = t_k[q];
             }
+            if (!failed && rkf78_integrate(&c, s, ta, sample_time(tc, i), &h, &unclipped)) failed = 1;
+            if (!failed) dt = h > unclipped ? h : unclipped;
         }
         emit(user, i, s);
     }

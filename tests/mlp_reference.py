@@ -2,10 +2,12 @@
 ReLU) for checking the GPU kernels.  With ``bf16=True`` the weights,
 normalised inputs and hidden activations are rounded to BF16 exactly where
 the GPU rounds them (round to nearest even of the FP32 value), and the
-backward pass treats the rounding as the identity, as the GPU does."""
+backward pass treats the rounding as the identity, as the GPU does.
+Normalised inputs are clamped to +-IN_CLAMP as on the GPU."""
 import numpy as np
 
 N_IN, H, N_OUT = 8, 64, 2
+IN_CLAMP = 8.0  # normalised inputs are clamped to +-8 standard deviations
 OFF = {}
 _o = 0
 for name, size in (("W1", H * N_IN), ("b1", H), ("W2", H * H), ("b2", H), ("W3", H * H), ("b3", H),
@@ -39,11 +41,12 @@ def loss_and_grad(params, x, t, in_mean, in_std, out_mean, out_std, bf16_round=T
     x32 = np.asarray(x, dtype=np.float32)
     if bf16_round:
         istd = (np.float32(1.0) / np.asarray(in_std, dtype=np.float32))
-        a0 = bf16((x32 - np.asarray(in_mean, dtype=np.float32)) * istd)
+        z = (x32 - np.asarray(in_mean, dtype=np.float32)) * istd
+        a0 = bf16(np.clip(z, -IN_CLAMP, IN_CLAMP))
         tn = ((np.asarray(t, dtype=np.float32) - np.asarray(out_mean, dtype=np.float32)) /
               np.asarray(out_std, dtype=np.float32)).astype(np.float64)
     else:
-        a0 = (np.asarray(x, np.float64) - in_mean) / in_std
+        a0 = np.clip((np.asarray(x, np.float64) - in_mean) / in_std, -IN_CLAMP, IN_CLAMP)
         tn = (np.asarray(t, np.float64) - out_mean) / out_std
     z1 = a0 @ W1.T + b1
     a1 = r(np.maximum(z1, 0))

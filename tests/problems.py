@@ -149,4 +149,15 @@ HONEYCOMB_DOUBLE_COS2 = Problem(
     grid=("hex", 16, 16),
 )
 
+# cos^2 envelope whose support edges (corners of A'') fall inside sampling
+# intervals; strong field and coarse sampling make whole-interval steps across
+# a corner visibly wrong
+HONEYCOMB_CORNERS = Problem(
+    "honeycomb_cos2_corners",
+    model=dict(kind=O.MODEL_HONEYCOMB, hopping=1.0),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=1.69, n_cycles=2.0, t0=0.05, cep=0.3),
+    time=(0.0, 6 * PI, 192, 50),
+    grid=("hex", 12, 12),
+)
+
 SMALL_CASES = [DIRAC_SMALL, DIRAC_COHERENT, DIRAC_RELAX_INIT, PIFLUX_SMALL, HONEYCOMB_SMALL, HONEYCOMB_DOUBLE_COS2]

@@ -8,8 +8,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_SMALL, HONEYCOMB_SMALL, PIFLUX_SMALL, SMALL_CASES,
-                            oracle_j, rel_l2)
+from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_SMALL, HONEYCOMB_CORNERS, HONEYCOMB_SMALL, PIFLUX_SMALL,
+                            SMALL_CASES, oracle_j, rel_l2)
 
 pytestmark = pytest.mark.gpu
 
@@ -101,8 +101,9 @@ def test_magnus_matches_oracle(hhg):
     assert rel_l2(j, j_ref) <= J_TOL
 
 
-def test_carry_proposal_matches_oracle(hhg):
-    pr = HONEYCOMB_SMALL
+@pytest.mark.parametrize("pr", [HONEYCOMB_SMALL, HONEYCOMB_CORNERS], ids=lambda p: p.name)
+def test_carry_proposal_matches_oracle(hhg, pr):
+    """Carried controller, including the stops at envelope corners."""
     _print_case(pr, proposal="carry")
     j_ref, st_ref = oracle_j(pr, proposal=O.PROPOSAL_CARRY)
     with pr.h_plan(hhg, solver=dict(proposal=1)) as plan:

@@ -26,8 +26,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_RELAX_INIT, DIRAC_SMALL, HONEYCOMB_SMALL, PIFLUX_SMALL,
-                            Problem, oracle_j, rel_l2)
+from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_RELAX_INIT, DIRAC_SMALL, HONEYCOMB_CORNERS, HONEYCOMB_SMALL,
+                            PIFLUX_SMALL, Problem, oracle_j, rel_l2)
 
 # --------------------------------------------------------------------------
 # Butcher trees and order conditions
@@ -320,6 +320,21 @@ def test_carry_proposal_within_criterion_of_tight_oracle():
     j, st = oracle_j(pr, proposal=O.PROPOSAL_CARRY)
     j_tight, _ = oracle_j(pr, rtol=1e-13, atol=1e-13)
     assert rel_l2(j, j_tight) <= 1e-9
+
+
+def test_carry_proposal_stops_at_envelope_corners():
+    """The carried controller takes whole-interval steps; across a corner of
+    the cos^2 envelope (A'' jumps at its support edges) such a step is wrong
+    far beyond its embedded error estimate (1.7e-8 relative L2 on this
+    problem).  Stopping at the corners keeps the carried solve within the
+    criterion of the tight oracle."""
+    pr = HONEYCOMB_CORNERS
+    j, st = oracle_j(pr, proposal=O.PROPOSAL_CARRY)
+    j_plain, st_plain = oracle_j(pr)
+    j_tight, _ = oracle_j(pr, rtol=1e-13, atol=1e-13)
+    assert rel_l2(j, j_tight) <= 1e-9
+    assert rel_l2(j_plain, j_tight) <= 1e-9
+    assert st.attempts < 0.5 * st_plain.attempts
 
 
 def test_magnus4_converges_at_order_4():
