@@ -32,11 +32,11 @@ LIB_PATH = os.path.join(_HERE, "libhhg_amd.so")
 # ---- enums (include/hhg_amd.h) ----
 SUCCESS, ERROR_INVALID_ARGUMENT, ERROR_HIP, ERROR_TOLERANCE, ERROR_OUT_OF_MEMORY, ERROR_UNSUPPORTED, \
     ERROR_NO_DEVICE, ERROR_NONFINITE = range(8)
-MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB = 0, 1, 2
+MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D = 0, 1, 2, 3
 RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2 = 0, 1, 2
-PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2 = 0, 1, 2, 3
-KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT = 0, 1, 2
-METHOD_RKF78, METHOD_MAGNUS4 = 0, 1
+PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS = 0, 1, 2, 3, 4
+KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER = 0, 1, 2, 3
+METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
 
 _STATUS_NAMES = {0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "TOLERANCE", 4: "OUT_OF_MEMORY",
@@ -115,7 +115,7 @@ EXPORTED_SYMBOLS = [
     "hhg_pulse_evaluate", "hhg_trapezoidal_spectrum", "hhg_plan_execute_accumulators", "hhg_accumulators_add",
     "hhg_accumulators_to_double", "hhg_plan_set_step_log", "hhg_plan_get_step_log", "hhg_mlp_param_count",
     "hhg_mlp_create", "hhg_mlp_destroy", "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params",
-    "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train",
+    "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -150,6 +150,7 @@ def lib() -> ctypes.CDLL:
     L.hhg_kgrid_build.argtypes = [ctypes.POINTER(KGridDesc), _DP, _DP, _DP]
     L.hhg_pulse_evaluate.argtypes = [ctypes.POINTER(PulseDesc), _DP, _DP, ctypes.c_int64]
     L.hhg_trapezoidal_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), ctypes.c_int32, _DP, _DP, _DP]
+    L.hhg_welch_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), _DP, _DP, _DP]
     L.hhg_accumulators_add.restype = None
     L.hhg_accumulators_add.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                        ctypes.c_int64]
@@ -176,7 +177,7 @@ def lib() -> ctypes.CDLL:
     for name in ("hhg_plan_create", "hhg_plan_job_offsets", "hhg_plan_execute", "hhg_plan_execute_accumulators",
                  "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
                  "hhg_trajectories", "hhg_gauss_legendre", "hhg_kgrid_build", "hhg_pulse_evaluate",
-                 "hhg_trapezoidal_spectrum"):
+                 "hhg_trapezoidal_spectrum", "hhg_welch_spectrum"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -247,6 +248,16 @@ class CosineLaser(Laser):
         return PulseDesc(PULSE_COS2, 0, self.amplitude, 0.0, self.n_cycles, 0.0, self.t_begin, 0.0, self.cep)
 
 
+class ContinuousLaser(Laser):
+    """Envelope 1 (ContinuousLaser.cpp:7-9): A(t) = amplitude sin(t + cep)."""
+
+    def __init__(self, amplitude: float, cep: float = 0.0):
+        self.amplitude, self.cep = amplitude, cep
+
+    def desc(self) -> PulseDesc:
+        return PulseDesc(PULSE_CONTINUOUS, 0, self.amplitude, 0.0, 0.0, 0.0, 0.0, 0.0, self.cep)
+
+
 class GaussLaser(Laser):
     """Gaussian envelope exp(-(t-c)^2 / 2 sigma^2) (GaussLaser.cpp:26-28).  ``window``
     is the half width of the time window in units of sigma."""
@@ -300,6 +311,11 @@ class KGrid:
     @staticmethod
     def hexagonal(nx, ny) -> "KGrid":
         return KGrid(KGRID_GL_HEX, nx, ny)
+
+    @staticmethod
+    def dirac_cylinder(n_z, n_kappa, max_k) -> "KGrid":
+        """(k_z, kappa) points of the 3D Dirac cone (kind KGRID_DIRAC_CYLINDER)."""
+        return KGrid(KGRID_DIRAC_CYLINDER, n_z, n_kappa, 0.0, max_k)
 
     @staticmethod
     def explicit(kx, ky, w) -> "KGrid":
@@ -626,4 +642,15 @@ def trapezoidal_spectrum(j: np.ndarray, time_config: TimeIntegrationConfig, zero
     f, re, im = np.empty(n), np.empty(n), np.empty(n)
     td = time_config.desc()
     _check(lib().hhg_trapezoidal_spectrum(_dp(j), ctypes.byref(td), zero_padding, _dp(f), _dp(re), _dp(im)))
+    return f, re, im
+
+
+def welch_spectrum(j: np.ndarray, time_config: TimeIntegrationConfig):
+    """(frequencies, real, imag) of a continuous-laser run: Welch window on the
+    first N samples and a real-to-complex DFT (sources/hhg.cpp:264-275)."""
+    j = np.ascontiguousarray(j, dtype=np.float64)
+    n = time_config.n_measurements // 2 + 1
+    f, re, im = np.empty(n), np.empty(n), np.empty(n)
+    td = time_config.desc()
+    _check(lib().hhg_welch_spectrum(_dp(j), ctypes.byref(td), _dp(f), _dp(re), _dp(im)))
     return f, re, im

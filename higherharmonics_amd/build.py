@@ -21,7 +21,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhhg_amd.so")
 RUN = os.path.join(HERE, "hhg_run")
 
-SOURCES = ["hhg_solve_dirac.hip", "hhg_solve_piflux.hip", "hhg_solve_honeycomb.hip", "hhg_kernels.hip",
+SOURCES = ["hhg_solve_dirac.hip", "hhg_solve_piflux.hip", "hhg_solve_honeycomb.hip", "hhg_solve_dirac3d.hip",
+           "hhg_kernels.hip",
            "hhg_mlp.hip", "hhg_plan.hip", "hhg_host.cpp"]
 HEADERS = ["hhg_device.hpp", "hhg_internal.hpp", "hhg_kernels.hpp", "hhg_host.hpp", "hhg_solve.hpp", "hhg_mlp.hpp"]
 
@@ -61,7 +62,7 @@ def build(verbose: bool = False) -> str:
     # the C++ driver over the host interface (cpp/hhg.hpp): a host-only program
     cxx = shutil.which("g++") or shutil.which("c++")
     cmd = [cxx, "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "cpp", "hhg_run.cpp"), "-o", RUN + ".tmp",
-           "-L" + HERE, "-lhhg_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+           "-L" + HERE, "-lhhg_amd", "-lz", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -3,10 +3,12 @@
 // same names and argument meanings:
 //
 //   HHG::TimeIntegrationConfig      sources/HHG/TimeIntegrationConfig.hpp:7-15
+//   HHG::Laser::ContinuousLaser     sources/HHG/Laser/ContinuousLaser.hpp:6-11
 //   HHG::Laser::CosineLaser         sources/HHG/Laser/CosineLaser.hpp:6-19
 //   HHG::Laser::GaussLaser          sources/HHG/Laser/GaussLaser.hpp:6-21
 //   HHG::Laser::DoubleLaser         sources/HHG/Laser/DoubleLaser.hpp:8-74
 //   Laser::laser_function(t)        sources/HHG/Laser/Laser.hpp:32
+//   HHG::Systems::DiracSystem       sources/HHG/Systems/DiracSystem.hpp:10-91 (3D cone)
 //   HHG::Systems::*::compute_current_density(laser, time_config, rank, n_ranks)
 //                                   sources/HHG/Systems/DiracSystem.hpp:58-59,
 //                                   PiFlux.hpp:92-93, Honeycomb.hpp:70-71
@@ -67,6 +69,25 @@ struct Laser {
         h_float A = 0.0;
         check(hhg_pulse_evaluate(&d, &t, &A, 1));
         return A;
+    }
+};
+
+// The reference's amplitude conversion (Laser.cpp:24-25): momentum amplitude
+// = model_ratio * 1e11 * E_0 / photon_energy, E_0 in MV/cm, photon energy in meV.
+inline h_float momentum_amplitude(h_float photon_energy, h_float E_0, h_float model_ratio) {
+    return model_ratio * 1e11 * E_0 / photon_energy;
+}
+
+// envelope 1 (ContinuousLaser.cpp:7-9)
+struct ContinuousLaser : Laser {
+    h_float cep{};
+    explicit ContinuousLaser(h_float amplitude, h_float cep_ = 0.0) : cep(cep_) { momentum_amplitude = amplitude; }
+    hhg_pulse_desc desc() const override {
+        hhg_pulse_desc d{};
+        d.kind = HHG_PULSE_CONTINUOUS;
+        d.amp_a = momentum_amplitude;
+        d.cep = cep;
+        return d;
     }
 };
 
@@ -254,6 +275,44 @@ class DiracSystem2D : public System {
     }
 };
 
+// The reference's 3D Dirac cone (DiracSystem.hpp:10-91) in its own units:
+// temperature in K, E_F in meV, v_F in m/s, band width in photon energies,
+// photon energy in meV, diagonal relaxation time in fs (<= 0: none).  Field
+// along z; points (k_z, kappa) on the grid of compute_current_density with
+// n_kappa Gauss-Legendre nodes in kappa.  Without relaxation the reference's
+// moment Magnus step is used (DiracSystem.cpp:285), with relaxation RKF78 at
+// its tolerances abs 1e-12, rel 1e-8 (DiracSystem.cpp:14-15, 361-406).
+class DiracSystem : public System {
+   public:
+    static constexpr h_float hbar = 6.582119569509065698e-13;  // meV s (GlobalDefinitions.hpp:29)
+    static constexpr h_float k_B = 0.08617333262145177434;     // meV / K (GlobalDefinitions.hpp:30)
+    h_float v_F, photon_energy;
+
+    DiracSystem(h_float temperature, h_float E_F, h_float v_F_, h_float band_width, h_float photon_energy_,
+                h_float diagonal_relaxation_time, int n_z, int n_kappa)
+        : v_F(v_F_), photon_energy(photon_energy_) {
+        model.kind = HHG_MODEL_DIRAC3D;
+        model.fermi_energy = E_F / photon_energy;
+        model.beta = temperature > 0.0 ? photon_energy / (k_B * temperature) : 0.0;  // 0: T = 0
+        if (diagonal_relaxation_time > 0.0) {
+            model.relax = HHG_RELAX_TO_INITIAL;
+            // inverse time (1e15 hbar) / (tau photon_energy) in omega_L (DiracSystem.cpp:64)
+            model.relax_time = (diagonal_relaxation_time * photon_energy) / (1e15 * hbar);
+            solver.method = HHG_METHOD_RKF78;
+            solver.atol = 1e-12;
+            solver.rtol = 1e-8;
+        } else {
+            solver.method = HHG_METHOD_MAGNUS_MOMENTS;
+        }
+        kgrid.kind = HHG_KGRID_DIRAC_CYLINDER;
+        kgrid.nx = n_z;
+        kgrid.ny = n_kappa;
+        kgrid.x1 = band_width;
+    }
+    // hbar v_F / photon energy (DiracSystem.hpp:30-32)
+    h_float laser_model_ratio(h_float photon_energy_) const { return hbar * v_F / photon_energy_; }
+};
+
 // PiFlux square lattice d = 2t (cos(kx - A), cos ky, 0) over [-pi, pi]^2
 class PiFlux2D : public System {
    public:
@@ -288,6 +347,15 @@ namespace Fourier {
 struct Spectrum {
     std::vector<h_float> frequencies, real, imag;
 };
+
+// continuous lasers: Welch window + real-to-complex DFT (sources/hhg.cpp:264-275)
+inline Spectrum welch_spectrum(const std::vector<h_float>& j, const TimeIntegrationConfig& tc) {
+    const size_t n = (size_t)tc.n_measurements / 2 + 1;
+    Spectrum s{std::vector<h_float>(n), std::vector<h_float>(n), std::vector<h_float>(n)};
+    const hhg_time_desc td = tc.desc();
+    check(hhg_welch_spectrum(j.data(), &td, s.frequencies.data(), s.real.data(), s.imag.data()));
+    return s;
+}
 
 inline Spectrum trapezoidal_spectrum(const std::vector<h_float>& j, const TimeIntegrationConfig& tc,
                                      int zero_padding = 8) {

@@ -2,15 +2,29 @@
 // reference's sources/hhg.cpp main(): read a key-value configuration, build
 // the system, laser and time grid (the Dispatcher constructors,
 // sources/HHG/Dispatch/*Dispatcher.cpp), compute j(t) on the GPU, take the
-// end-corrected spectrum and write one JSON document.
+// spectrum and write one JSON document (gzip-compressed when the output name
+// ends in .gz, as the reference's current_density.json.gz).
 //
-//   hhg_run <config> [output.json]
+//   hhg_run <config> [output.json[.gz]]
 //
 // Configuration: whitespace-separated "key value" lines, '#' starts a comment
-// (the reference's InputFileReader format, sources/hhg.cpp:58-80), in the
-// reference's dimensionless units (time 1/omega_L, energy hbar omega_L).
+// (the reference's InputFileReader format, sources/hhg.cpp:58-80).
+//
+// system_type Dirac: the reference's 3D Dirac cone in the reference's own
+// units and keys (sources/hhg.cpp:63-80, DiracDispatcher.cpp:15-58):
+//   T [K], E_F [meV], v_F [m/s], band_width [photon energies],
+//   field_amplitude [MV/cm], photon_energy [meV], diagonal_relaxation_time [fs]
+//   (<= 0: none; then the moment Magnus step, else RKF78), laser_type
+//   continuous | cosine, n_laser_cycles, t0_offset (cos^2 begin in units of
+//   pi), n_z; N = n_laser_cycles * 1024 samples (hhg.cpp:87-88); n_kappa
+//   (Gauss-Legendre nodes in kappa, default 32) replaces the reference's
+//   adaptive kappa trapezoid.  The output carries the reference's fields
+//   (hhg.cpp:301-326); continuous lasers use the Welch-window spectrum.
+//
+// Other systems in the reference's dimensionless units (time 1/omega_L,
+// energy hbar omega_L):
 //   system_type       Dirac2D | PiFlux2D | Honeycomb
-//   laser_type        cosine | gauss | double_gauss | double_cosine
+//   laser_type        cosine | gauss | double_gauss | double_cosine | continuous
 //   field_amplitude   A0 (momentum amplitude)
 //   n_laser_cycles    cos^2 length in cycles (default 8)
 //   t0_offset         window begin of the cos^2 pulse in units of pi (default -n_laser_cycles)
@@ -32,6 +46,8 @@
 #include <map>
 #include <sstream>
 #include <string>
+
+#include <zlib.h>
 
 #include "hhg.hpp"
 
@@ -72,6 +88,83 @@ void write_array(std::ostream& o, const char* name, const std::vector<double>& v
     o << "]" << (last ? "\n" : ",\n");
 }
 
+std::string num_str(double v) {
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    return buf;
+}
+
+// stdout, a plain file, or gzip when the name ends in .gz
+void emit(const std::string& text, int argc, char** argv) {
+    if (argc <= 2) {
+        std::cout << text;
+        return;
+    }
+    const std::string path = argv[2];
+    if (path.size() > 3 && path.compare(path.size() - 3, 3, ".gz") == 0) {
+        gzFile f = gzopen(path.c_str(), "wb");
+        if (!f || gzwrite(f, text.data(), (unsigned)text.size()) != (int)text.size() || gzclose(f) != Z_OK)
+            throw std::runtime_error("cannot write " + path);
+        return;
+    }
+    std::ofstream out(path);
+    if (!out || !(out << text)) throw std::runtime_error("cannot write " + path);
+}
+
+// The reference's 3D Dirac run in its own units (DiracDispatcher, hhg.cpp main).
+int run_dirac(const std::map<std::string, std::string>& kv, int argc, char** argv) {
+    using namespace HHG;
+    const double T = num(kv, "T", 0.0), E_F = num(kv, "E_F", 0.0), v_F = num(kv, "v_F", 1.5e5);
+    const double band_width = num(kv, "band_width", 10.0), E0 = num(kv, "field_amplitude", 1.0);
+    const double photon_energy = num(kv, "photon_energy", 5.25), tau = num(kv, "diagonal_relaxation_time", -1.0);
+    const std::string laser_type = str(kv, "laser_type", "continuous");
+    const int n_cycles = (int)num(kv, "n_laser_cycles", 8);
+    const double t0_offset = num(kv, "t0_offset", 0.0);
+    const int n_z = (int)num(kv, "n_z", 64), n_kappa = (int)num(kv, "n_kappa", 32);
+    const int N = (int)num(kv, "n_measurements", n_cycles * 1024);
+
+    Systems::DiracSystem sys(T, E_F, v_F, band_width, photon_energy, tau, n_z, n_kappa);
+    const double A0 = Laser::momentum_amplitude(photon_energy, E0, sys.laser_model_ratio(photon_energy));
+    std::unique_ptr<Laser::Laser> laser;
+    TimeIntegrationConfig tc;
+    if (laser_type == "continuous") {
+        laser = std::make_unique<Laser::ContinuousLaser>(A0);
+        tc = {-n_cycles * pi, n_cycles * pi, N, 50};
+    } else if (laser_type == "cosine") {
+        laser = std::make_unique<Laser::CosineLaser>(A0, (double)n_cycles, pi * t0_offset);
+        tc = {laser->t_begin, laser->t_end, N, 50};
+    } else {
+        throw std::invalid_argument("Laser type '" + laser_type + "' is not recognized!");
+    }
+    hhg_stats stats{};
+    const std::vector<double> j = sys.compute_current_density(laser.get(), tc, 0, 1, &stats);
+    const bool continuous = laser_type == "continuous";
+    const Fourier::Spectrum sp = continuous ? Fourier::welch_spectrum(j, tc) : Fourier::trapezoidal_spectrum(j, tc);
+    std::vector<double> A(N + (continuous ? 0 : 1));
+    for (size_t i = 0; i < A.size(); ++i) A[i] = laser->laser_function(tc.t_begin + (double)i * tc.measure_every());
+
+    std::ostringstream o;
+    o << "{\n";
+    write_array(o, "laser_function", A);
+    o << "  \"N\": " << N << ", \"t_begin\": " << num_str(tc.t_begin) << ", \"t_end\": " << num_str(tc.t_end)
+      << ", \"n_laser_cycles\": " << n_cycles << ", \"n_measurements\": " << N + (continuous ? 0 : 1) << ",\n";
+    write_array(o, "current_density_time", j);
+    write_array(o, "current_density_frequency_real", sp.real);
+    write_array(o, "current_density_frequency_imag", sp.imag);
+    o << "  \"T\": " << num_str(T) << ", \"E_F\": " << num_str(E_F) << ", \"v_F\": " << num_str(v_F)
+      << ", \"band_width\": " << num_str(band_width) << ", \"field_amplitude\": " << num_str(E0)
+      << ", \"photon_energy\": " << num_str(photon_energy) << ", \"laser_type\": \"" << laser_type
+      << "\", \"diagonal_relaxation_time\": " << num_str(tau)
+      << ", \"offdiagonal_relaxation_time\": " << num_str(num(kv, "offdiagonal_relaxation_time", -1.0)) << ",\n";
+    write_array(o, "frequencies", sp.frequencies);
+    o << "  \"zero_padding\": 8, \"system_type\": \"Dirac\", \"n_z\": " << n_z << ", \"n_kappa\": " << n_kappa
+      << ", \"t0_offset\": " << num_str(t0_offset) << ",\n";
+    o << "  \"momentum_amplitude\": " << num_str(A0) << ", \"trajectories\": " << stats.trajectories
+      << ", \"attempted_steps\": " << stats.attempts << ", \"solve_ms\": " << num_str(stats.solve_ms) << "\n}\n";
+    emit(o.str(), argc, argv);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -83,6 +176,7 @@ int main(int argc, char** argv) {
     try {
         const auto kv = read_config(argv[1]);
         const std::string system = str(kv, "system_type", "Dirac2D");
+        if (system == "Dirac") return run_dirac(kv, argc, argv);
         const std::string laser_type = str(kv, "laser_type", "cosine");
         const double A0 = num(kv, "field_amplitude", 0.5);
         const double n_cycles = num(kv, "n_laser_cycles", 8);
@@ -103,14 +197,19 @@ int main(int argc, char** argv) {
             else
                 laser = std::make_unique<Laser::DoubleLaser<Laser::GaussLaser>>(base, num(kv, "amplitude_b", A0),
                                                                                  num(kv, "delay", 0.0));
+        } else if (laser_type == "continuous") {
+            laser = std::make_unique<Laser::ContinuousLaser>(A0, num(kv, "cep", 0.0));
         } else {
             throw std::invalid_argument("Laser type '" + laser_type + "' is not recognized!");
         }
 
         const bool cos2 = laser_type == "cosine" || laser_type == "double_cosine";
         const double extra = num(kv, "extra_time", (cos2 && lattice) ? 2 * pi : 0.0);
-        TimeIntegrationConfig tc{laser->t_begin, laser->t_end + extra,
-                                 (int)num(kv, "n_measurements", n_cycles * 1024), (int)num(kv, "n_subdivisions", 50)};
+        const bool continuous = laser_type == "continuous";
+        const double t_b = continuous ? -n_cycles * pi : laser->t_begin;   // DiracDispatcher.cpp:34-35
+        const double t_e = continuous ? n_cycles * pi : laser->t_end + extra;
+        TimeIntegrationConfig tc{t_b, t_e, (int)num(kv, "n_measurements", n_cycles * 1024),
+                                 (int)num(kv, "n_subdivisions", 50)};
 
         std::unique_ptr<Systems::System> sys;
         const int nx = (int)num(kv, "nx", 128), ny = (int)num(kv, "ny", 64);
@@ -138,15 +237,14 @@ int main(int argc, char** argv) {
 
         hhg_stats stats{};
         const std::vector<double> j = sys->compute_current_density(laser.get(), tc, 0, 1, &stats);
-        const Fourier::Spectrum sp = Fourier::trapezoidal_spectrum(j, tc);
+        const Fourier::Spectrum sp = continuous ? Fourier::welch_spectrum(j, tc) : Fourier::trapezoidal_spectrum(j, tc);
         std::vector<double> times(j.size()), A(j.size());
         for (size_t i = 0; i < j.size(); ++i) {
             times[i] = tc.t_begin + (double)i * tc.measure_every();
             A[i] = laser->laser_function(times[i]);
         }
 
-        std::ofstream fout;
-        std::ostream& o = argc > 2 ? (fout.open(argv[2]), static_cast<std::ostream&>(fout)) : std::cout;
+        std::ostringstream o;
         o << "{\n";
         o << "  \"system_type\": \"" << system << "\", \"laser_type\": \"" << laser_type << "\",\n";
         o << "  \"t_begin\": " << tc.t_begin << ", \"t_end\": " << tc.t_end << ", \"n_measurements\": "
@@ -160,6 +258,7 @@ int main(int argc, char** argv) {
         write_array(o, "current_density_frequency_real", sp.real);
         write_array(o, "current_density_frequency_imag", sp.imag, true);
         o << "}\n";
+        emit(o.str(), argc, argv);
         return 0;
     } catch (const std::exception& e) {
         std::cerr << "hhg_run: " << e.what() << "\n";

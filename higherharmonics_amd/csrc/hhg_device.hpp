@@ -57,9 +57,11 @@ __device__ __forceinline__ double pulse_A(JobRef J, double t) {
     } else if constexpr (P == PULSE_DOUBLE_GAUSS) {
         const double tb = t - J.delay;
         return J.amp_a * env_gauss(J, t) * sin(t + J.cep) + J.amp_b * env_gauss(J, tb) * sin(tb + J.cep);
-    } else {
+    } else if constexpr (P == PULSE_DOUBLE_COS2) {
         const double tb = t - J.delay;
         return J.amp_a * env_cos2(J, t) * sin(t + J.cep) + J.amp_b * env_cos2(J, tb) * sin(tb + J.cep);
+    } else {
+        return J.amp_a * sin(t + J.cep);  // continuous: envelope 1
     }
 }
 
@@ -93,12 +95,13 @@ template <int P>
 struct PulseCache {
     static constexpr bool kGauss = (P == PULSE_GAUSS || P == PULSE_DOUBLE_GAUSS);
     static constexpr bool kDouble = (P == PULSE_DOUBLE_GAUSS || P == PULSE_DOUBLE_COS2);
+    static constexpr bool kCont = (P == PULSE_CONTINUOUS);
     double t;
     PulseComp a, b;
 
     __device__ __forceinline__ void prep_comp(JobRef J, double tc, PulseComp& c) {
         sincos(tc + J.cep, &c.S, &c.C);
-        if constexpr (!kGauss) sincos((tc - J.t0) * J.inv_n, &c.E0, &c.E1);
+        if constexpr (!kGauss && !kCont) sincos((tc - J.t0) * J.inv_n, &c.E0, &c.E1);
     }
 
     __device__ __forceinline__ void prepare(JobRef J, double t_) {
@@ -113,7 +116,9 @@ struct PulseCache {
                                               double cd) const {
         const double carrier = c.S * cd + c.C * sd;
         const double tt = tc + dlt;
-        if constexpr (kGauss) {
+        if constexpr (kCont) {
+            return carrier;
+        } else if constexpr (kGauss) {
             const double x = tt - J.t0;
             return exp(-(x * x) * J.inv_2s2) * carrier;
         } else {
@@ -153,17 +158,26 @@ struct Bloch {
     JobRef J;
     double kx, ky;
     double s0[3];
-    double cy;  // theta = 0 constant: PiFlux 2t cos ky, honeycomb cos(sqrt3/2 ky)
+    double cy;  // theta = 0 constant: PiFlux 2t cos ky, honeycomb cos(sqrt3/2 ky); 3D Dirac |k|
+    double ik;  // 3D Dirac: 1/|k|
 
     __device__ __forceinline__ Bloch(JobRef job, double kx_, double ky_) : J(job), kx(kx_), ky(ky_) {
+        ik = 0.0;
         if constexpr (M == MODEL_PIFLUX2D) cy = 2.0 * J.hop * cos(ky);
         else if constexpr (M == MODEL_HONEYCOMB) cy = cos(kHalfSqrt3 * ky);
+        else if constexpr (M == MODEL_DIRAC3D) cy = sqrt(kx * kx + ky * ky), ik = cy > 0.0 ? 1.0 / cy : 0.0;
         else cy = 0.0;
     }
 
-    // d(k - A e_theta)
+    // d(k - A e_theta); the 3D Dirac cone in the reference's band frame
+    // (DiracSystem.cpp:143-153; kx = k_z, ky = kappa, field along z):
+    // d = -(A kappa/|k|, 0, |k| - A k_z/|k|)
     __device__ __forceinline__ void d_of(double A, double d[3]) const {
-        if constexpr (M == MODEL_DIRAC2D) {
+        if constexpr (M == MODEL_DIRAC3D) {
+            d[0] = -A * ky * ik;
+            d[1] = 0.0;
+            d[2] = A * kx * ik - cy;
+        } else if constexpr (M == MODEL_DIRAC2D) {
             d[0] = J.v * (kx - A * J.ct);
             d[1] = J.v * (ky - A * J.st);
             d[2] = J.mass;
@@ -189,7 +203,13 @@ struct Bloch {
         double d[3];
         d_of(0.0, d);
         const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        if constexpr (M == MODEL_HONEYCOMB) {
+        if constexpr (M == MODEL_DIRAC3D) {
+            // DiracSystem.cpp:155-158: alpha0 = f(E_F + |k|), beta0 = f(E_F - |k|)
+            const double a0 = fermi(J.fermi + cy, J.beta), b0 = fermi(J.fermi - cy, J.beta);
+            s[0] = 2.0 * a0 * b0;
+            s[1] = 0.0;
+            s[2] = a0 * a0 - b0 * b0;
+        } else if constexpr (M == MODEL_HONEYCOMB) {
             const double g = e / J.hop;
             const double a2 = fermi(J.fermi - J.hop * g, J.beta);
             const double b2 = fermi(J.fermi + J.hop * g, J.beta);
@@ -256,7 +276,9 @@ struct Bloch {
 
     // current vertex g(s, A) along the polarisation
     __device__ __forceinline__ double vertex(double A, const double s[3]) const {
-        if constexpr (M == MODEL_DIRAC2D) {
+        if constexpr (M == MODEL_DIRAC3D) {
+            return s[2];  // band inversion; the weight carries k_z kappa/|k| (DiracSystem.cpp:279)
+        } else if constexpr (M == MODEL_DIRAC2D) {
             return -J.v * (J.ct * s[0] + J.st * s[1]);
         } else if constexpr (M == MODEL_PIFLUX2D) {
             return sin(kx - A) * s[0];
@@ -477,20 +499,12 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
 }
 
 // ---------------------------------------------------------------------------
-// Magnus 4 (two Gauss nodes) with the closed-form SO(3) exponential
-// (Rodrigues, cf. sources/HHG/Systems/DiracDetail/MagnusMatrix.cpp:64-74, here
-// with the small-angle series instead of dividing by theta^2).
+// Rotations s <- exp(W) s, W v = O x v: Rodrigues' formula with the half-angle
+// form of 1 - cos and a small-angle series (cf. the reference's closed form,
+// sources/HHG/Systems/DiracDetail/MagnusMatrix.cpp:64-74, which divides by
+// theta^2 directly).
 // ---------------------------------------------------------------------------
-template <class F>
-__device__ __forceinline__ void magnus4_step(const F& f, double s[3], double A1, double A2, double h) {
-    double w1[3], w2[3];
-    f.d_of(A1, w1);
-    f.d_of(A2, w2);
-    const double a = h, b = (kSqrt3 / 12.0) * 4.0 * h * h;  // omega = 2 d -> w2 x w1 scales by 4
-    double O[3];
-    O[0] = a * (w1[0] + w2[0]) + b * (w2[1] * w1[2] - w2[2] * w1[1]);
-    O[1] = a * (w1[1] + w2[1]) + b * (w2[2] * w1[0] - w2[0] * w1[2]);
-    O[2] = a * (w1[2] + w2[2]) + b * (w2[0] * w1[1] - w2[1] * w1[0]);
+__device__ __forceinline__ void rotate_about(const double O[3], double s[3]) {
     const double th2 = O[0] * O[0] + O[1] * O[1] + O[2] * O[2];
     double sa, sb;  // sin(th)/th, (1 - cos th)/th^2
     if (th2 < 1e-2) {
@@ -514,6 +528,73 @@ __device__ __forceinline__ void magnus4_step(const F& f, double s[3], double A1,
     s[0] += sa * x0 + sb * y0;
     s[1] += sa * x1 + sb * y1;
     s[2] += sa * x2 + sb * y2;
+}
+
+// Magnus 4 (two Gauss nodes): Omega = h/2 (w1 + w2) + sqrt3/12 h^2 (w2 x w1), w = 2 d.
+template <class F>
+__device__ __forceinline__ void magnus4_step(const F& f, double s[3], double A1, double A2, double h) {
+    double w1[3], w2[3];
+    f.d_of(A1, w1);
+    f.d_of(A2, w2);
+    const double a = h, b = (kSqrt3 / 12.0) * 4.0 * h * h;  // omega = 2 d -> w2 x w1 scales by 4
+    double O[3];
+    O[0] = a * (w1[0] + w2[0]) + b * (w2[1] * w1[2] - w2[2] * w1[1]);
+    O[1] = a * (w1[1] + w2[1]) + b * (w2[2] * w1[0] - w2[0] * w1[2]);
+    O[2] = a * (w1[2] + w2[2]) + b * (w2[0] * w1[1] - w2[1] * w1[0]);
+    rotate_about(O, s);
+}
+
+// ---------------------------------------------------------------------------
+// The reference's Magnus step of the 3D Dirac cone, one per sampling interval
+// (DiracSystem::time_evolution_magnus, DiracSystem.cpp:176-211).  Per lane:
+// K = 2|k| h, Kp = 2 kappa h, Kz = 2 k_z h (Magnus(2|k|, 2 kappa, 2 k_z, h),
+// MagnusMatrix.cpp:12-23); per interval the four pulse moments divided by |k|
+// (al, be, ga, de).  Omega = [[0, a, b], [-a, 0, c], [-b, -c, 0]] with the
+// entries of MagnusMatrix.cpp:35-62, grouped here by q = Kz^2 + Kp^2; then
+// s <- exp(Omega) s, a rotation about (-c, b, -a).
+// ---------------------------------------------------------------------------
+struct MomentLane {
+    double K, K2, K3, K4, Kp, Kp2, Kz, Kz2, q, ik;
+};
+
+__device__ __forceinline__ MomentLane moment_lane(double kz, double kappa, double h) {
+    MomentLane L;
+    const double kk = sqrt(kz * kz + kappa * kappa);
+    L.ik = 1.0 / kk;
+    L.K = 2.0 * kk * h;
+    L.K2 = L.K * L.K;
+    L.K3 = L.K2 * L.K;
+    L.K4 = L.K2 * L.K2;
+    L.Kp = 2.0 * kappa * h;
+    L.Kp2 = L.Kp * L.Kp;
+    L.Kz = 2.0 * kz * h;
+    L.Kz2 = L.Kz * L.Kz;
+    L.q = L.Kz2 + L.Kp2;
+    return L;
+}
+
+__device__ __forceinline__ void magnus_moments_step(const MomentLane& L, const double mom[4], double s[3]) {
+    const double al = mom[0] * L.ik, be = mom[1] * L.ik, ga = mom[2] * L.ik, de = mom[3] * L.ik;
+    const double K = L.K, K2 = L.K2, K3 = L.K3, K4 = L.K4, Kp = L.Kp, Kp2 = L.Kp2, Kz = L.Kz, Kz2 = L.Kz2, q = L.q;
+    const double al2 = al * al, al3 = al2 * al, al4 = al2 * al2, be2 = be * be, be3 = be2 * be, ga2 = ga * ga;
+    const double a = al * Kz - K +
+                     Kp2 * ((al2 * be2 - al3 * ga) * K * q * (1.0 / 2520.0) + al2 * ga * K2 * Kz * (1.0 / 1260.0) -
+                            al * be2 * K2 * Kz * (1.0 / 1080.0) - al * ga * K3 * (1.0 / 2520.0) +
+                            be2 * K3 * (1.0 / 1890.0) + be2 * K * (1.0 / 60.0) - be * de * K * (1.0 / 420.0) +
+                            ga2 * K * (1.0 / 420.0));
+    const double b = K * Kp *
+                     (-al4 * be * q * q + 4.0 * al3 * be * K * Kz * q - 6.0 * al2 * be * K2 * Kz2 -
+                      2.0 * al2 * be * K2 * Kp2 + (-42.0 * al2 * be + 18.0 * al2 * de - 36.0 * al * be * ga + 18.0 * be3) * q +
+                      (4.0 * al * be * K3 + 84.0 * al * be * K - 36.0 * al * de * K + 36.0 * be * ga * K) * Kz -
+                      be * K4 - 42.0 * be * K2 - 2520.0 * be + 18.0 * de * K2) *
+                     (1.0 / 15120.0);
+    const double c = Kp *
+                     (3.0 * (al2 * be2 - al3 * ga) * K * Kz * q + 9.0 * al2 * ga * K2 * Kz2 + 3.0 * al2 * ga * K2 * Kp2 -
+                      6.0 * al * be2 * K2 * Kz2 + al * be2 * K2 * Kp2 - 9.0 * al * ga * K3 * Kz - 7560.0 * al +
+                      (3.0 * be2 * K3 + 126.0 * be2 * K - 18.0 * be * de * K + 18.0 * ga2 * K) * Kz + 3.0 * ga * K4) *
+                     (1.0 / 7560.0);
+    const double O[3] = {-c, b, -a};
+    rotate_about(O, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -85,6 +85,10 @@ void gauss_legendre(int n, double* x, double* w) {
 
 int64_t kgrid_size(const hhg_kgrid_desc& g) {
     if (g.kind == HHG_KGRID_EXPLICIT) return g.n_points;
+    if (g.kind == HHG_KGRID_DIRAC_CYLINDER) {
+        if (g.nx < 2 || g.ny < 1) return 0;
+        return (int64_t)(g.nx - 2) * (int64_t)g.ny;  // z = 1..nx-1 without nx/2
+    }
     return (int64_t)g.nx * (int64_t)g.ny;
 }
 
@@ -95,6 +99,30 @@ hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* 
         std::memcpy(kx, g.kx, sizeof(double) * g.n_points);
         std::memcpy(ky, g.ky, sizeof(double) * g.n_points);
         std::memcpy(w, g.w, sizeof(double) * g.n_points);
+        return HHG_SUCCESS;
+    }
+    if (g.kind == HHG_KGRID_DIRAC_CYLINDER) {
+        // DiracSystem::compute_current_density (DiracSystem.cpp:280-306): k_z = (z - nx/2) dk,
+        // z = 1..nx-1 except nx/2, dk = 2 max_k / nx; kappa on (0, sqrt(max_k^2 - k_z^2))
+        // (DiracSystem.cpp:259-263) by ny-point Gauss-Legendre (the reference refines an adaptive
+        // trapezoid there); integrand weight k_z kappa / |k| (DiracSystem.cpp:279)
+        if (g.nx < 4 || g.ny < 1 || !(g.x1 > 0.0))
+            return fail(HHG_ERROR_INVALID_ARGUMENT, "Dirac cylinder grid needs nx >= 4, ny >= 1, x1 (band width) > 0");
+        std::vector<double> gk(g.ny), wk(g.ny);
+        gauss_legendre(g.ny, gk.data(), wk.data());
+        const double max_k = g.x1, dz = 2.0 * max_k / g.nx;
+        int64_t k = 0;
+        for (int z = 1; z < g.nx; ++z) {
+            if (z == g.nx / 2) continue;
+            const double kz = (z - g.nx / 2) * dz;
+            const double kmax = std::sqrt(max_k * max_k - kz * kz);
+            for (int r = 0; r < g.ny; ++r, ++k) {
+                const double ka = 0.5 * kmax * (gk[r] + 1.0);
+                kx[k] = kz;
+                ky[k] = ka;
+                w[k] = dz * 0.5 * kmax * wk[r] * kz * ka / std::sqrt(kz * kz + ka * ka);
+            }
+        }
         return HHG_SUCCESS;
     }
     if (g.nx <= 0 || g.ny <= 0) return fail(HHG_ERROR_INVALID_ARGUMENT, "k-grid needs nx, ny > 0");
@@ -160,6 +188,7 @@ double pulse_eval(const hhg_pulse_desc& p, double t) {
         const double tb = t - p.delay;
         return p.amp_a * env_cos2_h(p, t) * std::sin(t + p.cep) + p.amp_b * env_cos2_h(p, tb) * std::sin(tb + p.cep);
     }
+    case HHG_PULSE_CONTINUOUS: return p.amp_a * std::sin(t + p.cep);  // ContinuousLaser.cpp:7-9
     }
     return NAN;
 }
@@ -257,6 +286,29 @@ hhg_status trapezoidal_spectrum(const double* j, const hhg_time_desc& tc, int ze
     return HHG_SUCCESS;
 }
 
+// The continuous-laser spectrum of sources/hhg.cpp:264-275: the first N samples
+// of j times the Welch window 1 - (i - N/2)^2 / (N/2)^2 (WelchWindow.hpp:11-13),
+// real-to-complex DFT X_k = sum_i x_i exp(-2 pi i k i / N), k = 0..N/2 (FFTW r2c,
+// FFT.cpp:8-42), and the reference's frequency axis k / Delta (hhg.cpp:272-274).
+hhg_status welch_spectrum(const double* j, const hhg_time_desc& tc, double* freq, double* re, double* im) {
+    const int N = tc.n_measurements;
+    if (N < 2 || !(tc.t_end > tc.t_begin))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "spectrum needs n_measurements >= 2, t_end > t_begin");
+    std::vector<double> x(N);
+    for (int i = 0; i < N; ++i) {
+        const double u = (double)i - 0.5 * N;
+        x[i] = j[i] * (1.0 - u * u / (0.25 * (double)N * (double)N));
+    }
+    const std::vector<cplx> X = dft_any(x);
+    const double delta = (tc.t_end - tc.t_begin) / N;
+    for (int k = 0; k <= N / 2; ++k) {
+        freq[k] = (double)k / delta;
+        re[k] = X[k].real();
+        im[k] = X[k].imag();
+    }
+    return HHG_SUCCESS;
+}
+
 }  // namespace hhg
 
 // ---------------------------------------------------------------------------
@@ -286,7 +338,7 @@ hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, doub
 hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n) {
     hhg::clear_error();
     if (!p || (n > 0 && (!t || !A))) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_pulse_evaluate: null argument");
-    if (p->kind < 0 || p->kind > 3) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "unknown pulse kind");
+    if (p->kind < 0 || p->kind > 4) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "unknown pulse kind");
     for (int64_t i = 0; i < n; ++i) A[i] = hhg::pulse_eval(*p, t[i]);
     return HHG_SUCCESS;
 }
@@ -297,6 +349,12 @@ hhg_status hhg_trapezoidal_spectrum(const double* j, const hhg_time_desc* time, 
     if (!j || !time || !freq || !re || !im)
         return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_trapezoidal_spectrum: null argument");
     return hhg::trapezoidal_spectrum(j, *time, zero_padding, freq, re, im);
+}
+
+hhg_status hhg_welch_spectrum(const double* j, const hhg_time_desc* time, double* freq, double* re, double* im) {
+    hhg::clear_error();
+    if (!j || !time || !freq || !re || !im) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_welch_spectrum: null argument");
+    return hhg::welch_spectrum(j, *time, freq, re, im);
 }
 
 void hhg_accumulators_to_double(const uint64_t* acc, int64_t n, double* j) {

@@ -7,7 +7,8 @@
 //   items (int2)     : work items {job, chunk}; one item = 64 consecutive
 //                      k-points of one job = one wavefront
 //   At               : A(t_i), i = 0..N, per job (vertex evaluation)
-//   Am               : A at the two Gauss nodes of every Magnus substep
+//   Am               : A at the two Gauss nodes of every Magnus substep, or the
+//                      four pulse moments of every interval (moment Magnus)
 //   partial          : one row of N+1 wave sums per item, per job contiguous
 //   seg_sum/seg_comp : level-1 compensated sums over item segments
 //   j                : the result, sum over jobs of N+1 samples
@@ -16,8 +17,16 @@
 
 namespace hhg {
 
-enum : int { MODEL_DIRAC2D = 0, MODEL_PIFLUX2D = 1, MODEL_HONEYCOMB = 2, N_MODELS = 3 };
-enum : int { PULSE_COS2 = 0, PULSE_GAUSS = 1, PULSE_DOUBLE_GAUSS = 2, PULSE_DOUBLE_COS2 = 3, N_PULSES = 4 };
+enum : int { MODEL_DIRAC2D = 0, MODEL_PIFLUX2D = 1, MODEL_HONEYCOMB = 2, MODEL_DIRAC3D = 3, N_MODELS = 4 };
+enum : int {
+    PULSE_COS2 = 0,
+    PULSE_GAUSS = 1,
+    PULSE_DOUBLE_GAUSS = 2,
+    PULSE_DOUBLE_COS2 = 3,
+    PULSE_CONTINUOUS = 4,
+    N_PULSES = 5
+};
+enum : int { METHOD_RKF78 = 0, METHOD_MAGNUS4 = 1, METHOD_MAGNUS_MOMENTS = 2 };
 enum : int { RELAX_NONE = 0, RELAX_TO_INITIAL = 1, RELAX_T2 = 2 };
 enum : int { PROPOSAL_RESTART = 0, PROPOSAL_CARRY = 1, PROPOSAL_LEARNED = 2 };
 

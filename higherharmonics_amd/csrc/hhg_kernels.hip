@@ -2,18 +2,45 @@
 // dispatch onto the per-model solve kernels (hhg_solve_<model>.hip).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hhg_device.hpp"
 #include "hhg_kernels.hpp"
 
 namespace hhg {
 
-// A(t_i) for the vertex and A at the two Gauss nodes of every Magnus substep.
+// A(t_i) for the vertex; with magnus = 1 A at the two Gauss nodes of every
+// Magnus substep; with magnus = 2 the four pulse moments of every sampling
+// interval for the reference's Dirac Magnus step (Laser::magnus_coefficients,
+// sources/HHG/Laser/Laser.cpp:40-55): 30-point Gauss-Legendre on [0, 1] of
+// A(t_{i-1} + Delta x) against 1, 2x - 1, 6x(x - 1) + 1 and
+// 20x^3 - 30x^2 + 12x + 1 (sources/HHG/Laser/gauss.hpp:82-102; the last is the
+// shifted Legendre P3 with the sign of its constant term flipped, evaluated as
+// the reference evaluates it), scaled by 1, 3, 5, 7.
 template <int P>
-__global__ void pulse_table_kernel(const JobDev* jobs, int job, double* At, double* Am, int with_magnus) {
+__global__ void pulse_table_kernel(const JobDev* jobs, int job, double* At, double* Am, int with_magnus, Gauss30 g) {
     JobRef J = job_ref(jobs + job);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= J.n_meas) At[J.sample_base + i] = pulse_A<P>(J, J.t_begin + (double)i * J.delta);
-    if (with_magnus) {
+    if (with_magnus == 2) {
+        if (i < J.n_meas) {
+            const double t0 = J.t_begin + (double)i * J.delta;
+            double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+            for (int q = 0; q < 30; ++q) {
+                const double x = g.x[q];
+                const double a = g.w[q] * pulse_A<P>(J, t0 + J.delta * x);
+                m0 += a;
+                m1 += a * (2.0 * x - 1.0);
+                m2 += a * (6.0 * x * (x - 1.0) + 1.0);
+                m3 += a * (20.0 * x * x * x - 30.0 * x * x + 12.0 * x + 1.0);
+            }
+            double* out = Am + J.magnus_base + 4 * i;
+            out[0] = m0;
+            out[1] = 3.0 * m1;
+            out[2] = 5.0 * m2;
+            out[3] = 7.0 * m3;
+        }
+    } else if (with_magnus) {
         const int ms = J.magnus_sub;
         if (i < (int64_t)J.n_meas * ms) {
             const int64_t iv = i / ms, q = i % ms;
@@ -78,92 +105,71 @@ __global__ void finalize_kernel(const unsigned long long* acc, int64_t n, double
 // ---------------------------------------------------------------------------
 // Run-time model dispatch
 // ---------------------------------------------------------------------------
-hipError_t launch_solve(int method, int model, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
-                        int threads, size_t smem, hipStream_t st) {
+template <class F>
+hipError_t by_model(int model, F&& f) {
     switch (model) {
-    case MODEL_DIRAC2D: return solve_model<MODEL_DIRAC2D>(method, pulse, relax, learned, a, blocks, threads, smem, st, nullptr);
-    case MODEL_PIFLUX2D: return solve_model<MODEL_PIFLUX2D>(method, pulse, relax, learned, a, blocks, threads, smem, st, nullptr);
-    case MODEL_HONEYCOMB: return solve_model<MODEL_HONEYCOMB>(method, pulse, relax, learned, a, blocks, threads, smem, st, nullptr);
+    case MODEL_DIRAC2D: return f(std::integral_constant<int, MODEL_DIRAC2D>{});
+    case MODEL_PIFLUX2D: return f(std::integral_constant<int, MODEL_PIFLUX2D>{});
+    case MODEL_HONEYCOMB: return f(std::integral_constant<int, MODEL_HONEYCOMB>{});
+    case MODEL_DIRAC3D: return f(std::integral_constant<int, MODEL_DIRAC3D>{});
     }
     return hipErrorInvalidValue;
+}
+
+template <class F>
+hipError_t by_pulse(int pulse, F&& f) {
+    switch (pulse) {
+    case PULSE_COS2: return f(std::integral_constant<int, PULSE_COS2>{});
+    case PULSE_GAUSS: return f(std::integral_constant<int, PULSE_GAUSS>{});
+    case PULSE_DOUBLE_GAUSS: return f(std::integral_constant<int, PULSE_DOUBLE_GAUSS>{});
+    case PULSE_DOUBLE_COS2: return f(std::integral_constant<int, PULSE_DOUBLE_COS2>{});
+    case PULSE_CONTINUOUS: return f(std::integral_constant<int, PULSE_CONTINUOUS>{});
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_solve(int method, int model, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
+                        int threads, size_t smem, hipStream_t st) {
+    return by_model(model, [&](auto M) {
+        return solve_model<decltype(M)::value>(method, pulse, relax, learned, a, blocks, threads, smem, st, nullptr);
+    });
 }
 
 const void* solve_kernel_fn(int method, int model, int pulse, int relax, bool learned) {
     const void* fn = nullptr;
     KernelArgs dummy{};
-    hipError_t e = hipErrorInvalidValue;
-    switch (model) {
-    case MODEL_DIRAC2D: e = solve_model<MODEL_DIRAC2D>(method, pulse, relax, learned, dummy, 0, 0, 0, nullptr, &fn); break;
-    case MODEL_PIFLUX2D: e = solve_model<MODEL_PIFLUX2D>(method, pulse, relax, learned, dummy, 0, 0, 0, nullptr, &fn); break;
-    case MODEL_HONEYCOMB: e = solve_model<MODEL_HONEYCOMB>(method, pulse, relax, learned, dummy, 0, 0, 0, nullptr, &fn); break;
-    }
+    const hipError_t e = by_model(model, [&](auto M) {
+        return solve_model<decltype(M)::value>(method, pulse, relax, learned, dummy, 0, 0, 0, nullptr, &fn);
+    });
     return e == hipSuccess ? fn : nullptr;
 }
 
 hipError_t launch_trajectories(int method, int model, int pulse, int relax, const TrajArgs& x) {
-    switch (model) {
-    case MODEL_DIRAC2D: return traj_model<MODEL_DIRAC2D>(method, pulse, relax, x);
-    case MODEL_PIFLUX2D: return traj_model<MODEL_PIFLUX2D>(method, pulse, relax, x);
-    case MODEL_HONEYCOMB: return traj_model<MODEL_HONEYCOMB>(method, pulse, relax, x);
-    }
-    return hipErrorInvalidValue;
+    return by_model(model, [&](auto M) { return traj_model<decltype(M)::value>(method, pulse, relax, x); });
 }
 
 hipError_t launch_cost_probe(int model, int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem,
                              hipStream_t st) {
-    switch (model) {
-    case MODEL_DIRAC2D: return cost_model<MODEL_DIRAC2D>(pulse, a, n_probe, cost, smem, st);
-    case MODEL_PIFLUX2D: return cost_model<MODEL_PIFLUX2D>(pulse, a, n_probe, cost, smem, st);
-    case MODEL_HONEYCOMB: return cost_model<MODEL_HONEYCOMB>(pulse, a, n_probe, cost, smem, st);
-    }
-    return hipErrorInvalidValue;
+    return by_model(model, [&](auto M) { return cost_model<decltype(M)::value>(pulse, a, n_probe, cost, smem, st); });
 }
 
 hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Am,
-                              int with_magnus, hipStream_t st) {
+                              int with_magnus, const Gauss30& g, hipStream_t st) {
     const int blocks = (int)((n_threads + 255) / 256);
-    switch (pulse) {
-    case PULSE_COS2:
-        hipLaunchKernelGGL((pulse_table_kernel<PULSE_COS2>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Am, with_magnus);
-        break;
-    case PULSE_GAUSS:
-        hipLaunchKernelGGL((pulse_table_kernel<PULSE_GAUSS>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Am, with_magnus);
-        break;
-    case PULSE_DOUBLE_GAUSS:
-        hipLaunchKernelGGL((pulse_table_kernel<PULSE_DOUBLE_GAUSS>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Am,
-                           with_magnus);
-        break;
-    case PULSE_DOUBLE_COS2:
-        hipLaunchKernelGGL((pulse_table_kernel<PULSE_DOUBLE_COS2>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Am,
-                           with_magnus);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return by_pulse(pulse, [&](auto P) {
+        hipLaunchKernelGGL((pulse_table_kernel<decltype(P)::value>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Am,
+                           with_magnus, g);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_nominal_table(int pulse, const JobDev* jobs, int job, int64_t n_meas, double* tab,
                                 hipStream_t st) {
     const int blocks = (int)((n_meas + 127) / 128);
-    switch (pulse) {
-    case PULSE_COS2:
-        hipLaunchKernelGGL((nominal_table_kernel<PULSE_COS2>), dim3(blocks), dim3(128), 0, st, jobs, job, tab);
-        break;
-    case PULSE_GAUSS:
-        hipLaunchKernelGGL((nominal_table_kernel<PULSE_GAUSS>), dim3(blocks), dim3(128), 0, st, jobs, job, tab);
-        break;
-    case PULSE_DOUBLE_GAUSS:
-        hipLaunchKernelGGL((nominal_table_kernel<PULSE_DOUBLE_GAUSS>), dim3(blocks), dim3(128), 0, st, jobs, job,
-                           tab);
-        break;
-    case PULSE_DOUBLE_COS2:
-        hipLaunchKernelGGL((nominal_table_kernel<PULSE_DOUBLE_COS2>), dim3(blocks), dim3(128), 0, st, jobs, job, tab);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return by_pulse(pulse, [&](auto P) {
+        hipLaunchKernelGGL((nominal_table_kernel<decltype(P)::value>), dim3(blocks), dim3(128), 0, st, jobs, job, tab);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_clear(unsigned long long* p, int64_t n_words, hipStream_t st) {

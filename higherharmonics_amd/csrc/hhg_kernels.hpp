@@ -61,8 +61,13 @@ const void* solve_kernel_fn(int method, int model, int pulse, int relax, bool le
 hipError_t launch_trajectories(int method, int model, int pulse, int relax, const TrajArgs& x);
 hipError_t launch_cost_probe(int model, int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem,
                              hipStream_t st);
+// 30-point Gauss-Legendre nodes and weights on [0, 1] (moment Magnus tables)
+struct Gauss30 {
+    double x[30], w[30];
+};
+
 hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Am,
-                              int with_magnus, hipStream_t st);
+                              int with_magnus, const Gauss30& g, hipStream_t st);
 hipError_t launch_nominal_table(int pulse, const JobDev* jobs, int job, int64_t n_meas, double* tab,
                                 hipStream_t st);
 hipError_t launch_clear(unsigned long long* p, int64_t n_words, hipStream_t st);

@@ -100,26 +100,31 @@ hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n
     const hhg_pulse_desc& p = jd.pulse;
     const hhg_time_desc& t = jd.time;
     if (jd.kgrid < 0 || jd.kgrid >= n_kgrids) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "k-grid index out of range");
-    if (m.kind < 0 || m.kind > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown model kind");
+    if (m.kind < 0 || m.kind > 3) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown model kind");
     if (m.relax < 0 || m.relax > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown relaxation kind");
     if (m.relax != HHG_RELAX_NONE && !(m.relax_time > 0.0))
         return fail(HHG_ERROR_INVALID_ARGUMENT, at + "relaxation needs relax_time > 0");
-    if (m.kind != HHG_MODEL_DIRAC2D && m.theta != 0.0)
+    const bool lattice = m.kind == HHG_MODEL_PIFLUX2D || m.kind == HHG_MODEL_HONEYCOMB;
+    if (lattice && m.theta != 0.0)
         return fail(HHG_ERROR_UNSUPPORTED, at + "lattice models take the field along x (theta = 0)");
-    if (m.kind != HHG_MODEL_DIRAC2D && !(m.hopping > 0.0))
-        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "lattice models need hopping > 0");
-    if (p.kind < 0 || p.kind > 3) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown pulse kind");
+    if (m.kind == HHG_MODEL_DIRAC3D && m.theta != 0.0)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D Dirac cone takes the field along z (theta = 0)");
+    if (lattice && !(m.hopping > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "lattice models need hopping > 0");
+    if (p.kind < 0 || p.kind > 4) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown pulse kind");
     const bool gauss = p.kind == HHG_PULSE_GAUSS || p.kind == HHG_PULSE_DOUBLE_GAUSS;
+    const bool cos2 = p.kind == HHG_PULSE_COS2 || p.kind == HHG_PULSE_DOUBLE_COS2;
     if (gauss && !(p.sigma > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "Gaussian pulse needs sigma > 0");
-    if (!gauss && !(p.n_cycles > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "cos^2 pulse needs n_cycles > 0");
+    if (cos2 && !(p.n_cycles > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "cos^2 pulse needs n_cycles > 0");
     if (t.n_measurements < 1 || t.n_subdivisions < 1 || !(t.t_end > t.t_begin))
         return fail(HHG_ERROR_INVALID_ARGUMENT, at + "time grid needs N >= 1, n_subdivisions >= 1, t_end > t_begin");
     const double delta = (t.t_end - t.t_begin) / t.n_measurements;
-    if (delta > 0.25 || (!gauss && delta / p.n_cycles > 0.25))
+    if (delta > 0.25 || (cos2 && delta / p.n_cycles > 0.25))
         return fail(HHG_ERROR_UNSUPPORTED,
                     at + "sampling interval must not exceed 0.25 (>= 26 samples per optical cycle)");
-    if (sv.method == HHG_METHOD_MAGNUS4 && m.relax != HHG_RELAX_NONE)
+    if (sv.method != HHG_METHOD_RKF78 && m.relax != HHG_RELAX_NONE)
         return fail(HHG_ERROR_UNSUPPORTED, at + "the Magnus step is unitary: relaxation needs RKF78");
+    if (sv.method == HHG_METHOD_MAGNUS_MOMENTS && m.kind != HHG_MODEL_DIRAC3D)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the moment Magnus step is the 3D Dirac cone's (HHG_MODEL_DIRAC3D)");
     return HHG_SUCCESS;
 }
 
@@ -330,7 +335,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     const int n_ranks = d->n_ranks > 0 ? d->n_ranks : 1;
     if (d->rank < 0 || d->rank >= n_ranks) return fail(HHG_ERROR_INVALID_ARGUMENT, "rank outside [0, n_ranks)");
     const hhg_solver_desc& sv = d->solver;
-    if (sv.method != HHG_METHOD_RKF78 && sv.method != HHG_METHOD_MAGNUS4)
+    if (sv.method != HHG_METHOD_RKF78 && sv.method != HHG_METHOD_MAGNUS4 && sv.method != HHG_METHOD_MAGNUS_MOMENTS)
         return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown method");
     if (sv.method == HHG_METHOD_RKF78 && (!(sv.rtol >= 0.0) || !(sv.atol >= 0.0) || !(sv.rtol + sv.atol > 0.0)))
         return fail(HHG_ERROR_INVALID_ARGUMENT, "RKF78 needs rtol, atol >= 0, not both 0");
@@ -400,7 +405,8 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         J.sample_base = p->offsets[i];
         p->offsets[i + 1] = p->offsets[i] + J.n_meas + 1;
         J.magnus_base = magnus_words;
-        magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
+        if (sv.method == HHG_METHOD_MAGNUS4) magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
+        if (sv.method == HHG_METHOD_MAGNUS_MOMENTS) magnus_words += 4 * (int64_t)J.n_meas;
         J.nominal_base = nominal_words;
         if (sv.method == HHG_METHOD_RKF78) nominal_words += (int64_t)J.n_meas * hhg::kNomRow;
         J.row_base = 0;
@@ -412,7 +418,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     std::map<std::tuple<int, int, int>, std::vector<int>> by_key;
     for (int i = 0; i < d->n_jobs; ++i) {
         const auto& J = p->jobs[i];
-        const int pulse = sv.method == HHG_METHOD_MAGNUS4 ? 0 : J.pulse;
+        const int pulse = sv.method != HHG_METHOD_RKF78 ? 0 : J.pulse;  // Magnus kernels read only tables
         by_key[{J.model, pulse, J.relax}].push_back(i);
     }
     std::vector<int2> items;
@@ -460,11 +466,19 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     C(p->d_jobs, p->jobs.data(), sizeof(hhg::JobDev) * p->jobs.size(), "upload jobs");
     C(p->d_items, items.data(), sizeof(int2) * items.size(), "upload items");
     if (st != HHG_SUCCESS) return bail(st);
+    hhg::Gauss30 g30{};
+    {
+        double xi[30], wi[30];
+        hhg::gauss_legendre(30, xi, wi);
+        for (int q = 0; q < 30; ++q) g30.x[q] = 0.5 * (xi[q] + 1.0), g30.w[q] = 0.5 * wi[q];
+    }
+    const int with_magnus = sv.method == HHG_METHOD_MAGNUS4 ? 1 : sv.method == HHG_METHOD_MAGNUS_MOMENTS ? 2 : 0;
     for (int i = 0; i < d->n_jobs; ++i) {
         const auto& J = p->jobs[i];
         const int64_t n = std::max<int64_t>(J.n_meas + 1, (int64_t)J.n_meas * J.magnus_sub);
-        st = check_hip(hhg::launch_pulse_table(J.pulse, p->d_jobs, i, n, p->d_At, p->d_Am, J.magnus_sub > 0, p->stream),
-                       "pulse table");
+        st = check_hip(
+            hhg::launch_pulse_table(J.pulse, p->d_jobs, i, n, p->d_At, p->d_Am, with_magnus, g30, p->stream),
+            "pulse table");
         if (st != HHG_SUCCESS) return bail(st);
         if (p->d_nom) {
             st = check_hip(hhg::launch_nominal_table(J.pulse, p->d_jobs, i, J.n_meas, p->d_nom, p->stream),

@@ -241,6 +241,39 @@ __device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, Lane
     if (sink.overflow) lc.nonfin++;
 }
 
+// The reference's moment-based Magnus step of the 3D Dirac cone, one per
+// sampling interval; Am holds the four pulse moments of every interval.
+template <int M>
+__device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t kk = (int64_t)chunk * WAVE + lane;
+    const bool active = kk < J.n_k;
+    double kx = 1.0, ky = 1.0, w = 0.0;  // idle lanes evolve a harmless point with weight 0
+    if (active) {
+        const int64_t idx = J.k_base + (a.perm ? a.perm[J.k_base + kk] : kk);
+        kx = a.kx[idx];
+        ky = a.ky[idx];
+        w = a.w[idx];
+    }
+    Bloch<M, PULSE_CONTINUOUS, RELAX_NONE> f(J, kx, ky);  // the pulse enters only through the moments
+    double s[3];
+    f.initial(s);
+    const MomentLane L = moment_lane(kx, ky, J.delta);
+    const int N = J.n_meas;
+    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
+    const double* Am = a.Am + J.magnus_base;
+    sink.put(0, wave_sum(active ? w * s[2] : 0.0), lane);
+    for (int i = 1; i <= N; ++i) {
+        const double* m4 = Am + 4 * (int64_t)(i - 1);
+        const double mom[4] = {m4[0], m4[1], m4[2], m4[3]};
+        magnus_moments_step(L, mom, s);
+        sink.put(i, wave_sum(active ? w * s[2] : 0.0), lane);
+    }
+    if (active) lc.att += (uint32_t)N;
+    if (active && !(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) lc.nonfin++;
+    if (sink.overflow) lc.nonfin++;
+}
+
 }  // namespace solve_detail
 
 // One kernel per (method, model, pulse, relaxation, learned); a launch only
@@ -266,10 +299,12 @@ solve_kernel(KernelArgs a) {
         const int end = min(base + a.queue_chunk, a.n_items);
         for (int it = base; it < end; ++it) {
             const int2 item = load_uniform(a.items + it);
-            if constexpr (METHOD == 0)
+            if constexpr (METHOD == METHOD_RKF78)
                 solve_item_rkf78<M, P, R, LEARNED>(job_ref(a.jobs + item.x), item.y, a, lc);
-            else
+            else if constexpr (METHOD == METHOD_MAGNUS4)
                 solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc);
+            else
+                solve_item_magnus_moments<M>(job_ref(a.jobs + item.x), item.y, a, lc);
         }
     }
     flush_counters(a.ctr, lc);
@@ -319,7 +354,7 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
                 h_carry = h > unclipped ? h : unclipped;
             }
             while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
-        } else {
+        } else if constexpr (METHOD == 1) {
             const int ms = J.magnus_sub;
             const double h = J.delta / ms;
             for (int q = 0; q < ms; ++q) {
@@ -327,6 +362,11 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
                 magnus4_step(f, s, Am[m], Am[m + 1], h);
             }
             att += ms;
+        } else {
+            const double* m4 = Am + J.magnus_base + 4 * (int64_t)(i - 1);
+            const double mom[4] = {m4[0], m4[1], m4[2], m4[3]};
+            magnus_moments_step(moment_lane(kx[idx], ky[idx], J.delta), mom, s);
+            att += 1;
         }
         out[3 * i + 0] = s[0];
         out[3 * i + 1] = s[1];
@@ -401,7 +441,7 @@ struct ModelLaunch {
     }
     static hipError_t solve(int method, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
                             int threads, size_t smem, hipStream_t st, const void** fn) {
-        if (method == 1) {
+        if (method == METHOD_MAGNUS4) {
             if (fn) {
                 *fn = reinterpret_cast<const void*>(&solve_kernel<1, M, PULSE_COS2, RELAX_NONE, false>);
                 return hipSuccess;
@@ -410,11 +450,24 @@ struct ModelLaunch {
                                a);
             return hipGetLastError();
         }
+        if (method == METHOD_MAGNUS_MOMENTS) {
+            if constexpr (M == MODEL_DIRAC3D) {
+                if (fn) {
+                    *fn = reinterpret_cast<const void*>(&solve_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE, false>);
+                    return hipSuccess;
+                }
+                hipLaunchKernelGGL((solve_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE, false>), dim3(blocks),
+                                   dim3(threads), 0, st, a);
+                return hipGetLastError();
+            }
+            return hipErrorInvalidValue;
+        }
         switch (pulse) {
         case PULSE_COS2: return solve_r<PULSE_COS2>(relax, learned, a, blocks, threads, smem, st, fn);
         case PULSE_GAUSS: return solve_r<PULSE_GAUSS>(relax, learned, a, blocks, threads, smem, st, fn);
         case PULSE_DOUBLE_GAUSS: return solve_r<PULSE_DOUBLE_GAUSS>(relax, learned, a, blocks, threads, smem, st, fn);
         case PULSE_DOUBLE_COS2: return solve_r<PULSE_DOUBLE_COS2>(relax, learned, a, blocks, threads, smem, st, fn);
+        case PULSE_CONTINUOUS: return solve_r<PULSE_CONTINUOUS>(relax, learned, a, blocks, threads, smem, st, fn);
         }
         return hipErrorInvalidValue;
     }
@@ -422,7 +475,13 @@ struct ModelLaunch {
     template <int P, int R>
     static hipError_t traj_t(int method, const TrajArgs& x) {
         const int blocks = (int)((x.n_sel + 63) / 64);
-        if (method == 1)
+        if (method == METHOD_MAGNUS_MOMENTS) {
+            if constexpr (M == MODEL_DIRAC3D)
+                hipLaunchKernelGGL((trajectory_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE>), dim3(blocks), dim3(64), 0,
+                                   x.st, x.jobs, x.kx, x.ky, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
+            else
+                return hipErrorInvalidValue;
+        } else if (method == METHOD_MAGNUS4)
             hipLaunchKernelGGL((trajectory_kernel<1, M, PULSE_COS2, RELAX_NONE>), dim3(blocks), dim3(64), 0, x.st,
                                x.jobs, x.kx, x.ky, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
         else
@@ -445,6 +504,7 @@ struct ModelLaunch {
         case PULSE_GAUSS: return traj_r<PULSE_GAUSS>(method, relax, x);
         case PULSE_DOUBLE_GAUSS: return traj_r<PULSE_DOUBLE_GAUSS>(method, relax, x);
         case PULSE_DOUBLE_COS2: return traj_r<PULSE_DOUBLE_COS2>(method, relax, x);
+        case PULSE_CONTINUOUS: return traj_r<PULSE_CONTINUOUS>(method, relax, x);
         }
         return hipErrorInvalidValue;
     }
@@ -466,6 +526,10 @@ struct ModelLaunch {
             break;
         case PULSE_DOUBLE_COS2:
             hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_DOUBLE_COS2>), dim3(blocks), dim3(256), smem, st, a,
+                               n_probe, cost_out);
+            break;
+        case PULSE_CONTINUOUS:
+            hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_CONTINUOUS>), dim3(blocks), dim3(256), smem, st, a,
                                n_probe, cost_out);
             break;
         default:

@@ -57,7 +57,10 @@ typedef enum hhg_status {
 typedef enum hhg_model_kind {
     HHG_MODEL_DIRAC2D = 0,   /* d = (v qx, v qy, mass) */
     HHG_MODEL_PIFLUX2D = 1,  /* d = 2t (cos qx, cos qy, 0); cf. PiFlux.cpp:89-94 */
-    HHG_MODEL_HONEYCOMB = 2  /* d = t (Re g, -Im g, 0), g = sum_delta e^{i q.delta}; Honeycomb.cpp:44-50,130-138 */
+    HHG_MODEL_HONEYCOMB = 2, /* d = t (Re g, -Im g, 0), g = sum_delta e^{i q.delta}; Honeycomb.cpp:44-50,130-138 */
+    HHG_MODEL_DIRAC3D = 3    /* the reference's 3D Dirac cone in its band frame, field along z, point (k_z, kappa)
+                                in (kx, ky): d = -(A kappa/|k|, 0, |k| - A k_z/|k|), DiracSystem.cpp:138-174;
+                                current vertex s_z; lengths in hbar omega_L / v_F */
 } hhg_model_kind;
 
 typedef enum hhg_relax_kind {
@@ -83,7 +86,8 @@ typedef enum hhg_pulse_kind {
     HHG_PULSE_COS2 = 0,         /* CosineLaser.cpp:22-26, window (t0, t0 + 2 pi n) */
     HHG_PULSE_GAUSS = 1,        /* GaussLaser.cpp:26-28, centre t0, width sigma */
     HHG_PULSE_DOUBLE_GAUSS = 2, /* DoubleLaser<GaussLaser>: A(t) + B(t - delay), DoubleLaser.hpp:63-67 */
-    HHG_PULSE_DOUBLE_COS2 = 3   /* DoubleLaser<CosineLaser> */
+    HHG_PULSE_DOUBLE_COS2 = 3,  /* DoubleLaser<CosineLaser> */
+    HHG_PULSE_CONTINUOUS = 4    /* ContinuousLaser.cpp:7-9, envelope 1 */
 } hhg_pulse_kind;
 
 typedef struct hhg_pulse_desc {
@@ -111,7 +115,11 @@ typedef struct hhg_time_desc {
 typedef enum hhg_kgrid_kind {
     HHG_KGRID_GL_BOX = 0,  /* nx x ny Gauss-Legendre on [x0,x1] x [y0,y1], x-major */
     HHG_KGRID_GL_HEX = 1,  /* hexagonal BZ of Honeycomb.cpp:229-268 (nx x ny nodes) */
-    HHG_KGRID_EXPLICIT = 2 /* caller's points and weights */
+    HHG_KGRID_EXPLICIT = 2, /* caller's points and weights */
+    HHG_KGRID_DIRAC_CYLINDER = 3 /* (k_z, kappa) of the 3D Dirac cone: k_z = (z - nx/2) dk, z = 1..nx-1 except nx/2,
+                                    dk = 2 x1 / nx (DiracSystem.cpp:280-306); kappa: ny-point Gauss-Legendre on
+                                    (0, sqrt(x1^2 - k_z^2)) where the reference refines an adaptive trapezoid;
+                                    weight dk w_kappa k_z kappa / |k| (DiracSystem.cpp:279); x1 = band width */
 } hhg_kgrid_kind;
 
 typedef struct hhg_kgrid_desc {
@@ -135,7 +143,13 @@ typedef struct hhg_job_desc {
     int32_t pad;
 } hhg_job_desc;
 
-typedef enum hhg_method { HHG_METHOD_RKF78 = 0, HHG_METHOD_MAGNUS4 = 1 } hhg_method;
+typedef enum hhg_method {
+    HHG_METHOD_RKF78 = 0,
+    HHG_METHOD_MAGNUS4 = 1,        /* two-node Gauss Magnus, magnus_substeps per sampling interval */
+    HHG_METHOD_MAGNUS_MOMENTS = 2  /* the reference's Dirac Magnus step: one per sampling interval from four
+                                      30-point moments of A (Laser.cpp:40-55, MagnusMatrix.cpp:12-76,
+                                      DiracSystem.cpp:176-211); HHG_MODEL_DIRAC3D without relaxation only */
+} hhg_method;
 
 typedef enum hhg_proposal {
     HHG_PROPOSAL_RESTART = 0, /* reference: every interval starts at dt = Delta/n_sub */
@@ -252,6 +266,11 @@ hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* 
  * entries.  Keeps the reference's conventions (see hhg_amd spectrum notes). */
 hhg_status hhg_trapezoidal_spectrum(const double* j, const hhg_time_desc* time, int32_t zero_padding,
                                     double* freq, double* re, double* im);
+
+/* Spectrum of a continuous-laser run (sources/hhg.cpp:264-275): the first N
+ * samples of j times the Welch window (WelchWindow.hpp:11-13), real-to-complex
+ * DFT (FFTW r2c, FFT.cpp:8-42), frequencies k / Delta; outputs N/2 + 1 entries. */
+hhg_status hhg_welch_spectrum(const double* j, const hhg_time_desc* time, double* freq, double* re, double* im);
 
 /* ---- step logs (training data for the proposal network) ---- */
 

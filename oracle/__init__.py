@@ -23,10 +23,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
-MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB = 0, 1, 2
-PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2 = 0, 1, 2, 3
+MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D = 0, 1, 2, 3
+PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS = 0, 1, 2, 3, 4
 RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2 = 0, 1, 2
-METHOD_RKF78, METHOD_MAGNUS4 = 0, 1
+METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY = 0, 1
 
 
@@ -85,6 +85,8 @@ def lib(fast: bool = False) -> ctypes.CDLL:
                          ctypes.c_double, dp]
     L.or_vertex.restype = ctypes.c_double
     L.or_vertex.argtypes = [ctypes.POINTER(Model), ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+    L.or_magnus_moments.argtypes = [ctypes.POINTER(Pulse), ctypes.c_double, ctypes.c_double, dp]
+    L.or_magnus_moments_step.argtypes = [ctypes.c_double, ctypes.c_double, dp, ctypes.c_double, dp]
     L.or_trajectory.restype = ctypes.c_int
     L.or_trajectory.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
                                 ctypes.POINTER(Solver), ctypes.c_double, ctypes.c_double, dp, ctypes.POINTER(Stats)]
@@ -182,9 +184,45 @@ def current_density(m: Model, p: Pulse, tc: Time, sv: Solver, kx, ky, w, n_threa
     return j, st
 
 
+def magnus_moments(p: Pulse, t0: float, h: float) -> np.ndarray:
+    out = np.zeros(4)
+    lib().or_magnus_moments(ctypes.byref(p), t0, h, _dp(out))
+    return out
+
+
+def magnus_moments_step(kz: float, kappa: float, mom, h: float, s) -> np.ndarray:
+    mom = np.ascontiguousarray(mom, dtype=np.float64)
+    out = np.array(s, dtype=np.float64)
+    lib().or_magnus_moments_step(kz, kappa, _dp(mom), h, _dp(out))
+    return out
+
+
 # --------------------------------------------------------------------------
 # Quadrature grids (restating the reference's GL mappings)
 # --------------------------------------------------------------------------
+
+def dirac_cylinder(n_z: int, n_kappa: int, max_k: float):
+    """(k_z, kappa) points of the 3D Dirac cone with the field along z.
+    k_z on the uniform grid of DiracSystem::compute_current_density
+    (sources/HHG/Systems/DiracSystem.cpp:280-306): (z - n_z/2) dz, z = 1..n_z-1
+    except n_z/2, dz = 2 max_k / n_z, weight dz; kappa in (0, kappa_max(k_z)),
+    kappa_max = sqrt(max_k^2 - k_z^2) (DiracSystem.cpp:259-263), by n_kappa-point
+    Gauss-Legendre where the reference refines an adaptive trapezoid; the
+    integrand weight k_z kappa / |k| (DiracSystem.cpp:279)."""
+    gk, wk = gauss_legendre(n_kappa)
+    dz = 2.0 * max_k / n_z
+    kz_all, ka_all, w_all = [], [], []
+    for z in range(1, n_z):
+        if z == n_z // 2:
+            continue
+        kz = (z - n_z // 2) * dz
+        kmax = np.sqrt(max_k * max_k - kz * kz)
+        ka = 0.5 * kmax * (gk + 1.0)
+        kz_all.append(np.full(n_kappa, kz))
+        ka_all.append(ka)
+        w_all.append(dz * 0.5 * kmax * wk * kz * ka / np.sqrt(kz * kz + ka * ka))
+    return np.concatenate(kz_all), np.concatenate(ka_all), np.concatenate(w_all)
+
 
 def gl_box(nx: int, ny: int, x0: float, x1: float, y0: float, y1: float):
     """Tensor GL grid on [x0,x1]x[y0,y1], x-major (x index slowest)."""
@@ -237,3 +275,15 @@ def trapezoidal_spectrum(j: np.ndarray, t_begin: float, t_end: float, n_measurem
     out = delta * W * np.exp(1j * freq * t_begin) * Y + corr
     out = delta * out
     return freq, out.real.copy(), out.imag.copy()
+
+
+def welch_spectrum(j: np.ndarray, t_begin: float, t_end: float, n_measurements: int):
+    """sources/hhg.cpp:264-275 for continuous lasers: the first N samples times
+    the Welch window 1 - (i - N/2)^2 / (N/2)^2 (WelchWindow.hpp:11-13), numpy's
+    real FFT (the r2c transform of FFT.cpp:8-42), frequencies i / Delta."""
+    N = n_measurements
+    i = np.arange(N)
+    x = np.asarray(j[:N], dtype=np.float64) * (1.0 - (i - 0.5 * N) ** 2 / (0.25 * N * N))
+    X = np.fft.rfft(x)
+    delta = (t_end - t_begin) / N
+    return np.arange(N // 2 + 1) / delta, X.real, X.imag

@@ -78,6 +78,9 @@ double or_pulse_A(const or_pulse* p, double t) {
     case OR_PULSE_DOUBLE_COS2:
         return p->amp_a * env_cos2(t, p->t0, p->n_cycles) * sin(t + p->cep) +
                p->amp_b * env_cos2(t - p->delay, p->t0, p->n_cycles) * sin(t - p->delay + p->cep);
+    case OR_PULSE_CONTINUOUS:
+        /* ContinuousLaser: envelope 1 (sources/HHG/Laser/ContinuousLaser.cpp:7-9) */
+        return p->amp_a * sin(t + p->cep);
     default:
         return NAN;
     }
@@ -206,6 +209,22 @@ static void model_d(const or_model* m, double qx, double qy, double d[3]) {
     }
 }
 
+/* The field vector d of the Bloch equation ds/dt = 2 d x s at vector
+ * potential A.  2D models: d(k - A e_theta).  3D Dirac cone in the band frame
+ * of the reference (sources/HHG/Systems/DiracSystem.cpp:143-153), the point
+ * given as (k_z, kappa) in (kx, ky), field along z:
+ * d = -(A kappa / |k|, 0, |k| - A k_z / |k|). */
+static void model_field(const or_model* m, double kx, double ky, double A, double d[3]) {
+    if (m->kind == OR_MODEL_DIRAC3D) {
+        const double k = sqrt(kx * kx + ky * ky);
+        d[0] = -A * ky / k;
+        d[1] = 0.0;
+        d[2] = -(k - A * kx / k);
+        return;
+    }
+    model_d(m, kx - A * cos(m->theta), ky - A * sin(m->theta), d);
+}
+
 /* Fermi function, sources/HHG/GlobalDefinitions.hpp:36-41 (beta <= 0: T = 0). */
 static double fermi(double e, double beta) {
     if (!(beta > 0.0)) return e != 0.0 ? (e < 0.0 ? 1.0 : 0.0) : 0.5;
@@ -213,6 +232,17 @@ static double fermi(double e, double beta) {
 }
 
 void or_initial_state(const or_model* m, double kx, double ky, double s[3]) {
+    if (m->kind == OR_MODEL_DIRAC3D) {
+        /* DiracSystem.cpp:155-158: alpha0 = f(E_F + |k|), beta0 = f(E_F - |k|),
+         * s = (2 alpha0 beta0, 0, alpha0^2 - beta0^2) */
+        const double k = sqrt(kx * kx + ky * ky);
+        const double a0 = fermi(m->fermi_energy + k, m->beta);
+        const double b0 = fermi(m->fermi_energy - k, m->beta);
+        s[0] = 2.0 * a0 * b0;
+        s[1] = 0.0;
+        s[2] = a0 * a0 - b0 * b0;
+        return;
+    }
     double d[3];
     model_d(m, kx, ky, d);
     const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -244,9 +274,8 @@ void or_initial_state(const or_model* m, double kx, double ky, double s[3]) {
 void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const double s_init[3], const double s[3],
             double t, double out[3]) {
     const double A = or_pulse_A(p, t);
-    const double qx = kx - A * cos(m->theta), qy = ky - A * sin(m->theta);
     double d[3];
-    model_d(m, qx, qy, d);
+    model_field(m, kx, ky, A, d);
     const double wx = 2.0 * d[0], wy = 2.0 * d[1], wz = 2.0 * d[2];
     out[0] = wy * s[2] - wz * s[1];
     out[1] = wz * s[0] - wx * s[2];
@@ -263,6 +292,9 @@ void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const do
 }
 
 double or_vertex(const or_model* m, double kx, double ky, double A, const double s[3]) {
+    /* 3D Dirac cone: the band inversion rho = s_z (DiracSystem.cpp:167-169);
+     * the weight carries k_z kappa / |k| (DiracSystem.cpp:279) */
+    if (m->kind == OR_MODEL_DIRAC3D) return s[2];
     const double ct = cos(m->theta), st = sin(m->theta);
     const double qx = kx - A * ct, qy = ky - A * st;
     switch (m->kind) {
@@ -405,10 +437,9 @@ static void magnus4_step(const or_model* m, const or_pulse* p, double kx, double
                          double h) {
     const double c1 = 0.5 - OR_SQRT3 / 6.0, c2 = 0.5 + OR_SQRT3 / 6.0;
     double w1[3], w2[3];
-    const double ct = cos(m->theta), st = sin(m->theta);
     const double A1 = or_pulse_A(p, t + c1 * h), A2 = or_pulse_A(p, t + c2 * h);
-    model_d(m, kx - A1 * ct, ky - A1 * st, w1);
-    model_d(m, kx - A2 * ct, ky - A2 * st, w2);
+    model_field(m, kx, ky, A1, w1);
+    model_field(m, kx, ky, A2, w2);
     for (int q = 0; q < 3; ++q) { w1[q] *= 2.0; w2[q] *= 2.0; }
     /* Omega = h/2 (w1 + w2) + sqrt(3)/12 h^2 (w2 x w1) */
     const double f = OR_SQRT3 / 12.0 * h * h;
@@ -430,6 +461,80 @@ static void magnus4_step(const or_model* m, const or_pulse* p, double kx, double
     double oxs[3] = {O[1] * s[2] - O[2] * s[1], O[2] * s[0] - O[0] * s[2], O[0] * s[1] - O[1] * s[0]};
     double oxoxs[3] = {O[1] * oxs[2] - O[2] * oxs[1], O[2] * oxs[0] - O[0] * oxs[2], O[0] * oxs[1] - O[1] * oxs[0]};
     for (int q = 0; q < 3; ++q) s[q] += sa * oxs[q] + sb * oxoxs[q];
+}
+
+/* ------------------------------------------------------------------------- */
+/* The reference's moment-based Magnus step of the 3D Dirac cone              */
+/* ------------------------------------------------------------------------- */
+
+/* Laser::magnus_coefficients (sources/HHG/Laser/Laser.cpp:40-55): 30-point
+ * Gauss-Legendre on [0, 1] (sources/HHG/Laser/gauss.hpp:14-80) of A(t0 + h x)
+ * against 1 and the table's polynomials 2x - 1, 6x(x - 1) + 1 and
+ * 20x^3 - 30x^2 + 12x + 1 (gauss.hpp:82-102), scaled by 1, 3, 5, 7.  The last
+ * polynomial is the shifted Legendre P3 with the sign of its constant term
+ * flipped; it is evaluated as the reference evaluates it. */
+void or_magnus_moments(const or_pulse* p, double t0, double h, double mom[4]) {
+    double xi[30], wi[30];
+    or_gauss_legendre(30, xi, wi);
+    mom[0] = mom[1] = mom[2] = mom[3] = 0.0;
+    for (int i = 0; i < 30; ++i) {
+        const double x = 0.5 * (xi[i] + 1.0), w = 0.5 * wi[i];
+        const double a = w * or_pulse_A(p, t0 + h * x);
+        mom[0] += a;
+        mom[1] += a * (2.0 * x - 1.0);
+        mom[2] += a * (6.0 * x * (x - 1.0) + 1.0);
+        mom[3] += a * (20.0 * x * x * x - 30.0 * x * x + 12.0 * x + 1.0);
+    }
+    mom[1] *= 3.0;
+    mom[2] *= 5.0;
+    mom[3] *= 7.0;
+}
+
+/* DiracSystem::time_evolution_magnus (sources/HHG/Systems/DiracSystem.cpp:176-211):
+ * with K = 2|k| h, Kp = 2 kappa h, Kz = 2 k_z h (Magnus(2|k|, 2 kappa, 2 k_z, h),
+ * MagnusMatrix.cpp:12-23) and the moments divided by |k| (al, be, ga, de), the
+ * generator Omega = [[0, a, b], [-a, 0, c], [-b, -c, 0]] has the entries of
+ * MagnusMatrix.cpp:35-62 (grouped here by q = Kz^2 + Kp^2), and s <- exp(Omega) s.
+ * The exponential is Rodrigues' formula about w = (-c, b, -a) with the half-angle
+ * form of 1 - cos (the reference divides cos - 1 by theta^2 directly,
+ * MagnusMatrix.cpp:64-74; the two agree to rounding). */
+void or_magnus_moments_step(double kz, double kappa, const double mom[4], double h, double s[3]) {
+    const double kk = sqrt(kz * kz + kappa * kappa);
+    const double K = 2.0 * kk * h, Kp = 2.0 * kappa * h, Kz = 2.0 * kz * h;
+    const double al = mom[0] / kk, be = mom[1] / kk, ga = mom[2] / kk, de = mom[3] / kk;
+    const double K2 = K * K, K3 = K2 * K, K4 = K2 * K2;
+    const double Kp2 = Kp * Kp, Kz2 = Kz * Kz, q = Kz2 + Kp2;
+    const double al2 = al * al, al3 = al2 * al, al4 = al2 * al2, be2 = be * be, be3 = be2 * be, ga2 = ga * ga;
+    const double a = al * Kz - K +
+                     Kp2 * ((al2 * be2 - al3 * ga) * K * q / 2520.0 + al2 * ga * K2 * Kz / 1260.0 -
+                            al * be2 * K2 * Kz / 1080.0 - al * ga * K3 / 2520.0 + be2 * K3 / 1890.0 + be2 * K / 60.0 -
+                            be * de * K / 420.0 + ga2 * K / 420.0);
+    const double b = K * Kp *
+                     (-al4 * be * q * q + 4.0 * al3 * be * K * Kz * q - 6.0 * al2 * be * K2 * Kz2 -
+                      2.0 * al2 * be * K2 * Kp2 + (-42.0 * al2 * be + 18.0 * al2 * de - 36.0 * al * be * ga + 18.0 * be3) * q +
+                      (4.0 * al * be * K3 + 84.0 * al * be * K - 36.0 * al * de * K + 36.0 * be * ga * K) * Kz -
+                      be * K4 - 42.0 * be * K2 - 2520.0 * be + 18.0 * de * K2) /
+                     15120.0;
+    const double c = Kp *
+                     (3.0 * (al2 * be2 - al3 * ga) * K * Kz * q + 9.0 * al2 * ga * K2 * Kz2 + 3.0 * al2 * ga * K2 * Kp2 -
+                      6.0 * al * be2 * K2 * Kz2 + al * be2 * K2 * Kp2 - 9.0 * al * ga * K3 * Kz - 7560.0 * al +
+                      (3.0 * be2 * K3 + 126.0 * be2 * K - 18.0 * be * de * K + 18.0 * ga2 * K) * Kz + 3.0 * ga * K4) /
+                     7560.0;
+    const double O[3] = {-c, b, -a};
+    const double th2 = a * a + b * b + c * c;
+    const double th = sqrt(th2);
+    double sa, sb;
+    if (th < 1e-4) {
+        sa = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+        sb = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+    } else {
+        sa = sin(th) / th;
+        const double hs = sin(0.5 * th);
+        sb = 2.0 * hs * hs / th2;
+    }
+    double oxs[3] = {O[1] * s[2] - O[2] * s[1], O[2] * s[0] - O[0] * s[2], O[0] * s[1] - O[1] * s[0]};
+    double oxoxs[3] = {O[1] * oxs[2] - O[2] * oxs[1], O[2] * oxs[0] - O[0] * oxs[2], O[0] * oxs[1] - O[1] * oxs[0]};
+    for (int r = 0; r < 3; ++r) s[r] += sa * oxs[r] + sb * oxoxs[r];
 }
 
 /* ------------------------------------------------------------------------- */
@@ -507,6 +612,18 @@ static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const
     emit(user, 0, s);
     const int N = tc->n_measurements;
     const double delta = (tc->t_end - tc->t_begin) / N;
+    if (sv->method == OR_METHOD_MAGNUS_MOMENTS) {
+        /* one step per sampling interval, 3D Dirac cone only, no relaxation */
+        if (m->kind != OR_MODEL_DIRAC3D || m->relax != OR_RELAX_NONE) return -1;
+        for (int i = 1; i <= N; ++i) {
+            double mom[4];
+            or_magnus_moments(p, sample_time(tc, i - 1), delta, mom);
+            or_magnus_moments_step(kx, ky, mom, delta, s);
+            st->attempts += 1;
+            emit(user, i, s);
+        }
+        return 0;
+    }
     if (sv->method == OR_METHOD_MAGNUS4) {
         if (m->relax != OR_RELAX_NONE) return -1;
         const int ms = sv->magnus_substeps > 0 ? sv->magnus_substeps : 1;

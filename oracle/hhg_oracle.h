@@ -18,10 +18,10 @@
 extern "C" {
 #endif
 
-enum { OR_MODEL_DIRAC2D = 0, OR_MODEL_PIFLUX2D = 1, OR_MODEL_HONEYCOMB = 2 };
-enum { OR_PULSE_COS2 = 0, OR_PULSE_GAUSS = 1, OR_PULSE_DOUBLE_GAUSS = 2, OR_PULSE_DOUBLE_COS2 = 3 };
+enum { OR_MODEL_DIRAC2D = 0, OR_MODEL_PIFLUX2D = 1, OR_MODEL_HONEYCOMB = 2, OR_MODEL_DIRAC3D = 3 };
+enum { OR_PULSE_COS2 = 0, OR_PULSE_GAUSS = 1, OR_PULSE_DOUBLE_GAUSS = 2, OR_PULSE_DOUBLE_COS2 = 3, OR_PULSE_CONTINUOUS = 4 };
 enum { OR_RELAX_NONE = 0, OR_RELAX_TO_INITIAL = 1, OR_RELAX_T2 = 2 };
-enum { OR_METHOD_RKF78 = 0, OR_METHOD_MAGNUS4 = 1 };
+enum { OR_METHOD_RKF78 = 0, OR_METHOD_MAGNUS4 = 1, OR_METHOD_MAGNUS_MOMENTS = 2 };
 enum { OR_PROPOSAL_RESTART = 0, OR_PROPOSAL_CARRY = 1 };
 
 typedef struct {
@@ -87,6 +87,14 @@ void or_initial_state(const or_model* m, double kx, double ky, double s[3]);
 /* Bloch right-hand side ds/dt at time t (used by tests against scipy). */
 void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const double s_init[3],
             const double s[3], double t, double out[3]);
+
+/* The four pulse moments of one sampling interval [t0, t0 + h] that the
+ * reference's Dirac Magnus step uses (Laser::magnus_coefficients). */
+void or_magnus_moments(const or_pulse* p, double t0, double h, double mom[4]);
+
+/* One moment-based Magnus step of the 3D Dirac cone at (k_z, kappa) over an
+ * interval of length h with the interval's moments (DiracSystem::time_evolution_magnus). */
+void or_magnus_moments_step(double kz, double kappa, const double mom[4], double h, double s[3]);
 
 /* Current vertex g(s, A) of one k-point (unweighted). */
 double or_vertex(const or_model* m, double kx, double ky, double A, const double s[3]);

@@ -14,7 +14,7 @@ class Problem:
     model: dict
     pulse: dict
     time: tuple            # (t_begin, t_end, N, n_sub)
-    grid: tuple            # ("box", nx, ny, x0, x1, y0, y1) or ("hex", nx, ny)
+    grid: tuple            # ("box", nx, ny, x0, x1, y0, y1), ("hex", nx, ny) or ("cyl", n_z, n_kappa, max_k)
     solver: dict = field(default_factory=lambda: dict(method=0, rtol=1e-10, atol=1e-10, proposal=0))
 
     # ---- oracle side ----
@@ -35,6 +35,8 @@ class Problem:
     def o_grid(self):
         if self.grid[0] == "box":
             return O.gl_box(*self.grid[1:])
+        if self.grid[0] == "cyl":
+            return O.dirac_cylinder(*self.grid[1:])
         return O.gl_hex(*self.grid[1:])
 
     # ---- HIP side ----
@@ -60,6 +62,8 @@ class Problem:
     def h_grid(self, H):
         if self.grid[0] == "box":
             return H.KGrid.box(*self.grid[1:])
+        if self.grid[0] == "cyl":
+            return H.KGrid.dirac_cylinder(*self.grid[1:])
         return H.KGrid.hexagonal(*self.grid[1:])
 
     def h_solver(self, H, **over):
@@ -158,6 +162,23 @@ HONEYCOMB_CORNERS = Problem(
     pulse=dict(kind=O.PULSE_COS2, amp_a=1.69, n_cycles=2.0, t0=0.05, cep=0.3),
     time=(0.0, 6 * PI, 192, 50),
     grid=("hex", 12, 12),
+)
+
+# the reference's 3D Dirac cone (band frame, field along z) on its (k_z, kappa) grid
+DIRAC3D_CONTINUOUS = Problem(
+    "dirac3d_continuous",
+    model=dict(kind=O.MODEL_DIRAC3D),
+    pulse=dict(kind=O.PULSE_CONTINUOUS, amp_a=0.6),
+    time=(-4 * PI, 4 * PI, 1024, 50),
+    grid=("cyl", 16, 8, 4.0),
+)
+
+DIRAC3D_COS2_RELAX = Problem(
+    "dirac3d_cos2_relax",
+    model=dict(kind=O.MODEL_DIRAC3D, fermi_energy=0.2, beta=10.0, relax=O.RELAX_TO_INITIAL, relax_time=15.0),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=0.8, n_cycles=2.0, t0=0.0),
+    time=(0.0, 4 * PI, 512, 50),
+    grid=("cyl", 12, 6, 3.0),
 )
 
 SMALL_CASES = [DIRAC_SMALL, DIRAC_COHERENT, DIRAC_RELAX_INIT, PIFLUX_SMALL, HONEYCOMB_SMALL, HONEYCOMB_DOUBLE_COS2]

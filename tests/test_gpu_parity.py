@@ -8,8 +8,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_SMALL, HONEYCOMB_CORNERS, HONEYCOMB_SMALL, PIFLUX_SMALL,
-                            SMALL_CASES, oracle_j, rel_l2)
+from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
+                            HONEYCOMB_CORNERS, HONEYCOMB_SMALL, PIFLUX_SMALL, SMALL_CASES, oracle_j, rel_l2)
 
 pytestmark = pytest.mark.gpu
 
@@ -219,3 +219,87 @@ def test_piflux_delay_sweep_matches_oracle(hhg):
     with W.build_plan(hhg, jobs) as plan:
         for j, r in zip(plan.execute(), refs):
             assert rel_l2(j, r) <= J_TOL
+
+
+@pytest.mark.parametrize("pr", [DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX], ids=lambda p: p.name)
+@pytest.mark.parametrize("proposal", [0, 1], ids=["restart", "carry"])
+def test_dirac3d_rkf78_matches_oracle(hhg, pr, proposal):
+    """The reference's 3D Dirac cone (band frame, field along z) with RKF78 on
+    its (k_z, kappa) grid, continuous and cos^2 pulses, relaxation toward the
+    initial state at finite temperature."""
+    _print_case(pr, proposal=proposal)
+    j_ref, st_ref = oracle_j(pr, proposal=proposal)
+    with pr.h_plan(hhg, solver=dict(proposal=proposal)) as plan:
+        j = plan.execute()[0]
+        att = plan.stats.attempts
+    assert rel_l2(j, j_ref) <= J_TOL
+    assert abs(att - st_ref.attempts) <= 1e-3 * st_ref.attempts
+
+
+def test_dirac3d_moment_magnus_matches_oracle(hhg):
+    """The reference's moment-based Magnus step (one per sampling interval from
+    four 30-point pulse moments) against the oracle's restatement, j(t) and
+    per-k trajectories; and against RKF78 within the step's truncation error."""
+    pr = DIRAC3D_CONTINUOUS
+    _print_case(pr, method="magnus_moments")
+    j_ref, _ = oracle_j(pr, method=O.METHOD_MAGNUS_MOMENTS)
+    with pr.h_plan(hhg, solver=dict(method=2)) as plan:
+        j = plan.execute()[0]
+        traj = plan.trajectories(np.arange(0, 112, 7))
+    assert rel_l2(j, j_ref) <= 1e-12
+    kz, ka, _ = pr.o_grid()
+    for n, k in enumerate(range(0, 112, 7)):
+        s_ref, _, _ = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(method=O.METHOD_MAGNUS_MOMENTS),
+                                   kz[k], ka[k])
+        assert np.max(np.abs(traj[n] - s_ref)) < 1e-12
+    j_rk, _ = oracle_j(pr, rtol=1e-12, atol=1e-12)
+    assert rel_l2(j, j_rk) <= 1e-5
+
+
+@pytest.mark.parametrize("tau", [-1.0, 20.0], ids=["magnus", "relaxation"])
+def test_cpp_driver_reference_units_dirac(hhg, tmp_path, tau):
+    """hhg_run with the reference's own Dirac keys and units (T, E_F [meV], v_F
+    [m/s], band_width, field_amplitude [MV/cm], photon_energy [meV],
+    diagonal_relaxation_time [fs], continuous laser) writes the reference's
+    current_density fields, gzip-compressed; j(t) equals the oracle's run of the
+    converted parameters (moment Magnus without relaxation, RKF78 at the
+    reference's tolerances with it) and the spectrum is the Welch-window one."""
+    import gzip
+    import json
+    import math
+    import os
+    import subprocess
+    cfg = tmp_path / "dirac.cfg"
+    cfg.write_text("system_type Dirac\nT 0\nE_F 0\nv_F 1.5e5\nband_width 5\nfield_amplitude 1.2\n"
+                   f"photon_energy 5.25\ndiagonal_relaxation_time {tau}\nlaser_type continuous\n"
+                   "n_laser_cycles 2\nt0_offset 0\nn_z 12\nn_kappa 6\nn_measurements 512\n")
+    out = tmp_path / "current_density.json.gz"
+    exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
+    print(f"[case] hhg_run Dirac reference units tau={tau}", flush=True)
+    r = subprocess.run([exe, str(cfg), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(gzip.open(out).read())
+    for key in ("laser_function", "N", "t_begin", "t_end", "n_laser_cycles", "n_measurements", "current_density_time",
+                "current_density_frequency_real", "current_density_frequency_imag", "T", "E_F", "v_F", "band_width",
+                "field_amplitude", "photon_energy", "laser_type", "diagonal_relaxation_time", "frequencies",
+                "zero_padding", "system_type", "n_z", "t0_offset"):
+        assert key in d, key
+    # Laser.cpp:24-25 with model_ratio = hbar v_F / photon_energy (DiracSystem.hpp:30-32)
+    hbar = 6.582119569509065698e-13
+    A0 = (hbar * 1.5e5 / 5.25) * 1e11 * 1.2 / 5.25
+    assert abs(d["momentum_amplitude"] - A0) <= 1e-15 * A0
+    kz, ka, w = O.dirac_cylinder(12, 6, 5.0)
+    tc = O.time_grid(-2 * math.pi, 2 * math.pi, 512, 50)
+    p = O.pulse(O.PULSE_CONTINUOUS, amp_a=A0)
+    if tau > 0:
+        rate = 1e15 * hbar / (tau * 5.25)  # DiracSystem.cpp:64
+        m = O.model(O.MODEL_DIRAC3D, relax=O.RELAX_TO_INITIAL, relax_time=1.0 / rate)
+        sv = O.solver(rtol=1e-8, atol=1e-12)
+    else:
+        m = O.model(O.MODEL_DIRAC3D)
+        sv = O.solver(method=O.METHOD_MAGNUS_MOMENTS)
+    j_ref, _ = O.current_density(m, p, tc, sv, kz, ka, w)
+    j = np.array(d["current_density_time"])
+    assert rel_l2(j, j_ref) <= (1e-12 if tau < 0 else J_TOL)
+    f, re, im = O.welch_spectrum(j, -2 * math.pi, 2 * math.pi, 512)
+    assert np.allclose(d["current_density_frequency_real"], re, rtol=1e-12, atol=1e-12 * np.abs(re).max())

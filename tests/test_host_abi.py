@@ -71,6 +71,21 @@ def test_kgrids_match_oracle_and_areas(hhg):
     assert errs[0] < 1e-2 and 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5
 
 
+def test_dirac_cylinder_grid_matches_oracle(hhg):
+    """(k_z, kappa) points and weights of the 3D Dirac cone's grid, product vs oracle."""
+    for n_z, n_kappa, M in [(16, 8, 3.0), (33, 5, 1.5)]:
+        kx, ky, w = hhg.KGrid.dirac_cylinder(n_z, n_kappa, M).build()
+        okx, oky, ow = O.dirac_cylinder(n_z, n_kappa, M)
+        assert len(kx) == (n_z - 2) * n_kappa == len(okx)
+        assert np.array_equal(kx, okx) and np.allclose(ky, oky, rtol=1e-15, atol=0) and np.allclose(w, ow, rtol=1e-14)
+
+
+def test_continuous_pulse_matches_oracle(hhg):
+    t = np.linspace(-20.0, 20.0, 401)
+    A = hhg.ContinuousLaser(0.9, cep=0.2).laser_function(t)
+    assert np.max(np.abs(A - O.pulse_A(O.pulse(O.PULSE_CONTINUOUS, amp_a=0.9, cep=0.2), t))) < 1e-15
+
+
 @pytest.mark.parametrize("pr", SMALL_CASES, ids=lambda p: p.name)
 def test_pulse_matches_oracle(hhg, pr):
     t = np.linspace(pr.time[0] - 1.0, pr.time[1] + 1.0, 997)
@@ -87,6 +102,23 @@ def test_cos2_pulse_is_the_configured_cos2_form(hhg):
     t = np.linspace(-T / 2 + 1e-9, T / 2 - 1e-9, 4001)
     assert np.max(np.abs(las.laser_function(t) - 0.5 * np.cos(np.pi * t / T) ** 2 * np.sin(t))) < 1e-15
     assert np.all(las.laser_function(np.array([-T / 2 - 1, T / 2 + 1])) == 0.0)
+
+
+def test_welch_spectrum_matches_oracle_and_direct_sum(hhg):
+    """Continuous-laser spectrum: Welch window and r2c DFT, against numpy's FFT
+    and a direct sum on a small N."""
+    rng = np.random.default_rng(9)
+    for N in (64, 250):
+        tc = hhg.TimeIntegrationConfig(-N * 0.05, N * 0.05, N, 50)
+        j = rng.standard_normal(N + 1)
+        f, re, im = hhg.welch_spectrum(j, tc)
+        of, ore, oim = O.welch_spectrum(j, tc.t_begin, tc.t_end, N)
+        assert np.allclose(f, of, rtol=1e-15) and np.allclose(re, ore, atol=1e-12) and np.allclose(im, oim, atol=1e-12)
+        i = np.arange(N)
+        x = j[:N] * (1 - (i - N / 2) ** 2 / (N / 2) ** 2)
+        k = np.arange(N // 2 + 1)
+        direct = (x[None, :] * np.exp(-2j * np.pi * k[:, None] * i[None, :] / N)).sum(1)
+        assert np.allclose(re + 1j * im, direct, atol=1e-11)
 
 
 def test_spectrum_matches_oracle(hhg):

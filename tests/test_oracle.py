@@ -18,7 +18,9 @@ independent solvers, so that the oracle can serve as the parity reference.
 * Error budget: RKF78 at 1e-10 against the same oracle at 1e-13 stays
   within the 1e-9 relative j(t) criterion; Magnus-4 converges at order 4.
 """
+import json
 import math
+import os
 from fractions import Fraction
 from functools import lru_cache
 
@@ -26,6 +28,9 @@ import numpy as np
 import pytest
 
 import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
 from tests.problems import (CONFIG1, DIRAC_COHERENT, DIRAC_RELAX_INIT, DIRAC_SMALL, HONEYCOMB_CORNERS, HONEYCOMB_SMALL,
                             PIFLUX_SMALL, Problem, oracle_j, rel_l2)
 
@@ -437,3 +442,147 @@ def test_trapezoidal_weights_reproduce_analytic_fourier_integral():
         exact = math.sqrt(8 * math.pi) / 2 * (math.exp(-2 * (w - 3) ** 2) + math.exp(-2 * (w + 3) ** 2))
         e1, e2 = abs(integral(400, w) - exact), abs(integral(800, w) - exact)
         assert e1 < 0.01 * 2.5066 and 3.5 < e1 / e2 < 4.5, (w, e1, e2)  # 1% of the peak
+
+
+# --------------------------------------------------------------------------
+# The reference's 3D Dirac cone (band frame) and its moment Magnus step
+# --------------------------------------------------------------------------
+
+def _weyl_lab_projection(kz, kappa, pulse, t_samples):
+    """k^ . s(t) for the Weyl cone H = (k - A(t) z^) . sigma in the lab frame,
+    ds/dt = 2 d x s with d = (kappa, 0, k_z - A), from the lower band s = -k^
+    (DOP853, rtol 1e-13): the band inversion the reference's band-frame
+    equations evolve."""
+    from scipy.integrate import solve_ivp
+    kk = math.hypot(kz, kappa)
+    khat = np.array([kappa, 0.0, kz]) / kk
+
+    def f(t, s):
+        A = O.pulse_A(pulse, np.array([t]))[0]
+        d = np.array([kappa, 0.0, kz - A])
+        return 2.0 * np.cross(d, s)
+
+    sol = solve_ivp(f, (t_samples[0], t_samples[-1]), -khat, method="DOP853", rtol=1e-13, atol=1e-13,
+                    t_eval=t_samples)
+    return khat @ sol.y
+
+
+@pytest.mark.parametrize("kz,kappa", [(0.7, 1.1), (-1.3, 0.4), (0.2, 2.5)])
+def test_dirac3d_band_frame_matches_lab_frame_weyl_cone(kz, kappa):
+    """The band-frame RHS, initial state and vertex of the reference's 3D Dirac
+    cone (s_z = band inversion) against the lab-frame Weyl-cone evolution."""
+    m = O.model(O.MODEL_DIRAC3D)
+    p = O.pulse(O.PULSE_COS2, amp_a=0.8, n_cycles=2.0, t0=0.0)
+    tc = O.time_grid(0.0, 4 * math.pi, 256, 50)
+    s = O.trajectory(m, p, tc, O.solver(rtol=1e-12, atol=1e-12), kz, kappa)[0]
+    t = np.linspace(0.0, 4 * math.pi, 257)
+    ref = _weyl_lab_projection(kz, kappa, p, t)
+    assert np.max(np.abs(s[:, 2] - ref)) < 1e-9
+    assert abs(O.vertex(m, kz, kappa, 0.3, s[-1]) - s[-1, 2]) == 0.0
+
+
+def test_dirac3d_initial_state_fermi_occupations():
+    """DiracSystem.cpp:155-158 at finite temperature: (2 a b, 0, a^2 - b^2) with
+    a = f(E_F + |k|), b = f(E_F - |k|)."""
+    m = O.model(O.MODEL_DIRAC3D, fermi_energy=0.3, beta=5.0)
+    kz, kappa = 0.4, 0.3
+    k = math.hypot(kz, kappa)
+    f = lambda e: 1.0 / (1.0 + math.exp(5.0 * e))
+    a, b = f(0.3 + k), f(0.3 - k)
+    assert np.allclose(O.initial_state(m, kz, kappa), [2 * a * b, 0.0, a * a - b * b], rtol=0, atol=1e-16)
+
+
+def test_laser_gauss30_nodes_match_reference_table():
+    """The 30-point rule of the moments is the reference's table (gauss.hpp)."""
+    fx = json.load(open(os.path.join(GOLDEN, "laser_gauss30.json")))
+    nodes = np.array([float(v) for v in fx["positive_nodes"]])
+    weights = np.array([float(v) for v in fx["weights"]])
+    x, w = O.gauss_legendre(30)
+    assert np.max(np.abs(x[15:] - nodes)) < 2e-16 and np.max(np.abs(x[:15][::-1] + nodes)) < 2e-16
+    assert np.max(np.abs(w[15:] - weights)) < 2e-16
+
+
+def test_magnus_moments_closed_form():
+    """Moments of A = A0 sin(t) (continuous pulse) against their integrals:
+    (2n+1) A0 int_0^1 sin(t0 + h x) P_n(x) dx with the reference's polynomials
+    1, 2x - 1, 6x(x - 1) + 1, 20x^3 - 30x^2 + 12x + 1 (gauss.hpp:82-102)."""
+    import mpmath
+    mpmath.mp.dps = 30
+    p = O.pulse(O.PULSE_CONTINUOUS, amp_a=0.7)
+    polys = [lambda x: 1, lambda x: 2 * x - 1, lambda x: 6 * x * (x - 1) + 1,
+             lambda x: 20 * x ** 3 - 30 * x ** 2 + 12 * x + 1]
+    for t0, h in [(0.3, 0.05), (-2.0, 0.2), (5.0, 0.01)]:
+        mom = O.magnus_moments(p, t0, h)
+        ref = [float((2 * n + 1) * mpmath.mpf("0.7") *
+                     mpmath.quad(lambda x: mpmath.sin(t0 + h * x) * polys[n](x), [0, 1])) for n in range(4)]
+        assert np.all(np.abs(mom - np.array(ref)) <= 1e-15 * np.maximum(1.0, np.abs(ref)))
+
+
+def _axis_rotation(w, s):
+    """exp(W) s with W v = w x v (scipy expm)."""
+    from scipy.linalg import expm
+    W = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    return expm(W) @ s
+
+
+def test_magnus_moments_step_exact_for_a_constant_field():
+    """With the exact moments of a constant A (A, 0, 0, 0) the step is the exact
+    rotation about 2h (A kappa/|k|, 0, |k| - A k_z/|k|): the generator's
+    A-linear terms and the exponential (the reference's Magnus turns s the other
+    way round than its band-frame RKF78 equations, which leaves the band
+    inversion s_z unchanged)."""
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        kz, kappa, A, h = rng.uniform(-2, 2), rng.uniform(0.1, 2), rng.uniform(-1, 1), rng.uniform(0.001, 0.3)
+        s = rng.normal(size=3)
+        k = math.hypot(kz, kappa)
+        w = 2 * h * np.array([A * kappa / k, 0.0, k - A * kz / k])
+        out = O.magnus_moments_step(kz, kappa, [A, 0.0, 0.0, 0.0], h, s)
+        assert np.allclose(out, _axis_rotation(w, s), rtol=0, atol=1e-14)
+
+
+def _moment_magnus_band_inversion(kz, kappa, p, N, T, exact_p3):
+    x, w = O.gauss_legendre(30)
+    x, w = 0.5 * (x + 1), 0.5 * w
+    polys = [np.ones_like(x), 2 * x - 1, 6 * x * (x - 1) + 1, 20 * x ** 3 - 30 * x ** 2 + 12 * x + (-1 if exact_p3 else 1)]
+    h = T / N
+    s = O.initial_state(O.model(O.MODEL_DIRAC3D), kz, kappa)
+    for i in range(N):
+        A = O.pulse_A(p, i * h + h * x)
+        mom = [(2 * n + 1) * np.sum(w * A * polys[n]) for n in range(4)]
+        s = O.magnus_moments_step(kz, kappa, mom, h, s)
+    return s[2]
+
+
+def test_magnus_moments_step_converges_at_order_4_with_legendre_moments():
+    """The generator's entries (MagnusMatrix.cpp:35-62) make a 4th-order method
+    when fed the Legendre moments of A: the band inversion converges to the
+    lab-frame Weyl-cone solution at order 4.  fed the reference's own moments
+    (its third polynomial has the constant term's sign flipped) the step is of
+    order 3 — measured here, kept in the product for parity."""
+    kz, kappa, T = 0.7, 1.1, 4 * math.pi
+    p = O.pulse(O.PULSE_COS2, amp_a=0.5, n_cycles=2.0, t0=0.0)
+    ref = _weyl_lab_projection(kz, kappa, p, np.array([0.0, T]))[-1]
+    err = {flag: [abs(_moment_magnus_band_inversion(kz, kappa, p, N, T, flag) - ref) for N in (256, 512, 1024)]
+           for flag in (True, False)}
+    r4 = [err[True][i] / err[True][i + 1] for i in range(2)]
+    r3 = [err[False][i] / err[False][i + 1] for i in range(2)]
+    assert min(r4) > 13.0, r4
+    assert 6.5 < min(r3) and max(r3) < 10.0, r3
+    # the oracle's own moments are the reference's (third-order) ones
+    tc = O.time_grid(0.0, T, 512, 50)
+    s = O.trajectory(O.model(O.MODEL_DIRAC3D), p, tc, O.solver(method=O.METHOD_MAGNUS_MOMENTS), kz, kappa)[0]
+    assert abs(s[-1, 2] - _moment_magnus_band_inversion(kz, kappa, p, 512, T, False)) < 1e-13
+
+
+def test_dirac_cylinder_weights():
+    """Sum of k_z kappa/|k| over the half disc vanishes (antisymmetric in k_z);
+    the sum of |w| converges to M^3/3 at second order in the uniform k_z step."""
+    M = 3.0
+    errs = []
+    for n_z in (32, 64, 128):
+        kz, ka, w = O.dirac_cylinder(n_z, 24, M)
+        assert abs(w.sum()) < 1e-12
+        assert np.all(ka > 0) and np.all(kz != 0)
+        errs.append(abs(np.abs(w).sum() - M ** 3 / 3))
+    assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5
