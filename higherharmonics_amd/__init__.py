@@ -32,10 +32,10 @@ LIB_PATH = os.path.join(_HERE, "libhhg_amd.so")
 # ---- enums (include/hhg_amd.h) ----
 SUCCESS, ERROR_INVALID_ARGUMENT, ERROR_HIP, ERROR_TOLERANCE, ERROR_OUT_OF_MEMORY, ERROR_UNSUPPORTED, \
     ERROR_NO_DEVICE, ERROR_NONFINITE = range(8)
-MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D = 0, 1, 2, 3
-RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2 = 0, 1, 2
+MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D, MODEL_PIFLUX3D = 0, 1, 2, 3, 4
+RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2, RELAX_EQUILIBRIUM = 0, 1, 2, 3
 PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS = 0, 1, 2, 3, 4
-KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER = 0, 1, 2, 3
+KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_CONTINUUM = 0, 1, 2, 3, 4
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
 
@@ -53,7 +53,8 @@ class HHGError(RuntimeError):
 class ModelDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
                 ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
-                ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double)]
+                ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double),
+                ("relax_time_offdiag", ctypes.c_double)]
 
 
 class PulseDesc(ctypes.Structure):
@@ -71,7 +72,8 @@ class KGridDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("x0", ctypes.c_double), ("x1", ctypes.c_double), ("y0", ctypes.c_double), ("y1", ctypes.c_double),
                 ("kx", ctypes.POINTER(ctypes.c_double)), ("ky", ctypes.POINTER(ctypes.c_double)),
-                ("w", ctypes.POINTER(ctypes.c_double)), ("n_points", ctypes.c_int64)]
+                ("w", ctypes.POINTER(ctypes.c_double)), ("n_points", ctypes.c_int64),
+                ("kz", ctypes.POINTER(ctypes.c_double))]
 
 
 class JobDesc(ctypes.Structure):
@@ -147,7 +149,7 @@ def lib() -> ctypes.CDLL:
     L.hhg_gauss_legendre.argtypes = [ctypes.c_int32, _DP, _DP]
     L.hhg_kgrid_size.restype = ctypes.c_int64
     L.hhg_kgrid_size.argtypes = [ctypes.POINTER(KGridDesc)]
-    L.hhg_kgrid_build.argtypes = [ctypes.POINTER(KGridDesc), _DP, _DP, _DP]
+    L.hhg_kgrid_build.argtypes = [ctypes.POINTER(KGridDesc), _DP, _DP, _DP, _DP]
     L.hhg_pulse_evaluate.argtypes = [ctypes.POINTER(PulseDesc), _DP, _DP, ctypes.c_int64]
     L.hhg_trapezoidal_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), ctypes.c_int32, _DP, _DP, _DP]
     L.hhg_welch_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), _DP, _DP, _DP]
@@ -318,22 +320,36 @@ class KGrid:
         return KGrid(KGRID_DIRAC_CYLINDER, n_z, n_kappa, 0.0, max_k)
 
     @staticmethod
-    def explicit(kx, ky, w) -> "KGrid":
+    def piflux_continuum(z_range=32, n_coarse=4) -> "KGrid":
+        """(x, y, z) points of the 3D pi-flux lattice (kind KGRID_PIFLUX_CONTINUUM)."""
+        return KGrid(KGRID_PIFLUX_CONTINUUM, z_range, n_coarse)
+
+    @staticmethod
+    def explicit(kx, ky, w, kz=None) -> "KGrid":
         arrs = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in (kx, ky, w))
+        if kz is not None:
+            arrs = arrs + (np.ascontiguousarray(kz, dtype=np.float64),)
         return KGrid(KGRID_EXPLICIT, points=arrs)
 
     def desc(self) -> KGridDesc:
-        d = KGridDesc(self.kind, self.nx, self.ny, 0, self.x0, self.x1, self.y0, self.y1, None, None, None, 0)
+        d = KGridDesc(self.kind, self.nx, self.ny, 0, self.x0, self.x1, self.y0, self.y1, None, None, None, 0, None)
         if self.kind == KGRID_EXPLICIT:
-            kx, ky, w = self.points
+            kx, ky, w = self.points[:3]
             d.kx, d.ky, d.w, d.n_points = _dp(kx), _dp(ky), _dp(w), len(kx)
+            if len(self.points) > 3:
+                d.kz = _dp(self.points[3])
         return d
 
-    def build(self):
+    def build3(self):
+        """(kx, ky, kz, w); kz = 0 for 2D grids."""
         d = self.desc()
         n = lib().hhg_kgrid_size(ctypes.byref(d))
-        kx, ky, w = np.empty(n), np.empty(n), np.empty(n)
-        _check(lib().hhg_kgrid_build(ctypes.byref(d), _dp(kx), _dp(ky), _dp(w)))
+        kx, ky, kz, w = np.empty(n), np.empty(n), np.empty(n), np.empty(n)
+        _check(lib().hhg_kgrid_build(ctypes.byref(d), _dp(kx), _dp(ky), _dp(kz), _dp(w)))
+        return kx, ky, kz, w
+
+    def build(self):
+        kx, ky, _, w = self.build3()
         return kx, ky, w
 
 
@@ -623,6 +639,45 @@ class Honeycomb(_System):
     def model_desc(self) -> ModelDesc:
         return ModelDesc(MODEL_HONEYCOMB, self.relax, 0.0, 0.0, self.hopping, self.theta, self.fermi_energy,
                          self.beta, self.relax_time)
+
+
+class DiracSystem3D(_System):
+    """The reference's 3D Dirac cone in its band frame (DiracSystem.cpp:138-174),
+    field along z, on the (k_z, kappa) cylinder of DiracSystem.cpp:259-306;
+    lengths in hbar omega_L / v_F."""
+    kind = MODEL_DIRAC3D
+
+    def __init__(self, max_k=10.0, n_z=200, n_kappa=64, **kw):
+        self.max_k, self.n_z, self.n_kappa = max_k, n_z, n_kappa
+        super().__init__(**kw)
+
+    def default_grid(self) -> KGrid:
+        return KGrid.dirac_cylinder(self.n_z, self.n_kappa, self.max_k)
+
+    def model_desc(self) -> ModelDesc:
+        return ModelDesc(MODEL_DIRAC3D, self.relax, 0.0, 0.0, 0.0, 0.0, self.fermi_energy, self.beta,
+                         self.relax_time)
+
+
+class PiFlux3D(_System):
+    """The reference's 3D pi-flux lattice d = 2t (cos x, cos y, cos(z - A))
+    (PiFlux.cpp:73-111) on its continuum-limit quadrature (PiFlux.cpp:665-863);
+    relax=RELAX_EQUILIBRIUM relaxes toward the instantaneous equilibrium with
+    the diagonal time relax_time and the off-diagonal time relax_time_offdiag
+    (PiFlux.cpp:113-220)."""
+    kind = MODEL_PIFLUX3D
+
+    def __init__(self, hopping=1.0, z_range=32, n_coarse=4, relax_time_offdiag=0.0, **kw):
+        self.hopping, self.z_range, self.n_coarse = hopping, z_range, n_coarse
+        self.relax_time_offdiag = relax_time_offdiag
+        super().__init__(**kw)
+
+    def default_grid(self) -> KGrid:
+        return KGrid.piflux_continuum(self.z_range, self.n_coarse)
+
+    def model_desc(self) -> ModelDesc:
+        return ModelDesc(MODEL_PIFLUX3D, self.relax, 0.0, 0.0, self.hopping, 0.0, self.fermi_energy, self.beta,
+                         self.relax_time, self.relax_time_offdiag)
 
 
 # --------------------------------------------------------------------------

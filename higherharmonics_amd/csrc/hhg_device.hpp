@@ -151,29 +151,55 @@ __device__ __forceinline__ double fermi(double e, double beta) {
     return 1.0 / (1.0 + exp(beta * e));
 }
 
+// is_zero of the reference (GlobalDefinitions.hpp:61-67): biased exponent
+// below 0b01111010011, i.e. |x| < 2^-44.
+__device__ __forceinline__ bool ref_is_zero(double x) { return fabs(x) < 0x1p-44; }
+constexpr double kRescue = 1e-4;  // RESCUE_TRAFO, PiFlux.cpp:50
+
 // Lattice models take the field along x (theta = 0), as the reference's
-// Honeycomb (Honeycomb.hpp:88-90); the Dirac model accepts any theta.
+// Honeycomb (Honeycomb.hpp:88-90); the Dirac model accepts any theta; the 3D
+// models take it along z.
 template <int M, int P, int R>
 struct Bloch {
     JobRef J;
-    double kx, ky;
+    double kx, ky, kz;
     double s0[3];
-    double cy;  // theta = 0 constant: PiFlux 2t cos ky, honeycomb cos(sqrt3/2 ky); 3D Dirac |k|
+    double cy;  // theta = 0 constant: PiFlux 2t cos ky, honeycomb cos(sqrt3/2 ky); 3D Dirac |k|; 3D pi-flux cos ky
     double ik;  // 3D Dirac: 1/|k|
+    double cx;  // 3D pi-flux: cos kx
+    double eq[3];  // equilibrium relaxation: the state at the start of the step
 
-    __device__ __forceinline__ Bloch(JobRef job, double kx_, double ky_) : J(job), kx(kx_), ky(ky_) {
+    __device__ __forceinline__ Bloch(JobRef job, double kx_, double ky_, double kz_ = 0.0)
+        : J(job), kx(kx_), ky(ky_), kz(kz_) {
         ik = 0.0;
+        cx = 0.0;
         if constexpr (M == MODEL_PIFLUX2D) cy = 2.0 * J.hop * cos(ky);
         else if constexpr (M == MODEL_HONEYCOMB) cy = cos(kHalfSqrt3 * ky);
         else if constexpr (M == MODEL_DIRAC3D) cy = sqrt(kx * kx + ky * ky), ik = cy > 0.0 ? 1.0 / cy : 0.0;
+        else if constexpr (M == MODEL_PIFLUX3D) cx = cos(kx), cy = cos(ky);
         else cy = 0.0;
+    }
+
+    // the integrator calls this with the state at the start of every attempt
+    __device__ __forceinline__ void begin_step(const double s[3]) {
+        if constexpr (R == RELAX_EQUILIBRIUM) {
+            eq[0] = s[0];
+            eq[1] = s[1];
+            eq[2] = s[2];
+        }
     }
 
     // d(k - A e_theta); the 3D Dirac cone in the reference's band frame
     // (DiracSystem.cpp:143-153; kx = k_z, ky = kappa, field along z):
     // d = -(A kappa/|k|, 0, |k| - A k_z/|k|)
     __device__ __forceinline__ void d_of(double A, double d[3]) const {
-        if constexpr (M == MODEL_DIRAC3D) {
+        if constexpr (M == MODEL_PIFLUX3D) {
+            // d = 2t (cos kx, cos ky, cos(kz - A)) (PiFlux.cpp:89-94, prefactor 4t = 2 * 2t)
+            const double t2 = 2.0 * J.hop;
+            d[0] = t2 * cx;
+            d[1] = t2 * cy;
+            d[2] = t2 * cos(kz - A);
+        } else if constexpr (M == MODEL_DIRAC3D) {
             d[0] = -A * ky * ik;
             d[1] = 0.0;
             d[2] = A * kx * ik - cy;
@@ -203,7 +229,23 @@ struct Bloch {
         double d[3];
         d_of(0.0, d);
         const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        if constexpr (M == MODEL_DIRAC3D) {
+        if constexpr (M == MODEL_PIFLUX3D) {
+            // PiFlux::ic_sigma (PiFlux.cpp:519-549), alpha^2 = f(-E_F + 2t|m|),
+            // beta^2 = f(-E_F - 2t|m|) (PiFlux.cpp:420-426).  On the continuum
+            // grid the reference scales cos kz of the positive z nodes with
+            // cos kz < 0 by 1 - RESCUE^2/2 (PiFlux.cpp:825-829; the -z node of
+            // PiFlux.cpp:845-849 keeps the last x/y cosines and is not rescued).
+            double cz = cos(kz);
+            if (J.ic_rescue && kz > 0.0 && cz < 0.0) cz *= (1.0 - 0.5 * kRescue * kRescue);
+            const double disp = sqrt(cx * cx + cy * cy + cz * cz);
+            const double a2 = fermi(-J.fermi + 2.0 * J.hop * disp, J.beta);
+            const double b2 = fermi(-J.fermi - 2.0 * J.hop * disp, J.beta);
+            const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
+            const double ze = cz + disp, nrm = disp * ze;
+            s[0] = (diff * cx * ze + prod * (cy * cy + cz * ze)) / nrm;
+            s[1] = (diff * cy * ze - prod * cx * cy) / nrm;
+            s[2] = (diff * cz * ze - prod * cx * ze) / nrm;
+        } else if constexpr (M == MODEL_DIRAC3D) {
             // DiracSystem.cpp:155-158: alpha0 = f(E_F + |k|), beta0 = f(E_F - |k|)
             const double a0 = fermi(J.fermi + cy, J.beta), b0 = fermi(J.fermi - cy, J.beta);
             s[0] = 2.0 * a0 * b0;
@@ -246,8 +288,54 @@ struct Bloch {
     // A(t + dlt) (t = the prepared step start)
     __device__ __forceinline__ double pulse_at(double dlt) const { return pc.at(J, dlt); }
 
+    // Relaxation of the 3D pi-flux lattice toward the instantaneous
+    // equilibrium at m = (cos kx, cos ky, cz = cos(kz - A)): update_equilibrium_state
+    // (PiFlux.cpp:134-166) with the off-diagonal phase of the step-start state
+    // in the band frame (diagonal_sigma, PiFlux.cpp:1037-1049), subtracted as
+    // rd (s/3 - D) + ro (2s/3 - O) (PiFlux.cpp:174-179).
+    __device__ __forceinline__ void relax_equilibrium(double cz, const double s[3], double out[3]) const {
+        double mx = cx;
+        if (ref_is_zero(mx) && ref_is_zero(cy) && cz < 0.0) {
+            mx = fabs(cz) * kRescue;
+            cz *= (1.0 - 0.5 * kRescue * kRescue);
+        }
+        const double my = cy;
+        const double disp = sqrt(mx * mx + my * my + cz * cz);
+        const double a2 = fermi(-J.fermi + 2.0 * J.hop * disp, J.beta);
+        const double b2 = fermi(-J.fermi - 2.0 * J.hop * disp, J.beta);
+        const double diff = a2 - b2;
+        const double z = cz + disp, nz = 2.0 * disp * z;
+        const double xz = 2.0 * mx * z, yz = 2.0 * my * z, xy = 2.0 * mx * my;
+        const double g0 = (eq[0] * (z * z - mx * mx + my * my) - xy * eq[1] - xz * eq[2]) / nz;
+        const double g1 = (eq[1] * (z * z + mx * mx - my * my) - xy * eq[0] - yz * eq[2]) / nz;
+        const double xyl = sqrt(g0 * g0 + g1 * g1);
+        const double p2 = 2.0 * sqrt(a2 * b2);
+        const bool flat = ref_is_zero(xyl);
+        const double prod = p2 * (flat ? 1.0 : g0 / xyl);
+        const double imag = p2 * (flat ? 0.0 : g1 / xyl);
+        const double nrm = disp * z;
+        const double fd = diff * z / nrm;
+        const double inv = 1.0 / nrm;
+        const double O0 = (prod * (my * my + cz * z) + imag * (-mx * my)) * inv;
+        const double O1 = (prod * (-mx * my) + imag * (mx * mx + cz * z)) * inv;
+        const double O2 = (prod * (-mx * z) + imag * (-my * z)) * inv;
+        const double rd = J.relax_rate, ro = J.relax_rate2;
+        out[0] -= rd * (s[0] * (1.0 / 3.0) - mx * fd) + ro * (s[0] * (2.0 / 3.0) - O0);
+        out[1] -= rd * (s[1] * (1.0 / 3.0) - my * fd) + ro * (s[1] * (2.0 / 3.0) - O1);
+        out[2] -= rd * (s[2] * (1.0 / 3.0) - cz * fd) + ro * (s[2] * (2.0 / 3.0) - O2);
+    }
+
     // ds/dt = 2 d(k - A) x s - relaxation
     __device__ __forceinline__ void rhs_A(const double s[3], double A, double out[3]) const {
+        if constexpr (M == MODEL_PIFLUX3D) {
+            // 4t (m x s) (PiFlux.cpp:89-94)
+            const double cz = cos(kz - A), pf = 4.0 * J.hop;
+            out[0] = (cy * s[2] - cz * s[1]) * pf;
+            out[1] = (cz * s[0] - cx * s[2]) * pf;
+            out[2] = (cx * s[1] - cy * s[0]) * pf;
+            if constexpr (R == RELAX_EQUILIBRIUM) relax_equilibrium(cz, s, out);
+            return;
+        }
         double d[3];
         d_of(A, d);
         const double wx = 2.0 * d[0], wy = 2.0 * d[1], wz = 2.0 * d[2];
@@ -276,7 +364,9 @@ struct Bloch {
 
     // current vertex g(s, A) along the polarisation
     __device__ __forceinline__ double vertex(double A, const double s[3]) const {
-        if constexpr (M == MODEL_DIRAC3D) {
+        if constexpr (M == MODEL_PIFLUX3D) {
+            return sin(kz - A) * s[2];  // PiFlux.cpp:831-838
+        } else if constexpr (M == MODEL_DIRAC3D) {
             return s[2];  // band inversion; the weight carries k_z kappa/|k| (DiracSystem.cpp:279)
         } else if constexpr (M == MODEL_DIRAC2D) {
             return -J.v * (J.ct * s[0] + J.st * s[1]);
@@ -371,6 +461,7 @@ template <class F>
 __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double h, const double As[13],
                                                 double snew[3], double rtol, double atol) {
     double k[13][3];
+    f.begin_step(s);
     f.rhs_A(s, As[0], k[0]);
     const double k1[3] = {k[0][0], k[0][1], k[0][2]};
 #pragma unroll

@@ -89,16 +89,66 @@ int64_t kgrid_size(const hhg_kgrid_desc& g) {
         if (g.nx < 2 || g.ny < 1) return 0;
         return (int64_t)(g.nx - 2) * (int64_t)g.ny;  // z = 1..nx-1 without nx/2
     }
+    if (g.kind == HHG_KGRID_PIFLUX_CONTINUUM) {
+        if (g.nx < 1 || g.ny < 1) return 0;
+        const int64_t C = g.ny, F = 8 * C;
+        return 2 * (int64_t)g.nx * (2 * F * F + 20 * C * C);
+    }
     return (int64_t)g.nx * (int64_t)g.ny;
 }
 
-hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* w) {
+hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* kz, double* w) {
+    const int64_t n_all = kgrid_size(g);
+    if (kz) std::fill(kz, kz + std::max<int64_t>(0, n_all), 0.0);
     if (g.kind == HHG_KGRID_EXPLICIT) {
         if (g.n_points < 0 || (g.n_points > 0 && (!g.kx || !g.ky || !g.w)))
             return fail(HHG_ERROR_INVALID_ARGUMENT, "explicit k-grid needs kx, ky, w");
         std::memcpy(kx, g.kx, sizeof(double) * g.n_points);
         std::memcpy(ky, g.ky, sizeof(double) * g.n_points);
         std::memcpy(w, g.w, sizeof(double) * g.n_points);
+        if (kz && g.kz) std::memcpy(kz, g.kz, sizeof(double) * g.n_points);
+        return HHG_SUCCESS;
+    }
+    if (g.kind == HHG_KGRID_PIFLUX_CONTINUUM) {
+        // PiFlux::current_density_continuum_limit (PiFlux.cpp:794-863): z = +-pi x at the nx positive nodes of
+        // the 2 nx-point rule, weight w_z; improved_xy_integral (PiFlux.cpp:665-792): four Gauss patches on
+        // [0, pi] x [0, pi/2] around edge = 0.35 pi, each dimension a 2N-point rule for the patch's N
+        // (N_fine = 8 N_coarse, N_coarse = ny), weight w_x w_y (x_high - x_low)/2 (y_high - y_low)/2
+        if (g.nx < 1 || g.ny < 1) return fail(HHG_ERROR_INVALID_ARGUMENT, "pi-flux grid needs nx, ny >= 1");
+        if (!kz) return fail(HHG_ERROR_INVALID_ARGUMENT, "pi-flux grid needs the third coordinate");
+        const double pi = 3.14159265358979323846, edge = 0.35 * pi;
+        const int C = g.ny, F = 8 * C;
+        struct Patch {
+            double xl, xh;
+            int nx;
+            double yl, yh;
+            int ny;
+        };
+        const Patch patches[4] = {{edge, pi - edge, F, edge, 0.5 * pi, F / 2},
+                                  {0.0, pi, 3 * C, 0.0, edge, C},
+                                  {0.0, edge, C, edge, 0.5 * pi, C},
+                                  {pi - edge, pi, C, edge, 0.5 * pi, C}};
+        std::vector<double> gz(2 * g.nx), wz(2 * g.nx);
+        gauss_legendre(2 * g.nx, gz.data(), wz.data());
+        int64_t k = 0;
+        for (int z = g.nx; z < 2 * g.nx; ++z) {  // positive nodes
+            for (int sgn = 1; sgn >= -1; sgn -= 2) {
+                const double zz = sgn * pi * gz[z];
+                for (const Patch& P : patches) {
+                    std::vector<double> gx(2 * P.nx), wx(2 * P.nx), gy(2 * P.ny), wy(2 * P.ny);
+                    gauss_legendre(2 * P.nx, gx.data(), wx.data());
+                    gauss_legendre(2 * P.ny, gy.data(), wy.data());
+                    const double tw = 0.5 * (P.xh - P.xl) * 0.5 * (P.yh - P.yl);
+                    for (int i = 0; i < 2 * P.nx; ++i)
+                        for (int j = 0; j < 2 * P.ny; ++j, ++k) {
+                            kx[k] = 0.5 * (P.xh - P.xl) * gx[i] + 0.5 * (P.xh + P.xl);
+                            ky[k] = 0.5 * (P.yh - P.yl) * gy[j] + 0.5 * (P.yh + P.yl);
+                            kz[k] = zz;
+                            w[k] = wz[z] * (wx[i] * tw) * wy[j];
+                        }
+                }
+            }
+        }
         return HHG_SUCCESS;
     }
     if (g.kind == HHG_KGRID_DIRAC_CYLINDER) {
@@ -329,10 +379,10 @@ hhg_status hhg_gauss_legendre(int32_t n, double* x, double* w) {
 
 int64_t hhg_kgrid_size(const hhg_kgrid_desc* g) { return g ? hhg::kgrid_size(*g) : -1; }
 
-hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, double* w) {
+hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, double* kz, double* w) {
     hhg::clear_error();
     if (!g || !kx || !ky || !w) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_kgrid_build: null argument");
-    return hhg::kgrid_build(*g, kx, ky, w);
+    return hhg::kgrid_build(*g, kx, ky, kz, w);
 }
 
 hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n) {

@@ -17,7 +17,7 @@ const char* last_error();
 
 void gauss_legendre(int n, double* x, double* w);
 int64_t kgrid_size(const hhg_kgrid_desc& g);
-hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* w);
+hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* kz, double* w);
 double pulse_eval(const hhg_pulse_desc& p, double t);
 hhg_status trapezoidal_spectrum(const double* j, const hhg_time_desc& tc, int zero_padding, double* freq, double* re,
                                 double* im);

@@ -2,6 +2,7 @@
 //
 // Device memory of a plan (all FP64 unless stated):
 //   kx, ky, w        : SoA k-points of every k-grid, concatenated
+//   kz               : third coordinate (only when a job uses the 3D pi-flux lattice)
 //   perm (int64)     : optional lane -> k-point map per job (cost grouping)
 //   jobs (JobDev)    : one record per job
 //   items (int2)     : work items {job, chunk}; one item = 64 consecutive
@@ -17,7 +18,14 @@
 
 namespace hhg {
 
-enum : int { MODEL_DIRAC2D = 0, MODEL_PIFLUX2D = 1, MODEL_HONEYCOMB = 2, MODEL_DIRAC3D = 3, N_MODELS = 4 };
+enum : int {
+    MODEL_DIRAC2D = 0,
+    MODEL_PIFLUX2D = 1,
+    MODEL_HONEYCOMB = 2,
+    MODEL_DIRAC3D = 3,
+    MODEL_PIFLUX3D = 4,
+    N_MODELS = 5
+};
 enum : int {
     PULSE_COS2 = 0,
     PULSE_GAUSS = 1,
@@ -27,7 +35,7 @@ enum : int {
     N_PULSES = 5
 };
 enum : int { METHOD_RKF78 = 0, METHOD_MAGNUS4 = 1, METHOD_MAGNUS_MOMENTS = 2 };
-enum : int { RELAX_NONE = 0, RELAX_TO_INITIAL = 1, RELAX_T2 = 2 };
+enum : int { RELAX_NONE = 0, RELAX_TO_INITIAL = 1, RELAX_T2 = 2, RELAX_EQUILIBRIUM = 3 };
 enum : int { PROPOSAL_RESTART = 0, PROPOSAL_CARRY = 1, PROPOSAL_LEARNED = 2 };
 
 constexpr int WAVE = 64;
@@ -48,7 +56,8 @@ struct JobDev {
     double v, mass, hop;
     double ct, st;        // cos/sin of the polarisation angle
     double fermi, beta;   // beta <= 0: T = 0
-    double relax_rate;    // 1 / relax_time
+    double relax_rate;    // 1 / relax_time (equilibrium relaxation: the diagonal rate)
+    double relax_rate2;   // equilibrium relaxation: the off-diagonal rate
     // pulse
     int32_t pulse, theta_zero;
     double amp_a, amp_b;
@@ -59,7 +68,8 @@ struct JobDev {
     // time grid
     double t_begin, delta;
     int32_t n_meas, n_sub;
-    int32_t magnus_sub, pad0;
+    int32_t magnus_sub;
+    int32_t ic_rescue;    // 3D pi-flux continuum grid: rescued cos k_z in the initial state
     // k-grid slice and output placement
     int64_t k_base, n_k;
     int64_t row_base;     // first partial row double of this job

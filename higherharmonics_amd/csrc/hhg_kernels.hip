@@ -112,6 +112,7 @@ hipError_t by_model(int model, F&& f) {
     case MODEL_PIFLUX2D: return f(std::integral_constant<int, MODEL_PIFLUX2D>{});
     case MODEL_HONEYCOMB: return f(std::integral_constant<int, MODEL_HONEYCOMB>{});
     case MODEL_DIRAC3D: return f(std::integral_constant<int, MODEL_DIRAC3D>{});
+    case MODEL_PIFLUX3D: return f(std::integral_constant<int, MODEL_PIFLUX3D>{});
     }
     return hipErrorInvalidValue;
 }

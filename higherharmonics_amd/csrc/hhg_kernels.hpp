@@ -17,6 +17,7 @@ struct KernelArgs {
     int fallback_after;  // learned proposal: rejections before the plain controller takes over
     const double* kx;
     const double* ky;
+    const double* kz;  // 3D pi-flux lattice only (nullptr otherwise)
     const double* w;
     const int64_t* perm;  // nullptr: identity lane -> k map; else per job relative k index
     const double* At;
@@ -37,6 +38,7 @@ struct TrajArgs {
     const JobDev* jobs;
     const double* kx;
     const double* ky;
+    const double* kz;
     const int64_t* sel;
     int64_t n_sel;
     const double* Am;

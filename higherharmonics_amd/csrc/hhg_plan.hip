@@ -52,7 +52,7 @@ struct hhg_plan {
     std::vector<Group> groups;
     int num_cus = 0;
 
-    double *d_kx = nullptr, *d_ky = nullptr, *d_w = nullptr;
+    double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_w = nullptr;
     hhg::JobDev* d_jobs = nullptr;
     int2* d_items = nullptr;
     double *d_At = nullptr, *d_Am = nullptr, *d_nom = nullptr;
@@ -100,13 +100,24 @@ hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n
     const hhg_pulse_desc& p = jd.pulse;
     const hhg_time_desc& t = jd.time;
     if (jd.kgrid < 0 || jd.kgrid >= n_kgrids) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "k-grid index out of range");
-    if (m.kind < 0 || m.kind > 3) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown model kind");
-    if (m.relax < 0 || m.relax > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown relaxation kind");
+    if (m.kind < 0 || m.kind > 4) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown model kind");
+    if (m.relax < 0 || m.relax > 3) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown relaxation kind");
+    const bool piflux3d = m.kind == HHG_MODEL_PIFLUX3D;
+    if (piflux3d && m.relax != HHG_RELAX_NONE && m.relax != HHG_RELAX_EQUILIBRIUM)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D pi-flux lattice relaxes toward its equilibrium only");
+    if (!piflux3d && m.relax == HHG_RELAX_EQUILIBRIUM)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "equilibrium relaxation is the 3D pi-flux lattice's");
+    // the equilibrium relaxation reads the integrator's state at the start of
+    // every step (PiFlux.cpp:144), so j(t) depends on the step sequence: only
+    // the reference's restart controller reproduces it
+    if (m.relax == HHG_RELAX_EQUILIBRIUM && sv.proposal != HHG_PROPOSAL_RESTART)
+        return fail(HHG_ERROR_UNSUPPORTED,
+                    at + "equilibrium relaxation depends on the step sequence: it needs HHG_PROPOSAL_RESTART");
     if (m.relax != HHG_RELAX_NONE && !(m.relax_time > 0.0))
         return fail(HHG_ERROR_INVALID_ARGUMENT, at + "relaxation needs relax_time > 0");
-    const bool lattice = m.kind == HHG_MODEL_PIFLUX2D || m.kind == HHG_MODEL_HONEYCOMB;
+    const bool lattice = m.kind == HHG_MODEL_PIFLUX2D || m.kind == HHG_MODEL_HONEYCOMB || piflux3d;
     if (lattice && m.theta != 0.0)
-        return fail(HHG_ERROR_UNSUPPORTED, at + "lattice models take the field along x (theta = 0)");
+        return fail(HHG_ERROR_UNSUPPORTED, at + "lattice models take the field along one axis (theta = 0)");
     if (m.kind == HHG_MODEL_DIRAC3D && m.theta != 0.0)
         return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D Dirac cone takes the field along z (theta = 0)");
     if (lattice && !(m.hopping > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "lattice models need hopping > 0");
@@ -178,6 +189,7 @@ hhg::JobDev make_job(const hhg_job_desc& jd, const hhg_solver_desc& sv) {
     J.fermi = m.fermi_energy;
     J.beta = m.beta;
     J.relax_rate = m.relax != HHG_RELAX_NONE ? 1.0 / m.relax_time : 0.0;
+    J.relax_rate2 = m.relax == HHG_RELAX_EQUILIBRIUM && m.relax_time_offdiag > 0.0 ? 1.0 / m.relax_time_offdiag : 0.0;
     J.pulse = p.kind;
     J.theta_zero = m.theta == 0.0;
     J.amp_a = p.amp_a;
@@ -204,6 +216,7 @@ void destroy_device(hhg_plan* p) {
         if (e) hipEventDestroy(e);
     hipFree(p->d_kx);
     hipFree(p->d_ky);
+    hipFree(p->d_kz);
     hipFree(p->d_w);
     hipFree(p->d_jobs);
     hipFree(p->d_items);
@@ -258,6 +271,7 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.queue_chunk = g.queue_chunk;
         a.kx = p->d_kx;
         a.ky = p->d_ky;
+        a.kz = p->d_kz;
         a.w = p->d_w;
         a.perm = nullptr;
         a.At = p->d_At;
@@ -383,12 +397,14 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         grid_base[g + 1] = grid_base[g] + grid_size[g];
     }
     p->n_kpts = grid_base[d->n_kgrids];
-    std::vector<double> kx(p->n_kpts), ky(p->n_kpts), w(p->n_kpts);
+    std::vector<double> kx(p->n_kpts), ky(p->n_kpts), kz(p->n_kpts), w(p->n_kpts);
     for (int g = 0; g < d->n_kgrids; ++g) {
         const hhg_status st = hhg::kgrid_build(d->kgrids[g], kx.data() + grid_base[g], ky.data() + grid_base[g],
-                                               w.data() + grid_base[g]);
+                                               kz.data() + grid_base[g], w.data() + grid_base[g]);
         if (st != HHG_SUCCESS) return bail(st);
     }
+    bool has_kz = false;  // only the 3D pi-flux lattice reads the third coordinate
+    for (int i = 0; i < d->n_jobs; ++i) has_kz |= d->jobs[i].model.kind == HHG_MODEL_PIFLUX3D;
 
     // jobs: shard = contiguous block of 64-point chunks of each job's grid
     p->offsets.assign(d->n_jobs + 1, 0);
@@ -396,6 +412,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     for (int i = 0; i < d->n_jobs; ++i) {
         hhg::JobDev J = make_job(d->jobs[i], sv);
         const int g = d->jobs[i].kgrid;
+        J.ic_rescue = J.model == hhg::MODEL_PIFLUX3D && d->kgrids[g].kind == HHG_KGRID_PIFLUX_CONTINUUM;
         const int64_t n_chunks = (grid_size[g] + hhg::WAVE - 1) / hhg::WAVE;
         const int64_t per_rank = (n_chunks + n_ranks - 1) / n_ranks;
         const int64_t c0 = std::min(n_chunks, per_rank * d->rank), c1 = std::min(n_chunks, c0 + per_rank);
@@ -448,6 +465,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     };
     A((void**)&p->d_kx, nk, "kx");
     A((void**)&p->d_ky, nk, "ky");
+    if (has_kz) A((void**)&p->d_kz, nk, "kz");
     A((void**)&p->d_w, nk, "w");
     A((void**)&p->d_jobs, sizeof(hhg::JobDev) * p->jobs.size(), "jobs");
     A((void**)&p->d_items, sizeof(int2) * std::max<size_t>(1, items.size()), "items");
@@ -462,6 +480,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     };
     C(p->d_kx, kx.data(), sizeof(double) * p->n_kpts, "upload kx");
     C(p->d_ky, ky.data(), sizeof(double) * p->n_kpts, "upload ky");
+    if (has_kz) C(p->d_kz, kz.data(), sizeof(double) * p->n_kpts, "upload kz");
     C(p->d_w, w.data(), sizeof(double) * p->n_kpts, "upload w");
     C(p->d_jobs, p->jobs.data(), sizeof(hhg::JobDev) * p->jobs.size(), "upload jobs");
     C(p->d_items, items.data(), sizeof(int2) * items.size(), "upload items");
@@ -508,6 +527,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
             a.n_items = g.n_items;
             a.kx = p->d_kx;
             a.ky = p->d_ky;
+            a.kz = p->d_kz;
             a.At = p->d_At;
             a.mlp = p->d_mlp;
             if (st == HHG_SUCCESS)
@@ -715,7 +735,7 @@ hhg_status hhg_trajectories(hhg_plan* p, const int64_t* k_index, int64_t n_sel, 
         hhg::SolverDev sp{p->solver.rtol, p->solver.atol, p->solver.max_fail > 0 ? p->solver.max_fail : 500,
                           p->solver.proposal};
         const int pulse = p->solver.method == HHG_METHOD_MAGNUS4 ? 0 : J.pulse;
-        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, d_sel, n_sel, p->d_Am, sp, d_s, p->counters(0), p->stream};
+        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, p->d_kz, d_sel, n_sel, p->d_Am, sp, d_s, p->counters(0), p->stream};
         st = check_hip(hhg::launch_trajectories(p->solver.method, J.model, pulse, J.relax, x), "trajectory launch");
     }
     if (st == HHG_SUCCESS) st = check_hip(hipStreamSynchronize(p->stream), "synchronize");

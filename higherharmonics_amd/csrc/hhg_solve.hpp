@@ -95,19 +95,27 @@ __device__ __forceinline__ void interval_features(const F& f, JobRef J, const do
 
 extern __shared__ __align__(16) unsigned char solve_dyn_lds[];
 
+// third coordinate of a k-point (3D pi-flux lattice; the other models have none)
+template <int M>
+__device__ __forceinline__ double load_kz(const double* kz, int64_t idx) {
+    if constexpr (M == MODEL_PIFLUX3D) return kz[idx];
+    else return 0.0;
+}
+
 template <int M, int P, int R, bool LEARNED>
 __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t kk = (int64_t)chunk * WAVE + lane;
     const bool active = kk < J.n_k;
-    double kx = 0.0, ky = 0.0, w = 0.0;
+    double kx = 0.0, ky = 0.0, kz = 0.0, w = 0.0;
     if (active) {
         const int64_t idx = J.k_base + (a.perm ? a.perm[J.k_base + kk] : kk);
         kx = a.kx[idx];
         ky = a.ky[idx];
+        kz = load_kz<M>(a.kz, idx);
         w = a.w[idx];
     }
-    Bloch<M, P, R> f(J, kx, ky);
+    Bloch<M, P, R> f(J, kx, ky, kz);
     double s[3];
     f.initial(s);
     double gap_k;
@@ -211,14 +219,15 @@ __device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, Lane
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t kk = (int64_t)chunk * WAVE + lane;
     const bool active = kk < J.n_k;
-    double kx = 0.0, ky = 0.0, w = 0.0;
+    double kx = 0.0, ky = 0.0, kz = 0.0, w = 0.0;
     if (active) {
         const int64_t idx = J.k_base + (a.perm ? a.perm[J.k_base + kk] : kk);
         kx = a.kx[idx];
         ky = a.ky[idx];
+        kz = load_kz<M>(a.kz, idx);
         w = a.w[idx];
     }
-    Bloch<M, PULSE_COS2, RELAX_NONE> f(J, kx, ky);  // the pulse enters only through the Am table
+    Bloch<M, PULSE_COS2, RELAX_NONE> f(J, kx, ky, kz);  // the pulse enters only through the Am table
     double s[3];
     f.initial(s);
     const int N = J.n_meas;
@@ -314,13 +323,14 @@ solve_kernel(KernelArgs a) {
 // k-point, Bloch vector at every sample.
 template <int METHOD, int M, int P, int R>
 __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, const double* kx, const double* ky,
-                                                        const int64_t* sel, int64_t n_sel, const double* Am,
-                                                        SolverDev sp, double* s_out, Counters* ctr) {
+                                                        const double* kz, const int64_t* sel, int64_t n_sel,
+                                                        const double* Am, SolverDev sp, double* s_out,
+                                                        Counters* ctr) {
     const int64_t t_idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t_idx >= n_sel) return;
     JobRef J = job_ref(jobs);
     const int64_t idx = J.k_base + sel[t_idx];
-    Bloch<M, P, R> f(J, kx[idx], ky[idx]);
+    Bloch<M, P, R> f(J, kx[idx], ky[idx], solve_detail::load_kz<M>(kz, idx));
     double s[3];
     f.initial(s);
     const int N = J.n_meas;
@@ -394,7 +404,7 @@ __global__ __launch_bounds__(256) void cost_probe_kernel(KernelArgs a, int n_pro
     const int64_t kk = (int64_t)item.y * WAVE + lane;
     const bool active = kk < J.n_k;
     const int64_t idx = J.k_base + (active ? kk : 0);
-    Bloch<M, P, RELAX_NONE> f(J, a.kx[idx], a.ky[idx]);
+    Bloch<M, P, RELAX_NONE> f(J, a.kx[idx], a.ky[idx], load_kz<M>(a.kz, idx));
     double d[3];
     f.d_of(0.0, d);
     const double gap_k = 2.0 * sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -410,9 +420,15 @@ __global__ __launch_bounds__(256) void cost_probe_kernel(KernelArgs a, int n_pro
     if (active) cost[J.k_base + kk] = total;
 }
 
-// Run-time (method, pulse, relax, learned) -> template, for one model.
+// Run-time (method, pulse, relax, learned) -> template, for one model.  The
+// 3D pi-flux lattice has no relaxation or its equilibrium relaxation; the
+// other models have the other three.
 template <int M>
 struct ModelLaunch {
+    template <int R>
+    static constexpr bool has_relax =
+        M == MODEL_PIFLUX3D ? (R == RELAX_NONE || R == RELAX_EQUILIBRIUM) : (R != RELAX_EQUILIBRIUM);
+
     template <int P, int R, bool L>
     static hipError_t solve_t(const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,
                               const void** fn) {
@@ -426,8 +442,11 @@ struct ModelLaunch {
     template <int P, int R>
     static hipError_t solve_l(bool learned, const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,
                               const void** fn) {
-        return learned ? solve_t<P, R, true>(a, blocks, threads, smem, st, fn)
-                       : solve_t<P, R, false>(a, blocks, threads, smem, st, fn);
+        if constexpr (has_relax<R>)
+            return learned ? solve_t<P, R, true>(a, blocks, threads, smem, st, fn)
+                           : solve_t<P, R, false>(a, blocks, threads, smem, st, fn);
+        else
+            return hipErrorInvalidValue;
     }
     template <int P>
     static hipError_t solve_r(int relax, bool learned, const KernelArgs& a, int blocks, int threads, size_t smem,
@@ -436,6 +455,7 @@ struct ModelLaunch {
         case RELAX_NONE: return solve_l<P, RELAX_NONE>(learned, a, blocks, threads, smem, st, fn);
         case RELAX_TO_INITIAL: return solve_l<P, RELAX_TO_INITIAL>(learned, a, blocks, threads, smem, st, fn);
         case RELAX_T2: return solve_l<P, RELAX_T2>(learned, a, blocks, threads, smem, st, fn);
+        case RELAX_EQUILIBRIUM: return solve_l<P, RELAX_EQUILIBRIUM>(learned, a, blocks, threads, smem, st, fn);
         }
         return hipErrorInvalidValue;
     }
@@ -474,19 +494,24 @@ struct ModelLaunch {
 
     template <int P, int R>
     static hipError_t traj_t(int method, const TrajArgs& x) {
+        if constexpr (has_relax<R>) return traj_i<P, R>(method, x);
+        else return hipErrorInvalidValue;
+    }
+    template <int P, int R>
+    static hipError_t traj_i(int method, const TrajArgs& x) {
         const int blocks = (int)((x.n_sel + 63) / 64);
         if (method == METHOD_MAGNUS_MOMENTS) {
             if constexpr (M == MODEL_DIRAC3D)
                 hipLaunchKernelGGL((trajectory_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE>), dim3(blocks), dim3(64), 0,
-                                   x.st, x.jobs, x.kx, x.ky, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
+                                   x.st, x.jobs, x.kx, x.ky, x.kz, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
             else
                 return hipErrorInvalidValue;
         } else if (method == METHOD_MAGNUS4)
             hipLaunchKernelGGL((trajectory_kernel<1, M, PULSE_COS2, RELAX_NONE>), dim3(blocks), dim3(64), 0, x.st,
-                               x.jobs, x.kx, x.ky, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
+                               x.jobs, x.kx, x.ky, x.kz, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
         else
             hipLaunchKernelGGL((trajectory_kernel<0, M, P, R>), dim3(blocks), dim3(64), 0, x.st, x.jobs, x.kx, x.ky,
-                               x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
+                               x.kz, x.sel, x.n_sel, x.Am, x.sp, x.s_out, x.ctr);
         return hipGetLastError();
     }
     template <int P>
@@ -495,6 +520,7 @@ struct ModelLaunch {
         case RELAX_NONE: return traj_t<P, RELAX_NONE>(method, x);
         case RELAX_TO_INITIAL: return traj_t<P, RELAX_TO_INITIAL>(method, x);
         case RELAX_T2: return traj_t<P, RELAX_T2>(method, x);
+        case RELAX_EQUILIBRIUM: return traj_t<P, RELAX_EQUILIBRIUM>(method, x);
         }
         return hipErrorInvalidValue;
     }

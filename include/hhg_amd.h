@@ -58,15 +58,21 @@ typedef enum hhg_model_kind {
     HHG_MODEL_DIRAC2D = 0,   /* d = (v qx, v qy, mass) */
     HHG_MODEL_PIFLUX2D = 1,  /* d = 2t (cos qx, cos qy, 0); cf. PiFlux.cpp:89-94 */
     HHG_MODEL_HONEYCOMB = 2, /* d = t (Re g, -Im g, 0), g = sum_delta e^{i q.delta}; Honeycomb.cpp:44-50,130-138 */
-    HHG_MODEL_DIRAC3D = 3    /* the reference's 3D Dirac cone in its band frame, field along z, point (k_z, kappa)
+    HHG_MODEL_DIRAC3D = 3,   /* the reference's 3D Dirac cone in its band frame, field along z, point (k_z, kappa)
                                 in (kx, ky): d = -(A kappa/|k|, 0, |k| - A k_z/|k|), DiracSystem.cpp:138-174;
                                 current vertex s_z; lengths in hbar omega_L / v_F */
+    HHG_MODEL_PIFLUX3D = 4   /* the reference's 3D pi-flux lattice, field along z, point (x, y, z) in (kx, ky, kz):
+                                d = 2t (cos x, cos y, cos(z - A)), PiFlux.cpp:73-111; initial state PiFlux.cpp:519-549
+                                with the occupations of PiFlux.cpp:420-426; current vertex sin(z - A) s_z
+                                (PiFlux.cpp:831-838) */
 } hhg_model_kind;
 
 typedef enum hhg_relax_kind {
     HHG_RELAX_NONE = 0,
     HHG_RELAX_TO_INITIAL = 1, /* -(s - s0)/tau, DiracSystem.cpp:231-233 */
-    HHG_RELAX_T2 = 2          /* -(s - (s.d^)d^)/T2 about the instantaneous d(k - A(t)) */
+    HHG_RELAX_T2 = 2,         /* -(s - (s.d^)d^)/T2 about the instantaneous d(k - A(t)) */
+    HHG_RELAX_EQUILIBRIUM = 3 /* HHG_MODEL_PIFLUX3D: toward the instantaneous equilibrium of d(k - A(t)), diagonal
+                                 time relax_time, off-diagonal time relax_time_offdiag (PiFlux.cpp:113-220) */
 } hhg_relax_kind;
 
 typedef struct hhg_model_desc {
@@ -79,6 +85,7 @@ typedef struct hhg_model_desc {
     double fermi_energy; /* E_F in hbar omega_L */
     double beta;         /* 1/T in 1/(hbar omega_L); <= 0 means T = 0 */
     double relax_time;   /* tau or T2 in 1/omega_L */
+    double relax_time_offdiag; /* HHG_RELAX_EQUILIBRIUM: off-diagonal relaxation time in 1/omega_L (<= 0: none) */
 } hhg_model_desc;
 
 /* ---- pulses: A(t) = amplitude * envelope * sin(t + cep) ---- */
@@ -116,10 +123,18 @@ typedef enum hhg_kgrid_kind {
     HHG_KGRID_GL_BOX = 0,  /* nx x ny Gauss-Legendre on [x0,x1] x [y0,y1], x-major */
     HHG_KGRID_GL_HEX = 1,  /* hexagonal BZ of Honeycomb.cpp:229-268 (nx x ny nodes) */
     HHG_KGRID_EXPLICIT = 2, /* caller's points and weights */
-    HHG_KGRID_DIRAC_CYLINDER = 3 /* (k_z, kappa) of the 3D Dirac cone: k_z = (z - nx/2) dk, z = 1..nx-1 except nx/2,
-                                    dk = 2 x1 / nx (DiracSystem.cpp:280-306); kappa: ny-point Gauss-Legendre on
-                                    (0, sqrt(x1^2 - k_z^2)) where the reference refines an adaptive trapezoid;
-                                    weight dk w_kappa k_z kappa / |k| (DiracSystem.cpp:279); x1 = band width */
+    HHG_KGRID_DIRAC_CYLINDER = 3, /* (k_z, kappa) of the 3D Dirac cone: k_z = (z - nx/2) dk, z = 1..nx-1 except nx/2,
+                                     dk = 2 x1 / nx (DiracSystem.cpp:280-306); kappa: ny-point Gauss-Legendre on
+                                     (0, sqrt(x1^2 - k_z^2)) where the reference refines an adaptive trapezoid;
+                                     weight dk w_kappa k_z kappa / |k| (DiracSystem.cpp:279); x1 = band width */
+    HHG_KGRID_PIFLUX_CONTINUUM = 4 /* (x, y, z) of the 3D pi-flux lattice, PiFlux::current_density_continuum_limit
+                                      (PiFlux.cpp:794-863) with improved_xy_integral (PiFlux.cpp:665-792): z = +-pi x_z
+                                      at the nx positive nodes of the 2 nx-point Gauss-Legendre rule, four Gauss
+                                      patches in (x, y) on [0, pi] x [0, pi/2] with ny = N_coarse (PiFlux.hpp:12-29,
+                                      the reference's nx = 32, ny = 4); weights as the reference's, without Jacobians
+                                      of the z map.  Jobs on this grid start, like the reference's continuum
+                                      driver, from cos z scaled by 1 - 5e-9 at the +z nodes with cos z < 0
+                                      (PiFlux.cpp:825-829) */
 } hhg_kgrid_kind;
 
 typedef struct hhg_kgrid_desc {
@@ -132,6 +147,7 @@ typedef struct hhg_kgrid_desc {
     const double* ky;
     const double* w;
     int64_t n_points;
+    const double* kz; /* explicit grids only, optional: third coordinate (3D pi-flux points) */
 } hhg_kgrid_desc;
 
 /* One job = one (model, pulse, time grid, k-grid) current j(t). */
@@ -254,9 +270,10 @@ hhg_status hhg_trajectories(hhg_plan* plan, const int64_t* k_index, int64_t n_se
 /* Gauss-Legendre nodes/weights on [-1, 1], nodes ascending. */
 hhg_status hhg_gauss_legendre(int32_t n, double* x, double* w);
 
-/* Number of points of a k-grid, and the points/weights themselves. */
+/* Number of points of a k-grid, and the points/weights themselves (kz may be
+ * null except for HHG_KGRID_PIFLUX_CONTINUUM; it is 0 for 2D grids). */
 int64_t hhg_kgrid_size(const hhg_kgrid_desc* g);
-hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, double* w);
+hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, double* kz, double* w);
 
 /* A(t) at n times (host evaluation, Laser::laser_function). */
 hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n);

@@ -23,9 +23,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
-MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D = 0, 1, 2, 3
+MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D, MODEL_PIFLUX3D = 0, 1, 2, 3, 4
 PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS = 0, 1, 2, 3, 4
-RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2 = 0, 1, 2
+RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2, RELAX_EQUILIBRIUM = 0, 1, 2, 3
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY = 0, 1
 
@@ -33,7 +33,8 @@ PROPOSAL_RESTART, PROPOSAL_CARRY = 0, 1
 class Model(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
                 ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
-                ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double)]
+                ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double),
+                ("relax_time_offdiag", ctypes.c_double), ("ic_rescue", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class Pulse(ctypes.Structure):
@@ -80,19 +81,19 @@ def lib(fast: bool = False) -> ctypes.CDLL:
     L.or_pulse_A.argtypes = [ctypes.POINTER(Pulse), ctypes.c_double]
     L.or_gauss_legendre.argtypes = [ctypes.c_int, dp, dp]
     L.or_rkf78_tableau.argtypes = [dp, dp, dp, dp]
-    L.or_initial_state.argtypes = [ctypes.POINTER(Model), ctypes.c_double, ctypes.c_double, dp]
-    L.or_rhs.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.c_double, ctypes.c_double, dp, dp,
-                         ctypes.c_double, dp]
+    d3 = [ctypes.c_double] * 3
+    L.or_initial_state.argtypes = [ctypes.POINTER(Model), *d3, dp]
+    L.or_rhs.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), *d3, dp, dp, ctypes.c_double, dp]
     L.or_vertex.restype = ctypes.c_double
-    L.or_vertex.argtypes = [ctypes.POINTER(Model), ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+    L.or_vertex.argtypes = [ctypes.POINTER(Model), *d3, ctypes.c_double, dp]
     L.or_magnus_moments.argtypes = [ctypes.POINTER(Pulse), ctypes.c_double, ctypes.c_double, dp]
     L.or_magnus_moments_step.argtypes = [ctypes.c_double, ctypes.c_double, dp, ctypes.c_double, dp]
     L.or_trajectory.restype = ctypes.c_int
     L.or_trajectory.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
-                                ctypes.POINTER(Solver), ctypes.c_double, ctypes.c_double, dp, ctypes.POINTER(Stats)]
+                                ctypes.POINTER(Solver), *d3, dp, ctypes.POINTER(Stats)]
     L.or_current_density.restype = ctypes.c_int64
     L.or_current_density.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
-                                     ctypes.POINTER(Solver), dp, dp, dp, ctypes.c_int64, dp, ctypes.POINTER(Stats),
+                                     ctypes.POINTER(Solver), dp, dp, dp, dp, ctypes.c_int64, dp, ctypes.POINTER(Stats),
                                      ctypes.c_int]
     _libs[name] = L
     return L
@@ -103,8 +104,9 @@ def _dp(a: np.ndarray):
 
 
 def model(kind=MODEL_DIRAC2D, v=1.0, mass=0.1, hopping=1.0, theta=0.0, fermi_energy=0.0, beta=0.0,
-          relax=RELAX_NONE, relax_time=0.0) -> Model:
-    return Model(kind, relax, v, mass, hopping, theta, fermi_energy, beta, relax_time)
+          relax=RELAX_NONE, relax_time=0.0, relax_time_offdiag=0.0, ic_rescue=0) -> Model:
+    return Model(kind, relax, v, mass, hopping, theta, fermi_energy, beta, relax_time, relax_time_offdiag,
+                 ic_rescue, 0)
 
 
 def pulse(kind=PULSE_COS2, amp_a=0.5, amp_b=0.0, n_cycles=8.0, sigma=0.0, t0=0.0, delay=0.0, cep=0.0) -> Pulse:
@@ -139,29 +141,31 @@ def rkf78_tableau():
     return c, a.reshape(13, 13), b8, b7
 
 
-def initial_state(m: Model, kx: float, ky: float) -> np.ndarray:
+def initial_state(m: Model, kx: float, ky: float, kz: float = 0.0) -> np.ndarray:
     s = np.empty(3)
-    lib().or_initial_state(ctypes.byref(m), kx, ky, _dp(s))
+    lib().or_initial_state(ctypes.byref(m), kx, ky, kz, _dp(s))
     return s
 
 
-def rhs(m: Model, p: Pulse, kx, ky, s_init, s, t) -> np.ndarray:
+def rhs(m: Model, p: Pulse, kx, ky, s_ref, s, t, kz: float = 0.0) -> np.ndarray:
+    """ds/dt; s_ref is the initial state (relaxation toward it) or the state at
+    the start of the step (equilibrium relaxation)."""
     out = np.empty(3)
-    si = np.ascontiguousarray(s_init, dtype=np.float64)
+    si = np.ascontiguousarray(s_ref, dtype=np.float64)
     ss = np.ascontiguousarray(s, dtype=np.float64)
-    lib().or_rhs(ctypes.byref(m), ctypes.byref(p), kx, ky, _dp(si), _dp(ss), t, _dp(out))
+    lib().or_rhs(ctypes.byref(m), ctypes.byref(p), kx, ky, kz, _dp(si), _dp(ss), t, _dp(out))
     return out
 
 
-def vertex(m: Model, kx, ky, A, s) -> float:
+def vertex(m: Model, kx, ky, A, s, kz: float = 0.0) -> float:
     ss = np.ascontiguousarray(s, dtype=np.float64)
-    return lib().or_vertex(ctypes.byref(m), kx, ky, A, _dp(ss))
+    return lib().or_vertex(ctypes.byref(m), kx, ky, kz, A, _dp(ss))
 
 
-def trajectory(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float):
+def trajectory(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float, kz: float = 0.0):
     out = np.empty((tc.n_measurements + 1, 3))
     st = Stats()
-    r = lib().or_trajectory(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv), kx, ky,
+    r = lib().or_trajectory(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv), kx, ky, kz,
                             _dp(out), ctypes.byref(st))
     if r < 0:
         raise ValueError("unsupported model/solver combination")
@@ -169,16 +173,20 @@ def trajectory(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float):
 
 
 def current_density(m: Model, p: Pulse, tc: Time, sv: Solver, kx, ky, w, n_threads: int = 0,
-                    fast: bool = False):
+                    fast: bool = False, kz=None):
     """j(t_i), i = 0..N, and the solver statistics.  Deterministic for any
     thread count (per-chunk partial rows summed in chunk order)."""
     kx = np.ascontiguousarray(kx, dtype=np.float64)
     ky = np.ascontiguousarray(ky, dtype=np.float64)
     w = np.ascontiguousarray(w, dtype=np.float64)
+    kzp = None
+    if kz is not None:
+        kz = np.ascontiguousarray(kz, dtype=np.float64)
+        kzp = _dp(kz)
     j = np.zeros(tc.n_measurements + 1)
     st = Stats()
     r = lib(fast).or_current_density(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv),
-                                     _dp(kx), _dp(ky), _dp(w), len(kx), _dp(j), ctypes.byref(st), n_threads)
+                                     _dp(kx), _dp(ky), kzp, _dp(w), len(kx), _dp(j), ctypes.byref(st), n_threads)
     if r < 0:
         raise ValueError("unsupported model/solver combination")
     return j, st
@@ -222,6 +230,44 @@ def dirac_cylinder(n_z: int, n_kappa: int, max_k: float):
         ka_all.append(ka)
         w_all.append(dz * 0.5 * kmax * wk * kz * ka / np.sqrt(kz * kz + ka * ka))
     return np.concatenate(kz_all), np.concatenate(ka_all), np.concatenate(w_all)
+
+
+def piflux_continuum(z_range: int = 32, n_coarse: int = 4):
+    """(kx, ky, kz, w) of the 3D pi-flux continuum-limit quadrature
+    (PiFlux::current_density_continuum_limit, sources/HHG/Systems/PiFlux.cpp:794-863):
+    k_z = +-pi x_z over the z_range positive nodes of the 2 z_range-point rule,
+    weight w_z (the +z node first); (kx, ky) by improved_xy_integral
+    (PiFlux.cpp:665-792), four Gauss patches on [0, pi] x [0, pi/2] split at
+    edge = 0.35 pi, a 2N-point rule per side (N_fine = 8 N_coarse):
+    [edge, pi - edge] x [edge, pi/2] with (N_fine, N_fine/2),
+    [0, pi] x [0, edge] with (3 N_coarse, N_coarse),
+    [0, edge] x [edge, pi/2] and [pi - edge, pi] x [edge, pi/2] with (N_coarse, N_coarse).
+    Within a patch x is the slow index, nodes ascending."""
+    edge = 0.35 * np.pi
+    C = n_coarse
+    F = 8 * C
+    patches = [(edge, np.pi - edge, F, edge, 0.5 * np.pi, F // 2), (0.0, np.pi, 3 * C, 0.0, edge, C),
+               (0.0, edge, C, edge, 0.5 * np.pi, C), (np.pi - edge, np.pi, C, edge, 0.5 * np.pi, C)]
+    xy = []
+    for xl, xh, nx, yl, yh, ny in patches:
+        gx, wx = gauss_legendre(2 * nx)
+        gy, wy = gauss_legendre(2 * ny)
+        X = 0.5 * (xh - xl) * gx + 0.5 * (xh + xl)
+        Y = 0.5 * (yh - yl) * gy + 0.5 * (yh + yl)
+        W = np.outer(wx * (0.5 * (xh - xl) * 0.5 * (yh - yl)), wy).ravel()
+        xy.append((np.repeat(X, 2 * ny), np.tile(Y, 2 * nx), W))
+    px = np.concatenate([a[0] for a in xy])
+    py = np.concatenate([a[1] for a in xy])
+    pw = np.concatenate([a[2] for a in xy])
+    gz, wz = gauss_legendre(2 * z_range)
+    out = [], [], [], []
+    for z in range(z_range, 2 * z_range):
+        for sgn in (1.0, -1.0):
+            out[0].append(px)
+            out[1].append(py)
+            out[2].append(np.full(len(px), sgn * np.pi * gz[z]))
+            out[3].append(wz[z] * pw)
+    return tuple(np.concatenate(a) for a in out)
 
 
 def gl_box(nx: int, ny: int, x0: float, x1: float, y0: float, y1: float):

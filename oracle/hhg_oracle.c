@@ -16,6 +16,14 @@
  *       honeycomb: sources/HHG/Systems/Honeycomb.cpp:130-138, omega = 2t(Re g, -Im g, 0));
  *  - relaxation toward the initial state (sources/HHG/Systems/DiracSystem.cpp:231-233)
  *      and, for the 2D configurations, transverse T2 dephasing about d(k - A(t));
+ *  - the 3D pi-flux lattice ds/dt = 4t m x s, m = (cos kx, cos ky, cos(kz - A))
+ *      (sources/HHG/Systems/PiFlux.cpp:89-94), its thermal initial state
+ *      (PiFlux.cpp:519-549) and its relaxation toward the instantaneous
+ *      equilibrium (PiFlux.cpp:112-182, diagonal_sigma PiFlux.cpp:1037-1049);
+ *  - the 3D pi-flux lattice ds/dt = 4t m x s, m = (cos kx, cos ky, cos(kz - A))
+ *      (sources/HHG/Systems/PiFlux.cpp:89-94), its thermal initial state
+ *      (PiFlux.cpp:519-549) and its relaxation toward the instantaneous
+ *      equilibrium (PiFlux.cpp:112-182, diagonal_sigma PiFlux.cpp:1037-1049);
  *  - Boost.Odeint runge_kutta_fehlberg78 + make_controlled + integrate_adaptive,
  *      restarted at dt = Delta / n_subdivisions every sampling interval
  *      (sources/HHG/Systems/DiracSystem.cpp:159-173, Honeycomb.cpp:145-157,
@@ -214,7 +222,14 @@ static void model_d(const or_model* m, double qx, double qy, double d[3]) {
  * of the reference (sources/HHG/Systems/DiracSystem.cpp:143-153), the point
  * given as (k_z, kappa) in (kx, ky), field along z:
  * d = -(A kappa / |k|, 0, |k| - A k_z / |k|). */
-static void model_field(const or_model* m, double kx, double ky, double A, double d[3]) {
+static void model_field(const or_model* m, double kx, double ky, double kz, double A, double d[3]) {
+    if (m->kind == OR_MODEL_PIFLUX3D) {
+        /* d = 2t m, m = (cos kx, cos ky, cos(kz - A)) (PiFlux.cpp:89-94, prefactor 4t = 2 * 2t) */
+        d[0] = 2.0 * m->hopping * cos(kx);
+        d[1] = 2.0 * m->hopping * cos(ky);
+        d[2] = 2.0 * m->hopping * cos(kz - A);
+        return;
+    }
     if (m->kind == OR_MODEL_DIRAC3D) {
         const double k = sqrt(kx * kx + ky * ky);
         d[0] = -A * ky / k;
@@ -231,7 +246,72 @@ static double fermi(double e, double beta) {
     return 1.0 / (1.0 + exp(beta * e));
 }
 
-void or_initial_state(const or_model* m, double kx, double ky, double s[3]) {
+/* is_zero of sources/HHG/GlobalDefinitions.hpp:61-67: biased exponent below
+ * 0b01111010011, i.e. |x| < 2^-44 */
+static int or_is_zero(double x) { return fabs(x) < 0x1p-44; }
+
+/* RESCUE_TRAFO of sources/HHG/Systems/PiFlux.cpp:50 */
+#define OR_RESCUE 1e-4
+
+/* Thermal state of the 3D pi-flux lattice at m = (cx, cy, cz): PiFlux::ic_sigma
+ * (PiFlux.cpp:519-549) with alpha^2 = f(-E_F + 2t|m|), beta^2 = f(-E_F - 2t|m|)
+ * (PiFlux.cpp:420-426) and z_eps = cz + |m|. */
+static void piflux3d_ic(const or_model* m, double cx, double cy, double cz, double s[3]) {
+    const double disp = sqrt(cx * cx + cy * cy + cz * cz);
+    const double a2 = fermi(-m->fermi_energy + 2.0 * m->hopping * disp, m->beta);
+    const double b2 = fermi(-m->fermi_energy - 2.0 * m->hopping * disp, m->beta);
+    const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
+    const double ze = cz + disp, nrm = disp * ze;
+    s[0] = (diff * cx * ze + prod * (cy * cy + cz * ze)) / nrm;
+    s[1] = (diff * cy * ze - prod * cx * cy) / nrm;
+    s[2] = (diff * cz * ze - prod * cx * ze) / nrm;
+}
+
+/* Relaxation targets of the 3D pi-flux lattice at shifted m = (cx, cy, cz),
+ * update_equilibrium_state of PiFlux.cpp:134-166: the diagonal part is
+ * (alpha^2 - beta^2) m/|m|; the off-diagonal part keeps the band-frame phase
+ * of s_ref (diagonal_sigma, PiFlux.cpp:1037-1049) with modulus 2 alpha beta. */
+static void piflux3d_equilibrium(const or_model* m, double cx, double cy, double cz, const double s_ref[3],
+                                 double D[3], double O[3]) {
+    if (or_is_zero(cx) && or_is_zero(cy) && cz < 0.0) {
+        cx = fabs(cz) * OR_RESCUE;
+        cz *= (1.0 - 0.5 * OR_RESCUE * OR_RESCUE);
+    }
+    const double disp = sqrt(cx * cx + cy * cy + cz * cz);
+    const double a2 = fermi(-m->fermi_energy + 2.0 * m->hopping * disp, m->beta);
+    const double b2 = fermi(-m->fermi_energy - 2.0 * m->hopping * disp, m->beta);
+    const double diff = a2 - b2;
+    /* diagonal_sigma(s_ref, m), first two components */
+    const double z = cz + disp, nz = 2.0 * disp * z;
+    const double xz = 2.0 * cx * z, yz = 2.0 * cy * z, xy = 2.0 * cx * cy;
+    const double g0 = (s_ref[0] * (z * z - cx * cx + cy * cy) - xy * s_ref[1] - xz * s_ref[2]) / nz;
+    const double g1 = (s_ref[1] * (z * z + cx * cx - cy * cy) - xy * s_ref[0] - yz * s_ref[2]) / nz;
+    const double xyl = sqrt(g0 * g0 + g1 * g1);
+    const double p2 = 2.0 * sqrt(a2 * b2);
+    const double prod = p2 * (or_is_zero(xyl) ? 1.0 : g0 / xyl);
+    const double imag = p2 * (or_is_zero(xyl) ? 0.0 : g1 / xyl);
+    const double ze = cz + disp, nrm = disp * ze;
+    const double fd = diff * ze / nrm;
+    D[0] = cx * fd;
+    D[1] = cy * fd;
+    D[2] = cz * fd;
+    O[0] = (prod * (cy * cy + cz * ze) + imag * (-cx * cy)) / nrm;
+    O[1] = (prod * (-cx * cy) + imag * (cx * cx + cz * ze)) / nrm;
+    O[2] = (prod * (-cx * ze) + imag * (-cy * ze)) / nrm;
+}
+
+void or_initial_state(const or_model* m, double kx, double ky, double kz, double s[3]) {
+    if (m->kind == OR_MODEL_PIFLUX3D) {
+        /* the continuum-limit driver rescues cos k_z of the fresh momentum of
+         * every positive z node (PiFlux.cpp:825-829: cos kx = cos ky = 0 there);
+         * the x/y loops overwrite cos kx, so only the (1 - RESCUE^2/2) factor of
+         * cos kz survives into ic_sigma.  The -z node reuses the last x/y values
+         * (PiFlux.cpp:845-849) and is not rescued. */
+        double cz = cos(kz);
+        if (m->ic_rescue && kz > 0.0 && cz < 0.0) cz *= (1.0 - 0.5 * OR_RESCUE * OR_RESCUE);
+        piflux3d_ic(m, cos(kx), cos(ky), cz, s);
+        return;
+    }
     if (m->kind == OR_MODEL_DIRAC3D) {
         /* DiracSystem.cpp:155-158: alpha0 = f(E_F + |k|), beta0 = f(E_F - |k|),
          * s = (2 alpha0 beta0, 0, alpha0^2 - beta0^2) */
@@ -271,11 +351,27 @@ void or_initial_state(const or_model* m, double kx, double ky, double s[3]) {
     s[2] = f * d[2];
 }
 
-void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const double s_init[3], const double s[3],
-            double t, double out[3]) {
+void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, double kz, const double s_ref[3],
+            const double s[3], double t, double out[3]) {
     const double A = or_pulse_A(p, t);
+    if (m->kind == OR_MODEL_PIFLUX3D) {
+        /* PiFlux.cpp:89-94 / 169-180: 4t (m x s), then the relaxation */
+        const double cx = cos(kx), cy = cos(ky), cz = cos(kz - A), pf = 4.0 * m->hopping;
+        out[0] = (cy * s[2] - cz * s[1]) * pf;
+        out[1] = (cz * s[0] - cx * s[2]) * pf;
+        out[2] = (cx * s[1] - cy * s[0]) * pf;
+        if (m->relax == OR_RELAX_EQUILIBRIUM) {
+            double D[3], O[3];
+            piflux3d_equilibrium(m, cx, cy, cz, s_ref, D, O);
+            const double rd = 1.0 / m->relax_time;
+            const double ro = m->relax_time_offdiag > 0.0 ? 1.0 / m->relax_time_offdiag : 0.0;
+            for (int c = 0; c < 3; ++c) out[c] -= rd * (s[c] / 3.0 - D[c]) + ro * (s[c] * (2.0 / 3.0) - O[c]);
+        }
+        return;
+    }
+    const double* s_init = s_ref;
     double d[3];
-    model_field(m, kx, ky, A, d);
+    model_field(m, kx, ky, kz, A, d);
     const double wx = 2.0 * d[0], wy = 2.0 * d[1], wz = 2.0 * d[2];
     out[0] = wy * s[2] - wz * s[1];
     out[1] = wz * s[0] - wx * s[2];
@@ -291,7 +387,9 @@ void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const do
     }
 }
 
-double or_vertex(const or_model* m, double kx, double ky, double A, const double s[3]) {
+double or_vertex(const or_model* m, double kx, double ky, double kz, double A, const double s[3]) {
+    /* 3D pi-flux lattice: sin(k_z - A) s_z (PiFlux.cpp:831-838) */
+    if (m->kind == OR_MODEL_PIFLUX3D) return sin(kz - A) * s[2];
     /* 3D Dirac cone: the band inversion rho = s_z (DiracSystem.cpp:167-169);
      * the weight carries k_z kappa / |k| (DiracSystem.cpp:279) */
     if (m->kind == OR_MODEL_DIRAC3D) return s[2];
@@ -332,15 +430,16 @@ double or_vertex(const or_model* m, double kx, double ky, double A, const double
 typedef struct {
     const or_model* m;
     const or_pulse* p;
-    double kx, ky;
+    double kx, ky, kz;
     double s_init[3];
+    double s_step[3]; /* state at the start of the current attempt */
     double rtol, atol;
     int max_fail;
     or_stats* st;
 } or_ctx;
 
 static void ctx_rhs(or_ctx* c, const double s[3], double t, double out[3]) {
-    or_rhs(c->m, c->p, c->kx, c->ky, c->s_init, s, t, out);
+    or_rhs(c->m, c->p, c->kx, c->ky, c->kz, c->m->relax == OR_RELAX_EQUILIBRIUM ? c->s_step : c->s_init, s, t, out);
     c->st->rhs_evals++;
 }
 
@@ -352,6 +451,11 @@ static void ctx_rhs(or_ctx* c, const double s[3], double t, double out[3]) {
 static int rkf78_try_step(or_ctx* c, double s[3], double* t, double* dt) {
     double k[13][3];
     const double h = *dt;
+    /* equilibrium relaxation reads the state at the start of the step inside
+     * every stage (the integrator's own state variable, PiFlux.cpp:144) */
+    c->s_step[0] = s[0];
+    c->s_step[1] = s[1];
+    c->s_step[2] = s[2];
     ctx_rhs(c, s, *t, k[0]);
     for (int i = 1; i < 13; ++i) {
         double x[3];
@@ -433,13 +537,13 @@ static int rkf78_integrate(or_ctx* c, double s[3], double t0, double t1, double*
 /* Magnus 4 (two-node Gauss) with the closed-form rotation                   */
 /* ------------------------------------------------------------------------- */
 
-static void magnus4_step(const or_model* m, const or_pulse* p, double kx, double ky, double s[3], double t,
-                         double h) {
+static void magnus4_step(const or_model* m, const or_pulse* p, double kx, double ky, double kz, double s[3],
+                         double t, double h) {
     const double c1 = 0.5 - OR_SQRT3 / 6.0, c2 = 0.5 + OR_SQRT3 / 6.0;
     double w1[3], w2[3];
     const double A1 = or_pulse_A(p, t + c1 * h), A2 = or_pulse_A(p, t + c2 * h);
-    model_field(m, kx, ky, A1, w1);
-    model_field(m, kx, ky, A2, w2);
+    model_field(m, kx, ky, kz, A1, w1);
+    model_field(m, kx, ky, kz, A2, w2);
     for (int q = 0; q < 3; ++q) { w1[q] *= 2.0; w2[q] *= 2.0; }
     /* Omega = h/2 (w1 + w2) + sqrt(3)/12 h^2 (w2 x w1) */
     const double f = OR_SQRT3 / 12.0 * h * h;
@@ -594,22 +698,26 @@ static int envelope_corners(const or_pulse* p, const or_time* tc, double* t_k, i
 typedef void (*emit_fn)(void* user, int i, const double s[3]);
 
 static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv, double kx, double ky,
-                  or_stats* st, emit_fn emit, void* user) {
+                  double kz, or_stats* st, emit_fn emit, void* user) {
     or_ctx c;
     c.m = m;
     c.p = p;
     c.kx = kx;
     c.ky = ky;
+    c.kz = kz;
     c.rtol = sv->rtol;
     c.atol = sv->atol;
     c.max_fail = sv->max_fail > 0 ? sv->max_fail : 500;
     c.st = st;
     double s[3];
-    or_initial_state(m, kx, ky, s);
+    or_initial_state(m, kx, ky, kz, s);
     c.s_init[0] = s[0];
     c.s_init[1] = s[1];
     c.s_init[2] = s[2];
     emit(user, 0, s);
+    /* equilibrium relaxation exists for the 3D pi-flux lattice only, which has
+     * no other relaxation */
+    if ((m->relax == OR_RELAX_EQUILIBRIUM) != (m->kind == OR_MODEL_PIFLUX3D && m->relax != OR_RELAX_NONE)) return -1;
     const int N = tc->n_measurements;
     const double delta = (tc->t_end - tc->t_begin) / N;
     if (sv->method == OR_METHOD_MAGNUS_MOMENTS) {
@@ -630,7 +738,7 @@ static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const
         const double h = delta / ms;
         for (int i = 1; i <= N; ++i) {
             const double t0 = sample_time(tc, i - 1);
-            for (int q = 0; q < ms; ++q) magnus4_step(m, p, kx, ky, s, t0 + q * h, h);
+            for (int q = 0; q < ms; ++q) magnus4_step(m, p, kx, ky, kz, s, t0 + q * h, h);
             st->attempts += ms;
             emit(user, i, s);
         }
@@ -676,10 +784,10 @@ static void emit_traj(void* user, int i, const double s[3]) {
 }
 
 int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv, double kx, double ky,
-                  double* s_out, or_stats* st) {
+                  double kz, double* s_out, or_stats* st) {
     traj_user u = {s_out};
     or_stats local = {0, 0, 0, 0};
-    const int r = evolve(m, p, tc, sv, kx, ky, st ? st : &local, emit_traj, &u);
+    const int r = evolve(m, p, tc, sv, kx, ky, kz, st ? st : &local, emit_traj, &u);
     if (r > 0 && st) st->failed_trajectories++;
     return r;
 }
@@ -687,20 +795,20 @@ int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const
 typedef struct {
     const or_model* m;
     const double* A;  /* A(t_i) */
-    double kx, ky, w;
+    double kx, ky, kz, w;
     double* row;
 } acc_user;
 
 static void emit_acc(void* user, int i, const double s[3]) {
     acc_user* u = (acc_user*)user;
-    u->row[i] += u->w * or_vertex(u->m, u->kx, u->ky, u->A[i], s);
+    u->row[i] += u->w * or_vertex(u->m, u->kx, u->ky, u->kz, u->A[i], s);
 }
 
 #define OR_CHUNK 64
 
 int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
-                           const double* kx, const double* ky, const double* w, int64_t n_k, double* j_out,
-                           or_stats* st, int n_threads) {
+                           const double* kx, const double* ky, const double* kz, const double* w, int64_t n_k,
+                           double* j_out, or_stats* st, int n_threads) {
     const int N = tc->n_measurements;
     const int64_t n_chunks = (n_k + OR_CHUNK - 1) / OR_CHUNK;
     double* A = (double*)malloc(sizeof(double) * (size_t)(N + 1));
@@ -724,8 +832,9 @@ int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* 
         double* row = partial + (size_t)c * (size_t)(N + 1);
         const int64_t k_end = (c + 1) * OR_CHUNK < n_k ? (c + 1) * OR_CHUNK : n_k;
         for (int64_t k = c * OR_CHUNK; k < k_end; ++k) {
-            acc_user u = {m, A, kx[k], ky[k], w[k], row};
-            const int r = evolve(m, p, tc, sv, kx[k], ky[k], &ls, emit_acc, &u);
+            const double z = kz ? kz[k] : 0.0;
+            acc_user u = {m, A, kx[k], ky[k], z, w[k], row};
+            const int r = evolve(m, p, tc, sv, kx[k], ky[k], z, &ls, emit_acc, &u);
             if (r > 0) failed++;
             if (r < 0) unsupported++;
         }

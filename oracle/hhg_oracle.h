@@ -18,9 +18,9 @@
 extern "C" {
 #endif
 
-enum { OR_MODEL_DIRAC2D = 0, OR_MODEL_PIFLUX2D = 1, OR_MODEL_HONEYCOMB = 2, OR_MODEL_DIRAC3D = 3 };
+enum { OR_MODEL_DIRAC2D = 0, OR_MODEL_PIFLUX2D = 1, OR_MODEL_HONEYCOMB = 2, OR_MODEL_DIRAC3D = 3, OR_MODEL_PIFLUX3D = 4 };
 enum { OR_PULSE_COS2 = 0, OR_PULSE_GAUSS = 1, OR_PULSE_DOUBLE_GAUSS = 2, OR_PULSE_DOUBLE_COS2 = 3, OR_PULSE_CONTINUOUS = 4 };
-enum { OR_RELAX_NONE = 0, OR_RELAX_TO_INITIAL = 1, OR_RELAX_T2 = 2 };
+enum { OR_RELAX_NONE = 0, OR_RELAX_TO_INITIAL = 1, OR_RELAX_T2 = 2, OR_RELAX_EQUILIBRIUM = 3 };
 enum { OR_METHOD_RKF78 = 0, OR_METHOD_MAGNUS4 = 1, OR_METHOD_MAGNUS_MOMENTS = 2 };
 enum { OR_PROPOSAL_RESTART = 0, OR_PROPOSAL_CARRY = 1 };
 
@@ -33,7 +33,10 @@ typedef struct {
     double theta;      /* polarisation angle of A(t) in the kx-ky plane */
     double fermi_energy;
     double beta;       /* inverse temperature; <= 0 means T = 0 */
-    double relax_time;
+    double relax_time;        /* equilibrium relaxation: the diagonal time */
+    double relax_time_offdiag; /* equilibrium relaxation: the off-diagonal time (<= 0: none) */
+    int32_t ic_rescue;        /* 3D pi-flux continuum grid: the cos k_z rescue of the initial state */
+    int32_t pad;
 } or_model;
 
 typedef struct {
@@ -81,11 +84,15 @@ void or_gauss_legendre(int n, double* x, double* w);
  * b8[13] (propagated), b7[13] (embedded). */
 void or_rkf78_tableau(double* c, double* a, double* b8, double* b7);
 
-/* Initial Bloch vector of one k-point. */
-void or_initial_state(const or_model* m, double kx, double ky, double s[3]);
+/* Points are (kx, ky, kz); kz is used by the 3D pi-flux lattice only. */
 
-/* Bloch right-hand side ds/dt at time t (used by tests against scipy). */
-void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, const double s_init[3],
+/* Initial Bloch vector of one k-point. */
+void or_initial_state(const or_model* m, double kx, double ky, double kz, double s[3]);
+
+/* Bloch right-hand side ds/dt at time t (used by tests against scipy).
+ * s_ref: the initial state (relaxation toward it) or the state at the start
+ * of the step (equilibrium relaxation). */
+void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, double kz, const double s_ref[3],
             const double s[3], double t, double out[3]);
 
 /* The four pulse moments of one sampling interval [t0, t0 + h] that the
@@ -97,18 +104,18 @@ void or_magnus_moments(const or_pulse* p, double t0, double h, double mom[4]);
 void or_magnus_moments_step(double kz, double kappa, const double mom[4], double h, double s[3]);
 
 /* Current vertex g(s, A) of one k-point (unweighted). */
-double or_vertex(const or_model* m, double kx, double ky, double A, const double s[3]);
+double or_vertex(const or_model* m, double kx, double ky, double kz, double A, const double s[3]);
 
 /* One k-point: Bloch vector at every sample t_i, s_out[(N+1)*3]. Returns 0 or
  * 1 if the controller failed. */
 int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
-                  double kx, double ky, double* s_out, or_stats* st);
+                  double kx, double ky, double kz, double* s_out, or_stats* st);
 
-/* j(t_i) = sum_k w_k g_k(t_i), i = 0..N, deterministic for any thread count.
- * Returns the number of failed trajectories. */
+/* j(t_i) = sum_k w_k g_k(t_i), i = 0..N, deterministic for any thread count
+ * (kz may be null: all zero).  Returns the number of failed trajectories. */
 int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
-                           const double* kx, const double* ky, const double* w, int64_t n_k, double* j_out,
-                           or_stats* st, int n_threads);
+                           const double* kx, const double* ky, const double* kz, const double* w, int64_t n_k,
+                           double* j_out, or_stats* st, int n_threads);
 
 #ifdef __cplusplus
 }

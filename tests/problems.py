@@ -14,12 +14,17 @@ class Problem:
     model: dict
     pulse: dict
     time: tuple            # (t_begin, t_end, N, n_sub)
-    grid: tuple            # ("box", nx, ny, x0, x1, y0, y1), ("hex", nx, ny) or ("cyl", n_z, n_kappa, max_k)
+    grid: tuple            # ("box", nx, ny, x0, x1, y0, y1), ("hex", nx, ny), ("cyl", n_z, n_kappa, max_k)
+                           # or ("pfc", z_range, n_coarse)
     solver: dict = field(default_factory=lambda: dict(method=0, rtol=1e-10, atol=1e-10, proposal=0))
 
     # ---- oracle side ----
     def o_model(self):
-        return O.model(**self.model)
+        m = dict(self.model)
+        # the continuum grid's rescue of cos k_z belongs to the 3D pi-flux driver
+        if self.grid[0] == "pfc" and m.get("kind") == O.MODEL_PIFLUX3D:
+            m["ic_rescue"] = 1
+        return O.model(**m)
 
     def o_pulse(self):
         return O.pulse(**self.pulse)
@@ -33,18 +38,27 @@ class Problem:
         return O.solver(**d)
 
     def o_grid(self):
+        kx, ky, _, w = self.o_grid3()
+        return kx, ky, w
+
+    def o_grid3(self):
+        """(kx, ky, kz, w); kz = 0 for the 2D grids."""
+        if self.grid[0] == "pfc":
+            return O.piflux_continuum(*self.grid[1:])
         if self.grid[0] == "box":
-            return O.gl_box(*self.grid[1:])
-        if self.grid[0] == "cyl":
-            return O.dirac_cylinder(*self.grid[1:])
-        return O.gl_hex(*self.grid[1:])
+            kx, ky, w = O.gl_box(*self.grid[1:])
+        elif self.grid[0] == "cyl":
+            kx, ky, w = O.dirac_cylinder(*self.grid[1:])
+        else:
+            kx, ky, w = O.gl_hex(*self.grid[1:])
+        return kx, ky, np.zeros_like(kx), w
 
     # ---- HIP side ----
     def h_model(self, H):
         m = self.model
         return H.ModelDesc(m.get("kind", 0), m.get("relax", 0), m.get("v", 1.0), m.get("mass", 0.1),
                            m.get("hopping", 1.0), m.get("theta", 0.0), m.get("fermi_energy", 0.0),
-                           m.get("beta", 0.0), m.get("relax_time", 0.0))
+                           m.get("beta", 0.0), m.get("relax_time", 0.0), m.get("relax_time_offdiag", 0.0))
 
     def h_laser(self, H):
         p = self.pulse
@@ -64,6 +78,8 @@ class Problem:
             return H.KGrid.box(*self.grid[1:])
         if self.grid[0] == "cyl":
             return H.KGrid.dirac_cylinder(*self.grid[1:])
+        if self.grid[0] == "pfc":
+            return H.KGrid.piflux_continuum(*self.grid[1:])
         return H.KGrid.hexagonal(*self.grid[1:])
 
     def h_solver(self, H, **over):
@@ -80,9 +96,9 @@ class Problem:
 
 
 def oracle_j(pr: Problem, n_threads=0, **solver_over):
-    kx, ky, w = pr.o_grid()
+    kx, ky, kz, w = pr.o_grid3()
     return O.current_density(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(**solver_over), kx, ky, w,
-                             n_threads=n_threads)
+                             n_threads=n_threads, kz=kz)
 
 
 def rel_l2(a, b):
@@ -179,6 +195,25 @@ DIRAC3D_COS2_RELAX = Problem(
     pulse=dict(kind=O.PULSE_COS2, amp_a=0.8, n_cycles=2.0, t0=0.0),
     time=(0.0, 4 * PI, 512, 50),
     grid=("cyl", 12, 6, 3.0),
+)
+
+# the reference's 3D pi-flux lattice (field along z) on its continuum-limit
+# quadrature, z_range = 4, N_coarse = 1 (1184 points)
+PIFLUX3D_COS2 = Problem(
+    "piflux3d_cos2",
+    model=dict(kind=O.MODEL_PIFLUX3D, hopping=0.6),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=0.9, n_cycles=2.0, t0=0.0),
+    time=(0.0, 4 * PI, 256, 50),
+    grid=("pfc", 4, 1),
+)
+
+PIFLUX3D_EQUILIBRIUM = Problem(
+    "piflux3d_equilibrium",
+    model=dict(kind=O.MODEL_PIFLUX3D, hopping=0.6, fermi_energy=0.3, beta=6.0, relax=O.RELAX_EQUILIBRIUM,
+               relax_time=12.0, relax_time_offdiag=5.0),
+    pulse=dict(kind=O.PULSE_CONTINUOUS, amp_a=0.7),
+    time=(-2 * PI, 2 * PI, 256, 50),
+    grid=("pfc", 4, 1),
 )
 
 SMALL_CASES = [DIRAC_SMALL, DIRAC_COHERENT, DIRAC_RELAX_INIT, PIFLUX_SMALL, HONEYCOMB_SMALL, HONEYCOMB_DOUBLE_COS2]

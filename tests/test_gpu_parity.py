@@ -9,7 +9,8 @@ import pytest
 
 import oracle as O
 from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
-                            HONEYCOMB_CORNERS, HONEYCOMB_SMALL, PIFLUX_SMALL, SMALL_CASES, oracle_j, rel_l2)
+                            HONEYCOMB_CORNERS, HONEYCOMB_SMALL, PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX_SMALL,
+                            SMALL_CASES, oracle_j, rel_l2)
 
 pytestmark = pytest.mark.gpu
 
@@ -254,6 +255,41 @@ def test_dirac3d_moment_magnus_matches_oracle(hhg):
         assert np.max(np.abs(traj[n] - s_ref)) < 1e-12
     j_rk, _ = oracle_j(pr, rtol=1e-12, atol=1e-12)
     assert rel_l2(j, j_rk) <= 1e-5
+
+
+@pytest.mark.parametrize("pr,proposal", [(PIFLUX3D_COS2, 0), (PIFLUX3D_COS2, 1), (PIFLUX3D_EQUILIBRIUM, 0)],
+                         ids=["cos2-restart", "cos2-carry", "equilibrium-restart"])
+def test_piflux3d_rkf78_matches_oracle(hhg, pr, proposal):
+    """The reference's 3D pi-flux lattice (field along z) on its continuum-limit
+    grid: j(t), attempt counts and per-k trajectories against the oracle; the
+    relaxation toward the instantaneous equilibrium at finite temperature with
+    unequal diagonal and off-diagonal times."""
+    _print_case(pr, proposal=proposal)
+    j_ref, st_ref = oracle_j(pr, proposal=proposal)
+    sel = np.arange(0, 1184, 97)
+    with pr.h_plan(hhg, solver=dict(proposal=proposal)) as plan:
+        j = plan.execute()[0]
+        att = plan.stats.attempts
+        traj = plan.trajectories(sel)
+    err = rel_l2(j, j_ref)
+    print(f"  rel_l2={err:.3e} attempts gpu={att} oracle={st_ref.attempts}", flush=True)
+    assert err <= J_TOL
+    assert abs(att - st_ref.attempts) <= 1e-3 * st_ref.attempts
+    kx, ky, kz, _ = pr.o_grid3()
+    for n, k in enumerate(sel):
+        s_ref = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(proposal=proposal), kx[k], ky[k],
+                             kz[k])[0]
+        assert np.max(np.abs(traj[n] - s_ref)) < 1e-9
+
+
+def test_piflux3d_magnus4_matches_oracle(hhg):
+    """The two-node Magnus step on the 3D pi-flux lattice, against the oracle."""
+    pr = PIFLUX3D_COS2
+    _print_case(pr, method="magnus4")
+    j_ref, _ = oracle_j(pr, method=O.METHOD_MAGNUS4, magnus_substeps=2)
+    with pr.h_plan(hhg, solver=dict(method=1, magnus_substeps=2)) as plan:
+        j = plan.execute()[0]
+    assert rel_l2(j, j_ref) <= 1e-12
 
 
 @pytest.mark.parametrize("tau", [-1.0, 20.0], ids=["magnus", "relaxation"])

@@ -80,6 +80,21 @@ def test_dirac_cylinder_grid_matches_oracle(hhg):
         assert np.array_equal(kx, okx) and np.allclose(ky, oky, rtol=1e-15, atol=0) and np.allclose(w, ow, rtol=1e-14)
 
 
+def test_piflux_continuum_grid_matches_oracle(hhg):
+    """(x, y, z) points and weights of the 3D pi-flux continuum-limit grid,
+    product vs oracle, including the reference's default 32 x 4; the third
+    coordinate is zero for 2D grids and the explicit grid carries one."""
+    for z_range, n_coarse in [(4, 1), (6, 2), (32, 4)]:
+        kx, ky, kz, w = hhg.KGrid.piflux_continuum(z_range, n_coarse).build3()
+        okx, oky, okz, ow = O.piflux_continuum(z_range, n_coarse)
+        assert len(kx) == len(okx) == 2 * z_range * (2 * 64 * n_coarse ** 2 + 20 * n_coarse ** 2)
+        for a, b in ((kx, okx), (ky, oky), (kz, okz), (w, ow)):
+            assert np.allclose(a, b, rtol=1e-14, atol=0)
+    assert np.all(hhg.KGrid.box(4, 3, -1.0, 1.0, -1.0, 1.0).build3()[2] == 0.0)
+    e = hhg.KGrid.explicit([0.1, 0.2], [0.3, 0.4], [1.0, 2.0], kz=[0.5, -0.5]).build3()
+    assert np.array_equal(e[2], [0.5, -0.5])
+
+
 def test_continuous_pulse_matches_oracle(hhg):
     t = np.linspace(-20.0, 20.0, 401)
     A = hhg.ContinuousLaser(0.9, cep=0.2).laser_function(t)
@@ -181,3 +196,19 @@ def test_cpp_driver_rejects_unknown_laser(hhg, tmp_path):
     exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
     r = subprocess.run([exe, str(cfg)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "not recognized" in r.stderr
+
+
+@pytest.mark.parametrize("kind,relax,proposal,method", [
+    (4, 3, 1, 0),  # equilibrium relaxation with the carried proposal
+    (4, 2, 0, 0),  # the 3D pi-flux lattice has no T2 dephasing
+    (0, 3, 0, 0),  # equilibrium relaxation of another model
+    (4, 0, 0, 2),  # the moment Magnus step is the 3D Dirac cone's
+], ids=["equilibrium-carry", "piflux3d-t2", "dirac2d-equilibrium", "piflux3d-moments"])
+def test_plan_rejects_unsupported_combinations(hhg, kind, relax, proposal, method):
+    """Validation runs on the host before any device call and names the reason."""
+    m = hhg.ModelDesc(kind, relax, 1.0, 0.1, 0.6, 0.0, 0.0, 0.0, 10.0, 4.0)
+    job = hhg.Job(m, hhg.ContinuousLaser(0.5), hhg.TimeIntegrationConfig(0.0, 6.0, 64, 50), 0)
+    grid = hhg.KGrid.piflux_continuum(2, 1) if kind == 4 else hhg.KGrid.box(4, 4, -1.0, 1.0, -1.0, 1.0)
+    with pytest.raises(hhg.HHGError) as ei:
+        hhg.Plan([job], [grid], hhg.Solver(method=method, proposal=proposal))
+    assert ei.value.status == hhg.ERROR_UNSUPPORTED

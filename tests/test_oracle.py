@@ -586,3 +586,136 @@ def test_dirac_cylinder_weights():
         assert np.all(ka > 0) and np.all(kz != 0)
         errs.append(abs(np.abs(w).sum() - M ** 3 / 3))
     assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5
+
+
+# --------------------------------------------------------------------------
+# 3D pi-flux lattice (sources/HHG/Systems/PiFlux.cpp)
+# --------------------------------------------------------------------------
+
+def _fermi(e, beta):
+    return 1.0 / (1.0 + math.exp(beta * e)) if beta > 0 else (1.0 if e < 0 else (0.0 if e > 0 else 0.5))
+
+
+def test_piflux3d_initial_state_is_the_thermal_pure_state():
+    """ic_sigma (PiFlux.cpp:519-549): along m^ the occupation difference
+    alpha^2 - beta^2, transverse to m a coherence of modulus 2 alpha beta, so
+    |s| = alpha^2 + beta^2 (alpha^2 = f(-E_F + 2t|m|), beta^2 = f(-E_F - 2t|m|));
+    at T = 0, E_F = 0 the lower band, s = -m^."""
+    rng = np.random.default_rng(11)
+    for beta, ef in ((0.0, 0.0), (2.5, 0.4), (6.0, -0.7)):
+        mod = O.model(O.MODEL_PIFLUX3D, hopping=0.6, beta=beta, fermi_energy=ef)
+        for _ in range(6):
+            k = rng.uniform([0, 0, -math.pi], [math.pi, math.pi / 2, math.pi])
+            s = O.initial_state(mod, *k)
+            m = np.cos(k)
+            mh = m / np.linalg.norm(m)
+            a2 = _fermi(-ef + 2 * 0.6 * np.linalg.norm(m), beta)
+            b2 = _fermi(-ef - 2 * 0.6 * np.linalg.norm(m), beta)
+            assert abs(np.dot(s, mh) - (a2 - b2)) < 1e-14
+            assert abs(np.linalg.norm(s) - (a2 + b2)) < 1e-14
+            if beta == 0.0:
+                assert np.allclose(s, -mh, atol=1e-15)
+
+
+def test_piflux3d_continuum_rescue_of_the_initial_state():
+    """The continuum driver scales cos k_z by 1 - RESCUE^2/2 before ic_sigma at
+    the +z nodes with cos k_z < 0 only (PiFlux.cpp:825-829, 845-849)."""
+    mod = O.model(O.MODEL_PIFLUX3D, hopping=0.6, ic_rescue=1)
+    plain = O.model(O.MODEL_PIFLUX3D, hopping=0.6)
+    kx, ky = 0.9, 0.4
+    for kz, rescued in ((2.5, True), (-2.5, False), (1.0, False)):
+        s = O.initial_state(mod, kx, ky, kz)
+        m = np.array([math.cos(kx), math.cos(ky), math.cos(kz) * ((1 - 0.5e-8) if rescued else 1.0)])
+        assert np.allclose(s, -m / np.linalg.norm(m), rtol=0, atol=1e-14)
+        if rescued:
+            assert np.max(np.abs(s - O.initial_state(plain, kx, ky, kz))) > 1e-10
+
+
+def test_piflux3d_matches_rotated_2d_piflux():
+    """At cos kx = 0 the 3D lattice d = 2t(0, cos ky, cos(kz - A)) is the 2D
+    pi-flux d = 2t(cos(qx - A), cos qy, 0) at (qx, qy) = (kz - pi, ky) turned by
+    the rotation (a, b, c) -> (c, b, -a) about y; the vertices sin(kz - A) s_z
+    and sin(qx - A) s_x agree.  Trajectories and currents match at T = 0."""
+    p = O.pulse(O.PULSE_COS2, amp_a=0.8, n_cycles=2.0, t0=0.0)
+    tc = O.time_grid(0.0, 4 * math.pi, 128, 50)
+    sv = O.solver(rtol=1e-12, atol=1e-12)
+    m3 = O.model(O.MODEL_PIFLUX3D, hopping=0.6)
+    m2 = O.model(O.MODEL_PIFLUX2D, hopping=0.6)
+    for qx, qy in ((0.4, 1.1), (-2.2, 0.3), (2.9, 2.0)):
+        s3 = O.trajectory(m3, p, tc, sv, math.pi / 2, qy, qx + math.pi)[0]
+        s2 = O.trajectory(m2, p, tc, sv, qx, qy)[0]
+        rot = np.stack([s2[:, 2], s2[:, 1], -s2[:, 0]], axis=1)
+        assert np.max(np.abs(s3 - rot)) < 1e-9
+        A = O.pulse_A(p, np.linspace(0.0, 4 * math.pi, 129))
+        v3 = [O.vertex(m3, math.pi / 2, qy, a, s, kz=qx + math.pi) for a, s in zip(A, s3)]
+        v2 = [O.vertex(m2, qx, qy, a, s) for a, s in zip(A, s2)]
+        assert np.max(np.abs(np.array(v3) - np.array(v2))) < 1e-9
+
+
+def test_piflux3d_equilibrium_relaxation_fixed_point_and_rates():
+    """Relaxation toward the instantaneous equilibrium (PiFlux.cpp:134-180):
+    with equal diagonal and off-diagonal times the thermal state s0 (any T) is
+    its own target, so only the precession 4t m x s0 remains; at T = 0 from
+    s = -m^ the unequal rates leave -(2/3)(1/tau_d - 1/tau_o) m^ (the reference's
+    s/3 and 2s/3 split)."""
+    p = O.pulse(O.PULSE_COS2, amp_a=0.0, n_cycles=2.0, t0=0.0)
+    rng = np.random.default_rng(4)
+    for beta, ef in ((0.0, 0.0), (3.0, 0.5)):
+        for _ in range(4):
+            k = rng.uniform([0, 0, -math.pi], [math.pi, math.pi / 2, math.pi])
+            m = np.cos(k)
+            eq = O.model(O.MODEL_PIFLUX3D, hopping=0.6, beta=beta, fermi_energy=ef, relax=O.RELAX_EQUILIBRIUM,
+                         relax_time=7.0, relax_time_offdiag=7.0)
+            s0 = O.initial_state(eq, *k)
+            out = O.rhs(eq, p, k[0], k[1], s0, s0, 0.3, kz=k[2])
+            assert np.allclose(out, 4 * 0.6 * np.cross(m, s0), rtol=0, atol=1e-14)
+    k = np.array([0.8, 0.3, 2.0])
+    m = np.cos(k)
+    mh = m / np.linalg.norm(m)
+    eq = O.model(O.MODEL_PIFLUX3D, hopping=0.6, relax=O.RELAX_EQUILIBRIUM, relax_time=4.0, relax_time_offdiag=10.0)
+    out = O.rhs(eq, p, k[0], k[1], -mh, -mh, 0.3, kz=k[2])
+    assert np.allclose(out, -(2.0 / 3.0) * (1 / 4.0 - 1 / 10.0) * mh, rtol=0, atol=1e-15)
+
+
+def test_piflux3d_relaxation_after_the_pulse_in_closed_form():
+    """After the cos^2 pulse (A = 0) at T = 0 with tau_d = tau_o = tau the
+    equation is s' = 4t m x s - (s + m^)/tau: s + m^ turns about m at 4t|m| and
+    decays as exp(-t/tau).  The RKF78 trajectory from the end of the pulse
+    follows the closed form."""
+    kx, ky, kz, tau, t_hop = 0.7, 0.5, 1.9, 6.0, 0.6
+    T_p = 4 * math.pi
+    p = O.pulse(O.PULSE_COS2, amp_a=0.9, n_cycles=2.0, t0=0.0)
+    tc = O.time_grid(0.0, 2 * T_p, 256, 50)
+    mod = O.model(O.MODEL_PIFLUX3D, hopping=t_hop, relax=O.RELAX_EQUILIBRIUM, relax_time=tau,
+                  relax_time_offdiag=tau)
+    s = O.trajectory(mod, p, tc, O.solver(rtol=1e-12, atol=1e-13), kx, ky, kz)[0]
+    m = np.cos([kx, ky, kz])
+    mh = m / np.linalg.norm(m)
+    i1 = 128  # t = T_p, end of the pulse
+    x1 = s[i1] + mh
+    assert np.linalg.norm(x1) > 1e-3  # the pulse left the state away from equilibrium
+    for i in (160, 200, 256):
+        dt = (i - i1) * (2 * T_p / 256)
+        want = -mh + math.exp(-dt / tau) * _axis_rotation(4 * t_hop * m * dt, x1)
+        assert np.allclose(s[i], want, rtol=0, atol=1e-10)
+
+
+def test_piflux_continuum_grid_integrates_exactly():
+    """The continuum-limit quadrature (PiFlux.cpp:665-863) covers
+    [0, pi] x [0, pi/2] x (z = pi u, u in [-1, 1]) with weight du dx dy: the
+    weights sum to pi^2 and the four xy patches tile the rectangle without
+    overlap: x^3 y^3 u^2 (within every patch's polynomial degree) integrates
+    exactly; at the reference's default z_range = 32, N_coarse = 4 the smooth
+    cos^2 x cos^2 y cos^2(pi u) integrates to pi^2/8."""
+    kx, ky, kz, w = O.piflux_continuum(4, 1)
+    assert len(w) == 2 * 4 * (2 * 64 + 20)
+    assert abs(w.sum() - math.pi ** 2) < 1e-13
+    assert np.all((kx > 0) & (kx < math.pi) & (ky > 0) & (ky < math.pi / 2))
+    u = kz / math.pi
+    exact = (math.pi ** 4 / 4) * ((math.pi / 2) ** 4 / 4) * (2.0 / 3.0)
+    assert abs(np.dot(w, kx ** 3 * ky ** 3 * u ** 2) / exact - 1) < 1e-13
+    kx, ky, kz, w = O.piflux_continuum(32, 4)
+    assert len(w) == 2 * 32 * (2 * 32 * 32 + 20 * 16)
+    assert abs(w.sum() - math.pi ** 2) < 1e-12
+    f = np.cos(kx) ** 2 * np.cos(ky) ** 2 * np.cos(kz) ** 2
+    assert abs(np.dot(w, f) - math.pi ** 2 / 8) < 1e-12
