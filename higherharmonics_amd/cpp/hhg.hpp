@@ -313,6 +313,47 @@ class DiracSystem : public System {
     h_float laser_model_ratio(h_float photon_energy_) const { return hbar * v_F / photon_energy_; }
 };
 
+// The reference's 3D pi-flux lattice (PiFlux.hpp:25-147) in its own units:
+// temperature in K, E_F in meV, v_F in m/s, band width in photon energies,
+// photon energy in meV, relaxation times in fs (diagonal <= 0: none;
+// off-diagonal <= 0: the diagonal one, PiFluxDispatcher.cpp:18-34).  Field
+// along z on the continuum-limit grid (z_range, N_coarse; the reference's
+// 32 and 4, PiFlux.hpp:12-29); RKF78 at abs 1e-12, rel 1e-8 with the restart
+// controller (PiFlux.cpp:20-21, 96-110); with relaxation toward the
+// instantaneous equilibrium (PiFlux.cpp:113-220).
+class PiFlux : public System {
+   public:
+    static constexpr h_float sqrt_3 = 1.732050807568877293527446341;   // GlobalDefinitions.hpp:32
+    static constexpr h_float sqrt_12 = 3.464101615137754587054892683;  // GlobalDefinitions.hpp:33
+    h_float v_F, band_width, photon_energy;
+
+    PiFlux(h_float temperature, h_float E_F, h_float v_F_, h_float band_width_, h_float photon_energy_,
+           h_float diagonal_relaxation_time, h_float offdiagonal_relaxation_time, int z_range = 32, int n_coarse = 4)
+        : v_F(v_F_), band_width(band_width_), photon_energy(photon_energy_) {
+        model.kind = HHG_MODEL_PIFLUX3D;
+        model.fermi_energy = E_F / photon_energy;
+        model.beta = temperature > 0.0 ? photon_energy / (DiracSystem::k_B * temperature) : 0.0;  // 0: T = 0
+        model.hopping = band_width / sqrt_12;                                                     // PiFlux.cpp:66
+        if (diagonal_relaxation_time > 0.0) {
+            // inverse times (1e15 hbar) / (tau photon_energy) in omega_L (PiFlux.cpp:68-69)
+            const h_float tau_o = offdiagonal_relaxation_time > 0.0 ? offdiagonal_relaxation_time
+                                                                     : diagonal_relaxation_time;
+            model.relax = HHG_RELAX_EQUILIBRIUM;
+            model.relax_time = (diagonal_relaxation_time * photon_energy) / (1e15 * DiracSystem::hbar);
+            model.relax_time_offdiag = (tau_o * photon_energy) / (1e15 * DiracSystem::hbar);
+        }
+        solver.method = HHG_METHOD_RKF78;
+        solver.proposal = HHG_PROPOSAL_RESTART;
+        solver.atol = 1e-12;
+        solver.rtol = 1e-8;
+        kgrid.kind = HHG_KGRID_PIFLUX_CONTINUUM;
+        kgrid.nx = z_range;
+        kgrid.ny = n_coarse;
+    }
+    // the lattice constant sqrt3 hbar v_F / (photon_energy band_width) (PiFlux.cpp:67, PiFlux.hpp:78-80)
+    h_float laser_model_ratio() const { return sqrt_3 * DiracSystem::hbar * v_F / (photon_energy * band_width); }
+};
+
 // PiFlux square lattice d = 2t (cos(kx - A), cos ky, 0) over [-pi, pi]^2
 class PiFlux2D : public System {
    public:

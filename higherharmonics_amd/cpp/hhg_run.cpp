@@ -10,16 +10,21 @@
 // Configuration: whitespace-separated "key value" lines, '#' starts a comment
 // (the reference's InputFileReader format, sources/hhg.cpp:58-80).
 //
-// system_type Dirac: the reference's 3D Dirac cone in the reference's own
-// units and keys (sources/hhg.cpp:63-80, DiracDispatcher.cpp:15-58):
+// system_type Dirac | PiFlux: the reference's 3D Dirac cone and 3D pi-flux
+// lattice in the reference's own units and keys (sources/hhg.cpp:63-80,
+// DiracDispatcher.cpp:15-58, PiFluxDispatcher.cpp:14-110):
 //   T [K], E_F [meV], v_F [m/s], band_width [photon energies],
-//   field_amplitude [MV/cm], photon_energy [meV], diagonal_relaxation_time [fs]
-//   (<= 0: none; then the moment Magnus step, else RKF78), laser_type
+//   field_amplitude [MV/cm], photon_energy [meV], diagonal_relaxation_time and
+//   offdiagonal_relaxation_time [fs] (<= 0: none / the diagonal one), laser_type
 //   continuous | cosine, n_laser_cycles, t0_offset (cos^2 begin in units of
-//   pi), n_z; N = n_laser_cycles * 1024 samples (hhg.cpp:87-88); n_kappa
-//   (Gauss-Legendre nodes in kappa, default 32) replaces the reference's
-//   adaptive kappa trapezoid.  The output carries the reference's fields
-//   (hhg.cpp:301-326); continuous lasers use the Welch-window spectrum.
+//   pi); N = n_laser_cycles * 1024 samples (hhg.cpp:87-88).
+//   Dirac: without relaxation the moment Magnus step, else RKF78; n_z, and
+//   n_kappa (Gauss-Legendre nodes in kappa, default 32) in place of the
+//   reference's adaptive kappa trapezoid.
+//   PiFlux: RKF78, the continuum-limit grid z_range x n_coarse (32, 4), a cos^2
+//   run continues 2 pi after the pulse (PiFluxDispatcher.cpp:51-53).
+//   The output carries the reference's fields (hhg.cpp:301-326); continuous
+//   lasers use the Welch-window spectrum.
 //
 // Other systems in the reference's dimensionless units (time 1/omega_L,
 // energy hbar omega_L):
@@ -111,20 +116,27 @@ void emit(const std::string& text, int argc, char** argv) {
     if (!out || !(out << text)) throw std::runtime_error("cannot write " + path);
 }
 
-// The reference's 3D Dirac run in its own units (DiracDispatcher, hhg.cpp main).
-int run_dirac(const std::map<std::string, std::string>& kv, int argc, char** argv) {
+// The reference's 3D runs in its own units (Dirac/PiFlux dispatchers, hhg.cpp main).
+int run_reference_units(const std::string& system, const std::map<std::string, std::string>& kv, int argc,
+                        char** argv) {
     using namespace HHG;
     const double T = num(kv, "T", 0.0), E_F = num(kv, "E_F", 0.0), v_F = num(kv, "v_F", 1.5e5);
     const double band_width = num(kv, "band_width", 10.0), E0 = num(kv, "field_amplitude", 1.0);
     const double photon_energy = num(kv, "photon_energy", 5.25), tau = num(kv, "diagonal_relaxation_time", -1.0);
+    const double tau_o = num(kv, "offdiagonal_relaxation_time", -1.0);
     const std::string laser_type = str(kv, "laser_type", "continuous");
     const int n_cycles = (int)num(kv, "n_laser_cycles", 8);
     const double t0_offset = num(kv, "t0_offset", 0.0);
     const int n_z = (int)num(kv, "n_z", 64), n_kappa = (int)num(kv, "n_kappa", 32);
+    const int z_range = (int)num(kv, "z_range", 32), n_coarse = (int)num(kv, "n_coarse", 4);
     const int N = (int)num(kv, "n_measurements", n_cycles * 1024);
 
-    Systems::DiracSystem sys(T, E_F, v_F, band_width, photon_energy, tau, n_z, n_kappa);
-    const double A0 = Laser::momentum_amplitude(photon_energy, E0, sys.laser_model_ratio(photon_energy));
+    const bool dirac = system == "Dirac";
+    Systems::DiracSystem dsys(T, E_F, v_F, band_width, photon_energy, tau, n_z, n_kappa);
+    Systems::PiFlux psys(T, E_F, v_F, band_width, photon_energy, tau, tau_o, z_range, n_coarse);
+    const Systems::System& sys = dirac ? static_cast<const Systems::System&>(dsys) : psys;
+    const double ratio = dirac ? dsys.laser_model_ratio(photon_energy) : psys.laser_model_ratio();
+    const double A0 = Laser::momentum_amplitude(photon_energy, E0, ratio);
     std::unique_ptr<Laser::Laser> laser;
     TimeIntegrationConfig tc;
     if (laser_type == "continuous") {
@@ -132,7 +144,8 @@ int run_dirac(const std::map<std::string, std::string>& kv, int argc, char** arg
         tc = {-n_cycles * pi, n_cycles * pi, N, 50};
     } else if (laser_type == "cosine") {
         laser = std::make_unique<Laser::CosineLaser>(A0, (double)n_cycles, pi * t0_offset);
-        tc = {laser->t_begin, laser->t_end, N, 50};
+        // the lattice continues one cycle after the pulse so the relaxation sets in
+        tc = {laser->t_begin, laser->t_end + (dirac ? 0.0 : 2.0 * pi), N, 50};
     } else {
         throw std::invalid_argument("Laser type '" + laser_type + "' is not recognized!");
     }
@@ -155,10 +168,14 @@ int run_dirac(const std::map<std::string, std::string>& kv, int argc, char** arg
       << ", \"band_width\": " << num_str(band_width) << ", \"field_amplitude\": " << num_str(E0)
       << ", \"photon_energy\": " << num_str(photon_energy) << ", \"laser_type\": \"" << laser_type
       << "\", \"diagonal_relaxation_time\": " << num_str(tau)
-      << ", \"offdiagonal_relaxation_time\": " << num_str(num(kv, "offdiagonal_relaxation_time", -1.0)) << ",\n";
+      << ", \"offdiagonal_relaxation_time\": " << num_str(tau_o) << ",\n";
     write_array(o, "frequencies", sp.frequencies);
-    o << "  \"zero_padding\": 8, \"system_type\": \"Dirac\", \"n_z\": " << n_z << ", \"n_kappa\": " << n_kappa
-      << ", \"t0_offset\": " << num_str(t0_offset) << ",\n";
+    o << "  \"zero_padding\": 8, \"system_type\": \"" << system << "\", \"n_z\": " << n_z;
+    if (dirac)
+        o << ", \"n_kappa\": " << n_kappa;
+    else
+        o << ", \"z_range\": " << z_range << ", \"n_coarse\": " << n_coarse;
+    o << ", \"t0_offset\": " << num_str(t0_offset) << ",\n";
     o << "  \"momentum_amplitude\": " << num_str(A0) << ", \"trajectories\": " << stats.trajectories
       << ", \"attempted_steps\": " << stats.attempts << ", \"solve_ms\": " << num_str(stats.solve_ms) << "\n}\n";
     emit(o.str(), argc, argv);
@@ -176,7 +193,7 @@ int main(int argc, char** argv) {
     try {
         const auto kv = read_config(argv[1]);
         const std::string system = str(kv, "system_type", "Dirac2D");
-        if (system == "Dirac") return run_dirac(kv, argc, argv);
+        if (system == "Dirac" || system == "PiFlux") return run_reference_units(system, kv, argc, argv);
         const std::string laser_type = str(kv, "laser_type", "cosine");
         const double A0 = num(kv, "field_amplitude", 0.5);
         const double n_cycles = num(kv, "n_laser_cycles", 8);

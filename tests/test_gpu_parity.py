@@ -339,3 +339,64 @@ def test_cpp_driver_reference_units_dirac(hhg, tmp_path, tau):
     assert rel_l2(j, j_ref) <= (1e-12 if tau < 0 else J_TOL)
     f, re, im = O.welch_spectrum(j, -2 * math.pi, 2 * math.pi, 512)
     assert np.allclose(d["current_density_frequency_real"], re, rtol=1e-12, atol=1e-12 * np.abs(re).max())
+
+
+@pytest.mark.parametrize("laser,tau", [("continuous", -1.0), ("cosine", 10.0)], ids=["continuous", "cosine-relaxation"])
+def test_cpp_driver_reference_units_piflux(hhg, tmp_path, laser, tau):
+    """hhg_run with the reference's PiFlux keys and units (band width in photon
+    energies, relaxation times in fs, the off-diagonal one separate) on a small
+    continuum-limit grid: parameters convert as PiFlux.cpp:56-70 and
+    Laser.cpp:24-25 (model ratio = lattice constant), the cos^2 run continues
+    2 pi after the pulse (PiFluxDispatcher.cpp:51-53), and j(t) equals the
+    oracle's RKF78 run at the reference's tolerances.
+
+    The reference's equilibrium relaxation makes j(t) sensitive to rounding:
+    its band frame divides by cos z + |m|, which nearly vanishes where
+    cos(z - A) sweeps through -|m|, and the target coherence takes the phase of
+    the step-start state there.  Two IEEE builds of the oracle (strict, and
+    -ffast-math with FMA, the reference's Release flags) then differ by up to
+    ~1e-7 with identical step counts; with relaxation the bar is ten times
+    that measured spread (and never below 1e-9)."""
+    import gzip
+    import json
+    import math
+    import os
+    import subprocess
+    cfg = tmp_path / "piflux.cfg"
+    cfg.write_text("system_type PiFlux\nT 30\nE_F 2\nv_F 1.5e3\nband_width 10\nfield_amplitude 8\n"
+                   f"photon_energy 5.25\ndiagonal_relaxation_time {tau}\noffdiagonal_relaxation_time 4\n"
+                   f"laser_type {laser}\nn_laser_cycles 1\nt0_offset 0\nz_range 4\nn_coarse 1\n"
+                   "n_measurements 256\n")
+    out = tmp_path / "current_density.json.gz"
+    exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
+    print(f"[case] hhg_run PiFlux reference units laser={laser} tau={tau}", flush=True)
+    r = subprocess.run([exe, str(cfg), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(gzip.open(out).read())
+    assert d["system_type"] == "PiFlux" and d["z_range"] == 4 and d["n_coarse"] == 1
+    hbar, k_B = 6.582119569509065698e-13, 0.08617333262145177434
+    lattice = math.sqrt(3) * hbar * 1.5e3 / (5.25 * 10.0)
+    A0 = lattice * 1e11 * 8.0 / 5.25
+    assert abs(d["momentum_amplitude"] - A0) <= 1e-15 * A0
+    kx, ky, kz, w = O.piflux_continuum(4, 1)
+    if laser == "continuous":
+        t_b, t_e = -math.pi, math.pi
+        p = O.pulse(O.PULSE_CONTINUOUS, amp_a=A0)
+    else:
+        t_b, t_e = 0.0, 4 * math.pi
+        p = O.pulse(O.PULSE_COS2, amp_a=A0, n_cycles=1.0, t0=0.0)
+    assert d["t_begin"] == t_b and abs(d["t_end"] - t_e) < 1e-15
+    kw = dict(hopping=10.0 / math.sqrt(12), fermi_energy=2.0 / 5.25, beta=5.25 / (k_B * 30.0), ic_rescue=1)
+    if tau > 0:
+        kw.update(relax=O.RELAX_EQUILIBRIUM, relax_time=tau * 5.25 / (1e15 * hbar),
+                  relax_time_offdiag=4.0 * 5.25 / (1e15 * hbar))
+    m = O.model(O.MODEL_PIFLUX3D, **kw)
+    tc, sv = O.time_grid(t_b, t_e, 256, 50), O.solver(rtol=1e-8, atol=1e-12)
+    j_ref, st_ref = O.current_density(m, p, tc, sv, kx, ky, w, kz=kz)
+    j_fast, _ = O.current_density(m, p, tc, sv, kx, ky, w, kz=kz, fast=True)
+    spread = rel_l2(j_fast, j_ref)
+    j = np.array(d["current_density_time"])
+    err = rel_l2(j, j_ref)
+    print(f"  rel_l2={err:.3e} oracle strict-vs-fast spread={spread:.3e}", flush=True)
+    assert err <= (J_TOL if tau < 0 else max(J_TOL, 10 * spread))
+    assert abs(d["attempted_steps"] - st_ref.attempts) <= 1e-3 * st_ref.attempts
