@@ -246,6 +246,8 @@ class System {
         return plan.compute(stats)[0];
     }
 
+    virtual ~System() = default;
+
    protected:
     System() = default;
 };
@@ -368,9 +370,16 @@ class PiFlux2D : public System {
     }
 };
 
-// Honeycomb nearest-neighbour model over the hexagonal BZ (Honeycomb.cpp:225-336)
+// Honeycomb nearest-neighbour model over the hexagonal BZ (Honeycomb.cpp:225-336).
+// Dimensionless: hopping t and the GL sizes.  Reference units (Honeycomb.cpp:83-92):
+// temperature in K, E_F in meV, v_F in m/s, band width in photon energies, photon
+// energy in meV, diagonal relaxation time in fs (<= 0: none; else relaxation toward
+// the instantaneous equilibrium, Honeycomb.cpp:160-199), N_k positive GL nodes per
+// axis (Honeycomb.hpp:12-21); RKF78 at abs 1e-12, rel 1e-8 (Honeycomb.cpp:14-15).
 class Honeycomb : public System {
    public:
+    h_float v_F = 0.0, band_width = 0.0, photon_energy = 0.0;
+
     Honeycomb(h_float hopping, int nx, int ny) {
         model.kind = HHG_MODEL_HONEYCOMB;
         model.hopping = hopping;
@@ -378,6 +387,26 @@ class Honeycomb : public System {
         kgrid.nx = nx;
         kgrid.ny = ny;
     }
+    Honeycomb(h_float temperature, h_float E_F, h_float v_F_, h_float band_width_, h_float photon_energy_,
+              h_float diagonal_relaxation_time, int N_k = 64)
+        : Honeycomb(band_width_ / 6.0, 2 * N_k, 2 * N_k) {  // hopping W/6 (Honeycomb.cpp:90)
+        v_F = v_F_;
+        band_width = band_width_;
+        photon_energy = photon_energy_;
+        model.fermi_energy = E_F / photon_energy;
+        model.beta = temperature > 0.0 ? photon_energy / (DiracSystem::k_B * temperature) : 0.0;  // 0: T = 0
+        if (diagonal_relaxation_time > 0.0) {
+            // inverse time (1e15 hbar) / (tau photon_energy) in omega_L (Honeycomb.cpp:92)
+            model.relax = HHG_RELAX_EQUILIBRIUM;
+            model.relax_time = (diagonal_relaxation_time * photon_energy) / (1e15 * DiracSystem::hbar);
+        }
+        solver.method = HHG_METHOD_RKF78;
+        solver.proposal = HHG_PROPOSAL_RESTART;
+        solver.atol = 1e-12;
+        solver.rtol = 1e-8;
+    }
+    // the lattice constant 4 hbar v_F / (photon_energy band_width) (Honeycomb.cpp:91, Honeycomb.hpp:60-62)
+    h_float laser_model_ratio() const { return 4.0 * DiracSystem::hbar * v_F / (photon_energy * band_width); }
 };
 
 }  // namespace Systems

@@ -10,9 +10,10 @@
 // Configuration: whitespace-separated "key value" lines, '#' starts a comment
 // (the reference's InputFileReader format, sources/hhg.cpp:58-80).
 //
-// system_type Dirac | PiFlux: the reference's 3D Dirac cone and 3D pi-flux
-// lattice in the reference's own units and keys (sources/hhg.cpp:63-80,
-// DiracDispatcher.cpp:15-58, PiFluxDispatcher.cpp:14-110):
+// system_type Dirac | PiFlux | Honeycomb: the reference's 3D Dirac cone, 3D
+// pi-flux lattice and honeycomb lattice in the reference's own units and keys
+// (sources/hhg.cpp:63-80, DiracDispatcher.cpp:15-58, PiFluxDispatcher.cpp:14-110,
+// HoneycombDispatcher.cpp:12-60):
 //   T [K], E_F [meV], v_F [m/s], band_width [photon energies],
 //   field_amplitude [MV/cm], photon_energy [meV], diagonal_relaxation_time and
 //   offdiagonal_relaxation_time [fs] (<= 0: none / the diagonal one), laser_type
@@ -23,12 +24,14 @@
 //   reference's adaptive kappa trapezoid.
 //   PiFlux: RKF78, the continuum-limit grid z_range x n_coarse (32, 4), a cos^2
 //   run continues 2 pi after the pulse (PiFluxDispatcher.cpp:51-53).
+//   Honeycomb: RKF78, the hexagonal-BZ grid with N_k positive nodes per axis
+//   (64), relaxation toward the instantaneous equilibrium, cos^2 + 2 pi
+//   (HoneycombDispatcher.cpp:31-35).
 //   The output carries the reference's fields (hhg.cpp:301-326); continuous
 //   lasers use the Welch-window spectrum.
 //
-// Other systems in the reference's dimensionless units (time 1/omega_L,
-// energy hbar omega_L):
-//   system_type       Dirac2D | PiFlux2D | Honeycomb
+// The 2D models in dimensionless units (time 1/omega_L, energy hbar omega_L):
+//   system_type       Dirac2D | PiFlux2D | Honeycomb2D
 //   laser_type        cosine | gauss | double_gauss | double_cosine | continuous
 //   field_amplitude   A0 (momentum amplitude)
 //   n_laser_cycles    cos^2 length in cycles (default 8)
@@ -131,11 +134,17 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     const int z_range = (int)num(kv, "z_range", 32), n_coarse = (int)num(kv, "n_coarse", 4);
     const int N = (int)num(kv, "n_measurements", n_cycles * 1024);
 
-    const bool dirac = system == "Dirac";
+    const int N_k = (int)num(kv, "N_k", 64);
+    const bool dirac = system == "Dirac", honeycomb = system == "Honeycomb";
     Systems::DiracSystem dsys(T, E_F, v_F, band_width, photon_energy, tau, n_z, n_kappa);
     Systems::PiFlux psys(T, E_F, v_F, band_width, photon_energy, tau, tau_o, z_range, n_coarse);
-    const Systems::System& sys = dirac ? static_cast<const Systems::System&>(dsys) : psys;
-    const double ratio = dirac ? dsys.laser_model_ratio(photon_energy) : psys.laser_model_ratio();
+    Systems::Honeycomb hsys(T, E_F, v_F, band_width, photon_energy, tau, N_k);
+    const Systems::System& sys = dirac ? static_cast<const Systems::System&>(dsys)
+                                 : honeycomb ? static_cast<const Systems::System&>(hsys)
+                                             : psys;
+    const double ratio = dirac       ? dsys.laser_model_ratio(photon_energy)
+                         : honeycomb ? hsys.laser_model_ratio()
+                                     : psys.laser_model_ratio();
     const double A0 = Laser::momentum_amplitude(photon_energy, E0, ratio);
     std::unique_ptr<Laser::Laser> laser;
     TimeIntegrationConfig tc;
@@ -173,6 +182,8 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     o << "  \"zero_padding\": 8, \"system_type\": \"" << system << "\", \"n_z\": " << n_z;
     if (dirac)
         o << ", \"n_kappa\": " << n_kappa;
+    else if (honeycomb)
+        o << ", \"N_k\": " << N_k;
     else
         o << ", \"z_range\": " << z_range << ", \"n_coarse\": " << n_coarse;
     o << ", \"t0_offset\": " << num_str(t0_offset) << ",\n";
@@ -193,7 +204,8 @@ int main(int argc, char** argv) {
     try {
         const auto kv = read_config(argv[1]);
         const std::string system = str(kv, "system_type", "Dirac2D");
-        if (system == "Dirac" || system == "PiFlux") return run_reference_units(system, kv, argc, argv);
+        if (system == "Dirac" || system == "PiFlux" || system == "Honeycomb")
+            return run_reference_units(system, kv, argc, argv);
         const std::string laser_type = str(kv, "laser_type", "cosine");
         const double A0 = num(kv, "field_amplitude", 0.5);
         const double n_cycles = num(kv, "n_laser_cycles", 8);
@@ -238,7 +250,7 @@ int main(int argc, char** argv) {
             sys = std::move(d);
         } else if (system == "PiFlux2D") {
             sys = std::make_unique<Systems::PiFlux2D>(num(kv, "hopping", 1.0), nx, ny);
-        } else if (system == "Honeycomb") {
+        } else if (system == "Honeycomb2D") {
             sys = std::make_unique<Systems::Honeycomb>(num(kv, "hopping", 1.0), nx, ny);
         } else {
             throw std::invalid_argument("System type '" + system + "' is not recognized!");

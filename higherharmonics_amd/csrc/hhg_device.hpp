@@ -222,12 +222,31 @@ struct Bloch {
         }
     }
 
+    // Honeycomb::ic_sigma (Honeycomb.cpp:355-361) at the field d = t(Re g, -Im g, 0),
+    // e = |d|, with alpha^2 = f(E_F - t|g|), beta^2 = f(E_F + t|g|)
+    // (Honeycomb.cpp:347-353); also the honeycomb's instantaneous equilibrium.
+    __device__ __forceinline__ void honeycomb_state(const double d[3], double e, double s[3]) const {
+        const double g = e / J.hop;
+        const double a2 = fermi(J.fermi - J.hop * g, J.beta);
+        const double b2 = fermi(J.fermi + J.hop * g, J.beta);
+        const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
+        if (e == 0.0) {
+            s[0] = s[1] = 0.0;
+        } else {
+            s[0] = -diff * (d[0] / J.hop) / g;
+            s[1] = diff * (-d[1] / J.hop) / g;
+        }
+        s[2] = -prod;
+    }
+
     // Initial Bloch vector: Honeycomb::ic_sigma (Honeycomb.cpp:355-361) with
     // the occupations of Honeycomb.cpp:347-353; the 2D Dirac/PiFlux models use
     // the band-diagonal thermal state (n_up - n_low) d^, = -d^ at T = 0, E_F = 0.
     __device__ __forceinline__ void initial(double s[3]) {
         double d[3];
-        d_of(0.0, d);
+        // the relaxing honeycomb starts in the equilibrium at A(t_begin) (Honeycomb.cpp:193-194)
+        if constexpr (M == MODEL_HONEYCOMB && R == RELAX_EQUILIBRIUM) d_of(pulse_A<P>(J, J.t_begin), d);
+        else d_of(0.0, d);
         const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         if constexpr (M == MODEL_PIFLUX3D) {
             // PiFlux::ic_sigma (PiFlux.cpp:519-549), alpha^2 = f(-E_F + 2t|m|),
@@ -252,17 +271,7 @@ struct Bloch {
             s[1] = 0.0;
             s[2] = a0 * a0 - b0 * b0;
         } else if constexpr (M == MODEL_HONEYCOMB) {
-            const double g = e / J.hop;
-            const double a2 = fermi(J.fermi - J.hop * g, J.beta);
-            const double b2 = fermi(J.fermi + J.hop * g, J.beta);
-            const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
-            if (e == 0.0) {
-                s[0] = s[1] = 0.0;
-            } else {
-                s[0] = -diff * (d[0] / J.hop) / g;
-                s[1] = diff * (-d[1] / J.hop) / g;
-            }
-            s[2] = -prod;
+            honeycomb_state(d, e, s);
         } else {
             if (e == 0.0) {
                 s[0] = s[1] = s[2] = 0.0;
@@ -347,6 +356,14 @@ struct Bloch {
             out[0] -= (s[0] - s0[0]) * r;
             out[1] -= (s[1] - s0[1]) * r;
             out[2] -= (s[2] - s0[2]) * r;
+        } else if constexpr (R == RELAX_EQUILIBRIUM) {
+            // honeycomb: -(s - s_eq(k - A))/tau (Honeycomb.cpp:180-191)
+            const double r = J.relax_rate;
+            double se[3];
+            honeycomb_state(d, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), se);
+            out[0] -= r * (s[0] - se[0]);
+            out[1] -= r * (s[1] - se[1]);
+            out[2] -= r * (s[2] - se[2]);
         } else if constexpr (R == RELAX_T2) {
             const double r = J.relax_rate;
             const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];

@@ -105,12 +105,13 @@ hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n
     const bool piflux3d = m.kind == HHG_MODEL_PIFLUX3D;
     if (piflux3d && m.relax != HHG_RELAX_NONE && m.relax != HHG_RELAX_EQUILIBRIUM)
         return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D pi-flux lattice relaxes toward its equilibrium only");
-    if (!piflux3d && m.relax == HHG_RELAX_EQUILIBRIUM)
-        return fail(HHG_ERROR_UNSUPPORTED, at + "equilibrium relaxation is the 3D pi-flux lattice's");
-    // the equilibrium relaxation reads the integrator's state at the start of
-    // every step (PiFlux.cpp:144), so j(t) depends on the step sequence: only
-    // the reference's restart controller reproduces it
-    if (m.relax == HHG_RELAX_EQUILIBRIUM && sv.proposal != HHG_PROPOSAL_RESTART)
+    if (!piflux3d && m.kind != HHG_MODEL_HONEYCOMB && m.relax == HHG_RELAX_EQUILIBRIUM)
+        return fail(HHG_ERROR_UNSUPPORTED,
+                    at + "equilibrium relaxation is the 3D pi-flux and honeycomb lattices'");
+    // the pi-flux equilibrium relaxation reads the integrator's state at the
+    // start of every step (PiFlux.cpp:144), so j(t) depends on the step
+    // sequence: only the reference's restart controller reproduces it
+    if (piflux3d && m.relax == HHG_RELAX_EQUILIBRIUM && sv.proposal != HHG_PROPOSAL_RESTART)
         return fail(HHG_ERROR_UNSUPPORTED,
                     at + "equilibrium relaxation depends on the step sequence: it needs HHG_PROPOSAL_RESTART");
     if (m.relax != HHG_RELAX_NONE && !(m.relax_time > 0.0))

@@ -422,12 +422,14 @@ __global__ __launch_bounds__(256) void cost_probe_kernel(KernelArgs a, int n_pro
 
 // Run-time (method, pulse, relax, learned) -> template, for one model.  The
 // 3D pi-flux lattice has no relaxation or its equilibrium relaxation; the
-// other models have the other three.
+// honeycomb lattice has all four; the other models have all but the
+// equilibrium relaxation.
 template <int M>
 struct ModelLaunch {
     template <int R>
-    static constexpr bool has_relax =
-        M == MODEL_PIFLUX3D ? (R == RELAX_NONE || R == RELAX_EQUILIBRIUM) : (R != RELAX_EQUILIBRIUM);
+    static constexpr bool has_relax = M == MODEL_PIFLUX3D    ? (R == RELAX_NONE || R == RELAX_EQUILIBRIUM)
+                                      : M == MODEL_HONEYCOMB ? true
+                                                             : (R != RELAX_EQUILIBRIUM);
 
     template <int P, int R, bool L>
     static hipError_t solve_t(const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,

@@ -71,8 +71,11 @@ typedef enum hhg_relax_kind {
     HHG_RELAX_NONE = 0,
     HHG_RELAX_TO_INITIAL = 1, /* -(s - s0)/tau, DiracSystem.cpp:231-233 */
     HHG_RELAX_T2 = 2,         /* -(s - (s.d^)d^)/T2 about the instantaneous d(k - A(t)) */
-    HHG_RELAX_EQUILIBRIUM = 3 /* HHG_MODEL_PIFLUX3D: toward the instantaneous equilibrium of d(k - A(t)), diagonal
-                                 time relax_time, off-diagonal time relax_time_offdiag (PiFlux.cpp:113-220) */
+    HHG_RELAX_EQUILIBRIUM = 3 /* toward the instantaneous equilibrium of d(k - A(t)).  HHG_MODEL_PIFLUX3D: diagonal
+                                 time relax_time, off-diagonal time relax_time_offdiag, coherence phase of the
+                                 step-start state (PiFlux.cpp:113-220; restart proposal only).  HHG_MODEL_HONEYCOMB:
+                                 -(s - s_eq(k - A))/relax_time, starting in s_eq(k - A(t_begin))
+                                 (Honeycomb.cpp:160-199) */
 } hhg_relax_kind;
 
 typedef struct hhg_model_desc {

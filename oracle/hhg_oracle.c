@@ -16,10 +16,14 @@
  *       honeycomb: sources/HHG/Systems/Honeycomb.cpp:130-138, omega = 2t(Re g, -Im g, 0));
  *  - relaxation toward the initial state (sources/HHG/Systems/DiracSystem.cpp:231-233)
  *      and, for the 2D configurations, transverse T2 dephasing about d(k - A(t));
+ *  - the honeycomb relaxation toward the instantaneous equilibrium at k - A(t)
+ *      (sources/HHG/Systems/Honeycomb.cpp:160-199);
  *  - the 3D pi-flux lattice ds/dt = 4t m x s, m = (cos kx, cos ky, cos(kz - A))
  *      (sources/HHG/Systems/PiFlux.cpp:89-94), its thermal initial state
  *      (PiFlux.cpp:519-549) and its relaxation toward the instantaneous
  *      equilibrium (PiFlux.cpp:112-182, diagonal_sigma PiFlux.cpp:1037-1049);
+ *  - the honeycomb relaxation toward the instantaneous equilibrium at k - A(t)
+ *      (sources/HHG/Systems/Honeycomb.cpp:160-199);
  *  - the 3D pi-flux lattice ds/dt = 4t m x s, m = (cos kx, cos ky, cos(kz - A))
  *      (sources/HHG/Systems/PiFlux.cpp:89-94), its thermal initial state
  *      (PiFlux.cpp:519-549) and its relaxation toward the instantaneous
@@ -300,6 +304,23 @@ static void piflux3d_equilibrium(const or_model* m, double cx, double cy, double
     O[2] = (prod * (-cx * ze) + imag * (-cy * ze)) / nrm;
 }
 
+/* Honeycomb::ic_sigma (Honeycomb.cpp:355-361) at the field d = t(Re g, -Im g, 0)
+ * with the occupations of Honeycomb.cpp:347-353: alpha2 = f(E_F - t|g|),
+ * beta2 = f(E_F + t|g|).  Also the instantaneous equilibrium of the honeycomb
+ * relaxation, evaluated at the shifted momentum (Honeycomb.cpp:169-177). */
+static void honeycomb_state(const or_model* m, const double d[3], double s[3]) {
+    const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double g = e / m->hopping; /* |g| */
+    const double a2 = fermi(m->fermi_energy - m->hopping * g, m->beta);
+    const double b2 = fermi(m->fermi_energy + m->hopping * g, m->beta);
+    const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
+    if (e == 0.0) { s[0] = s[1] = 0.0; s[2] = -prod; return; }
+    /* Re g = d0 / t, Im g = -d1 / t */
+    s[0] = -diff * (d[0] / m->hopping) / g;
+    s[1] = diff * (-d[1] / m->hopping) / g;
+    s[2] = -prod;
+}
+
 void or_initial_state(const or_model* m, double kx, double ky, double kz, double s[3]) {
     if (m->kind == OR_MODEL_PIFLUX3D) {
         /* the continuum-limit driver rescues cos k_z of the fresh momentum of
@@ -327,17 +348,7 @@ void or_initial_state(const or_model* m, double kx, double ky, double kz, double
     model_d(m, kx, ky, d);
     const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     if (m->kind == OR_MODEL_HONEYCOMB) {
-        /* Honeycomb::ic_sigma (Honeycomb.cpp:355-361) with the occupations of
-         * Honeycomb.cpp:347-353: alpha2 = f(E_F - t|g|), beta2 = f(E_F + t|g|). */
-        const double g = e / m->hopping; /* |g| */
-        const double a2 = fermi(m->fermi_energy - m->hopping * g, m->beta);
-        const double b2 = fermi(m->fermi_energy + m->hopping * g, m->beta);
-        const double diff = a2 - b2, prod = 2.0 * sqrt(a2 * b2);
-        if (e == 0.0) { s[0] = s[1] = 0.0; s[2] = -prod; return; }
-        /* Re g = d0 / t, Im g = -d1 / t */
-        s[0] = -diff * (d[0] / m->hopping) / g;
-        s[1] = diff * (-d[1] / m->hopping) / g;
-        s[2] = -prod;
+        honeycomb_state(m, d, s);
         return;
     }
     /* 2D Dirac / PiFlux: band-diagonal thermal state (n_upper - n_lower) d_hat,
@@ -379,6 +390,12 @@ void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, double k
     if (m->relax == OR_RELAX_TO_INITIAL) {
         const double r = 1.0 / m->relax_time;
         for (int c = 0; c < 3; ++c) out[c] -= (s[c] - s_init[c]) * r;
+    } else if (m->relax == OR_RELAX_EQUILIBRIUM) {
+        /* honeycomb: -(s - s_eq(k - A))/tau (Honeycomb.cpp:180-191) */
+        const double r = 1.0 / m->relax_time;
+        double se[3];
+        honeycomb_state(m, d, se);
+        for (int c = 0; c < 3; ++c) out[c] -= r * (s[c] - se[c]);
     } else if (m->relax == OR_RELAX_T2) {
         const double r = 1.0 / m->relax_time;
         const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -710,14 +727,22 @@ static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const
     c.max_fail = sv->max_fail > 0 ? sv->max_fail : 500;
     c.st = st;
     double s[3];
-    or_initial_state(m, kx, ky, kz, s);
+    if (m->kind == OR_MODEL_HONEYCOMB && m->relax == OR_RELAX_EQUILIBRIUM) {
+        /* the relaxing honeycomb starts in the equilibrium at A(t_begin)
+         * (Honeycomb.cpp:193-194) */
+        const double A0 = or_pulse_A(p, tc->t_begin);
+        or_initial_state(m, kx - A0 * cos(m->theta), ky - A0 * sin(m->theta), kz, s);
+    } else {
+        or_initial_state(m, kx, ky, kz, s);
+    }
     c.s_init[0] = s[0];
     c.s_init[1] = s[1];
     c.s_init[2] = s[2];
     emit(user, 0, s);
-    /* equilibrium relaxation exists for the 3D pi-flux lattice only, which has
-     * no other relaxation */
-    if ((m->relax == OR_RELAX_EQUILIBRIUM) != (m->kind == OR_MODEL_PIFLUX3D && m->relax != OR_RELAX_NONE)) return -1;
+    /* equilibrium relaxation exists for the 3D pi-flux lattice, which has no
+     * other relaxation, and the honeycomb lattice */
+    if (m->relax == OR_RELAX_EQUILIBRIUM && m->kind != OR_MODEL_PIFLUX3D && m->kind != OR_MODEL_HONEYCOMB) return -1;
+    if (m->kind == OR_MODEL_PIFLUX3D && m->relax != OR_RELAX_NONE && m->relax != OR_RELAX_EQUILIBRIUM) return -1;
     const int N = tc->n_measurements;
     const double delta = (tc->t_end - tc->t_begin) / N;
     if (sv->method == OR_METHOD_MAGNUS_MOMENTS) {

@@ -216,4 +216,23 @@ PIFLUX3D_EQUILIBRIUM = Problem(
     grid=("pfc", 4, 1),
 )
 
+# the reference's honeycomb relaxation toward the instantaneous equilibrium
+HONEYCOMB_EQUILIBRIUM = Problem(
+    "honeycomb_equilibrium",
+    model=dict(kind=O.MODEL_HONEYCOMB, hopping=1.0, fermi_energy=0.4, beta=3.0, relax=O.RELAX_EQUILIBRIUM,
+               relax_time=6.0),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=1.2, n_cycles=2.0, t0=0.0, cep=0.2),
+    time=(0.0, 6 * PI, 384, 50),
+    grid=("hex", 12, 12),
+)
+
+HONEYCOMB_EQUILIBRIUM_CONT = Problem(
+    "honeycomb_equilibrium_continuous",
+    model=dict(kind=O.MODEL_HONEYCOMB, hopping=1.0, fermi_energy=0.2, beta=5.0, relax=O.RELAX_EQUILIBRIUM,
+               relax_time=3.0),
+    pulse=dict(kind=O.PULSE_CONTINUOUS, amp_a=0.9),
+    time=(0.7, 0.7 + 4 * PI, 256, 50),
+    grid=("hex", 12, 12),
+)
+
 SMALL_CASES = [DIRAC_SMALL, DIRAC_COHERENT, DIRAC_RELAX_INIT, PIFLUX_SMALL, HONEYCOMB_SMALL, HONEYCOMB_DOUBLE_COS2]

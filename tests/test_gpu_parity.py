@@ -9,8 +9,8 @@ import pytest
 
 import oracle as O
 from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
-                            HONEYCOMB_CORNERS, HONEYCOMB_SMALL, PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX_SMALL,
-                            SMALL_CASES, oracle_j, rel_l2)
+                            HONEYCOMB_CORNERS, HONEYCOMB_EQUILIBRIUM, HONEYCOMB_EQUILIBRIUM_CONT, HONEYCOMB_SMALL,
+                            PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX_SMALL, SMALL_CASES, oracle_j, rel_l2)
 
 pytestmark = pytest.mark.gpu
 
@@ -282,6 +282,31 @@ def test_piflux3d_rkf78_matches_oracle(hhg, pr, proposal):
         assert np.max(np.abs(traj[n] - s_ref)) < 1e-9
 
 
+@pytest.mark.parametrize("pr,proposal", [(HONEYCOMB_EQUILIBRIUM, 0), (HONEYCOMB_EQUILIBRIUM, 1),
+                                         (HONEYCOMB_EQUILIBRIUM_CONT, 0)],
+                         ids=["cos2-restart", "cos2-carry", "continuous-restart"])
+def test_honeycomb_equilibrium_relaxation_matches_oracle(hhg, pr, proposal):
+    """The honeycomb's relaxation toward the instantaneous equilibrium at
+    k - A(t), finite temperature and E_F, starting in the equilibrium at
+    A(t_begin) (nonzero for the continuous pulse): j(t), attempts and per-k
+    trajectories against the oracle."""
+    _print_case(pr, proposal=proposal)
+    j_ref, st_ref = oracle_j(pr, proposal=proposal)
+    sel = np.array([0, 17, 77, 143])
+    with pr.h_plan(hhg, solver=dict(proposal=proposal)) as plan:
+        j = plan.execute()[0]
+        att = plan.stats.attempts
+        traj = plan.trajectories(sel)
+    err = rel_l2(j, j_ref)
+    print(f"  rel_l2={err:.3e} attempts gpu={att} oracle={st_ref.attempts}", flush=True)
+    assert err <= J_TOL
+    assert abs(att - st_ref.attempts) <= 1e-3 * st_ref.attempts
+    kx, ky, _ = pr.o_grid()
+    for n, k in enumerate(sel):
+        s_ref = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(proposal=proposal), kx[k], ky[k])[0]
+        assert np.max(np.abs(traj[n] - s_ref)) < 1e-9
+
+
 def test_piflux3d_magnus4_matches_oracle(hhg):
     """The two-node Magnus step on the 3D pi-flux lattice, against the oracle."""
     pr = PIFLUX3D_COS2
@@ -400,3 +425,40 @@ def test_cpp_driver_reference_units_piflux(hhg, tmp_path, laser, tau):
     print(f"  rel_l2={err:.3e} oracle strict-vs-fast spread={spread:.3e}", flush=True)
     assert err <= (J_TOL if tau < 0 else max(J_TOL, 10 * spread))
     assert abs(d["attempted_steps"] - st_ref.attempts) <= 1e-3 * st_ref.attempts
+
+
+def test_cpp_driver_reference_units_honeycomb(hhg, tmp_path):
+    """hhg_run with the reference's Honeycomb keys and units: t = W/6, lattice
+    constant 4 hbar v_F/(E_ph W) as the laser's model ratio (Honeycomb.cpp:83-92),
+    relaxation toward the instantaneous equilibrium from the diagonal time in fs,
+    cos^2 + 2 pi (HoneycombDispatcher.cpp:31-35), GL over the hexagonal BZ with
+    N_k positive nodes per axis; j(t) equals the oracle's RKF78 run at the
+    reference's tolerances."""
+    import gzip
+    import json
+    import math
+    import os
+    import subprocess
+    cfg = tmp_path / "honeycomb.cfg"
+    cfg.write_text("system_type Honeycomb\nT 40\nE_F 3\nv_F 1.5e3\nband_width 12\nfield_amplitude 2\n"
+                   "photon_energy 5.25\ndiagonal_relaxation_time 150\nlaser_type cosine\nn_laser_cycles 1\n"
+                   "t0_offset 0\nN_k 6\nn_measurements 256\n")
+    out = tmp_path / "current_density.json.gz"
+    exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
+    print("[case] hhg_run Honeycomb reference units", flush=True)
+    r = subprocess.run([exe, str(cfg), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(gzip.open(out).read())
+    assert d["system_type"] == "Honeycomb" and d["N_k"] == 6
+    hbar, k_B = 6.582119569509065698e-13, 0.08617333262145177434
+    A0 = (4 * hbar * 1.5e3 / (5.25 * 12.0)) * 1e11 * 2.0 / 5.25
+    assert abs(d["momentum_amplitude"] - A0) <= 1e-15 * A0
+    assert d["t_begin"] == 0.0 and abs(d["t_end"] - 4 * math.pi) < 1e-15
+    m = O.model(O.MODEL_HONEYCOMB, hopping=2.0, fermi_energy=3.0 / 5.25, beta=5.25 / (k_B * 40.0),
+                relax=O.RELAX_EQUILIBRIUM, relax_time=150.0 * 5.25 / (1e15 * hbar))
+    kx, ky, w = O.gl_hex(12, 12)
+    j_ref, _ = O.current_density(m, O.pulse(O.PULSE_COS2, amp_a=A0, n_cycles=1.0, t0=0.0),
+                                 O.time_grid(0.0, 4 * math.pi, 256, 50), O.solver(rtol=1e-8, atol=1e-12), kx, ky, w)
+    err = rel_l2(np.array(d["current_density_time"]), j_ref)
+    print(f"  rel_l2={err:.3e}", flush=True)
+    assert err <= J_TOL

@@ -719,3 +719,39 @@ def test_piflux_continuum_grid_integrates_exactly():
     assert abs(w.sum() - math.pi ** 2) < 1e-12
     f = np.cos(kx) ** 2 * np.cos(ky) ** 2 * np.cos(kz) ** 2
     assert abs(np.dot(w, f) - math.pi ** 2 / 8) < 1e-12
+
+
+def test_honeycomb_equilibrium_relaxation_fixed_point_and_closed_form():
+    """Honeycomb relaxation toward the instantaneous equilibrium
+    (Honeycomb.cpp:160-199): without a field the thermal state is its own
+    target, so only the precession remains; after a cos^2 pulse at T = 0 the
+    target is -w^ (w = 2t(Re g, -Im g, 0)) and s + w^ turns about w while it
+    decays as exp(-t/tau).  With A(t_begin) != 0 the run starts in the
+    equilibrium at k - A(t_begin)."""
+    p0 = O.pulse(O.PULSE_COS2, amp_a=0.0, n_cycles=2.0, t0=0.0)
+    m = O.model(O.MODEL_HONEYCOMB, hopping=1.0, beta=2.0, fermi_energy=0.3, relax=O.RELAX_EQUILIBRIUM,
+                relax_time=5.0)
+    for kx, ky in ((0.7, 0.2), (-1.1, 0.9)):
+        s0 = O.initial_state(m, kx, ky)
+        g = np.sum(np.exp(1j * (NN[:, 0] * kx + NN[:, 1] * ky)))
+        w = 2.0 * np.array([g.real, -g.imag, 0.0])
+        assert np.allclose(O.rhs(m, p0, kx, ky, s0, s0, 0.4), np.cross(w, s0), rtol=0, atol=1e-14)
+    kx, ky, tau = 0.6, 0.35, 4.0
+    p = O.pulse(O.PULSE_COS2, amp_a=0.8, n_cycles=2.0, t0=0.0)
+    T_p = 4 * math.pi
+    mt = O.model(O.MODEL_HONEYCOMB, hopping=1.0, relax=O.RELAX_EQUILIBRIUM, relax_time=tau)
+    s = O.trajectory(mt, p, O.time_grid(0.0, 2 * T_p, 256, 50), O.solver(rtol=1e-12, atol=1e-13), kx, ky)[0]
+    g = np.sum(np.exp(1j * (NN[:, 0] * kx + NN[:, 1] * ky)))
+    w = 2.0 * np.array([g.real, -g.imag, 0.0])
+    wh = w / np.linalg.norm(w)
+    x1 = s[128] + wh
+    assert np.linalg.norm(x1) > 1e-3
+    for i in (150, 200, 256):
+        dt = (i - 128) * (2 * T_p / 256)
+        want = -wh + math.exp(-dt / tau) * _axis_rotation(w * dt, x1)
+        assert np.allclose(s[i], want, rtol=0, atol=1e-10)
+    # continuous pulse with A(t_begin) = A0 sin(0.7) != 0
+    pc = O.pulse(O.PULSE_CONTINUOUS, amp_a=0.5)
+    s = O.trajectory(mt, pc, O.time_grid(0.7, 1.7, 16, 50), O.solver(), kx, ky)[0]
+    A0 = 0.5 * math.sin(0.7)
+    assert np.allclose(s[0], O.initial_state(mt, kx - A0, ky), rtol=0, atol=1e-15)
