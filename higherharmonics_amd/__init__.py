@@ -34,7 +34,7 @@ SUCCESS, ERROR_INVALID_ARGUMENT, ERROR_HIP, ERROR_TOLERANCE, ERROR_OUT_OF_MEMORY
     ERROR_NO_DEVICE, ERROR_NONFINITE = range(8)
 MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D, MODEL_PIFLUX3D = 0, 1, 2, 3, 4
 RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2, RELAX_EQUILIBRIUM = 0, 1, 2, 3
-PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS = 0, 1, 2, 3, 4
+PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS, PULSE_SPLINE = 0, 1, 2, 3, 4, 5
 KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_CONTINUUM = 0, 1, 2, 3, 4
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
@@ -60,7 +60,10 @@ class ModelDesc(ctypes.Structure):
 class PulseDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("amp_a", ctypes.c_double),
                 ("amp_b", ctypes.c_double), ("n_cycles", ctypes.c_double), ("sigma", ctypes.c_double),
-                ("t0", ctypes.c_double), ("delay", ctypes.c_double), ("cep", ctypes.c_double)]
+                ("t0", ctypes.c_double), ("delay", ctypes.c_double), ("cep", ctypes.c_double),
+                ("samples", ctypes.POINTER(ctypes.c_double)), ("n_samples", ctypes.c_int64),
+                ("sample_t0", ctypes.c_double), ("sample_dt", ctypes.c_double), ("slope_begin", ctypes.c_double),
+                ("slope_end", ctypes.c_double), ("t_cut", ctypes.c_double)]
 
 
 class TimeDesc(ctypes.Structure):
@@ -118,6 +121,7 @@ EXPORTED_SYMBOLS = [
     "hhg_accumulators_to_double", "hhg_plan_set_step_log", "hhg_plan_get_step_log", "hhg_mlp_param_count",
     "hhg_mlp_create", "hhg_mlp_destroy", "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params",
     "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
+    "hhg_spline_coefficients",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -151,6 +155,8 @@ def lib() -> ctypes.CDLL:
     L.hhg_kgrid_size.argtypes = [ctypes.POINTER(KGridDesc)]
     L.hhg_kgrid_build.argtypes = [ctypes.POINTER(KGridDesc), _DP, _DP, _DP, _DP]
     L.hhg_pulse_evaluate.argtypes = [ctypes.POINTER(PulseDesc), _DP, _DP, ctypes.c_int64]
+    L.hhg_spline_coefficients.argtypes = [_DP, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          _DP]
     L.hhg_trapezoidal_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), ctypes.c_int32, _DP, _DP, _DP]
     L.hhg_welch_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), _DP, _DP, _DP]
     L.hhg_accumulators_add.restype = None
@@ -293,6 +299,37 @@ class DoubleLaser(Laser):
         d.amp_b = self.amplitude_b
         d.delay = self.delay
         return d
+
+
+class SplineLaser(Laser):
+    """A sampled pulse through the clamped cubic spline the reference builds with
+    Boost's cardinal_cubic_b_spline (Laser.hpp:8-11, 32-37): samples[i] = A(t0 + i dt),
+    end slopes (the reference passes 0, 0), A = 0 outside [t0, t_cut] (t_cut None:
+    the last sample time).  The samples are read when a plan is created."""
+
+    def __init__(self, samples, t0: float, dt: float, slope_begin: float = 0.0, slope_end: float = 0.0,
+                 t_cut: Optional[float] = None):
+        self.samples = np.ascontiguousarray(samples, dtype=np.float64)
+        self.t0, self.dt, self.slope_begin, self.slope_end = t0, dt, slope_begin, slope_end
+        self.t_begin = t0
+        self.t_end = t0 + (len(self.samples) - 1) * dt if t_cut is None else t_cut
+        self.t_cut = t_cut
+
+    def desc(self) -> PulseDesc:
+        d = PulseDesc(PULSE_SPLINE)
+        d.samples, d.n_samples = _dp(self.samples), len(self.samples)
+        d.sample_t0, d.sample_dt = self.t0, self.dt
+        d.slope_begin, d.slope_end = self.slope_begin, self.slope_end
+        d.t_cut = self.t_cut if self.t_cut is not None else self.t0
+        return d
+
+
+def spline_coefficients(samples, dt: float, slope_begin: float = 0.0, slope_end: float = 0.0) -> np.ndarray:
+    """(n - 1, 4) coefficients c of the clamped cubic spline, A(t_i + x dt) = c0 + x (c1 + x (c2 + x c3))."""
+    y = np.ascontiguousarray(samples, dtype=np.float64)
+    coef = np.empty(4 * max(len(y) - 1, 0))
+    _check(lib().hhg_spline_coefficients(_dp(y), len(y), dt, slope_begin, slope_end, _dp(coef)))
+    return coef.reshape(-1, 4)
 
 
 @dataclass

@@ -48,9 +48,24 @@ __device__ __forceinline__ double env_gauss(JobRef J, double t) {
     return exp(-(x * x) * J.inv_2s2);
 }
 
+// Spline pulse: the clamped cubic spline of the samples, per interval a cubic
+// in x = (t - t_i) / h (coefficients built on the host); the lanes of a wave
+// usually evaluate the same interval, so the 32-byte row is one cached load.
+__device__ __forceinline__ double spline_A(JobRef J, double t) {
+    if (!(t >= J.spl_t0) || t > J.spl_t_last) return 0.0;
+    const double u = (t - J.spl_t0) * J.spl_inv_h;
+    int i = (int)u;
+    i = i < J.spl_n - 1 ? i : J.spl_n - 1;
+    const double x = u - (double)i;
+    const double* c = J.spl + 4 * (int64_t)i;
+    return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+}
+
 template <int P>
 __device__ __forceinline__ double pulse_A(JobRef J, double t) {
-    if constexpr (P == PULSE_COS2) {
+    if constexpr (P == PULSE_SPLINE) {
+        return spline_A(J, t);
+    } else if constexpr (P == PULSE_COS2) {
         return J.amp_a * env_cos2(J, t) * sin(t + J.cep);
     } else if constexpr (P == PULSE_GAUSS) {
         return J.amp_a * env_gauss(J, t) * sin(t + J.cep);
@@ -96,6 +111,7 @@ struct PulseCache {
     static constexpr bool kGauss = (P == PULSE_GAUSS || P == PULSE_DOUBLE_GAUSS);
     static constexpr bool kDouble = (P == PULSE_DOUBLE_GAUSS || P == PULSE_DOUBLE_COS2);
     static constexpr bool kCont = (P == PULSE_CONTINUOUS);
+    static constexpr bool kSpline = (P == PULSE_SPLINE);
     double t;
     PulseComp a, b;
 
@@ -106,6 +122,7 @@ struct PulseCache {
 
     __device__ __forceinline__ void prepare(JobRef J, double t_) {
         t = t_;
+        if constexpr (kSpline) return;
         prep_comp(J, t, a);
         if constexpr (kDouble) prep_comp(J, t - J.delay, b);
     }
@@ -132,6 +149,7 @@ struct PulseCache {
 
     // A(t + dlt)
     __device__ __forceinline__ double at(JobRef J, double dlt) const {
+        if constexpr (kSpline) return spline_A(J, t + dlt);
         double sd, cd;
         small_sincos(dlt, sd, cd);
         double r = J.amp_a * comp_at(J, a, t, dlt, sd, cd);

@@ -1,6 +1,7 @@
 // hhg_host.cpp — host utilities of the C ABI (no GPU needed): Gauss-Legendre
 // nodes, Brillouin-zone grids, host pulse evaluation and the end-corrected
 // spectrum.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -226,6 +227,67 @@ static double env_gauss_h(const hhg_pulse_desc& p, double t) {
     return std::exp(-(x * x) / (2.0 * p.sigma * p.sigma));
 }
 
+// Clamped cubic spline through n >= 2 samples spaced h with end slopes d0, d1:
+// the C2 piecewise cubic that the reference's Boost cardinal_cubic_b_spline
+// represents in a B-spline basis (Laser.hpp:8-11).  Solved here for the second
+// derivatives M_i (tridiagonal, diagonally dominant, Thomas elimination):
+//   2 M_0 + M_1 = 6/h ((y_1 - y_0)/h - d0)
+//   M_{i-1} + 4 M_i + M_{i+1} = 6/h^2 (y_{i+1} - 2 y_i + y_{i-1})
+//   M_{n-2} + 2 M_{n-1} = 6/h (d1 - (y_{n-1} - y_{n-2})/h)
+// then stored per interval as a cubic in x = (t - t_i)/h.
+hhg_status spline_coefficients(const double* y, int64_t n, double h, double d0, double d1, double* coef) {
+    if (n < 2 || !(h > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, "spline needs n >= 2 samples and dt > 0");
+    std::vector<double> diag(n), rhs(n), M(n);
+    const double s6 = 6.0 / h;
+    for (int64_t i = 0; i < n; ++i) {
+        diag[i] = (i == 0 || i == n - 1) ? 2.0 : 4.0;
+        if (i == 0)
+            rhs[i] = s6 * ((y[1] - y[0]) / h - d0);
+        else if (i == n - 1)
+            rhs[i] = s6 * (d1 - (y[n - 1] - y[n - 2]) / h);
+        else
+            rhs[i] = s6 * ((y[i + 1] - 2.0 * y[i] + y[i - 1]) / h);
+    }
+    // off-diagonals are 1: forward elimination, back substitution
+    for (int64_t i = 1; i < n; ++i) {
+        const double f = 1.0 / diag[i - 1];
+        diag[i] -= f;
+        rhs[i] -= f * rhs[i - 1];
+    }
+    M[n - 1] = rhs[n - 1] / diag[n - 1];
+    for (int64_t i = n - 2; i >= 0; --i) M[i] = (rhs[i] - M[i + 1]) / diag[i];
+    const double h2 = h * h;
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        double* c = coef + 4 * i;
+        c[0] = y[i];
+        c[1] = (y[i + 1] - y[i]) - h2 * (2.0 * M[i] + M[i + 1]) / 6.0;
+        c[2] = 0.5 * h2 * M[i];
+        c[3] = h2 * (M[i + 1] - M[i]) / 6.0;
+    }
+    return HHG_SUCCESS;
+}
+
+SplinePulse::SplinePulse(const hhg_pulse_desc& p) {
+    n = p.n_samples - 1;
+    coef.assign(4 * std::max<int64_t>(n, 0), 0.0);
+    status = spline_coefficients(p.samples, p.n_samples, p.sample_dt, p.slope_begin, p.slope_end, coef.data());
+    t0 = p.sample_t0;
+    inv_h = 1.0 / p.sample_dt;
+    const double last = p.sample_t0 + (double)n * p.sample_dt;
+    t_last = p.t_cut > p.sample_t0 ? std::min(p.t_cut, last) : last;
+}
+
+// the device's spline_A with the same operations
+double SplinePulse::operator()(double t) const {
+    if (!(t >= t0) || t > t_last) return 0.0;
+    const double u = (t - t0) * inv_h;
+    int64_t i = (int64_t)u;
+    i = i < n - 1 ? i : n - 1;
+    const double x = u - (double)i;
+    const double* c = coef.data() + 4 * i;
+    return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+}
+
 double pulse_eval(const hhg_pulse_desc& p, double t) {
     switch (p.kind) {
     case HHG_PULSE_COS2: return p.amp_a * env_cos2_h(p, t) * std::sin(t + p.cep);
@@ -388,9 +450,23 @@ hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, doub
 hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n) {
     hhg::clear_error();
     if (!p || (n > 0 && (!t || !A))) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_pulse_evaluate: null argument");
-    if (p->kind < 0 || p->kind > 4) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "unknown pulse kind");
+    if (p->kind < 0 || p->kind > 5) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "unknown pulse kind");
+    if (p->kind == HHG_PULSE_SPLINE) {
+        if (!p->samples) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "spline pulse needs samples");
+        const hhg::SplinePulse sp(*p);
+        if (sp.status != HHG_SUCCESS) return sp.status;
+        for (int64_t i = 0; i < n; ++i) A[i] = sp(t[i]);
+        return HHG_SUCCESS;
+    }
     for (int64_t i = 0; i < n; ++i) A[i] = hhg::pulse_eval(*p, t[i]);
     return HHG_SUCCESS;
+}
+
+hhg_status hhg_spline_coefficients(const double* samples, int64_t n, double dt, double slope_begin, double slope_end,
+                                   double* coef) {
+    hhg::clear_error();
+    if (!samples || !coef) return hhg::fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_spline_coefficients: null argument");
+    return hhg::spline_coefficients(samples, n, dt, slope_begin, slope_end, coef);
 }
 
 hhg_status hhg_trapezoidal_spectrum(const double* j, const hhg_time_desc* time, int32_t zero_padding, double* freq,

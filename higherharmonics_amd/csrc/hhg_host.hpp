@@ -2,6 +2,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/hhg_amd.h"
 
@@ -18,7 +19,18 @@ const char* last_error();
 void gauss_legendre(int n, double* x, double* w);
 int64_t kgrid_size(const hhg_kgrid_desc& g);
 hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* kz, double* w);
-double pulse_eval(const hhg_pulse_desc& p, double t);
+double pulse_eval(const hhg_pulse_desc& p, double t);  // analytic kinds
+hhg_status spline_coefficients(const double* y, int64_t n, double h, double d0, double d1, double* coef);
+
+// A spline pulse's coefficient table and its evaluation (HHG_PULSE_SPLINE).
+struct SplinePulse {
+    std::vector<double> coef;  // 4 per interval
+    int64_t n = 0;             // intervals
+    double t0 = 0.0, inv_h = 0.0, t_last = 0.0;
+    hhg_status status = HHG_SUCCESS;
+    explicit SplinePulse(const hhg_pulse_desc& p);
+    double operator()(double t) const;
+};
 hhg_status trapezoidal_spectrum(const double* j, const hhg_time_desc& tc, int zero_padding, double* freq, double* re,
                                 double* im);
 

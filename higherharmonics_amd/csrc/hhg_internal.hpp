@@ -32,7 +32,8 @@ enum : int {
     PULSE_DOUBLE_GAUSS = 2,
     PULSE_DOUBLE_COS2 = 3,
     PULSE_CONTINUOUS = 4,
-    N_PULSES = 5
+    PULSE_SPLINE = 5,
+    N_PULSES = 6
 };
 enum : int { METHOD_RKF78 = 0, METHOD_MAGNUS4 = 1, METHOD_MAGNUS_MOMENTS = 2 };
 enum : int { RELAX_NONE = 0, RELAX_TO_INITIAL = 1, RELAX_T2 = 2, RELAX_EQUILIBRIUM = 3 };
@@ -65,6 +66,11 @@ struct JobDev {
     double t_env_end;     // t0 + 2 pi n_cycles (cos^2)
     double inv_2s2;       // 1 / (2 sigma^2) (Gaussian)
     double t0, delay, cep;
+    // spline pulse: 4 coefficients per interval, intervals of length 1/spl_inv_h
+    // from spl_t0; A = 0 outside [spl_t0, spl_t_last]
+    const double* spl;
+    double spl_t0, spl_inv_h, spl_t_last;
+    int32_t spl_n, pad3;
     // time grid
     double t_begin, delta;
     int32_t n_meas, n_sub;

@@ -125,6 +125,7 @@ hipError_t by_pulse(int pulse, F&& f) {
     case PULSE_DOUBLE_GAUSS: return f(std::integral_constant<int, PULSE_DOUBLE_GAUSS>{});
     case PULSE_DOUBLE_COS2: return f(std::integral_constant<int, PULSE_DOUBLE_COS2>{});
     case PULSE_CONTINUOUS: return f(std::integral_constant<int, PULSE_CONTINUOUS>{});
+    case PULSE_SPLINE: return f(std::integral_constant<int, PULSE_SPLINE>{});
     }
     return hipErrorInvalidValue;
 }

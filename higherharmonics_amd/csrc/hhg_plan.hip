@@ -53,6 +53,7 @@ struct hhg_plan {
     int num_cus = 0;
 
     double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_w = nullptr;
+    double* d_spl = nullptr;  // spline pulse coefficients of every spline job, concatenated
     hhg::JobDev* d_jobs = nullptr;
     int2* d_items = nullptr;
     double *d_At = nullptr, *d_Am = nullptr, *d_nom = nullptr;
@@ -122,7 +123,12 @@ hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n
     if (m.kind == HHG_MODEL_DIRAC3D && m.theta != 0.0)
         return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D Dirac cone takes the field along z (theta = 0)");
     if (lattice && !(m.hopping > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "lattice models need hopping > 0");
-    if (p.kind < 0 || p.kind > 4) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown pulse kind");
+    if (p.kind < 0 || p.kind > 5) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown pulse kind");
+    if (p.kind == HHG_PULSE_SPLINE &&
+        (!p.samples || p.n_samples < 2 || p.n_samples > (int64_t)1 << 30 || !(p.sample_dt > 0.0) ||
+         !std::isfinite(p.sample_t0) || !std::isfinite(p.slope_begin) || !std::isfinite(p.slope_end)))
+        return fail(HHG_ERROR_INVALID_ARGUMENT,
+                    at + "spline pulse needs 2 <= n_samples <= 2^30 samples, sample_dt > 0, finite t0 and slopes");
     const bool gauss = p.kind == HHG_PULSE_GAUSS || p.kind == HHG_PULSE_DOUBLE_GAUSS;
     const bool cos2 = p.kind == HHG_PULSE_COS2 || p.kind == HHG_PULSE_DOUBLE_COS2;
     if (gauss && !(p.sigma > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "Gaussian pulse needs sigma > 0");
@@ -218,6 +224,7 @@ void destroy_device(hhg_plan* p) {
     hipFree(p->d_kx);
     hipFree(p->d_ky);
     hipFree(p->d_kz);
+    hipFree(p->d_spl);
     hipFree(p->d_w);
     hipFree(p->d_jobs);
     hipFree(p->d_items);
@@ -410,6 +417,8 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     // jobs: shard = contiguous block of 64-point chunks of each job's grid
     p->offsets.assign(d->n_jobs + 1, 0);
     int64_t magnus_words = 0, nominal_words = 0;
+    std::vector<double> spl;             // spline pulse coefficients, concatenated
+    std::vector<int64_t> spl_off(d->n_jobs, -1);
     for (int i = 0; i < d->n_jobs; ++i) {
         hhg::JobDev J = make_job(d->jobs[i], sv);
         const int g = d->jobs[i].kgrid;
@@ -428,6 +437,17 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         J.nominal_base = nominal_words;
         if (sv.method == HHG_METHOD_RKF78) nominal_words += (int64_t)J.n_meas * hhg::kNomRow;
         J.row_base = 0;
+        const hhg_pulse_desc& pd = d->jobs[i].pulse;
+        if (pd.kind == HHG_PULSE_SPLINE) {
+            const hhg::SplinePulse sp(pd);
+            if (sp.status != HHG_SUCCESS) return bail(sp.status);
+            spl_off[i] = (int64_t)spl.size();
+            spl.insert(spl.end(), sp.coef.begin(), sp.coef.end());
+            J.spl_t0 = sp.t0;
+            J.spl_inv_h = sp.inv_h;
+            J.spl_t_last = sp.t_last;
+            J.spl_n = (int32_t)sp.n;
+        }
         p->jobs.push_back(J);
     }
     p->n_out = p->offsets[d->n_jobs];
@@ -483,6 +503,12 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     C(p->d_ky, ky.data(), sizeof(double) * p->n_kpts, "upload ky");
     if (has_kz) C(p->d_kz, kz.data(), sizeof(double) * p->n_kpts, "upload kz");
     C(p->d_w, w.data(), sizeof(double) * p->n_kpts, "upload w");
+    if (!spl.empty()) {
+        A((void**)&p->d_spl, sizeof(double) * spl.size(), "spline coefficients");
+        C(p->d_spl, spl.data(), sizeof(double) * spl.size(), "upload spline coefficients");
+        for (int i = 0; i < d->n_jobs; ++i)
+            if (spl_off[i] >= 0) p->jobs[i].spl = p->d_spl + spl_off[i];
+    }
     C(p->d_jobs, p->jobs.data(), sizeof(hhg::JobDev) * p->jobs.size(), "upload jobs");
     C(p->d_items, items.data(), sizeof(int2) * items.size(), "upload items");
     if (st != HHG_SUCCESS) return bail(st);

@@ -490,6 +490,7 @@ struct ModelLaunch {
         case PULSE_DOUBLE_GAUSS: return solve_r<PULSE_DOUBLE_GAUSS>(relax, learned, a, blocks, threads, smem, st, fn);
         case PULSE_DOUBLE_COS2: return solve_r<PULSE_DOUBLE_COS2>(relax, learned, a, blocks, threads, smem, st, fn);
         case PULSE_CONTINUOUS: return solve_r<PULSE_CONTINUOUS>(relax, learned, a, blocks, threads, smem, st, fn);
+        case PULSE_SPLINE: return solve_r<PULSE_SPLINE>(relax, learned, a, blocks, threads, smem, st, fn);
         }
         return hipErrorInvalidValue;
     }
@@ -533,6 +534,7 @@ struct ModelLaunch {
         case PULSE_DOUBLE_GAUSS: return traj_r<PULSE_DOUBLE_GAUSS>(method, relax, x);
         case PULSE_DOUBLE_COS2: return traj_r<PULSE_DOUBLE_COS2>(method, relax, x);
         case PULSE_CONTINUOUS: return traj_r<PULSE_CONTINUOUS>(method, relax, x);
+        case PULSE_SPLINE: return traj_r<PULSE_SPLINE>(method, relax, x);
         }
         return hipErrorInvalidValue;
     }
@@ -559,6 +561,10 @@ struct ModelLaunch {
         case PULSE_CONTINUOUS:
             hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_CONTINUOUS>), dim3(blocks), dim3(256), smem, st, a,
                                n_probe, cost_out);
+            break;
+        case PULSE_SPLINE:
+            hipLaunchKernelGGL((cost_probe_kernel<M, PULSE_SPLINE>), dim3(blocks), dim3(256), smem, st, a, n_probe,
+                               cost_out);
             break;
         default:
             return hipErrorInvalidValue;

@@ -97,7 +97,11 @@ typedef enum hhg_pulse_kind {
     HHG_PULSE_GAUSS = 1,        /* GaussLaser.cpp:26-28, centre t0, width sigma */
     HHG_PULSE_DOUBLE_GAUSS = 2, /* DoubleLaser<GaussLaser>: A(t) + B(t - delay), DoubleLaser.hpp:63-67 */
     HHG_PULSE_DOUBLE_COS2 = 3,  /* DoubleLaser<CosineLaser> */
-    HHG_PULSE_CONTINUOUS = 4    /* ContinuousLaser.cpp:7-9, envelope 1 */
+    HHG_PULSE_CONTINUOUS = 4,   /* ContinuousLaser.cpp:7-9, envelope 1 */
+    HHG_PULSE_SPLINE = 5        /* a sampled pulse through the cubic spline the reference builds with Boost's
+                                   cardinal_cubic_b_spline (Laser.hpp:8-11, 32-37; DoubleLaser.hpp:48-73,
+                                   ExperimentalLaser.cpp:229-257): the C2 interpolant of the samples with the given
+                                   end slopes; A = 0 outside [sample_t0, t_cut] */
 } hhg_pulse_kind;
 
 typedef struct hhg_pulse_desc {
@@ -110,6 +114,14 @@ typedef struct hhg_pulse_desc {
     double t0;       /* cos^2 window begin / Gaussian centre of pulse A */
     double delay;    /* pulse B delay */
     double cep;      /* carrier-envelope phase */
+    /* HHG_PULSE_SPLINE: samples[i] = A(sample_t0 + i sample_dt), i < n_samples (>= 2); the end slopes
+     * (the reference passes 0 and 0); t_cut: A = 0 beyond it (the reference's Laser::t_end, Laser.hpp:35;
+     * t_cut <= sample_t0 means the last sample time).  The samples are read at plan creation. */
+    const double* samples;
+    int64_t n_samples;
+    double sample_t0, sample_dt;
+    double slope_begin, slope_end;
+    double t_cut;
 } hhg_pulse_desc;
 
 /* TimeIntegrationConfig: samples t_i = t_begin + i (t_end - t_begin)/N, i = 0..N;
@@ -277,6 +289,13 @@ hhg_status hhg_gauss_legendre(int32_t n, double* x, double* w);
  * null except for HHG_KGRID_PIFLUX_CONTINUUM; it is 0 for 2D grids). */
 int64_t hhg_kgrid_size(const hhg_kgrid_desc* g);
 hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, double* kz, double* w);
+
+/* Coefficients of the clamped cubic spline through n >= 2 samples spaced dt (the
+ * function Boost's cardinal_cubic_b_spline builds, Laser.hpp:8-11): for interval i,
+ * 4 doubles c with A(t_i + x dt) = c0 + x (c1 + x (c2 + x c3)), x in [0, 1];
+ * coef holds 4 (n - 1) doubles. */
+hhg_status hhg_spline_coefficients(const double* samples, int64_t n, double dt, double slope_begin, double slope_end,
+                                   double* coef);
 
 /* A(t) at n times (host evaluation, Laser::laser_function). */
 hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n);

@@ -24,7 +24,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D, MODEL_PIFLUX3D = 0, 1, 2, 3, 4
-PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS = 0, 1, 2, 3, 4
+PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS, PULSE_SPLINE = 0, 1, 2, 3, 4, 5
 RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2, RELAX_EQUILIBRIUM = 0, 1, 2, 3
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY = 0, 1
@@ -40,7 +40,9 @@ class Model(ctypes.Structure):
 class Pulse(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("amp_a", ctypes.c_double),
                 ("amp_b", ctypes.c_double), ("n_cycles", ctypes.c_double), ("sigma", ctypes.c_double),
-                ("t0", ctypes.c_double), ("delay", ctypes.c_double), ("cep", ctypes.c_double)]
+                ("t0", ctypes.c_double), ("delay", ctypes.c_double), ("cep", ctypes.c_double),
+                ("coef", ctypes.POINTER(ctypes.c_double)), ("n_intervals", ctypes.c_int64),
+                ("s_t0", ctypes.c_double), ("s_dt", ctypes.c_double), ("s_t_last", ctypes.c_double)]
 
 
 class Time(ctypes.Structure):
@@ -80,6 +82,8 @@ def lib(fast: bool = False) -> ctypes.CDLL:
     L.or_pulse_A.restype = ctypes.c_double
     L.or_pulse_A.argtypes = [ctypes.POINTER(Pulse), ctypes.c_double]
     L.or_gauss_legendre.argtypes = [ctypes.c_int, dp, dp]
+    L.or_spline_coefficients.restype = ctypes.c_int
+    L.or_spline_coefficients.argtypes = [dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
     L.or_rkf78_tableau.argtypes = [dp, dp, dp, dp]
     d3 = [ctypes.c_double] * 3
     L.or_initial_state.argtypes = [ctypes.POINTER(Model), *d3, dp]
@@ -111,6 +115,29 @@ def model(kind=MODEL_DIRAC2D, v=1.0, mass=0.1, hopping=1.0, theta=0.0, fermi_ene
 
 def pulse(kind=PULSE_COS2, amp_a=0.5, amp_b=0.0, n_cycles=8.0, sigma=0.0, t0=0.0, delay=0.0, cep=0.0) -> Pulse:
     return Pulse(kind, 0, amp_a, amp_b, n_cycles, sigma, t0, delay, cep)
+
+
+def spline_coefficients(samples, dt: float, slope_begin: float = 0.0, slope_end: float = 0.0) -> np.ndarray:
+    """(n - 1, 4) coefficients of the clamped cubic spline (slope form)."""
+    y = np.ascontiguousarray(samples, dtype=np.float64)
+    coef = np.empty(4 * max(len(y) - 1, 0))
+    if lib().or_spline_coefficients(_dp(y), len(y), dt, slope_begin, slope_end, _dp(coef)) != 0:
+        raise ValueError("bad spline arguments")
+    return coef.reshape(-1, 4)
+
+
+def spline_pulse(samples, t0: float, dt: float, slope_begin: float = 0.0, slope_end: float = 0.0,
+                 t_cut=None) -> Pulse:
+    """A sampled pulse through its clamped cubic spline, A = 0 outside
+    [t0, t_cut] (t_cut None: the last sample time)."""
+    coef = np.ascontiguousarray(spline_coefficients(samples, dt, slope_begin, slope_end).ravel())
+    last = t0 + (len(samples) - 1) * dt
+    p = Pulse(PULSE_SPLINE)
+    p.coef, p.n_intervals = _dp(coef), len(samples) - 1
+    p.s_t0, p.s_dt = t0, dt
+    p.s_t_last = last if t_cut is None or t_cut <= t0 else min(t_cut, last)
+    p._keep = coef  # the C side reads the coefficients through p.coef
+    return p
 
 
 def time_grid(t_begin, t_end, n_measurements, n_subdivisions=50) -> Time:

@@ -93,9 +93,69 @@ double or_pulse_A(const or_pulse* p, double t) {
     case OR_PULSE_CONTINUOUS:
         /* ContinuousLaser: envelope 1 (sources/HHG/Laser/ContinuousLaser.cpp:7-9) */
         return p->amp_a * sin(t + p->cep);
+    case OR_PULSE_SPLINE: {
+        /* Laser::laser_function of a splined pulse (Laser.hpp:32-37): 0 beyond t_end */
+        if (!(t >= p->s_t0) || t > p->s_t_last) return 0.0;
+        const double u = (t - p->s_t0) * (1.0 / p->s_dt);
+        int64_t i = (int64_t)u;
+        if (i > p->n_intervals - 1) i = p->n_intervals - 1;
+        const double x = u - (double)i;
+        const double* c = p->coef + 4 * i;
+        return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+    }
     default:
         return NAN;
     }
+}
+
+/* The clamped cubic spline in its slope (Hermite) form: the slopes m_i solve
+ * m_{i-1} + 4 m_i + m_{i+1} = 3 (y_{i+1} - y_{i-1}) / h with m_0 = d0,
+ * m_{n-1} = d1 (C2 continuity at the inner nodes), and each interval is the
+ * cubic Hermite interpolant of (y_i, m_i), (y_{i+1}, m_{i+1}).  Boost's
+ * cardinal_cubic_b_spline (Laser.hpp:8-11, a third-party header absent here)
+ * represents the same unique C2 interpolant with these end slopes in a
+ * B-spline basis. */
+int or_spline_coefficients(const double* y, int64_t n, double dt, double d0, double d1, double* coef) {
+    if (n < 2 || !(dt > 0.0) || !y || !coef) return -1;
+    double* m = (double*)malloc(sizeof(double) * (size_t)n);
+    double* cp = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!m || !cp) {
+        free(m);
+        free(cp);
+        return -1;
+    }
+    m[0] = d0;
+    m[n - 1] = d1;
+    if (n > 2) {
+        /* unknowns m_1..m_{n-2}: tridiagonal (1, 4, 1), Thomas with the known end slopes moved right */
+        const int64_t k = n - 2;
+        for (int64_t j = 0; j < k; ++j) {
+            const int64_t i = j + 1;
+            double r = 3.0 * (y[i + 1] - y[i - 1]) / dt;
+            if (j == 0) r -= d0;
+            if (j == k - 1) r -= d1;
+            if (j == 0) {
+                cp[j] = 1.0 / 4.0;
+                m[i] = r / 4.0;
+            } else {
+                const double den = 4.0 - cp[j - 1];
+                cp[j] = 1.0 / den;
+                m[i] = (r - m[i - 1]) / den;
+            }
+        }
+        for (int64_t j = k - 2; j >= 0; --j) m[j + 1] -= cp[j] * m[j + 2];
+    }
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        double* c = coef + 4 * i;
+        const double dy = y[i + 1] - y[i];
+        c[0] = y[i];
+        c[1] = dt * m[i];
+        c[2] = 3.0 * dy - dt * (2.0 * m[i] + m[i + 1]);
+        c[3] = -2.0 * dy + dt * (m[i] + m[i + 1]);
+    }
+    free(m);
+    free(cp);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------- */

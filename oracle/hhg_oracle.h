@@ -19,7 +19,14 @@ extern "C" {
 #endif
 
 enum { OR_MODEL_DIRAC2D = 0, OR_MODEL_PIFLUX2D = 1, OR_MODEL_HONEYCOMB = 2, OR_MODEL_DIRAC3D = 3, OR_MODEL_PIFLUX3D = 4 };
-enum { OR_PULSE_COS2 = 0, OR_PULSE_GAUSS = 1, OR_PULSE_DOUBLE_GAUSS = 2, OR_PULSE_DOUBLE_COS2 = 3, OR_PULSE_CONTINUOUS = 4 };
+enum {
+    OR_PULSE_COS2 = 0,
+    OR_PULSE_GAUSS = 1,
+    OR_PULSE_DOUBLE_GAUSS = 2,
+    OR_PULSE_DOUBLE_COS2 = 3,
+    OR_PULSE_CONTINUOUS = 4,
+    OR_PULSE_SPLINE = 5
+};
 enum { OR_RELAX_NONE = 0, OR_RELAX_TO_INITIAL = 1, OR_RELAX_T2 = 2, OR_RELAX_EQUILIBRIUM = 3 };
 enum { OR_METHOD_RKF78 = 0, OR_METHOD_MAGNUS4 = 1, OR_METHOD_MAGNUS_MOMENTS = 2 };
 enum { OR_PROPOSAL_RESTART = 0, OR_PROPOSAL_CARRY = 1 };
@@ -49,6 +56,11 @@ typedef struct {
     double t0;         /* cos^2: window begin; Gaussian: centre of pulse A */
     double delay;      /* B pulse delay relative to A */
     double cep;        /* carrier-envelope phase added to the carrier */
+    /* OR_PULSE_SPLINE: per interval 4 coefficients (or_spline_coefficients) of
+     * the cubic in x = (t - t_i)/dt; A = 0 outside [s_t0, s_t_last] */
+    const double* coef;
+    int64_t n_intervals;
+    double s_t0, s_dt, s_t_last;
 } or_pulse;
 
 typedef struct {
@@ -76,6 +88,12 @@ typedef struct {
 
 /* Pulse A(t) (vector potential including its amplitude). */
 double or_pulse_A(const or_pulse* p, double t);
+
+/* Clamped cubic spline through n >= 2 samples spaced dt with end slopes d0, d1
+ * (the interpolant of Boost's cardinal_cubic_b_spline): coef[4 (n - 1)], per
+ * interval c with y(t_i + x dt) = c0 + x (c1 + x (c2 + x c3)).  Returns 0, or -1
+ * for bad arguments. */
+int or_spline_coefficients(const double* y, int64_t n, double dt, double d0, double d1, double* coef);
 
 /* Gauss-Legendre nodes/weights on [-1,1], ascending nodes. */
 void or_gauss_legendre(int n, double* x, double* w);

@@ -27,6 +27,10 @@ class Problem:
         return O.model(**m)
 
     def o_pulse(self):
+        if self.pulse.get("kind") == O.PULSE_SPLINE:
+            p = dict(self.pulse)
+            del p["kind"]
+            return O.spline_pulse(**p)
         return O.pulse(**self.pulse)
 
     def o_time(self):
@@ -63,6 +67,10 @@ class Problem:
     def h_laser(self, H):
         p = self.pulse
         kind = p.get("kind", 0)
+        if kind == O.PULSE_SPLINE:
+            q = dict(p)
+            del q["kind"]
+            return H.SplineLaser(**q)
 
         class _L(H.Laser):
             def desc(_self):
@@ -232,6 +240,25 @@ HONEYCOMB_EQUILIBRIUM_CONT = Problem(
                relax_time=3.0),
     pulse=dict(kind=O.PULSE_CONTINUOUS, amp_a=0.9),
     time=(0.7, 0.7 + 4 * PI, 256, 50),
+    grid=("hex", 12, 12),
+)
+
+# a sampled A/B cos^2 pulse through its clamped cubic spline over 1.5 x the
+# window as the reference's DoubleLaser table (DoubleLaser.hpp:48-73; it takes
+# 1000 samples, 300 here make the spline error visible in j)
+def _double_cos2_samples(n=300, n_cycles=2.0, amp_a=1.1, amp_b=0.5, delay=2.5):
+    t_max = 1.5 * 2 * PI * n_cycles
+    t = np.arange(n) * (t_max / (n - 1))
+    p = O.pulse(O.PULSE_DOUBLE_COS2, amp_a=amp_a, amp_b=amp_b, n_cycles=n_cycles, t0=0.0, delay=delay)
+    return O.pulse_A(p, t), t_max / (n - 1)
+
+
+_SPL_Y, _SPL_DT = _double_cos2_samples()
+HONEYCOMB_SPLINE = Problem(
+    "honeycomb_spline_double_cos2",
+    model=dict(kind=O.MODEL_HONEYCOMB, hopping=1.0),
+    pulse=dict(kind=O.PULSE_SPLINE, samples=_SPL_Y, t0=0.0, dt=_SPL_DT),
+    time=(0.0, 6 * PI, 384, 50),
     grid=("hex", 12, 12),
 )
 

@@ -10,6 +10,7 @@ import pytest
 import oracle as O
 from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
                             HONEYCOMB_CORNERS, HONEYCOMB_EQUILIBRIUM, HONEYCOMB_EQUILIBRIUM_CONT, HONEYCOMB_SMALL,
+                            HONEYCOMB_SPLINE,
                             PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX_SMALL, SMALL_CASES, oracle_j, rel_l2)
 
 pytestmark = pytest.mark.gpu
@@ -305,6 +306,35 @@ def test_honeycomb_equilibrium_relaxation_matches_oracle(hhg, pr, proposal):
     for n, k in enumerate(sel):
         s_ref = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(proposal=proposal), kx[k], ky[k])[0]
         assert np.max(np.abs(traj[n] - s_ref)) < 1e-9
+
+
+@pytest.mark.parametrize("proposal", [0, 1], ids=["restart", "carry"])
+def test_spline_pulse_matches_oracle(hhg, proposal):
+    """A sampled A/B cos^2 pulse through its clamped cubic spline (the
+    reference's DoubleLaser table over 1.5 x the window, end slopes 0): j(t),
+    attempts and per-k trajectories against the oracle; the splined j differs
+    from the analytic pulse's by the spline error."""
+    pr = HONEYCOMB_SPLINE
+    _print_case(pr, proposal=proposal)
+    j_ref, st_ref = oracle_j(pr, proposal=proposal)
+    sel = np.array([0, 31, 99])
+    with pr.h_plan(hhg, solver=dict(proposal=proposal)) as plan:
+        j = plan.execute()[0]
+        att = plan.stats.attempts
+        traj = plan.trajectories(sel)
+    err = rel_l2(j, j_ref)
+    print(f"  rel_l2={err:.3e} attempts gpu={att} oracle={st_ref.attempts}", flush=True)
+    assert err <= J_TOL
+    assert abs(att - st_ref.attempts) <= 1e-3 * st_ref.attempts
+    kx, ky, _ = pr.o_grid()
+    for n, k in enumerate(sel):
+        s_ref = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(proposal=proposal), kx[k], ky[k])[0]
+        assert np.max(np.abs(traj[n] - s_ref)) < 1e-9
+    from tests.problems import Problem
+    analytic = Problem("analytic", pr.model, dict(kind=O.PULSE_DOUBLE_COS2, amp_a=1.1, amp_b=0.5, n_cycles=2.0,
+                                                  t0=0.0, delay=2.5), pr.time, pr.grid)
+    j_an, _ = oracle_j(analytic)
+    assert 2e-8 < rel_l2(j, j_an) < 1e-3
 
 
 def test_piflux3d_magnus4_matches_oracle(hhg):

@@ -212,3 +212,22 @@ def test_plan_rejects_unsupported_combinations(hhg, kind, relax, proposal, metho
     with pytest.raises(hhg.HHGError) as ei:
         hhg.Plan([job], [grid], hhg.Solver(method=method, proposal=proposal))
     assert ei.value.status == hhg.ERROR_UNSUPPORTED
+
+
+def test_spline_pulse_matches_oracle(hhg):
+    """The product's clamped spline (second-derivative form) equals the
+    oracle's (slope form) coefficient for coefficient, and its host evaluation
+    equals the oracle's A(t), including the cut-off beyond t_cut."""
+    rng = np.random.default_rng(8)
+    for n in (2, 5, 1000):
+        y = rng.normal(size=n)
+        d0, d1 = rng.normal(size=2)
+        c = hhg.spline_coefficients(y, 0.05, d0, d1)
+        co = O.spline_coefficients(y, 0.05, d0, d1)
+        assert np.allclose(c, co, rtol=0, atol=1e-12 * max(1.0, np.abs(co).max()))
+    y = np.sin(np.linspace(0.0, 6.0, 300)) ** 2
+    laser = hhg.SplineLaser(y, 0.3, 0.02, 0.1, -0.2, t_cut=5.0)
+    p = O.spline_pulse(y, 0.3, 0.02, 0.1, -0.2, t_cut=5.0)
+    t = np.linspace(0.0, 7.0, 2001)
+    assert np.max(np.abs(laser.laser_function(t) - O.pulse_A(p, t))) < 1e-14
+    assert np.all(laser.laser_function([0.29, 5.01, 6.5]) == 0.0)

@@ -755,3 +755,45 @@ def test_honeycomb_equilibrium_relaxation_fixed_point_and_closed_form():
     s = O.trajectory(mt, pc, O.time_grid(0.7, 1.7, 16, 50), O.solver(), kx, ky)[0]
     A0 = 0.5 * math.sin(0.7)
     assert np.allclose(s[0], O.initial_state(mt, kx - A0, ky), rtol=0, atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# Splined pulses (Laser.hpp:8-11, 32-37; DoubleLaser.hpp:48-73)
+# --------------------------------------------------------------------------
+
+def test_spline_is_the_clamped_cubic_interpolant():
+    """The oracle's slope-form spline equals scipy's clamped CubicSpline (an
+    independent construction of the unique C2 interpolant with given end
+    slopes) at and between the nodes, and honours the end slopes."""
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(3)
+    for n, d0, d1 in ((2, 0.0, 0.0), (3, 0.4, -1.0), (17, 0.0, 0.0), (200, 1.5, 0.25)):
+        dt = 0.37
+        y = rng.normal(size=n)
+        p = O.spline_pulse(y, 1.0, dt, d0, d1)
+        cs = CubicSpline(1.0 + dt * np.arange(n), y, bc_type=((1, d0), (1, d1)))
+        t = np.sort(np.concatenate([1.0 + dt * np.arange(n), rng.uniform(1.0, 1.0 + dt * (n - 1), 300)]))
+        assert np.allclose(O.pulse_A(p, t), cs(t), rtol=0, atol=1e-12 * max(1.0, np.abs(y).max()))
+        c = O.spline_coefficients(y, dt, d0, d1)
+        assert abs(c[0, 1] / dt - d0) < 1e-13 and abs((c[-1, 1] + 2 * c[-1, 2] + 3 * c[-1, 3]) / dt - d1) < 1e-12
+    # zero outside [t0, t_cut]
+    p = O.spline_pulse(np.ones(5), 0.0, 1.0, t_cut=2.5)
+    assert np.array_equal(O.pulse_A(p, [-0.1, 2.6, 4.0]), [0.0, 0.0, 0.0]) and O.pulse_A(p, 2.4)[0] != 0.0
+
+
+def test_spline_of_a_sampled_pulse_converges_at_fourth_order():
+    """A clamped spline of a smooth pulse whose slope vanishes at both ends (the
+    reference's DoubleLaser<GaussLaser> table with end slopes 0) converges at
+    h^4.  (A cos^2 envelope's corners, where A'' jumps, limit that pulse's table
+    to lower order near the corners.)"""
+    T = 1.5 * 4 * math.pi
+    p = O.pulse(O.PULSE_DOUBLE_GAUSS, amp_a=1.1, amp_b=0.5, sigma=1.6, t0=0.5 * T - 1.0, delay=2.5)
+    t = np.linspace(0.2, T - 0.2, 3001)
+    ref = O.pulse_A(p, t)
+    errs = []
+    for n in (250, 500, 1000):
+        dt = T / (n - 1)
+        sp = O.spline_pulse(O.pulse_A(p, np.arange(n) * dt), 0.0, dt)
+        errs.append(np.max(np.abs(O.pulse_A(sp, t) - ref)))
+    assert errs[-1] < 1e-5
+    assert errs[0] / errs[1] > 12 and errs[1] / errs[2] > 12
