@@ -121,7 +121,8 @@ EXPORTED_SYMBOLS = [
     "hhg_accumulators_to_double", "hhg_plan_set_step_log", "hhg_plan_get_step_log", "hhg_mlp_param_count",
     "hhg_mlp_create", "hhg_mlp_destroy", "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params",
     "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
-    "hhg_spline_coefficients",
+    "hhg_spline_coefficients", "hhg_reference_double_laser", "hhg_reference_experimental_laser_size",
+    "hhg_reference_experimental_laser",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -157,6 +158,14 @@ def lib() -> ctypes.CDLL:
     L.hhg_pulse_evaluate.argtypes = [ctypes.POINTER(PulseDesc), _DP, _DP, ctypes.c_int64]
     L.hhg_spline_coefficients.argtypes = [_DP, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           _DP]
+    L.hhg_reference_double_laser.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int32, _DP,
+                                             ctypes.POINTER(PulseDesc), _DP]
+    L.hhg_reference_experimental_laser_size.restype = ctypes.c_int64
+    L.hhg_reference_experimental_laser_size.argtypes = [ctypes.c_double]
+    L.hhg_reference_experimental_laser.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int32, _DP, _DP, ctypes.c_int32, _DP,
+                                                   ctypes.c_int64, ctypes.POINTER(PulseDesc), _DP]
     L.hhg_trapezoidal_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), ctypes.c_int32, _DP, _DP, _DP]
     L.hhg_welch_spectrum.argtypes = [_DP, ctypes.POINTER(TimeDesc), _DP, _DP, _DP]
     L.hhg_accumulators_add.restype = None
@@ -322,6 +331,44 @@ class SplineLaser(Laser):
         d.slope_begin, d.slope_end = self.slope_begin, self.slope_end
         d.t_cut = self.t_cut if self.t_cut is not None else self.t0
         return d
+
+
+ACTIVE_BOTH, ACTIVE_A, ACTIVE_B = 0, 1, 2
+
+
+def _spline_from_desc(samples: np.ndarray, d: PulseDesc, amp: float) -> SplineLaser:
+    laser = SplineLaser(samples, d.sample_t0, d.sample_dt, d.slope_begin, d.slope_end, t_cut=d.t_cut)
+    laser.momentum_amplitude = amp
+    return laser
+
+
+def reference_double_laser(gauss: bool, photon_energy: float, E0: float, model_ratio: float, n_cycles: float,
+                           delay_ps: float, active: int = ACTIVE_BOTH) -> SplineLaser:
+    """The reference's DoubleLaser<CosineLaser | GaussLaser> (DoubleLaser.hpp:8-74) in its
+    units: photon energy in units of the experiment's 6.0914 meV, E0 in MV/cm, delay in ps."""
+    samples = np.empty(1000)
+    d, amp = PulseDesc(), ctypes.c_double()
+    _check(lib().hhg_reference_double_laser(int(gauss), photon_energy, E0, model_ratio, n_cycles, delay_ps, active,
+                                            _dp(samples), ctypes.byref(d), ctypes.byref(amp)))
+    return _spline_from_desc(samples, d, amp.value)
+
+
+def reference_experimental_laser(photon_energy: float, E0: float, model_ratio: float, shift_ps: float, E_A, E_B,
+                                 active: int = ACTIVE_BOTH) -> SplineLaser:
+    """The reference's ExperimentalLaser (ExperimentalLaser.cpp:8-257) from measured fields
+    E_A, E_B [MV/cm] of its experiment layout (201 samples 0.03335555 ps apart)."""
+    ea = np.ascontiguousarray(E_A, dtype=np.float64)
+    eb = np.ascontiguousarray(E_B, dtype=np.float64)
+    if len(ea) != len(eb):
+        raise ValueError("E_A and E_B differ in length")
+    n = lib().hhg_reference_experimental_laser_size(photon_energy)
+    if n < 2:
+        raise ValueError("photon_energy must be > 0")
+    samples = np.empty(n)
+    d, amp = PulseDesc(), ctypes.c_double()
+    _check(lib().hhg_reference_experimental_laser(photon_energy, E0, model_ratio, shift_ps, active, _dp(ea), _dp(eb),
+                                                  len(ea), _dp(samples), n, ctypes.byref(d), ctypes.byref(amp)))
+    return _spline_from_desc(samples, d, amp.value)
 
 
 def spline_coefficients(samples, dt: float, slope_begin: float = 0.0, slope_end: float = 0.0) -> np.ndarray:

@@ -153,6 +153,66 @@ struct DoubleLaser : Laser {
     }
 };
 
+// A sampled pulse through its clamped cubic spline (HHG_PULSE_SPLINE): the
+// tables the reference builds with Boost's cardinal_cubic_b_spline
+// (Laser.hpp:8-11, 32-37), A = 0 beyond t_end.
+struct SplineLaser : Laser {
+    std::vector<h_float> samples;
+    h_float sample_t0{}, sample_dt{}, slope_begin{}, slope_end{};
+    SplineLaser() = default;
+    SplineLaser(std::vector<h_float> samples_, h_float t0, h_float dt, h_float t_cut, h_float amplitude = 0.0)
+        : samples(std::move(samples_)), sample_t0(t0), sample_dt(dt) {
+        momentum_amplitude = amplitude;
+        t_begin = t0;
+        t_end = t_cut;
+    }
+    hhg_pulse_desc desc() const override {
+        hhg_pulse_desc d{};
+        d.kind = HHG_PULSE_SPLINE;
+        d.samples = samples.data();
+        d.n_samples = (int64_t)samples.size();
+        d.sample_t0 = sample_t0;
+        d.sample_dt = sample_dt;
+        d.slope_begin = slope_begin;
+        d.slope_end = slope_end;
+        d.t_cut = t_end;
+        return d;
+    }
+};
+
+enum class Active { Both = HHG_ACTIVE_BOTH, A = HHG_ACTIVE_A, B = HHG_ACTIVE_B };
+
+// The experiment's photon energy in meV (ExperimentParams.hpp:12): the tabulated
+// lasers take their photon energy in units of it.
+constexpr h_float exp_photon_energy = 6.091420442472572;
+
+// DoubleLaser<CosineLaser | GaussLaser> in the reference's units (DoubleLaser.hpp:26-35)
+inline SplineLaser reference_double_laser(bool gauss, h_float photon_energy, h_float E_0, h_float model_ratio,
+                                          h_float n_cycles, h_float delay_ps, Active active = Active::Both) {
+    std::vector<h_float> samples(1000);
+    hhg_pulse_desc d{};
+    h_float amp = 0.0;
+    check(hhg_reference_double_laser(gauss ? 1 : 0, photon_energy, E_0, model_ratio, n_cycles, delay_ps,
+                                     (int32_t)active, samples.data(), &d, &amp));
+    return SplineLaser(std::move(samples), d.sample_t0, d.sample_dt, d.t_cut, amp);
+}
+
+// ExperimentalLaser in the reference's units (ExperimentalLaser.cpp:8-26) from the
+// measured fields E_A, E_B [MV/cm] (201 samples each)
+inline SplineLaser reference_experimental_laser(h_float photon_energy, h_float E_0, h_float model_ratio,
+                                                h_float shift_ps, const std::vector<h_float>& E_A,
+                                                const std::vector<h_float>& E_B, Active active = Active::Both) {
+    if (E_A.size() != E_B.size()) throw std::invalid_argument("E_A and E_B differ in length");
+    const int64_t n = hhg_reference_experimental_laser_size(photon_energy);
+    if (n < 2) throw std::invalid_argument("photon_energy must be > 0");
+    std::vector<h_float> samples((size_t)n);
+    hhg_pulse_desc d{};
+    h_float amp = 0.0;
+    check(hhg_reference_experimental_laser(photon_energy, E_0, model_ratio, shift_ps, (int32_t)active, E_A.data(),
+                                           E_B.data(), (int32_t)E_A.size(), samples.data(), n, &d, &amp));
+    return SplineLaser(std::move(samples), d.sample_t0, d.sample_dt, d.t_cut, amp);
+}
+
 }  // namespace Laser
 
 namespace Dispatch {

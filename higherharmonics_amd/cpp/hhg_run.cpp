@@ -27,6 +27,11 @@
 //   Honeycomb: RKF78, the hexagonal-BZ grid with N_k positive nodes per axis
 //   (64), relaxation toward the instantaneous equilibrium, cos^2 + 2 pi
 //   (HoneycombDispatcher.cpp:31-35).
+//   Tabulated lasers (cubic-spline pulses): dcos, dcosA, dcosB, dgauss, dgaussA,
+//   dgaussB (PiFlux; B delayed by t0_offset ps) and exp, expA, expB (all three;
+//   experiment_data names a file of the measured fields, two columns E_A E_B in
+//   MV/cm, 201 rows; B shifted by t0_offset ps); their photon_energy is in
+//   units of the experiment's 6.0914 meV (Dispatcher.cpp:10-19).
 //   The output carries the reference's fields (hhg.cpp:301-326); continuous
 //   lasers use the Welch-window spectrum.
 //
@@ -136,18 +141,32 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
 
     const int N_k = (int)num(kv, "N_k", 64);
     const bool dirac = system == "Dirac", honeycomb = system == "Honeycomb";
-    Systems::DiracSystem dsys(T, E_F, v_F, band_width, photon_energy, tau, n_z, n_kappa);
-    Systems::PiFlux psys(T, E_F, v_F, band_width, photon_energy, tau, tau_o, z_range, n_coarse);
-    Systems::Honeycomb hsys(T, E_F, v_F, band_width, photon_energy, tau, N_k);
+    // the tabulated lasers take the photon energy in units of the experiment's
+    // (Dispatcher.cpp:10-19); the lattices' systems see the scaled energy, the
+    // Dirac system the raw one with the scaled energy only in its laser ratio
+    // (DiracDispatcher.cpp:38-40)
+    const bool is_exp = laser_type == "exp" || laser_type == "expA" || laser_type == "expB";
+    const bool is_double = laser_type.rfind("dcos", 0) == 0 || laser_type.rfind("dgauss", 0) == 0;
+    const bool scaled = is_exp || is_double;
+    const double e_sys = (scaled && !dirac) ? photon_energy * Laser::exp_photon_energy : photon_energy;
+    Systems::DiracSystem dsys(T, E_F, v_F, band_width, e_sys, tau, n_z, n_kappa);
+    Systems::PiFlux psys(T, E_F, v_F, band_width, e_sys, tau, tau_o, z_range, n_coarse);
+    Systems::Honeycomb hsys(T, E_F, v_F, band_width, e_sys, tau, N_k);
     const Systems::System& sys = dirac ? static_cast<const Systems::System&>(dsys)
                                  : honeycomb ? static_cast<const Systems::System&>(hsys)
                                              : psys;
-    const double ratio = dirac       ? dsys.laser_model_ratio(photon_energy)
+    const double ratio = dirac ? dsys.laser_model_ratio(scaled ? photon_energy * Laser::exp_photon_energy : photon_energy)
                          : honeycomb ? hsys.laser_model_ratio()
                                      : psys.laser_model_ratio();
     const double A0 = Laser::momentum_amplitude(photon_energy, E0, ratio);
+    double laser_photon_energy = photon_energy;
     std::unique_ptr<Laser::Laser> laser;
     TimeIntegrationConfig tc;
+    auto active_of = [&](const std::string& base) {
+        if (laser_type == base + "A") return Laser::Active::A;
+        if (laser_type == base + "B") return Laser::Active::B;
+        return Laser::Active::Both;
+    };
     if (laser_type == "continuous") {
         laser = std::make_unique<Laser::ContinuousLaser>(A0);
         tc = {-n_cycles * pi, n_cycles * pi, N, 50};
@@ -155,6 +174,32 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         laser = std::make_unique<Laser::CosineLaser>(A0, (double)n_cycles, pi * t0_offset);
         // the lattice continues one cycle after the pulse so the relaxation sets in
         tc = {laser->t_begin, laser->t_end + (dirac ? 0.0 : 2.0 * pi), N, 50};
+    } else if (is_double && !dirac && !honeycomb) {
+        // DoubleLaser<CosineLaser | GaussLaser> with the B delay t0_offset in ps (PiFluxDispatcher.cpp:74-104)
+        const bool gauss = laser_type.rfind("dgauss", 0) == 0;
+        laser = std::make_unique<Laser::SplineLaser>(Laser::reference_double_laser(
+            gauss, photon_energy, E0, ratio, (double)n_cycles, t0_offset, active_of(gauss ? "dgauss" : "dcos")));
+        laser_photon_energy = photon_energy * Laser::exp_photon_energy;
+        tc = {laser->t_begin, laser->t_end, N, 50};
+    } else if (is_exp) {
+        // ExperimentalLaser from the measured fields in experiment_data (two columns E_A E_B
+        // in MV/cm, 201 rows), B shifted by t0_offset ps (DiracDispatcher.cpp:38-51)
+        const std::string path = str(kv, "experiment_data", "");
+        std::ifstream in(path);
+        if (!in) throw std::invalid_argument("laser_type " + laser_type + " needs experiment_data <file>");
+        std::vector<double> ea, eb;
+        std::string line;
+        while (std::getline(in, line)) {
+            const auto hash = line.find('#');
+            if (hash != std::string::npos) line.resize(hash);
+            std::istringstream ls(line);
+            double a, b;
+            if (ls >> a >> b) ea.push_back(a), eb.push_back(b);
+        }
+        laser = std::make_unique<Laser::SplineLaser>(
+            Laser::reference_experimental_laser(photon_energy, E0, ratio, t0_offset, ea, eb, active_of("exp")));
+        laser_photon_energy = photon_energy * Laser::exp_photon_energy;
+        tc = {laser->t_begin, laser->t_end, N, 50};
     } else {
         throw std::invalid_argument("Laser type '" + laser_type + "' is not recognized!");
     }
@@ -175,7 +220,7 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     write_array(o, "current_density_frequency_imag", sp.imag);
     o << "  \"T\": " << num_str(T) << ", \"E_F\": " << num_str(E_F) << ", \"v_F\": " << num_str(v_F)
       << ", \"band_width\": " << num_str(band_width) << ", \"field_amplitude\": " << num_str(E0)
-      << ", \"photon_energy\": " << num_str(photon_energy) << ", \"laser_type\": \"" << laser_type
+      << ", \"photon_energy\": " << num_str(laser_photon_energy) << ", \"laser_type\": \"" << laser_type
       << "\", \"diagonal_relaxation_time\": " << num_str(tau)
       << ", \"offdiagonal_relaxation_time\": " << num_str(tau_o) << ",\n";
     write_array(o, "frequencies", sp.frequencies);
@@ -187,7 +232,7 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     else
         o << ", \"z_range\": " << z_range << ", \"n_coarse\": " << n_coarse;
     o << ", \"t0_offset\": " << num_str(t0_offset) << ",\n";
-    o << "  \"momentum_amplitude\": " << num_str(A0) << ", \"trajectories\": " << stats.trajectories
+    o << "  \"momentum_amplitude\": " << num_str(laser->momentum_amplitude) << ", \"trajectories\": " << stats.trajectories
       << ", \"attempted_steps\": " << stats.attempts << ", \"solve_ms\": " << num_str(stats.solve_ms) << "\n}\n";
     emit(o.str(), argc, argv);
     return 0;

@@ -288,6 +288,144 @@ double SplinePulse::operator()(double t) const {
     return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
 }
 
+// ---------------------------------------------------------------------------
+// The reference's tabulated lasers in its own units.  The experiment's layout
+// and units are data of the reference (ExperimentParams.hpp:5-13) and are kept
+// as it defines them.
+// ---------------------------------------------------------------------------
+namespace ref_exp {
+constexpr double exp_dt = 0.03335555;  // ps between measurements
+constexpr int N_extra = 16, N_experiment = 201, begin_zero = 8, zero_buff = 8;
+constexpr double laser_end = 6.67111 + (N_extra + begin_zero + zero_buff) * exp_dt;  // ps
+constexpr double exp_photon_energy = 6.091420442472572;                                // meV
+constexpr double unified_t_max = 1.5 * laser_end;                                      // ps
+constexpr double hbar = 6.582119569509065698e-13;                                      // meV s
+}  // namespace ref_exp
+
+static double add_active(int active, double a, double b) {
+    return active == HHG_ACTIVE_BOTH ? a + b : (active == HHG_ACTIVE_A ? a : b);
+}
+
+static hhg_pulse_desc spline_desc(const double* samples, int64_t n, double dt, double t_cut) {
+    hhg_pulse_desc d{};
+    d.kind = HHG_PULSE_SPLINE;
+    d.samples = samples;
+    d.n_samples = n;
+    d.sample_t0 = 0.0;
+    d.sample_dt = dt;
+    d.t_cut = t_cut;
+    return d;
+}
+
+// DoubleLaser<CosineLaser | GaussLaser> (DoubleLaser.hpp:8-74): A = base pulse of
+// field 1.6 E0, B = 0.7 E0 at t - shift, 1000 samples over [0, 1.5 duration]
+hhg_status reference_double_laser(int gauss, double photon_energy, double E0, double ratio, double n_cycles,
+                                  double delay_ps, int active, double* samples, hhg_pulse_desc* out, double* amp) {
+    using namespace ref_exp;
+    if (!(photon_energy > 0.0) || !(n_cycles > 0.0) || active < 0 || active > 2 || !samples || !out)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "double laser needs photon_energy > 0, n_cycles > 0, a known "
+                                                "active pulse and output buffers");
+    constexpr double pi = 3.14159265358979323846;
+    const double e_ph = exp_photon_energy * photon_energy;
+    const double duration = gauss ? 2.0 * pi * 3.0 * n_cycles : 2.0 * pi * n_cycles;  // base_duration
+    const double t_max = 1.5 * duration;
+    constexpr int N = 1000;
+    const double dt = t_max / (N - 1);
+    const double shift = photon_energy * (delay_ps * exp_photon_energy / (1e12 * hbar));
+    hhg_pulse_desc a{};
+    a.kind = gauss ? HHG_PULSE_GAUSS : HHG_PULSE_COS2;
+    a.amp_a = ratio * 1e11 * (E0 * 1.6) / e_ph;  // Laser.cpp:24-25
+    if (gauss) {
+        a.sigma = pi * n_cycles / std::sqrt(2.0 * std::log(1.0 / 0.05));  // GaussLaser.cpp:5-7
+        a.t0 = 0.5 * duration;
+    } else {
+        a.n_cycles = n_cycles;
+    }
+    hhg_pulse_desc b = a;
+    b.amp_a = ratio * 1e11 * (E0 * 0.7) / e_ph;
+    for (int i = 0; i < N; ++i) {
+        const double t = i * dt;
+        samples[i] = add_active(active, pulse_eval(a, t), pulse_eval(b, t - shift));
+    }
+    *out = spline_desc(samples, N, dt, t_max);
+    if (amp) *amp = ratio * 1e11 * E0 / e_ph;
+    return HHG_SUCCESS;
+}
+
+// ExperimentalLaser::vector_potential (ExperimentalLaser.cpp:28-58): A = -sum E dt
+// after begin_zero zeros, a quartic tail that returns A smoothly to 0 over
+// N_extra - 1 steps, zero_buff + 1 zeros
+static std::vector<double> experiment_vector_potential(const double* E, double dt) {
+    using namespace ref_exp;
+    std::vector<double> ret(N_experiment + N_extra + zero_buff + begin_zero, 0.0);
+    for (int i = 0; i < N_experiment; ++i) ret[i + begin_zero] = ret[i + begin_zero - 1] - E[i] * dt;
+    const double last = ret[N_experiment + begin_zero - 1];
+    const double prime = -E[N_experiment - 1];
+    const double primeprime = 0.5 * (E[N_experiment - 2] - E[N_experiment - 1]) / dt;
+    const double end = (N_extra - 1) * dt;
+    const double third =
+        -4.0 * (primeprime / (2.0 * end) + (3.0 * prime) / (4.0 * end * end) + last / (end * end * end));
+    const double fourth = -0.25 * (3.0 * third / end + 2.0 * primeprime / (end * end) + prime / (end * end * end));
+    for (int i = 1; i < N_extra; ++i) {
+        const double t = i * dt;
+        ret[N_experiment + begin_zero - 1 + i] =
+            last + prime * t + primeprime * t * t + third * t * t * t + fourth * t * t * t * t;
+    }
+    for (size_t i = ret.size() - begin_zero - 1; i < ret.size(); ++i) ret[i] = 0.0;
+    return ret;
+}
+
+int64_t reference_experimental_size(double photon_energy) {
+    using namespace ref_exp;
+    if (!(photon_energy > 0.0)) return -1;
+    const double e_ph = photon_energy * exp_photon_energy;
+    const double t_end = unified_t_max * (e_ph / (1e12 * hbar));
+    const double dt = e_ph * (exp_dt / (1e12 * hbar));
+    return (int64_t)(t_end / dt) + 2;  // N_temp + 1 with N_temp = t_end / dt + 1
+}
+
+hhg_status reference_experimental_laser(double photon_energy, double E0, double ratio, double shift_ps, int active,
+                                        const double* E_A, const double* E_B, int n_measured, double* samples,
+                                        int64_t n_samples, hhg_pulse_desc* out, double* amp) {
+    using namespace ref_exp;
+    if (!(photon_energy > 0.0) || active < 0 || active > 2 || !E_A || !E_B || !samples || !out)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "experimental laser needs photon_energy > 0, a known active "
+                                                "pulse, both fields and output buffers");
+    if (n_measured != N_experiment)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "experimental laser needs the experiment's 201 field samples");
+    if (n_samples != reference_experimental_size(photon_energy))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "experimental laser: n_samples must be "
+                                                "hhg_reference_experimental_laser_size(photon_energy)");
+    const double e_ph = photon_energy * exp_photon_energy;
+    const double to_unitless = e_ph / (1e12 * hbar);
+    const double t_end = unified_t_max * to_unitless;
+    const double dt = e_ph * (exp_dt / (1e12 * hbar));
+    const double unitless_laser_end = e_ph * (laser_end / (1e12 * hbar));
+    const double shift = shift_ps * to_unitless;
+    const std::vector<double> va = experiment_vector_potential(E_A, exp_dt);
+    const std::vector<double> vb = experiment_vector_potential(E_B, exp_dt);
+    const SplinePulse sa(spline_desc(va.data(), (int64_t)va.size(), dt, 0.0));
+    const SplinePulse sb(spline_desc(vb.data(), (int64_t)vb.size(), dt, 0.0));
+    if (sa.status != HHG_SUCCESS) return sa.status;
+    if (sb.status != HHG_SUCCESS) return sb.status;
+    const double scale = (0.1 / hbar) * E0 * ratio;  // (MV ps / cm) -> Peierls phase, ExperimentalLaser.cpp:220-225
+    for (int64_t i = 0; i < n_samples; ++i) {
+        const double t = 0.0 + dt * (double)i;
+        double a, b;
+        if (shift >= 0.0) {
+            a = t <= unitless_laser_end ? sa(t) : 0.0;
+            b = (t - shift >= 0.0 && t - shift <= unitless_laser_end) ? sb(t - shift) : 0.0;
+        } else {
+            a = (t + shift >= 0.0 && t + shift <= unitless_laser_end) ? sa(t + shift) : 0.0;
+            b = t <= unitless_laser_end ? sb(t) : 0.0;
+        }
+        samples[i] = scale * add_active(active, a, b);
+    }
+    *out = spline_desc(samples, n_samples, dt, t_end);
+    if (amp) *amp = ratio * 1e11 * E0 / e_ph;
+    return HHG_SUCCESS;
+}
+
 double pulse_eval(const hhg_pulse_desc& p, double t) {
     switch (p.kind) {
     case HHG_PULSE_COS2: return p.amp_a * env_cos2_h(p, t) * std::sin(t + p.cep);
@@ -460,6 +598,27 @@ hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* 
     }
     for (int64_t i = 0; i < n; ++i) A[i] = hhg::pulse_eval(*p, t[i]);
     return HHG_SUCCESS;
+}
+
+hhg_status hhg_reference_double_laser(int32_t gauss, double photon_energy, double E0, double model_ratio,
+                                      double n_cycles, double delay_ps, int32_t active, double* samples,
+                                      hhg_pulse_desc* out, double* momentum_amplitude) {
+    hhg::clear_error();
+    return hhg::reference_double_laser(gauss, photon_energy, E0, model_ratio, n_cycles, delay_ps, active, samples, out,
+                                       momentum_amplitude);
+}
+
+int64_t hhg_reference_experimental_laser_size(double photon_energy) {
+    return hhg::reference_experimental_size(photon_energy);
+}
+
+hhg_status hhg_reference_experimental_laser(double photon_energy, double E0, double model_ratio, double shift_ps,
+                                            int32_t active, const double* E_A, const double* E_B, int32_t n_measured,
+                                            double* samples, int64_t n_samples, hhg_pulse_desc* out,
+                                            double* momentum_amplitude) {
+    hhg::clear_error();
+    return hhg::reference_experimental_laser(photon_energy, E0, model_ratio, shift_ps, active, E_A, E_B, n_measured,
+                                             samples, n_samples, out, momentum_amplitude);
 }
 
 hhg_status hhg_spline_coefficients(const double* samples, int64_t n, double dt, double slope_begin, double slope_end,

@@ -297,6 +297,34 @@ hhg_status hhg_kgrid_build(const hhg_kgrid_desc* g, double* kx, double* ky, doub
 hhg_status hhg_spline_coefficients(const double* samples, int64_t n, double dt, double slope_begin, double slope_end,
                                    double* coef);
 
+/* ---- the reference's tabulated lasers, in its units ---- */
+typedef enum hhg_active_pulse {
+    HHG_ACTIVE_BOTH = 0, /* Active::Both */
+    HHG_ACTIVE_A = 1,    /* Active::A */
+    HHG_ACTIVE_B = 2     /* Active::B */
+} hhg_active_pulse;
+
+/* DoubleLaser<CosineLaser> (gauss = 0) or DoubleLaser<GaussLaser> (gauss = 1), DoubleLaser.hpp:8-74:
+ * photon_energy in units of the experiment's 6.0914 meV (ExperimentParams.hpp:12), E0 in MV/cm,
+ * model_ratio the system's laser_model_ratio, delay of pulse B in ps; pulses A and B of field 1.6 E0 and
+ * 0.7 E0; samples[1000] over [0, 1.5 x the base duration] become a HHG_PULSE_SPLINE pulse in *out
+ * (out->samples points to samples).  *momentum_amplitude (may be null) receives the laser's own
+ * amplitude (Laser.cpp:33-38). */
+hhg_status hhg_reference_double_laser(int32_t gauss, double photon_energy, double E0, double model_ratio,
+                                      double n_cycles, double delay_ps, int32_t active, double* samples,
+                                      hhg_pulse_desc* out, double* momentum_amplitude);
+
+/* ExperimentalLaser (ExperimentalLaser.cpp:8-257) from measured fields E_A, E_B [MV/cm] of the
+ * reference's experiment layout (201 samples 0.03335555 ps apart, ExperimentParams.hpp:5-11): the
+ * vector potential of each field with its polynomial tail, splined, shifted by shift_ps (B later for
+ * shift >= 0), scaled by 0.1 E0 model_ratio / hbar and tabulated on the spline's own step.  Returns the
+ * number of samples for photon_energy (units of 6.0914 meV), or fills samples[n] and *out. */
+int64_t hhg_reference_experimental_laser_size(double photon_energy);
+hhg_status hhg_reference_experimental_laser(double photon_energy, double E0, double model_ratio, double shift_ps,
+                                            int32_t active, const double* E_A, const double* E_B, int32_t n_measured,
+                                            double* samples, int64_t n_samples, hhg_pulse_desc* out,
+                                            double* momentum_amplitude);
+
 /* A(t) at n times (host evaluation, Laser::laser_function). */
 hhg_status hhg_pulse_evaluate(const hhg_pulse_desc* p, const double* t, double* A, int64_t n);
 

@@ -233,6 +233,80 @@ def magnus_moments_step(kz: float, kappa: float, mom, h: float, s) -> np.ndarray
 
 
 # --------------------------------------------------------------------------
+# The reference's tabulated lasers (ExperimentParams.hpp, DoubleLaser.hpp,
+# ExperimentalLaser.cpp) in its units
+# --------------------------------------------------------------------------
+
+REF_HBAR = 6.582119569509065698e-13   # meV s, GlobalDefinitions.hpp:29
+EXP_DT = 0.03335555                   # ps, ExperimentParams.hpp:5
+EXP_N_EXTRA, EXP_N, EXP_BEGIN_ZERO, EXP_ZERO_BUFF = 16, 201, 8, 8   # ExperimentParams.hpp:6-9
+EXP_LASER_END = 6.67111 + (EXP_N_EXTRA + EXP_BEGIN_ZERO + EXP_ZERO_BUFF) * EXP_DT   # ps, :11
+EXP_PHOTON_ENERGY = 6.091420442472572  # meV, :12
+
+
+def reference_double_laser_samples(gauss, photon_energy, E0, model_ratio, n_cycles, delay_ps, active=0):
+    """(samples, dt, t_max) of DoubleLaser<Cosine|Gauss>Laser (DoubleLaser.hpp:26-73):
+    A of field 1.6 E0 and B of 0.7 E0 at t - shift, 1000 samples on [0, 1.5 x duration]."""
+    e_ph = EXP_PHOTON_ENERGY * photon_energy
+    duration = 2 * np.pi * (3.0 if gauss else 1.0) * n_cycles
+    t_max = 1.5 * duration
+    dt = t_max / 999
+    shift = photon_energy * delay_ps * EXP_PHOTON_ENERGY / (1e12 * REF_HBAR)
+    amp = lambda f: model_ratio * 1e11 * f / e_ph
+    if gauss:
+        sig = np.pi * n_cycles / np.sqrt(2 * np.log(20.0))
+        pa = pulse(PULSE_GAUSS, amp_a=amp(1.6 * E0), sigma=sig, t0=0.5 * duration)
+        pb = pulse(PULSE_GAUSS, amp_a=amp(0.7 * E0), sigma=sig, t0=0.5 * duration)
+    else:
+        pa = pulse(PULSE_COS2, amp_a=amp(1.6 * E0), n_cycles=n_cycles, t0=0.0)
+        pb = pulse(PULSE_COS2, amp_a=amp(0.7 * E0), n_cycles=n_cycles, t0=0.0)
+    t = np.arange(1000) * dt
+    A, B = pulse_A(pa, t), pulse_A(pb, t - shift)
+    return (A + B if active == 0 else (A if active == 1 else B)), dt, t_max
+
+
+def experiment_vector_potential(E, dt):
+    """ExperimentalLaser's vector_potential (ExperimentalLaser.cpp:28-58): eight zeros,
+    A_i = -sum_{j<=i} E_j dt, then over N_extra - 1 steps the quartic with the last
+    value, slope -E_last and half-curvature (E_{N-2} - E_{N-1})/(2 dt) that reaches 0
+    with zero slope (solved here as a 5x5 system), then zeros."""
+    E = np.asarray(E, dtype=np.float64)
+    A = np.concatenate([np.zeros(EXP_BEGIN_ZERO), -np.cumsum(E) * dt])
+    last, slope, half_curv = A[-1], -E[-1], 0.5 * (E[-2] - E[-1]) / dt
+    e = (EXP_N_EXTRA - 1) * dt
+    M = np.array([[1, 0, 0, 0, 0], [0, 1, 0, 0, 0], [0, 0, 1, 0, 0], [1, e, e ** 2, e ** 3, e ** 4],
+                  [0, 1, 2 * e, 3 * e ** 2, 4 * e ** 3]], dtype=np.float64)
+    c = np.linalg.solve(M, [last, slope, half_curv, 0.0, 0.0])
+    t = np.arange(1, EXP_N_EXTRA - 1) * dt
+    tail = c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4])))
+    return np.concatenate([A, tail, np.zeros(EXP_ZERO_BUFF + 2)])
+
+
+def reference_experimental_samples(photon_energy, E0, model_ratio, shift_ps, E_A, E_B, active=0):
+    """(samples, dt, t_end) of ExperimentalLaser::compute_spline (ExperimentalLaser.cpp:60-257)."""
+    e_ph = photon_energy * EXP_PHOTON_ENERGY
+    to_u = e_ph / (1e12 * REF_HBAR)
+    t_end = 1.5 * EXP_LASER_END * to_u
+    dt = e_ph * (EXP_DT / (1e12 * REF_HBAR))
+    end_u = e_ph * (EXP_LASER_END / (1e12 * REF_HBAR))
+    shift = shift_ps * to_u
+    sa = spline_pulse(experiment_vector_potential(E_A, EXP_DT), 0.0, dt)
+    sb = spline_pulse(experiment_vector_potential(E_B, EXP_DT), 0.0, dt)
+    n = int(t_end / dt) + 2
+    t = dt * np.arange(n)
+    if shift >= 0:
+        a = np.where(t <= end_u, pulse_A(sa, t), 0.0)
+        tb = t - shift
+        b = np.where((tb >= 0) & (tb <= end_u), pulse_A(sb, tb), 0.0)
+    else:
+        ta = t + shift
+        a = np.where((ta >= 0) & (ta <= end_u), pulse_A(sa, ta), 0.0)
+        b = np.where(t <= end_u, pulse_A(sb, t), 0.0)
+    v = a + b if active == 0 else (a if active == 1 else b)
+    return (0.1 / REF_HBAR) * E0 * model_ratio * v, dt, t_end
+
+
+# --------------------------------------------------------------------------
 # Quadrature grids (restating the reference's GL mappings)
 # --------------------------------------------------------------------------
 

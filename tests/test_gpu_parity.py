@@ -492,3 +492,83 @@ def test_cpp_driver_reference_units_honeycomb(hhg, tmp_path):
     err = rel_l2(np.array(d["current_density_time"]), j_ref)
     print(f"  rel_l2={err:.3e}", flush=True)
     assert err <= J_TOL
+
+
+def _run_driver(hhg, tmp_path, text, name):
+    import gzip
+    import json
+    import os
+    import subprocess
+    cfg = tmp_path / f"{name}.cfg"
+    cfg.write_text(text)
+    out = tmp_path / f"{name}.json.gz"
+    exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
+    print(f"[case] hhg_run {name}", flush=True)
+    r = subprocess.run([exe, str(cfg), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return json.loads(gzip.open(out).read())
+
+
+@pytest.mark.parametrize("laser", ["dcos", "dgaussB"])
+def test_cpp_driver_piflux_double_laser(hhg, tmp_path, laser):
+    """hhg_run PiFlux with the reference's tabulated DoubleLaser (photon energy in
+    units of the experiment's, B delayed in ps): the system sees the scaled photon
+    energy, the table is the oracle's, and j(t) equals the oracle's run."""
+    import math
+    d = _run_driver(hhg, tmp_path, f"system_type PiFlux\nT 0\nE_F 0\nv_F 1.5e3\nband_width 10\nfield_amplitude 1.5\n"
+                    f"photon_energy 0.9\ndiagonal_relaxation_time -1\nlaser_type {laser}\nn_laser_cycles 1\n"
+                    "t0_offset 0.2\nz_range 3\nn_coarse 1\nn_measurements 512\n", laser)
+    hbar = O.REF_HBAR
+    e_ph = 0.9 * O.EXP_PHOTON_ENERGY
+    ratio = math.sqrt(3) * hbar * 1.5e3 / (e_ph * 10.0)
+    gauss = laser.startswith("dgauss")
+    active = 2 if laser.endswith("B") else (1 if laser.endswith("A") else 0)
+    y, dt, t_max = O.reference_double_laser_samples(gauss, 0.9, 1.5, ratio, 1.0, 0.2, active)
+    assert abs(d["photon_energy"] - e_ph) < 1e-12 and d["t_begin"] == 0.0 and abs(d["t_end"] - t_max) < 1e-12
+    kx, ky, kz, w = O.piflux_continuum(3, 1)
+    m = O.model(O.MODEL_PIFLUX3D, hopping=10.0 / math.sqrt(12), ic_rescue=1)
+    j_ref, _ = O.current_density(m, O.spline_pulse(y, 0.0, dt, t_cut=t_max), O.time_grid(0.0, t_max, 512, 50),
+                                 O.solver(rtol=1e-8, atol=1e-12), kx, ky, w, kz=kz)
+    err = rel_l2(np.array(d["current_density_time"]), j_ref)
+    print(f"  rel_l2={err:.3e}", flush=True)
+    assert err <= J_TOL
+
+
+@pytest.mark.parametrize("system", ["Honeycomb", "Dirac"])
+def test_cpp_driver_experimental_laser(hhg, tmp_path, system):
+    """hhg_run with laser_type exp and the measured fields from a file (synthetic
+    stand-ins of the reference experiment's shape): the ExperimentalLaser table
+    and j(t) against the oracle (Dirac: moment Magnus on the raw photon energy
+    with the scaled one in the laser ratio, DiracDispatcher.cpp:38-40)."""
+    import math
+    from tests.test_host_abi import _synthetic_fields
+    ea, eb = _synthetic_fields()
+    data = tmp_path / "fields.txt"
+    data.write_text("# E_A E_B [MV/cm]\n" + "".join(f"{a:.17g} {b:.17g}\n" for a, b in zip(ea, eb)))
+    extra = "N_k 6\n" if system == "Honeycomb" else "n_z 8\nn_kappa 4\n"
+    d = _run_driver(hhg, tmp_path, f"system_type {system}\nT 0\nE_F 0\nv_F 1.5e3\nband_width 12\nfield_amplitude 1.0\n"
+                    "photon_energy 0.9\ndiagonal_relaxation_time -1\nlaser_type exp\nn_laser_cycles 1\n"
+                    f"t0_offset 0.4\nn_measurements 512\nexperiment_data {data}\n" + extra, "exp_" + system)
+    hbar = O.REF_HBAR
+    e_ph = 0.9 * O.EXP_PHOTON_ENERGY
+    if system == "Honeycomb":
+        ratio = 4 * hbar * 1.5e3 / (e_ph * 12.0)
+    else:
+        ratio = hbar * 1.5e3 / e_ph
+    y, dt, t_end = O.reference_experimental_samples(0.9, 1.0, ratio, 0.4, ea, eb)
+    assert abs(d["t_end"] / t_end - 1) < 1e-15
+    p = O.spline_pulse(y, 0.0, dt, t_cut=t_end)
+    tc = O.time_grid(0.0, t_end, 512, 50)
+    if system == "Honeycomb":
+        kx, ky, w = O.gl_hex(12, 12)
+        j_ref, _ = O.current_density(O.model(O.MODEL_HONEYCOMB, hopping=2.0), p, tc, O.solver(rtol=1e-8, atol=1e-12),
+                                     kx, ky, w)
+        tol = J_TOL
+    else:
+        kx, ky, w = O.dirac_cylinder(8, 4, 12.0)
+        j_ref, _ = O.current_density(O.model(O.MODEL_DIRAC3D), p, tc, O.solver(method=O.METHOD_MAGNUS_MOMENTS),
+                                     kx, ky, w)
+        tol = 1e-12
+    err = rel_l2(np.array(d["current_density_time"]), j_ref)
+    print(f"  rel_l2={err:.3e}", flush=True)
+    assert err <= tol

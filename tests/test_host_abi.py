@@ -231,3 +231,50 @@ def test_spline_pulse_matches_oracle(hhg):
     t = np.linspace(0.0, 7.0, 2001)
     assert np.max(np.abs(laser.laser_function(t) - O.pulse_A(p, t))) < 1e-14
     assert np.all(laser.laser_function([0.29, 5.01, 6.5]) == 0.0)
+
+
+def _synthetic_fields():
+    """Measured-field stand-ins of the reference experiment's shape (201 samples,
+    MV/cm): two chirped, Gaussian-modulated carriers."""
+    i = np.arange(201)
+    ea = 1.3 * np.exp(-((i - 90) / 22.0) ** 2) * np.sin(0.33 * i + 0.002 * i ** 2)
+    eb = 0.6 * np.exp(-((i - 110) / 30.0) ** 2) * np.sin(0.3 * i + 1.0)
+    return ea, eb
+
+
+@pytest.mark.parametrize("gauss", [0, 1], ids=["cos2", "gauss"])
+@pytest.mark.parametrize("active", [0, 1, 2], ids=["both", "A", "B"])
+def test_reference_double_laser_table_matches_oracle(hhg, gauss, active):
+    """DoubleLaser<Cosine|Gauss>Laser's 1000-sample table in the reference's units."""
+    laser = hhg.reference_double_laser(gauss, 1.1, 0.8, 0.05, 2.0, 0.35, active)
+    y, dt, t_max = O.reference_double_laser_samples(gauss, 1.1, 0.8, 0.05, 2.0, 0.35, active)
+    assert len(laser.samples) == 1000 and abs(laser.dt - dt) < 1e-15 * dt and laser.t_end == t_max
+    assert np.allclose(laser.samples, y, rtol=0, atol=1e-13 * np.abs(y).max())
+    assert abs(laser.momentum_amplitude - 0.05 * 1e11 * 0.8 / (O.EXP_PHOTON_ENERGY * 1.1)) < 1e-12
+    t = np.linspace(0.0, 1.1 * t_max, 777)
+    ref = O.pulse_A(O.spline_pulse(y, 0.0, dt, t_cut=t_max), t)
+    assert np.allclose(laser.laser_function(t), ref, rtol=0, atol=1e-12 * np.abs(y).max())
+
+
+@pytest.mark.parametrize("shift", [0.0, 1.7, -0.9])
+@pytest.mark.parametrize("active", [0, 1, 2], ids=["both", "A", "B"])
+def test_reference_experimental_laser_table_matches_oracle(hhg, shift, active):
+    """ExperimentalLaser's table from measured fields (synthetic stand-ins of the
+    same shape): vector potential, quartic tail, spline, shift and scale."""
+    ea, eb = _synthetic_fields()
+    laser = hhg.reference_experimental_laser(0.9, 1.3, 0.02, shift, ea, eb, active)
+    y, dt, t_end = O.reference_experimental_samples(0.9, 1.3, 0.02, shift, ea, eb, active)
+    assert len(laser.samples) == len(y) and abs(laser.dt / dt - 1) < 1e-15 and abs(laser.t_end / t_end - 1) < 1e-15
+    assert np.allclose(laser.samples, y, rtol=0, atol=1e-12 * np.abs(y).max())
+    # the tail returns A to zero with zero slope, so the table ends in zeros
+    assert np.all(laser.samples[-3:] == 0.0)
+
+
+def test_experiment_vector_potential_is_the_field_integral():
+    """-dA/dt = E at the measured samples (first differences of the oracle's
+    vector potential), and the quartic tail meets 0 with zero slope."""
+    ea, _ = _synthetic_fields()
+    A = O.experiment_vector_potential(ea, O.EXP_DT)
+    assert len(A) == 201 + 16 + 8 + 8
+    assert np.allclose(-(A[8:209] - A[7:208]) / O.EXP_DT, ea, rtol=0, atol=1e-12)
+    assert np.all(A[:8] == 0.0) and np.all(A[223:] == 0.0)
