@@ -284,6 +284,111 @@ def config5_leg(H, n=256, log_n=32):
     return res
 
 
+# The reference's own systems in its units, run through the reference-units
+# driver (hhg_run) as a drop-in for its dispatchers.  Parameters follow the
+# reference's params/params.config (T 300 K, E_F 118 meV, v_F 1.5e6 m/s,
+# 5.25 meV, 1.6 MV/cm, band width 300, tau 15 fs) with 4 laser cycles
+# (N = 4096), and params/dcos.config for the tabulated A/B pulse.
+REF_COMMON = ("T 300\nE_F 118\nv_F 1.5e6\nband_width 300\nfield_amplitude 1.6\nphoton_energy 5.25\n"
+              "offdiagonal_relaxation_time -1\nn_laser_cycles 4\nt0_offset 0\n")
+REF_LEGS = [
+    ("dirac3d_continuous_moment_magnus",
+     "system_type Dirac\ndiagonal_relaxation_time -1\nlaser_type continuous\n" + REF_COMMON,
+     "n_z 402\nn_kappa 256\n", "n_z 12\nn_kappa 6\n"),
+    ("piflux3d_cos2_equilibrium_relaxation",
+     "system_type PiFlux\ndiagonal_relaxation_time 15\nlaser_type cosine\n" + REF_COMMON,
+     "z_range 32\nn_coarse 4\n", "z_range 2\nn_coarse 1\n"),
+    ("honeycomb_cos2_equilibrium_relaxation",
+     "system_type Honeycomb\ndiagonal_relaxation_time 15\nlaser_type cosine\n" + REF_COMMON,
+     "N_k 256\n", "N_k 6\n"),
+    ("piflux3d_dcos_spline_pulse",
+     "system_type PiFlux\nT 300\nE_F 118\nv_F 1e6\nband_width 200\nfield_amplitude 1\nphoton_energy 1\n"
+     "diagonal_relaxation_time 10\noffdiagonal_relaxation_time -1\nlaser_type dcos\nn_laser_cycles 2\n"
+     "t0_offset 0\n", "z_range 32\nn_coarse 4\n", "z_range 2\nn_coarse 1\n"),
+]
+# Algorithmic FLOPs of the reference systems' right-hand sides (same convention):
+# pi-flux 3D 4t m x s with cos(z - A): 15; its equilibrium relaxation (|m|, two
+# Fermi factors, band-frame phase, targets, 3 x 7 update): 100 more; honeycomb
+# RHS 34 plus its equilibrium state at k - A and the 3 x 3 update: 25 more.
+# One moment-Magnus step of the 3D Dirac cone (generator polynomials, Rodrigues
+# rotation): 175 per k-point and interval.
+F_RHS_REF = {"piflux3d": 15.0, "piflux3d_eq": 115.0, "honeycomb_eq": 59.0}
+F_MOMENT_STEP = 175.0
+
+
+def reference_models_leg(H, repeats=3, cpu_s=5.0):
+    """Each reference system through hhg_run: trajectories/s from the kernel
+    time of `repeats` executions of one plan (the first not counted), an audit
+    of the same physics on a small grid against the oracle, and the oracle on
+    a seeded sample of the full grid on the host cores."""
+    import subprocess
+    import tempfile
+    import oracle as O
+    from oracle import reference_units as R
+    exe = os.path.join(os.path.dirname(H.LIB_PATH), "hhg_run")
+    threads = max(1, min(16, os.cpu_count() or 1))
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        def run(text, name):
+            cfg = os.path.join(tmp, name + ".cfg")
+            out = os.path.join(tmp, name + ".json")
+            with open(cfg, "w") as fh:
+                fh.write(text)
+            r = subprocess.run([exe, cfg, out], capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(r.stderr)
+            with open(out) as fh:
+                return json.load(fh)
+
+        for name, base, full, small in REF_LEGS:
+            print(f"[bench] reference model {name}", file=sys.stderr, flush=True)
+            d = run(base + full + f"timing_repeats {repeats}\n", name)
+            runs = d["solve_ms_runs"][1:] or d["solve_ms_runs"]
+            ms = float(np.mean(runs))
+            n_traj = d["trajectories"]
+            cfg = R.parse(base + full)
+            st = R.setup(cfg)
+            N = st["time"].n_measurements
+            if cfg["system_type"] == "Dirac":
+                flops = n_traj * N * F_MOMENT_STEP
+            else:
+                key = ("piflux3d_eq" if float(cfg["diagonal_relaxation_time"]) > 0 else "piflux3d") \
+                    if cfg["system_type"] == "PiFlux" else "honeycomb_eq"
+                flops = d["attempted_steps"] * (F_STEP + 13 * F_RHS_REF[key])
+            # audit: the same configuration on a small grid, driver vs oracle
+            a = run(base + small, name + "_audit")
+            sa = R.setup(R.parse(base + small))
+            j_ref, _ = O.current_density(sa["model"], sa["pulse"], sa["time"], sa["solver"], sa["kx"], sa["ky"],
+                                         sa["w"], kz=sa["kz"], n_threads=threads)
+            j_fast, _ = O.current_density(sa["model"], sa["pulse"], sa["time"], sa["solver"], sa["kx"], sa["ky"],
+                                          sa["w"], kz=sa["kz"], n_threads=threads, fast=True)
+            audit = rel_l2(a["current_density_time"], j_ref)
+            # the rounding sensitivity of the configuration itself: the oracle's
+            # strict build against its -ffast-math build
+            spread = rel_l2(j_fast, j_ref)
+            # CPU: the optimised oracle on a seeded sample of the full grid
+            rng = np.random.default_rng(20261203)
+            pilot = rng.choice(len(st["kx"]), size=min(len(st["kx"]), threads * 64), replace=False)
+            t0 = time.perf_counter()
+            O.current_density(st["model"], st["pulse"], st["time"], st["solver"], st["kx"][pilot], st["ky"][pilot],
+                              st["w"][pilot], kz=st["kz"][pilot], n_threads=threads, fast=True)
+            per = (time.perf_counter() - t0) / len(pilot)
+            n = int(min(len(st["kx"]), max(len(pilot), cpu_s / max(per, 1e-9))))
+            sel = rng.choice(len(st["kx"]), size=n, replace=False)
+            t0 = time.perf_counter()
+            O.current_density(st["model"], st["pulse"], st["time"], st["solver"], st["kx"][sel], st["ky"][sel],
+                              st["w"][sel], kz=st["kz"][sel], n_threads=threads, fast=True)
+            cpu = n / (time.perf_counter() - t0)
+            res[name] = {"trajectories": n_traj, "samples": N + 1, "kernel_ms": ms,
+                         "trajectories_per_s": n_traj / (ms * 1e-3), "attempts": d["attempted_steps"],
+                         "rejects": d.get("rejects"), "fp64_tflops": flops / (ms * 1e-3) / 1e12,
+                         "audit_rel_l2_vs_oracle": audit, "audit_trajectories": a["trajectories"],
+                         "audit_oracle_strict_vs_fast": spread, "audit_bar": max(1e-9, 10 * spread),
+                         "cpu_trajectories_per_s": cpu, "cpu_sample": n, "cpu_threads": threads,
+                         "gpu_over_cpu": n_traj / (ms * 1e-3) / cpu}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +409,8 @@ def main():
                     help="also run the honeycomb learned-vs-plain configuration (16 pulses on 768 x 768)")
     ap.add_argument("--config5", action="store_true",
                     help="also run the mixed 128-pulse sweep, plain vs learned + grouping, with a 1%% audit")
+    ap.add_argument("--reference-models", action="store_true",
+                    help="also run the reference's 3D Dirac, 3D pi-flux and honeycomb systems in its units")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -443,6 +550,8 @@ def main():
         out["config4_honeycomb_learned"] = config4_leg(H)
     if args.config5:
         out["config5_mixed_learned"] = config5_leg(H)
+    if args.reference_models:
+        out["reference_models"] = reference_models_leg(H)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out))

@@ -20,6 +20,7 @@
 // std::runtime_error otherwise).  Every solve runs on the GPU.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -304,6 +305,28 @@ class System {
         job.kgrid = 0;
         Dispatch::Plan plan({kgrid}, {job}, solver, rank, n_ranks);
         return plan.compute(stats)[0];
+    }
+
+    // the same, executing one plan `repeats` times (timing): the solve time of
+    // every execution in *solve_ms, the statistics of the last in *stats
+    std::vector<h_float> compute_current_density_repeated(const Laser::Laser* laser,
+                                                          const TimeIntegrationConfig& time_config, int repeats,
+                                                          std::vector<double>* solve_ms,
+                                                          hhg_stats* stats = nullptr) const {
+        hhg_job_desc job{};
+        job.model = model;
+        job.pulse = laser->desc();
+        job.time = time_config.desc();
+        job.kgrid = 0;
+        Dispatch::Plan plan({kgrid}, {job}, solver);
+        std::vector<h_float> j;
+        for (int r = 0; r < std::max(1, repeats); ++r) {
+            hhg_stats st{};
+            j = plan.compute(&st)[0];
+            if (solve_ms) solve_ms->push_back(st.solve_ms);
+            if (stats) *stats = st;
+        }
+        return j;
     }
 
     virtual ~System() = default;

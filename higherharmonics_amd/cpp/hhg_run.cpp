@@ -204,7 +204,9 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         throw std::invalid_argument("Laser type '" + laser_type + "' is not recognized!");
     }
     hhg_stats stats{};
-    const std::vector<double> j = sys.compute_current_density(laser.get(), tc, 0, 1, &stats);
+    std::vector<double> runs_ms;
+    const int repeats = (int)num(kv, "timing_repeats", 1);  // > 1: execute the plan that often (timing)
+    const std::vector<double> j = sys.compute_current_density_repeated(laser.get(), tc, repeats, &runs_ms, &stats);
     const bool continuous = laser_type == "continuous";
     const Fourier::Spectrum sp = continuous ? Fourier::welch_spectrum(j, tc) : Fourier::trapezoidal_spectrum(j, tc);
     std::vector<double> A(N + (continuous ? 0 : 1));
@@ -233,7 +235,10 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         o << ", \"z_range\": " << z_range << ", \"n_coarse\": " << n_coarse;
     o << ", \"t0_offset\": " << num_str(t0_offset) << ",\n";
     o << "  \"momentum_amplitude\": " << num_str(laser->momentum_amplitude) << ", \"trajectories\": " << stats.trajectories
-      << ", \"attempted_steps\": " << stats.attempts << ", \"solve_ms\": " << num_str(stats.solve_ms) << "\n}\n";
+      << ", \"attempted_steps\": " << stats.attempts << ", \"rejects\": " << stats.rejects
+      << ", \"solve_ms\": " << num_str(stats.solve_ms) << ",\n";
+    write_array(o, "solve_ms_runs", runs_ms, true);
+    o << "}\n";
     emit(o.str(), argc, argv);
     return 0;
 }
