@@ -285,26 +285,15 @@ def config5_leg(H, n=256, log_n=32):
 
 
 # The reference's own systems in its units, run through the reference-units
-# driver (hhg_run) as a drop-in for its dispatchers.  Parameters follow the
-# reference's params/params.config (T 300 K, E_F 118 meV, v_F 1.5e6 m/s,
-# 5.25 meV, 1.6 MV/cm, band width 300, tau 15 fs) with 4 laser cycles
-# (N = 4096), and params/dcos.config for the tabulated A/B pulse.
-REF_COMMON = ("T 300\nE_F 118\nv_F 1.5e6\nband_width 300\nfield_amplitude 1.6\nphoton_energy 5.25\n"
-              "offdiagonal_relaxation_time -1\nn_laser_cycles 4\nt0_offset 0\n")
+# driver (hhg_run) as a drop-in for its dispatchers: the configurations in
+# higherharmonics_amd/configs (the reference's params/params.config parameters
+# with 4 laser cycles, and its params/dcos.config), with the grid overrides of
+# the small audit runs.
 REF_LEGS = [
-    ("dirac3d_continuous_moment_magnus",
-     "system_type Dirac\ndiagonal_relaxation_time -1\nlaser_type continuous\n" + REF_COMMON,
-     "n_z 402\nn_kappa 256\n", "n_z 12\nn_kappa 6\n"),
-    ("piflux3d_cos2_equilibrium_relaxation",
-     "system_type PiFlux\ndiagonal_relaxation_time 15\nlaser_type cosine\n" + REF_COMMON,
-     "z_range 32\nn_coarse 4\n", "z_range 2\nn_coarse 1\n"),
-    ("honeycomb_cos2_equilibrium_relaxation",
-     "system_type Honeycomb\ndiagonal_relaxation_time 15\nlaser_type cosine\n" + REF_COMMON,
-     "N_k 256\n", "N_k 6\n"),
-    ("piflux3d_dcos_spline_pulse",
-     "system_type PiFlux\nT 300\nE_F 118\nv_F 1e6\nband_width 200\nfield_amplitude 1\nphoton_energy 1\n"
-     "diagonal_relaxation_time 10\noffdiagonal_relaxation_time -1\nlaser_type dcos\nn_laser_cycles 2\n"
-     "t0_offset 0\n", "z_range 32\nn_coarse 4\n", "z_range 2\nn_coarse 1\n"),
+    ("dirac3d_continuous_moment_magnus", "dirac_params.cfg", "n_z 12\nn_kappa 6\n"),
+    ("piflux3d_cos2_equilibrium_relaxation", "piflux_params.cfg", "z_range 2\nn_coarse 1\n"),
+    ("honeycomb_cos2_equilibrium_relaxation", "honeycomb_params.cfg", "N_k 6\n"),
+    ("piflux3d_dcos_spline_pulse", "piflux_dcos.cfg", "z_range 2\nn_coarse 1\n"),
 ]
 # Algorithmic FLOPs of the reference systems' right-hand sides (same convention):
 # pi-flux 3D 4t m x s with cos(z - A): 15; its equilibrium relaxation (|m|, two
@@ -340,13 +329,15 @@ def reference_models_leg(H, repeats=3, cpu_s=5.0):
             with open(out) as fh:
                 return json.load(fh)
 
-        for name, base, full, small in REF_LEGS:
+        for name, cfg_file, small in REF_LEGS:
             print(f"[bench] reference model {name}", file=sys.stderr, flush=True)
-            d = run(base + full + f"timing_repeats {repeats}\n", name)
+            with open(os.path.join(ROOT, "higherharmonics_amd", "configs", cfg_file)) as fh:
+                base = fh.read()
+            d = run(base + f"timing_repeats {repeats}\n", name)
             runs = d["solve_ms_runs"][1:] or d["solve_ms_runs"]
             ms = float(np.mean(runs))
             n_traj = d["trajectories"]
-            cfg = R.parse(base + full)
+            cfg = R.parse(base)
             st = R.setup(cfg)
             N = st["time"].n_measurements
             if cfg["system_type"] == "Dirac":
