@@ -18,9 +18,20 @@
 
 namespace hhg {
 
-#ifndef HHG_WAVES_PER_EU
-#define HHG_WAVES_PER_EU 2
+// Waves per SIMD each solve kernel is compiled for (measured on MI355X): the
+// Magnus kernels fit 4 without spills (config 2's Magnus-4 leg 16.8 -> 21.1 M
+// trajectories/s going from 2 to 4); the RKF78 kernels need 168-256 VGPRs and
+// run at 2, except the learned 2D Dirac kernel, whose inference latency
+// hides better at 3 despite 58 spilled VGPRs (111.6 -> 106.8 ms per solve).
+// HHG_WAVES_PER_EU overrides all of them (occupancy experiments).
+template <int METHOD, int M, bool LEARNED>
+struct SolveOccupancy {
+#ifdef HHG_WAVES_PER_EU
+    static constexpr int waves = HHG_WAVES_PER_EU;
+#else
+    static constexpr int waves = METHOD != METHOD_RKF78 ? 4 : (LEARNED && M == MODEL_DIRAC2D ? 3 : 2);
 #endif
+};
 
 namespace solve_detail {
 
@@ -291,7 +302,8 @@ __device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs&
 // makes each wave solve one item, a grid of the resident wave count is the
 // persistent schedule.
 template <int METHOD, int M, int P, int R, bool LEARNED>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(HHG_WAVES_PER_EU, HHG_WAVES_PER_EU))) void
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SolveOccupancy<METHOD, M, LEARNED>::waves,
+                                                                      SolveOccupancy<METHOD, M, LEARNED>::waves))) void
 solve_kernel(KernelArgs a) {
     using namespace solve_detail;
     const int lane = threadIdx.x & (WAVE - 1);
