@@ -35,7 +35,7 @@ SUCCESS, ERROR_INVALID_ARGUMENT, ERROR_HIP, ERROR_TOLERANCE, ERROR_OUT_OF_MEMORY
 MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D, MODEL_PIFLUX3D = 0, 1, 2, 3, 4
 RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2, RELAX_EQUILIBRIUM = 0, 1, 2, 3
 PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS, PULSE_SPLINE = 0, 1, 2, 3, 4, 5
-KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_CONTINUUM = 0, 1, 2, 3, 4
+KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_CONTINUUM, KGRID_PIFLUX_UNIFORM = 0, 1, 2, 3, 4, 5
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
 
@@ -54,7 +54,7 @@ class ModelDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
                 ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
                 ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double),
-                ("relax_time_offdiag", ctypes.c_double)]
+                ("relax_time_offdiag", ctypes.c_double), ("split_dispersion", ctypes.c_double)]
 
 
 class PulseDesc(ctypes.Structure):
@@ -407,6 +407,11 @@ class KGrid:
     def piflux_continuum(z_range=32, n_coarse=4) -> "KGrid":
         """(x, y, z) points of the 3D pi-flux lattice (kind KGRID_PIFLUX_CONTINUUM)."""
         return KGrid(KGRID_PIFLUX_CONTINUUM, z_range, n_coarse)
+
+    @staticmethod
+    def piflux_uniform(n: int) -> "KGrid":
+        """(x, y, +-z) pi/n of the 3D pi-flux split current, weight 1/n^3 (kind KGRID_PIFLUX_UNIFORM)."""
+        return KGrid(KGRID_PIFLUX_UNIFORM, n)
 
     @staticmethod
     def explicit(kx, ky, w, kz=None) -> "KGrid":

@@ -184,17 +184,27 @@ struct Bloch {
     double s0[3];
     double cy;  // theta = 0 constant: PiFlux 2t cos ky, honeycomb cos(sqrt3/2 ky); 3D Dirac |k|; 3D pi-flux cos ky
     double ik;  // 3D Dirac: 1/|k|
-    double cx;  // 3D pi-flux: cos kx
+    double cx;  // 3D pi-flux: cos kx (the reference's rescued value where it applies)
     double eq[3];  // equilibrium relaxation: the state at the start of the step
+    bool rescued;  // 3D pi-flux, uniform grid: the reference's rescue applies to this point
 
     __device__ __forceinline__ Bloch(JobRef job, double kx_, double ky_, double kz_ = 0.0)
         : J(job), kx(kx_), ky(ky_), kz(kz_) {
         ik = 0.0;
         cx = 0.0;
+        rescued = false;
         if constexpr (M == MODEL_PIFLUX2D) cy = 2.0 * J.hop * cos(ky);
         else if constexpr (M == MODEL_HONEYCOMB) cy = cos(kHalfSqrt3 * ky);
         else if constexpr (M == MODEL_DIRAC3D) cy = sqrt(kx * kx + ky * ky), ik = cy > 0.0 ? 1.0 / cy : 0.0;
-        else if constexpr (M == MODEL_PIFLUX3D) cx = cos(kx), cy = cos(ky);
+        else if constexpr (M == MODEL_PIFLUX3D) {
+            cx = cos(kx), cy = cos(ky);
+            // uniform grid (PiFlux.cpp:978-986): where cos x and cos y vanish and
+            // cos z < 0 the reference replaces cos x by RESCUE |cos z| for the run
+            if (J.ic_rescue == 2) {
+                const double c = cos(kz);
+                if (ref_is_zero(cx) && ref_is_zero(cy) && c < 0.0) cx = fabs(c) * kRescue, rescued = true;
+            }
+        }
         else cy = 0.0;
     }
 
@@ -273,7 +283,7 @@ struct Bloch {
             // cos kz < 0 by 1 - RESCUE^2/2 (PiFlux.cpp:825-829; the -z node of
             // PiFlux.cpp:845-849 keeps the last x/y cosines and is not rescued).
             double cz = cos(kz);
-            if (J.ic_rescue && kz > 0.0 && cz < 0.0) cz *= (1.0 - 0.5 * kRescue * kRescue);
+            if ((J.ic_rescue == 1 && kz > 0.0 && cz < 0.0) || rescued) cz *= (1.0 - 0.5 * kRescue * kRescue);
             const double disp = sqrt(cx * cx + cy * cy + cz * cz);
             const double a2 = fermi(-J.fermi + 2.0 * J.hop * disp, J.beta);
             const double b2 = fermi(-J.fermi - 2.0 * J.hop * disp, J.beta);
@@ -395,6 +405,12 @@ struct Bloch {
             out[1] -= (s[1] - proj * d[1]) * r;
             out[2] -= (s[2] - proj * d[2]) * r;
         }
+    }
+
+    // 3D pi-flux: the instantaneous dispersion |m(k - A)| (PiFlux.cpp:400-402, 1001)
+    __device__ __forceinline__ double dispersion_at(double A) const {
+        const double c = cos(kz - A);
+        return sqrt(cx * cx + cy * cy + c * c);
     }
 
     // current vertex g(s, A) along the polarisation

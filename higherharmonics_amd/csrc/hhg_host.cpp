@@ -95,6 +95,10 @@ int64_t kgrid_size(const hhg_kgrid_desc& g) {
         const int64_t C = g.ny, F = 8 * C;
         return 2 * (int64_t)g.nx * (2 * F * F + 20 * C * C);
     }
+    if (g.kind == HHG_KGRID_PIFLUX_UNIFORM) {
+        if (g.nx < 1 || g.nx > 2048) return 0;
+        return 2 * (int64_t)g.nx * g.nx * g.nx;
+    }
     return (int64_t)g.nx * (int64_t)g.ny;
 }
 
@@ -148,6 +152,25 @@ hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* 
                             w[k] = wz[z] * (wx[i] * tw) * wy[j];
                         }
                 }
+            }
+        }
+        return HHG_SUCCESS;
+    }
+    if (g.kind == HHG_KGRID_PIFLUX_UNIFORM) {
+        // PiFlux::current_per_energy (PiFlux.cpp:954-1035): k = (x, y, z) pi/N for i = z N^2 + y N + x,
+        // each followed by its -z copy (k.invert()), weight 1/N^3 (the final division)
+        if (g.nx < 1 || g.nx > 2048) return fail(HHG_ERROR_INVALID_ARGUMENT, "uniform pi-flux grid needs 1 <= nx <= 2048");
+        if (!kz) return fail(HHG_ERROR_INVALID_ARGUMENT, "pi-flux grid needs the third coordinate");
+        const int64_t N = g.nx;
+        const double pi = 3.14159265358979323846, d = pi / (double)N, wt = 1.0 / (double)(N * N * N);
+        int64_t k = 0;
+        for (int64_t i = 0; i < N * N * N; ++i) {
+            const int64_t z = i / (N * N), y = (i / N) % N, x = i % N;
+            for (int sgn = 1; sgn >= -1; sgn -= 2, ++k) {
+                kx[k] = x * d;
+                ky[k] = y * d;
+                kz[k] = sgn * (d * z);
+                w[k] = wt;
             }
         }
         return HHG_SUCCESS;

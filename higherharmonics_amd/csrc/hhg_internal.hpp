@@ -75,7 +75,8 @@ struct JobDev {
     double t_begin, delta;
     int32_t n_meas, n_sub;
     int32_t magnus_sub;
-    int32_t ic_rescue;    // 3D pi-flux continuum grid: rescued cos k_z in the initial state
+    int32_t ic_rescue;    // 3D pi-flux rescue of the reference's drivers: 1 continuum grid, 2 uniform grid
+    double split_cut;     // 3D pi-flux: > 0 splits j at |m(k - A)| = split_cut into two outputs
     // k-grid slice and output placement
     int64_t k_base, n_k;
     int64_t row_base;     // first partial row double of this job

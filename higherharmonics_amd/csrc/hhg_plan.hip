@@ -141,6 +141,8 @@ hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n
                     at + "sampling interval must not exceed 0.25 (>= 26 samples per optical cycle)");
     if (sv.method != HHG_METHOD_RKF78 && m.relax != HHG_RELAX_NONE)
         return fail(HHG_ERROR_UNSUPPORTED, at + "the Magnus step is unitary: relaxation needs RKF78");
+    if (m.split_dispersion != 0.0 && (!piflux3d || sv.method != HHG_METHOD_RKF78 || !(m.split_dispersion > 0.0)))
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the split current is the 3D pi-flux lattice's with RKF78 and a cut > 0");
     if (sv.method == HHG_METHOD_MAGNUS_MOMENTS && m.kind != HHG_MODEL_DIRAC3D)
         return fail(HHG_ERROR_UNSUPPORTED, at + "the moment Magnus step is the 3D Dirac cone's (HHG_MODEL_DIRAC3D)");
     return HHG_SUCCESS;
@@ -197,6 +199,7 @@ hhg::JobDev make_job(const hhg_job_desc& jd, const hhg_solver_desc& sv) {
     J.beta = m.beta;
     J.relax_rate = m.relax != HHG_RELAX_NONE ? 1.0 / m.relax_time : 0.0;
     J.relax_rate2 = m.relax == HHG_RELAX_EQUILIBRIUM && m.relax_time_offdiag > 0.0 ? 1.0 / m.relax_time_offdiag : 0.0;
+    J.split_cut = m.split_dispersion;
     J.pulse = p.kind;
     J.theta_zero = m.theta == 0.0;
     J.amp_a = p.amp_a;
@@ -422,7 +425,10 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     for (int i = 0; i < d->n_jobs; ++i) {
         hhg::JobDev J = make_job(d->jobs[i], sv);
         const int g = d->jobs[i].kgrid;
-        J.ic_rescue = J.model == hhg::MODEL_PIFLUX3D && d->kgrids[g].kind == HHG_KGRID_PIFLUX_CONTINUUM;
+        if (J.model == hhg::MODEL_PIFLUX3D)
+            J.ic_rescue = d->kgrids[g].kind == HHG_KGRID_PIFLUX_CONTINUUM ? 1
+                          : d->kgrids[g].kind == HHG_KGRID_PIFLUX_UNIFORM  ? 2
+                                                                           : 0;
         const int64_t n_chunks = (grid_size[g] + hhg::WAVE - 1) / hhg::WAVE;
         const int64_t per_rank = (n_chunks + n_ranks - 1) / n_ranks;
         const int64_t c0 = std::min(n_chunks, per_rank * d->rank), c1 = std::min(n_chunks, c0 + per_rank);
@@ -430,7 +436,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         J.n_k = std::max<int64_t>(0, std::min(grid_size[g], c1 * hhg::WAVE) - c0 * hhg::WAVE);
         J.n_items = (int32_t)(c1 - c0);
         J.sample_base = p->offsets[i];
-        p->offsets[i + 1] = p->offsets[i] + J.n_meas + 1;
+        p->offsets[i + 1] = p->offsets[i] + (J.split_cut > 0.0 ? 2 : 1) * (int64_t)(J.n_meas + 1);
         J.magnus_base = magnus_words;
         if (sv.method == HHG_METHOD_MAGNUS4) magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
         if (sv.method == HHG_METHOD_MAGNUS_MOMENTS) magnus_words += 4 * (int64_t)J.n_meas;

@@ -19,17 +19,18 @@
 namespace hhg {
 
 // Waves per SIMD each solve kernel is compiled for (measured on MI355X): the
-// Magnus kernels fit 4 without spills (config 2's Magnus-4 leg 16.8 -> 21.1 M
-// trajectories/s going from 2 to 4); the RKF78 kernels need 168-256 VGPRs and
-// run at 2, except the learned 2D Dirac kernel, whose inference latency
-// hides better at 3 despite 58 spilled VGPRs (111.6 -> 106.8 ms per solve).
+// Magnus kernels fit 4 without spills (config 2's Magnus-4 leg 16.8 -> 22.6 M
+// trajectories/s going from 2 to 4); the RKF78 kernels need 200-256 VGPRs and
+// run at 2.  At 3 the learned 2D Dirac kernel ran 4-7 % faster but spilled 58
+// VGPRs to scratch in its interval loop (43 GB of scratch writes per solve);
+// at 4 the plain kernel gained 3-6 % and the carried controller lost 11 %.
 // HHG_WAVES_PER_EU overrides all of them (occupancy experiments).
 template <int METHOD, int M, bool LEARNED>
 struct SolveOccupancy {
 #ifdef HHG_WAVES_PER_EU
     static constexpr int waves = HHG_WAVES_PER_EU;
 #else
-    static constexpr int waves = METHOD != METHOD_RKF78 ? 4 : (LEARNED && M == MODEL_DIRAC2D ? 3 : 2);
+    static constexpr int waves = METHOD != METHOD_RKF78 ? 4 : 2;
 #endif
 };
 
@@ -138,7 +139,22 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const int N = J.n_meas;
     SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
     const double* At = a.At + J.sample_base;
-    sink.put(0, wave_sum(active ? w * f.vertex(At[0], s) : 0.0), lane);
+    // 3D pi-flux split current (PiFlux.cpp:998-1006): the samples of points
+    // with |m(k - A)| >= split_cut go to the job's second output
+    SampleSink sink_hi{a.acc + 2 * (J.sample_base + N + 1), N, 0.0, false};
+    const bool split = M == MODEL_PIFLUX3D && J.split_cut > 0.0;
+    auto emit = [&](int i, double g) {
+        if constexpr (M == MODEL_PIFLUX3D) {
+            if (split) {
+                const bool lo = f.dispersion_at(At[i]) < J.split_cut;
+                sink.put(i, wave_sum(lo ? g : 0.0), lane);
+                sink_hi.put(i, wave_sum(lo ? 0.0 : g), lane);
+                return;
+            }
+        }
+        sink.put(i, wave_sum(g), lane);
+    };
+    emit(0, active ? w * f.vertex(At[0], s) : 0.0);
 
     int status = active ? 0 : 3;
     const bool restart = a.sp.proposal == PROPOSAL_RESTART;
@@ -217,12 +233,11 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
             }
         }
         while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
-        const double g = (status != 3) ? w * f.vertex(At[i], s) : 0.0;
-        sink.put(i, wave_sum(g), lane);
+        emit(i, (status != 3) ? w * f.vertex(At[i], s) : 0.0);
     }
     if (status == 1) lc.fail++;
     if (status == 2) lc.nonfin++;
-    if (sink.overflow) lc.nonfin++;
+    if (sink.overflow || sink_hi.overflow) lc.nonfin++;
 }
 
 template <int M>

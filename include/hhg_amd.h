@@ -89,6 +89,9 @@ typedef struct hhg_model_desc {
     double beta;         /* 1/T in 1/(hbar omega_L); <= 0 means T = 0 */
     double relax_time;   /* tau or T2 in 1/omega_L */
     double relax_time_offdiag; /* HHG_RELAX_EQUILIBRIUM: off-diagonal relaxation time in 1/omega_L (<= 0: none) */
+    double split_dispersion;   /* HHG_MODEL_PIFLUX3D with RKF78, > 0: the job's output is two currents of N + 1
+                                  samples, first the points with |m(k - A(t_i))| below this value, then the rest
+                                  (PiFlux::current_per_energy, PiFlux.cpp:954-1035, cut sqrt(3)/2) */
 } hhg_model_desc;
 
 /* ---- pulses: A(t) = amplitude * envelope * sin(t + cep) ---- */
@@ -142,7 +145,7 @@ typedef enum hhg_kgrid_kind {
                                      dk = 2 x1 / nx (DiracSystem.cpp:280-306); kappa: ny-point Gauss-Legendre on
                                      (0, sqrt(x1^2 - k_z^2)) where the reference refines an adaptive trapezoid;
                                      weight dk w_kappa k_z kappa / |k| (DiracSystem.cpp:279); x1 = band width */
-    HHG_KGRID_PIFLUX_CONTINUUM = 4 /* (x, y, z) of the 3D pi-flux lattice, PiFlux::current_density_continuum_limit
+    HHG_KGRID_PIFLUX_CONTINUUM = 4, /* (x, y, z) of the 3D pi-flux lattice, PiFlux::current_density_continuum_limit
                                       (PiFlux.cpp:794-863) with improved_xy_integral (PiFlux.cpp:665-792): z = +-pi x_z
                                       at the nx positive nodes of the 2 nx-point Gauss-Legendre rule, four Gauss
                                       patches in (x, y) on [0, pi] x [0, pi/2] with ny = N_coarse (PiFlux.hpp:12-29,
@@ -150,6 +153,11 @@ typedef enum hhg_kgrid_kind {
                                       of the z map.  Jobs on this grid start, like the reference's continuum
                                       driver, from cos z scaled by 1 - 5e-9 at the +z nodes with cos z < 0
                                       (PiFlux.cpp:825-829) */
+    HHG_KGRID_PIFLUX_UNIFORM = 5   /* (x pi/N, y pi/N, +-z pi/N), x, y, z < N = nx, weight 1/N^3 each: the uniform
+                                      grid of PiFlux::current_per_energy (PiFlux.cpp:954-1035).  Where cos x and
+                                      cos y vanish (is_zero) and cos z < 0 the reference's rescue replaces cos x by
+                                      1e-4 |cos z| for the whole run and scales the initial cos z by 1 - 5e-9
+                                      (PiFlux.cpp:978-986); jobs on this grid do the same */
 } hhg_kgrid_kind;
 
 typedef struct hhg_kgrid_desc {

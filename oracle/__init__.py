@@ -34,7 +34,8 @@ class Model(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
                 ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
                 ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double),
-                ("relax_time_offdiag", ctypes.c_double), ("ic_rescue", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("relax_time_offdiag", ctypes.c_double), ("ic_rescue", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("split_dispersion", ctypes.c_double)]
 
 
 class Pulse(ctypes.Structure):
@@ -108,9 +109,9 @@ def _dp(a: np.ndarray):
 
 
 def model(kind=MODEL_DIRAC2D, v=1.0, mass=0.1, hopping=1.0, theta=0.0, fermi_energy=0.0, beta=0.0,
-          relax=RELAX_NONE, relax_time=0.0, relax_time_offdiag=0.0, ic_rescue=0) -> Model:
+          relax=RELAX_NONE, relax_time=0.0, relax_time_offdiag=0.0, ic_rescue=0, split_dispersion=0.0) -> Model:
     return Model(kind, relax, v, mass, hopping, theta, fermi_energy, beta, relax_time, relax_time_offdiag,
-                 ic_rescue, 0)
+                 ic_rescue, 0, split_dispersion)
 
 
 def pulse(kind=PULSE_COS2, amp_a=0.5, amp_b=0.0, n_cycles=8.0, sigma=0.0, t0=0.0, delay=0.0, cep=0.0) -> Pulse:
@@ -210,7 +211,8 @@ def current_density(m: Model, p: Pulse, tc: Time, sv: Solver, kx, ky, w, n_threa
     if kz is not None:
         kz = np.ascontiguousarray(kz, dtype=np.float64)
         kzp = _dp(kz)
-    j = np.zeros(tc.n_measurements + 1)
+    split = m.kind == MODEL_PIFLUX3D and m.split_dispersion > 0
+    j = np.zeros((2 if split else 1) * (tc.n_measurements + 1))
     st = Stats()
     r = lib(fast).or_current_density(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv),
                                      _dp(kx), _dp(ky), kzp, _dp(w), len(kx), _dp(j), ctypes.byref(st), n_threads)
@@ -369,6 +371,18 @@ def piflux_continuum(z_range: int = 32, n_coarse: int = 4):
             out[2].append(np.full(len(px), sgn * np.pi * gz[z]))
             out[3].append(wz[z] * pw)
     return tuple(np.concatenate(a) for a in out)
+
+
+def piflux_uniform(n: int):
+    """(kx, ky, kz, w) of PiFlux::current_per_energy's grid (PiFlux.cpp:954-1035):
+    (x, y, z) pi/n for i = z n^2 + y n + x, each followed by its -z copy, weight 1/n^3."""
+    i = np.arange(n ** 3)
+    z, y, x = i // (n * n), (i // n) % n, i % n
+    d = np.pi / n
+    kx = np.repeat(x * d, 2)
+    ky = np.repeat(y * d, 2)
+    kz = np.stack([d * z, -(d * z)], axis=1).ravel()
+    return kx, ky, kz, np.full(2 * n ** 3, 1.0 / n ** 3)
 
 
 def gl_box(nx: int, ny: int, x0: float, x1: float, y0: float, y1: float):

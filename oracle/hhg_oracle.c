@@ -281,6 +281,23 @@ static void model_d(const or_model* m, double qx, double qy, double d[3]) {
     }
 }
 
+/* is_zero of sources/HHG/GlobalDefinitions.hpp:61-67: biased exponent below
+ * 0b01111010011, i.e. |x| < 2^-44 */
+static int or_is_zero(double x) { return fabs(x) < 0x1p-44; }
+
+/* RESCUE_TRAFO of sources/HHG/Systems/PiFlux.cpp:50 */
+#define OR_RESCUE 1e-4
+
+/* cos k_x of a 3D pi-flux point: on the uniform grid of the split current the
+ * reference replaces it by RESCUE |cos k_z| where cos k_x and cos k_y vanish
+ * and cos k_z < 0, for the whole run (PiFlux.cpp:978-986). */
+static double piflux_cos_x(const or_model* m, double kx, double ky, double kz, int* rescued) {
+    const double cx = cos(kx), cz = cos(kz);
+    const int r = m->ic_rescue == 2 && or_is_zero(cx) && or_is_zero(cos(ky)) && cz < 0.0;
+    if (rescued) *rescued = r;
+    return r ? fabs(cz) * OR_RESCUE : cx;
+}
+
 /* The field vector d of the Bloch equation ds/dt = 2 d x s at vector
  * potential A.  2D models: d(k - A e_theta).  3D Dirac cone in the band frame
  * of the reference (sources/HHG/Systems/DiracSystem.cpp:143-153), the point
@@ -289,7 +306,7 @@ static void model_d(const or_model* m, double qx, double qy, double d[3]) {
 static void model_field(const or_model* m, double kx, double ky, double kz, double A, double d[3]) {
     if (m->kind == OR_MODEL_PIFLUX3D) {
         /* d = 2t m, m = (cos kx, cos ky, cos(kz - A)) (PiFlux.cpp:89-94, prefactor 4t = 2 * 2t) */
-        d[0] = 2.0 * m->hopping * cos(kx);
+        d[0] = 2.0 * m->hopping * piflux_cos_x(m, kx, ky, kz, NULL);
         d[1] = 2.0 * m->hopping * cos(ky);
         d[2] = 2.0 * m->hopping * cos(kz - A);
         return;
@@ -309,13 +326,6 @@ static double fermi(double e, double beta) {
     if (!(beta > 0.0)) return e != 0.0 ? (e < 0.0 ? 1.0 : 0.0) : 0.5;
     return 1.0 / (1.0 + exp(beta * e));
 }
-
-/* is_zero of sources/HHG/GlobalDefinitions.hpp:61-67: biased exponent below
- * 0b01111010011, i.e. |x| < 2^-44 */
-static int or_is_zero(double x) { return fabs(x) < 0x1p-44; }
-
-/* RESCUE_TRAFO of sources/HHG/Systems/PiFlux.cpp:50 */
-#define OR_RESCUE 1e-4
 
 /* Thermal state of the 3D pi-flux lattice at m = (cx, cy, cz): PiFlux::ic_sigma
  * (PiFlux.cpp:519-549) with alpha^2 = f(-E_F + 2t|m|), beta^2 = f(-E_F - 2t|m|)
@@ -388,9 +398,11 @@ void or_initial_state(const or_model* m, double kx, double ky, double kz, double
          * the x/y loops overwrite cos kx, so only the (1 - RESCUE^2/2) factor of
          * cos kz survives into ic_sigma.  The -z node reuses the last x/y values
          * (PiFlux.cpp:845-849) and is not rescued. */
+        int rescued = 0;
+        const double cx = piflux_cos_x(m, kx, ky, kz, &rescued);
         double cz = cos(kz);
-        if (m->ic_rescue && kz > 0.0 && cz < 0.0) cz *= (1.0 - 0.5 * OR_RESCUE * OR_RESCUE);
-        piflux3d_ic(m, cos(kx), cos(ky), cz, s);
+        if ((m->ic_rescue == 1 && kz > 0.0 && cz < 0.0) || rescued) cz *= (1.0 - 0.5 * OR_RESCUE * OR_RESCUE);
+        piflux3d_ic(m, cx, cos(ky), cz, s);
         return;
     }
     if (m->kind == OR_MODEL_DIRAC3D) {
@@ -427,7 +439,7 @@ void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, double k
     const double A = or_pulse_A(p, t);
     if (m->kind == OR_MODEL_PIFLUX3D) {
         /* PiFlux.cpp:89-94 / 169-180: 4t (m x s), then the relaxation */
-        const double cx = cos(kx), cy = cos(ky), cz = cos(kz - A), pf = 4.0 * m->hopping;
+        const double cx = piflux_cos_x(m, kx, ky, kz, NULL), cy = cos(ky), cz = cos(kz - A), pf = 4.0 * m->hopping;
         out[0] = (cy * s[2] - cz * s[1]) * pf;
         out[1] = (cz * s[0] - cx * s[2]) * pf;
         out[2] = (cx * s[1] - cy * s[0]) * pf;
@@ -882,11 +894,22 @@ typedef struct {
     const double* A;  /* A(t_i) */
     double kx, ky, kz, w;
     double* row;
+    int split_offset; /* 0: no split; else N + 1, the row offset of the part above the cut */
 } acc_user;
 
 static void emit_acc(void* user, int i, const double s[3]) {
     acc_user* u = (acc_user*)user;
-    u->row[i] += u->w * or_vertex(u->m, u->kx, u->ky, u->kz, u->A[i], s);
+    const double g = u->w * or_vertex(u->m, u->kx, u->ky, u->kz, u->A[i], s);
+    if (u->split_offset) {
+        /* PiFlux::current_per_energy (PiFlux.cpp:998-1006): the instantaneous
+         * dispersion |m(k - A)| against the cut */
+        const double cx = piflux_cos_x(u->m, u->kx, u->ky, u->kz, NULL), cy = cos(u->ky);
+        const double cz = cos(u->kz - u->A[i]);
+        const double disp = sqrt(cx * cx + cy * cy + cz * cz);
+        u->row[disp < u->m->split_dispersion ? i : u->split_offset + i] += g;
+        return;
+    }
+    u->row[i] += g;
 }
 
 #define OR_CHUNK 64
@@ -895,9 +918,11 @@ int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* 
                            const double* kx, const double* ky, const double* kz, const double* w, int64_t n_k,
                            double* j_out, or_stats* st, int n_threads) {
     const int N = tc->n_measurements;
+    const int split = m->kind == OR_MODEL_PIFLUX3D && m->split_dispersion > 0.0;
+    const int W = split ? 2 * (N + 1) : N + 1; /* output samples */
     const int64_t n_chunks = (n_k + OR_CHUNK - 1) / OR_CHUNK;
     double* A = (double*)malloc(sizeof(double) * (size_t)(N + 1));
-    double* partial = (double*)calloc((size_t)n_chunks * (size_t)(N + 1), sizeof(double));
+    double* partial = (double*)calloc((size_t)n_chunks * (size_t)W, sizeof(double));
     if (!A || !partial) {
         free(A);
         free(partial);
@@ -914,11 +939,11 @@ int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* 
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : failed, attempts, rejects, rhs, unsupported)
     for (int64_t c = 0; c < n_chunks; ++c) {
         or_stats ls = {0, 0, 0, 0};
-        double* row = partial + (size_t)c * (size_t)(N + 1);
+        double* row = partial + (size_t)c * (size_t)W;
         const int64_t k_end = (c + 1) * OR_CHUNK < n_k ? (c + 1) * OR_CHUNK : n_k;
         for (int64_t k = c * OR_CHUNK; k < k_end; ++k) {
             const double z = kz ? kz[k] : 0.0;
-            acc_user u = {m, A, kx[k], ky[k], z, w[k], row};
+            acc_user u = {m, A, kx[k], ky[k], z, w[k], row, split ? N + 1 : 0};
             const int r = evolve(m, p, tc, sv, kx[k], ky[k], z, &ls, emit_acc, &u);
             if (r > 0) failed++;
             if (r < 0) unsupported++;
@@ -928,10 +953,10 @@ int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* 
         rhs += ls.rhs_evals;
     }
     /* deterministic compensated (Neumaier) sum over chunks in index order */
-    for (int i = 0; i <= N; ++i) {
+    for (int i = 0; i < W; ++i) {
         double sum = 0.0, comp = 0.0;
         for (int64_t c = 0; c < n_chunks; ++c) {
-            const double x = partial[(size_t)c * (size_t)(N + 1) + i];
+            const double x = partial[(size_t)c * (size_t)W + i];
             const double t = sum + x;
             if (fabs(sum) >= fabs(x))
                 comp += (sum - t) + x;

@@ -42,8 +42,9 @@ typedef struct {
     double beta;       /* inverse temperature; <= 0 means T = 0 */
     double relax_time;        /* equilibrium relaxation: the diagonal time */
     double relax_time_offdiag; /* equilibrium relaxation: the off-diagonal time (<= 0: none) */
-    int32_t ic_rescue;        /* 3D pi-flux continuum grid: the cos k_z rescue of the initial state */
+    int32_t ic_rescue;        /* 3D pi-flux rescue of the reference's drivers: 1 continuum grid, 2 uniform grid */
     int32_t pad;
+    double split_dispersion;  /* 3D pi-flux, > 0: j split at |m(k - A)| into two outputs of N + 1 samples */
 } or_model;
 
 typedef struct {
@@ -130,7 +131,9 @@ int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const
                   double kx, double ky, double kz, double* s_out, or_stats* st);
 
 /* j(t_i) = sum_k w_k g_k(t_i), i = 0..N, deterministic for any thread count
- * (kz may be null: all zero).  Returns the number of failed trajectories. */
+ * (kz may be null: all zero); with split_dispersion > 0, j_out holds 2 (N + 1)
+ * samples (below the cut, then the rest).  Returns the number of failed
+ * trajectories. */
 int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
                            const double* kx, const double* ky, const double* kz, const double* w, int64_t n_k,
                            double* j_out, or_stats* st, int n_threads);

@@ -21,9 +21,11 @@ class Problem:
     # ---- oracle side ----
     def o_model(self):
         m = dict(self.model)
-        # the continuum grid's rescue of cos k_z belongs to the 3D pi-flux driver
+        # the reference drivers' rescues of cos k belong to their grids
         if self.grid[0] == "pfc" and m.get("kind") == O.MODEL_PIFLUX3D:
             m["ic_rescue"] = 1
+        if self.grid[0] == "pfu" and m.get("kind") == O.MODEL_PIFLUX3D:
+            m["ic_rescue"] = 2
         return O.model(**m)
 
     def o_pulse(self):
@@ -49,6 +51,8 @@ class Problem:
         """(kx, ky, kz, w); kz = 0 for the 2D grids."""
         if self.grid[0] == "pfc":
             return O.piflux_continuum(*self.grid[1:])
+        if self.grid[0] == "pfu":
+            return O.piflux_uniform(*self.grid[1:])
         if self.grid[0] == "box":
             kx, ky, w = O.gl_box(*self.grid[1:])
         elif self.grid[0] == "cyl":
@@ -62,7 +66,8 @@ class Problem:
         m = self.model
         return H.ModelDesc(m.get("kind", 0), m.get("relax", 0), m.get("v", 1.0), m.get("mass", 0.1),
                            m.get("hopping", 1.0), m.get("theta", 0.0), m.get("fermi_energy", 0.0),
-                           m.get("beta", 0.0), m.get("relax_time", 0.0), m.get("relax_time_offdiag", 0.0))
+                           m.get("beta", 0.0), m.get("relax_time", 0.0), m.get("relax_time_offdiag", 0.0),
+                           m.get("split_dispersion", 0.0))
 
     def h_laser(self, H):
         p = self.pulse
@@ -88,6 +93,8 @@ class Problem:
             return H.KGrid.dirac_cylinder(*self.grid[1:])
         if self.grid[0] == "pfc":
             return H.KGrid.piflux_continuum(*self.grid[1:])
+        if self.grid[0] == "pfu":
+            return H.KGrid.piflux_uniform(*self.grid[1:])
         return H.KGrid.hexagonal(*self.grid[1:])
 
     def h_solver(self, H, **over):
@@ -241,6 +248,17 @@ HONEYCOMB_EQUILIBRIUM_CONT = Problem(
     pulse=dict(kind=O.PULSE_CONTINUOUS, amp_a=0.9),
     time=(0.7, 0.7 + 4 * PI, 256, 50),
     grid=("hex", 12, 12),
+)
+
+# the reference's split current of the 3D pi-flux lattice (PiFlux.cpp:954-1035):
+# uniform grid N = 8 (1024 points, including the rescued x = y = N/2 points),
+# cut sqrt(3)/2
+PIFLUX3D_SPLIT = Problem(
+    "piflux3d_split_current",
+    model=dict(kind=O.MODEL_PIFLUX3D, hopping=0.6, fermi_energy=0.2, beta=4.0, split_dispersion=0.5 * math.sqrt(3)),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=0.9, n_cycles=2.0, t0=0.0),
+    time=(0.0, 4 * PI, 256, 50),
+    grid=("pfu", 8),
 )
 
 # a sampled A/B cos^2 pulse through its clamped cubic spline over 1.5 x the

@@ -278,3 +278,26 @@ def test_experiment_vector_potential_is_the_field_integral():
     assert len(A) == 201 + 16 + 8 + 8
     assert np.allclose(-(A[8:209] - A[7:208]) / O.EXP_DT, ea, rtol=0, atol=1e-12)
     assert np.all(A[:8] == 0.0) and np.all(A[223:] == 0.0)
+
+
+def test_piflux_uniform_grid_matches_oracle(hhg):
+    for n in (1, 3, 8):
+        kx, ky, kz, w = hhg.KGrid.piflux_uniform(n).build3()
+        okx, oky, okz, ow = O.piflux_uniform(n)
+        for a, b in ((kx, okx), (ky, oky), (kz, okz), (w, ow)):
+            assert np.array_equal(a, b)
+
+
+def test_plan_rejects_split_current_elsewhere(hhg):
+    """The split current is the 3D pi-flux lattice's, with RKF78."""
+    import math
+    m = hhg.ModelDesc(0, 0, 1.0, 0.1, 0.6, 0.0, 0.0, 0.0, 0.0, 0.0, 0.5 * math.sqrt(3))
+    job = hhg.Job(m, hhg.ContinuousLaser(0.5), hhg.TimeIntegrationConfig(0.0, 6.0, 64, 50), 0)
+    with pytest.raises(hhg.HHGError) as ei:
+        hhg.Plan([job], [hhg.KGrid.box(4, 4, -1.0, 1.0, -1.0, 1.0)], hhg.Solver())
+    assert ei.value.status == hhg.ERROR_UNSUPPORTED
+    m4 = hhg.ModelDesc(4, 0, 1.0, 0.1, 0.6, 0.0, 0.0, 0.0, 0.0, 0.0, 0.5 * math.sqrt(3))
+    job = hhg.Job(m4, hhg.ContinuousLaser(0.5), hhg.TimeIntegrationConfig(0.0, 6.0, 64, 50), 0)
+    with pytest.raises(hhg.HHGError) as ei:
+        hhg.Plan([job], [hhg.KGrid.piflux_uniform(2)], hhg.Solver(method=1))
+    assert ei.value.status == hhg.ERROR_UNSUPPORTED

@@ -797,3 +797,48 @@ def test_spline_of_a_sampled_pulse_converges_at_fourth_order():
         errs.append(np.max(np.abs(O.pulse_A(sp, t) - ref)))
     assert errs[-1] < 1e-5
     assert errs[0] / errs[1] > 12 and errs[1] / errs[2] > 12
+
+
+def test_piflux_uniform_grid_and_its_rescue():
+    """The split current's uniform grid (PiFlux.cpp:954-1035): weights 1/N^3 summing
+    to 2 (+-z), equispaced sums of cos^2 exact; where cos x and cos y vanish and
+    cos z < 0 (x = y = N/2) the reference replaces cos x by 1e-4 |cos z| for the
+    run and scales the initial cos z by 1 - 5e-9 (PiFlux.cpp:978-986)."""
+    for n in (2, 5, 8):
+        kx, ky, kz, w = O.piflux_uniform(n)
+        assert len(w) == 2 * n ** 3 and abs(w.sum() - 2.0) < 1e-14
+        assert abs(np.dot(w, np.cos(kx) ** 2) - 1.0) < 1e-14 and abs(np.dot(w, np.cos(kz) ** 2) - 1.0) < 1e-14
+    m = O.model(O.MODEL_PIFLUX3D, hopping=0.6, ic_rescue=2)
+    plain = O.model(O.MODEL_PIFLUX3D, hopping=0.6)
+    kx = ky = 4 * (math.pi / 8)  # x = y = N/2 of N = 8
+    for kz, rescued in ((5 * math.pi / 8, True), (-5 * math.pi / 8, True), (3 * math.pi / 8, False)):
+        cz = math.cos(kz)
+        mv = np.array([abs(cz) * 1e-4 if rescued else math.cos(kx), math.cos(ky), cz * ((1 - 0.5e-8) if rescued else 1)])
+        assert np.allclose(O.initial_state(m, kx, ky, kz), -mv / np.linalg.norm(mv), rtol=0, atol=1e-14)
+        # the rescued cos x drives the whole run
+        s = np.array([0.3, -0.2, 0.9])
+        p = O.pulse(O.PULSE_COS2, amp_a=0.0, n_cycles=1.0, t0=0.0)
+        mt = np.array([mv[0] if rescued else math.cos(kx), math.cos(ky), cz])
+        assert np.allclose(O.rhs(m, p, kx, ky, s, s, 0.5, kz=kz), 4 * 0.6 * np.cross(mt, s), rtol=0, atol=1e-15)
+        if rescued:
+            # without the rescue the band frame divides by cos z + |m| = 0 there
+            assert np.all(np.isfinite(O.initial_state(m, kx, ky, kz)))
+            assert not np.all(np.isfinite(O.initial_state(plain, kx, ky, kz)))
+
+
+def test_split_current_partitions_the_current():
+    """The split current's two parts sum to the unsplit current, and extreme cuts
+    put everything on one side (PiFlux.cpp:998-1006)."""
+    kx, ky, kz, w = O.piflux_uniform(4)
+    p = O.pulse(O.PULSE_COS2, amp_a=0.9, n_cycles=1.0, t0=0.0)
+    tc = O.time_grid(0.0, 2 * math.pi, 64, 50)
+    sv = O.solver()
+    base = dict(hopping=0.6, fermi_energy=0.2, beta=4.0, ic_rescue=2)
+    j, _ = O.current_density(O.model(O.MODEL_PIFLUX3D, **base), p, tc, sv, kx, ky, w, kz=kz)
+    js, _ = O.current_density(O.model(O.MODEL_PIFLUX3D, split_dispersion=0.5 * math.sqrt(3), **base), p, tc, sv,
+                              kx, ky, w, kz=kz)
+    lo, hi = js[:65], js[65:]
+    assert np.max(np.abs(lo + hi - j)) < 1e-14 * np.abs(j).max()
+    assert np.abs(lo).max() > 1e-3 * np.abs(j).max() and np.abs(hi).max() > 1e-3 * np.abs(j).max()
+    jall, _ = O.current_density(O.model(O.MODEL_PIFLUX3D, split_dispersion=10.0, **base), p, tc, sv, kx, ky, w, kz=kz)
+    assert np.array_equal(jall[65:], np.zeros(65)) and np.max(np.abs(jall[:65] - j)) < 1e-14 * np.abs(j).max()
