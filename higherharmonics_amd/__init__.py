@@ -36,6 +36,7 @@ MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D, MODEL_PIFLUX3D = 
 RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2, RELAX_EQUILIBRIUM = 0, 1, 2, 3
 PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS, PULSE_SPLINE = 0, 1, 2, 3, 4, 5
 KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_CONTINUUM, KGRID_PIFLUX_UNIFORM = 0, 1, 2, 3, 4, 5
+KGRID_PIFLUX_OCCUPATION = 6
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
 
@@ -122,7 +123,7 @@ EXPORTED_SYMBOLS = [
     "hhg_mlp_create", "hhg_mlp_destroy", "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params",
     "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
     "hhg_spline_coefficients", "hhg_reference_double_laser", "hhg_reference_experimental_laser_size",
-    "hhg_reference_experimental_laser",
+    "hhg_reference_experimental_laser", "hhg_occupations",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -151,6 +152,7 @@ def lib() -> ctypes.CDLL:
     L.hhg_plan_capture_graph.argtypes = [P]
     L.hhg_plan_destroy.argtypes = [P]
     L.hhg_trajectories.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
+    L.hhg_occupations.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
     L.hhg_gauss_legendre.argtypes = [ctypes.c_int32, _DP, _DP]
     L.hhg_kgrid_size.restype = ctypes.c_int64
     L.hhg_kgrid_size.argtypes = [ctypes.POINTER(KGridDesc)]
@@ -193,7 +195,7 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).restype = ctypes.c_int
     for name in ("hhg_plan_create", "hhg_plan_job_offsets", "hhg_plan_execute", "hhg_plan_execute_accumulators",
                  "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
-                 "hhg_trajectories", "hhg_gauss_legendre", "hhg_kgrid_build", "hhg_pulse_evaluate",
+                 "hhg_trajectories", "hhg_occupations", "hhg_gauss_legendre", "hhg_kgrid_build", "hhg_pulse_evaluate",
                  "hhg_trapezoidal_spectrum", "hhg_welch_spectrum"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
@@ -414,6 +416,11 @@ class KGrid:
         return KGrid(KGRID_PIFLUX_UNIFORM, n)
 
     @staticmethod
+    def piflux_occupation(n: int) -> "KGrid":
+        """(x pi/n, pi/2, z pi/n), i = x n + z, of the 3D pi-flux occupation maps (kind KGRID_PIFLUX_OCCUPATION)."""
+        return KGrid(KGRID_PIFLUX_OCCUPATION, n)
+
+    @staticmethod
     def explicit(kx, ky, w, kz=None) -> "KGrid":
         arrs = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in (kx, ky, w))
         if kz is not None:
@@ -573,6 +580,7 @@ class Plan:
         off = (ctypes.c_int64 * (len(jobs) + 1))()
         _check(L.hhg_plan_job_offsets(h, off))
         self.offsets = list(off)
+        self._n_samples0 = jobs[0].time_config.n_measurements + 1 if jobs else 0  # job 0's samples per k-point
         self.stats = Stats()
 
     def execute_flat(self, allow_failures: bool = False) -> np.ndarray:
@@ -615,10 +623,19 @@ class Plan:
     def trajectories(self, k_index) -> np.ndarray:
         """Bloch vectors [n_sel][N+1][3] of job 0 for the given k indices."""
         sel = np.ascontiguousarray(k_index, dtype=np.int64)
-        n = self.offsets[1] - self.offsets[0]
+        n = self._n_samples0
         out = np.empty((len(sel), n, 3))
         _check(lib().hhg_trajectories(self._h, sel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(sel),
                                       _dp(out)))
+        return out
+
+    def occupations(self, k_index) -> np.ndarray:
+        """Band occupations [n_sel][N+1][2] = (lower, upper) of job 0 (3D pi-flux
+        lattice, equilibrium relaxation, RKF78) for the given k indices."""
+        sel = np.ascontiguousarray(k_index, dtype=np.int64)
+        out = np.empty((len(sel), self._n_samples0, 2))
+        _check(lib().hhg_occupations(self._h, sel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(sel),
+                                     _dp(out)))
         return out
 
     def close(self) -> None:

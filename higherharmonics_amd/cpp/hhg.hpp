@@ -21,6 +21,8 @@
 #pragma once
 
 #include <algorithm>
+#include <array>
+#include <cstddef>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -273,6 +275,13 @@ class Plan {
         return out;
     }
 
+    // (lower, upper) band occupations of job 0 for the listed k indices: [n_sel][N+1][2]
+    std::vector<h_float> occupations(const std::vector<int64_t>& k_index, int n_samples) {
+        std::vector<h_float> out(k_index.size() * (size_t)n_samples * 2);
+        check(hhg_occupations(plan_.get(), k_index.data(), (int64_t)k_index.size(), out.data()));
+        return out;
+    }
+
     void autotune() { check(hhg_plan_autotune(plan_.get(), nullptr)); }
     void capture_graph() { check(hhg_plan_capture_graph(plan_.get())); }
 
@@ -437,6 +446,77 @@ class PiFlux : public System {
     }
     // the lattice constant sqrt3 hbar v_F / (photon_energy band_width) (PiFlux.cpp:67, PiFlux.hpp:78-80)
     h_float laser_model_ratio() const { return sqrt_3 * DiracSystem::hbar * v_F / (photon_energy * band_width); }
+
+    // PiFlux::compute_occupation_numbers(laser, time_config, N) (PiFlux.cpp:909-952): the
+    // (lower, upper) band occupations on the slice (x pi/N, pi/2, z pi/N) at every sample,
+    // stored as the reference's OccupationContainer maps [sample][x][column] with column
+    // (z - round(A(t_i) / (pi/N))) mod N (coordinate_shift, PiFlux.cpp:918-921); needs the
+    // relaxation (a diagonal relaxation time > 0)
+    struct Occupations {
+        int N = 0, n_samples = 0;
+        std::vector<h_float> lower, upper;  // [n_samples][N][N]
+    };
+    Occupations compute_occupation_numbers(const Laser::Laser* laser, const TimeIntegrationConfig& time_config,
+                                           int N) const {
+        if (model.relax != HHG_RELAX_EQUILIBRIUM)
+            throw std::invalid_argument("occupation numbers need the relaxation (diagonal_relaxation_time > 0)");
+        hhg_job_desc job{};
+        job.model = model;
+        job.pulse = laser->desc();
+        job.time = time_config.desc();
+        job.kgrid = 0;
+        hhg_kgrid_desc g{};
+        g.kind = HHG_KGRID_PIFLUX_OCCUPATION;
+        g.nx = N;
+        Dispatch::Plan plan({g}, {job}, solver);
+        Occupations o;
+        o.N = N;
+        o.n_samples = time_config.n_measurements + 1;
+        std::vector<int64_t> sel((size_t)N * N);
+        for (size_t i = 0; i < sel.size(); ++i) sel[i] = (int64_t)i;
+        const std::vector<h_float> occ = plan.occupations(sel, o.n_samples);
+        o.lower.assign((size_t)o.n_samples * N * N, 0.0);
+        o.upper.assign((size_t)o.n_samples * N * N, 0.0);
+        const h_float dz = pi / N, me = time_config.measure_every();
+        h_float t_begin = time_config.t_begin, t_end = t_begin + me;  // accumulated as PiFlux.cpp:941-949
+        for (int i = 0; i < o.n_samples; ++i) {
+            const int shift = (int)std::round(laser->laser_function(t_begin) / dz);
+            for (int x = 0; x < N; ++x)
+                for (int z = 0; z < N; ++z) {
+                    const int col = (((z - shift) % N) + N) % N;
+                    const size_t src = (((size_t)x * N + z) * o.n_samples + i) * 2;
+                    const size_t dst = ((size_t)i * N + x) * N + col;
+                    o.lower[dst] = occ[src];
+                    o.upper[dst] = occ[src + 1];
+                }
+            t_begin = t_end;
+            t_end += me;
+        }
+        return o;
+    }
+
+    // PiFlux::current_per_energy(laser, time_config, N) (PiFlux.cpp:954-1035): the current
+    // on the uniform grid (x, y, +-z) pi/N split by the instantaneous dispersion
+    // |m(k - A(t_i))| at each sample: {below sqrt3/2 (Dirac-like states), the rest},
+    // n_measurements + 1 samples each
+    std::array<std::vector<h_float>, 2> current_per_energy(const Laser::Laser* laser,
+                                                           const TimeIntegrationConfig& time_config, int N,
+                                                           hhg_stats* stats = nullptr) const {
+        hhg_job_desc job{};
+        job.model = model;
+        job.model.split_dispersion = 0.5 * sqrt_3;
+        job.pulse = laser->desc();
+        job.time = time_config.desc();
+        job.kgrid = 0;
+        hhg_kgrid_desc g{};
+        g.kind = HHG_KGRID_PIFLUX_UNIFORM;
+        g.nx = N;
+        Dispatch::Plan plan({g}, {job}, solver);
+        const std::vector<h_float> j = plan.compute(stats)[0];
+        const size_t n = j.size() / 2;
+        return {std::vector<h_float>(j.begin(), j.begin() + (std::ptrdiff_t)n),
+                std::vector<h_float>(j.begin() + (std::ptrdiff_t)n, j.end())};
+    }
 };
 
 // PiFlux square lattice d = 2t (cos(kx - A), cos ky, 0) over [-pi, pi]^2

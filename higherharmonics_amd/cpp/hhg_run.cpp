@@ -34,6 +34,12 @@
 //   units of the experiment's 6.0914 meV (Dispatcher.cpp:10-19).
 //   The output carries the reference's fields (hhg.cpp:301-326); continuous
 //   lasers use the Welch-window spectrum.
+//   PiFlux only: occupations yes | no | only writes occupations.json.gz (band
+//   occupation maps lower_band / upper_band [500][n_z][n_z] of the relaxing
+//   lattice, hhg.cpp:120-170) and split_current yes | no | only writes
+//   split_current.json.gz (dirac_current / non_dirac_current on the n_z^3
+//   uniform grid, hhg.cpp:174-219), both next to the output file; "only" skips
+//   what follows.  With either, tabulated pulses stop at 0.75 t_end.
 //
 // The 2D models in dimensionless units (time 1/omega_L, energy hbar omega_L):
 //   system_type       Dirac2D | PiFlux2D | Honeycomb2D
@@ -54,6 +60,7 @@
 //                     HoneycombDispatcher.cpp:35, PiFluxDispatcher.cpp:53)
 //   method            rkf78 | magnus4;  rtol, atol, magnus_substeps
 #include <cstdio>
+#include <ctime>
 #include <fstream>
 #include <iostream>
 #include <map>
@@ -107,13 +114,49 @@ std::string num_str(double v) {
     return buf;
 }
 
-// stdout, a plain file, or gzip when the name ends in .gz
-void emit(const std::string& text, int argc, char** argv) {
-    if (argc <= 2) {
-        std::cout << text;
-        return;
+// the reference's SI strings: std::to_string of a double ("%f")
+std::string fixed_str(double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%f", v);
+    return buf;
+}
+
+std::string time_stamp() {
+    const std::time_t now = std::time(nullptr);
+    char buf[64];
+    std::strftime(buf, sizeof buf, "%Y-%m-%d %H:%M:%S", std::localtime(&now));
+    return buf;
+}
+
+// [n_s][N][N] maps as nested arrays
+void write_maps(std::ostream& o, const char* name, const std::vector<double>& v, int n_s, int N, bool last = false) {
+    o << "  \"" << name << "\": [";
+    char buf[40];
+    for (int i = 0; i < n_s; ++i) {
+        o << (i ? ", [" : "[");
+        for (int x = 0; x < N; ++x) {
+            o << (x ? ", [" : "[");
+            for (int z = 0; z < N; ++z) {
+                std::snprintf(buf, sizeof buf, "%.17g", v[((size_t)i * N + x) * N + z]);
+                o << (z ? ", " : "") << buf;
+            }
+            o << "]";
+        }
+        o << "]";
     }
+    o << "]" << (last ? "\n" : ",\n");
+}
+
+// a file next to the main output (the working directory without one)
+std::string sibling(int argc, char** argv, const std::string& name) {
+    if (argc <= 2) return name;
     const std::string path = argv[2];
+    const auto slash = path.find_last_of('/');
+    return slash == std::string::npos ? name : path.substr(0, slash + 1) + name;
+}
+
+// a plain file, or gzip when the name ends in .gz
+void write_file(const std::string& path, const std::string& text) {
     if (path.size() > 3 && path.compare(path.size() - 3, 3, ".gz") == 0) {
         gzFile f = gzopen(path.c_str(), "wb");
         if (!f || gzwrite(f, text.data(), (unsigned)text.size()) != (int)text.size() || gzclose(f) != Z_OK)
@@ -122,6 +165,15 @@ void emit(const std::string& text, int argc, char** argv) {
     }
     std::ofstream out(path);
     if (!out || !(out << text)) throw std::runtime_error("cannot write " + path);
+}
+
+// stdout, a plain file, or gzip when the name ends in .gz
+void emit(const std::string& text, int argc, char** argv) {
+    if (argc <= 2) {
+        std::cout << text;
+        return;
+    }
+    write_file(argv[2], text);
 }
 
 // The reference's 3D runs in its own units (Dirac/PiFlux dispatchers, hhg.cpp main).
@@ -141,6 +193,13 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
 
     const int N_k = (int)num(kv, "N_k", 64);
     const bool dirac = system == "Dirac", honeycomb = system == "Honeycomb";
+    // occupation maps and split current: yes | no | only (hhg.cpp:79-80, 120-219); with
+    // either the PiFlux dispatcher stops tabulated pulses at 0.75 t_end (PiFluxDispatcher.cpp:44)
+    const std::string occupations = str(kv, "occupations", "no"), split_current = str(kv, "split_current", "no");
+    const bool extra_outputs = occupations != "no" || split_current != "no";
+    if (extra_outputs && system != "PiFlux")
+        throw std::invalid_argument("occupations and split_current are outputs of system_type PiFlux");
+    const double t_end_scale = extra_outputs ? 0.75 : 1.0;
     // the tabulated lasers take the photon energy in units of the experiment's
     // (Dispatcher.cpp:10-19); the lattices' systems see the scaled energy, the
     // Dirac system the raw one with the scaled energy only in its laser ratio
@@ -180,7 +239,7 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         laser = std::make_unique<Laser::SplineLaser>(Laser::reference_double_laser(
             gauss, photon_energy, E0, ratio, (double)n_cycles, t0_offset, active_of(gauss ? "dgauss" : "dcos")));
         laser_photon_energy = photon_energy * Laser::exp_photon_energy;
-        tc = {laser->t_begin, laser->t_end, N, 50};
+        tc = {laser->t_begin, t_end_scale * laser->t_end, N, 50};
     } else if (is_exp) {
         // ExperimentalLaser from the measured fields in experiment_data (two columns E_A E_B
         // in MV/cm, 201 rows), B shifted by t0_offset ps (DiracDispatcher.cpp:38-51)
@@ -199,15 +258,62 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         laser = std::make_unique<Laser::SplineLaser>(
             Laser::reference_experimental_laser(photon_energy, E0, ratio, t0_offset, ea, eb, active_of("exp")));
         laser_photon_energy = photon_energy * Laser::exp_photon_energy;
-        tc = {laser->t_begin, laser->t_end, N, 50};
+        tc = {laser->t_begin, t_end_scale * laser->t_end, N, 50};
     } else {
         throw std::invalid_argument("Laser type '" + laser_type + "' is not recognized!");
     }
+    const bool continuous = laser_type == "continuous";
+
+    // the fields every output of hhg.cpp carries (hhg.cpp:141-159) and the PiFlux
+    // dispatcher's special_information (PiFluxDispatcher.cpp:136-141)
+    auto head = [&](std::ostringstream& o, const TimeIntegrationConfig& t) {
+        const int n_t = t.n_measurements;
+        std::vector<double> lf(n_t + (continuous ? 0 : 1));
+        for (size_t i = 0; i < lf.size(); ++i) lf[i] = laser->laser_function(t.t_begin + (double)i * t.measure_every());
+        o << "{\n  \"time\": \"" << time_stamp() << "\", \"N\": " << n_t << ",\n";
+        write_array(o, "laser_function", lf);
+        o << "  \"t_begin\": " << num_str(t.t_begin) << ", \"t_end\": " << num_str(t.t_end)
+          << ", \"n_laser_cycles\": " << n_cycles << ", \"n_measurements\": " << n_t + (continuous ? 0 : 1)
+          << ",\n  \"T\": " << num_str(T) << ", \"E_F\": " << num_str(E_F) << ", \"v_F\": " << num_str(v_F)
+          << ", \"band_width\": " << num_str(band_width) << ", \"field_amplitude\": " << num_str(E0)
+          << ", \"photon_energy\": " << num_str(laser_photon_energy) << ", \"laser_type\": \"" << laser_type
+          << "\", \"diagonal_relaxation_time\": " << num_str(tau)
+          << ", \"offdiagonal_relaxation_time\": " << num_str(tau_o) << ",\n  \"system_type\": \"" << system
+          << "\", \"n_z\": " << n_z << ", \"t0_offset\": " << num_str(t0_offset) << ", \"lattice_constant\": \""
+          << fixed_str(1e12 * psys.laser_model_ratio()) << " pm\", \"hopping_element\": \""
+          << fixed_str(psys.model.hopping * e_sys) << " meV\",\n";
+    };
+    if (occupations != "no") {
+        // PiFlux::compute_occupation_numbers on n_z x n_z points, 499 intervals (hhg.cpp:120-170)
+        TimeIntegrationConfig tco = tc;
+        tco.n_measurements = 499;
+        const auto occ = psys.compute_occupation_numbers(laser.get(), tco, n_z);
+        const int n_out = tco.n_measurements + (continuous ? 0 : 1);
+        std::ostringstream o;
+        head(o, tco);
+        write_maps(o, "lower_band", occ.lower, n_out, n_z);
+        write_maps(o, "upper_band", occ.upper, n_out, n_z, true);
+        o << "}\n";
+        write_file(sibling(argc, argv, "occupations.json.gz"), o.str());
+        if (occupations == "only") return 0;
+    }
+    if (split_current != "no") {
+        // PiFlux::current_per_energy on the n_z^3 x 2 uniform grid (hhg.cpp:174-219)
+        hhg_stats sst{};
+        const auto sc = psys.current_per_energy(laser.get(), tc, n_z, &sst);
+        std::ostringstream o;
+        head(o, tc);
+        write_array(o, "dirac_current", sc[0]);
+        write_array(o, "non_dirac_current", sc[1], true);
+        o << "}\n";
+        write_file(sibling(argc, argv, "split_current.json.gz"), o.str());
+        if (split_current == "only") return 0;
+    }
+
     hhg_stats stats{};
     std::vector<double> runs_ms;
     const int repeats = (int)num(kv, "timing_repeats", 1);  // > 1: execute the plan that often (timing)
     const std::vector<double> j = sys.compute_current_density_repeated(laser.get(), tc, repeats, &runs_ms, &stats);
-    const bool continuous = laser_type == "continuous";
     const Fourier::Spectrum sp = continuous ? Fourier::welch_spectrum(j, tc) : Fourier::trapezoidal_spectrum(j, tc);
     std::vector<double> A(N + (continuous ? 0 : 1));
     for (size_t i = 0; i < A.size(); ++i) A[i] = laser->laser_function(tc.t_begin + (double)i * tc.measure_every());

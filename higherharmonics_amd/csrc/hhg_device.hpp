@@ -176,8 +176,11 @@ constexpr double kRescue = 1e-4;  // RESCUE_TRAFO, PiFlux.cpp:50
 
 // Lattice models take the field along x (theta = 0), as the reference's
 // Honeycomb (Honeycomb.hpp:88-90); the Dirac model accepts any theta; the 3D
-// models take it along z.
-template <int M, int P, int R>
+// models take it along z.  STICKY (3D pi-flux, equilibrium relaxation): the
+// shifted momentum of the equilibrium targets persists across right-hand-side
+// calls, as in PiFlux::evolve_occupation_numbers (PiFlux.cpp:238, 252-257), so
+// its rescued cos x stays once set.
+template <int M, int P, int R, bool STICKY = false>
 struct Bloch {
     JobRef J;
     double kx, ky, kz;
@@ -187,6 +190,8 @@ struct Bloch {
     double cx;  // 3D pi-flux: cos kx (the reference's rescued value where it applies)
     double eq[3];  // equilibrium relaxation: the state at the start of the step
     bool rescued;  // 3D pi-flux, uniform grid: the reference's rescue applies to this point
+    mutable double eq_cx = 0.0;    // STICKY: cos x of the persistent shifted momentum
+    mutable double last_cz = 0.0;  // STICKY: its cos z as left by the last right-hand-side call
 
     __device__ __forceinline__ Bloch(JobRef job, double kx_, double ky_, double kz_ = 0.0)
         : J(job), kx(kx_), ky(ky_), kz(kz_) {
@@ -332,7 +337,10 @@ struct Bloch {
     // rd (s/3 - D) + ro (2s/3 - O) (PiFlux.cpp:174-179).
     __device__ __forceinline__ void relax_equilibrium(double cz, const double s[3], double out[3]) const {
         double mx = cx;
-        if (ref_is_zero(mx) && ref_is_zero(cy) && cz < 0.0) {
+        if constexpr (STICKY) {
+            cz = shift_frame(cz);
+            mx = eq_cx;
+        } else if (ref_is_zero(mx) && ref_is_zero(cy) && cz < 0.0) {
             mx = fabs(cz) * kRescue;
             cz *= (1.0 - 0.5 * kRescue * kRescue);
         }
@@ -360,6 +368,40 @@ struct Bloch {
         out[0] -= rd * (s[0] * (1.0 / 3.0) - mx * fd) + ro * (s[0] * (2.0 / 3.0) - O0);
         out[1] -= rd * (s[1] * (1.0 / 3.0) - my * fd) + ro * (s[1] * (2.0 / 3.0) - O1);
         out[2] -= rd * (s[2] * (1.0 / 3.0) - cz * fd) + ro * (s[2] * (2.0 / 3.0) - O2);
+    }
+
+    // STICKY: update_z of the persistent shifted momentum to cos(z - A) = cz and
+    // its rescue (PiFlux.cpp:253-257); returns its cos z
+    __device__ __forceinline__ double shift_frame(double cz) const {
+        if (ref_is_zero(eq_cx) && ref_is_zero(cy) && cz < 0.0) {
+            eq_cx = fabs(cz) * kRescue;
+            cz *= (1.0 - 0.5 * kRescue * kRescue);
+        }
+        last_cz = cz;
+        return cz;
+    }
+
+    // STICKY: the shifted momentum's first update at A(t_begin), before the
+    // integration (PiFlux.cpp:285)
+    __device__ __forceinline__ void begin_run(double A0) {
+        eq_cx = cx;
+        shift_frame(cos(kz - A0));
+    }
+
+    // STICKY: compute_occupations of PiFlux.cpp:228-236 with the persistent
+    // shifted momentum: the state in its band frame (diagonal_sigma,
+    // PiFlux.cpp:1037-1049) d, then (lower, upper) = sqrt((|d| -+ d_z) / 2)
+    __device__ __forceinline__ void occupations(const double s[3], double o[2]) const {
+        const double mx = eq_cx, my = cy, mz = last_cz;
+        const double disp = sqrt(mx * mx + my * my + mz * mz);
+        const double z = mz + disp, nrm = 2.0 * disp * z;
+        const double xz = 2.0 * mx * z, yz = 2.0 * my * z, xy = 2.0 * mx * my;
+        const double d0 = (s[0] * (z * z - mx * mx + my * my) - xy * s[1] - xz * s[2]) / nrm;
+        const double d1 = (s[1] * (z * z + mx * mx - my * my) - xy * s[0] - yz * s[2]) / nrm;
+        const double d2 = (s[2] * (z * z - mx * mx - my * my) + xz * s[0] + yz * s[1]) / nrm;
+        const double n = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        o[0] = sqrt(0.5 * (-d2 + n));
+        o[1] = sqrt(0.5 * (d2 + n));
     }
 
     // ds/dt = 2 d(k - A) x s - relaxation

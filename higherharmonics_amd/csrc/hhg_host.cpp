@@ -99,6 +99,10 @@ int64_t kgrid_size(const hhg_kgrid_desc& g) {
         if (g.nx < 1 || g.nx > 2048) return 0;
         return 2 * (int64_t)g.nx * g.nx * g.nx;
     }
+    if (g.kind == HHG_KGRID_PIFLUX_OCCUPATION) {
+        if (g.nx < 1 || g.nx > 65536) return 0;
+        return (int64_t)g.nx * g.nx;
+    }
     return (int64_t)g.nx * (int64_t)g.ny;
 }
 
@@ -172,6 +176,22 @@ hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* 
                 kz[k] = sgn * (d * z);
                 w[k] = wt;
             }
+        }
+        return HHG_SUCCESS;
+    }
+    if (g.kind == HHG_KGRID_PIFLUX_OCCUPATION) {
+        // PiFlux::compute_occupation_numbers (PiFlux.cpp:909-952): k = (x pi/N, pi/2, z pi/N)
+        // for i = x N + z; the weight 1/N^2 is not used by the reference (it stores per-k maps)
+        if (g.nx < 1 || g.nx > 65536)
+            return fail(HHG_ERROR_INVALID_ARGUMENT, "pi-flux occupation grid needs 1 <= nx <= 65536");
+        if (!kz) return fail(HHG_ERROR_INVALID_ARGUMENT, "pi-flux grid needs the third coordinate");
+        const int64_t N = g.nx;
+        const double pi = 3.14159265358979323846, d = pi / (double)N, wt = 1.0 / (double)(N * N);
+        for (int64_t i = 0; i < N * N; ++i) {
+            kx[i] = (double)(i / N) * d;
+            ky[i] = 0.5 * pi;
+            kz[i] = d * (double)(i % N);
+            w[i] = wt;
         }
         return HHG_SUCCESS;
     }

@@ -61,6 +61,8 @@ hipError_t launch_solve(int method, int model, int pulse, int relax, bool learne
                         int threads, size_t smem, hipStream_t st);
 const void* solve_kernel_fn(int method, int model, int pulse, int relax, bool learned);
 hipError_t launch_trajectories(int method, int model, int pulse, int relax, const TrajArgs& x);
+// 3D pi-flux lattice with equilibrium relaxation, RKF78: x.s_out = occupations [n_sel][N+1][2]
+hipError_t launch_occupations(int pulse, const TrajArgs& x);
 hipError_t launch_cost_probe(int model, int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem,
                              hipStream_t st);
 // 30-point Gauss-Legendre nodes and weights on [0, 1] (moment Magnus tables)

@@ -427,8 +427,9 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         const int g = d->jobs[i].kgrid;
         if (J.model == hhg::MODEL_PIFLUX3D)
             J.ic_rescue = d->kgrids[g].kind == HHG_KGRID_PIFLUX_CONTINUUM ? 1
-                          : d->kgrids[g].kind == HHG_KGRID_PIFLUX_UNIFORM  ? 2
-                                                                           : 0;
+                          : (d->kgrids[g].kind == HHG_KGRID_PIFLUX_UNIFORM ||
+                             d->kgrids[g].kind == HHG_KGRID_PIFLUX_OCCUPATION) ? 2
+                                                                               : 0;
         const int64_t n_chunks = (grid_size[g] + hhg::WAVE - 1) / hhg::WAVE;
         const int64_t per_rank = (n_chunks + n_ranks - 1) / n_ranks;
         const int64_t c0 = std::min(n_chunks, per_rank * d->rank), c1 = std::min(n_chunks, c0 + per_rank);
@@ -775,6 +776,39 @@ hhg_status hhg_trajectories(hhg_plan* p, const int64_t* k_index, int64_t n_sel, 
     if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(s_out, d_s, out_bytes, hipMemcpyDeviceToHost), "download");
     hipFree(d_sel);
     hipFree(d_s);
+    return st;
+}
+
+hhg_status hhg_occupations(hhg_plan* p, const int64_t* k_index, int64_t n_sel, double* occ_out) {
+    hhg::clear_error();
+    if (!p || (n_sel > 0 && (!k_index || !occ_out)))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_occupations: null argument");
+    const hhg::JobDev& J = p->jobs[0];
+    if (J.model != hhg::MODEL_PIFLUX3D || J.relax != hhg::RELAX_EQUILIBRIUM || p->solver.method != HHG_METHOD_RKF78)
+        return fail(HHG_ERROR_INVALID_ARGUMENT,
+                    "hhg_occupations: job 0 must be the 3D pi-flux lattice with equilibrium relaxation, solver RKF78");
+    if (n_sel <= 0) return HHG_SUCCESS;
+    for (int64_t i = 0; i < n_sel; ++i)
+        if (k_index[i] < 0 || k_index[i] >= J.n_k)
+            return fail(HHG_ERROR_INVALID_ARGUMENT, "k index outside job 0's k-points");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    int64_t* d_sel = nullptr;
+    double* d_o = nullptr;
+    const size_t out_bytes = sizeof(double) * 2 * (size_t)(J.n_meas + 1) * (size_t)n_sel;
+    hhg_status st = check_hip(hipMalloc(&d_sel, sizeof(int64_t) * n_sel), "selection");
+    if (st == HHG_SUCCESS) st = check_hip(hipMalloc(&d_o, out_bytes), "occupation output");
+    if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(d_sel, k_index, sizeof(int64_t) * n_sel, hipMemcpyHostToDevice), "upload");
+    if (st == HHG_SUCCESS) st = check_hip(hhg::launch_clear(p->d_words, p->n_words, p->stream), "clear");
+    if (st == HHG_SUCCESS) {
+        hhg::SolverDev sp{p->solver.rtol, p->solver.atol, p->solver.max_fail > 0 ? p->solver.max_fail : 500,
+                          p->solver.proposal};
+        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, p->d_kz, d_sel, n_sel, p->d_Am, sp, d_o, p->counters(0), p->stream};
+        st = check_hip(hhg::launch_occupations(J.pulse, x), "occupation launch");
+    }
+    if (st == HHG_SUCCESS) st = check_hip(hipStreamSynchronize(p->stream), "synchronize");
+    if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(occ_out, d_o, out_bytes, hipMemcpyDeviceToHost), "download");
+    hipFree(d_sel);
+    hipFree(d_o);
     return st;
 }
 

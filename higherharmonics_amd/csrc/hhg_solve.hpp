@@ -415,6 +415,62 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
     if (status == 2) atomicAdd(&ctr->nonfinite, 1ull);
 }
 
+// Band occupations of the 3D pi-flux lattice with equilibrium relaxation
+// (PiFlux::evolve_occupation_numbers, PiFlux.cpp:222-318): one lane per
+// selected k-point, RKF78 with the persistent shifted momentum, (lower, upper)
+// at every sample.
+template <int P>
+__global__ __launch_bounds__(64) void occupation_kernel(const JobDev* jobs, const double* kx, const double* ky,
+                                                        const double* kz, const int64_t* sel, int64_t n_sel,
+                                                        SolverDev sp, double* occ_out, Counters* ctr) {
+    const int64_t t_idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t_idx >= n_sel) return;
+    JobRef J = job_ref(jobs);
+    const int64_t idx = J.k_base + sel[t_idx];
+    Bloch<MODEL_PIFLUX3D, P, RELAX_EQUILIBRIUM, true> f(J, kx[idx], ky[idx], kz[idx]);
+    double s[3], o[2];
+    f.initial(s);
+    f.begin_run(pulse_A<P>(J, J.t_begin));
+    const int N = J.n_meas;
+    double* out = occ_out + t_idx * (int64_t)(N + 1) * 2;
+    f.occupations(s, o);
+    out[0] = o[0];
+    out[1] = o[1];
+    uint32_t att = 0, rej = 0;
+    int status = 0;
+    const double h_restart = J.delta / J.n_sub;
+    double h_carry = h_restart;
+    IntervalCtl ctl{0, h_restart, false};
+    IntervalOut io{};
+    int kq = 0;
+    for (int i = 1; i <= N; ++i) {
+        if (status == 0) {
+            const double t0 = J.t_begin + (double)(i - 1) * J.delta;
+            const double t1 = J.t_begin + (double)i * J.delta;
+            double h = (sp.proposal == PROPOSAL_RESTART) ? h_restart : h_carry;
+            double unclipped = h;
+            double ta = t0;
+            for (int q = kq;; ++q) {  // envelope corners, as in the solve kernel
+                const bool stop = q < J.n_kinks && J.kink_i[q] == i;
+                const double tb = stop ? J.kink_t[q] : t1;
+                if (status == 0) rkf78_interval(f, s, ta, tb, h, unclipped, sp, att, rej, status, nullptr, ctl, io);
+                if (!stop) break;
+                h = h > unclipped ? h : unclipped;
+                ta = tb;
+            }
+            h_carry = h > unclipped ? h : unclipped;
+        }
+        while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
+        f.occupations(s, o);
+        out[2 * i + 0] = o[0];
+        out[2 * i + 1] = o[1];
+    }
+    atomicAdd(&ctr->attempts, (unsigned long long)att);
+    atomicAdd(&ctr->rejects, (unsigned long long)rej);
+    if (status == 1) atomicAdd(&ctr->failed, 1ull);
+    if (status == 2) atomicAdd(&ctr->nonfinite, 1ull);
+}
+
 // Predicted cost of every k-point of a group's items for lockstep grouping:
 // the network's attempts-per-interval output summed over n_probe evenly
 // spaced intervals (features with h_prev = Delta and a 1e-3 error ratio).

@@ -15,6 +15,11 @@
  *   - per-k time_evolution_sigma / time_evolution_magnus /
  *       time_evolution_diagonal_relaxation (DiracSystem.hpp:40-47,
  *       Honeycomb.hpp:64-68, PiFlux.hpp:80-81) -> hhg_trajectories
+ *   - PiFlux::current_per_energy(laser, time_config, N) (PiFlux.hpp:98-99,
+ *       PiFlux.cpp:954-1035) -> a job with hhg_model_desc.split_dispersion on
+ *       HHG_KGRID_PIFLUX_UNIFORM
+ *   - PiFlux::compute_occupation_numbers(laser, time_config, N) (PiFlux.hpp:95-96,
+ *       PiFlux.cpp:909-952) -> hhg_occupations over HHG_KGRID_PIFLUX_OCCUPATION
  *   - Laser::laser_function(t) (sources/HHG/Laser/Laser.hpp:32) -> hhg_pulse_evaluate
  *   - TimeIntegrationConfig {t_begin, t_end, n_measurements, n_subdivisions}
  *       (sources/HHG/TimeIntegrationConfig.hpp:7-15) -> hhg_time_desc
@@ -153,11 +158,14 @@ typedef enum hhg_kgrid_kind {
                                       of the z map.  Jobs on this grid start, like the reference's continuum
                                       driver, from cos z scaled by 1 - 5e-9 at the +z nodes with cos z < 0
                                       (PiFlux.cpp:825-829) */
-    HHG_KGRID_PIFLUX_UNIFORM = 5   /* (x pi/N, y pi/N, +-z pi/N), x, y, z < N = nx, weight 1/N^3 each: the uniform
+    HHG_KGRID_PIFLUX_UNIFORM = 5,  /* (x pi/N, y pi/N, +-z pi/N), x, y, z < N = nx, weight 1/N^3 each: the uniform
                                       grid of PiFlux::current_per_energy (PiFlux.cpp:954-1035).  Where cos x and
                                       cos y vanish (is_zero) and cos z < 0 the reference's rescue replaces cos x by
                                       1e-4 |cos z| for the whole run and scales the initial cos z by 1 - 5e-9
                                       (PiFlux.cpp:978-986); jobs on this grid do the same */
+    HHG_KGRID_PIFLUX_OCCUPATION = 6 /* (x pi/N, pi/2, z pi/N) for i = x N + z, x, z < N = nx, weight 1/N^2: the
+                                      slice of PiFlux::compute_occupation_numbers (PiFlux.cpp:909-952), with the
+                                      rescue of HHG_KGRID_PIFLUX_UNIFORM (PiFlux.cpp:935-938) */
 } hhg_kgrid_kind;
 
 typedef struct hhg_kgrid_desc {
@@ -287,6 +295,19 @@ hhg_status hhg_plan_destroy(hhg_plan* plan);
  * its k-grid: s_out[n_sel][N+1][3].  Replaces time_evolution_sigma /
  * time_evolution_magnus for inspection and tests. */
 hhg_status hhg_trajectories(hhg_plan* plan, const int64_t* k_index, int64_t n_sel, double* s_out);
+
+/* Band occupations of job 0 for the listed k indices of its k-grid:
+ * occ_out[n_sel][N+1][2] = (lower, upper) at every sample.  Replaces
+ * PiFlux::evolve_occupation_numbers (PiFlux.cpp:222-318, called from
+ * compute_occupation_numbers, PiFlux.cpp:909-952): job 0 is the 3D pi-flux
+ * lattice with HHG_RELAX_EQUILIBRIUM and the plan's solver RKF78.  As in the
+ * reference the shifted momentum of the equilibrium targets persists across
+ * right-hand-side calls, so once its rescue (cos x, cos y zero, cos(z - A) < 0)
+ * has set cos x = 1e-4 |cos(z - A)| that value stays for the rest of the run;
+ * the occupations are sqrt((|d| -+ d_z)/2) of the state in the band frame of
+ * that shifted momentum as left by the last right-hand-side call of the
+ * interval (diagonal_sigma, PiFlux.cpp:1037-1049). */
+hhg_status hhg_occupations(hhg_plan* plan, const int64_t* k_index, int64_t n_sel, double* occ_out);
 
 /* ---- host utilities (no GPU needed) ---- */
 

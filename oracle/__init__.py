@@ -96,6 +96,9 @@ def lib(fast: bool = False) -> ctypes.CDLL:
     L.or_trajectory.restype = ctypes.c_int
     L.or_trajectory.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
                                 ctypes.POINTER(Solver), *d3, dp, ctypes.POINTER(Stats)]
+    L.or_occupations.restype = ctypes.c_int
+    L.or_occupations.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
+                                 ctypes.POINTER(Solver), *d3, dp, ctypes.POINTER(Stats)]
     L.or_current_density.restype = ctypes.c_int64
     L.or_current_density.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
                                      ctypes.POINTER(Solver), dp, dp, dp, dp, ctypes.c_int64, dp, ctypes.POINTER(Stats),
@@ -197,6 +200,18 @@ def trajectory(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float, k
                             _dp(out), ctypes.byref(st))
     if r < 0:
         raise ValueError("unsupported model/solver combination")
+    return out, st, r
+
+
+def occupations(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float, kz: float, fast: bool = False):
+    """(lower, upper) band occupations [N+1][2] of one 3D pi-flux point with
+    equilibrium relaxation (PiFlux::evolve_occupation_numbers, PiFlux.cpp:222-318)."""
+    out = np.empty((tc.n_measurements + 1, 2))
+    st = Stats()
+    r = lib(fast).or_occupations(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv), kx, ky, kz,
+                             _dp(out), ctypes.byref(st))
+    if r < 0:
+        raise ValueError("occupations need the 3D pi-flux lattice with equilibrium relaxation and RKF78")
     return out, st, r
 
 
@@ -383,6 +398,34 @@ def piflux_uniform(n: int):
     ky = np.repeat(y * d, 2)
     kz = np.stack([d * z, -(d * z)], axis=1).ravel()
     return kx, ky, kz, np.full(2 * n ** 3, 1.0 / n ** 3)
+
+
+def piflux_occupation(n: int):
+    """(kx, ky, kz, w) of PiFlux::compute_occupation_numbers' slice (PiFlux.cpp:909-952):
+    (x pi/n, pi/2, z pi/n) for i = x n + z, weight 1/n^2 (unused by the reference)."""
+    i = np.arange(n * n)
+    d = np.pi / n
+    return (i // n) * d, np.full(n * n, 0.5 * np.pi), d * (i % n), np.full(n * n, 1.0 / n ** 2)
+
+
+def occupation_maps(lower_upper, A, n: int, t_begin: float, delta: float):
+    """The reference's occupation maps [N+1][n][n] (lower, upper) from the per-point
+    occupations [n*n][N+1][2] of the piflux_occupation grid: point (x, z) at sample
+    i goes to column (z - round(A(t_i) / dz)) mod n (coordinate_shift,
+    PiFlux.cpp:918-921, 940-946); t_i accumulates t_begin + delta + ... as there."""
+    occ = np.asarray(lower_upper)
+    n_s = occ.shape[1]
+    lower = np.zeros((n_s, n, n))
+    upper = np.zeros((n_s, n, n))
+    dz = np.pi / n
+    x = np.arange(n * n) // n
+    z = np.arange(n * n) % n
+    for i in range(n_s):
+        shift = int(np.round(A[i] / dz)) if np.isfinite(A[i]) else 0
+        col = ((z - shift) % n + n) % n
+        lower[i, x, col] = occ[:, i, 0]
+        upper[i, x, col] = occ[:, i, 1]
+    return lower, upper
 
 
 def gl_box(nx: int, ny: int, x0: float, x1: float, y0: float, y1: float):

@@ -516,6 +516,22 @@ double or_vertex(const or_model* m, double kx, double ky, double kz, double A, c
 /* RKF78 with the odeint controller                                           */
 /* ------------------------------------------------------------------------- */
 
+/* The shifted momentum of PiFlux::evolve_occupation_numbers (PiFlux.cpp:238,
+ * 252-257): one object for the whole run, so that the rescue of its cos x,
+ * once made, stays; cz is its cos z as the last update left it. */
+typedef struct {
+    double cx, cz;
+} or_frame;
+
+/* update_z to z - A and the rescue (PiFlux.cpp:253-257) */
+static void frame_update(or_frame* f, double cy, double cz) {
+    if (or_is_zero(f->cx) && or_is_zero(cy) && cz < 0.0) {
+        f->cx = fabs(cz) * OR_RESCUE;
+        cz *= (1.0 - 0.5 * OR_RESCUE * OR_RESCUE);
+    }
+    f->cz = cz;
+}
+
 typedef struct {
     const or_model* m;
     const or_pulse* p;
@@ -525,11 +541,30 @@ typedef struct {
     double rtol, atol;
     int max_fail;
     or_stats* st;
+    or_frame* frame;  /* non-null: the occupation run's persistent shifted momentum */
 } or_ctx;
 
 static void ctx_rhs(or_ctx* c, const double s[3], double t, double out[3]) {
-    or_rhs(c->m, c->p, c->kx, c->ky, c->kz, c->m->relax == OR_RELAX_EQUILIBRIUM ? c->s_step : c->s_init, s, t, out);
     c->st->rhs_evals++;
+    if (c->frame) {
+        /* right_side of PiFlux.cpp:287-299: 4t (m x s) at the point's own m,
+         * the equilibrium targets at the persistent shifted momentum */
+        const or_model* m = c->m;
+        const double A = or_pulse_A(c->p, t);
+        const double cx = piflux_cos_x(m, c->kx, c->ky, c->kz, NULL), cy = cos(c->ky), cz = cos(c->kz - A);
+        const double pf = 4.0 * m->hopping;
+        out[0] = (cy * s[2] - cz * s[1]) * pf;
+        out[1] = (cz * s[0] - cx * s[2]) * pf;
+        out[2] = (cx * s[1] - cy * s[0]) * pf;
+        frame_update(c->frame, cy, cz);
+        double D[3], O[3];
+        piflux3d_equilibrium(m, c->frame->cx, cy, c->frame->cz, c->s_step, D, O);
+        const double rd = 1.0 / m->relax_time;
+        const double ro = m->relax_time_offdiag > 0.0 ? 1.0 / m->relax_time_offdiag : 0.0;
+        for (int q = 0; q < 3; ++q) out[q] -= rd * (s[q] / 3.0 - D[q]) + ro * (s[q] * (2.0 / 3.0) - O[q]);
+        return;
+    }
+    or_rhs(c->m, c->p, c->kx, c->ky, c->kz, c->m->relax == OR_RELAX_EQUILIBRIUM ? c->s_step : c->s_init, s, t, out);
 }
 
 /* controlled_runge_kutta::try_step for an explicit error stepper: derivative at
@@ -787,8 +822,9 @@ static int envelope_corners(const or_pulse* p, const or_time* tc, double* t_k, i
 typedef void (*emit_fn)(void* user, int i, const double s[3]);
 
 static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv, double kx, double ky,
-                  double kz, or_stats* st, emit_fn emit, void* user) {
+                  double kz, or_stats* st, emit_fn emit, void* user, or_frame* frame) {
     or_ctx c;
+    c.frame = frame;
     c.m = m;
     c.p = p;
     c.kx = kx;
@@ -810,6 +846,15 @@ static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const
     c.s_init[0] = s[0];
     c.s_init[1] = s[1];
     c.s_init[2] = s[2];
+    if (frame) {
+        /* the occupation run: 3D pi-flux lattice, equilibrium relaxation, RKF78;
+         * the shifted momentum starts as the point's own and is updated to
+         * A(t_begin) before the integration (PiFlux.cpp:238, 285) */
+        if (m->kind != OR_MODEL_PIFLUX3D || m->relax != OR_RELAX_EQUILIBRIUM || sv->method != OR_METHOD_RKF78)
+            return -1;
+        frame->cx = piflux_cos_x(m, kx, ky, kz, NULL);
+        frame_update(frame, cos(ky), cos(kz - or_pulse_A(p, tc->t_begin)));
+    }
     emit(user, 0, s);
     /* equilibrium relaxation exists for the 3D pi-flux lattice, which has no
      * other relaxation, and the honeycomb lattice */
@@ -884,7 +929,40 @@ int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const
                   double kz, double* s_out, or_stats* st) {
     traj_user u = {s_out};
     or_stats local = {0, 0, 0, 0};
-    const int r = evolve(m, p, tc, sv, kx, ky, kz, st ? st : &local, emit_traj, &u);
+    const int r = evolve(m, p, tc, sv, kx, ky, kz, st ? st : &local, emit_traj, &u, NULL);
+    if (r > 0 && st) st->failed_trajectories++;
+    return r;
+}
+
+typedef struct {
+    double* out;
+    const or_frame* frame;
+    double cy;
+} occ_user;
+
+/* compute_occupations of PiFlux.cpp:228-236: the state in the band frame of
+ * the shifted momentum (diagonal_sigma, PiFlux.cpp:1037-1049), then
+ * (lower, upper) = sqrt((|d| -+ d_z) / 2) */
+static void emit_occ(void* user, int i, const double s[3]) {
+    occ_user* u = (occ_user*)user;
+    const double mx = u->frame->cx, my = u->cy, mz = u->frame->cz;
+    const double disp = sqrt(mx * mx + my * my + mz * mz);
+    const double z = mz + disp, nrm = 2.0 * disp * z;
+    const double xz = 2.0 * mx * z, yz = 2.0 * my * z, xy = 2.0 * mx * my;
+    const double d0 = (s[0] * (z * z - mx * mx + my * my) - xy * s[1] - xz * s[2]) / nrm;
+    const double d1 = (s[1] * (z * z + mx * mx - my * my) - xy * s[0] - yz * s[2]) / nrm;
+    const double d2 = (s[2] * (z * z - mx * mx - my * my) + xz * s[0] + yz * s[1]) / nrm;
+    const double n = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    u->out[2 * i + 0] = sqrt(0.5 * (-d2 + n));
+    u->out[2 * i + 1] = sqrt(0.5 * (d2 + n));
+}
+
+int or_occupations(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv, double kx, double ky,
+                   double kz, double* occ_out, or_stats* st) {
+    or_frame frame = {0.0, 0.0};
+    occ_user u = {occ_out, &frame, cos(ky)};
+    or_stats local = {0, 0, 0, 0};
+    const int r = evolve(m, p, tc, sv, kx, ky, kz, st ? st : &local, emit_occ, &u, &frame);
     if (r > 0 && st) st->failed_trajectories++;
     return r;
 }
@@ -944,7 +1022,7 @@ int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* 
         for (int64_t k = c * OR_CHUNK; k < k_end; ++k) {
             const double z = kz ? kz[k] : 0.0;
             acc_user u = {m, A, kx[k], ky[k], z, w[k], row, split ? N + 1 : 0};
-            const int r = evolve(m, p, tc, sv, kx[k], ky[k], z, &ls, emit_acc, &u);
+            const int r = evolve(m, p, tc, sv, kx[k], ky[k], z, &ls, emit_acc, &u, NULL);
             if (r > 0) failed++;
             if (r < 0) unsupported++;
         }

@@ -130,6 +130,15 @@ double or_vertex(const or_model* m, double kx, double ky, double kz, double A, c
 int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
                   double kx, double ky, double kz, double* s_out, or_stats* st);
 
+/* Band occupations of one point of the 3D pi-flux lattice with equilibrium
+ * relaxation (PiFlux::evolve_occupation_numbers, PiFlux.cpp:222-318):
+ * occ_out[(N+1)*2] = (lower, upper) at every sample.  The shifted momentum of
+ * the equilibrium targets persists across right-hand-side calls (its rescued
+ * cos x stays once set).  Returns 0, 1 if the controller failed, -1 if the
+ * model, relaxation or method is not that. */
+int or_occupations(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
+                   double kx, double ky, double kz, double* occ_out, or_stats* st);
+
 /* j(t_i) = sum_k w_k g_k(t_i), i = 0..N, deterministic for any thread count
  * (kz may be null: all zero); with split_dispersion > 0, j_out holds 2 (N + 1)
  * samples (below the cut, then the rest).  Returns the number of failed

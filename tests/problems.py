@@ -24,7 +24,7 @@ class Problem:
         # the reference drivers' rescues of cos k belong to their grids
         if self.grid[0] == "pfc" and m.get("kind") == O.MODEL_PIFLUX3D:
             m["ic_rescue"] = 1
-        if self.grid[0] == "pfu" and m.get("kind") == O.MODEL_PIFLUX3D:
+        if self.grid[0] in ("pfu", "pfo") and m.get("kind") == O.MODEL_PIFLUX3D:
             m["ic_rescue"] = 2
         return O.model(**m)
 
@@ -53,6 +53,8 @@ class Problem:
             return O.piflux_continuum(*self.grid[1:])
         if self.grid[0] == "pfu":
             return O.piflux_uniform(*self.grid[1:])
+        if self.grid[0] == "pfo":
+            return O.piflux_occupation(*self.grid[1:])
         if self.grid[0] == "box":
             kx, ky, w = O.gl_box(*self.grid[1:])
         elif self.grid[0] == "cyl":
@@ -95,6 +97,8 @@ class Problem:
             return H.KGrid.piflux_continuum(*self.grid[1:])
         if self.grid[0] == "pfu":
             return H.KGrid.piflux_uniform(*self.grid[1:])
+        if self.grid[0] == "pfo":
+            return H.KGrid.piflux_occupation(*self.grid[1:])
         return H.KGrid.hexagonal(*self.grid[1:])
 
     def h_solver(self, H, **over):
@@ -259,6 +263,19 @@ PIFLUX3D_SPLIT = Problem(
     pulse=dict(kind=O.PULSE_COS2, amp_a=0.9, n_cycles=2.0, t0=0.0),
     time=(0.0, 4 * PI, 256, 50),
     grid=("pfu", 8),
+)
+
+# the occupation slice (x, pi/2, z) pi/8 of the relaxing 3D pi-flux lattice: the
+# x = pi/2 column has cos x = cos y = 0, where the persistent rescue of the
+# occupation run acts (PiFlux.cpp:909-952, 252-257)
+PIFLUX3D_OCCUPATION = Problem(
+    "piflux3d_occupation",
+    model=dict(kind=O.MODEL_PIFLUX3D, hopping=0.6, fermi_energy=0.1, beta=4.0, relax=O.RELAX_EQUILIBRIUM,
+               relax_time=5.0, relax_time_offdiag=3.0),
+    pulse=dict(kind=O.PULSE_COS2, amp_a=1.5, n_cycles=1.0, t0=0.0),
+    time=(0.0, 2 * PI, 40, 50),
+    grid=("pfo", 8),
+    solver=dict(rtol=1e-8, atol=1e-12),
 )
 
 # a sampled A/B cos^2 pulse through its clamped cubic spline over 1.5 x the

@@ -4,6 +4,8 @@ Parity bar (FP64 floating point): relative L2 error of j(t) <= 1e-9 against
 the oracle on the same grid, pulse, time grid and controller semantics; the
 per-harmonic spectrum within 1e-6 relative down to 1e-12 of its peak.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -11,7 +13,8 @@ import oracle as O
 from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
                             HONEYCOMB_CORNERS, HONEYCOMB_EQUILIBRIUM, HONEYCOMB_EQUILIBRIUM_CONT, HONEYCOMB_SMALL,
                             HONEYCOMB_SPLINE,
-                            PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX_SMALL, SMALL_CASES, oracle_j, rel_l2)
+                            PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX3D_OCCUPATION, PIFLUX3D_SPLIT, PIFLUX_SMALL,
+                            SMALL_CASES, oracle_j, rel_l2)
 
 pytestmark = pytest.mark.gpu
 
@@ -572,3 +575,198 @@ def test_cpp_driver_experimental_laser(hhg, tmp_path, system):
     err = rel_l2(np.array(d["current_density_time"]), j_ref)
     print(f"  rel_l2={err:.3e}", flush=True)
     assert err <= tol
+
+
+def test_piflux3d_split_current_matches_oracle(hhg):
+    """The reference's split current on the 3D pi-flux lattice (PiFlux.cpp:954-1035):
+    on the uniform grid with its zero-gap rescue, the current of the states with
+    dispersion below sqrt(3)/2 and of the rest, each against the oracle; the two
+    halves sum to the unsplit current, and the rescued points' trajectories agree."""
+    pr = PIFLUX3D_SPLIT
+    _print_case(pr)
+    j_ref, st_ref = oracle_j(pr)
+    n = pr.time[2] + 1
+    kx, ky, kz, _ = pr.o_grid3()
+    rescued = np.flatnonzero((np.abs(np.cos(kx)) < 1e-12) & (np.abs(np.cos(ky)) < 1e-12) & (np.cos(kz) < 0))
+    assert len(rescued) > 0
+    sel = np.concatenate([rescued, [0, 77, 500]])
+    with pr.h_plan(hhg) as plan:
+        j = plan.execute()[0]
+        st = plan.stats
+        traj = plan.trajectories(sel)
+    assert j.shape == (2 * n,)
+    lo, hi = j[:n], j[n:]
+    e_lo, e_hi = rel_l2(lo, j_ref[:n]), rel_l2(hi, j_ref[n:])
+    print(f"  rel_l2 below={e_lo:.3e} above={e_hi:.3e} attempts gpu={st.attempts} oracle={st_ref.attempts}", flush=True)
+    assert e_lo <= J_TOL and e_hi <= J_TOL
+    assert st.failed_trajectories == 0 and st.nonfinite_trajectories == 0
+    assert abs(st.attempts - st_ref.attempts) <= 1e-3 * st_ref.attempts
+    for i, k in enumerate(sel):
+        s_ref = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(), kx[k], ky[k], kz[k])[0]
+        assert np.max(np.abs(traj[i] - s_ref)) < 1e-9
+    from tests.problems import Problem
+    model = dict(pr.model)
+    del model["split_dispersion"]
+    whole = Problem("unsplit", model, pr.pulse, pr.time, pr.grid)
+    with whole.h_plan(hhg) as plan:
+        j_all = plan.execute()[0]
+    assert np.max(np.abs(lo + hi - j_all)) <= 1e-13 * np.abs(j_all).max()
+
+
+def _occupation_classes(kx, ky, kz, A_fine):
+    """Per point of an occupation slice: 'stable', or 'persistent' (cos kx = cos ky = 0,
+    cos kz >= 0 and cos(kz - A) turns negative: the persistent rescue sets cos x to
+    1e-4 |cos(kz - A)| at the first such right-hand-side call, a value fixed by where
+    that call lands next to the sign change) or 'degenerate' (all three cosines
+    vanish: the start sits on the Dirac point).  On the last two the reference's
+    relaxation is ill-conditioned: its band frame divides by cos z + |m| with |m|
+    down to 1e-8, and its two builds differ far above rounding."""
+    tiny = 2.0 ** -44
+    out = []
+    for x, y, z in zip(kx, ky, kz):
+        col = abs(math.cos(x)) < tiny and abs(math.cos(y)) < tiny
+        if col and abs(math.cos(z)) < tiny:
+            out.append("degenerate")
+        elif col and math.cos(z) >= 0 and np.min(np.cos(z - A_fine)) < 0:
+            out.append("persistent")
+        else:
+            out.append("stable")
+    return out
+
+
+def _check_occupations(occ, ref, fast, classes, label, column):
+    """Stable points: within max(bar, 10 x the oracle's strict-vs-fast spread of
+    that point), bar = 1e-9, and 1e-7 on the cos kx = cos ky = 0 column, where
+    the band frame's cos z + |m| cancels down to |m|^2 / 2|cos z| with
+    |m| ~ 1e-4 |cos z| (one ulp of cos z there is 2e-8 relative); the
+    ill-conditioned ones: finite, equal at the first sample."""
+    worst = 0.0
+    for k, c in enumerate(classes):
+        err = float(np.max(np.abs(occ[k] - ref[k])))
+        spread = float(np.max(np.abs(fast[k] - ref[k])))
+        if c == "stable":
+            assert err <= max(1e-7 if column[k] else 1e-9, 10 * spread), (label, k, err, spread)
+            worst = max(worst, err)
+        else:
+            assert np.all(np.isfinite(occ[k])) and np.all(occ[k] >= 0), (label, k)
+            assert np.max(np.abs(occ[k][0] - ref[k][0])) <= 1e-12, (label, k)
+            print(f"  {label} point {k} ({c}): max err {err:.3e}, oracle spread {spread:.3e}", flush=True)
+    print(f"  {label}: stable points max err {worst:.3e}", flush=True)
+
+
+def test_piflux3d_occupations_match_oracle(hhg):
+    """PiFlux::compute_occupation_numbers' per-k evolution (PiFlux.cpp:222-318,
+    909-952) on the (x, pi/2, z) pi/8 slice: (lower, upper) occupations at every
+    sample against the oracle on the well-conditioned points (all but the
+    persistent-rescue and degenerate points of the x = pi/2 column).  The
+    persistent rescue itself: the occupations follow sqrt((|s| -+ s.m^)/2) of
+    the plain relaxing trajectory on every point but the persistent ones, where
+    the occupation run leaves it.  A plan of another model or relaxation is
+    refused."""
+    import math
+    pr = PIFLUX3D_OCCUPATION
+    _print_case(pr)
+    n, N = pr.grid[1], pr.time[2]
+    kx, ky, kz, _ = pr.o_grid3()
+    with pr.h_plan(hhg) as plan:
+        occ = plan.occupations(np.arange(n * n))
+        traj = plan.trajectories(np.arange(n * n))
+    assert occ.shape == (n * n, N + 1, 2)
+    args = (pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver())
+    ref = np.array([O.occupations(*args, kx[k], ky[k], kz[k])[0] for k in range(n * n)])
+    fast = np.array([O.occupations(*args, kx[k], ky[k], kz[k], fast=True)[0] for k in range(n * n)])
+    p = pr.o_pulse()
+    classes = _occupation_classes(kx, ky, kz, O.pulse_A(p, np.linspace(pr.time[0], pr.time[1], 4001)))
+    assert classes.count("persistent") >= 2 and classes.count("degenerate") == 1
+    _check_occupations(occ, ref, fast, classes, "slice", np.abs(np.cos(kx)) < 2.0 ** -44)
+    # the GPU's occupation run against its own plain trajectories
+    A = O.pulse_A(p, pr.time[0] + np.arange(N + 1) * ((pr.time[1] - pr.time[0]) / N))
+    tiny = 2.0 ** -44
+    for k, c in enumerate(classes):
+        if c == "degenerate":
+            continue
+        cx, cy, c0 = math.cos(kx[k]), math.cos(ky[k]), math.cos(kz[k])
+        if abs(cx) < tiny and abs(cy) < tiny and c0 < 0:
+            cx = abs(c0) * 1e-4
+        mv = np.stack([np.full(N + 1, cx), np.full(N + 1, cy), np.cos(kz[k] - A)], axis=1)
+        pr_ = np.sum(traj[k] * mv, axis=1) / np.linalg.norm(mv, axis=1)
+        ns = np.linalg.norm(traj[k], axis=1)
+        dev = max(np.max(np.abs(occ[k, :, 0] - np.sqrt(np.maximum(ns - pr_, 0.0) / 2))),
+                  np.max(np.abs(occ[k, :, 1] - np.sqrt((ns + pr_) / 2))))
+        if c == "persistent":
+            assert dev > 1e-6, k
+        else:
+            assert dev < (1e-7 if abs(math.cos(kx[k])) < tiny else 1e-12), (k, dev)
+    with PIFLUX3D_COS2.h_plan(hhg) as plan:
+        with pytest.raises(hhg.HHGError):
+            plan.occupations(np.arange(4))
+
+
+def test_cpp_driver_piflux_split_current_and_occupations(hhg, tmp_path):
+    """hhg_run PiFlux with occupations yes and split_current yes (hhg.cpp:120-219):
+    the tabulated pulse stops at 0.75 t_end (PiFluxDispatcher.cpp:44),
+    split_current.json.gz holds the Dirac-like and remaining currents on the
+    n_z^3 x 2 uniform grid, occupations.json.gz the (lower, upper) maps
+    [500][n_z][n_z] shifted by round(A(t_i)/dz) columns; each against the
+    oracle.  The relaxation toward the equilibrium is rounding sensitive (its
+    band frame divides by cos z + |m|), so the bars are ten times the oracle's
+    own strict-vs-fast-math spread, and never below 1e-9."""
+    import gzip
+    import json
+    import math
+    n_z, N = 4, 256
+    d = _run_driver(hhg, tmp_path, "system_type PiFlux\nT 20\nE_F 1\nv_F 1.5e3\nband_width 10\nfield_amplitude 1.5\n"
+                    "photon_energy 0.9\ndiagonal_relaxation_time 20\noffdiagonal_relaxation_time 10\n"
+                    f"laser_type dcos\nn_laser_cycles 1\nt0_offset 0.2\nz_range 3\nn_coarse 1\nn_measurements {N}\n"
+                    f"n_z {n_z}\noccupations yes\nsplit_current yes\n", "current_density")
+    hbar, k_B = O.REF_HBAR, 0.08617333262145177434
+    e_ph = 0.9 * O.EXP_PHOTON_ENERGY
+    ratio = math.sqrt(3) * hbar * 1.5e3 / (e_ph * 10.0)
+    y, dt, t_max = O.reference_double_laser_samples(False, 0.9, 1.5, ratio, 1.0, 0.2, 0)
+    assert d["t_begin"] == 0.0 and abs(d["t_end"] - 0.75 * t_max) < 1e-12
+    p = O.spline_pulse(y, 0.0, dt, t_cut=t_max)
+    m = O.model(O.MODEL_PIFLUX3D, hopping=10.0 / math.sqrt(12), fermi_energy=1.0 / e_ph, beta=e_ph / (k_B * 20.0),
+                relax=O.RELAX_EQUILIBRIUM, relax_time=20.0 * e_ph / (1e15 * hbar),
+                relax_time_offdiag=10.0 * e_ph / (1e15 * hbar), ic_rescue=2,
+                split_dispersion=0.5 * math.sqrt(3))
+    sv = O.solver(rtol=1e-8, atol=1e-12)
+
+    sc = json.loads(gzip.open(tmp_path / "split_current.json.gz").read())
+    assert sc["N"] == N and len(sc["laser_function"]) == N + 1 and sc["n_z"] == n_z
+    assert sc["lattice_constant"].endswith(" pm") and sc["hopping_element"].endswith(" meV")
+    kx, ky, kz, w = O.piflux_uniform(n_z)
+    tc = O.time_grid(0.0, 0.75 * t_max, N, 50)
+    j_ref, _ = O.current_density(m, p, tc, sv, kx, ky, w, kz=kz)
+    j_fast, _ = O.current_density(m, p, tc, sv, kx, ky, w, kz=kz, fast=True)
+    j = np.concatenate([sc["dirac_current"], sc["non_dirac_current"]])
+    spread = rel_l2(j_fast, j_ref)
+    err = rel_l2(j, j_ref)
+    print(f"  split rel_l2={err:.3e} oracle spread={spread:.3e}", flush=True)
+    assert err <= max(J_TOL, 10 * spread)
+
+    oc = json.loads(gzip.open(tmp_path / "occupations.json.gz").read())
+    assert oc["N"] == 499 and oc["n_measurements"] == 500
+    lower, upper = np.array(oc["lower_band"]), np.array(oc["upper_band"])
+    assert lower.shape == (500, n_z, n_z) and upper.shape == (500, n_z, n_z)
+    tco = O.time_grid(0.0, 0.75 * t_max, 499, 50)
+    ox, oy, oz, _ = O.piflux_occupation(n_z)
+    mo = O.model(O.MODEL_PIFLUX3D, hopping=10.0 / math.sqrt(12), fermi_energy=1.0 / e_ph, beta=e_ph / (k_B * 20.0),
+                 relax=O.RELAX_EQUILIBRIUM, relax_time=20.0 * e_ph / (1e15 * hbar),
+                 relax_time_offdiag=10.0 * e_ph / (1e15 * hbar), ic_rescue=2)
+    strict = np.array([O.occupations(mo, p, tco, sv, ox[k], oy[k], oz[k])[0] for k in range(n_z * n_z)])
+    fast = np.array([O.occupations(mo, p, tco, sv, ox[k], oy[k], oz[k], fast=True)[0] for k in range(n_z * n_z)])
+    # per point from the maps: (x, z) sits at column (z - round(A(t_i)/dz)) mod n_z,
+    # t_i accumulated as PiFlux.cpp:941-949
+    me, dz = 0.75 * t_max / 499, math.pi / n_z
+    t, shifts = 0.0, []
+    for _ in range(500):
+        shifts.append(int(np.round(float(O.pulse_A(p, t)[0]) / dz)))
+        t += me
+    occ = np.empty((n_z * n_z, 500, 2))
+    for k in range(n_z * n_z):
+        x, z = divmod(k, n_z)
+        cols = [((z - sh) % n_z + n_z) % n_z for sh in shifts]
+        occ[k, :, 0] = lower[np.arange(500), x, cols]
+        occ[k, :, 1] = upper[np.arange(500), x, cols]
+    classes = _occupation_classes(ox, oy, oz, O.pulse_A(p, np.linspace(0.0, 0.75 * t_max, 4001)))
+    _check_occupations(occ, strict, fast, classes, "driver", np.abs(np.cos(ox)) < 2.0 ** -44)

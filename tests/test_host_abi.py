@@ -288,6 +288,14 @@ def test_piflux_uniform_grid_matches_oracle(hhg):
             assert np.array_equal(a, b)
 
 
+def test_piflux_occupation_grid_matches_oracle(hhg):
+    for n in (1, 4, 9):
+        kx, ky, kz, w = hhg.KGrid.piflux_occupation(n).build3()
+        okx, oky, okz, ow = O.piflux_occupation(n)
+        for a, b in ((kx, okx), (ky, oky), (kz, okz), (w, ow)):
+            assert np.array_equal(a, b)
+
+
 def test_plan_rejects_split_current_elsewhere(hhg):
     """The split current is the 3D pi-flux lattice's, with RKF78."""
     import math

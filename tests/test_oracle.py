@@ -799,6 +799,66 @@ def test_spline_of_a_sampled_pulse_converges_at_fourth_order():
     assert errs[0] / errs[1] > 12 and errs[1] / errs[2] > 12
 
 
+def test_occupations_closed_form_and_persistent_rescue():
+    """PiFlux::evolve_occupation_numbers (PiFlux.cpp:222-318).  diagonal_sigma
+    rotates the shifted m onto z, so the occupations are sqrt((|s| -+ s.m^)/2)
+    of the state: away from the x = pi/2 column they follow from the plain
+    trajectory with m = (cos kx, cos ky, cos(kz - A(t_i))) (up to the
+    cancellation in cos z + |m| of the reference's band frame where cos z < 0
+    and |m| ~ 1e-4).  Its shifted momentum persists across right-hand-side
+    calls, so once cos(kz - A) < 0 has rescued its vanishing cos x that value
+    stays: the occupation run leaves the plain trajectory exactly on the points
+    of the x = pi/2 column whose cos kz >= 0 (not rescued at the start) and
+    whose cos(kz - A) turns negative during the run."""
+    n, N = 8, 40
+    kx, ky, kz, _ = O.piflux_occupation(n)
+    m = O.model(O.MODEL_PIFLUX3D, hopping=0.6, fermi_energy=0.1, beta=4.0, relax=O.RELAX_EQUILIBRIUM,
+                relax_time=5.0, relax_time_offdiag=3.0, ic_rescue=2)
+    p = O.pulse(O.PULSE_COS2, amp_a=1.5, n_cycles=1.0, t0=0.0)
+    tc = O.time_grid(0.0, 2 * math.pi, N, 50)
+    sv = O.solver(rtol=1e-8, atol=1e-12)
+    A = O.pulse_A(p, np.arange(N + 1) * (2 * math.pi / N))
+    A_fine = O.pulse_A(p, np.linspace(0.0, 2 * math.pi, 4001))
+    tiny = 2.0 ** -44
+    for k in range(n * n):
+        occ, _, r = O.occupations(m, p, tc, sv, kx[k], ky[k], kz[k])
+        s, _, _ = O.trajectory(m, p, tc, sv, kx[k], ky[k], kz[k])
+        assert r == 0
+        cx, cy, c0 = math.cos(kx[k]), math.cos(ky[k]), math.cos(kz[k])
+        rescued = abs(cx) < tiny and abs(cy) < tiny and c0 < 0
+        if rescued:
+            cx = abs(c0) * 1e-4
+        mv = np.stack([np.full(N + 1, cx), np.full(N + 1, cy), np.cos(kz[k] - A)], axis=1)
+        mh = mv / np.linalg.norm(mv, axis=1)[:, None]
+        ns, pr = np.linalg.norm(s, axis=1), np.sum(s * mh, axis=1)
+        dev = max(np.max(np.abs(occ[:, 0] - np.sqrt(np.maximum(ns - pr, 0.0) / 2))),
+                  np.max(np.abs(occ[:, 1] - np.sqrt((ns + pr) / 2))))
+        persistent = abs(cx) < tiny and not rescued and np.min(np.cos(kz[k] - A_fine)) < 0
+        if persistent:
+            assert dev > 1e-6, k
+            continue
+        tol = 1e-7 if abs(math.cos(kx[k])) < tiny else 1e-12  # the band frame's cancellation at |m| ~ 1e-4
+        assert dev < tol, k
+        assert np.allclose(occ[:, 0] ** 2 + occ[:, 1] ** 2, ns, rtol=0, atol=tol)
+    # the occupation run is defined for the relaxing pi-flux lattice only
+    with pytest.raises(ValueError):
+        O.occupations(O.model(O.MODEL_PIFLUX3D, hopping=0.6), p, tc, sv, 0.3, 0.5 * math.pi, 1.0)
+
+
+def test_occupation_maps_place_every_point_once():
+    """coordinate_shift of PiFlux.cpp:918-921: point (x, z) at sample i goes to
+    column (z - round(A(t_i)/dz)) mod n, a permutation of each row."""
+    n = 6
+    occ = np.random.default_rng(3).random((n * n, 3, 2))
+    A = np.array([0.0, 2.2 * math.pi / n, -7.6 * math.pi / n])
+    lo, up = O.occupation_maps(occ, A, n, 0.0, 1.0)
+    for i, shift in enumerate((0, 2, -8)):
+        for x in range(n):
+            for z in range(n):
+                c = (z - shift) % n
+                assert lo[i, x, c] == occ[x * n + z, i, 0] and up[i, x, c] == occ[x * n + z, i, 1]
+
+
 def test_piflux_uniform_grid_and_its_rescue():
     """The split current's uniform grid (PiFlux.cpp:954-1035): weights 1/N^3 summing
     to 2 (+-z), equispaced sums of cos^2 exact; where cos x and cos y vanish and
