@@ -27,6 +27,7 @@
 //   Honeycomb: RKF78, the hexagonal-BZ grid with N_k positive nodes per axis
 //   (64), relaxation toward the instantaneous equilibrium, cos^2 + 2 pi
 //   (HoneycombDispatcher.cpp:31-35).
+//   gauss (PiFlux): the reference's GaussLaser over [t0_offset, t0_offset + 6 pi n].
 //   Tabulated lasers (cubic-spline pulses): dcos, dcosA, dcosB, dgauss, dgaussA,
 //   dgaussB (PiFlux; B delayed by t0_offset ps) and exp, expA, expB (all three;
 //   experiment_data names a file of the measured fields, two columns E_A E_B in
@@ -233,6 +234,17 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         laser = std::make_unique<Laser::CosineLaser>(A0, (double)n_cycles, pi * t0_offset);
         // the lattice continues one cycle after the pulse so the relaxation sets in
         tc = {laser->t_begin, laser->t_end + (dirac ? 0.0 : 2.0 * pi), N, 50};
+    } else if (laser_type == "gauss" && !dirac && !honeycomb) {
+        // GaussLaser(n_cycles, begin shift t0_offset): window [t0, t0 + 6 pi n], centre in the
+        // middle, sigma = pi n / sqrt(2 ln 20) (GaussLaser.cpp:9-12, GaussLaser.hpp:13-16,
+        // PiFluxDispatcher.cpp:86-88)
+        const double base = 2.0 * pi * 3.0 * n_cycles;
+        auto g = std::make_unique<Laser::GaussLaser>(A0, Laser::GaussLaser::reference_sigma(n_cycles),
+                                                     t0_offset + 0.5 * base);
+        g->t_begin = t0_offset;
+        g->t_end = t0_offset + base;
+        laser = std::move(g);
+        tc = {laser->t_begin, t_end_scale * laser->t_end, N, 50};
     } else if (is_double && !dirac && !honeycomb) {
         // DoubleLaser<CosineLaser | GaussLaser> with the B delay t0_offset in ps (PiFluxDispatcher.cpp:74-104)
         const bool gauss = laser_type.rfind("dgauss", 0) == 0;

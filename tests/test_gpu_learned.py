@@ -214,8 +214,10 @@ def test_learned_mixed_plan_grouped_graph(hhg):
         plan.capture_graph()
         replays = [plan.execute() for _ in range(3)]
     for a, b, t in zip(grouped, ungrouped, tight):
-        # same per-k trajectories; only the FP64 sums inside each wave see other k-points
-        assert rel_l2(a, b) <= 1e-13
+        # same per-k trajectories; only the FP64 sums inside each wave see other
+        # k-points: 64-term sums round at ~1e-14 of sum |w g|, which exceeds |j|
+        # tenfold where j changes sign, so the two groupings agree to ~1e-13
+        assert rel_l2(a, b) <= 1e-12
         assert rel_l2(a, t) <= 1e-9
     for r in replays:
         for a, b in zip(r, grouped):

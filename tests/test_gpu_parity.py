@@ -770,3 +770,25 @@ def test_cpp_driver_piflux_split_current_and_occupations(hhg, tmp_path):
         occ[k, :, 1] = upper[np.arange(500), x, cols]
     classes = _occupation_classes(ox, oy, oz, O.pulse_A(p, np.linspace(0.0, 0.75 * t_max, 4001)))
     _check_occupations(occ, strict, fast, classes, "driver", np.abs(np.cos(ox)) < 2.0 ** -44)
+
+
+def test_cpp_driver_piflux_gauss_laser(hhg, tmp_path):
+    """hhg_run PiFlux with laser_type gauss: the reference's GaussLaser over
+    [t0_offset, t0_offset + 6 pi n] with the centre in the middle and
+    sigma = pi n / sqrt(2 ln 20) (GaussLaser.cpp:9-12, GaussLaser.hpp:13-16);
+    j(t) against the oracle's run."""
+    d = _run_driver(hhg, tmp_path, "system_type PiFlux\nT 0\nE_F 0\nv_F 1.5e3\nband_width 10\nfield_amplitude 4\n"
+                    "photon_energy 5.25\ndiagonal_relaxation_time -1\nlaser_type gauss\nn_laser_cycles 1\n"
+                    "t0_offset 0.5\nz_range 3\nn_coarse 1\nn_measurements 512\n", "gauss")
+    ratio = math.sqrt(3) * O.REF_HBAR * 1.5e3 / (5.25 * 10.0)
+    A0 = ratio * 1e11 * 4.0 / 5.25
+    base = 6 * math.pi
+    assert d["t_begin"] == 0.5 and abs(d["t_end"] - (0.5 + base)) < 1e-14
+    p = O.pulse(O.PULSE_GAUSS, amp_a=A0, sigma=math.pi / math.sqrt(2 * math.log(20.0)), t0=0.5 + 0.5 * base)
+    kx, ky, kz, w = O.piflux_continuum(3, 1)
+    m = O.model(O.MODEL_PIFLUX3D, hopping=10.0 / math.sqrt(12), ic_rescue=1)
+    j_ref, _ = O.current_density(m, p, O.time_grid(0.5, 0.5 + base, 512, 50), O.solver(rtol=1e-8, atol=1e-12),
+                                 kx, ky, w, kz=kz)
+    err = rel_l2(np.array(d["current_density_time"]), j_ref)
+    print(f"  rel_l2={err:.3e}", flush=True)
+    assert err <= J_TOL
