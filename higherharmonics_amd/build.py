@@ -47,6 +47,19 @@ def _compile(src: str) -> str:
     return obj
 
 
+def _git_commit() -> str:
+    """HEAD of the repository the sources come from (recorded in the driver's metadata)."""
+    try:
+        r = subprocess.run(["git", "rev-parse", "HEAD"], cwd=HERE, capture_output=True, text=True, timeout=10)
+        dirty = subprocess.run(["git", "status", "--porcelain", "--untracked-files=no"], cwd=HERE,
+                               capture_output=True, text=True, timeout=10)
+        if r.returncode == 0:
+            return r.stdout.strip() + ("-dirty" if dirty.stdout.strip() else "")
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
 def build(verbose: bool = False) -> str:
     """Compile all sources and link the shared library; returns its path."""
     os.makedirs(BUILD, exist_ok=True)
@@ -60,7 +73,8 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     # the C++ driver over the host interface (cpp/hhg.hpp): a host-only program
     cxx = shutil.which("g++") or shutil.which("c++")
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "cpp", "hhg_run.cpp"), "-o", RUN + ".tmp",
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-DHHG_GIT_COMMIT=\"{_git_commit()}\"",
+           os.path.join(HERE, "cpp", "hhg_run.cpp"), "-o", RUN + ".tmp",
            "-L" + HERE, "-lhhg_amd", "-lz", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

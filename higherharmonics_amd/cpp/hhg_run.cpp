@@ -41,6 +41,8 @@
 //   split_current.json.gz (dirac_current / non_dirac_current on the n_z^3
 //   uniform grid, hhg.cpp:174-219), both next to the output file; "only" skips
 //   what follows.  With either, tabulated pulses stop at 0.75 t_end.
+//   Every file output gets a <name>metadata.json.gz beside it (build commit,
+//   date, compiler, ABI version), the role of the reference's metadata files.
 //
 // The 2D models in dimensionless units (time 1/omega_L, energy hbar omega_L):
 //   system_type       Dirac2D | PiFlux2D | Honeycomb2D
@@ -148,6 +150,22 @@ void write_maps(std::ostream& o, const char* name, const std::vector<double>& v,
     o << "]" << (last ? "\n" : ",\n");
 }
 
+#ifndef HHG_GIT_COMMIT
+#define HHG_GIT_COMMIT "unknown"
+#endif
+
+// the metadata file written beside every output (the role of the reference's
+// metadata.json.gz, hhg.cpp:165-169, 212-216, 337-341: information on the build
+// that produced the data)
+std::string metadata_json() {
+    std::ostringstream o;
+    o << "{\n  \"hhg_git_commit\": \"" << HHG_GIT_COMMIT << "\",\n  \"hhg_build_date\": \"" << __DATE__ << " "
+      << __TIME__ << "\",\n  \"hhg_compiler\": \"" << __VERSION__ << "\",\n  \"hhg_abi_version\": "
+      << hhg_abi_version() << ",\n  \"hhg_target\": \"gfx950\",\n  \"hhg_run_time\": \"" << time_stamp()
+      << "\"\n}\n";
+    return o.str();
+}
+
 // a file next to the main output (the working directory without one)
 std::string sibling(int argc, char** argv, const std::string& name) {
     if (argc <= 2) return name;
@@ -168,13 +186,14 @@ void write_file(const std::string& path, const std::string& text) {
     if (!out || !(out << text)) throw std::runtime_error("cannot write " + path);
 }
 
-// stdout, a plain file, or gzip when the name ends in .gz
+// stdout, or a plain file (gzip when the name ends in .gz) with metadata.json.gz beside it
 void emit(const std::string& text, int argc, char** argv) {
     if (argc <= 2) {
         std::cout << text;
         return;
     }
     write_file(argv[2], text);
+    write_file(sibling(argc, argv, "metadata.json.gz"), metadata_json());
 }
 
 // The reference's 3D runs in its own units (Dirac/PiFlux dispatchers, hhg.cpp main).
@@ -307,6 +326,7 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         write_maps(o, "upper_band", occ.upper, n_out, n_z, true);
         o << "}\n";
         write_file(sibling(argc, argv, "occupations.json.gz"), o.str());
+        write_file(sibling(argc, argv, "occupations_metadata.json.gz"), metadata_json());
         if (occupations == "only") return 0;
     }
     if (split_current != "no") {
@@ -319,6 +339,7 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         write_array(o, "non_dirac_current", sc[1], true);
         o << "}\n";
         write_file(sibling(argc, argv, "split_current.json.gz"), o.str());
+        write_file(sibling(argc, argv, "split_current_metadata.json.gz"), metadata_json());
         if (split_current == "only") return 0;
     }
 

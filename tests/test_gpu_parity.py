@@ -731,6 +731,9 @@ def test_cpp_driver_piflux_split_current_and_occupations(hhg, tmp_path):
                 split_dispersion=0.5 * math.sqrt(3))
     sv = O.solver(rtol=1e-8, atol=1e-12)
 
+    for meta in ("metadata", "split_current_metadata", "occupations_metadata"):
+        md = json.loads(gzip.open(tmp_path / f"{meta}.json.gz").read())
+        assert md["hhg_git_commit"] and md["hhg_abi_version"] == hhg.lib().hhg_abi_version()
     sc = json.loads(gzip.open(tmp_path / "split_current.json.gz").read())
     assert sc["N"] == N and len(sc["laser_function"]) == N + 1 and sc["n_z"] == n_z
     assert sc["lattice_constant"].endswith(" pm") and sc["hopping_element"].endswith(" meV")
