@@ -278,6 +278,17 @@ PIFLUX3D_OCCUPATION = Problem(
     solver=dict(rtol=1e-8, atol=1e-12),
 )
 
+# the same slice under a continuous pulse that starts at A = -0.9: the shifted
+# frame is updated to A(t_begin) before the integration (PiFlux.cpp:285)
+PIFLUX3D_OCCUPATION_CONT = Problem(
+    "piflux3d_occupation_continuous",
+    model=dict(PIFLUX3D_OCCUPATION.model),
+    pulse=dict(kind=O.PULSE_CONTINUOUS, amp_a=1.5),
+    time=(-2.5, -2.5 + 2 * PI, 40, 50),
+    grid=("pfo", 8),
+    solver=dict(rtol=1e-8, atol=1e-12),
+)
+
 # a sampled A/B cos^2 pulse through its clamped cubic spline over 1.5 x the
 # window as the reference's DoubleLaser table (DoubleLaser.hpp:48-73; it takes
 # 1000 samples, 300 here make the spline error visible in j)

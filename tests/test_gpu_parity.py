@@ -13,8 +13,8 @@ import oracle as O
 from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
                             HONEYCOMB_CORNERS, HONEYCOMB_EQUILIBRIUM, HONEYCOMB_EQUILIBRIUM_CONT, HONEYCOMB_SMALL,
                             HONEYCOMB_SPLINE,
-                            PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX3D_OCCUPATION, PIFLUX3D_SPLIT, PIFLUX_SMALL,
-                            SMALL_CASES, oracle_j, rel_l2)
+                            PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX3D_OCCUPATION, PIFLUX3D_OCCUPATION_CONT,
+                            PIFLUX3D_SPLIT, PIFLUX_SMALL, SMALL_CASES, oracle_j, rel_l2)
 
 pytestmark = pytest.mark.gpu
 
@@ -654,7 +654,8 @@ def _check_occupations(occ, ref, fast, classes, label, column):
     print(f"  {label}: stable points max err {worst:.3e}", flush=True)
 
 
-def test_piflux3d_occupations_match_oracle(hhg):
+@pytest.mark.parametrize("pr", [PIFLUX3D_OCCUPATION, PIFLUX3D_OCCUPATION_CONT], ids=lambda p: p.name)
+def test_piflux3d_occupations_match_oracle(hhg, pr):
     """PiFlux::compute_occupation_numbers' per-k evolution (PiFlux.cpp:222-318,
     909-952) on the (x, pi/2, z) pi/8 slice: (lower, upper) occupations at every
     sample against the oracle on the well-conditioned points (all but the
@@ -663,8 +664,6 @@ def test_piflux3d_occupations_match_oracle(hhg):
     the plain relaxing trajectory on every point but the persistent ones, where
     the occupation run leaves it.  A plan of another model or relaxation is
     refused."""
-    import math
-    pr = PIFLUX3D_OCCUPATION
     _print_case(pr)
     n, N = pr.grid[1], pr.time[2]
     kx, ky, kz, _ = pr.o_grid3()

@@ -799,7 +799,8 @@ def test_spline_of_a_sampled_pulse_converges_at_fourth_order():
     assert errs[0] / errs[1] > 12 and errs[1] / errs[2] > 12
 
 
-def test_occupations_closed_form_and_persistent_rescue():
+@pytest.mark.parametrize("pulse", ["cos2", "continuous"])
+def test_occupations_closed_form_and_persistent_rescue(pulse):
     """PiFlux::evolve_occupation_numbers (PiFlux.cpp:222-318).  diagonal_sigma
     rotates the shifted m onto z, so the occupations are sqrt((|s| -+ s.m^)/2)
     of the state: away from the x = pi/2 column they follow from the plain
@@ -809,16 +810,22 @@ def test_occupations_closed_form_and_persistent_rescue():
     calls, so once cos(kz - A) < 0 has rescued its vanishing cos x that value
     stays: the occupation run leaves the plain trajectory exactly on the points
     of the x = pi/2 column whose cos kz >= 0 (not rescued at the start) and
-    whose cos(kz - A) turns negative during the run."""
+    whose cos(kz - A) turns negative during the run, A(t_begin) included (the
+    continuous pulse starts at A = -0.9: the first sample is already in the band
+    frame of k - A(t_begin))."""
     n, N = 8, 40
     kx, ky, kz, _ = O.piflux_occupation(n)
     m = O.model(O.MODEL_PIFLUX3D, hopping=0.6, fermi_energy=0.1, beta=4.0, relax=O.RELAX_EQUILIBRIUM,
                 relax_time=5.0, relax_time_offdiag=3.0, ic_rescue=2)
-    p = O.pulse(O.PULSE_COS2, amp_a=1.5, n_cycles=1.0, t0=0.0)
-    tc = O.time_grid(0.0, 2 * math.pi, N, 50)
+    if pulse == "cos2":
+        p, t0 = O.pulse(O.PULSE_COS2, amp_a=1.5, n_cycles=1.0, t0=0.0), 0.0
+    else:
+        p, t0 = O.pulse(O.PULSE_CONTINUOUS, amp_a=1.5), -2.5
+        assert abs(O.pulse_A(p, t0)[0]) > 0.8
+    tc = O.time_grid(t0, t0 + 2 * math.pi, N, 50)
     sv = O.solver(rtol=1e-8, atol=1e-12)
-    A = O.pulse_A(p, np.arange(N + 1) * (2 * math.pi / N))
-    A_fine = O.pulse_A(p, np.linspace(0.0, 2 * math.pi, 4001))
+    A = O.pulse_A(p, t0 + np.arange(N + 1) * (2 * math.pi / N))
+    A_fine = O.pulse_A(p, np.linspace(t0, t0 + 2 * math.pi, 4001))
     tiny = 2.0 ** -44
     for k in range(n * n):
         occ, _, r = O.occupations(m, p, tc, sv, kx[k], ky[k], kz[k])
