@@ -4,41 +4,64 @@
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline
  * that bench.py times.  Nothing in the product links or loads this file.
  *
- * What it restates (reference paths relative to the reference repository):
+ * What it restates (reference paths relative to the reference repository),
+ * and what pins each part (tests/test_oracle.py, tests/test_host_abi.py):
  *  - pulses: Laser::laser_function = A0 * envelope(t) * sin(t)
  *      (sources/HHG/Laser/Laser.hpp:32-54), cos^2 envelope
  *      (sources/HHG/Laser/CosineLaser.cpp:22-26), Gaussian envelope
  *      (sources/HHG/Laser/GaussLaser.cpp:26-28), A/B pair where B is the same
  *      base pulse evaluated at t - delay (sources/HHG/Laser/DoubleLaser.hpp:63-67),
- *      evaluated analytically instead of through the reference's cubic B-spline;
+ *      continuous A0 sin t; pinned by closed forms and E = -dA/dt against
+ *      finite differences;
+ *  - sampled pulses through the clamped cubic spline (the function Boost's
+ *      cardinal_cubic_b_spline with end slopes represents, Laser.hpp:8-11),
+ *      here in slope (Hermite) form; pinned against scipy's clamped CubicSpline
+ *      and by convergence; the last bits of Boost's B-spline evaluation are
+ *      parity unpinned (Boost is absent);
  *  - Bloch equation ds/dt = 2 d(k - A(t)) x s in the lab frame
  *      (PiFlux: sources/HHG/Systems/PiFlux.cpp:89-94, prefactor 4t = 2 * 2t;
  *       honeycomb: sources/HHG/Systems/Honeycomb.cpp:130-138, omega = 2t(Re g, -Im g, 0));
+ *      the 3D Dirac cone in its band frame (DiracSystem.cpp:143-153); pinned
+ *      against scipy DOP853 on the von Neumann equation of H = d.sigma built
+ *      independently (the 3D Dirac cone against the lab-frame Weyl cone);
  *  - relaxation toward the initial state (sources/HHG/Systems/DiracSystem.cpp:231-233)
  *      and, for the 2D configurations, transverse T2 dephasing about d(k - A(t));
+ *      pinned by the closed-form decay without field;
  *  - the honeycomb relaxation toward the instantaneous equilibrium at k - A(t)
- *      (sources/HHG/Systems/Honeycomb.cpp:160-199);
+ *      (sources/HHG/Systems/Honeycomb.cpp:160-199); pinned by the thermal state
+ *      being its own target and the post-pulse closed form;
  *  - the 3D pi-flux lattice ds/dt = 4t m x s, m = (cos kx, cos ky, cos(kz - A))
  *      (sources/HHG/Systems/PiFlux.cpp:89-94), its thermal initial state
- *      (PiFlux.cpp:519-549) and its relaxation toward the instantaneous
- *      equilibrium (PiFlux.cpp:112-182, diagonal_sigma PiFlux.cpp:1037-1049);
- *  - the honeycomb relaxation toward the instantaneous equilibrium at k - A(t)
- *      (sources/HHG/Systems/Honeycomb.cpp:160-199);
- *  - the 3D pi-flux lattice ds/dt = 4t m x s, m = (cos kx, cos ky, cos(kz - A))
- *      (sources/HHG/Systems/PiFlux.cpp:89-94), its thermal initial state
- *      (PiFlux.cpp:519-549) and its relaxation toward the instantaneous
- *      equilibrium (PiFlux.cpp:112-182, diagonal_sigma PiFlux.cpp:1037-1049);
+ *      (PiFlux.cpp:519-549), its relaxation toward the instantaneous
+ *      equilibrium (PiFlux.cpp:112-182, diagonal_sigma PiFlux.cpp:1037-1049)
+ *      and the drivers' rescues of vanishing cosines (PiFlux.cpp:825-829,
+ *      978-993); pinned by |s| of the pure state, the equal-rate and T = 0
+ *      fixed points, the post-pulse closed form and the rescued point's
+ *      finiteness;
+ *  - the split current (PiFlux::current_per_energy, PiFlux.cpp:954-1035):
+ *      pinned by the halves summing to the unsplit current and by the cuts
+ *      that put everything on one side;
+ *  - the occupation run (PiFlux::evolve_occupation_numbers, PiFlux.cpp:222-318)
+ *      with its persistent shifted momentum: pinned by the closed form
+ *      sqrt((|s| -+ s.m^)/2) of the plain trajectory and by the set of points
+ *      on which the persistent rescue changes the run; the rescued cos x value
+ *      itself is pinned only through that set;
  *  - Boost.Odeint runge_kutta_fehlberg78 + make_controlled + integrate_adaptive,
  *      restarted at dt = Delta / n_subdivisions every sampling interval
  *      (sources/HHG/Systems/DiracSystem.cpp:159-173, Honeycomb.cpp:145-157,
- *       PiFlux.cpp:96-110; tolerances passed by the caller);
+ *       PiFlux.cpp:96-110; tolerances passed by the caller); the tableau by
+ *      all order conditions in exact rationals, the controller by its step
+ *      sequence on problems with known error behaviour;
  *  - the quadrature-weighted current sum j(t_i) = sum_k w_k g_k(t_i)
  *      (sources/hhg_gauss.cpp:71-133, sources/HHG/Systems/Honeycomb.cpp:225-336,
- *       sources/HHG/Systems/PiFlux.cpp:794-863);
+ *       sources/HHG/Systems/PiFlux.cpp:794-863); pinned by exact quadrature of
+ *      polynomials and the reference's own Gauss-Legendre tables;
  *  - a 4th-order (two-node Gauss) Magnus step with the closed-form SO(3)
  *      exponential, the structural analogue of the reference's Magnus path
  *      (sources/HHG/Systems/DiracSystem.cpp:176-211,
- *       sources/HHG/Systems/DiracDetail/MagnusMatrix.cpp:64-74).
+ *       sources/HHG/Systems/DiracDetail/MagnusMatrix.cpp:64-74), and the
+ *      reference's moment Magnus step (MagnusMatrix.cpp:25-74, Laser.cpp:40-55);
+ *      pinned by their observed orders (4, and 3 with the reference's moments).
  *
  * Boost.Odeint is a third-party dependency absent from the reference tree
  * (cmake/user_settings.cmake:3 points at Boost 1.78.0).  Its published
