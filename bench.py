@@ -25,6 +25,7 @@ of the same workload on the host cores.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--small]
 """
 import argparse
+import gzip
 import json
 import math
 import os
@@ -290,10 +291,15 @@ def config5_leg(H, n=256, log_n=32):
 # with 4 laser cycles, and its params/dcos.config), with the grid overrides of
 # the small audit runs.
 REF_LEGS = [
-    ("dirac3d_continuous_moment_magnus", "dirac_params.cfg", "n_z 12\nn_kappa 6\n"),
-    ("piflux3d_cos2_equilibrium_relaxation", "piflux_params.cfg", "z_range 2\nn_coarse 1\n"),
-    ("honeycomb_cos2_equilibrium_relaxation", "honeycomb_params.cfg", "N_k 6\n"),
-    ("piflux3d_dcos_spline_pulse", "piflux_dcos.cfg", "z_range 2\nn_coarse 1\n"),
+    ("dirac3d_continuous_moment_magnus", "dirac_params.cfg", "n_z 12\nn_kappa 6\n", ""),
+    ("piflux3d_cos2_equilibrium_relaxation", "piflux_params.cfg", "z_range 2\nn_coarse 1\n", ""),
+    ("honeycomb_cos2_equilibrium_relaxation", "honeycomb_params.cfg", "N_k 6\n", ""),
+    ("piflux3d_dcos_spline_pulse", "piflux_dcos.cfg", "z_range 2\nn_coarse 1\n", ""),
+    # PiFlux::current_per_energy on the uniform 32^3 x 2 grid (split_current.json.gz);
+    # the audit grid is odd: an even one holds the point where all three cosines
+    # vanish, on which the relaxation starts on the Dirac point and its two oracle
+    # builds already differ by percent
+    ("piflux3d_split_current_equilibrium", "piflux_params.cfg", "n_z 5\n", "split_current only\nn_z 32\n"),
 ]
 # Algorithmic FLOPs of the reference systems' right-hand sides (same convention):
 # pi-flux 3D 4t m x s with cos(z - A): 15; its equilibrium relaxation (|m|, two
@@ -319,20 +325,28 @@ def reference_models_leg(H, repeats=3, cpu_s=5.0):
     res = {}
     with tempfile.TemporaryDirectory() as tmp:
         def run(text, name):
-            cfg = os.path.join(tmp, name + ".cfg")
-            out = os.path.join(tmp, name + ".json")
+            # each run in its own folder: the split current is written beside the output
+            folder = os.path.join(tmp, name)
+            os.makedirs(folder)
+            cfg = os.path.join(folder, "run.cfg")
+            out = os.path.join(folder, "current_density.json")
             with open(cfg, "w") as fh:
                 fh.write(text)
             r = subprocess.run([exe, cfg, out], capture_output=True, text=True, timeout=600)
             if r.returncode != 0:
                 raise RuntimeError(r.stderr)
+            if "split_current only" in text:
+                with gzip.open(os.path.join(folder, "split_current.json.gz")) as fh:
+                    d = json.load(fh)
+                d["current_density_time"] = d["dirac_current"] + d["non_dirac_current"]
+                return d
             with open(out) as fh:
                 return json.load(fh)
 
-        for name, cfg_file, small in REF_LEGS:
+        for name, cfg_file, small, extra in REF_LEGS:
             print(f"[bench] reference model {name}", file=sys.stderr, flush=True)
             with open(os.path.join(ROOT, "higherharmonics_amd", "configs", cfg_file)) as fh:
-                base = fh.read()
+                base = fh.read() + extra
             d = run(base + f"timing_repeats {repeats}\n", name)
             runs = d["solve_ms_runs"][1:] or d["solve_ms_runs"]
             ms = float(np.mean(runs))
@@ -353,7 +367,7 @@ def reference_models_leg(H, repeats=3, cpu_s=5.0):
                                          sa["w"], kz=sa["kz"], n_threads=threads)
             j_fast, _ = O.current_density(sa["model"], sa["pulse"], sa["time"], sa["solver"], sa["kx"], sa["ky"],
                                           sa["w"], kz=sa["kz"], n_threads=threads, fast=True)
-            audit = rel_l2(a["current_density_time"], j_ref)
+            audit = rel_l2(np.asarray(a["current_density_time"]), j_ref)
             # the rounding sensitivity of the configuration itself: the oracle's
             # strict build against its -ffast-math build
             spread = rel_l2(j_fast, j_ref)
@@ -400,6 +414,8 @@ def main():
                     help="also run the honeycomb learned-vs-plain configuration (16 pulses on 768 x 768)")
     ap.add_argument("--config5", action="store_true",
                     help="also run the mixed 128-pulse sweep, plain vs learned + grouping, with a 1%% audit")
+    ap.add_argument("--reference-models-only", action="store_true",
+                    help="only the reference systems' leg (one JSON object)")
     ap.add_argument("--reference-models", action="store_true",
                     help="also run the reference's 3D Dirac, 3D pi-flux and honeycomb systems in its units")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -421,6 +437,9 @@ def main():
         sys.exit(2)
 
     import higherharmonics_amd as H
+    if args.reference_models_only:
+        print(json.dumps({"reference_models": reference_models_leg(H)}))
+        return
     wl = workload(args.small)
     K = wl["nx"] * wl["ny"]
     f_traj_rhs = F_STEP + 13 * F_RHS["dirac_t2"]

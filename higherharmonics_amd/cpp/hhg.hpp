@@ -501,7 +501,8 @@ class PiFlux : public System {
     // n_measurements + 1 samples each
     std::array<std::vector<h_float>, 2> current_per_energy(const Laser::Laser* laser,
                                                            const TimeIntegrationConfig& time_config, int N,
-                                                           hhg_stats* stats = nullptr) const {
+                                                           hhg_stats* stats = nullptr, int repeats = 1,
+                                                           std::vector<double>* solve_ms = nullptr) const {
         hhg_job_desc job{};
         job.model = model;
         job.model.split_dispersion = 0.5 * sqrt_3;
@@ -512,7 +513,13 @@ class PiFlux : public System {
         g.kind = HHG_KGRID_PIFLUX_UNIFORM;
         g.nx = N;
         Dispatch::Plan plan({g}, {job}, solver);
-        const std::vector<h_float> j = plan.compute(stats)[0];
+        std::vector<h_float> j;
+        for (int r = 0; r < std::max(1, repeats); ++r) {  // > 1: the same plan again (timing)
+            hhg_stats st{};
+            j = plan.compute(&st)[0];
+            if (solve_ms) solve_ms->push_back(st.solve_ms);
+            if (stats) *stats = st;
+        }
         const size_t n = j.size() / 2;
         return {std::vector<h_float>(j.begin(), j.begin() + (std::ptrdiff_t)n),
                 std::vector<h_float>(j.begin() + (std::ptrdiff_t)n, j.end())};

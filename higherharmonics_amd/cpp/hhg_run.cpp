@@ -332,11 +332,15 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     if (split_current != "no") {
         // PiFlux::current_per_energy on the n_z^3 x 2 uniform grid (hhg.cpp:174-219)
         hhg_stats sst{};
-        const auto sc = psys.current_per_energy(laser.get(), tc, n_z, &sst);
+        std::vector<double> sruns;
+        const auto sc = psys.current_per_energy(laser.get(), tc, n_z, &sst, (int)num(kv, "timing_repeats", 1), &sruns);
         std::ostringstream o;
         head(o, tc);
         write_array(o, "dirac_current", sc[0]);
-        write_array(o, "non_dirac_current", sc[1], true);
+        write_array(o, "non_dirac_current", sc[1]);
+        o << "  \"trajectories\": " << sst.trajectories << ", \"attempted_steps\": " << sst.attempts
+          << ", \"rejects\": " << sst.rejects << ", \"solve_ms\": " << num_str(sst.solve_ms) << ",\n";
+        write_array(o, "solve_ms_runs", sruns, true);
         o << "}\n";
         write_file(sibling(argc, argv, "split_current.json.gz"), o.str());
         write_file(sibling(argc, argv, "split_current_metadata.json.gz"), metadata_json());

@@ -16,7 +16,11 @@ never does.  Restates, from the reference's sources:
   the dispatchers (continuous [-n pi, n pi]; cos^2 from pi t0_offset, the
   lattices one cycle longer) and N = n_cycles 1024 (hhg.cpp:87-88); the
   tabulated DoubleLaser of the pi-flux lattice with the photon energy in units
-  of the experiment's (Dispatcher.cpp:10-19).
+  of the experiment's (Dispatcher.cpp:10-19); the pi-flux GaussLaser over
+  [t0_offset, t0_offset + 6 pi n] (GaussLaser.cpp:9-12, GaussLaser.hpp:13-16);
+* split_current (hhg.cpp:174-219, PiFlux.cpp:954-1035): the uniform n_z grid with
+  its rescue and the sqrt3/2 cut; with it or occupations the tabulated pulses
+  stop at 0.75 t_end (PiFluxDispatcher.cpp:44-104).
 """
 from __future__ import annotations
 
@@ -25,8 +29,9 @@ import math
 import numpy as np
 
 from . import (MODEL_DIRAC3D, MODEL_HONEYCOMB, MODEL_PIFLUX3D, METHOD_MAGNUS_MOMENTS, PULSE_CONTINUOUS, PULSE_COS2,
-               RELAX_EQUILIBRIUM, RELAX_TO_INITIAL, EXP_PHOTON_ENERGY, REF_HBAR, dirac_cylinder, gl_hex, model,
-               piflux_continuum, pulse, reference_double_laser_samples, solver, spline_pulse, time_grid)
+               PULSE_GAUSS, RELAX_EQUILIBRIUM, RELAX_TO_INITIAL, EXP_PHOTON_ENERGY, REF_HBAR, dirac_cylinder, gl_hex,
+               model, piflux_continuum, piflux_uniform, pulse, reference_double_laser_samples, solver, spline_pulse,
+               time_grid)
 
 K_B = 0.08617333262145177434  # meV / K, GlobalDefinitions.hpp:30
 
@@ -56,6 +61,8 @@ def setup(cfg: dict, grid_override: dict | None = None) -> dict:
     g = dict(cfg)
     g.update(grid_override or {})
     tabulated = laser.startswith("dcos") or laser.startswith("dgauss")
+    split = system == "PiFlux" and cfg.get("split_current", "no") != "no"
+    t_scale = 0.75 if (split or cfg.get("occupations", "no") != "no") else 1.0
     e_sys = e_in * EXP_PHOTON_ENERGY if (tabulated and system != "Dirac") else e_in
     beta = e_sys / (K_B * T) if T > 0 else 0.0
     rate = lambda t_fs: 1e15 * REF_HBAR / (t_fs * e_sys)
@@ -72,8 +79,13 @@ def setup(cfg: dict, grid_override: dict | None = None) -> dict:
             sv = solver(method=METHOD_MAGNUS_MOMENTS)
     elif system == "PiFlux":
         ratio = math.sqrt(3) * REF_HBAR * v_F / (e_sys * W)
-        kx, ky, kz, w = piflux_continuum(int(g.get("z_range", 32)), int(g.get("n_coarse", 4)))
-        kw = dict(hopping=W / math.sqrt(12), fermi_energy=E_F / e_sys, beta=beta, ic_rescue=1)
+        if split:
+            kx, ky, kz, w = piflux_uniform(int(g.get("n_z", 64)))
+            kw = dict(hopping=W / math.sqrt(12), fermi_energy=E_F / e_sys, beta=beta, ic_rescue=2,
+                      split_dispersion=0.5 * math.sqrt(3))
+        else:
+            kx, ky, kz, w = piflux_continuum(int(g.get("z_range", 32)), int(g.get("n_coarse", 4)))
+            kw = dict(hopping=W / math.sqrt(12), fermi_energy=E_F / e_sys, beta=beta, ic_rescue=1)
         if tau > 0:
             kw.update(relax=RELAX_EQUILIBRIUM, relax_time=1.0 / rate(tau),
                       relax_time_offdiag=1.0 / rate(tau_o if tau_o > 0 else tau))
@@ -100,7 +112,11 @@ def setup(cfg: dict, grid_override: dict | None = None) -> dict:
         gauss = laser.startswith("dgauss")
         active = 1 if laser.endswith("A") else (2 if laser.endswith("B") else 0)
         y, dt, t_max = reference_double_laser_samples(gauss, e_in, E0, ratio, float(n_cyc), t0_off, active)
-        p, t_b, t_e = spline_pulse(y, 0.0, dt, t_cut=t_max), 0.0, t_max
+        p, t_b, t_e = spline_pulse(y, 0.0, dt, t_cut=t_max), 0.0, t_scale * t_max
+    elif laser == "gauss" and system == "PiFlux":
+        base = 6 * math.pi * n_cyc
+        p = pulse(PULSE_GAUSS, amp_a=A0, sigma=math.pi * n_cyc / math.sqrt(2 * math.log(20.0)), t0=t0_off + 0.5 * base)
+        t_b, t_e = t0_off, t_scale * (t0_off + base)
     else:
         raise ValueError(laser)
     return dict(model=m, pulse=p, time=time_grid(t_b, t_e, N, 50), solver=sv, kx=kx, ky=ky, kz=kz, w=w,
