@@ -909,3 +909,32 @@ def test_split_current_partitions_the_current():
     assert np.abs(lo).max() > 1e-3 * np.abs(j).max() and np.abs(hi).max() > 1e-3 * np.abs(j).max()
     jall, _ = O.current_density(O.model(O.MODEL_PIFLUX3D, split_dispersion=10.0, **base), p, tc, sv, kx, ky, w, kz=kz)
     assert np.array_equal(jall[65:], np.zeros(65)) and np.max(np.abs(jall[:65] - j)) < 1e-14 * np.abs(j).max()
+
+
+def test_reference_units_setup_of_the_pi_flux_outputs():
+    """The oracle's restatement of the reference's dispatcher set-up for the pi-flux
+    outputs (PiFluxDispatcher.cpp:44-104, hhg.cpp:120-219, PiFlux.cpp:954-1035):
+    split_current selects the uniform n_z grid with its rescue and the sqrt3/2
+    cut; with split_current or occupations a tabulated pulse stops at 0.75 t_end
+    (the whole window scaled, not its length); the GaussLaser window is
+    [t0_offset, t0_offset + 6 pi n] with the centre in the middle and
+    sigma = pi n / sqrt(2 ln 20)."""
+    from oracle import reference_units as R
+    base = ("system_type PiFlux\nT 20\nE_F 1\nv_F 1.5e3\nband_width 10\nfield_amplitude 1.5\nphoton_energy 0.9\n"
+            "diagonal_relaxation_time 20\nn_laser_cycles 1\nt0_offset 0.2\n")
+    plain = R.setup(R.parse(base + "laser_type dcos\n"))
+    split = R.setup(R.parse(base + "laser_type dcos\nsplit_current only\nn_z 3\n"))
+    occ = R.setup(R.parse(base + "laser_type dcos\noccupations yes\n"))
+    assert len(split["kx"]) == 2 * 27 and split["model"].ic_rescue == 2
+    assert abs(split["model"].split_dispersion - math.sqrt(3) / 2) < 1e-16
+    assert plain["model"].split_dispersion == 0.0 and plain["model"].ic_rescue == 1
+    for d in (split, occ):
+        assert d["time"].t_begin == 0.0 and abs(d["time"].t_end - 0.75 * plain["time"].t_end) < 1e-14
+    g = R.setup(R.parse(base + "laser_type gauss\nsplit_current yes\nn_z 2\n"))
+    p = g["pulse"]
+    assert p.kind == O.PULSE_GAUSS and abs(p.t0 - (0.2 + 3 * math.pi)) < 1e-14
+    assert abs(p.sigma - math.pi / math.sqrt(2 * math.log(20.0))) < 1e-15
+    assert g["time"].t_begin == 0.2 and abs(g["time"].t_end - 0.75 * (0.2 + 6 * math.pi)) < 1e-14
+    # the cos^2 window is not scaled (PiFluxDispatcher.cpp:49-53)
+    c = R.setup(R.parse(base + "laser_type cosine\noccupations yes\n"))
+    assert abs(c["time"].t_end - (0.2 * math.pi + 4 * math.pi)) < 1e-14
