@@ -794,3 +794,32 @@ def test_cpp_driver_piflux_gauss_laser(hhg, tmp_path):
     err = rel_l2(np.array(d["current_density_time"]), j_ref)
     print(f"  rel_l2={err:.3e}", flush=True)
     assert err <= J_TOL
+
+
+@pytest.mark.parametrize("config", ["config2_dirac", "config3_piflux", "config4_honeycomb"])
+def test_full_size_field_reversal(hhg, config):
+    """BASELINE configurations 2-4 at their full grids (524,288, 262,144 and 589,824
+    k-points) through a property that holds at any size: reversing the field maps
+    k -> -k on the symmetric grids, so j(-A) = -j(A) for the Dirac and PiFlux
+    models, and j(-A) = +j(A) for the honeycomb, whose current vertex is the
+    reference's Re(G) s_x + Im(G) s_y with G = sum_d d e^{i d.(k - A)} (no factor i,
+    Honeycomb.cpp:52-61, 273-279).  Both signs run as two jobs of one plan."""
+    from higherharmonics_amd import workloads as W
+    if config == "config2_dirac":
+        job, sign = W.dirac2d()[0], -1.0
+    elif config == "config3_piflux":
+        job, sign = W.piflux_sweep(amps=(0.7333,), delays=[2 * 2 * math.pi])[0], -1.0
+    else:
+        job, sign = W.honeycomb_cos2(n_pulses=1)[0], 1.0
+    flipped = dict(job, pulse=dict(job["pulse"], amp_a=-job["pulse"]["amp_a"],
+                                   amp_b=-job["pulse"].get("amp_b", 0.0)))
+    n_k = W.trajectories([job])
+    print(f"[case] {config} full grid {job['grid'][:3]} N={job['time'][2]} k-points={n_k}", flush=True)
+    with W.build_plan(hhg, [job, flipped], hhg.Solver()) as plan:
+        j_plus, j_minus = plan.execute()
+        st = plan.stats
+    err = rel_l2(j_minus, sign * j_plus)
+    print(f"  rel_l2(j(-A), {'+' if sign > 0 else '-'}j(A)) = {err:.3e}, attempts {st.attempts}", flush=True)
+    assert st.failed_trajectories == 0 and st.nonfinite_trajectories == 0
+    assert np.abs(j_plus).max() > 1e-6
+    assert err <= 1e-12
