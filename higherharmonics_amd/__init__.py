@@ -36,7 +36,7 @@ MODEL_DIRAC2D, MODEL_PIFLUX2D, MODEL_HONEYCOMB, MODEL_DIRAC3D, MODEL_PIFLUX3D = 
 RELAX_NONE, RELAX_TO_INITIAL, RELAX_T2, RELAX_EQUILIBRIUM = 0, 1, 2, 3
 PULSE_COS2, PULSE_GAUSS, PULSE_DOUBLE_GAUSS, PULSE_DOUBLE_COS2, PULSE_CONTINUOUS, PULSE_SPLINE = 0, 1, 2, 3, 4, 5
 KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_CONTINUUM, KGRID_PIFLUX_UNIFORM = 0, 1, 2, 3, 4, 5
-KGRID_PIFLUX_OCCUPATION = 6
+KGRID_PIFLUX_OCCUPATION, KGRID_DIRAC_GAUSS = 6, 7
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
 
@@ -404,6 +404,11 @@ class KGrid:
     def dirac_cylinder(n_z, n_kappa, max_k) -> "KGrid":
         """(k_z, kappa) points of the 3D Dirac cone (kind KGRID_DIRAC_CYLINDER)."""
         return KGrid(KGRID_DIRAC_CYLINDER, n_z, n_kappa, 0.0, max_k)
+
+    @staticmethod
+    def dirac_gauss(n_z: int, n_kappa: int, max_k: float) -> "KGrid":
+        """(k_z, kappa) of the 3D Dirac cone by Gauss-Legendre in both (kind KGRID_DIRAC_GAUSS)."""
+        return KGrid(KGRID_DIRAC_GAUSS, n_z, n_kappa, 0.0, max_k)
 
     @staticmethod
     def piflux_continuum(z_range=32, n_coarse=4) -> "KGrid":

@@ -405,6 +405,11 @@ class DiracSystem : public System {
     }
     // hbar v_F / photon energy (DiracSystem.hpp:30-32)
     h_float laser_model_ratio(h_float photon_energy_) const { return hbar * v_F / photon_energy_; }
+    // Gauss-Legendre in k_z as well, the quadrature of sources/hhg_gauss.cpp:71-133
+    DiracSystem& gauss_in_z() {
+        kgrid.kind = HHG_KGRID_DIRAC_GAUSS;
+        return *this;
+    }
 };
 
 // The reference's 3D pi-flux lattice (PiFlux.hpp:25-147) in its own units:

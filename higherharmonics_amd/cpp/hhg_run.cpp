@@ -21,7 +21,8 @@
 //   pi); N = n_laser_cycles * 1024 samples (hhg.cpp:87-88).
 //   Dirac: without relaxation the moment Magnus step, else RKF78; n_z, and
 //   n_kappa (Gauss-Legendre nodes in kappa, default 32) in place of the
-//   reference's adaptive kappa trapezoid.
+//   reference's adaptive kappa trapezoid; z_rule gauss puts n_z Gauss-Legendre
+//   nodes in k_z too, the quadrature of the reference's hhg_gauss.cpp.
 //   PiFlux: RKF78, the continuum-limit grid z_range x n_coarse (32, 4), a cos^2
 //   run continues 2 pi after the pulse (PiFluxDispatcher.cpp:51-53).
 //   Honeycomb: RKF78, the hexagonal-BZ grid with N_k positive nodes per axis
@@ -229,6 +230,11 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     const bool scaled = is_exp || is_double;
     const double e_sys = (scaled && !dirac) ? photon_energy * Laser::exp_photon_energy : photon_energy;
     Systems::DiracSystem dsys(T, E_F, v_F, band_width, e_sys, tau, n_z, n_kappa);
+    const std::string z_rule = str(kv, "z_rule", "uniform");
+    if (z_rule == "gauss")
+        dsys.gauss_in_z();  // hhg_gauss.cpp's Gauss-Legendre (k_z, kappa) quadrature
+    else if (z_rule != "uniform")
+        throw std::invalid_argument("z_rule is uniform (DiracSystem.cpp:280-306) or gauss (hhg_gauss.cpp:71-133)");
     Systems::PiFlux psys(T, E_F, v_F, band_width, e_sys, tau, tau_o, z_range, n_coarse);
     Systems::Honeycomb hsys(T, E_F, v_F, band_width, e_sys, tau, N_k);
     const Systems::System& sys = dirac ? static_cast<const Systems::System&>(dsys)

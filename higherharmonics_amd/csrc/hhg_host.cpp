@@ -195,6 +195,29 @@ hhg_status kgrid_build(const hhg_kgrid_desc& g, double* kx, double* ky, double* 
         }
         return HHG_SUCCESS;
     }
+    if (g.kind == HHG_KGRID_DIRAC_GAUSS) {
+        // sources/hhg_gauss.cpp:71-133: Gauss-Legendre in k_z (x1 x_z, both signs) and in kappa
+        // (kappa_max (x_r + 1) / 2, DiracSystem.cpp:257-267), integrand weight k_z kappa / |k|
+        // (hhg_gauss.cpp:24-26), with the substitutions' Jacobians x1 and kappa_max / 2
+        if (g.nx < 1 || g.ny < 1 || !(g.x1 > 0.0))
+            return fail(HHG_ERROR_INVALID_ARGUMENT, "Dirac Gauss grid needs nx, ny >= 1, x1 (band width) > 0");
+        std::vector<double> gz(g.nx), wz(g.nx), gk(g.ny), wk(g.ny);
+        gauss_legendre(g.nx, gz.data(), wz.data());
+        gauss_legendre(g.ny, gk.data(), wk.data());
+        const double max_k = g.x1;
+        int64_t k = 0;
+        for (int z = 0; z < g.nx; ++z) {
+            const double kz = max_k * gz[z];
+            const double kmax = std::sqrt(max_k * max_k - kz * kz);
+            for (int r = 0; r < g.ny; ++r, ++k) {
+                const double ka = 0.5 * kmax * (gk[r] + 1.0);
+                kx[k] = kz;
+                ky[k] = ka;
+                w[k] = max_k * wz[z] * 0.5 * kmax * wk[r] * kz * ka / std::sqrt(kz * kz + ka * ka);
+            }
+        }
+        return HHG_SUCCESS;
+    }
     if (g.kind == HHG_KGRID_DIRAC_CYLINDER) {
         // DiracSystem::compute_current_density (DiracSystem.cpp:280-306): k_z = (z - nx/2) dk,
         // z = 1..nx-1 except nx/2, dk = 2 max_k / nx; kappa on (0, sqrt(max_k^2 - k_z^2))

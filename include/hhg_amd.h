@@ -163,9 +163,16 @@ typedef enum hhg_kgrid_kind {
                                       cos y vanish (is_zero) and cos z < 0 the reference's rescue replaces cos x by
                                       1e-4 |cos z| for the whole run and scales the initial cos z by 1 - 5e-9
                                       (PiFlux.cpp:978-986); jobs on this grid do the same */
-    HHG_KGRID_PIFLUX_OCCUPATION = 6 /* (x pi/N, pi/2, z pi/N) for i = x N + z, x, z < N = nx, weight 1/N^2: the
+    HHG_KGRID_PIFLUX_OCCUPATION = 6, /* (x pi/N, pi/2, z pi/N) for i = x N + z, x, z < N = nx, weight 1/N^2: the
                                       slice of PiFlux::compute_occupation_numbers (PiFlux.cpp:909-952), with the
                                       rescue of HHG_KGRID_PIFLUX_UNIFORM (PiFlux.cpp:935-938) */
+    HHG_KGRID_DIRAC_GAUSS = 7 /* (k_z, kappa) of the 3D Dirac cone by Gauss-Legendre in both, the quadrature of
+                                 sources/hhg_gauss.cpp:71-133: k_z = x1 x_z over the nx-point rule (both signs),
+                                 kappa = kappa_max(k_z) (x_r + 1) / 2 over the ny-point rule,
+                                 kappa_max = sqrt(x1^2 - k_z^2) (DiracSystem.cpp:257-267); weight
+                                 x1 w_z kappa_max / 2 w_r k_z kappa / |k| (hhg_gauss.cpp:24-26, 77; the Jacobians
+                                 x1 and kappa_max / 2 of its substitutions included, so that it integrates what
+                                 HHG_KGRID_DIRAC_CYLINDER does); x1 = band width */
 } hhg_kgrid_kind;
 
 typedef struct hhg_kgrid_desc {

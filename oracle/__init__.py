@@ -327,6 +327,21 @@ def reference_experimental_samples(photon_energy, E0, model_ratio, shift_ps, E_A
 # Quadrature grids (restating the reference's GL mappings)
 # --------------------------------------------------------------------------
 
+def dirac_gauss(n_z: int, n_kappa: int, max_k: float):
+    """(k_z, kappa, w) of the Gauss-Legendre (k_z, kappa) quadrature of
+    sources/hhg_gauss.cpp:71-133: k_z = max_k x_z over the n_z-point rule, kappa =
+    kappa_max (x_r + 1) / 2 over the n_kappa-point rule, kappa_max =
+    sqrt(max_k^2 - k_z^2); weight with the Jacobians max_k and kappa_max / 2 times
+    the integrand weight k_z kappa / |k| (hhg_gauss.cpp:24-26)."""
+    gz, wz = gauss_legendre(n_z)
+    gk, wk = gauss_legendre(n_kappa)
+    kz = np.repeat(max_k * gz, n_kappa)
+    kmax = np.sqrt(max_k * max_k - kz * kz)
+    ka = 0.5 * kmax * (np.tile(gk, n_z) + 1.0)
+    w = max_k * np.repeat(wz, n_kappa) * 0.5 * kmax * np.tile(wk, n_z) * kz * ka / np.sqrt(kz * kz + ka * ka)
+    return kz, ka, w
+
+
 def dirac_cylinder(n_z: int, n_kappa: int, max_k: float):
     """(k_z, kappa) points of the 3D Dirac cone with the field along z.
     k_z on the uniform grid of DiracSystem::compute_current_density

@@ -29,7 +29,8 @@ import math
 import numpy as np
 
 from . import (MODEL_DIRAC3D, MODEL_HONEYCOMB, MODEL_PIFLUX3D, METHOD_MAGNUS_MOMENTS, PULSE_CONTINUOUS, PULSE_COS2,
-               PULSE_GAUSS, RELAX_EQUILIBRIUM, RELAX_TO_INITIAL, EXP_PHOTON_ENERGY, REF_HBAR, dirac_cylinder, gl_hex,
+               PULSE_GAUSS, RELAX_EQUILIBRIUM, RELAX_TO_INITIAL, EXP_PHOTON_ENERGY, REF_HBAR, dirac_cylinder, dirac_gauss,
+               gl_hex,
                model, piflux_continuum, piflux_uniform, pulse, reference_double_laser_samples, solver, spline_pulse,
                time_grid)
 
@@ -69,7 +70,8 @@ def setup(cfg: dict, grid_override: dict | None = None) -> dict:
     sv = solver(rtol=1e-8, atol=1e-12)
     if system == "Dirac":
         ratio = REF_HBAR * v_F / e_sys
-        kx, ky, w = dirac_cylinder(int(g.get("n_z", 64)), int(g.get("n_kappa", 32)), W)
+        rule = dirac_gauss if g.get("z_rule", "uniform") == "gauss" else dirac_cylinder
+        kx, ky, w = rule(int(g.get("n_z", 64)), int(g.get("n_kappa", 32)), W)
         kz = np.zeros_like(kx)
         if tau > 0:
             m = model(MODEL_DIRAC3D, fermi_energy=E_F / e_sys, beta=beta, relax=RELAX_TO_INITIAL,

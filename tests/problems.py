@@ -59,6 +59,8 @@ class Problem:
             kx, ky, w = O.gl_box(*self.grid[1:])
         elif self.grid[0] == "cyl":
             kx, ky, w = O.dirac_cylinder(*self.grid[1:])
+        elif self.grid[0] == "dgl":
+            kx, ky, w = O.dirac_gauss(*self.grid[1:])
         else:
             kx, ky, w = O.gl_hex(*self.grid[1:])
         return kx, ky, np.zeros_like(kx), w
@@ -93,6 +95,8 @@ class Problem:
             return H.KGrid.box(*self.grid[1:])
         if self.grid[0] == "cyl":
             return H.KGrid.dirac_cylinder(*self.grid[1:])
+        if self.grid[0] == "dgl":
+            return H.KGrid.dirac_gauss(*self.grid[1:])
         if self.grid[0] == "pfc":
             return H.KGrid.piflux_continuum(*self.grid[1:])
         if self.grid[0] == "pfu":
@@ -215,6 +219,12 @@ DIRAC3D_COS2_RELAX = Problem(
     time=(0.0, 4 * PI, 512, 50),
     grid=("cyl", 12, 6, 3.0),
 )
+
+# the same two on the Gauss-Legendre (k_z, kappa) rule of hhg_gauss.cpp:71-133
+DIRAC3D_GAUSS_CONTINUOUS = Problem("dirac3d_gauss_continuous", DIRAC3D_CONTINUOUS.model, DIRAC3D_CONTINUOUS.pulse,
+                                   DIRAC3D_CONTINUOUS.time, ("dgl", 12, 8, 4.0))
+DIRAC3D_GAUSS_COS2_RELAX = Problem("dirac3d_gauss_cos2_relax", DIRAC3D_COS2_RELAX.model, DIRAC3D_COS2_RELAX.pulse,
+                                   DIRAC3D_COS2_RELAX.time, ("dgl", 10, 6, 3.0))
 
 # the reference's 3D pi-flux lattice (field along z) on its continuum-limit
 # quadrature, z_range = 4, N_coarse = 1 (1184 points)

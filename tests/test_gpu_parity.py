@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
+from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC3D_GAUSS_CONTINUOUS,
+                            DIRAC3D_GAUSS_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
                             HONEYCOMB_CORNERS, HONEYCOMB_EQUILIBRIUM, HONEYCOMB_EQUILIBRIUM_CONT, HONEYCOMB_SMALL,
                             HONEYCOMB_SPLINE,
                             PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX3D_OCCUPATION, PIFLUX3D_OCCUPATION_CONT,
@@ -226,7 +227,8 @@ def test_piflux_delay_sweep_matches_oracle(hhg):
             assert rel_l2(j, r) <= J_TOL
 
 
-@pytest.mark.parametrize("pr", [DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX], ids=lambda p: p.name)
+@pytest.mark.parametrize("pr", [DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC3D_GAUSS_CONTINUOUS,
+                                DIRAC3D_GAUSS_COS2_RELAX], ids=lambda p: p.name)
 @pytest.mark.parametrize("proposal", [0, 1], ids=["restart", "carry"])
 def test_dirac3d_rkf78_matches_oracle(hhg, pr, proposal):
     """The reference's 3D Dirac cone (band frame, field along z) with RKF78 on
@@ -241,19 +243,20 @@ def test_dirac3d_rkf78_matches_oracle(hhg, pr, proposal):
     assert abs(att - st_ref.attempts) <= 1e-3 * st_ref.attempts
 
 
-def test_dirac3d_moment_magnus_matches_oracle(hhg):
+@pytest.mark.parametrize("pr", [DIRAC3D_CONTINUOUS, DIRAC3D_GAUSS_CONTINUOUS], ids=lambda p: p.name)
+def test_dirac3d_moment_magnus_matches_oracle(hhg, pr):
     """The reference's moment-based Magnus step (one per sampling interval from
     four 30-point pulse moments) against the oracle's restatement, j(t) and
     per-k trajectories; and against RKF78 within the step's truncation error."""
-    pr = DIRAC3D_CONTINUOUS
     _print_case(pr, method="magnus_moments")
     j_ref, _ = oracle_j(pr, method=O.METHOD_MAGNUS_MOMENTS)
+    kz, ka, _ = pr.o_grid()
+    sel = np.arange(0, len(kz), 7)
     with pr.h_plan(hhg, solver=dict(method=2)) as plan:
         j = plan.execute()[0]
-        traj = plan.trajectories(np.arange(0, 112, 7))
+        traj = plan.trajectories(sel)
     assert rel_l2(j, j_ref) <= 1e-12
-    kz, ka, _ = pr.o_grid()
-    for n, k in enumerate(range(0, 112, 7)):
+    for n, k in enumerate(sel):
         s_ref, _, _ = O.trajectory(pr.o_model(), pr.o_pulse(), pr.o_time(), pr.o_solver(method=O.METHOD_MAGNUS_MOMENTS),
                                    kz[k], ka[k])
         assert np.max(np.abs(traj[n] - s_ref)) < 1e-12
@@ -350,14 +353,16 @@ def test_piflux3d_magnus4_matches_oracle(hhg):
     assert rel_l2(j, j_ref) <= 1e-12
 
 
-@pytest.mark.parametrize("tau", [-1.0, 20.0], ids=["magnus", "relaxation"])
-def test_cpp_driver_reference_units_dirac(hhg, tmp_path, tau):
+@pytest.mark.parametrize("tau,z_rule", [(-1.0, "uniform"), (20.0, "uniform"), (20.0, "gauss")],
+                         ids=["magnus", "relaxation", "relaxation-gauss-z"])
+def test_cpp_driver_reference_units_dirac(hhg, tmp_path, tau, z_rule):
     """hhg_run with the reference's own Dirac keys and units (T, E_F [meV], v_F
     [m/s], band_width, field_amplitude [MV/cm], photon_energy [meV],
     diagonal_relaxation_time [fs], continuous laser) writes the reference's
     current_density fields, gzip-compressed; j(t) equals the oracle's run of the
     converted parameters (moment Magnus without relaxation, RKF78 at the
-    reference's tolerances with it) and the spectrum is the Welch-window one."""
+    reference's tolerances with it) and the spectrum is the Welch-window one;
+    z_rule gauss takes hhg_gauss.cpp's Gauss-Legendre (k_z, kappa) rule."""
     import gzip
     import json
     import math
@@ -366,10 +371,10 @@ def test_cpp_driver_reference_units_dirac(hhg, tmp_path, tau):
     cfg = tmp_path / "dirac.cfg"
     cfg.write_text("system_type Dirac\nT 0\nE_F 0\nv_F 1.5e5\nband_width 5\nfield_amplitude 1.2\n"
                    f"photon_energy 5.25\ndiagonal_relaxation_time {tau}\nlaser_type continuous\n"
-                   "n_laser_cycles 2\nt0_offset 0\nn_z 12\nn_kappa 6\nn_measurements 512\n")
+                   f"n_laser_cycles 2\nt0_offset 0\nn_z 12\nn_kappa 6\nn_measurements 512\nz_rule {z_rule}\n")
     out = tmp_path / "current_density.json.gz"
     exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
-    print(f"[case] hhg_run Dirac reference units tau={tau}", flush=True)
+    print(f"[case] hhg_run Dirac reference units tau={tau} z_rule={z_rule}", flush=True)
     r = subprocess.run([exe, str(cfg), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     d = json.loads(gzip.open(out).read())
@@ -382,7 +387,7 @@ def test_cpp_driver_reference_units_dirac(hhg, tmp_path, tau):
     hbar = 6.582119569509065698e-13
     A0 = (hbar * 1.5e5 / 5.25) * 1e11 * 1.2 / 5.25
     assert abs(d["momentum_amplitude"] - A0) <= 1e-15 * A0
-    kz, ka, w = O.dirac_cylinder(12, 6, 5.0)
+    kz, ka, w = (O.dirac_gauss if z_rule == "gauss" else O.dirac_cylinder)(12, 6, 5.0)
     tc = O.time_grid(-2 * math.pi, 2 * math.pi, 512, 50)
     p = O.pulse(O.PULSE_CONTINUOUS, amp_a=A0)
     if tau > 0:

@@ -288,6 +288,15 @@ def test_piflux_uniform_grid_matches_oracle(hhg):
             assert np.array_equal(a, b)
 
 
+def test_dirac_gauss_grid_matches_oracle(hhg):
+    for n in ((8, 6), (9, 5), (64, 48)):
+        kx, ky, kz, w = hhg.KGrid.dirac_gauss(*n, 2.5).build3()
+        ox, oy, ow = O.dirac_gauss(*n, 2.5)
+        for a, b in ((kx, ox), (ky, oy), (w, ow)):
+            assert np.array_equal(a, b)
+        assert not np.any(kz)
+
+
 def test_piflux_occupation_grid_matches_oracle(hhg):
     for n in (1, 4, 9):
         kx, ky, kz, w = hhg.KGrid.piflux_occupation(n).build3()

@@ -938,3 +938,25 @@ def test_reference_units_setup_of_the_pi_flux_outputs():
     # the cos^2 window is not scaled (PiFluxDispatcher.cpp:49-53)
     c = R.setup(R.parse(base + "laser_type cosine\noccupations yes\n"))
     assert abs(c["time"].t_end - (0.2 * math.pi + 4 * math.pi)) < 1e-14
+
+
+def test_dirac_gauss_quadrature_converges():
+    """The Gauss-Legendre (k_z, kappa) rule of sources/hhg_gauss.cpp:71-133 with its
+    substitutions' Jacobians: over the half disk k_z^2 + kappa^2 <= W^2, kappa >= 0,
+    the integral of k_z (k_z kappa / |k|) is W^4 / 6 (polar coordinates:
+    int r^3 dr int cos^2 sin), reached at fourth order (the |k_z| kink of the inner
+    integral W - |k_z| limits Gauss-Legendre), against second order for the
+    uniform-k_z cylinder rule of DiracSystem.cpp:280-306; the weight is odd in k_z,
+    so constants integrate to zero."""
+    W = 2.5
+    exact = W ** 4 / 6
+    errs, errs_cyl = [], []
+    for n in (16, 32, 64):
+        kz, ka, w = O.dirac_gauss(n, n, W)
+        errs.append(abs(np.sum(w * kz) - exact) / exact)
+        assert abs(np.sum(w)) < 1e-14 * np.sum(np.abs(w))
+        kz, ka, w = O.dirac_cylinder(n, n, W)
+        errs_cyl.append(abs(np.sum(w * kz) - exact) / exact)
+    assert errs[-1] < 1e-6
+    assert errs[0] / errs[1] > 14 and errs[1] / errs[2] > 14
+    assert 3.5 < errs_cyl[0] / errs_cyl[1] < 4.5 and errs[-1] < errs_cyl[-1] / 100
