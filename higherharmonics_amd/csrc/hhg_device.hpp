@@ -415,6 +415,45 @@ struct Bloch {
             if constexpr (R == RELAX_EQUILIBRIUM) relax_equilibrium(cz, s, out);
             return;
         }
+        if constexpr (M != MODEL_DIRAC3D && R != RELAX_EQUILIBRIUM) {
+            // w = 2 d(k - A) with the factor 2 folded into the model's constant:
+            // (2c) x equals 2 (c x) exactly, and the T2 projection of s on w equals
+            // the one on d exactly (every product and sum scales by a power of
+            // two), so this is the general path below with up to three
+            // multiplications fewer per evaluation
+            double wx, wy, wz;
+            if constexpr (M == MODEL_DIRAC2D) {
+                const double tv = 2.0 * J.v;
+                wx = tv * (kx - A * J.ct), wy = tv * (ky - A * J.st), wz = 2.0 * J.mass;
+            } else if constexpr (M == MODEL_PIFLUX2D) {
+                wx = (4.0 * J.hop) * cos(kx - A), wy = 2.0 * cy, wz = 0.0;
+            } else {
+                double sh, ch;
+                sincos(0.5 * (kx - A), &sh, &ch);
+                const double cq = (ch - sh) * (ch + sh), sq = 2.0 * sh * ch, th = 2.0 * J.hop;
+                wx = th * (cq + 2.0 * ch * cy), wy = -th * (sq - 2.0 * sh * cy), wz = 0.0;
+            }
+            out[0] = wy * s[2] - wz * s[1];
+            out[1] = wz * s[0] - wx * s[2];
+            out[2] = wx * s[1] - wy * s[0];
+            if constexpr (R == RELAX_TO_INITIAL) {
+                const double r = J.relax_rate;
+                out[0] -= (s[0] - s0[0]) * r;
+                out[1] -= (s[1] - s0[1]) * r;
+                out[2] -= (s[2] - s0[2]) * r;
+            } else if constexpr (R == RELAX_T2) {
+                const double r = J.relax_rate;
+                const double ww = wx * wx + wy * wy + wz * wz;
+                double y = __builtin_amdgcn_rcp(ww);
+                y = fma(fma(-ww, y, 1.0), y, y);
+                y = fma(fma(-ww, y, 1.0), y, y);
+                const double proj = ww > 0.0 ? (wx * s[0] + wy * s[1] + wz * s[2]) * y : 0.0;
+                out[0] -= (s[0] - proj * wx) * r;
+                out[1] -= (s[1] - proj * wy) * r;
+                out[2] -= (s[2] - proj * wz) * r;
+            }
+            return;
+        }
         double d[3];
         d_of(A, d);
         const double wx = 2.0 * d[0], wy = 2.0 * d[1], wz = 2.0 * d[2];
