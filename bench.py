@@ -52,11 +52,11 @@ F_MLP = 2.0 * (8 * 64 + 2 * 64 * 64 + 64 * 2)
 
 # HBM bytes per solve launch of this workload from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch, scaled by 2.0 and 1.0, the factors a plain 1 GiB copy
-# reads): plain 27,404.8 x 2.0 + 1,050,496 x 1.0, learned 15,009.3 x 2.0 + 1,050,496 x 1.0;
+# reads): plain 27,889.1 x 2.0 + 1,050,496 x 1.0, learned 14,680.2 x 2.0 + 1,050,496 x 1.0;
 # profiles/solve_kernel_dirac_t2_1024x512_counters.md and
 # profiles/solve_kernel_learned_dirac_t2_1024x512_counters.md.
-TRAFFIC_BYTES = (27404.8 * 2.0 + 1050496.0 * 1.0) * 1024
-TRAFFIC_BYTES_LEARNED = (15009.3 * 2.0 + 1050496.0 * 1.0) * 1024
+TRAFFIC_BYTES = (27889.1 * 2.0 + 1050496.0 * 1.0) * 1024
+TRAFFIC_BYTES_LEARNED = (14680.2 * 2.0 + 1050496.0 * 1.0) * 1024
 
 
 def workload(small: bool):
