@@ -318,3 +318,17 @@ def test_plan_rejects_split_current_elsewhere(hhg):
     with pytest.raises(hhg.HHGError) as ei:
         hhg.Plan([job], [hhg.KGrid.piflux_uniform(2)], hhg.Solver(method=1))
     assert ei.value.status == hhg.ERROR_UNSUPPORTED
+
+
+def test_per_point_entry_points_reject_null_arguments(hhg):
+    """hhg_trajectories and hhg_occupations report a missing plan or buffer as
+    INVALID_ARGUMENT with a message, before any device work."""
+    import ctypes
+    L = hhg.lib()
+    sel = np.zeros(1, dtype=np.int64)
+    out = np.zeros(8)
+    for fn in (L.hhg_trajectories, L.hhg_occupations):
+        st = fn(None, sel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 1,
+                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        assert st == hhg.ERROR_INVALID_ARGUMENT
+        assert L.hhg_last_error_string()
