@@ -82,9 +82,9 @@ def make_plan(H, wl, method=0, relax=2, proposal=0, magnus_substeps=5, mlp=None,
 def train_proposal(H, wl):
     """The step-proposal network trained on accepted-step logs of the same
     configuration on a 256 x 128 grid of the same box (every 64th interval of
-    every k-point, about 2.1M records per round): round 1 from a
-    carried-controller solve, round 2 adds a learned-proposal solve with the
-    round-1 network."""
+    every k-point, about 2.1M records per training round): the first from a
+    carried-controller solve, the second adds a learned-proposal solve with the
+    first network."""
     from higherharmonics_amd import learn
     coarse = dict(wl, nx=256, ny=128)
     t0 = time.perf_counter()
@@ -432,7 +432,7 @@ def main():
         print(json.dumps({"copy_probe_bytes": 8 * n}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it shards)",
+        print("bench.py measures one GPU; runs on several GPUs are not supported",
               file=sys.stderr)
         sys.exit(2)
 

@@ -132,7 +132,7 @@ def rel_l2(a, b):
 PI = math.pi
 T1 = 16 * PI  # 8 cycles
 
-# BASELINE configuration 1 (full): Dirac-2D, v=1, Delta=0.1, cos^2 A0=0.5, 8 cycles,
+# Configuration 1 (full): Dirac-2D, v=1, Delta=0.1, cos^2 A0=0.5, 8 cycles,
 # T2 = 10 cycles, GL 128 x 64 on |k| <= 3, RKF78 1e-10, 4096 samples.
 CONFIG1 = Problem(
     "config1_dirac_128x64",

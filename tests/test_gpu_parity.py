@@ -56,7 +56,7 @@ def test_trajectories_match_oracle(hhg, pr):
 
 
 def test_config1_full_grid(hhg):
-    """BASELINE configuration 1 in full (8192 k-points, 4096 samples)."""
+    """Configuration 1 in full: Dirac-2D on the 128 x 64 grid (8192 k-points, 4096 samples)."""
     pr = CONFIG1
     _print_case(pr)
     j_ref, st_ref = oracle_j(pr, n_threads=16)
@@ -803,8 +803,9 @@ def test_cpp_driver_piflux_gauss_laser(hhg, tmp_path):
 
 @pytest.mark.parametrize("config", ["config2_dirac", "config3_piflux", "config4_honeycomb"])
 def test_full_size_field_reversal(hhg, config):
-    """BASELINE configurations 2-4 at their full grids (524,288, 262,144 and 589,824
-    k-points) through a property that holds at any size: reversing the field maps
+    """Configurations 2-4 at full size (Dirac-2D 1024 x 512, PiFlux-2D 512 x 512 and
+    honeycomb 768 x 768: 524,288, 262,144 and 589,824 k-points) through a property
+    that holds at any size: reversing the field maps
     k -> -k on the symmetric grids, so j(-A) = -j(A) for the Dirac and PiFlux
     models, and j(-A) = +j(A) for the honeycomb, whose current vertex is the
     reference's Re(G) s_x + Im(G) s_y with G = sum_d d e^{i d.(k - A)} (no factor i,
