@@ -1,6 +1,6 @@
 """CPU oracle for the HigherHarmonics Bloch hot path.
 
-TEST INFRASTRUCTURE ONLY.  This package is the checker against which the HIP
+TEST INFRASTRUCTURE ONLY.  This package is the oracle against which the HIP
 product path is compared, and the CPU baseline that bench.py times.  Only
 ``tests/``, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
 import it; product code never does.

@@ -1,8 +1,8 @@
 /*
  * hhg_oracle.c — CPU restatement of the HigherHarmonics per-k-point Bloch solve.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline
- * that bench.py times.  Nothing in the product links or loads this file.
+ * TEST INFRASTRUCTURE ONLY: the oracle the HIP path is compared with, and the
+ * CPU baseline that bench.py times.  Nothing in the product links or loads this file.
  *
  * What it restates (reference paths relative to the reference repository),
  * and what pins each part (tests/test_oracle.py, tests/test_host_abi.py):

@@ -2,8 +2,8 @@
  * hhg_oracle.h — CPU restatement of the HigherHarmonics per-k-point Bloch
  * solve and current reduction.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
- * path and the CPU baseline that bench.py times.  Only tests/, the smoke
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP product path
+ * is compared with, and the CPU baseline that bench.py times.  Only tests/, the smoke
  * entry point and bench.py's CPU-baseline leg load it.  Product code never
  * links, loads or calls it.
  *

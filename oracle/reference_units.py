@@ -1,6 +1,6 @@
 """The reference's configuration keys and units, restated for the oracle.
 
-TEST INFRASTRUCTURE ONLY (the checker side of the reference-units driver):
+TEST INFRASTRUCTURE ONLY (the oracle's side of the reference-units driver):
 ``tests/`` and bench.py's CPU-baseline and audit legs use it; product code
 never does.  Restates, from the reference's sources:
 
