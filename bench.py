@@ -148,6 +148,33 @@ def cpu_baseline(wl, target_s=10.0):
                       f"oracle -O3 -ffast-math build, OpenMP {threads} threads, {wall:.1f} s"}
 
 
+def config1_full(H, threads=None):
+    """The Dirac-2D workload on the 128 x 64 grid (8,192 trajectories), the
+    reference's own CPU-sized case, in full on both sides: the plain GPU kernel
+    (mean of three executions after one warmup) and the oracle's optimised build
+    on the host cores, with j(t) compared between them."""
+    import oracle as O
+    wl = dict(workload(False), nx=128, ny=64)
+    plan = make_plan(H, wl)
+    _, ms, st, j_gpu = time_plan(plan, 3, 1)
+    plan.close()
+    T = wl["T"]
+    kx, ky, w = O.gl_box(wl["nx"], wl["ny"], -3.0, 3.0, -3.0, 3.0)
+    m = O.model(O.MODEL_DIRAC2D, v=wl["v"], mass=wl["mass"], relax=O.RELAX_T2, relax_time=wl["T2"])
+    p = O.pulse(O.PULSE_COS2, wl["A0"], n_cycles=wl["n_cycles"], t0=-T / 2)
+    tc = O.time_grid(-T / 2, T / 2, wl["N"], wl["n_sub"])
+    sv = O.solver(rtol=wl["rtol"], atol=wl["atol"])
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    j_cpu, _ = O.current_density(m, p, tc, sv, kx, ky, w, n_threads=threads, fast=True)
+    cpu_s = time.perf_counter() - t0
+    K = wl["nx"] * wl["ny"]
+    return {"trajectories": K, "gpu_plain_kernel_ms": ms, "gpu_trajectories_per_s": K / (ms * 1e-3),
+            "gpu_attempts": st["attempts"], "cpu_s": cpu_s, "cpu_trajectories_per_s": K / cpu_s,
+            "cpu_threads": threads, "gpu_over_cpu": (K / (ms * 1e-3)) / (K / cpu_s),
+            "rel_l2_gpu_vs_cpu": rel_l2(j_gpu, j_cpu)}
+
+
 def sweep_extras(H):
     """Reduced-size sweeps of configurations 3-5 (same structure, fewer
     k-points per job): one plan each, autotuned, graph-captured, timed once
@@ -565,6 +592,8 @@ def main():
         out["reference_models"] = reference_models_leg(H)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
+        print("[bench] 128 x 64 grid in full on GPU and CPU", file=sys.stderr, flush=True)
+        out["config1_full"] = config1_full(H)
     print(json.dumps(out))
 
 
