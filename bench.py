@@ -203,6 +203,35 @@ def sweep_extras(H):
     return res
 
 
+def config3_full_leg(H):
+    """Configuration 3 at full size: PiFlux-2D on the 512 x 512 grid, 32 A/B
+    delays x 4 amplitudes of the double Gaussian (33.5M trajectories, 10,405 to
+    15,360 samples per job), plain RKF78, one persistent launch over all 128 jobs;
+    one execution (no warmup: the run takes tens of seconds), the launch shape
+    the autotuner picks on the reduced sweep of the same jobs."""
+    from higherharmonics_amd import workloads as W
+    small = W.piflux_sweep(n=128)
+    plan = W.build_plan(H, small, persistent=1)
+    tuned = plan.autotune()
+    plan.close()
+    jobs = W.piflux_sweep()
+    K = W.trajectories(jobs)
+    print(f"[bench] configuration 3 in full: {len(jobs)} jobs, {K} trajectories, launch {tuned}", file=sys.stderr,
+          flush=True)
+    plan = W.build_plan(H, jobs, persistent=1, waves_per_block=tuned["waves_per_block"],
+                        queue_chunk=tuned["queue_chunk"])
+    t0 = time.perf_counter()
+    plan.execute()
+    wall = time.perf_counter() - t0
+    st = plan.stats.as_dict()
+    plan.close()
+    return {"jobs": len(jobs), "trajectories": K,
+            "samples_per_job": [min(j["time"][2] for j in jobs), max(j["time"][2] for j in jobs)],
+            "trajectories_per_s": K / wall, "wall_s": wall, "solve_ms": st["solve_ms"], "attempts": st["attempts"],
+            "rejects": st["rejects"], "fp64_tflops": st["attempts"] * (F_STEP + 13 * F_RHS["piflux"]) /
+            (st["solve_ms"] * 1e-3) / 1e12, "launch": tuned}
+
+
 def config4_leg(H, n=768, n_pulses=16, log_n=96):
     """Honeycomb over the hexagonal BZ (GL n x n), n_pulses seeded cos^2 pulses
     (10 cycles, A0 ~ U[0.1, 1.5], CEP ~ U[0, 2 pi]), 512 samples per cycle:
@@ -437,6 +466,8 @@ def main():
     ap.add_argument("--infer-every", type=int, default=1,
                     help="learned leg: network proposal every this many sampling intervals")
     ap.add_argument("--sweeps", action="store_true", help="also time reduced configurations 3-5")
+    ap.add_argument("--config3-full", action="store_true",
+                    help="only configuration 3 at full size (33.5M trajectories, one launch; one JSON object)")
     ap.add_argument("--config4", action="store_true",
                     help="also run the honeycomb learned-vs-plain configuration (16 pulses on 768 x 768)")
     ap.add_argument("--config5", action="store_true",
@@ -466,6 +497,9 @@ def main():
     import higherharmonics_amd as H
     if args.reference_models_only:
         print(json.dumps({"reference_models": reference_models_leg(H)}))
+        return
+    if args.config3_full:
+        print(json.dumps({"config3_full": config3_full_leg(H)}))
         return
     wl = workload(args.small)
     K = wl["nx"] * wl["ny"]
