@@ -123,7 +123,7 @@ EXPORTED_SYMBOLS = [
     "hhg_mlp_create", "hhg_mlp_destroy", "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params",
     "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
     "hhg_spline_coefficients", "hhg_reference_double_laser", "hhg_reference_experimental_laser_size",
-    "hhg_reference_experimental_laser", "hhg_occupations",
+    "hhg_reference_experimental_laser", "hhg_occupations", "hhg_plan_job_failures",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -153,6 +153,7 @@ def lib() -> ctypes.CDLL:
     L.hhg_plan_destroy.argtypes = [P]
     L.hhg_trajectories.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
     L.hhg_occupations.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
+    L.hhg_plan_job_failures.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.hhg_gauss_legendre.argtypes = [ctypes.c_int32, _DP, _DP]
     L.hhg_kgrid_size.restype = ctypes.c_int64
     L.hhg_kgrid_size.argtypes = [ctypes.POINTER(KGridDesc)]
@@ -195,7 +196,8 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).restype = ctypes.c_int
     for name in ("hhg_plan_create", "hhg_plan_job_offsets", "hhg_plan_execute", "hhg_plan_execute_accumulators",
                  "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
-                 "hhg_trajectories", "hhg_occupations", "hhg_gauss_legendre", "hhg_kgrid_build", "hhg_pulse_evaluate",
+                 "hhg_trajectories", "hhg_occupations", "hhg_plan_job_failures", "hhg_gauss_legendre",
+                 "hhg_kgrid_build", "hhg_pulse_evaluate",
                  "hhg_trapezoidal_spectrum", "hhg_welch_spectrum"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
@@ -633,6 +635,14 @@ class Plan:
         _check(lib().hhg_trajectories(self._h, sel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(sel),
                                       _dp(out)))
         return out
+
+    def job_failures(self):
+        """(failed, nonfinite) trajectory counts per job of the last execution."""
+        f = np.zeros(self.n_jobs, dtype=np.int64)
+        n = np.zeros(self.n_jobs, dtype=np.int64)
+        _check(lib().hhg_plan_job_failures(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                           n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return f, n
 
     def occupations(self, k_index) -> np.ndarray:
         """Band occupations [n_sel][N+1][2] = (lower, upper) of job 0 (3D pi-flux

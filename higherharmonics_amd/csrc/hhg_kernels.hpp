@@ -32,6 +32,7 @@ struct KernelArgs {
     int log_stride;
     int pad;
     SolverDev sp;
+    unsigned long long* job_fail;  // per job: failed, non-finite trajectories (nullptr: not counted)
 };
 
 struct TrajArgs {

@@ -79,6 +79,8 @@ struct hhg_plan {
     int64_t n_counter_blocks() const { return std::max<int64_t>(1, (int64_t)groups.size()); }
     hhg::Counters* counters(int g) const { return reinterpret_cast<hhg::Counters*>(d_words) + g; }
     unsigned long long* acc() const { return d_words + n_counter_blocks() * kCounterWords; }
+    // per job: failed and non-finite trajectories of the last execution, after the accumulators
+    unsigned long long* job_fail() const { return acc() + 2 * n_out; }
 };
 
 namespace {
@@ -290,6 +292,7 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.nom = p->d_nom;
         a.acc = p->acc();
         a.ctr = p->counters((int)gi);
+        a.job_fail = p->job_fail();
         a.perm = p->d_perm;
         a.mlp = p->d_mlp;
         a.infer_every = p->solver.mlp_infer_every > 0 ? p->solver.mlp_infer_every : 1;
@@ -485,7 +488,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         return bail(fail(HHG_ERROR_HIP, "stream creation failed"));
     for (auto& e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(HHG_ERROR_HIP, "event creation failed"));
-    p->n_words = p->n_counter_blocks() * kCounterWords + 2 * p->n_out;
+    p->n_words = p->n_counter_blocks() * kCounterWords + 2 * p->n_out + 2 * (int64_t)d->n_jobs;
     const size_t nk = sizeof(double) * std::max<int64_t>(1, p->n_kpts);
     hhg_status st = HHG_SUCCESS;
     auto A = [&](void** ptr, size_t bytes, const char* what) {
@@ -777,6 +780,22 @@ hhg_status hhg_trajectories(hhg_plan* p, const int64_t* k_index, int64_t n_sel, 
     hipFree(d_sel);
     hipFree(d_s);
     return st;
+}
+
+hhg_status hhg_plan_job_failures(hhg_plan* p, int64_t* failed, int64_t* nonfinite) {
+    hhg::clear_error();
+    if (!p || !failed || !nonfinite) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_job_failures: null argument");
+    const int n_jobs = (int)p->jobs.size();
+    std::vector<unsigned long long> w(2 * (size_t)n_jobs);
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    HHG_TRY(hipStreamSynchronize(p->stream), "synchronize");
+    HHG_TRY(hipMemcpy(w.data(), p->job_fail(), sizeof(unsigned long long) * w.size(), hipMemcpyDeviceToHost),
+            "job failure counts");
+    for (int j = 0; j < n_jobs; ++j) {
+        failed[j] = (int64_t)w[2 * j];
+        nonfinite[j] = (int64_t)w[2 * j + 1];
+    }
+    return HHG_SUCCESS;
 }
 
 hhg_status hhg_occupations(hhg_plan* p, const int64_t* k_index, int64_t n_sel, double* occ_out) {

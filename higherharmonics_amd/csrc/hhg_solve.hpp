@@ -335,12 +335,18 @@ solve_kernel(KernelArgs a) {
         const int end = min(base + a.queue_chunk, a.n_items);
         for (int it = base; it < end; ++it) {
             const int2 item = load_uniform(a.items + it);
+            const uint32_t fail0 = lc.fail, nonfin0 = lc.nonfin;
             if constexpr (METHOD == METHOD_RKF78)
                 solve_item_rkf78<M, P, R, LEARNED>(job_ref(a.jobs + item.x), item.y, a, lc);
             else if constexpr (METHOD == METHOD_MAGNUS4)
                 solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc);
             else
                 solve_item_magnus_moments<M>(job_ref(a.jobs + item.x), item.y, a, lc);
+            if (a.job_fail) {  // the item's failures, attributed to its job
+                const uint32_t df = wave_sum_u32(lc.fail - fail0), dn = wave_sum_u32(lc.nonfin - nonfin0);
+                if (lane == 0 && df) atomicAdd(a.job_fail + 2 * item.x, (unsigned long long)df);
+                if (lane == 0 && dn) atomicAdd(a.job_fail + 2 * item.x + 1, (unsigned long long)dn);
+            }
         }
     }
     flush_counters(a.ctr, lc);

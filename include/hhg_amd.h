@@ -298,6 +298,13 @@ hhg_status hhg_plan_capture_graph(hhg_plan* plan);
 
 hhg_status hhg_plan_destroy(hhg_plan* plan);
 
+/* Per-job counts of the last execution: failed[j] trajectories whose controller
+ * gave up (HHG_ERROR_TOLERANCE) and nonfinite[j] whose state or sum became
+ * non-finite, for j < n_jobs (the per-pulse counts behind the plan-wide totals
+ * of hhg_stats; the reference throws from odeint on the first such
+ * trajectory, DiracSystem.cpp:167-173). */
+hhg_status hhg_plan_job_failures(hhg_plan* plan, int64_t* failed, int64_t* nonfinite);
+
 /* Per-k Bloch vectors at every sample of job 0 for the listed k indices of
  * its k-grid: s_out[n_sel][N+1][3].  Replaces time_evolution_sigma /
  * time_evolution_magnus for inspection and tests. */

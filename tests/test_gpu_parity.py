@@ -151,6 +151,18 @@ def test_controller_failure_is_reported(hhg):
         assert ei.value.status == hhg.ERROR_TOLERANCE
         assert plan.stats.failed_trajectories == 24 * 16
         assert plan.stats.nonfinite_trajectories == 0
+        failed, nonfinite = plan.job_failures()
+        assert list(failed) == [24 * 16] and list(nonfinite) == [0]
+    # per-job attribution: beside it a job without field, whose thermal state is
+    # stationary (zero error estimate), fails nowhere
+    quiet = Problem("no_field", pr.model, dict(pr.pulse, amp_a=0.0), pr.time, ("box", 8, 8, -3.0, 3.0, -3.0, 3.0))
+    with hhg.Plan([hhg.Job(pr.h_model(hhg), pr.h_laser(hhg), pr.h_time(hhg), 0),
+                   hhg.Job(quiet.h_model(hhg), quiet.h_laser(hhg), quiet.h_time(hhg), 1)],
+                  [pr.h_grid(hhg), quiet.h_grid(hhg)], pr.h_solver(hhg)) as plan:
+        with pytest.raises(hhg.HHGError):
+            plan.execute()
+        failed, nonfinite = plan.job_failures()
+        assert list(failed) == [24 * 16, 0] and list(nonfinite) == [0, 0]
 
 
 def test_cpp_driver_matches_python_path(hhg, tmp_path):
