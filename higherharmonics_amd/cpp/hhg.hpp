@@ -282,6 +282,14 @@ class Plan {
         return out;
     }
 
+    // failed and non-finite trajectories per job of the last compute()
+    std::pair<std::vector<int64_t>, std::vector<int64_t>> job_failures() const {
+        const size_t n = offsets_.size() - 1;
+        std::vector<int64_t> failed(n), nonfinite(n);
+        check(hhg_plan_job_failures(plan_.get(), failed.data(), nonfinite.data()));
+        return {failed, nonfinite};
+    }
+
     void autotune() { check(hhg_plan_autotune(plan_.get(), nullptr)); }
     void capture_graph() { check(hhg_plan_capture_graph(plan_.get())); }
 
