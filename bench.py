@@ -57,6 +57,10 @@ F_MLP = 2.0 * (8 * 64 + 2 * 64 * 64 + 64 * 2)
 # profiles/solve_kernel_learned_dirac_t2_1024x512_counters.md.
 TRAFFIC_BYTES = (27889.1 * 2.0 + 1050496.0 * 1.0) * 1024
 TRAFFIC_BYTES_LEARNED = (14680.2 * 2.0 + 1050496.0 * 1.0) * 1024
+# Hardware FP64 FLOP per solve launch from the same counter runs (SQ_INSTS_VALU_FMA_F64 x 2 +
+# MUL + ADD + TRANS, x 64 lanes): the rocprof view of the FP64 rate beside the algorithmic one.
+HW_FP64_FLOP = (2 * 64123772928 + 15535849472 + 4631150592 + 2147508224) * 64
+HW_FP64_FLOP_LEARNED = (2 * 16208044032 + 4093788160 + 1477033984 + 536895488) * 64
 
 
 def workload(small: bool):
@@ -603,6 +607,8 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": (TRAFFIC_BYTES_LEARNED if use_learned else TRAFFIC_BYTES) if not args.small else None,
                      "kernel_ms": solve_ms, "algorithmic_flop_per_launch": flops,
+                     "hardware_fp64_tflops": None if args.small else
+                     (HW_FP64_FLOP_LEARNED if use_learned else HW_FP64_FLOP) / (solve_ms * 1e-3) / 1e12,
                      "kernel": "learned-proposal solve" if use_learned else "plain solve"},
     }
     if args.extras:
