@@ -589,9 +589,19 @@ __device__ __forceinline__ void stage_pulses(F& f, double t, double h, const dou
 
 // Like odeint's try_step for a non-FSAL stepper, each attempt evaluates the
 // derivative at its start, so nothing but the state crosses attempts.
+// Below this ratio of |err_i| to its scale every component's error ratio rounds
+// to at most the controller's growth threshold 5^-8 (the factor 1 - 2^-50 covers
+// the rounding of the product and of the quotient).
+constexpr double kTinyErrRatio = (1.0 / 390625.0) * (1.0 - 0x1p-50);
+
+// skip_tiny: when every component has scale >= 2^-1000 (so scale * ratio is a
+// normal number) and |err_i| < kTinyErrRatio * scale, return 0 without the
+// divisions; the controller then takes the same branch (accept, largest growth)
+// as for the exact ratio, which only step logs and network features would read.
+// NaN, infinite and zero-scale cases fail the test and take the exact path.
 template <class F>
 __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double h, const double As[13],
-                                                double snew[3], double rtol, double atol) {
+                                                double snew[3], double rtol, double atol, bool skip_tiny = false) {
     double k[13][3];
     f.begin_step(s);
     f.rhs_A(s, As[0], k[0]);
@@ -609,8 +619,8 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
         }
         f.rhs_A(x, As[i], k[i]);
     }
-    double err = 0.0;
-    bool nan = false;
+    double e[3], sc[3];
+    bool tiny = skip_tiny;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         double acc = 0.0;
@@ -618,9 +628,16 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
         for (int j = 0; j < 13; ++j)
             if (F78::b8[j] != 0.0) acc = fma(F78::b8[j], k[j][q], acc);
         snew[q] = fma(h, acc, s[q]);
-        const double e = h * (41.0 / 840.0) * ((k[0][q] + k[10][q]) - (k[11][q] + k[12][q]));
-        const double sc = atol + rtol * (fabs(s[q]) + fabs(h) * fabs(k1[q]));
-        const double r = fabs(e) / sc;
+        e[q] = h * (41.0 / 840.0) * ((k[0][q] + k[10][q]) - (k[11][q] + k[12][q]));
+        sc[q] = atol + rtol * (fabs(s[q]) + fabs(h) * fabs(k1[q]));
+        tiny = tiny && sc[q] >= 0x1p-1000 && fabs(e[q]) < sc[q] * kTinyErrRatio;
+    }
+    if (tiny) return 0.0;
+    double err = 0.0;
+    bool nan = false;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double r = fabs(e[q]) / sc[q];
         nan |= (r != r);
         err = r > err ? r : err;
     }
@@ -632,6 +649,7 @@ struct IntervalCtl {
     int fallback_after;  // rejections of the first step after which it restarts at h_fallback (0: never)
     double h_fallback;   // the plain controller's restart step Delta / n_sub
     bool plain_start;    // this interval starts at h_fallback as the plain controller
+    bool skip_tiny_err;  // the error ratio's value is not read (no step log, no network): see rkf78_attempt
 };
 struct IntervalOut {
     double h_acc;        // size of the first accepted step of the interval (0: none yet)
@@ -684,7 +702,7 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
         double As[13];
         stage_pulses(f, t, h, cand, As);
         double snew[3];
-        const double err = rkf78_attempt(f, s, h, As, snew, sp.rtol, sp.atol);
+        const double err = rkf78_attempt(f, s, h, As, snew, sp.rtol, sp.atol, ctl.skip_tiny_err);
         ++n_att;
         ++io.attempts;
         if (err > 1.0) {

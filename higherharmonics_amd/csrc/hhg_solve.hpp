@@ -160,10 +160,10 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const bool restart = a.sp.proposal == PROPOSAL_RESTART;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{LEARNED ? a.fallback_after : 0, h_restart, false};
+    const bool logging = a.log != nullptr;
+    IntervalCtl ctl{LEARNED ? a.fallback_after : 0, h_restart, false, !LEARNED && !logging};
     IntervalOut io{};
     io.last_err = 1e-3;
-    const bool logging = a.log != nullptr;
     int next_infer = 1;  // interval of the next network proposal
     int kq = 0;          // next envelope corner
     for (int i = 1; i <= N; ++i) {
