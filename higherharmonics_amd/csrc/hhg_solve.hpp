@@ -375,7 +375,7 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
     int status = 0;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{0, h_restart, false};
+    IntervalCtl ctl{0, h_restart, false, true};  // nothing here reads the error ratio
     IntervalOut io{};
     int kq = 0;
     for (int i = 1; i <= N; ++i) {
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(64) void occupation_kernel(const JobDev* jobs, cons
     int status = 0;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{0, h_restart, false};
+    IntervalCtl ctl{0, h_restart, false, true};  // nothing here reads the error ratio
     IntervalOut io{};
     int kq = 0;
     for (int i = 1; i <= N; ++i) {
