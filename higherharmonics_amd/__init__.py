@@ -39,6 +39,7 @@ KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_C
 KGRID_PIFLUX_OCCUPATION, KGRID_DIRAC_GAUSS = 6, 7
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
+INFER_GATED, INFER_SCHEDULED = 0, 1
 
 _STATUS_NAMES = {0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "TOLERANCE", 4: "OUT_OF_MEMORY",
                  5: "UNSUPPORTED", 6: "NO_DEVICE", 7: "NONFINITE"}
@@ -88,12 +89,14 @@ class JobDesc(ctypes.Structure):
 class SolverDesc(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("proposal", ctypes.c_int32), ("rtol", ctypes.c_double),
                 ("atol", ctypes.c_double), ("max_fail", ctypes.c_int32), ("magnus_substeps", ctypes.c_int32),
-                ("mlp", ctypes.c_void_p), ("mlp_infer_every", ctypes.c_int32), ("fallback_after", ctypes.c_int32)]
+                ("mlp", ctypes.c_void_p), ("mlp_infer_every", ctypes.c_int32), ("fallback_after", ctypes.c_int32),
+                ("max_steps", ctypes.c_int32), ("infer_policy", ctypes.c_int32)]
 
 
 class LaunchDesc(ctypes.Structure):
     _fields_ = [("waves_per_block", ctypes.c_int32), ("queue_chunk", ctypes.c_int32),
-                ("persistent", ctypes.c_int32), ("group_by_cost", ctypes.c_int32)]
+                ("persistent", ctypes.c_int32), ("group_by_cost", ctypes.c_int32),
+                ("count_wave_attempts", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class PlanDesc(ctypes.Structure):
@@ -106,7 +109,8 @@ class PlanDesc(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("trajectories", ctypes.c_int64), ("attempts", ctypes.c_int64), ("rejects", ctypes.c_int64),
                 ("failed_trajectories", ctypes.c_int64), ("nonfinite_trajectories", ctypes.c_int64),
-                ("proposal_fallbacks", ctypes.c_int64), ("solve_ms", ctypes.c_double),
+                ("proposal_fallbacks", ctypes.c_int64), ("wave_attempts", ctypes.c_int64),
+                ("accumulator_overflows", ctypes.c_int64), ("solve_ms", ctypes.c_double),
                 ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
     def as_dict(self) -> dict:
@@ -123,7 +127,7 @@ EXPORTED_SYMBOLS = [
     "hhg_mlp_create", "hhg_mlp_destroy", "hhg_mlp_set_normalization", "hhg_mlp_get_params", "hhg_mlp_set_params",
     "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
     "hhg_spline_coefficients", "hhg_reference_double_laser", "hhg_reference_experimental_laser_size",
-    "hhg_reference_experimental_laser", "hhg_occupations", "hhg_plan_job_failures",
+    "hhg_reference_experimental_laser", "hhg_occupations", "hhg_plan_job_failures", "hhg_plan_last_stats",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -148,6 +152,7 @@ def lib() -> ctypes.CDLL:
     L.hhg_plan_execute.argtypes = [P, _DP, ctypes.POINTER(Stats)]
     L.hhg_plan_execute_accumulators.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]
     L.hhg_plan_execute_device.argtypes = [P, _DP, P]
+    L.hhg_plan_last_stats.argtypes = [P, ctypes.POINTER(Stats)]
     L.hhg_plan_autotune.argtypes = [P, ctypes.POINTER(LaunchDesc)]
     L.hhg_plan_capture_graph.argtypes = [P]
     L.hhg_plan_destroy.argtypes = [P]
@@ -195,7 +200,7 @@ def lib() -> ctypes.CDLL:
                  "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train"):
         getattr(L, name).restype = ctypes.c_int
     for name in ("hhg_plan_create", "hhg_plan_job_offsets", "hhg_plan_execute", "hhg_plan_execute_accumulators",
-                 "hhg_plan_execute_device", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
+                 "hhg_plan_execute_device", "hhg_plan_last_stats", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
                  "hhg_trajectories", "hhg_occupations", "hhg_plan_job_failures", "hhg_gauss_legendre",
                  "hhg_kgrid_build", "hhg_pulse_evaluate",
                  "hhg_trapezoidal_spectrum", "hhg_welch_spectrum"):
@@ -549,10 +554,13 @@ class Solver:
     mlp: Optional["MLP"] = None
     infer_every: int = 1
     fallback_after: int = 0
+    max_steps: int = 0
+    infer_policy: int = INFER_GATED
 
     def desc(self) -> SolverDesc:
         return SolverDesc(self.method, self.proposal, self.rtol, self.atol, self.max_fail, self.magnus_substeps,
-                          self.mlp.handle if self.mlp is not None else None, self.infer_every, self.fallback_after)
+                          self.mlp.handle if self.mlp is not None else None, self.infer_every, self.fallback_after,
+                          self.max_steps, self.infer_policy)
 
 
 @dataclass
@@ -570,7 +578,7 @@ class Plan:
 
     def __init__(self, jobs: Sequence[Job], kgrids: Sequence[KGrid], solver: Solver = Solver(),
                  rank: int = 0, n_ranks: int = 1, device: int = 0, waves_per_block: int = 0,
-                 queue_chunk: int = 0, persistent: int = 1, group_by_cost: int = 0):
+                 queue_chunk: int = 0, persistent: int = 1, group_by_cost: int = 0, count_wave_attempts: int = 0):
         L = lib()
         self._solver = solver  # keeps the network alive as long as the plan
         self._kgrids = list(kgrids)  # keep explicit arrays alive
@@ -578,7 +586,8 @@ class Plan:
         jd = (JobDesc * len(jobs))(*[JobDesc(j.model, j.laser.desc(), j.time_config.desc(), j.kgrid, 0)
                                      for j in jobs])
         d = PlanDesc(len(kgrids), len(jobs), kg, jd, solver.desc(),
-                     LaunchDesc(waves_per_block, queue_chunk, persistent, group_by_cost), rank, n_ranks, device, 0)
+                     LaunchDesc(waves_per_block, queue_chunk, persistent, group_by_cost, count_wave_attempts, 0),
+                     rank, n_ranks, device, 0)
         h = ctypes.c_void_p()
         _check(L.hhg_plan_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -599,6 +608,17 @@ class Plan:
     def execute(self, allow_failures: bool = False):
         j = self.execute_flat(allow_failures)
         return [j[self.offsets[i]:self.offsets[i + 1]] for i in range(self.n_jobs)]
+
+    def execute_device(self, j_dev_ptr: int, stream: int = 0) -> None:
+        """Enqueues a solve that leaves j in device memory at j_dev_ptr (n_out doubles)."""
+        _check(lib().hhg_plan_execute_device(self._h, ctypes.cast(j_dev_ptr, _DP), stream or None))
+
+    def last_stats(self, allow_failures: bool = False) -> "Stats":
+        """Waits for the last execution and returns its statistics (raises on its failures)."""
+        st = Stats()
+        allow = (ERROR_TOLERANCE, ERROR_NONFINITE) if allow_failures else ()
+        self.status = _check(lib().hhg_plan_last_stats(self._h, ctypes.byref(st)), allow)
+        return st
 
     def execute_accumulators(self) -> np.ndarray:
         acc = np.empty(2 * self.n_out, dtype=np.uint64)

@@ -229,6 +229,8 @@ struct Solver {
     const hhg_mlp* mlp = nullptr;
     int mlp_infer_every = 1;
     int fallback_after = 0;
+    int max_steps = 0;  // accepted steps per sampling interval before a trajectory fails (0: the library's)
+    int infer_policy = HHG_INFER_GATED;
     hhg_solver_desc desc() const {
         hhg_solver_desc d{};
         d.method = method;
@@ -240,6 +242,8 @@ struct Solver {
         d.mlp = mlp;
         d.mlp_infer_every = mlp_infer_every;
         d.fallback_after = fallback_after;
+        d.max_steps = max_steps;
+        d.infer_policy = infer_policy;
         return d;
     }
 };

@@ -182,8 +182,12 @@ constexpr double kRescue = 1e-4;  // RESCUE_TRAFO, PiFlux.cpp:50
 // its rescued cos x stays once set.
 template <int M, int P, int R, bool STICKY = false>
 struct Bloch {
+    static constexpr int kModel = M;
+    static constexpr int kRelax = R;
+    static constexpr bool kPhase = phase_table(M, R);  // stage values are (sin f A, cos f A)
     JobRef J;
     double kx, ky, kz;
+    double pk_s = 0.0, pk_c = 0.0;  // phase models: sin and cos of f kx (f kz for the 3D pi-flux lattice)
     double s0[3];
     double cy;  // theta = 0 constant: PiFlux 2t cos ky, honeycomb cos(sqrt3/2 ky); 3D Dirac |k|; 3D pi-flux cos ky
     double ik;  // 3D Dirac: 1/|k|
@@ -211,6 +215,13 @@ struct Bloch {
             }
         }
         else cy = 0.0;
+        if constexpr (model_has_phase(M)) sincos(model_phase_scale(M) * (M == MODEL_PIFLUX3D ? kz : kx), &pk_s, &pk_c);
+    }
+
+    // phase models: sin and cos of f (k - A) from (sin f A, cos f A)
+    __device__ __forceinline__ void shifted_phase(double sa, double ca, double& sn, double& cs) const {
+        sn = pk_s * ca - pk_c * sa;
+        cs = pk_c * ca + pk_s * sa;
     }
 
     // the integrator calls this with the state at the start of every attempt
@@ -247,12 +258,17 @@ struct Bloch {
         } else {
             double sh, ch;
             sincos(0.5 * (kx - A), &sh, &ch);
-            const double cq = (ch - sh) * (ch + sh);  // cos qx
-            const double sq = 2.0 * sh * ch;          // sin qx
-            d[0] = J.hop * (cq + 2.0 * ch * cy);      // t Re g
-            d[1] = -J.hop * (sq - 2.0 * sh * cy);     // -t Im g
-            d[2] = 0.0;
+            honeycomb_d(sh, ch, d);
         }
+    }
+
+    // honeycomb d from (sin, cos) of qx/2, qx = kx - A
+    __device__ __forceinline__ void honeycomb_d(double sh, double ch, double d[3]) const {
+        const double cq = (ch - sh) * (ch + sh);  // cos qx
+        const double sq = 2.0 * sh * ch;          // sin qx
+        d[0] = J.hop * (cq + 2.0 * ch * cy);      // t Re g
+        d[1] = -J.hop * (sq - 2.0 * sh * cy);     // -t Im g
+        d[2] = 0.0;
     }
 
     // Honeycomb::ic_sigma (Honeycomb.cpp:355-361) at the field d = t(Re g, -Im g, 0),
@@ -406,13 +422,11 @@ struct Bloch {
 
     // ds/dt = 2 d(k - A) x s - relaxation
     __device__ __forceinline__ void rhs_A(const double s[3], double A, double out[3]) const {
-        if constexpr (M == MODEL_PIFLUX3D) {
-            // 4t (m x s) (PiFlux.cpp:89-94)
-            const double cz = cos(kz - A), pf = 4.0 * J.hop;
-            out[0] = (cy * s[2] - cz * s[1]) * pf;
-            out[1] = (cz * s[0] - cx * s[2]) * pf;
-            out[2] = (cx * s[1] - cy * s[0]) * pf;
-            if constexpr (R == RELAX_EQUILIBRIUM) relax_equilibrium(cz, s, out);
+        if constexpr (model_has_phase(M)) {
+            double sn = 0.0, cs;
+            if constexpr (M == MODEL_HONEYCOMB) sincos(0.5 * (kx - A), &sn, &cs);
+            else cs = cos((M == MODEL_PIFLUX3D ? kz : kx) - A);
+            rhs_sc(s, sn, cs, out);
             return;
         }
         if constexpr (M != MODEL_DIRAC3D && R != RELAX_EQUILIBRIUM) {
@@ -421,37 +435,9 @@ struct Bloch {
             // the one on d exactly (every product and sum scales by a power of
             // two), so this is the general path below with up to three
             // multiplications fewer per evaluation
-            double wx, wy, wz;
-            if constexpr (M == MODEL_DIRAC2D) {
-                const double tv = 2.0 * J.v;
-                wx = tv * (kx - A * J.ct), wy = tv * (ky - A * J.st), wz = 2.0 * J.mass;
-            } else if constexpr (M == MODEL_PIFLUX2D) {
-                wx = (4.0 * J.hop) * cos(kx - A), wy = 2.0 * cy, wz = 0.0;
-            } else {
-                double sh, ch;
-                sincos(0.5 * (kx - A), &sh, &ch);
-                const double cq = (ch - sh) * (ch + sh), sq = 2.0 * sh * ch, th = 2.0 * J.hop;
-                wx = th * (cq + 2.0 * ch * cy), wy = -th * (sq - 2.0 * sh * cy), wz = 0.0;
-            }
-            out[0] = wy * s[2] - wz * s[1];
-            out[1] = wz * s[0] - wx * s[2];
-            out[2] = wx * s[1] - wy * s[0];
-            if constexpr (R == RELAX_TO_INITIAL) {
-                const double r = J.relax_rate;
-                out[0] -= (s[0] - s0[0]) * r;
-                out[1] -= (s[1] - s0[1]) * r;
-                out[2] -= (s[2] - s0[2]) * r;
-            } else if constexpr (R == RELAX_T2) {
-                const double r = J.relax_rate;
-                const double ww = wx * wx + wy * wy + wz * wz;
-                double y = __builtin_amdgcn_rcp(ww);
-                y = fma(fma(-ww, y, 1.0), y, y);
-                y = fma(fma(-ww, y, 1.0), y, y);
-                const double proj = ww > 0.0 ? (wx * s[0] + wy * s[1] + wz * s[2]) * y : 0.0;
-                out[0] -= (s[0] - proj * wx) * r;
-                out[1] -= (s[1] - proj * wy) * r;
-                out[2] -= (s[2] - proj * wz) * r;
-            }
+            const double tv = 2.0 * J.v;
+            const double wx = tv * (kx - A * J.ct), wy = tv * (ky - A * J.st), wz = 2.0 * J.mass;
+            rhs_w(s, wx, wy, wz, out);
             return;
         }
         double d[3];
@@ -465,14 +451,6 @@ struct Bloch {
             out[0] -= (s[0] - s0[0]) * r;
             out[1] -= (s[1] - s0[1]) * r;
             out[2] -= (s[2] - s0[2]) * r;
-        } else if constexpr (R == RELAX_EQUILIBRIUM) {
-            // honeycomb: -(s - s_eq(k - A))/tau (Honeycomb.cpp:180-191)
-            const double r = J.relax_rate;
-            double se[3];
-            honeycomb_state(d, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), se);
-            out[0] -= r * (s[0] - se[0]);
-            out[1] -= r * (s[1] - se[1]);
-            out[2] -= r * (s[2] - se[2]);
         } else if constexpr (R == RELAX_T2) {
             const double r = J.relax_rate;
             const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -488,10 +466,91 @@ struct Bloch {
         }
     }
 
+    // phase models at a stage whose field phase is (sin f A, cos f A)
+    __device__ __forceinline__ void rhs_phase(const double s[3], double sa, double ca, double out[3]) const {
+        double sn, cs;
+        shifted_phase(sa, ca, sn, cs);
+        rhs_sc(s, sn, cs, out);
+    }
+
+    // w x s - relaxation for w = 2 d; relaxation toward the initial state or T2
+    __device__ __forceinline__ void rhs_w(const double s[3], double wx, double wy, double wz, double out[3]) const {
+        out[0] = wy * s[2] - wz * s[1];
+        out[1] = wz * s[0] - wx * s[2];
+        out[2] = wx * s[1] - wy * s[0];
+        if constexpr (R == RELAX_TO_INITIAL) {
+            const double r = J.relax_rate;
+            out[0] -= (s[0] - s0[0]) * r;
+            out[1] -= (s[1] - s0[1]) * r;
+            out[2] -= (s[2] - s0[2]) * r;
+        } else if constexpr (R == RELAX_T2) {
+            const double r = J.relax_rate;
+            const double ww = wx * wx + wy * wy + wz * wz;
+            double y = __builtin_amdgcn_rcp(ww);
+            y = fma(fma(-ww, y, 1.0), y, y);
+            y = fma(fma(-ww, y, 1.0), y, y);
+            const double proj = ww > 0.0 ? (wx * s[0] + wy * s[1] + wz * s[2]) * y : 0.0;
+            out[0] -= (s[0] - proj * wx) * r;
+            out[1] -= (s[1] - proj * wy) * r;
+            out[2] -= (s[2] - proj * wz) * r;
+        }
+    }
+
+    // phase models: sn, cs = sin, cos of f (k - A) (pi-flux: only cs is read)
+    __device__ __forceinline__ void rhs_sc(const double s[3], double sn, double cs, double out[3]) const {
+        if constexpr (M == MODEL_PIFLUX3D) {
+            // 4t (m x s) (PiFlux.cpp:89-94), cz = cos(kz - A)
+            const double cz = cs, pf = 4.0 * J.hop;
+            out[0] = (cy * s[2] - cz * s[1]) * pf;
+            out[1] = (cz * s[0] - cx * s[2]) * pf;
+            out[2] = (cx * s[1] - cy * s[0]) * pf;
+            if constexpr (R == RELAX_EQUILIBRIUM) relax_equilibrium(cz, s, out);
+        } else if constexpr (M == MODEL_PIFLUX2D) {
+            rhs_w(s, (4.0 * J.hop) * cs, 2.0 * cy, 0.0, out);
+        } else if constexpr (R != RELAX_EQUILIBRIUM) {
+            // honeycomb, w = 2 d with the factor 2 folded into t (exact)
+            const double cq = (cs - sn) * (cs + sn), sq = 2.0 * sn * cs, th = 2.0 * J.hop;
+            rhs_w(s, th * (cq + 2.0 * cs * cy), -th * (sq - 2.0 * sn * cy), 0.0, out);
+        } else {
+            // honeycomb: -(s - s_eq(k - A))/tau (Honeycomb.cpp:180-191)
+            double d[3];
+            honeycomb_d(sn, cs, d);
+            const double wx = 2.0 * d[0], wy = 2.0 * d[1], wz = 2.0 * d[2];
+            out[0] = wy * s[2] - wz * s[1];
+            out[1] = wz * s[0] - wx * s[2];
+            out[2] = wx * s[1] - wy * s[0];
+            const double r = J.relax_rate;
+            double se[3];
+            honeycomb_state(d, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), se);
+            out[0] -= r * (s[0] - se[0]);
+            out[1] -= r * (s[1] - se[1]);
+            out[2] -= r * (s[2] - se[2]);
+        }
+    }
+
     // 3D pi-flux: the instantaneous dispersion |m(k - A)| (PiFlux.cpp:400-402, 1001)
     __device__ __forceinline__ double dispersion_at(double A) const {
         const double c = cos(kz - A);
         return sqrt(cx * cx + cy * cy + c * c);
+    }
+    __device__ __forceinline__ double dispersion_phase(double sa, double ca) const {
+        double sn, c;
+        shifted_phase(sa, ca, sn, c);
+        return sqrt(cx * cx + cy * cy + c * c);
+    }
+
+    // phase models: the current vertex at a sample whose field phase is (sin f A, cos f A)
+    __device__ __forceinline__ double vertex_phase(double sa, double ca, const double s[3]) const {
+        double sn, cs;
+        shifted_phase(sa, ca, sn, cs);
+        if constexpr (M == MODEL_PIFLUX3D) {
+            return sn * s[2];  // sin(kz - A) s_z
+        } else if constexpr (M == MODEL_PIFLUX2D) {
+            return sn * s[0];  // sin(kx - A) s_x
+        } else {
+            const double cq = (cs - sn) * (cs + sn), sq = 2.0 * sn * cs;
+            return (cq - cs * cy) * s[0] + (sq + sn * cy) * s[1];
+        }
     }
 
     // current vertex g(s, A) along the polarisation
@@ -559,32 +618,63 @@ __device__ __forceinline__ double max_growth() {
     return 0.9 / sqrt(sqrt(sqrt(e)));
 }
 
-// The pulse does not depend on the state, so A at the 13 stage times of a
-// step is produced first (c_7 = c_3, c_11 = c_0 = 0, c_12 = c_10 are shared):
-// from the nominal-step table row `cand` when its (t, h) equal the step's,
-// else from the per-step pulse cache.  All 15 table words are loaded at once
-// so the match test and the values arrive together.
+// The pulse does not depend on the state, so the field at the stage times of
+// a step is produced first: from the nominal-step table row `cand` when its
+// (t, h) equal the step's, else from the per-step pulse cache.  All the row's
+// words are loaded at once so the match test and the values arrive together.
+// Stage values: A at the 13 stage times, or for the phase models
+// (sin f A, cos f A) at the 10 distinct stage times (model_has_phase).
+template <bool PHASE>
+struct StageVals {
+    double A[13];
+};
+template <>
+struct StageVals<true> {
+    double S[kPhaseStages], C[kPhaseStages];
+};
+
 template <class F>
-__device__ __forceinline__ void stage_pulses(F& f, double t, double h, const double* cand, double As[13]) {
+__device__ __forceinline__ void stage_pulses(F& f, double t, double h, const double* cand,
+                                             StageVals<F::kPhase>& sv) {
+    constexpr int W = nom_stride(F::kModel, F::kRelax);
     bool hit = false;
     if (cand) {
-        double cv[kNomStride];
+        double cv[W];
 #pragma unroll
-        for (int j = 0; j < kNomStride; ++j) cv[j] = cand[j];
+        for (int j = 0; j < W; ++j) cv[j] = cand[j];
         hit = (cv[0] == t) && (cv[1] == h);
+        if constexpr (F::kPhase) {
 #pragma unroll
-        for (int j = 0; j < 13; ++j) As[j] = cv[2 + j];
+            for (int d = 0; d < kPhaseStages; ++d) sv.S[d] = cv[2 + 2 * d], sv.C[d] = cv[3 + 2 * d];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 13; ++j) sv.A[j] = cv[2 + j];
+        }
     }
     if (!hit) {
         f.prepare(t);
+        if constexpr (F::kPhase) {
+            constexpr double fs = model_phase_scale(F::kModel);
 #pragma unroll
-        for (int i = 0; i < 13; ++i) {
-            if (i == 7) As[i] = As[3];
-            else if (i == 11) As[i] = As[0];
-            else if (i == 12) As[i] = As[10];
-            else As[i] = f.pulse_at(F78::c[i] * h);
+            for (int d = 0; d < kPhaseStages; ++d) sincos(fs * f.pulse_at(F78::c[phase_stage_of(d)] * h), &sv.S[d], &sv.C[d]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 13; ++i) {
+                if (i == 7) sv.A[i] = sv.A[3];
+                else if (i == 11) sv.A[i] = sv.A[0];
+                else if (i == 12) sv.A[i] = sv.A[10];
+                else sv.A[i] = f.pulse_at(F78::c[i] * h);
+            }
         }
     }
+}
+
+// right-hand side at stage i
+template <class F>
+__device__ __forceinline__ void stage_rhs(const F& f, const StageVals<F::kPhase>& sv, int i, const double x[3],
+                                          double out[3]) {
+    if constexpr (F::kPhase) f.rhs_phase(x, sv.S[phase_stage(i)], sv.C[phase_stage(i)], out);
+    else f.rhs_A(x, sv.A[i], out);
 }
 
 // Like odeint's try_step for a non-FSAL stepper, each attempt evaluates the
@@ -594,17 +684,21 @@ __device__ __forceinline__ void stage_pulses(F& f, double t, double h, const dou
 // the rounding of the product and of the quotient).
 constexpr double kTinyErrRatio = (1.0 / 390625.0) * (1.0 - 0x1p-50);
 
-// skip_tiny: when every component has scale >= 2^-1000 (so scale * ratio is a
-// normal number) and |err_i| < kTinyErrRatio * scale, return 0 without the
-// divisions; the controller then takes the same branch (accept, largest growth)
-// as for the exact ratio, which only step logs and network features would read.
-// NaN, infinite and zero-scale cases fail the test and take the exact path.
+// tiny_mode TINY_ZERO: when every component has scale >= 2^-1000 (so
+// scale * ratio is a normal number) and |err_i| < kTinyErrRatio * scale, return
+// 0 without the divisions; the exact ratio is then at most 5^-8 and the
+// controller takes the same branch (accept, largest growth) whatever its value.
+// Only step logs read the value below 5^-8 (the network's feature clamps the
+// ratio at 5^-8).  NaN, infinite and zero-scale cases fail the test and take
+// the exact path.  TINY_EXACT always computes the exact ratio.
+enum : int { TINY_EXACT = 0, TINY_ZERO = 1 };
+
 template <class F>
-__device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double h, const double As[13],
-                                                double snew[3], double rtol, double atol, bool skip_tiny = false) {
+__device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double h, const StageVals<F::kPhase>& sv,
+                                                double snew[3], double rtol, double atol, int tiny_mode = TINY_EXACT) {
     double k[13][3];
     f.begin_step(s);
-    f.rhs_A(s, As[0], k[0]);
+    stage_rhs(f, sv, 0, s, k[0]);
     const double k1[3] = {k[0][0], k[0][1], k[0][2]};
 #pragma unroll
     for (int i = 1; i < 13; ++i) {
@@ -617,10 +711,10 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
                 if (F78::a[i][j] != 0.0) acc = fma(F78::a[i][j], k[j][q], acc);
             x[q] = fma(h, acc, s[q]);
         }
-        f.rhs_A(x, As[i], k[i]);
+        stage_rhs(f, sv, i, x, k[i]);
     }
     double e[3], sc[3];
-    bool tiny = skip_tiny;
+    bool tiny = tiny_mode != TINY_EXACT;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         double acc = 0.0;
@@ -649,7 +743,7 @@ struct IntervalCtl {
     int fallback_after;  // rejections of the first step after which it restarts at h_fallback (0: never)
     double h_fallback;   // the plain controller's restart step Delta / n_sub
     bool plain_start;    // this interval starts at h_fallback as the plain controller
-    bool skip_tiny_err;  // the error ratio's value is not read (no step log, no network): see rkf78_attempt
+    int tiny_mode;       // rkf78_attempt's TINY_*: TINY_ZERO unless a step log reads the error ratios
 };
 struct IntervalOut {
     double h_acc;        // size of the first accepted step of the interval (0: none yet)
@@ -665,6 +759,13 @@ struct IntervalOut {
 // and on return the controller's proposal; unclipped the step that was in
 // force before the final clip.  status: 0 ok, 1 controller gave up, 2 the
 // state became non-finite.
+//
+// Forward-progress guard (not in odeint, whose loop would repeat such steps
+// forever): the lane fails with status 1 when an accepted step leaves t
+// unchanged (h below half an ulp of t) or when the interval takes more than
+// sp.max_steps accepted steps.  Every other loop exit is odeint's, so every
+// lane leaves this loop after at most max_steps accepted and max_fail + 1
+// attempts per step.
 //
 // nom: this interval's row of the nominal-step table (nullptr: none).
 template <class F>
@@ -683,6 +784,10 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
     for (;;) {
         if (fresh) {
             if (!(t1 - t > kDblEps)) break;
+            if (step + 1 >= sp.max_steps) {
+                status = 1;
+                break;
+            }
             unclipped = h;
             if ((t + h) - t1 > kDblEps) h = t1 - t;
             tries = 0;
@@ -691,18 +796,19 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
         }
         // candidate slot: the restart sequence's step, or the whole-interval
         // step for a carried or proposed first step
+        constexpr int W = nom_stride(F::kModel, F::kRelax);
         const double* cand = nullptr;
         if (nom) {
             if (sp.proposal == PROPOSAL_RESTART || io.fell_back || ctl.plain_start) {
-                if (step < kNomRestartSlots) cand = nom + step * kNomStride;
+                if (step < kNomRestartSlots) cand = nom + step * W;
             } else if (step == 0) {
-                cand = nom + kNomRestartSlots * kNomStride;
+                cand = nom + kNomRestartSlots * W;
             }
         }
-        double As[13];
-        stage_pulses(f, t, h, cand, As);
+        StageVals<F::kPhase> sv;
+        stage_pulses(f, t, h, cand, sv);
         double snew[3];
-        const double err = rkf78_attempt(f, s, h, As, snew, sp.rtol, sp.atol, ctl.skip_tiny_err);
+        const double err = rkf78_attempt(f, s, h, sv, snew, sp.rtol, sp.atol, ctl.tiny_mode);
         ++n_att;
         ++io.attempts;
         if (err > 1.0) {
@@ -720,7 +826,12 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
                 break;
             }
         } else {
-            t += h;
+            const double t_next = t + h;
+            if (!(t_next > t)) {
+                status = 1;
+                break;
+            }
+            t = t_next;
             s[0] = snew[0];
             s[1] = snew[1];
             s[2] = snew[2];

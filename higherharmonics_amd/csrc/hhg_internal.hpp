@@ -14,6 +14,7 @@
 //   seg_sum/seg_comp : level-1 compensated sums over item segments
 //   j                : the result, sum over jobs of N+1 samples
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace hhg {
@@ -50,6 +51,30 @@ constexpr int kNomSlots = 7;
 constexpr int kNomRestartSlots = 6;
 constexpr int kNomStride = 15;  // t, h, 13 stage values
 constexpr int kNomRow = kNomSlots * kNomStride;
+
+// Lattice models read the field only through the phase f A of the shifted
+// momentum (honeycomb: sin and cos of (kx - A)/2; pi-flux: cos(kx - A) or
+// cos(kz - A)), and sin(f (k - A)) = sin(f k) cos(f A) - cos(f k) sin(f A).
+// Without the equilibrium relaxation their nominal-step table stores
+// (sin f A, cos f A) at the 10 distinct Fehlberg stage times instead of A
+// (c_7 = c_3, c_11 = c_0, c_12 = c_10), so a hit replaces the per-lane sincos
+// of every stage by an angle subtraction (the equilibrium-relaxation kernels,
+// already at their register limit, spill more with it and keep A); the
+// sample table holds (sin f A_i, cos f A_i) of every phase model.
+__host__ __device__ constexpr bool model_has_phase(int m) {
+    return m == MODEL_PIFLUX2D || m == MODEL_HONEYCOMB || m == MODEL_PIFLUX3D;
+}
+__host__ __device__ constexpr bool phase_table(int m, int r) { return model_has_phase(m) && r != RELAX_EQUILIBRIUM; }
+__host__ __device__ constexpr double model_phase_scale(int m) { return m == MODEL_HONEYCOMB ? 0.5 : 1.0; }
+constexpr int kPhaseStages = 10;
+// Fehlberg stage -> distinct stage time
+__host__ __device__ constexpr int phase_stage(int i) {
+    return i == 7 ? 3 : i == 11 ? 0 : i == 12 ? 9 : (i >= 8 ? i - 1 : i);
+}
+// the distinct stage times' Fehlberg stage
+__host__ __device__ constexpr int phase_stage_of(int d) { return d >= 7 ? d + 1 : d; }
+__host__ __device__ constexpr int nom_stride(int m, int r) { return phase_table(m, r) ? 2 + 2 * kPhaseStages : kNomStride; }
+__host__ __device__ constexpr int nom_row(int m, int r) { return kNomSlots * nom_stride(m, r); }
 
 struct JobDev {
     // model
@@ -100,13 +125,23 @@ struct Counters {
     unsigned long long fallbacks;
     unsigned long long queue;  // work-queue head
     unsigned long long log_count;
-    unsigned long long pad;
+    unsigned long long wave_attempts;  // attempts issued per wave (the largest lane count of every interval)
+    unsigned long long overflows;      // lanes whose sample sums left the accumulators' range
+    unsigned long long pad[7];
 };
 
 struct SolverDev {
     double rtol, atol;
-    int32_t max_fail;
+    int32_t max_fail;   // rejections of one step before the trajectory fails
     int32_t proposal;
+    int32_t max_steps;  // accepted steps per sampling interval before the trajectory fails
+    int32_t pad;
 };
+
+// Accepted steps per sampling interval before a trajectory is marked failed
+// when the caller gives none: far above any converging problem (the restart
+// controller takes 4 per interval), low enough that a step size pinned at the
+// rounding floor of t ends the trajectory in a fraction of a second.
+constexpr int kDefaultMaxSteps = 100000;
 
 }  // namespace hhg

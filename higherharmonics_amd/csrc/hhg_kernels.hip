@@ -18,10 +18,18 @@ namespace hhg {
 // shifted Legendre P3 with the sign of its constant term flipped, evaluated as
 // the reference evaluates it), scaled by 1, 3, 5, 7.
 template <int P>
-__global__ void pulse_table_kernel(const JobDev* jobs, int job, double* At, double* Am, int with_magnus, Gauss30 g) {
+__global__ void pulse_table_kernel(const JobDev* jobs, int job, double* At, double* Pt, double* Am, int with_magnus,
+                                   Gauss30 g) {
     JobRef J = job_ref(jobs + job);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= J.n_meas) At[J.sample_base + i] = pulse_A<P>(J, J.t_begin + (double)i * J.delta);
+    if (i <= J.n_meas) {
+        const double A = pulse_A<P>(J, J.t_begin + (double)i * J.delta);
+        At[J.sample_base + i] = A;
+        double sa = 0.0, ca = 1.0;
+        if (model_has_phase(J.model)) sincos(model_phase_scale(J.model) * A, &sa, &ca);
+        Pt[2 * (J.sample_base + i)] = sa;
+        Pt[2 * (J.sample_base + i) + 1] = ca;
+    }
     if (with_magnus == 2) {
         if (i < J.n_meas) {
             const double t0 = J.t_begin + (double)i * J.delta;
@@ -56,25 +64,35 @@ __global__ void pulse_table_kernel(const JobDev* jobs, int job, double* At, doub
 // Nominal-step table of one job: one thread per sampling interval replays the
 // controller's maximal-growth sequence from the restart step (the same
 // floating-point operations as rkf78_interval) and the whole-interval step,
-// and stores A at the 13 stage times of each candidate.  Unused slots hold
-// t = NaN so they never match.
+// and stores A at the 13 stage times of each candidate, or for the phase
+// models (model_has_phase) sin and cos of f A at the 10 distinct stage times.
+// Unused slots hold t = NaN so they never match.
 template <int P>
 __global__ void nominal_table_kernel(const JobDev* jobs, int job, double* tab) {
     JobRef J = job_ref(jobs + job);
     const int64_t ii = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ii >= J.n_meas) return;
-    double* row = tab + J.nominal_base + ii * kNomRow;
+    const int model = J.model;
+    const bool phase = phase_table(model, J.relax);
+    const int W = nom_stride(model, J.relax);
+    double* row = tab + J.nominal_base + ii * nom_row(model, J.relax);
     const double t0 = J.t_begin + (double)ii * J.delta;
     const double t1 = J.t_begin + (double)(ii + 1) * J.delta;
     auto put = [&](int slot, double t, double h) {
-        double* c = row + slot * kNomStride;
+        double* c = row + slot * W;
         c[0] = t;
         c[1] = h;
-        for (int j = 0; j < 13; ++j) c[2 + j] = pulse_A<P>(J, t + F78::c[j] * h);
+        if (phase) {
+            const double fs = model_phase_scale(model);
+            for (int d = 0; d < kPhaseStages; ++d)
+                sincos(fs * pulse_A<P>(J, t + F78::c[phase_stage_of(d)] * h), &c[2 + 2 * d], &c[3 + 2 * d]);
+        } else {
+            for (int j = 0; j < 13; ++j) c[2 + j] = pulse_A<P>(J, t + F78::c[j] * h);
+        }
     };
     auto put_empty = [&](int slot) {
-        double* c = row + slot * kNomStride;
-        for (int j = 0; j < kNomStride; ++j) c[j] = __builtin_nan("");
+        double* c = row + slot * W;
+        for (int j = 0; j < W; ++j) c[j] = __builtin_nan("");
     };
     double t = t0, h = J.delta / J.n_sub;
     for (int slot = 0; slot < kNomRestartSlots; ++slot) {
@@ -155,12 +173,12 @@ hipError_t launch_cost_probe(int model, int pulse, const KernelArgs& a, int n_pr
     return by_model(model, [&](auto M) { return cost_model<decltype(M)::value>(pulse, a, n_probe, cost, smem, st); });
 }
 
-hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Am,
-                              int with_magnus, const Gauss30& g, hipStream_t st) {
+hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Pt,
+                              double* Am, int with_magnus, const Gauss30& g, hipStream_t st) {
     const int blocks = (int)((n_threads + 255) / 256);
     return by_pulse(pulse, [&](auto P) {
-        hipLaunchKernelGGL((pulse_table_kernel<decltype(P)::value>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Am,
-                           with_magnus, g);
+        hipLaunchKernelGGL((pulse_table_kernel<decltype(P)::value>), dim3(blocks), dim3(256), 0, st, jobs, job, At, Pt,
+                           Am, with_magnus, g);
         return hipGetLastError();
     });
 }

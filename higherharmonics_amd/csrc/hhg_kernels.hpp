@@ -13,14 +13,17 @@ struct KernelArgs {
     const int2* items;  // {job, chunk}
     int n_items;
     int queue_chunk;
-    int infer_every;     // learned proposal: intervals between network evaluations
+    int infer_every;     // learned proposal, scheduled policy: intervals between network evaluations
     int fallback_after;  // learned proposal: rejections before the plain controller takes over
+    int infer_policy;    // learned proposal: 0 gated (first interval and after a rejection), 1 scheduled
+    int count_waves;     // nonzero: count the attempts every wave issues (Counters::wave_attempts)
     const double* kx;
     const double* ky;
     const double* kz;  // 3D pi-flux lattice only (nullptr otherwise)
     const double* w;
     const int64_t* perm;  // nullptr: identity lane -> k map; else per job relative k index
     const double* At;
+    const double* Pt;  // phase models: (sin f A_i, cos f A_i) per sample (model_has_phase)
     const double* Am;
     const double* nom;  // nominal-step pulse table (nullptr: evaluate every stage)
     unsigned long long* acc;  // 2 words per output sample
@@ -71,8 +74,8 @@ struct Gauss30 {
     double x[30], w[30];
 };
 
-hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Am,
-                              int with_magnus, const Gauss30& g, hipStream_t st);
+hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Pt,
+                              double* Am, int with_magnus, const Gauss30& g, hipStream_t st);
 hipError_t launch_nominal_table(int pulse, const JobDev* jobs, int job, int64_t n_meas, double* tab,
                                 hipStream_t st);
 hipError_t launch_clear(unsigned long long* p, int64_t n_words, hipStream_t st);

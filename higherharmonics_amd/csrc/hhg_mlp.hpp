@@ -9,7 +9,7 @@
 //
 // Features of one k-point at the start of a sampling interval t0:
 //   0 kx, 1 ky, 2 |d(k - A(t0))|, 3 2|d(k)| (gap at k), 4 A(t0), 5 E(t0) = -dA/dt,
-//   6 log(h_prev / Delta) (the controller's carried step), 7 log10 of the last error ratio
+//   6 log(h_prev / Delta) (the controller's carried step), 7 log10 of the last error ratio clamped below at 5^-8
 // Outputs: 0 log(h / Delta) of the first step to attempt (trained on the step the
 // error model of the first accepted step allows), 1 log(attempts in the interval).
 #pragma once

@@ -56,11 +56,16 @@ struct hhg_plan {
     double* d_spl = nullptr;  // spline pulse coefficients of every spline job, concatenated
     hhg::JobDev* d_jobs = nullptr;
     int2* d_items = nullptr;
-    double *d_At = nullptr, *d_Am = nullptr, *d_nom = nullptr;
+    double *d_At = nullptr, *d_Pt = nullptr, *d_Am = nullptr, *d_nom = nullptr;
     unsigned long long* d_words = nullptr;  // [counters per group][accumulators]
     int64_t n_words = 0;
     double* d_j = nullptr;
+    hhg::Counters* d_aux = nullptr;  // counters of the trajectory and occupation launches
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_done = nullptr;  // recorded after every execution, on the stream it used
+    bool executed = false;
+    bool last_host = false;        // the last execution was hhg_plan_execute[_accumulators]
+    double last_wall_ms = 0.0;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     const hhg::MlpDev* d_mlp = nullptr;  // owned by the hhg_mlp handle
@@ -96,6 +101,16 @@ hhg_status check_hip(hipError_t e, const char* what) {
         const hhg_status _st = check_hip((expr), (what));   \
         if (_st != HHG_SUCCESS) return _st;                 \
     } while (0)
+
+hhg::SolverDev solver_dev(const hhg_solver_desc& sv) {
+    hhg::SolverDev sp{};
+    sp.rtol = sv.rtol;
+    sp.atol = sv.atol;
+    sp.max_fail = sv.max_fail > 0 ? sv.max_fail : 500;
+    sp.proposal = sv.proposal;
+    sp.max_steps = sv.max_steps > 0 ? sv.max_steps : hhg::kDefaultMaxSteps;
+    return sp;
+}
 
 hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n_kgrids, int idx) {
     const std::string at = "job " + std::to_string(idx) + ": ";
@@ -226,6 +241,8 @@ void destroy_device(hhg_plan* p) {
     if (p->graph) hipGraphDestroy(p->graph);
     for (auto& e : p->ev)
         if (e) hipEventDestroy(e);
+    if (p->ev_done) hipEventDestroy(p->ev_done);
+    hipFree(p->d_aux);
     hipFree(p->d_kx);
     hipFree(p->d_ky);
     hipFree(p->d_kz);
@@ -234,6 +251,7 @@ void destroy_device(hhg_plan* p) {
     hipFree(p->d_jobs);
     hipFree(p->d_items);
     hipFree(p->d_At);
+    hipFree(p->d_Pt);
     hipFree(p->d_Am);
     hipFree(p->d_nom);
     hipFree(p->d_perm);
@@ -288,6 +306,7 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.w = p->d_w;
         a.perm = nullptr;
         a.At = p->d_At;
+        a.Pt = p->d_Pt;
         a.Am = p->d_Am;
         a.nom = p->d_nom;
         a.acc = p->acc();
@@ -297,14 +316,13 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.mlp = p->d_mlp;
         a.infer_every = p->solver.mlp_infer_every > 0 ? p->solver.mlp_infer_every : 1;
         a.fallback_after = p->solver.fallback_after > 0 ? p->solver.fallback_after : 2;
+        a.infer_policy = p->solver.infer_policy;
+        a.count_waves = p->launch.count_wave_attempts;
         a.log = p->d_log;
         a.log_count = &p->counters(0)->log_count;
         a.log_cap = p->log_cap;
         a.log_stride = p->log_stride;
-        a.sp.rtol = p->solver.rtol;
-        a.sp.atol = p->solver.atol;
-        a.sp.max_fail = p->solver.max_fail > 0 ? p->solver.max_fail : 500;
-        a.sp.proposal = p->solver.proposal;
+        a.sp = solver_dev(p->solver);
         int blocks = g.blocks;
         if (n_items_limit > 0) blocks = std::min(blocks, (a.n_items + g.waves_per_block - 1) / g.waves_per_block);
         HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, p->learned(), a, std::max(1, blocks),
@@ -328,26 +346,42 @@ hhg_status collect_stats(hhg_plan* p, hhg_stats* stats, double wall_ms) {
         s.failed_trajectories += (int64_t)x.failed;
         s.nonfinite_trajectories += (int64_t)x.nonfinite;
         s.proposal_fallbacks += (int64_t)x.fallbacks;
+        s.wave_attempts += (int64_t)x.wave_attempts;
+        s.accumulator_overflows += (int64_t)x.overflows;
     }
     float a = 0.f, b = 0.f;
     if (hipEventElapsedTime(&a, p->ev[0], p->ev[1]) == hipSuccess) s.solve_ms = a;
     if (hipEventElapsedTime(&b, p->ev[1], p->ev[2]) == hipSuccess) s.reduce_ms = b;
     s.total_ms = wall_ms;
     if (stats) *stats = s;
+    if (s.accumulator_overflows > 0)
+        return fail(HHG_ERROR_NONFINITE, std::to_string(s.accumulator_overflows) +
+                                             " lanes' sample sums left the accumulators' range (|sum| >= 2^46)");
     if (s.nonfinite_trajectories > 0)
         return fail(HHG_ERROR_NONFINITE, std::to_string(s.nonfinite_trajectories) + " trajectories became non-finite");
     if (s.failed_trajectories > 0)
         return fail(HHG_ERROR_TOLERANCE,
-                    std::to_string(s.failed_trajectories) + " trajectories exceeded the step controller's attempts");
+                    std::to_string(s.failed_trajectories) + " trajectories exceeded the step controller's limits");
     return HHG_SUCCESS;
 }
 
 hhg_status run(hhg_plan* p, hipStream_t st, int n_items_limit = 0) {
     if (p->graph_exec && n_items_limit == 0 && st == p->stream) {
         HHG_TRY(hipGraphLaunch(p->graph_exec, st), "graph launch");
-        return HHG_SUCCESS;
+    } else {
+        const hhg_status r = enqueue(p, st, n_items_limit);
+        if (r != HHG_SUCCESS) return r;
     }
-    return enqueue(p, st, n_items_limit);
+    HHG_TRY(hipEventRecord(p->ev_done, st), "event");
+    p->executed = true;
+    return HHG_SUCCESS;
+}
+
+// waits for the last execution, on whichever stream it ran
+hhg_status wait_last(hhg_plan* p) {
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    if (p->executed) HHG_TRY(hipEventSynchronize(p->ev_done), "synchronize");
+    return HHG_SUCCESS;
 }
 
 }  // namespace
@@ -374,8 +408,10 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         return fail(HHG_ERROR_INVALID_ARGUMENT, "grouping by predicted cost needs a network (solver.mlp)");
     if (sv.mlp && hhg::mlp_device(sv.mlp) != d->device)
         return fail(HHG_ERROR_INVALID_ARGUMENT, "the network lives on another device than the plan");
-    if (sv.mlp_infer_every < 0 || sv.fallback_after < 0)
-        return fail(HHG_ERROR_INVALID_ARGUMENT, "mlp_infer_every and fallback_after must be >= 0");
+    if (sv.mlp_infer_every < 0 || sv.fallback_after < 0 || sv.max_steps < 0)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "mlp_infer_every, fallback_after and max_steps must be >= 0");
+    if (sv.infer_policy != HHG_INFER_GATED && sv.infer_policy != HHG_INFER_SCHEDULED)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown infer_policy");
     for (int i = 0; i < d->n_jobs; ++i) {
         const hhg_status st = validate_job(d->jobs[i], sv, d->n_kgrids, i);
         if (st != HHG_SUCCESS) return st;
@@ -445,7 +481,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         if (sv.method == HHG_METHOD_MAGNUS4) magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
         if (sv.method == HHG_METHOD_MAGNUS_MOMENTS) magnus_words += 4 * (int64_t)J.n_meas;
         J.nominal_base = nominal_words;
-        if (sv.method == HHG_METHOD_RKF78) nominal_words += (int64_t)J.n_meas * hhg::kNomRow;
+        if (sv.method == HHG_METHOD_RKF78) nominal_words += (int64_t)J.n_meas * hhg::nom_row(J.model, J.relax);
         J.row_base = 0;
         const hhg_pulse_desc& pd = d->jobs[i].pulse;
         if (pd.kind == HHG_PULSE_SPLINE) {
@@ -488,6 +524,8 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         return bail(fail(HHG_ERROR_HIP, "stream creation failed"));
     for (auto& e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(HHG_ERROR_HIP, "event creation failed"));
+    if (hipEventCreateWithFlags(&p->ev_done, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(HHG_ERROR_HIP, "event creation failed"));
     p->n_words = p->n_counter_blocks() * kCounterWords + 2 * p->n_out + 2 * (int64_t)d->n_jobs;
     const size_t nk = sizeof(double) * std::max<int64_t>(1, p->n_kpts);
     hhg_status st = HHG_SUCCESS;
@@ -501,10 +539,12 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     A((void**)&p->d_jobs, sizeof(hhg::JobDev) * p->jobs.size(), "jobs");
     A((void**)&p->d_items, sizeof(int2) * std::max<size_t>(1, items.size()), "items");
     A((void**)&p->d_At, sizeof(double) * p->n_out, "pulse samples");
+    A((void**)&p->d_Pt, 2 * sizeof(double) * p->n_out, "pulse sample phases");
     A((void**)&p->d_Am, sizeof(double) * std::max<int64_t>(1, magnus_words), "magnus table");
     if (nominal_words > 0) A((void**)&p->d_nom, sizeof(double) * nominal_words, "nominal-step table");
     A((void**)&p->d_words, sizeof(unsigned long long) * p->n_words, "accumulators");
     A((void**)&p->d_j, sizeof(double) * p->n_out, "j");
+    A((void**)&p->d_aux, sizeof(hhg::Counters), "trajectory counters");
     if (st != HHG_SUCCESS) return bail(st);
     auto C = [&](void* dst, const void* src, size_t bytes, const char* what) {
         if (st == HHG_SUCCESS && bytes) st = check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), what);
@@ -533,7 +573,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         const auto& J = p->jobs[i];
         const int64_t n = std::max<int64_t>(J.n_meas + 1, (int64_t)J.n_meas * J.magnus_sub);
         st = check_hip(
-            hhg::launch_pulse_table(J.pulse, p->d_jobs, i, n, p->d_At, p->d_Am, with_magnus, g30, p->stream),
+            hhg::launch_pulse_table(J.pulse, p->d_jobs, i, n, p->d_At, p->d_Pt, p->d_Am, with_magnus, g30, p->stream),
             "pulse table");
         if (st != HHG_SUCCESS) return bail(st);
         if (p->d_nom) {
@@ -622,6 +662,7 @@ hhg_status hhg_plan_execute(hhg_plan* p, double* j_out, hhg_stats* stats) {
     HHG_TRY(hipMemcpyAsync(j_out, p->d_j, sizeof(double) * p->n_out, hipMemcpyDeviceToHost, p->stream), "copy j");
     HHG_TRY(hipStreamSynchronize(p->stream), "synchronize");
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    p->last_wall_ms = ms;
     return collect_stats(p, stats, ms);
 }
 
@@ -636,6 +677,7 @@ hhg_status hhg_plan_execute_accumulators(hhg_plan* p, uint64_t* acc_out, hhg_sta
             "copy accumulators");
     HHG_TRY(hipStreamSynchronize(p->stream), "synchronize");
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    p->last_wall_ms = ms;
     return collect_stats(p, stats, ms);
 }
 
@@ -646,8 +688,18 @@ hhg_status hhg_plan_execute_device(hhg_plan* p, double* j_dev, void* stream) {
     hipStream_t st = stream ? (hipStream_t)stream : p->stream;
     hhg_status r = run(p, st);
     if (r != HHG_SUCCESS) return r;
+    p->last_wall_ms = 0.0;
     HHG_TRY(hipMemcpyAsync(j_dev, p->d_j, sizeof(double) * p->n_out, hipMemcpyDeviceToDevice, st), "copy j");
     return HHG_SUCCESS;
+}
+
+hhg_status hhg_plan_last_stats(hhg_plan* p, hhg_stats* stats) {
+    hhg::clear_error();
+    if (!p || !stats) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_last_stats: null argument");
+    if (!p->executed) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_last_stats: the plan has not been executed");
+    const hhg_status st = wait_last(p);
+    if (st != HHG_SUCCESS) return st;
+    return collect_stats(p, stats, p->last_wall_ms);
 }
 
 hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
@@ -685,6 +737,10 @@ hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
     p->launch.queue_chunk = best_c;
     p->launch.persistent = 1;
     for (auto& g : p->groups) size_group(p, g, best_w, best_c, 1);
+    // the tuning runs were partial solves: leave no counts of them behind
+    HHG_TRY(hhg::launch_clear(p->d_words, p->n_words, p->stream), "clear");
+    HHG_TRY(hipStreamSynchronize(p->stream), "autotune sync");
+    p->executed = false;
     if (chosen) *chosen = p->launch;
     return HHG_SUCCESS;
 }
@@ -731,7 +787,8 @@ hhg_status hhg_plan_get_step_log(hhg_plan* p, float* records, int64_t capacity, 
     hhg::clear_error();
     if (!p || !n_records || (capacity > 0 && !records))
         return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_get_step_log: null argument");
-    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    const hhg_status ws = wait_last(p);
+    if (ws != HHG_SUCCESS) return ws;
     hhg::Counters c{};
     HHG_TRY(hipMemcpy(&c, p->d_words, sizeof(c), hipMemcpyDeviceToHost), "log count");
     const int64_t n = std::min<int64_t>((int64_t)c.log_count, p->log_cap);
@@ -767,12 +824,13 @@ hhg_status hhg_trajectories(hhg_plan* p, const int64_t* k_index, int64_t n_sel, 
     hhg_status st = check_hip(hipMalloc(&d_sel, sizeof(int64_t) * n_sel), "selection");
     if (st == HHG_SUCCESS) st = check_hip(hipMalloc(&d_s, out_bytes), "trajectory output");
     if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(d_sel, k_index, sizeof(int64_t) * n_sel, hipMemcpyHostToDevice), "upload");
-    if (st == HHG_SUCCESS) st = check_hip(hhg::launch_clear(p->d_words, p->n_words, p->stream), "clear");
+    const int64_t aux_words = (int64_t)(sizeof(hhg::Counters) / sizeof(unsigned long long));
+    if (st == HHG_SUCCESS)
+        st = check_hip(hhg::launch_clear(reinterpret_cast<unsigned long long*>(p->d_aux), aux_words, p->stream), "clear");
     if (st == HHG_SUCCESS) {
-        hhg::SolverDev sp{p->solver.rtol, p->solver.atol, p->solver.max_fail > 0 ? p->solver.max_fail : 500,
-                          p->solver.proposal};
+        const hhg::SolverDev sp = solver_dev(p->solver);
         const int pulse = p->solver.method == HHG_METHOD_MAGNUS4 ? 0 : J.pulse;
-        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, p->d_kz, d_sel, n_sel, p->d_Am, sp, d_s, p->counters(0), p->stream};
+        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, p->d_kz, d_sel, n_sel, p->d_Am, sp, d_s, p->d_aux, p->stream};
         st = check_hip(hhg::launch_trajectories(p->solver.method, J.model, pulse, J.relax, x), "trajectory launch");
     }
     if (st == HHG_SUCCESS) st = check_hip(hipStreamSynchronize(p->stream), "synchronize");
@@ -787,7 +845,8 @@ hhg_status hhg_plan_job_failures(hhg_plan* p, int64_t* failed, int64_t* nonfinit
     if (!p || !failed || !nonfinite) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_job_failures: null argument");
     const int n_jobs = (int)p->jobs.size();
     std::vector<unsigned long long> w(2 * (size_t)n_jobs);
-    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    const hhg_status ws = wait_last(p);
+    if (ws != HHG_SUCCESS) return ws;
     HHG_TRY(hipStreamSynchronize(p->stream), "synchronize");
     HHG_TRY(hipMemcpy(w.data(), p->job_fail(), sizeof(unsigned long long) * w.size(), hipMemcpyDeviceToHost),
             "job failure counts");
@@ -817,11 +876,12 @@ hhg_status hhg_occupations(hhg_plan* p, const int64_t* k_index, int64_t n_sel, d
     hhg_status st = check_hip(hipMalloc(&d_sel, sizeof(int64_t) * n_sel), "selection");
     if (st == HHG_SUCCESS) st = check_hip(hipMalloc(&d_o, out_bytes), "occupation output");
     if (st == HHG_SUCCESS) st = check_hip(hipMemcpy(d_sel, k_index, sizeof(int64_t) * n_sel, hipMemcpyHostToDevice), "upload");
-    if (st == HHG_SUCCESS) st = check_hip(hhg::launch_clear(p->d_words, p->n_words, p->stream), "clear");
+    const int64_t aux_words = (int64_t)(sizeof(hhg::Counters) / sizeof(unsigned long long));
+    if (st == HHG_SUCCESS)
+        st = check_hip(hhg::launch_clear(reinterpret_cast<unsigned long long*>(p->d_aux), aux_words, p->stream), "clear");
     if (st == HHG_SUCCESS) {
-        hhg::SolverDev sp{p->solver.rtol, p->solver.atol, p->solver.max_fail > 0 ? p->solver.max_fail : 500,
-                          p->solver.proposal};
-        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, p->d_kz, d_sel, n_sel, p->d_Am, sp, d_o, p->counters(0), p->stream};
+        const hhg::SolverDev sp = solver_dev(p->solver);
+        hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, p->d_kz, d_sel, n_sel, p->d_Am, sp, d_o, p->d_aux, p->stream};
         st = check_hip(hhg::launch_occupations(J.pulse, x), "occupation launch");
     }
     if (st == HHG_SUCCESS) st = check_hip(hipStreamSynchronize(p->stream), "synchronize");

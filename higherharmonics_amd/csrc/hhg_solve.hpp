@@ -66,26 +66,52 @@ struct SampleSink {
     }
 };
 
+// Per-lane counts of one work item; flush_counters adds their 64-bit wave
+// sums to the launch's counters and the item's job after every item, so no
+// 32-bit lane or wave total can wrap however long the launch runs.
 struct LaneCounters {
-    uint32_t att = 0, rej = 0, fail = 0, nonfin = 0, fallback = 0;
+    uint32_t att = 0, rej = 0, fail = 0, nonfin = 0, fallback = 0, overflow = 0;
+    unsigned long long wave_att = 0;  // lane 0: attempts the wave issued (count_waves launches)
 };
 
-__device__ __forceinline__ void flush_counters(Counters* ctr, LaneCounters& lc) {
-    const uint32_t a = wave_sum_u32(lc.att), r = wave_sum_u32(lc.rej), f = wave_sum_u32(lc.fail),
-                   n = wave_sum_u32(lc.nonfin), b = wave_sum_u32(lc.fallback);
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off, WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void flush_counters(Counters* ctr, unsigned long long* job_fail, int job, LaneCounters& lc) {
+    const unsigned long long a = wave_sum_u64(lc.att), r = wave_sum_u64(lc.rej), f = wave_sum_u64(lc.fail),
+                             n = wave_sum_u64(lc.nonfin), b = wave_sum_u64(lc.fallback),
+                             o = wave_sum_u64(lc.overflow);
     if ((threadIdx.x & (WAVE - 1)) == 0) {
-        if (a) atomicAdd(&ctr->attempts, (unsigned long long)a);
-        if (r) atomicAdd(&ctr->rejects, (unsigned long long)r);
-        if (f) atomicAdd(&ctr->failed, (unsigned long long)f);
-        if (n) atomicAdd(&ctr->nonfinite, (unsigned long long)n);
-        if (b) atomicAdd(&ctr->fallbacks, (unsigned long long)b);
+        if (a) atomicAdd(&ctr->attempts, a);
+        if (r) atomicAdd(&ctr->rejects, r);
+        if (f) atomicAdd(&ctr->failed, f);
+        if (n) atomicAdd(&ctr->nonfinite, n);
+        if (b) atomicAdd(&ctr->fallbacks, b);
+        if (o) atomicAdd(&ctr->overflows, o);
+        if (lc.wave_att) atomicAdd(&ctr->wave_attempts, lc.wave_att);
+        if (job_fail && f) atomicAdd(job_fail + 2 * job, f);
+        if (job_fail && n) atomicAdd(job_fail + 2 * job + 1, n);
     }
     lc = LaneCounters{};
 }
 
 // Network features of one k-point at the start of interval i (times
 // t_{i-1}..t_i): k, |d(k - A)|, gap at k, A, E = -dA/dt (central difference
-// of the sample table), log(h_prev / Delta), log10 of the last error ratio.
+// of the sample table), log(h_prev / Delta), log10 of the last error ratio
+// clamped below at 5^-8 (the controller grows the step by its largest factor
+// for every ratio below 5^-8, and the tiny-error path reports those as 0).
 template <class F>
 __device__ __forceinline__ void interval_features(const F& f, JobRef J, const double* At, int i, double gap_k,
                                                   double h_prev, double last_err, float feat[kMlpIn]) {
@@ -102,7 +128,7 @@ __device__ __forceinline__ void interval_features(const F& f, JobRef J, const do
     feat[4] = (float)A0;
     feat[5] = -dA;
     feat[6] = __logf((float)h_prev * inv_delta);
-    feat[7] = __log10f((float)(last_err > 1e-16 ? last_err : 1e-16));
+    feat[7] = __log10f((float)(last_err > (1.0 / 390625.0) ? last_err : (1.0 / 390625.0)));
 }
 
 extern __shared__ __align__(16) unsigned char solve_dyn_lds[];
@@ -139,6 +165,7 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const int N = J.n_meas;
     SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
     const double* At = a.At + J.sample_base;
+    const double* Pt = a.Pt + 2 * J.sample_base;  // phase models: (sin f A_i, cos f A_i)
     // 3D pi-flux split current (PiFlux.cpp:998-1006): the samples of points
     // with |m(k - A)| >= split_cut go to the job's second output
     SampleSink sink_hi{a.acc + 2 * (J.sample_base + N + 1), N, 0.0, false};
@@ -146,7 +173,7 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     auto emit = [&](int i, double g) {
         if constexpr (M == MODEL_PIFLUX3D) {
             if (split) {
-                const bool lo = f.dispersion_at(At[i]) < J.split_cut;
+                const bool lo = f.dispersion_phase(Pt[2 * i], Pt[2 * i + 1]) < J.split_cut;
                 sink.put(i, wave_sum(lo ? g : 0.0), lane);
                 sink_hi.put(i, wave_sum(lo ? 0.0 : g), lane);
                 return;
@@ -154,90 +181,114 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
         }
         sink.put(i, wave_sum(g), lane);
     };
-    emit(0, active ? w * f.vertex(At[0], s) : 0.0);
+    auto vertex = [&](int i) {
+        if constexpr (Bloch<M, P, R>::kPhase) return f.vertex_phase(Pt[2 * i], Pt[2 * i + 1], s);
+        else return f.vertex(At[i], s);
+    };
+    emit(0, active ? w * vertex(0) : 0.0);
 
     int status = active ? 0 : 3;
     const bool restart = a.sp.proposal == PROPOSAL_RESTART;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
     const bool logging = a.log != nullptr;
-    IntervalCtl ctl{LEARNED ? a.fallback_after : 0, h_restart, false, !LEARNED && !logging};
+    // step logs record exact error ratios; the network's feature clamps the
+    // last ratio at 5^-8, which the tiny-error path's 0 satisfies
+    const IntervalCtl ctl{0, h_restart, false, logging ? TINY_EXACT : TINY_ZERO};
     IntervalOut io{};
     io.last_err = 1e-3;
-    int next_infer = 1;  // interval of the next network proposal
-    int kq = 0;          // next envelope corner
+    int next_infer = 1;     // scheduled policy: interval of the next network proposal
+    bool rej_prev = false;  // the lane's previous interval had a rejected attempt
+    int kq = 0;             // next envelope corner
+    // One sampling interval from its first step h under the controls c: up to
+    // an envelope corner inside the interval (listed only for carried and
+    // learned proposals) and on from it with the controller's carried step
+    // (the table's whole-interval step does not apply there).  Called at two
+    // sites in the learned kernel, so that intervals without a network
+    // proposal run exactly the carried controller's code.
+    auto integrate = [&](int i, double h, IntervalCtl c) {
+        const double t0 = J.t_begin + (double)(i - 1) * J.delta;
+        const double t1 = J.t_begin + (double)i * J.delta;
+        double unclipped = h;
+        const bool corner = kq < J.n_kinks && J.kink_i[kq] == i;
+        const double* nom = (a.nom && !corner) ? a.nom + J.nominal_base + (int64_t)(i - 1) * nom_row(M, R) : nullptr;
+        double ta = t0;
+        for (int q = kq;; ++q) {
+            const bool stop = q < J.n_kinks && J.kink_i[q] == i;
+            const double tb = stop ? J.kink_t[q] : t1;
+            if (status == 0) {
+                rkf78_interval(f, s, ta, tb, h, unclipped, a.sp, lc.att, lc.rej, status, nom, c, io);
+                if (io.fell_back) lc.fallback++;
+            }
+            if (!stop) break;
+            h = h > unclipped ? h : unclipped;
+            ta = tb;
+            c.fallback_after = 0;  // the proposal concerned the interval's first step only
+        }
+        h_carry = h > unclipped ? h : unclipped;
+        return corner;
+    };
     for (int i = 1; i <= N; ++i) {
         float feat[kMlpIn];
         const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
-        // learned: the network proposes the first step of every infer_every-th
-        // interval, the controller's carried proposal starts the others
-        double h_ratio = -1.0;
-        if (LEARNED && i == next_infer) {
-            next_infer += a.infer_every;
+        // learned: the network proposes the first step of an interval, the
+        // controller's carried proposal starts the others.  Gated policy (0): a
+        // lane asks for a proposal in the first interval and after an interval
+        // whose carried step was rejected; scheduled policy (1): every lane, every
+        // infer_every-th interval.  The wave runs the network when any lane asks.
+        bool want = false;
+        if constexpr (LEARNED) {
+            if (a.infer_policy == 0) {
+                want = status == 0 && (i == 1 || rej_prev);
+            } else if (i == next_infer) {
+                next_infer += a.infer_every;
+                want = true;
+            }
+        }
+        const bool use_net = LEARNED && __any(want);
+        if (use_net || log_this) interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
+        const uint32_t att0 = lc.att, rej0 = lc.rej;
+        bool corner = false;
+        if (use_net) {
             // every lane of the wave takes part in the inference
-            interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
             float out[kMlpOut];
             mlp_infer_wave(a.mlp, solve_dyn_lds, feat, out);
             const float r = __expf(out[0]);
             // a non-finite proposal hands the interval to the plain controller
-            h_ratio = (r == r && r > 0.0f) ? (double)fminf(fmaxf(r, 1e-4f), 4.5f) : 0.0;
-        } else if (log_this) {
-            interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
-        }
-        if (status == 0) {
-            const double t0 = J.t_begin + (double)(i - 1) * J.delta;
-            const double t1 = J.t_begin + (double)i * J.delta;
-            double h;
-            if (LEARNED) {
-                ctl.plain_start = h_ratio == 0.0;
+            const double h_ratio = !want ? -1.0 : (r == r && r > 0.0f) ? (double)fminf(fmaxf(r, 1e-4f), 4.5f) : 0.0;
+            if (status == 0) {
+                IntervalCtl c{a.fallback_after, h_restart, h_ratio == 0.0, ctl.tiny_mode};
+                double h = h_carry;
                 if (h_ratio > 0.0) h = h_ratio * J.delta;
                 else if (h_ratio == 0.0) h = h_restart, lc.fallback++;
-                else h = h_carry;
-            } else {
-                h = restart ? h_restart : h_carry;
+                corner = integrate(i, h, c);
             }
-            double unclipped = h;
-            // an envelope corner inside the interval (listed only for carried and
-            // learned proposals): integrate up to it, continue from there with the
-            // controller's carried step; the table's whole-interval step does not apply
-            const bool corner = kq < J.n_kinks && J.kink_i[kq] == i;
-            const double* nom = (a.nom && !corner) ? a.nom + J.nominal_base + (int64_t)(i - 1) * kNomRow : nullptr;
-            IntervalCtl c = ctl;
-            double ta = t0;
-            for (int q = kq;; ++q) {
-                const bool stop = q < J.n_kinks && J.kink_i[q] == i;
-                const double tb = stop ? J.kink_t[q] : t1;
-                if (status == 0) {
-                    rkf78_interval(f, s, ta, tb, h, unclipped, a.sp, lc.att, lc.rej, status, nom, c, io);
-                    if (io.fell_back) lc.fallback++;
-                }
-                if (!stop) break;
-                h = h > unclipped ? h : unclipped;
-                ta = tb;
-                c.fallback_after = 0;  // the proposal concerned the interval's first step only
-            }
-            h_carry = h > unclipped ? h : unclipped;
-            if (log_this && status == 0 && !corner) {
-                const unsigned long long slot = atomicAdd(a.log_count, 1ull);
-                if (slot < (unsigned long long)a.log_cap) {
-                    float* rec = a.log + slot * (kMlpIn + kMlpOut);
+        } else if (status == 0) {
+            corner = integrate(i, restart ? h_restart : h_carry, ctl);
+        }
+        if (log_this && status == 0 && !corner) {
+            const unsigned long long slot = atomicAdd(a.log_count, 1ull);
+            if (slot < (unsigned long long)a.log_cap) {
+                float* rec = a.log + slot * (kMlpIn + kMlpOut);
 #pragma unroll
-                    for (int q = 0; q < kMlpIn; ++q) rec[q] = feat[q];
-                    // target: the step the error model of the first accepted step
-                    // allows, h 0.9 err^(-1/8) without the growth cap (near-independent
-                    // of the step that was tried: err scales as h^8)
-                    const double e = io.err_acc > 1e-16 ? io.err_acc : 1e-16;
-                    rec[kMlpIn] = (float)(log(io.h_acc / J.delta) + log(0.9) - 0.125 * log(e));
-                    rec[kMlpIn + 1] = (float)log((double)io.attempts);
-                }
+                for (int q = 0; q < kMlpIn; ++q) rec[q] = feat[q];
+                // target: the step the error model of the first accepted step
+                // allows, h 0.9 err^(-1/8) without the growth cap (near-independent
+                // of the step that was tried: err scales as h^8)
+                const double e = io.err_acc > 1e-16 ? io.err_acc : 1e-16;
+                rec[kMlpIn] = (float)(log(io.h_acc / J.delta) + log(0.9) - 0.125 * log(e));
+                rec[kMlpIn + 1] = (float)log((double)io.attempts);
             }
         }
+        rej_prev = lc.rej != rej0;
+        // the wave issues as many attempts in this interval as its busiest lane
+        if (a.count_waves) lc.wave_att += wave_max_u32(lc.att - att0);
         while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
-        emit(i, (status != 3) ? w * f.vertex(At[i], s) : 0.0);
+        emit(i, (status != 3) ? w * vertex(i) : 0.0);
     }
     if (status == 1) lc.fail++;
     if (status == 2) lc.nonfin++;
-    if (sink.overflow || sink_hi.overflow) lc.nonfin++;
+    if (sink.overflow || sink_hi.overflow) lc.overflow++;
 }
 
 template <int M>
@@ -272,8 +323,9 @@ __device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, Lane
         sink.put(i, wave_sum(g), lane);
     }
     if (active) lc.att += (uint32_t)N * (uint32_t)ms;
+    if (a.count_waves) lc.wave_att += (unsigned long long)N * (unsigned long long)ms;
     if (active && !(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) lc.nonfin++;
-    if (sink.overflow) lc.nonfin++;
+    if (sink.overflow) lc.overflow++;
 }
 
 // The reference's moment-based Magnus step of the 3D Dirac cone, one per
@@ -305,8 +357,9 @@ __device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs&
         sink.put(i, wave_sum(active ? w * s[2] : 0.0), lane);
     }
     if (active) lc.att += (uint32_t)N;
+    if (a.count_waves) lc.wave_att += (unsigned long long)N;
     if (active && !(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) lc.nonfin++;
-    if (sink.overflow) lc.nonfin++;
+    if (sink.overflow) lc.overflow++;
 }
 
 }  // namespace solve_detail
@@ -326,7 +379,6 @@ solve_kernel(KernelArgs a) {
         mlp_stage_lds(a.mlp, solve_dyn_lds);
         __syncthreads();
     }
-    LaneCounters lc;
     for (;;) {
         int base = 0;
         if (lane == 0) base = (int)atomicAdd(&a.ctr->queue, (unsigned long long)a.queue_chunk);
@@ -335,21 +387,17 @@ solve_kernel(KernelArgs a) {
         const int end = min(base + a.queue_chunk, a.n_items);
         for (int it = base; it < end; ++it) {
             const int2 item = load_uniform(a.items + it);
-            const uint32_t fail0 = lc.fail, nonfin0 = lc.nonfin;
+            LaneCounters lc;
             if constexpr (METHOD == METHOD_RKF78)
                 solve_item_rkf78<M, P, R, LEARNED>(job_ref(a.jobs + item.x), item.y, a, lc);
             else if constexpr (METHOD == METHOD_MAGNUS4)
                 solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc);
             else
                 solve_item_magnus_moments<M>(job_ref(a.jobs + item.x), item.y, a, lc);
-            if (a.job_fail) {  // the item's failures, attributed to its job
-                const uint32_t df = wave_sum_u32(lc.fail - fail0), dn = wave_sum_u32(lc.nonfin - nonfin0);
-                if (lane == 0 && df) atomicAdd(a.job_fail + 2 * item.x, (unsigned long long)df);
-                if (lane == 0 && dn) atomicAdd(a.job_fail + 2 * item.x + 1, (unsigned long long)dn);
-            }
+            // the item's counts, its failures attributed to its job
+            flush_counters(a.ctr, a.job_fail, item.x, lc);
         }
     }
-    flush_counters(a.ctr, lc);
 }
 
 // Per-k trajectories of one job (tests / inspection): one lane per selected
@@ -375,7 +423,7 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
     int status = 0;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{0, h_restart, false, true};  // nothing here reads the error ratio
+    IntervalCtl ctl{0, h_restart, false, TINY_ZERO};  // nothing here reads the error ratio
     IntervalOut io{};
     int kq = 0;
     for (int i = 1; i <= N; ++i) {
@@ -446,7 +494,7 @@ __global__ __launch_bounds__(64) void occupation_kernel(const JobDev* jobs, cons
     int status = 0;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{0, h_restart, false, true};  // nothing here reads the error ratio
+    IntervalCtl ctl{0, h_restart, false, TINY_ZERO};  // nothing here reads the error ratio
     IntervalOut io{};
     int kq = 0;
     for (int i = 1; i <= N; ++i) {

@@ -57,9 +57,9 @@ def piflux_sweep(n=512, n_delays=32, amps=(0.2, 0.4667, 0.7333, 1.0), per_cycle=
     return jobs
 
 
-def honeycomb_cos2(n=768, n_pulses=16, n_cycles=10, per_cycle=512, seed=SEED):
+def honeycomb_cos2(n=768, n_pulses=16, n_cycles=10, per_cycle=512, seed=SEED, A0_range=(0.1, 1.5)):
     rng = np.random.default_rng(seed)
-    A0 = rng.uniform(0.1, 1.5, n_pulses)
+    A0 = rng.uniform(A0_range[0], A0_range[1], n_pulses)
     cep = rng.uniform(0.0, TWO_PI, n_pulses)
     t_end = TWO_PI * n_cycles + TWO_PI
     N = (n_cycles + 1) * per_cycle

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define HHG_AMD_ABI_VERSION 1
+#define HHG_AMD_ABI_VERSION 2
 
 typedef enum hhg_status {
     HHG_SUCCESS = 0,
@@ -211,6 +211,15 @@ typedef enum hhg_proposal {
     HHG_PROPOSAL_LEARNED = 2  /* MLP proposes each interval's first step; controller still judges */
 } hhg_proposal;
 
+/* When the network proposes a first step (HHG_PROPOSAL_LEARNED).  Either way
+ * the other intervals start from the controller's carried step. */
+typedef enum hhg_infer_policy {
+    HHG_INFER_GATED = 0,    /* a trajectory asks for a proposal in its first interval and after an interval in
+                               which an attempt was rejected (where the carried step did not do); a wave runs
+                               the network when any of its trajectories asks */
+    HHG_INFER_SCHEDULED = 1 /* every trajectory, every mlp_infer_every-th interval */
+} hhg_infer_policy;
+
 typedef struct hhg_mlp hhg_mlp;
 
 typedef struct hhg_solver_desc {
@@ -221,9 +230,13 @@ typedef struct hhg_solver_desc {
     int32_t max_fail;        /* attempts per step before a trajectory fails (odeint: 500) */
     int32_t magnus_substeps; /* Magnus steps per sampling interval */
     const hhg_mlp* mlp;      /* HHG_PROPOSAL_LEARNED and cost grouping: the step-proposal network */
-    int32_t mlp_infer_every; /* intervals between network evaluations (0: every interval) */
+    int32_t mlp_infer_every; /* HHG_INFER_SCHEDULED: intervals between network evaluations (0: every interval) */
     int32_t fallback_after;  /* rejections of a proposed first step before the plain controller
                                 restarts the interval at Delta/n_sub (0: 2) */
+    int32_t max_steps;       /* accepted steps per sampling interval before a trajectory fails (0: 100000); a
+                                trajectory also fails when an accepted step leaves t unchanged.  odeint has
+                                neither bound and would loop forever in both cases */
+    int32_t infer_policy;    /* hhg_infer_policy, for HHG_PROPOSAL_LEARNED */
 } hhg_solver_desc;
 
 typedef struct hhg_launch_desc {
@@ -232,6 +245,8 @@ typedef struct hhg_launch_desc {
     int32_t persistent;      /* 1 = persistent grid + device work queue, 0 = one wave per item */
     int32_t group_by_cost;   /* 1 = order k-points inside a job by the network's predicted step count
                                 (needs solver.mlp) so that lanes of a wave need similar work */
+    int32_t count_wave_attempts; /* 1 = count hhg_stats.wave_attempts (a few instructions per interval) */
+    int32_t pad;
 } hhg_launch_desc;
 
 typedef struct hhg_plan_desc {
@@ -254,6 +269,11 @@ typedef struct hhg_stats {
     int64_t failed_trajectories; /* controller gave up (HHG_ERROR_TOLERANCE) */
     int64_t nonfinite_trajectories;
     int64_t proposal_fallbacks;  /* learned proposals replaced by the plain controller */
+    int64_t wave_attempts;       /* launch.count_wave_attempts: attempts issued per wavefront of 64
+                                    trajectories, i.e. the sum over waves and intervals of the largest
+                                    per-trajectory attempt count (0 when not counted) */
+    int64_t accumulator_overflows; /* wavefront lanes whose sample sums left the exact accumulators'
+                                      range (|sum| >= 2^46): j is wrong where this is nonzero */
     double solve_ms;             /* device time of the solve kernel(s) */
     double reduce_ms;            /* device time of the j reduction */
     double total_ms;             /* host wall time of hhg_plan_execute */
@@ -286,8 +306,15 @@ hhg_status hhg_plan_execute(hhg_plan* plan, double* j_out, hhg_stats* stats);
 /* As hhg_plan_execute but leaves j in device memory j_dev (hhg_plan_output_size
  * doubles, caller-allocated on the plan's device) and does not synchronise
  * the host beyond what the plan's stream requires; stream is a hipStream_t or
- * NULL for the plan's own stream. */
+ * NULL for the plan's own stream.  It does not wait for the solve, so it
+ * reports no controller failures: hhg_plan_last_stats does. */
 hhg_status hhg_plan_execute_device(hhg_plan* plan, double* j_dev, void* stream);
+
+/* Waits for the plan's last execution (on whichever stream it was enqueued)
+ * and returns its statistics and status as hhg_plan_execute would have
+ * (HHG_ERROR_TOLERANCE / HHG_ERROR_NONFINITE with j still written).
+ * total_ms is 0 after hhg_plan_execute_device. */
+hhg_status hhg_plan_last_stats(hhg_plan* plan, hhg_stats* stats);
 
 /* Tries launch shapes on the plan's own work and keeps the fastest
  * (waves_per_block x queue_chunk); writes the choice into chosen (may be NULL). */
@@ -298,11 +325,14 @@ hhg_status hhg_plan_capture_graph(hhg_plan* plan);
 
 hhg_status hhg_plan_destroy(hhg_plan* plan);
 
-/* Per-job counts of the last execution: failed[j] trajectories whose controller
- * gave up (HHG_ERROR_TOLERANCE) and nonfinite[j] whose state or sum became
- * non-finite, for j < n_jobs (the per-pulse counts behind the plan-wide totals
- * of hhg_stats; the reference throws from odeint on the first such
- * trajectory, DiracSystem.cpp:167-173). */
+/* Per-job counts of the last execution (waits for it on whichever stream it
+ * was enqueued): failed[j] trajectories whose controller gave up
+ * (HHG_ERROR_TOLERANCE) and nonfinite[j] whose state became non-finite, for
+ * j < n_jobs (the per-pulse counts behind the plan-wide totals of hhg_stats;
+ * the reference throws from odeint on the first such trajectory,
+ * DiracSystem.cpp:167-173).  hhg_trajectories and hhg_occupations keep their
+ * own counters and leave these alone; hhg_plan_autotune runs partial solves
+ * and leaves zeros. */
 hhg_status hhg_plan_job_failures(hhg_plan* plan, int64_t* failed, int64_t* nonfinite);
 
 /* Per-k Bloch vectors at every sample of job 0 for the listed k indices of

@@ -53,7 +53,8 @@ class Time(ctypes.Structure):
 
 class Solver(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("proposal", ctypes.c_int32), ("rtol", ctypes.c_double),
-                ("atol", ctypes.c_double), ("max_fail", ctypes.c_int32), ("magnus_substeps", ctypes.c_int32)]
+                ("atol", ctypes.c_double), ("max_fail", ctypes.c_int32), ("magnus_substeps", ctypes.c_int32),
+                ("max_steps", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -96,6 +97,10 @@ def lib(fast: bool = False) -> ctypes.CDLL:
     L.or_trajectory.restype = ctypes.c_int
     L.or_trajectory.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
                                 ctypes.POINTER(Solver), *d3, dp, ctypes.POINTER(Stats)]
+    L.or_trajectory_attempts.restype = ctypes.c_int
+    L.or_trajectory_attempts.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
+                                         ctypes.POINTER(Solver), *d3, ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(Stats)]
     L.or_occupations.restype = ctypes.c_int
     L.or_occupations.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), ctypes.POINTER(Time),
                                  ctypes.POINTER(Solver), *d3, dp, ctypes.POINTER(Stats)]
@@ -149,8 +154,8 @@ def time_grid(t_begin, t_end, n_measurements, n_subdivisions=50) -> Time:
 
 
 def solver(method=METHOD_RKF78, rtol=1e-10, atol=1e-10, proposal=PROPOSAL_RESTART, max_fail=500,
-           magnus_substeps=1) -> Solver:
-    return Solver(method, proposal, rtol, atol, max_fail, magnus_substeps)
+           magnus_substeps=1, max_steps=0) -> Solver:
+    return Solver(method, proposal, rtol, atol, max_fail, magnus_substeps, max_steps, 0)
 
 
 def pulse_A(p: Pulse, t) -> np.ndarray:
@@ -201,6 +206,23 @@ def trajectory(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float, k
     if r < 0:
         raise ValueError("unsupported model/solver combination")
     return out, st, r
+
+
+def attempts_per_interval(m: Model, p: Pulse, tc: Time, sv: Solver, kx, ky, kz=None, fast: bool = False):
+    """Attempted steps of every k-point in every sampling interval, int32
+    [n_k][N] (one trajectory per k-point, serial)."""
+    L = lib(fast)
+    kx = np.atleast_1d(np.asarray(kx, dtype=np.float64))
+    ky = np.atleast_1d(np.asarray(ky, dtype=np.float64))
+    kz = np.zeros_like(kx) if kz is None else np.atleast_1d(np.asarray(kz, dtype=np.float64))
+    out = np.zeros((len(kx), tc.n_measurements), dtype=np.int32)
+    for i in range(len(kx)):
+        r = L.or_trajectory_attempts(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv),
+                                     float(kx[i]), float(ky[i]), float(kz[i]),
+                                     out[i].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), None)
+        if r < 0:
+            raise ValueError("unsupported model/solver combination")
+    return out
 
 
 def occupations(m: Model, p: Pulse, tc: Time, sv: Solver, kx: float, ky: float, kz: float, fast: bool = False):

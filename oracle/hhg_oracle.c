@@ -563,6 +563,7 @@ typedef struct {
     double s_step[3]; /* state at the start of the current attempt */
     double rtol, atol;
     int max_fail;
+    long max_steps;
     or_stats* st;
     or_frame* frame;  /* non-null: the occupation run's persistent shifted momentum */
 } or_ctx;
@@ -654,17 +655,22 @@ static int rkf78_try_step(or_ctx* c, double s[3], double* t, double* dt) {
  * return dt holds the controller's proposal; last_unclipped the proposal that
  * was in force before the final step was clipped to t1).  Loop and clip
  * tests use the absolute DBL_EPSILON exactly like odeint's less_with_sign.
- * Returns 0, or 1 when a step needed more than max_fail attempts or made no
- * progress. */
+ * Returns 0, or 1 when a step needed more than max_fail attempts, when an
+ * accepted step left t unchanged (a step below half an ulp of t: odeint would
+ * repeat it forever), or when the interval took more than max_steps accepted
+ * steps. */
 static int rkf78_integrate(or_ctx* c, double s[3], double t0, double t1, double* dt, double* last_unclipped) {
     double t = t0;
     double h = *dt;
     double unclipped = h;
-    long guard = 0;
+    long steps = 0;
     while (t1 - t > DBL_EPSILON) {
+        if (steps >= c->max_steps) return 1;
         unclipped = h;
         if ((t + h) - t1 > DBL_EPSILON) h = t1 - t;
         int tries = 0;
+        const double t_prev = t;
+        const double s_prev[3] = {s[0], s[1], s[2]};
         for (;;) {
             if (!(h > 0.0) || !isfinite(h)) return 1;
             const int ok = rkf78_try_step(c, s, &t, &h);
@@ -672,7 +678,13 @@ static int rkf78_integrate(or_ctx* c, double s[3], double t0, double t1, double*
             if (ok) break;
             if (tries > c->max_fail) return 1;
         }
-        if (++guard > 100000000L) return 1;
+        if (!(t > t_prev)) {  /* the step is discarded: the state stays where it was */
+            s[0] = s_prev[0];
+            s[1] = s_prev[1];
+            s[2] = s_prev[2];
+            return 1;
+        }
+        ++steps;
         if (!isfinite(s[0]) || !isfinite(s[1]) || !isfinite(s[2])) return 1;
     }
     *dt = h;
@@ -856,6 +868,7 @@ static int evolve(const or_model* m, const or_pulse* p, const or_time* tc, const
     c.rtol = sv->rtol;
     c.atol = sv->atol;
     c.max_fail = sv->max_fail > 0 ? sv->max_fail : 500;
+    c.max_steps = sv->max_steps > 0 ? sv->max_steps : 100000;
     c.st = st;
     double s[3];
     if (m->kind == OR_MODEL_HONEYCOMB && m->relax == OR_RELAX_EQUILIBRIUM) {
@@ -954,6 +967,29 @@ int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const
     or_stats local = {0, 0, 0, 0};
     const int r = evolve(m, p, tc, sv, kx, ky, kz, st ? st : &local, emit_traj, &u, NULL);
     if (r > 0 && st) st->failed_trajectories++;
+    return r;
+}
+
+typedef struct {
+    int32_t* att;
+    const or_stats* st;
+    int64_t seen;
+} att_user;
+
+static void emit_att(void* user, int i, const double s[3]) {
+    (void)s;
+    att_user* u = (att_user*)user;
+    if (i > 0) u->att[i - 1] = (int32_t)(u->st->attempts - u->seen);
+    u->seen = u->st->attempts;
+}
+
+int or_trajectory_attempts(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv, double kx,
+                           double ky, double kz, int32_t* att_out, or_stats* st) {
+    or_stats local = {0, 0, 0, 0};
+    or_stats* use = st ? st : &local;
+    att_user u = {att_out, use, use->attempts};
+    const int r = evolve(m, p, tc, sv, kx, ky, kz, use, emit_att, &u, NULL);
+    if (r > 0) use->failed_trajectories++;
     return r;
 }
 

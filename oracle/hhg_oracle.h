@@ -78,6 +78,9 @@ typedef struct {
     double atol;
     int32_t max_fail;        /* consecutive rejections before a trajectory is marked failed */
     int32_t magnus_substeps; /* Magnus steps per sampling interval */
+    int32_t max_steps;       /* accepted steps per sampling interval before a trajectory is marked failed
+                                (<= 0: 100000) */
+    int32_t pad;
 } or_solver;
 
 typedef struct {
@@ -129,6 +132,11 @@ double or_vertex(const or_model* m, double kx, double ky, double kz, double A, c
  * 1 if the controller failed. */
 int or_trajectory(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
                   double kx, double ky, double kz, double* s_out, or_stats* st);
+
+/* One k-point's attempted steps in every sampling interval, att_out[N]
+ * (interval i = t_{i-1}..t_i at index i - 1).  Returns as or_trajectory. */
+int or_trajectory_attempts(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
+                           double kx, double ky, double kz, int32_t* att_out, or_stats* st);
 
 /* Band occupations of one point of the 3D pi-flux lattice with equilibrium
  * relaxation (PiFlux::evolve_occupation_numbers, PiFlux.cpp:222-318):

@@ -110,7 +110,7 @@ class Problem:
         d.update(over)
         return H.Solver(method=d.get("method", 0), rtol=d.get("rtol", 1e-10), atol=d.get("atol", 1e-10),
                         proposal=d.get("proposal", 0), max_fail=d.get("max_fail", 500),
-                        magnus_substeps=d.get("magnus_substeps", 1))
+                        magnus_substeps=d.get("magnus_substeps", 1), max_steps=d.get("max_steps", 0))
 
     def h_plan(self, H, **kw):
         solver_over = kw.pop("solver", {})

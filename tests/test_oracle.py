@@ -308,6 +308,45 @@ def test_reject_shrinks_and_failure_is_reported():
     assert r == 1 and st.rejects >= 2
 
 
+# A step the error test accepts but that cannot advance t: at t ~ 1e10 half an
+# ulp of t is 9.5e-7, while |d| ~ 5e5 (v = 1e6) turning with the field A(t)
+# lets RKF78 at 1e-10 accept only steps near 1e-7.  odeint's loop would accept
+# such steps forever (t + h == t).
+NO_PROGRESS = dict(model=dict(kind=O.MODEL_DIRAC2D, v=1.0e6, mass=0.1),
+                   pulse=dict(kind=O.PULSE_CONTINUOUS, amp_a=0.5),
+                   time=(1.0e10, 1.0e10 + 1.0, 8, 50), k=(0.4, 0.2))
+
+
+def test_step_that_cannot_advance_t_fails_instead_of_spinning():
+    """The forward-progress guard: the first accepted step that leaves t
+    unchanged ends the trajectory as failed after a handful of attempts (the
+    rejections that shrink Delta/50 to the accepted size, then that step)."""
+    c = NO_PROGRESS
+    assert 1e10 + 1e-7 == 1e10  # the premise: such a step is below half an ulp of t
+    out, st, r = O.trajectory(O.model(**c["model"]), O.pulse(**c["pulse"]), O.time_grid(*c["time"]),
+                              O.solver(rtol=1e-10, atol=1e-10), *c["k"])
+    assert r == 1
+    assert st.rejects >= 1 and st.attempts <= 50
+    # the remaining samples keep the frozen initial state
+    assert np.all(np.isfinite(out)) and np.array_equal(out[-1], out[0])
+
+
+def test_accepted_step_cap_per_interval():
+    """max_steps bounds the accepted steps of one sampling interval: the
+    restart controller needs 4 per interval at n_sub = 50, so a cap of 3 fails
+    the trajectory before its 4th step (exactly 3 attempts) and a cap of 4
+    changes nothing."""
+    m = O.model(O.MODEL_DIRAC2D, v=1.0, mass=0.1)
+    p = O.pulse(O.PULSE_COS2, 0.2, n_cycles=2.0, t0=-2 * math.pi)
+    tc = O.time_grid(-2 * math.pi, 2 * math.pi, 400, 50)
+    ref, st_ref, r = O.trajectory(m, p, tc, O.solver(rtol=1e-6, atol=1e-6), 0.3, 0.2)
+    assert r == 0 and st_ref.attempts == 4 * 400
+    _, st, r = O.trajectory(m, p, tc, O.solver(rtol=1e-6, atol=1e-6, max_steps=3), 0.3, 0.2)
+    assert r == 1 and st.attempts == 3
+    out, st, r = O.trajectory(m, p, tc, O.solver(rtol=1e-6, atol=1e-6, max_steps=4), 0.3, 0.2)
+    assert r == 0 and st.attempts == st_ref.attempts and np.array_equal(out, ref)
+
+
 # --------------------------------------------------------------------------
 # Error budget and Magnus order
 # --------------------------------------------------------------------------
