@@ -377,9 +377,10 @@ def uneven_steps_leg(H, n=768, n_pulses=16, per_cycle=32, log_n=96, audit_frac=0
     """The headline's pulses sampled at 32 per cycle: whole-interval steps are
     error-limited and the busiest k-points take about twice the attempts of
     the quietest, so a wave issues as many attempts per interval as its
-    busiest lane.  Legs: plain, carried, learned (gated) and learned with
+    busiest lane.  Legs: plain, carried, learned (first-interval proposal),
+    the network proposing every interval's first step, and learned with
     step-count grouping by the network's cost estimate; each counts the
-    attempts its waves issue; oracle audit of the grouped learned leg."""
+    attempts its waves issue; oracle audit of the learned and carried legs."""
     from higherharmonics_amd import workloads as W
     jobs = W.honeycomb_cos2(n=n, n_pulses=n_pulses, per_cycle=per_cycle)
     small = W.honeycomb_cos2(n=log_n, n_pulses=n_pulses, per_cycle=per_cycle)
@@ -389,17 +390,23 @@ def uneven_steps_leg(H, n=768, n_pulses=16, per_cycle=32, log_n=96, audit_frac=0
     print(f"[bench] uneven-step leg: {per_cycle} samples/cycle, {K} trajectories", file=sys.stderr, flush=True)
     net, train = train_on_jobs(H, W, small)
     legs = {}
+    every = H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net, infer_policy=H.INFER_SCHEDULED)
     for name, sv, kw in [("plain", H.Solver(), {}), ("carry", H.Solver(proposal=H.PROPOSAL_CARRY), {}),
                          ("learned", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {}),
+                         ("learned_every_interval", every, {}),
                          ("learned_grouped", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {"group_by_cost": 1})]:
         print(f"[bench] uneven-step leg {name}", file=sys.stderr, flush=True)
         leg = dict(run_leg(H, W, jobs, sv, 3, 1, count_wave_attempts=1, **kw), steps=3)
         legs[name] = leg_summary(leg, K, f_attempt, vertex)
-    audit = oracle_audit(H, W, jobs, {"learned_grouped": H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net),
+    audit = oracle_audit(H, W, jobs, {"learned": H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net),
+                                      "learned_every_interval": every,
                                       "carry": H.Solver(proposal=H.PROPOSAL_CARRY)}, audit_frac)
     net.close()
     return {"workload": f"headline pulses on hex {n}x{n} at {per_cycle} samples/cycle, RKF78 rtol=atol=1e-10",
             "trajectories": K, "training": train, "legs": legs, "audit": audit,
+            "learned_vs_carry": legs["learned"]["trajectories_per_s"] / legs["carry"]["trajectories_per_s"],
+            "learned_every_interval_vs_carry": legs["learned_every_interval"]["trajectories_per_s"] /
+            legs["carry"]["trajectories_per_s"],
             "learned_grouped_vs_carry": legs["learned_grouped"]["trajectories_per_s"] /
             legs["carry"]["trajectories_per_s"],
             "wave_attempts_learned_grouped_over_carry": legs["learned_grouped"]["wave_attempts"] /

@@ -39,7 +39,7 @@ KGRID_GL_BOX, KGRID_GL_HEX, KGRID_EXPLICIT, KGRID_DIRAC_CYLINDER, KGRID_PIFLUX_C
 KGRID_PIFLUX_OCCUPATION, KGRID_DIRAC_GAUSS = 6, 7
 METHOD_RKF78, METHOD_MAGNUS4, METHOD_MAGNUS_MOMENTS = 0, 1, 2
 PROPOSAL_RESTART, PROPOSAL_CARRY, PROPOSAL_LEARNED = 0, 1, 2
-INFER_GATED, INFER_SCHEDULED = 0, 1
+INFER_FIRST, INFER_SCHEDULED = 0, 1
 
 _STATUS_NAMES = {0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "TOLERANCE", 4: "OUT_OF_MEMORY",
                  5: "UNSUPPORTED", 6: "NO_DEVICE", 7: "NONFINITE"}
@@ -555,7 +555,7 @@ class Solver:
     infer_every: int = 1
     fallback_after: int = 0
     max_steps: int = 0
-    infer_policy: int = INFER_GATED
+    infer_policy: int = INFER_FIRST
 
     def desc(self) -> SolverDesc:
         return SolverDesc(self.method, self.proposal, self.rtol, self.atol, self.max_fail, self.magnus_substeps,

@@ -230,7 +230,7 @@ struct Solver {
     int mlp_infer_every = 1;
     int fallback_after = 0;
     int max_steps = 0;  // accepted steps per sampling interval before a trajectory fails (0: the library's)
-    int infer_policy = HHG_INFER_GATED;
+    int infer_policy = HHG_INFER_FIRST;
     hhg_solver_desc desc() const {
         hhg_solver_desc d{};
         d.method = method;

@@ -742,7 +742,6 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
 struct IntervalCtl {
     int fallback_after;  // rejections of the first step after which it restarts at h_fallback (0: never)
     double h_fallback;   // the plain controller's restart step Delta / n_sub
-    bool plain_start;    // this interval starts at h_fallback as the plain controller
     int tiny_mode;       // rkf78_attempt's TINY_*: TINY_ZERO unless a step log reads the error ratios
 };
 struct IntervalOut {
@@ -799,7 +798,7 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
         constexpr int W = nom_stride(F::kModel, F::kRelax);
         const double* cand = nullptr;
         if (nom) {
-            if (sp.proposal == PROPOSAL_RESTART || io.fell_back || ctl.plain_start) {
+            if (sp.proposal == PROPOSAL_RESTART || io.fell_back) {
                 if (step < kNomRestartSlots) cand = nom + step * W;
             } else if (step == 0) {
                 cand = nom + kNomRestartSlots * W;

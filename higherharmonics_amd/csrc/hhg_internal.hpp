@@ -109,6 +109,7 @@ struct JobDev {
     int64_t magnus_base;  // offset of this job in Am
     int64_t nominal_base; // offset of this job in the nominal-step table (N rows of kNomRow)
     int64_t item_base;    // first item of this job (items are job-major before reordering)
+    int64_t traj_base;    // first trajectory of this job in per-trajectory arrays (first-step proposals)
     int32_t n_items, pad1;
     // corners of the cos^2 envelopes inside sampling intervals (carried and
     // learned proposals stop there): times, interval index i (t_{i-1} < t < t_i)

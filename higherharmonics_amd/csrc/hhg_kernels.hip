@@ -148,14 +148,14 @@ hipError_t by_pulse(int pulse, F&& f) {
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_solve(int method, int model, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
+hipError_t launch_solve(int method, int model, int pulse, int relax, int learned, const KernelArgs& a, int blocks,
                         int threads, size_t smem, hipStream_t st) {
     return by_model(model, [&](auto M) {
         return solve_model<decltype(M)::value>(method, pulse, relax, learned, a, blocks, threads, smem, st, nullptr);
     });
 }
 
-const void* solve_kernel_fn(int method, int model, int pulse, int relax, bool learned) {
+const void* solve_kernel_fn(int method, int model, int pulse, int relax, int learned) {
     const void* fn = nullptr;
     KernelArgs dummy{};
     const hipError_t e = by_model(model, [&](auto M) {
@@ -171,6 +171,11 @@ hipError_t launch_trajectories(int method, int model, int pulse, int relax, cons
 hipError_t launch_cost_probe(int model, int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem,
                              hipStream_t st) {
     return by_model(model, [&](auto M) { return cost_model<decltype(M)::value>(pulse, a, n_probe, cost, smem, st); });
+}
+
+hipError_t launch_first_step(int model, int pulse, const KernelArgs& a, double* h_first, size_t smem,
+                             hipStream_t st) {
+    return by_model(model, [&](auto M) { return first_step_model<decltype(M)::value>(pulse, a, h_first, smem, st); });
 }
 
 hipError_t launch_pulse_table(int pulse, const JobDev* jobs, int job, int64_t n_threads, double* At, double* Pt,

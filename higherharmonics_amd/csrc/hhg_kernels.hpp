@@ -26,6 +26,7 @@ struct KernelArgs {
     const double* Pt;  // phase models: (sin f A_i, cos f A_i) per sample (model_has_phase)
     const double* Am;
     const double* nom;  // nominal-step pulse table (nullptr: evaluate every stage)
+    const double* h_first;  // per trajectory: the first interval's first step (nullptr: Delta/n_sub)
     unsigned long long* acc;  // 2 words per output sample
     Counters* ctr;
     const MlpDev* mlp;        // learned proposal / cost probe network
@@ -54,20 +55,25 @@ struct TrajArgs {
 
 // per-model entry points (explicitly instantiated in hhg_solve_<model>.hip)
 template <int M>
-hipError_t solve_model(int method, int pulse, int relax, bool learned, const KernelArgs& a, int blocks, int threads,
+hipError_t solve_model(int method, int pulse, int relax, int learned, const KernelArgs& a, int blocks, int threads,
                        size_t smem, hipStream_t st, const void** fn);
 template <int M>
 hipError_t traj_model(int method, int pulse, int relax, const TrajArgs& x);
 template <int M>
 hipError_t cost_model(int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem, hipStream_t st);
 
-hipError_t launch_solve(int method, int model, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
+hipError_t launch_solve(int method, int model, int pulse, int relax, int learned, const KernelArgs& a, int blocks,
                         int threads, size_t smem, hipStream_t st);
-const void* solve_kernel_fn(int method, int model, int pulse, int relax, bool learned);
+const void* solve_kernel_fn(int method, int model, int pulse, int relax, int learned);
 hipError_t launch_trajectories(int method, int model, int pulse, int relax, const TrajArgs& x);
 // 3D pi-flux lattice with equilibrium relaxation, RKF78: x.s_out = occupations [n_sel][N+1][2]
 hipError_t launch_occupations(int pulse, const TrajArgs& x);
 hipError_t launch_cost_probe(int model, int pulse, const KernelArgs& a, int n_probe, float* cost, size_t smem,
+                             hipStream_t st);
+template <int M>
+hipError_t first_step_model(int pulse, const KernelArgs& a, double* h_first, size_t smem, hipStream_t st);
+// the network's first-interval step of every trajectory of a group's items (h_first[traj_base + k])
+hipError_t launch_first_step(int model, int pulse, const KernelArgs& a, double* h_first, size_t smem,
                              hipStream_t st);
 // 30-point Gauss-Legendre nodes and weights on [0, 1] (moment Magnus tables)
 struct Gauss30 {

@@ -57,6 +57,7 @@ struct hhg_plan {
     hhg::JobDev* d_jobs = nullptr;
     int2* d_items = nullptr;
     double *d_At = nullptr, *d_Pt = nullptr, *d_Am = nullptr, *d_nom = nullptr;
+    double* d_h_first = nullptr;  // learned first-interval steps, one per trajectory
     unsigned long long* d_words = nullptr;  // [counters per group][accumulators]
     int64_t n_words = 0;
     double* d_j = nullptr;
@@ -75,8 +76,13 @@ struct hhg_plan {
     int log_stride = 16;
 
     bool learned() const { return solver.proposal == HHG_PROPOSAL_LEARNED; }
+    // learned first-interval proposals: first_step_kernel, then the carried controller's kernel
+    bool first_step() const { return learned() && solver.infer_policy != HHG_INFER_SCHEDULED; }
+    // the solve kernels' learned-proposal instantiation (LEARN_* of hhg_solve.hpp): the network runs
+    // inside the solve kernel only for the scheduled policy
+    int solve_learn() const { return learned() && solver.infer_policy == HHG_INFER_SCHEDULED ? 2 : 0; }
     size_t smem(int /*waves_per_block*/) const {
-        return learned() ? (size_t)hhg::kMlpLdsBytes : 0;
+        return solve_learn() ? (size_t)hhg::kMlpLdsBytes : 0;
     }
 
     // at least one counter block, so trajectory launches have one even when
@@ -252,6 +258,7 @@ void destroy_device(hhg_plan* p) {
     hipFree(p->d_items);
     hipFree(p->d_At);
     hipFree(p->d_Pt);
+    hipFree(p->d_h_first);
     hipFree(p->d_Am);
     hipFree(p->d_nom);
     hipFree(p->d_perm);
@@ -262,14 +269,14 @@ void destroy_device(hhg_plan* p) {
 }
 
 int max_block_threads(const hhg_plan* p, const Group& g) {
-    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->learned());
+    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn());
     hipFuncAttributes attr{};
     if (!fn || hipFuncGetAttributes(&attr, fn) != hipSuccess) return 256;
     return attr.maxThreadsPerBlock;
 }
 
 int resident_blocks(const hhg_plan* p, const Group& g, int threads) {
-    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->learned());
+    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn());
     int per_cu = 0;
     if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, p->smem(threads / 64)) != hipSuccess ||
         per_cu < 1)
@@ -325,7 +332,11 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.sp = solver_dev(p->solver);
         int blocks = g.blocks;
         if (n_items_limit > 0) blocks = std::min(blocks, (a.n_items + g.waves_per_block - 1) / g.waves_per_block);
-        HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, p->learned(), a, std::max(1, blocks),
+        if (p->first_step()) {
+            HHG_TRY(hhg::launch_first_step(g.model, g.pulse, a, p->d_h_first, hhg::kMlpLdsBytes, st), "first-step launch");
+            a.h_first = p->d_h_first;
+        }
+        HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn(), a, std::max(1, blocks),
                                   64 * g.waves_per_block, p->smem(g.waves_per_block), st),
                 "solve launch");
     }
@@ -410,7 +421,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         return fail(HHG_ERROR_INVALID_ARGUMENT, "the network lives on another device than the plan");
     if (sv.mlp_infer_every < 0 || sv.fallback_after < 0 || sv.max_steps < 0)
         return fail(HHG_ERROR_INVALID_ARGUMENT, "mlp_infer_every, fallback_after and max_steps must be >= 0");
-    if (sv.infer_policy != HHG_INFER_GATED && sv.infer_policy != HHG_INFER_SCHEDULED)
+    if (sv.infer_policy != HHG_INFER_FIRST && sv.infer_policy != HHG_INFER_SCHEDULED)
         return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown infer_policy");
     for (int i = 0; i < d->n_jobs; ++i) {
         const hhg_status st = validate_job(d->jobs[i], sv, d->n_kgrids, i);
@@ -458,7 +469,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
 
     // jobs: shard = contiguous block of 64-point chunks of each job's grid
     p->offsets.assign(d->n_jobs + 1, 0);
-    int64_t magnus_words = 0, nominal_words = 0;
+    int64_t magnus_words = 0, nominal_words = 0, traj_total = 0;
     std::vector<double> spl;             // spline pulse coefficients, concatenated
     std::vector<int64_t> spl_off(d->n_jobs, -1);
     for (int i = 0; i < d->n_jobs; ++i) {
@@ -476,6 +487,8 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         J.n_k = std::max<int64_t>(0, std::min(grid_size[g], c1 * hhg::WAVE) - c0 * hhg::WAVE);
         J.n_items = (int32_t)(c1 - c0);
         J.sample_base = p->offsets[i];
+        J.traj_base = traj_total;
+        traj_total += J.n_k;
         p->offsets[i + 1] = p->offsets[i] + (J.split_cut > 0.0 ? 2 : 1) * (int64_t)(J.n_meas + 1);
         J.magnus_base = magnus_words;
         if (sv.method == HHG_METHOD_MAGNUS4) magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
@@ -545,6 +558,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     A((void**)&p->d_words, sizeof(unsigned long long) * p->n_words, "accumulators");
     A((void**)&p->d_j, sizeof(double) * p->n_out, "j");
     A((void**)&p->d_aux, sizeof(hhg::Counters), "trajectory counters");
+    if (p->first_step()) A((void**)&p->d_h_first, sizeof(double) * std::max<int64_t>(1, traj_total), "first steps");
     if (st != HHG_SUCCESS) return bail(st);
     auto C = [&](void* dst, const void* src, size_t bytes, const char* what) {
         if (st == HHG_SUCCESS && bytes) st = check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), what);

@@ -25,7 +25,13 @@ namespace hhg {
 // VGPRs to scratch in its interval loop (43 GB of scratch writes per solve);
 // at 4 the plain kernel gained 3-6 % and the carried controller lost 11 %.
 // HHG_WAVES_PER_EU overrides all of them (occupancy experiments).
-template <int METHOD, int M, bool LEARNED>
+// Learned-proposal kernels: LEARN_NONE (plain and carried controllers; the
+// learned first-interval policy is first_step_kernel followed by this kernel
+// reading KernelArgs::h_first) and LEARN_SCHEDULED (the network proposes the
+// first step of every infer_every-th interval inside the loop).
+enum : int { LEARN_NONE = 0, LEARN_SCHEDULED = 2 };
+
+template <int METHOD, int M, int LEARN>
 struct SolveOccupancy {
 #ifdef HHG_WAVES_PER_EU
     static constexpr int waves = HHG_WAVES_PER_EU;
@@ -140,7 +146,7 @@ __device__ __forceinline__ double load_kz(const double* kz, int64_t idx) {
     else return 0.0;
 }
 
-template <int M, int P, int R, bool LEARNED>
+template <int M, int P, int R, int LEARN>
 __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t kk = (int64_t)chunk * WAVE + lane;
@@ -190,81 +196,73 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     int status = active ? 0 : 3;
     const bool restart = a.sp.proposal == PROPOSAL_RESTART;
     const double h_restart = J.delta / J.n_sub;
-    double h_carry = h_restart;
+    // the learned first-interval policy's proposal (first_step_kernel)
+    double h_carry = (a.h_first && active) ? a.h_first[J.traj_base + kk] : h_restart;
     const bool logging = a.log != nullptr;
     // step logs record exact error ratios; the network's feature clamps the
-    // last ratio at 5^-8, which the tiny-error path's 0 satisfies
-    const IntervalCtl ctl{0, h_restart, false, logging ? TINY_EXACT : TINY_ZERO};
+    // last ratio at 5^-8, which the tiny-error path's 0 satisfies.  Scheduled
+    // learned solves hand an interval whose first step was rejected
+    // fallback_after times back to the restart step Delta/n_sub.
+    const IntervalCtl ctl{LEARN == LEARN_SCHEDULED ? a.fallback_after : 0, h_restart,
+                          logging ? TINY_EXACT : TINY_ZERO};
     IntervalOut io{};
     io.last_err = 1e-3;
-    int next_infer = 1;     // scheduled policy: interval of the next network proposal
-    bool rej_prev = false;  // the lane's previous interval had a rejected attempt
-    int kq = 0;             // next envelope corner
-    // One sampling interval from its first step h under the controls c: up to
-    // an envelope corner inside the interval (listed only for carried and
-    // learned proposals) and on from it with the controller's carried step
-    // (the table's whole-interval step does not apply there).  Called at two
-    // sites in the learned kernel, so that intervals without a network
-    // proposal run exactly the carried controller's code.
-    auto integrate = [&](int i, double h, IntervalCtl c) {
-        const double t0 = J.t_begin + (double)(i - 1) * J.delta;
-        const double t1 = J.t_begin + (double)i * J.delta;
-        double unclipped = h;
-        const bool corner = kq < J.n_kinks && J.kink_i[kq] == i;
-        const double* nom = (a.nom && !corner) ? a.nom + J.nominal_base + (int64_t)(i - 1) * nom_row(M, R) : nullptr;
-        double ta = t0;
-        for (int q = kq;; ++q) {
-            const bool stop = q < J.n_kinks && J.kink_i[q] == i;
-            const double tb = stop ? J.kink_t[q] : t1;
-            if (status == 0) {
-                rkf78_interval(f, s, ta, tb, h, unclipped, a.sp, lc.att, lc.rej, status, nom, c, io);
-                if (io.fell_back) lc.fallback++;
-            }
-            if (!stop) break;
-            h = h > unclipped ? h : unclipped;
-            ta = tb;
-            c.fallback_after = 0;  // the proposal concerned the interval's first step only
+    // learned: the network proposes an interval's first step in place of the
+    // controller's carried step (a non-finite proposal restarts the interval at
+    // Delta/n_sub); the interval itself runs the carried controller's code
+    auto propose = [&](int i, bool want, float feat[kMlpIn]) {
+        // every lane of the wave takes part in the inference
+        interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
+        float out[kMlpOut];
+        mlp_infer_wave(a.mlp, solve_dyn_lds, feat, out);
+        const float r = __expf(out[0]);
+        if (want && status == 0) {
+            if (r == r && r > 0.0f) h_carry = (double)fminf(fmaxf(r, 1e-4f), 4.5f) * J.delta;
+            else h_carry = h_restart, lc.fallback++;
         }
-        h_carry = h > unclipped ? h : unclipped;
-        return corner;
     };
+    int next_infer = 1;  // scheduled: interval of the next network proposal
+    int kq = 0;  // next envelope corner
     for (int i = 1; i <= N; ++i) {
         float feat[kMlpIn];
         const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
-        // learned: the network proposes the first step of an interval, the
-        // controller's carried proposal starts the others.  Gated policy (0): a
-        // lane asks for a proposal in the first interval and after an interval
-        // whose carried step was rejected; scheduled policy (1): every lane, every
-        // infer_every-th interval.  The wave runs the network when any lane asks.
-        bool want = false;
-        if constexpr (LEARNED) {
-            if (a.infer_policy == 0) {
-                want = status == 0 && (i == 1 || rej_prev);
-            } else if (i == next_infer) {
+        bool featured = false;
+        if constexpr (LEARN == LEARN_SCHEDULED) {
+            if (i == next_infer) {
                 next_infer += a.infer_every;
-                want = true;
+                propose(i, true, feat);
+                featured = true;
             }
         }
-        const bool use_net = LEARNED && __any(want);
-        if (use_net || log_this) interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
-        const uint32_t att0 = lc.att, rej0 = lc.rej;
+        if (log_this && !featured) interval_features(f, J, At, i, gap_k, h_carry, io.last_err, feat);
+        const uint32_t att0 = lc.att;
         bool corner = false;
-        if (use_net) {
-            // every lane of the wave takes part in the inference
-            float out[kMlpOut];
-            mlp_infer_wave(a.mlp, solve_dyn_lds, feat, out);
-            const float r = __expf(out[0]);
-            // a non-finite proposal hands the interval to the plain controller
-            const double h_ratio = !want ? -1.0 : (r == r && r > 0.0f) ? (double)fminf(fmaxf(r, 1e-4f), 4.5f) : 0.0;
-            if (status == 0) {
-                IntervalCtl c{a.fallback_after, h_restart, h_ratio == 0.0, ctl.tiny_mode};
-                double h = h_carry;
-                if (h_ratio > 0.0) h = h_ratio * J.delta;
-                else if (h_ratio == 0.0) h = h_restart, lc.fallback++;
-                corner = integrate(i, h, c);
+        if (status == 0) {
+            const double t0 = J.t_begin + (double)(i - 1) * J.delta;
+            const double t1 = J.t_begin + (double)i * J.delta;
+            double h = restart ? h_restart : h_carry;
+            double unclipped = h;
+            // an envelope corner inside the interval (listed only for carried and
+            // learned proposals): integrate up to it, continue from there with the
+            // controller's carried step; the table's whole-interval step does not apply
+            corner = kq < J.n_kinks && J.kink_i[kq] == i;
+            const double* nom =
+                (a.nom && !corner) ? a.nom + J.nominal_base + (int64_t)(i - 1) * nom_row(M, R) : nullptr;
+            IntervalCtl c = ctl;
+            double ta = t0;
+            for (int q = kq;; ++q) {
+                const bool stop = q < J.n_kinks && J.kink_i[q] == i;
+                const double tb = stop ? J.kink_t[q] : t1;
+                if (status == 0) {
+                    rkf78_interval(f, s, ta, tb, h, unclipped, a.sp, lc.att, lc.rej, status, nom, c, io);
+                    if (io.fell_back) lc.fallback++;
+                }
+                if (!stop) break;
+                h = h > unclipped ? h : unclipped;
+                ta = tb;
+                c.fallback_after = 0;  // the fallback concerns the interval's first step only
             }
-        } else if (status == 0) {
-            corner = integrate(i, restart ? h_restart : h_carry, ctl);
+            h_carry = h > unclipped ? h : unclipped;
         }
         if (log_this && status == 0 && !corner) {
             const unsigned long long slot = atomicAdd(a.log_count, 1ull);
@@ -280,7 +278,6 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
                 rec[kMlpIn + 1] = (float)log((double)io.attempts);
             }
         }
-        rej_prev = lc.rej != rej0;
         // the wave issues as many attempts in this interval as its busiest lane
         if (a.count_waves) lc.wave_att += wave_max_u32(lc.att - att0);
         while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
@@ -369,13 +366,13 @@ __device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs&
 // queue counter until the list is exhausted: a grid of one wave per item
 // makes each wave solve one item, a grid of the resident wave count is the
 // persistent schedule.
-template <int METHOD, int M, int P, int R, bool LEARNED>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SolveOccupancy<METHOD, M, LEARNED>::waves,
-                                                                      SolveOccupancy<METHOD, M, LEARNED>::waves))) void
+template <int METHOD, int M, int P, int R, int LEARN>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SolveOccupancy<METHOD, M, LEARN>::waves,
+                                                                      SolveOccupancy<METHOD, M, LEARN>::waves))) void
 solve_kernel(KernelArgs a) {
     using namespace solve_detail;
     const int lane = threadIdx.x & (WAVE - 1);
-    if constexpr (LEARNED) {
+    if constexpr (LEARN != LEARN_NONE) {
         mlp_stage_lds(a.mlp, solve_dyn_lds);
         __syncthreads();
     }
@@ -389,7 +386,7 @@ solve_kernel(KernelArgs a) {
             const int2 item = load_uniform(a.items + it);
             LaneCounters lc;
             if constexpr (METHOD == METHOD_RKF78)
-                solve_item_rkf78<M, P, R, LEARNED>(job_ref(a.jobs + item.x), item.y, a, lc);
+                solve_item_rkf78<M, P, R, LEARN>(job_ref(a.jobs + item.x), item.y, a, lc);
             else if constexpr (METHOD == METHOD_MAGNUS4)
                 solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc);
             else
@@ -423,7 +420,7 @@ __global__ __launch_bounds__(64) void trajectory_kernel(const JobDev* jobs, cons
     int status = 0;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{0, h_restart, false, TINY_ZERO};  // nothing here reads the error ratio
+    IntervalCtl ctl{0, h_restart, TINY_ZERO};  // nothing here reads the error ratio
     IntervalOut io{};
     int kq = 0;
     for (int i = 1; i <= N; ++i) {
@@ -494,7 +491,7 @@ __global__ __launch_bounds__(64) void occupation_kernel(const JobDev* jobs, cons
     int status = 0;
     const double h_restart = J.delta / J.n_sub;
     double h_carry = h_restart;
-    IntervalCtl ctl{0, h_restart, false, TINY_ZERO};  // nothing here reads the error ratio
+    IntervalCtl ctl{0, h_restart, TINY_ZERO};  // nothing here reads the error ratio
     IntervalOut io{};
     int kq = 0;
     for (int i = 1; i <= N; ++i) {
@@ -557,6 +554,40 @@ __global__ __launch_bounds__(256) void cost_probe_kernel(KernelArgs a, int n_pro
     if (active) cost[J.k_base + kk] = total;
 }
 
+// First-interval proposals of the learned solve (HHG_INFER_FIRST): for every
+// trajectory of a group's items the network's first step at the start of the
+// first interval, with the features the solve's own first interval has
+// (carried step Delta/n_sub, last error ratio 1e-3); a non-finite proposal
+// leaves Delta/n_sub and counts a fallback.  The solve then runs the carried
+// controller's kernel from these steps.
+template <int M, int P>
+__global__ __launch_bounds__(256) void first_step_kernel(KernelArgs a, double* h_first) {
+    using namespace solve_detail;
+    const int lane = threadIdx.x & (WAVE - 1);
+    mlp_stage_lds(a.mlp, solve_dyn_lds);
+    __syncthreads();
+    const int it = uniform((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
+    if (it >= a.n_items) return;
+    const int2 item = load_uniform(a.items + it);
+    JobRef J = job_ref(a.jobs + item.x);
+    const int64_t kk = (int64_t)item.y * WAVE + lane;
+    const bool active = kk < J.n_k;
+    const int64_t idx = J.k_base + (active ? (a.perm ? a.perm[J.k_base + kk] : kk) : 0);
+    Bloch<M, P, RELAX_NONE> f(J, a.kx[idx], a.ky[idx], load_kz<M>(a.kz, idx));
+    double d[3];
+    f.d_of(0.0, d);
+    const double gap_k = 2.0 * sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double h_restart = J.delta / J.n_sub;
+    float feat[kMlpIn], out[kMlpOut];
+    interval_features(f, J, a.At + J.sample_base, 1, gap_k, h_restart, 1e-3, feat);
+    mlp_infer_wave(a.mlp, solve_dyn_lds, feat, out);
+    const float r = __expf(out[0]);
+    const bool ok = r == r && r > 0.0f;
+    if (active) h_first[J.traj_base + kk] = ok ? (double)fminf(fmaxf(r, 1e-4f), 4.5f) * J.delta : h_restart;
+    const unsigned long long nf = wave_sum_u64((active && !ok) ? 1ull : 0ull);
+    if (lane == 0 && nf) atomicAdd(&a.ctr->fallbacks, nf);
+}
+
 // Run-time (method, pulse, relax, learned) -> template, for one model.  The
 // 3D pi-flux lattice has no relaxation or its equilibrium relaxation; the
 // honeycomb lattice has all four; the other models have all but the
@@ -568,7 +599,7 @@ struct ModelLaunch {
                                       : M == MODEL_HONEYCOMB ? true
                                                              : (R != RELAX_EQUILIBRIUM);
 
-    template <int P, int R, bool L>
+    template <int P, int R, int L>
     static hipError_t solve_t(const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,
                               const void** fn) {
         if (fn) {
@@ -579,16 +610,18 @@ struct ModelLaunch {
         return hipGetLastError();
     }
     template <int P, int R>
-    static hipError_t solve_l(bool learned, const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,
+    static hipError_t solve_l(int learn, const KernelArgs& a, int blocks, int threads, size_t smem, hipStream_t st,
                               const void** fn) {
-        if constexpr (has_relax<R>)
-            return learned ? solve_t<P, R, true>(a, blocks, threads, smem, st, fn)
-                           : solve_t<P, R, false>(a, blocks, threads, smem, st, fn);
-        else
-            return hipErrorInvalidValue;
+        if constexpr (has_relax<R>) {
+            switch (learn) {
+            case LEARN_NONE: return solve_t<P, R, LEARN_NONE>(a, blocks, threads, smem, st, fn);
+            case LEARN_SCHEDULED: return solve_t<P, R, LEARN_SCHEDULED>(a, blocks, threads, smem, st, fn);
+            }
+        }
+        return hipErrorInvalidValue;
     }
     template <int P>
-    static hipError_t solve_r(int relax, bool learned, const KernelArgs& a, int blocks, int threads, size_t smem,
+    static hipError_t solve_r(int relax, int learned, const KernelArgs& a, int blocks, int threads, size_t smem,
                               hipStream_t st, const void** fn) {
         switch (relax) {
         case RELAX_NONE: return solve_l<P, RELAX_NONE>(learned, a, blocks, threads, smem, st, fn);
@@ -598,24 +631,24 @@ struct ModelLaunch {
         }
         return hipErrorInvalidValue;
     }
-    static hipError_t solve(int method, int pulse, int relax, bool learned, const KernelArgs& a, int blocks,
+    static hipError_t solve(int method, int pulse, int relax, int learned, const KernelArgs& a, int blocks,
                             int threads, size_t smem, hipStream_t st, const void** fn) {
         if (method == METHOD_MAGNUS4) {
             if (fn) {
-                *fn = reinterpret_cast<const void*>(&solve_kernel<1, M, PULSE_COS2, RELAX_NONE, false>);
+                *fn = reinterpret_cast<const void*>(&solve_kernel<1, M, PULSE_COS2, RELAX_NONE, LEARN_NONE>);
                 return hipSuccess;
             }
-            hipLaunchKernelGGL((solve_kernel<1, M, PULSE_COS2, RELAX_NONE, false>), dim3(blocks), dim3(threads), 0, st,
+            hipLaunchKernelGGL((solve_kernel<1, M, PULSE_COS2, RELAX_NONE, LEARN_NONE>), dim3(blocks), dim3(threads), 0, st,
                                a);
             return hipGetLastError();
         }
         if (method == METHOD_MAGNUS_MOMENTS) {
             if constexpr (M == MODEL_DIRAC3D) {
                 if (fn) {
-                    *fn = reinterpret_cast<const void*>(&solve_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE, false>);
+                    *fn = reinterpret_cast<const void*>(&solve_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE, LEARN_NONE>);
                     return hipSuccess;
                 }
-                hipLaunchKernelGGL((solve_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE, false>), dim3(blocks),
+                hipLaunchKernelGGL((solve_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE, LEARN_NONE>), dim3(blocks),
                                    dim3(threads), 0, st, a);
                 return hipGetLastError();
             }
@@ -676,6 +709,36 @@ struct ModelLaunch {
         return hipErrorInvalidValue;
     }
 
+    static hipError_t first_step(int pulse, const KernelArgs& a, double* h_first, size_t smem, hipStream_t st) {
+        const int blocks = (a.n_items + 3) / 4;
+        switch (pulse) {
+        case PULSE_COS2:
+            hipLaunchKernelGGL((first_step_kernel<M, PULSE_COS2>), dim3(blocks), dim3(256), smem, st, a, h_first);
+            break;
+        case PULSE_GAUSS:
+            hipLaunchKernelGGL((first_step_kernel<M, PULSE_GAUSS>), dim3(blocks), dim3(256), smem, st, a, h_first);
+            break;
+        case PULSE_DOUBLE_GAUSS:
+            hipLaunchKernelGGL((first_step_kernel<M, PULSE_DOUBLE_GAUSS>), dim3(blocks), dim3(256), smem, st, a,
+                               h_first);
+            break;
+        case PULSE_DOUBLE_COS2:
+            hipLaunchKernelGGL((first_step_kernel<M, PULSE_DOUBLE_COS2>), dim3(blocks), dim3(256), smem, st, a,
+                               h_first);
+            break;
+        case PULSE_CONTINUOUS:
+            hipLaunchKernelGGL((first_step_kernel<M, PULSE_CONTINUOUS>), dim3(blocks), dim3(256), smem, st, a,
+                               h_first);
+            break;
+        case PULSE_SPLINE:
+            hipLaunchKernelGGL((first_step_kernel<M, PULSE_SPLINE>), dim3(blocks), dim3(256), smem, st, a, h_first);
+            break;
+        default:
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+
     static hipError_t cost(int pulse, const KernelArgs& a, int n_probe, float* cost_out, size_t smem, hipStream_t st) {
         const int blocks = (a.n_items + 3) / 4;
         switch (pulse) {
@@ -715,7 +778,7 @@ struct ModelLaunch {
 namespace hhg {
 
 template <int M>
-hipError_t solve_model(int method, int pulse, int relax, bool learned, const KernelArgs& a, int blocks, int threads,
+hipError_t solve_model(int method, int pulse, int relax, int learned, const KernelArgs& a, int blocks, int threads,
                        size_t smem, hipStream_t st, const void** fn) {
     return ModelLaunch<M>::solve(method, pulse, relax, learned, a, blocks, threads, smem, st, fn);
 }
@@ -730,12 +793,18 @@ hipError_t cost_model(int pulse, const KernelArgs& a, int n_probe, float* cost, 
     return ModelLaunch<M>::cost(pulse, a, n_probe, cost, smem, st);
 }
 
+template <int M>
+hipError_t first_step_model(int pulse, const KernelArgs& a, double* h_first, size_t smem, hipStream_t st) {
+    return ModelLaunch<M>::first_step(pulse, a, h_first, smem, st);
+}
+
 }  // namespace hhg
 
 #define HHG_INSTANTIATE_MODEL(M)                                                                                 \
     namespace hhg {                                                                                              \
-    template hipError_t solve_model<M>(int, int, int, bool, const KernelArgs&, int, int, size_t, hipStream_t,    \
+    template hipError_t solve_model<M>(int, int, int, int, const KernelArgs&, int, int, size_t, hipStream_t,     \
                                        const void**);                                                            \
     template hipError_t traj_model<M>(int, int, int, const TrajArgs&);                                           \
     template hipError_t cost_model<M>(int, const KernelArgs&, int, float*, size_t, hipStream_t);                 \
+    template hipError_t first_step_model<M>(int, const KernelArgs&, double*, size_t, hipStream_t);              \
     }

@@ -211,13 +211,14 @@ typedef enum hhg_proposal {
     HHG_PROPOSAL_LEARNED = 2  /* MLP proposes each interval's first step; controller still judges */
 } hhg_proposal;
 
-/* When the network proposes a first step (HHG_PROPOSAL_LEARNED).  Either way
- * the other intervals start from the controller's carried step. */
+/* When the network proposes an interval's first step in place of the
+ * controller's carried step (HHG_PROPOSAL_LEARNED).  The other intervals start
+ * from the carried step; the interval runs the carried controller either way. */
 typedef enum hhg_infer_policy {
-    HHG_INFER_GATED = 0,    /* a trajectory asks for a proposal in its first interval and after an interval in
-                               which an attempt was rejected (where the carried step did not do); a wave runs
-                               the network when any of its trajectories asks */
-    HHG_INFER_SCHEDULED = 1 /* every trajectory, every mlp_infer_every-th interval */
+    HHG_INFER_FIRST = 0,    /* the first interval only: where the sampling interval is shorter than the
+                               error-limited step the carried step is then one attempt per interval, and
+                               the interval loop is exactly the carried controller's */
+    HHG_INFER_SCHEDULED = 1 /* every trajectory, every mlp_infer_every-th interval (fallback_after applies) */
 } hhg_infer_policy;
 
 typedef struct hhg_mlp hhg_mlp;
@@ -231,8 +232,9 @@ typedef struct hhg_solver_desc {
     int32_t magnus_substeps; /* Magnus steps per sampling interval */
     const hhg_mlp* mlp;      /* HHG_PROPOSAL_LEARNED and cost grouping: the step-proposal network */
     int32_t mlp_infer_every; /* HHG_INFER_SCHEDULED: intervals between network evaluations (0: every interval) */
-    int32_t fallback_after;  /* rejections of a proposed first step before the plain controller
-                                restarts the interval at Delta/n_sub (0: 2) */
+    int32_t fallback_after;  /* HHG_INFER_SCHEDULED: rejections of an interval's first step (proposed or
+                                carried) after which the interval restarts at Delta/n_sub (0: 2); a
+                                non-finite proposal restarts its interval there at once (both policies) */
     int32_t max_steps;       /* accepted steps per sampling interval before a trajectory fails (0: 100000); a
                                 trajectory also fails when an accepted step leaves t unchanged.  odeint has
                                 neither bound and would loop forever in both cases */
