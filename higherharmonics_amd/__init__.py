@@ -128,6 +128,7 @@ EXPORTED_SYMBOLS = [
     "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
     "hhg_spline_coefficients", "hhg_reference_double_laser", "hhg_reference_experimental_laser_size",
     "hhg_reference_experimental_laser", "hhg_occupations", "hhg_plan_job_failures", "hhg_plan_last_stats",
+    "hhg_trajectories_noalloc",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -157,6 +158,7 @@ def lib() -> ctypes.CDLL:
     L.hhg_plan_capture_graph.argtypes = [P]
     L.hhg_plan_destroy.argtypes = [P]
     L.hhg_trajectories.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
+    L.hhg_trajectories_noalloc.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
     L.hhg_occupations.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
     L.hhg_plan_job_failures.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.hhg_gauss_legendre.argtypes = [ctypes.c_int32, _DP, _DP]
@@ -201,7 +203,7 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).restype = ctypes.c_int
     for name in ("hhg_plan_create", "hhg_plan_job_offsets", "hhg_plan_execute", "hhg_plan_execute_accumulators",
                  "hhg_plan_execute_device", "hhg_plan_last_stats", "hhg_plan_autotune", "hhg_plan_capture_graph", "hhg_plan_destroy",
-                 "hhg_trajectories", "hhg_occupations", "hhg_plan_job_failures", "hhg_gauss_legendre",
+                 "hhg_trajectories", "hhg_trajectories_noalloc", "hhg_occupations", "hhg_plan_job_failures", "hhg_gauss_legendre",
                  "hhg_kgrid_build", "hhg_pulse_evaluate",
                  "hhg_trapezoidal_spectrum", "hhg_welch_spectrum"):
         getattr(L, name).restype = ctypes.c_int
@@ -654,6 +656,14 @@ class Plan:
         out = np.empty((len(sel), n, 3))
         _check(lib().hhg_trajectories(self._h, sel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(sel),
                                       _dp(out)))
+        return out
+
+    def trajectories_noalloc(self, k_index) -> np.ndarray:
+        """As trajectories(), through the entry point that allocates no device memory."""
+        sel = np.ascontiguousarray(k_index, dtype=np.int64)
+        out = np.empty((len(sel), self._n_samples0, 3))
+        _check(lib().hhg_trajectories_noalloc(self._h, sel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                              len(sel), _dp(out)))
         return out
 
     def job_failures(self):

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -35,6 +36,39 @@ struct Group {
 };
 
 std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
+
+}  // namespace
+
+// Static workspace of hhg_trajectories_noalloc: the selection indices, the
+// trajectory output and the counters of one call, for callers that cannot
+// allocate device memory.  Every GPU the library runs on holds its own copy
+// for the life of the process; calls on one GPU share it, so calls that
+// overlap on one GPU race.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // per device: the address of its copy of scratch
+
+// The device address of `scratch` on `device`, looked up once per device
+// with that device current (the caller has made it current).
+hhg_status scratch_on_device(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(HHG_ERROR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return HHG_SUCCESS;
+}
+
+constexpr size_t round_up_256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
@@ -852,6 +886,45 @@ hhg_status hhg_trajectories(hhg_plan* p, const int64_t* k_index, int64_t n_sel, 
     hipFree(d_sel);
     hipFree(d_s);
     return st;
+}
+
+hhg_status hhg_trajectories_noalloc(hhg_plan* p, const int64_t* k_index, int64_t n_sel, double* s_out) {
+    hhg::clear_error();
+    if (!p || (n_sel > 0 && (!k_index || !s_out)))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_trajectories_noalloc: null argument");
+    if (n_sel <= 0) return HHG_SUCCESS;
+    const hhg::JobDev& J = p->jobs[0];
+    for (int64_t i = 0; i < n_sel; ++i)
+        if (k_index[i] < 0 || k_index[i] >= J.n_k)
+            return fail(HHG_ERROR_INVALID_ARGUMENT, "k index outside job 0's k-points");
+    // workspace layout: counters, selection, output (each 256-byte aligned)
+    const size_t off_sel = round_up_256(sizeof(hhg::Counters));
+    const size_t out_bytes = sizeof(double) * 3 * (size_t)(J.n_meas + 1) * (size_t)n_sel;
+    const size_t off_out = off_sel + round_up_256(sizeof(int64_t) * (size_t)n_sel);
+    if (n_sel > (int64_t)(kScratchBytes / sizeof(int64_t)) || off_out + out_bytes > kScratchBytes)
+        return fail(HHG_ERROR_INVALID_ARGUMENT,
+                    "hhg_trajectories_noalloc: " + std::to_string(n_sel) + " trajectories of " +
+                        std::to_string(J.n_meas + 1) + " samples need more than the " +
+                        std::to_string(kScratchBytes >> 20) + " MiB static workspace");
+    HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
+    unsigned char* base = nullptr;
+    const hhg_status sst = scratch_on_device(p->device, &base);
+    if (sst != HHG_SUCCESS) return sst;
+    auto* d_ctr = reinterpret_cast<hhg::Counters*>(base);
+    auto* d_sel = reinterpret_cast<int64_t*>(base + off_sel);
+    auto* d_s = reinterpret_cast<double*>(base + off_out);
+    HHG_TRY(hipMemcpyAsync(d_sel, k_index, sizeof(int64_t) * n_sel, hipMemcpyHostToDevice, p->stream), "upload");
+    // the counters start at zero (fill kernel: the sequence may be captured)
+    HHG_TRY(hhg::launch_clear(reinterpret_cast<unsigned long long*>(d_ctr),
+                              (int64_t)(sizeof(hhg::Counters) / sizeof(unsigned long long)), p->stream),
+            "clear");
+    const hhg::SolverDev sp = solver_dev(p->solver);
+    const int pulse = p->solver.method == HHG_METHOD_MAGNUS4 ? 0 : J.pulse;
+    hhg::TrajArgs x{p->d_jobs, p->d_kx, p->d_ky, p->d_kz, d_sel, n_sel, p->d_Am, sp, d_s, d_ctr, p->stream};
+    HHG_TRY(hhg::launch_trajectories(p->solver.method, J.model, pulse, J.relax, x), "trajectory launch");
+    HHG_TRY(hipMemcpyAsync(s_out, d_s, out_bytes, hipMemcpyDeviceToHost, p->stream), "download");
+    HHG_TRY(hipStreamSynchronize(p->stream), "synchronize");
+    return HHG_SUCCESS;
 }
 
 hhg_status hhg_plan_job_failures(hhg_plan* p, int64_t* failed, int64_t* nonfinite) {

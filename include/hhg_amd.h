@@ -342,6 +342,17 @@ hhg_status hhg_plan_job_failures(hhg_plan* plan, int64_t* failed, int64_t* nonfi
  * time_evolution_magnus for inspection and tests. */
 hhg_status hhg_trajectories(hhg_plan* plan, const int64_t* k_index, int64_t n_sel, double* s_out);
 
+/* hhg_trajectories without device allocation: the same kernel, inputs and
+ * outputs (s_out[n_sel][N+1][3] in host memory, bitwise equal), with the
+ * selection, the output staging and the counters in a static device
+ * workspace of 16 MiB that every GPU the library runs on holds for the life of
+ * the process.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * Calls needing more than 16 MiB (8 n_sel + 24 n_sel (N + 1) bytes plus 512)
+ * fail with HHG_ERROR_INVALID_ARGUMENT; calls that overlap on one GPU share
+ * the workspace and race. */
+hhg_status hhg_trajectories_noalloc(hhg_plan* plan, const int64_t* k_index, int64_t n_sel, double* s_out);
+
 /* Band occupations of job 0 for the listed k indices of its k-grid:
  * occ_out[n_sel][N+1][2] = (lower, upper) at every sample.  Replaces
  * PiFlux::evolve_occupation_numbers (PiFlux.cpp:222-318, called from
