@@ -630,6 +630,81 @@ hhg_status welch_spectrum(const double* j, const hhg_time_desc& tc, double* freq
 // ---------------------------------------------------------------------------
 // C ABI: host utilities
 // ---------------------------------------------------------------------------
+namespace hhg {
+
+// Checks one job description against the solver before any device work: every
+// refusal is INVALID_ARGUMENT (bad or non-finite values) or UNSUPPORTED (a
+// combination not provided), with a message naming the job.
+hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n_kgrids, int idx) {
+    const std::string at = "job " + std::to_string(idx) + ": ";
+    const hhg_model_desc& m = jd.model;
+    const hhg_pulse_desc& p = jd.pulse;
+    const hhg_time_desc& t = jd.time;
+    if (jd.kgrid < 0 || jd.kgrid >= n_kgrids) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "k-grid index out of range");
+    // every number that enters a right-hand side or a pulse must be finite (a
+    // NaN or infinity would only surface as non-finite trajectories)
+    const double finite_needed[] = {m.v, m.mass, m.hopping, m.theta, m.fermi_energy, m.beta, m.relax_time,
+                                    m.relax_time_offdiag, m.split_dispersion, p.amp_a, p.amp_b, p.n_cycles, p.sigma,
+                                    p.t0, p.delay, p.cep, t.t_begin, t.t_end};
+    for (double x : finite_needed)
+        if (!std::isfinite(x))
+            return fail(HHG_ERROR_INVALID_ARGUMENT, at + "model, pulse and time parameters must be finite");
+    if (p.kind == HHG_PULSE_SPLINE && p.samples && p.n_samples > 0 && p.n_samples <= (int64_t)1 << 30) {
+        for (int64_t i = 0; i < p.n_samples; ++i)
+            if (!std::isfinite(p.samples[i]))
+                return fail(HHG_ERROR_INVALID_ARGUMENT, at + "spline pulse sample " + std::to_string(i) + " is not finite");
+        if (!std::isfinite(p.sample_dt) || !std::isfinite(p.t_cut))
+            return fail(HHG_ERROR_INVALID_ARGUMENT, at + "spline pulse spacing and cut must be finite");
+    }
+    if (m.kind < 0 || m.kind > 4) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown model kind");
+    if (m.relax < 0 || m.relax > 3) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown relaxation kind");
+    const bool piflux3d = m.kind == HHG_MODEL_PIFLUX3D;
+    if (piflux3d && m.relax != HHG_RELAX_NONE && m.relax != HHG_RELAX_EQUILIBRIUM)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D pi-flux lattice relaxes toward its equilibrium only");
+    if (!piflux3d && m.kind != HHG_MODEL_HONEYCOMB && m.relax == HHG_RELAX_EQUILIBRIUM)
+        return fail(HHG_ERROR_UNSUPPORTED,
+                    at + "equilibrium relaxation is the 3D pi-flux and honeycomb lattices'");
+    // the pi-flux equilibrium relaxation reads the integrator's state at the
+    // start of every step (PiFlux.cpp:144), so j(t) depends on the step
+    // sequence: only the reference's restart controller reproduces it
+    if (piflux3d && m.relax == HHG_RELAX_EQUILIBRIUM && sv.proposal != HHG_PROPOSAL_RESTART)
+        return fail(HHG_ERROR_UNSUPPORTED,
+                    at + "equilibrium relaxation depends on the step sequence: it needs HHG_PROPOSAL_RESTART");
+    if (m.relax != HHG_RELAX_NONE && !(m.relax_time > 0.0))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "relaxation needs relax_time > 0");
+    const bool lattice = m.kind == HHG_MODEL_PIFLUX2D || m.kind == HHG_MODEL_HONEYCOMB || piflux3d;
+    if (lattice && m.theta != 0.0)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "lattice models take the field along one axis (theta = 0)");
+    if (m.kind == HHG_MODEL_DIRAC3D && m.theta != 0.0)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D Dirac cone takes the field along z (theta = 0)");
+    if (lattice && !(m.hopping > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "lattice models need hopping > 0");
+    if (p.kind < 0 || p.kind > 5) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown pulse kind");
+    if (p.kind == HHG_PULSE_SPLINE &&
+        (!p.samples || p.n_samples < 2 || p.n_samples > (int64_t)1 << 30 || !(p.sample_dt > 0.0) ||
+         !std::isfinite(p.sample_t0) || !std::isfinite(p.slope_begin) || !std::isfinite(p.slope_end)))
+        return fail(HHG_ERROR_INVALID_ARGUMENT,
+                    at + "spline pulse needs 2 <= n_samples <= 2^30 samples, sample_dt > 0, finite t0 and slopes");
+    const bool gauss = p.kind == HHG_PULSE_GAUSS || p.kind == HHG_PULSE_DOUBLE_GAUSS;
+    const bool cos2 = p.kind == HHG_PULSE_COS2 || p.kind == HHG_PULSE_DOUBLE_COS2;
+    if (gauss && !(p.sigma > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "Gaussian pulse needs sigma > 0");
+    if (cos2 && !(p.n_cycles > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "cos^2 pulse needs n_cycles > 0");
+    if (t.n_measurements < 1 || t.n_subdivisions < 1 || !(t.t_end > t.t_begin))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "time grid needs N >= 1, n_subdivisions >= 1, t_end > t_begin");
+    const double delta = (t.t_end - t.t_begin) / t.n_measurements;
+    if (delta > 0.25 || (cos2 && delta / p.n_cycles > 0.25))
+        return fail(HHG_ERROR_UNSUPPORTED,
+                    at + "sampling interval must not exceed 0.25 (>= 26 samples per optical cycle)");
+    if (sv.method != HHG_METHOD_RKF78 && m.relax != HHG_RELAX_NONE)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the Magnus step is unitary: relaxation needs RKF78");
+    if (m.split_dispersion != 0.0 && (!piflux3d || sv.method != HHG_METHOD_RKF78 || !(m.split_dispersion > 0.0)))
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the split current is the 3D pi-flux lattice's with RKF78 and a cut > 0");
+    if (sv.method == HHG_METHOD_MAGNUS_MOMENTS && m.kind != HHG_MODEL_DIRAC3D)
+        return fail(HHG_ERROR_UNSUPPORTED, at + "the moment Magnus step is the 3D Dirac cone's (HHG_MODEL_DIRAC3D)");
+    return HHG_SUCCESS;
+}
+
+}  // namespace hhg
+
 extern "C" {
 
 int32_t hhg_abi_version(void) { return HHG_AMD_ABI_VERSION; }

@@ -31,6 +31,7 @@ struct SplinePulse {
     explicit SplinePulse(const hhg_pulse_desc& p);
     double operator()(double t) const;
 };
+hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n_kgrids, int idx);
 hhg_status trapezoidal_spectrum(const double* j, const hhg_time_desc& tc, int zero_padding, double* freq, double* re,
                                 double* im);
 

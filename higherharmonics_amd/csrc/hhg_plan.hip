@@ -6,6 +6,7 @@
 // DiracDispatcher.cpp:25-62, PiFluxDispatcher.cpp:46-105,
 // HoneycombDispatcher.cpp:28-56) and then computes j(t) (Dispatcher::compute).
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <chrono>
@@ -36,6 +37,15 @@ struct Group {
 };
 
 std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
+
+// A roctx range around a host call (rocprofv3 --marker-trace shows it beside
+// the kernels the call enqueued).
+struct TraceRange {
+    explicit TraceRange(const char* what) { roctxRangePushA(what); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 }  // namespace
 
@@ -150,59 +160,6 @@ hhg::SolverDev solver_dev(const hhg_solver_desc& sv) {
     sp.proposal = sv.proposal;
     sp.max_steps = sv.max_steps > 0 ? sv.max_steps : hhg::kDefaultMaxSteps;
     return sp;
-}
-
-hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n_kgrids, int idx) {
-    const std::string at = "job " + std::to_string(idx) + ": ";
-    const hhg_model_desc& m = jd.model;
-    const hhg_pulse_desc& p = jd.pulse;
-    const hhg_time_desc& t = jd.time;
-    if (jd.kgrid < 0 || jd.kgrid >= n_kgrids) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "k-grid index out of range");
-    if (m.kind < 0 || m.kind > 4) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown model kind");
-    if (m.relax < 0 || m.relax > 3) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown relaxation kind");
-    const bool piflux3d = m.kind == HHG_MODEL_PIFLUX3D;
-    if (piflux3d && m.relax != HHG_RELAX_NONE && m.relax != HHG_RELAX_EQUILIBRIUM)
-        return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D pi-flux lattice relaxes toward its equilibrium only");
-    if (!piflux3d && m.kind != HHG_MODEL_HONEYCOMB && m.relax == HHG_RELAX_EQUILIBRIUM)
-        return fail(HHG_ERROR_UNSUPPORTED,
-                    at + "equilibrium relaxation is the 3D pi-flux and honeycomb lattices'");
-    // the pi-flux equilibrium relaxation reads the integrator's state at the
-    // start of every step (PiFlux.cpp:144), so j(t) depends on the step
-    // sequence: only the reference's restart controller reproduces it
-    if (piflux3d && m.relax == HHG_RELAX_EQUILIBRIUM && sv.proposal != HHG_PROPOSAL_RESTART)
-        return fail(HHG_ERROR_UNSUPPORTED,
-                    at + "equilibrium relaxation depends on the step sequence: it needs HHG_PROPOSAL_RESTART");
-    if (m.relax != HHG_RELAX_NONE && !(m.relax_time > 0.0))
-        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "relaxation needs relax_time > 0");
-    const bool lattice = m.kind == HHG_MODEL_PIFLUX2D || m.kind == HHG_MODEL_HONEYCOMB || piflux3d;
-    if (lattice && m.theta != 0.0)
-        return fail(HHG_ERROR_UNSUPPORTED, at + "lattice models take the field along one axis (theta = 0)");
-    if (m.kind == HHG_MODEL_DIRAC3D && m.theta != 0.0)
-        return fail(HHG_ERROR_UNSUPPORTED, at + "the 3D Dirac cone takes the field along z (theta = 0)");
-    if (lattice && !(m.hopping > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "lattice models need hopping > 0");
-    if (p.kind < 0 || p.kind > 5) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "unknown pulse kind");
-    if (p.kind == HHG_PULSE_SPLINE &&
-        (!p.samples || p.n_samples < 2 || p.n_samples > (int64_t)1 << 30 || !(p.sample_dt > 0.0) ||
-         !std::isfinite(p.sample_t0) || !std::isfinite(p.slope_begin) || !std::isfinite(p.slope_end)))
-        return fail(HHG_ERROR_INVALID_ARGUMENT,
-                    at + "spline pulse needs 2 <= n_samples <= 2^30 samples, sample_dt > 0, finite t0 and slopes");
-    const bool gauss = p.kind == HHG_PULSE_GAUSS || p.kind == HHG_PULSE_DOUBLE_GAUSS;
-    const bool cos2 = p.kind == HHG_PULSE_COS2 || p.kind == HHG_PULSE_DOUBLE_COS2;
-    if (gauss && !(p.sigma > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "Gaussian pulse needs sigma > 0");
-    if (cos2 && !(p.n_cycles > 0.0)) return fail(HHG_ERROR_INVALID_ARGUMENT, at + "cos^2 pulse needs n_cycles > 0");
-    if (t.n_measurements < 1 || t.n_subdivisions < 1 || !(t.t_end > t.t_begin))
-        return fail(HHG_ERROR_INVALID_ARGUMENT, at + "time grid needs N >= 1, n_subdivisions >= 1, t_end > t_begin");
-    const double delta = (t.t_end - t.t_begin) / t.n_measurements;
-    if (delta > 0.25 || (cos2 && delta / p.n_cycles > 0.25))
-        return fail(HHG_ERROR_UNSUPPORTED,
-                    at + "sampling interval must not exceed 0.25 (>= 26 samples per optical cycle)");
-    if (sv.method != HHG_METHOD_RKF78 && m.relax != HHG_RELAX_NONE)
-        return fail(HHG_ERROR_UNSUPPORTED, at + "the Magnus step is unitary: relaxation needs RKF78");
-    if (m.split_dispersion != 0.0 && (!piflux3d || sv.method != HHG_METHOD_RKF78 || !(m.split_dispersion > 0.0)))
-        return fail(HHG_ERROR_UNSUPPORTED, at + "the split current is the 3D pi-flux lattice's with RKF78 and a cut > 0");
-    if (sv.method == HHG_METHOD_MAGNUS_MOMENTS && m.kind != HHG_MODEL_DIRAC3D)
-        return fail(HHG_ERROR_UNSUPPORTED, at + "the moment Magnus step is the 3D Dirac cone's (HHG_MODEL_DIRAC3D)");
-    return HHG_SUCCESS;
 }
 
 // Corners of the cos^2 envelopes (A'' jumps at the support edges) strictly
@@ -412,8 +369,10 @@ hhg_status collect_stats(hhg_plan* p, hhg_stats* stats, double wall_ms) {
 
 hhg_status run(hhg_plan* p, hipStream_t st, int n_items_limit = 0) {
     if (p->graph_exec && n_items_limit == 0 && st == p->stream) {
+        TraceRange range("hhg graph launch");
         HHG_TRY(hipGraphLaunch(p->graph_exec, st), "graph launch");
     } else {
+        TraceRange range("hhg enqueue");
         const hhg_status r = enqueue(p, st, n_items_limit);
         if (r != HHG_SUCCESS) return r;
     }
@@ -444,8 +403,9 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     const hhg_solver_desc& sv = d->solver;
     if (sv.method != HHG_METHOD_RKF78 && sv.method != HHG_METHOD_MAGNUS4 && sv.method != HHG_METHOD_MAGNUS_MOMENTS)
         return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown method");
-    if (sv.method == HHG_METHOD_RKF78 && (!(sv.rtol >= 0.0) || !(sv.atol >= 0.0) || !(sv.rtol + sv.atol > 0.0)))
-        return fail(HHG_ERROR_INVALID_ARGUMENT, "RKF78 needs rtol, atol >= 0, not both 0");
+    if (sv.method == HHG_METHOD_RKF78 && (!(sv.rtol >= 0.0) || !(sv.atol >= 0.0) || !(sv.rtol + sv.atol > 0.0) ||
+                                          !std::isfinite(sv.rtol) || !std::isfinite(sv.atol)))
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "RKF78 needs finite rtol, atol >= 0, not both 0");
     if (sv.proposal < 0 || sv.proposal > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown proposal");
     if (sv.proposal == HHG_PROPOSAL_LEARNED && (!sv.mlp || sv.method != HHG_METHOD_RKF78))
         return fail(HHG_ERROR_INVALID_ARGUMENT, "the learned proposal needs RKF78 and a network (solver.mlp)");
@@ -458,7 +418,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     if (sv.infer_policy != HHG_INFER_FIRST && sv.infer_policy != HHG_INFER_SCHEDULED)
         return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown infer_policy");
     for (int i = 0; i < d->n_jobs; ++i) {
-        const hhg_status st = validate_job(d->jobs[i], sv, d->n_kgrids, i);
+        const hhg_status st = hhg::validate_job(d->jobs[i], sv, d->n_kgrids, i);
         if (st != HHG_SUCCESS) return st;
     }
     int n_dev = 0;
@@ -703,6 +663,7 @@ hhg_status hhg_plan_job_offsets(const hhg_plan* p, int64_t* offsets) {
 hhg_status hhg_plan_execute(hhg_plan* p, double* j_out, hhg_stats* stats) {
     hhg::clear_error();
     if (!p || !j_out) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_execute: null argument");
+    TraceRange range("hhg_plan_execute");
     const auto t0 = std::chrono::steady_clock::now();
     HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
     hhg_status st = run(p, p->stream);
@@ -717,6 +678,7 @@ hhg_status hhg_plan_execute(hhg_plan* p, double* j_out, hhg_stats* stats) {
 hhg_status hhg_plan_execute_accumulators(hhg_plan* p, uint64_t* acc_out, hhg_stats* stats) {
     hhg::clear_error();
     if (!p || !acc_out) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_execute_accumulators: null argument");
+    TraceRange range("hhg_plan_execute_accumulators");
     const auto t0 = std::chrono::steady_clock::now();
     HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
     hhg_status st = run(p, p->stream);
@@ -732,6 +694,7 @@ hhg_status hhg_plan_execute_accumulators(hhg_plan* p, uint64_t* acc_out, hhg_sta
 hhg_status hhg_plan_execute_device(hhg_plan* p, double* j_dev, void* stream) {
     hhg::clear_error();
     if (!p || !j_dev) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_execute_device: null argument");
+    TraceRange range("hhg_plan_execute_device");
     HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
     hipStream_t st = stream ? (hipStream_t)stream : p->stream;
     hhg_status r = run(p, st);
@@ -753,6 +716,7 @@ hhg_status hhg_plan_last_stats(hhg_plan* p, hhg_stats* stats) {
 hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
     hhg::clear_error();
     if (!p) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_autotune: null argument");
+    TraceRange range("hhg_plan_autotune");
     HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
     if (p->graph_exec) return fail(HHG_ERROR_UNSUPPORTED, "autotune before capturing the graph");
     // Tune on a prefix of every group's items: enough waves to fill the chip a
@@ -796,6 +760,7 @@ hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
 hhg_status hhg_plan_capture_graph(hhg_plan* p) {
     hhg::clear_error();
     if (!p) return fail(HHG_ERROR_INVALID_ARGUMENT, "hhg_plan_capture_graph: null argument");
+    TraceRange range("hhg_plan_capture_graph");
     HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
     if (p->graph_exec) return HHG_SUCCESS;
     HHG_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");

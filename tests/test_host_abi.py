@@ -156,6 +156,13 @@ def test_spectrum_matches_oracle(hhg):
     (lambda j, s: setattr(s, "rtol", -1.0), 1),
     (lambda j, s: (setattr(s, "method", 1), setattr(j.model, "relax", 2), setattr(j.model, "relax_time", 3.0)), 5),
     (lambda j, s: setattr(j, "kgrid", 3), 1),
+    # non-finite inputs are refused before any device work
+    (lambda j, s: setattr(j.pulse, "amp_a", float("nan")), 1),
+    (lambda j, s: setattr(j.pulse, "cep", float("inf")), 1),
+    (lambda j, s: setattr(j.pulse, "delay", float("nan")), 1),
+    (lambda j, s: setattr(j.model, "hopping", float("nan")), 1),
+    (lambda j, s: setattr(j.time, "t_end", float("inf")), 1),
+    (lambda j, s: setattr(s, "rtol", float("inf")), 1),
 ])
 def test_plan_rejects_invalid_descriptions(hhg, mutate, status):
     from tests.problems import HONEYCOMB_SMALL as pr
@@ -169,6 +176,20 @@ def test_plan_rejects_invalid_descriptions(hhg, mutate, status):
     st = hhg.lib().hhg_plan_create(ctypes.byref(d), ctypes.byref(h))
     assert st == status
     assert hhg.lib().hhg_last_error_string()
+
+
+def test_plan_rejects_nonfinite_spline_sample(hhg):
+    """A NaN among a sampled pulse's samples is refused at plan creation
+    (INVALID_ARGUMENT naming the sample), not carried into the trajectories."""
+    from tests.problems import HONEYCOMB_SMALL as pr
+    samples = np.sin(np.linspace(0.0, 6.0, 200)) * 0.3
+    samples[77] = np.nan
+    laser = hhg.SplineLaser(samples, 0.0, 0.05)
+    with pytest.raises(hhg.HHGError) as ei:
+        hhg.Plan([hhg.Job(pr.h_model(hhg), laser, hhg.TimeIntegrationConfig(0.0, 9.0, 64, 50), 0)], [pr.h_grid(hhg)],
+                 hhg.Solver())
+    assert ei.value.status == hhg.ERROR_INVALID_ARGUMENT
+    assert "77" in str(ei.value)
 
 
 def test_gauss_legendre_1024_against_mpmath(hhg):
@@ -332,3 +353,12 @@ def test_per_point_entry_points_reject_null_arguments(hhg):
                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         assert st == hhg.ERROR_INVALID_ARGUMENT
         assert L.hhg_last_error_string()
+
+
+def test_plan_calls_are_traced_with_roctx_ranges(hhg):
+    """Plan execution, graph launch, autotuning and capture open roctx ranges
+    (rocprofv3 --marker-trace shows them): the library imports the range calls."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", hhg.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    imported = {line.split()[-1] for line in out.splitlines() if line.split()[:1] == ["U"] or " U " in line}
+    assert {"roctxRangePushA", "roctxRangePop"} <= imported

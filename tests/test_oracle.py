@@ -276,6 +276,43 @@ def test_relaxation_terms():
     assert np.allclose(O.rhs(m2, p, 0.5, 0.25, s0, s0, 1.0), 0.0, atol=1e-16)
 
 
+@pytest.mark.parametrize("mass,T2,A0", [(5.0, 0.5, 1.0), (5.0, 2.0, 1.0), (8.0, 1.0, 2.0)])
+def test_t2_dephasing_follows_shifted_band_adiabatically(mass, T2, A0):
+    """Field on, T2 dephasing: with a gap (2 x mass) far above the photon
+    energy the lower-band state follows the instantaneous -d^(k - A(t)) with a
+    small lag delta.  With s = -d^ + delta, ds/dt = 2 d x s - (s - (s.d^)d^)/T2
+    and a quasi-static lag, 2|d| d^ x delta - g delta = -d d^/dt (g = 1/T2), so
+    delta = (g d d^/dt + 2|d| d^ x d d^/dt) / (4|d|^2 + g^2).  That first-order
+    term takes out 90 % of the deviation (a wrong sign would double it); using
+    d(k) instead of d(k - A) in the precession or in the projection would leave
+    s near -d^(k) instead, a deviation more than five times larger."""
+    m = O.model(O.MODEL_DIRAC2D, v=1.0, mass=mass, relax=O.RELAX_T2, relax_time=T2)
+    T = 8 * math.pi
+    p = O.pulse(O.PULSE_COS2, A0, n_cycles=4.0, t0=-T / 2)
+    kx, ky = 0.3, 0.2
+    s, _, r = O.trajectory(m, p, O.time_grid(-T / 2, T / 2, 1024, 50), O.solver(rtol=1e-12, atol=1e-12), kx, ky)
+    assert r == 0
+    t = -T / 2 + np.arange(1025) * (T / 1024)
+
+    def field(tt):
+        A = O.pulse_A(p, tt)
+        d = np.stack([kx - A, np.full_like(A, ky), np.full_like(A, mass)], axis=1)
+        return d, d / np.linalg.norm(d, axis=1)[:, None]
+
+    d, dh = field(t)
+    ddh = (field(t + 1e-5)[1] - field(t - 1e-5)[1]) / 2e-5
+    nd, g = np.linalg.norm(d, axis=1)[:, None], 1.0 / T2
+    delta = (g * ddh + 2 * nd * np.cross(dh, ddh)) / (4 * nd ** 2 + g * g)
+    dev = np.max(np.linalg.norm(s + dh, axis=1))
+    d_unshifted = np.array([kx, ky, mass]) / np.linalg.norm([kx, ky, mass])
+    assert dev < 0.03
+    assert np.max(np.linalg.norm(s + d_unshifted, axis=1)) > 5 * dev
+    # dephasing of delta shortens s slightly (by 4e-3 at T2 = 0.5): compare directions
+    u = s / np.linalg.norm(s, axis=1)[:, None]
+    assert np.max(np.linalg.norm(u + dh - delta, axis=1)) < 0.15 * dev
+    assert np.max(np.linalg.norm(u + dh + delta, axis=1)) > 1.5 * dev
+
+
 # --------------------------------------------------------------------------
 # Controller semantics (published Boost.Odeint algorithm)
 # --------------------------------------------------------------------------
