@@ -972,16 +972,36 @@ constexpr double kFixScaleHi = 65536.0;                 // 2^16
 constexpr double kFixScaleLo = 18446744073709551616.0;  // 2^64
 constexpr double kFixLimit = 4611686018427387904.0;     // 2^62 (|x 2^16| bound)
 
-__device__ __forceinline__ bool fix_accumulate(unsigned long long* acc, double x) {
+__device__ __forceinline__ bool fix_split(double x, unsigned long long& lo, unsigned long long& hi) {
     const double y = x * kFixScaleHi;
     if (!(fabs(y) < kFixLimit)) return false;
     const double hi_d = floor(y);
     const double frac = y - hi_d;  // exact, in [0, 1)
-    const unsigned long long lo = (unsigned long long)(frac * kFixScaleLo);
-    const long long hi = (long long)hi_d;
+    lo = (unsigned long long)(frac * kFixScaleLo);
+    hi = (unsigned long long)(long long)hi_d;
+    return true;
+}
+
+// acc (2 words) += (lo, hi) as one 128-bit two's-complement addition; the
+// carry comes from the low word's old value.  Addition modulo 2^128 is
+// associative, so partial sums may be added in any grouping and order.
+__device__ __forceinline__ void fix_add(unsigned long long* acc, unsigned long long lo, unsigned long long hi) {
     const unsigned long long old = atomicAdd(acc, lo);
     const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-    atomicAdd(acc + 1, (unsigned long long)hi + carry);
+    atomicAdd(acc + 1, hi + carry);
+}
+
+// fix_add on the block's LDS copy (workgroup-scope atomics)
+__device__ __forceinline__ void fix_add_lds(unsigned long long* acc, unsigned long long lo, unsigned long long hi) {
+    const unsigned long long old = __hip_atomic_fetch_add(acc, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
+    __hip_atomic_fetch_add(acc + 1, hi + carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ bool fix_accumulate(unsigned long long* acc, double x) {
+    unsigned long long lo, hi;
+    if (!fix_split(x, lo, hi)) return false;
+    fix_add(acc, lo, hi);
     return true;
 }
 

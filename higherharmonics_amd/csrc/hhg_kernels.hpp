@@ -15,8 +15,12 @@ struct KernelArgs {
     int queue_chunk;
     int infer_every;     // learned proposal, scheduled policy: intervals between network evaluations
     int fallback_after;  // learned proposal: rejections before the plain controller takes over
-    int infer_policy;    // learned proposal: 0 gated (first interval and after a rejection), 1 scheduled
+    int infer_policy;    // learned proposal: HHG_INFER_FIRST or HHG_INFER_SCHEDULED
     int count_waves;     // nonzero: count the attempts every wave issues (Counters::wave_attempts)
+    int block_acc;       // nonzero: block accumulation in LDS (the block pops waves x queue_chunk items)
+    int acc_lds_off;     // byte offset of the LDS accumulators in the dynamic LDS
+    int acc_lds_samples; // samples the LDS accumulators hold (the group's largest job output)
+    int pad2;
     const double* kx;
     const double* ky;
     const double* kz;  // 3D pi-flux lattice only (nullptr otherwise)

@@ -34,6 +34,8 @@ struct Group {
     int model, pulse, relax;
     int item_begin, n_items;
     int waves_per_block, queue_chunk, blocks;
+    int acc_samples;  // the largest output of the group's jobs (samples)
+    int block_acc;    // block accumulation in LDS (size_group decides)
 };
 
 std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
@@ -95,6 +97,7 @@ struct hhg_plan {
     int64_t n_kpts = 0;
     std::vector<Group> groups;
     int num_cus = 0;
+    size_t lds_per_block = 0;  // the device's dynamic LDS limit per block
 
     double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_w = nullptr;
     double* d_spl = nullptr;  // spline pulse coefficients of every spline job, concatenated
@@ -125,8 +128,12 @@ struct hhg_plan {
     // the solve kernels' learned-proposal instantiation (LEARN_* of hhg_solve.hpp): the network runs
     // inside the solve kernel only for the scheduled policy
     int solve_learn() const { return learned() && solver.infer_policy == HHG_INFER_SCHEDULED ? 2 : 0; }
-    size_t smem(int /*waves_per_block*/) const {
-        return solve_learn() ? (size_t)hhg::kMlpLdsBytes : 0;
+    // dynamic LDS of a group's solve launch: the network (scheduled learned
+    // policy), then the block accumulators (2 words per sample)
+    size_t mlp_lds() const { return solve_learn() ? (size_t)hhg::kMlpLdsBytes : 0; }
+    size_t acc_lds_off() const { return (mlp_lds() + 15) & ~(size_t)15; }
+    size_t smem(const Group& g) const {
+        return g.block_acc ? acc_lds_off() + 16 * (size_t)g.acc_samples : mlp_lds();
     }
 
     // at least one counter block, so trajectory launches have one even when
@@ -266,21 +273,37 @@ int max_block_threads(const hhg_plan* p, const Group& g) {
     return attr.maxThreadsPerBlock;
 }
 
-int resident_blocks(const hhg_plan* p, const Group& g, int threads) {
+int blocks_per_cu(const hhg_plan* p, const Group& g, int threads, size_t smem) {
     const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn());
     int per_cu = 0;
-    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, p->smem(threads / 64)) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    return per_cu * p->num_cus;
+    if (smem > p->lds_per_block) return 0;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, smem) != hipSuccess) return 0;
+    return per_cu;
 }
 
+int resident_blocks(const hhg_plan* p, const Group& g, int threads) {
+    return std::max(1, blocks_per_cu(p, g, threads, p->smem(g))) * p->num_cus;
+}
+
+// Launch shape of a group.  Persistent launches with block_reduce 2
+// accumulate in LDS per block where the block's LDS copy of the largest job
+// output fits; waves_per_block 0 takes 8 there (one block of the kernel's 8
+// waves per CU), else 4.
 void size_group(const hhg_plan* p, Group& g, int waves_per_block, int queue_chunk, int persistent) {
-    g.waves_per_block = waves_per_block;
+    const bool want_block = persistent && p->launch.block_reduce == 2;
+    auto fits = [&](int wpb) {
+        Group t = g;
+        t.block_acc = 1;
+        return blocks_per_cu(p, t, 64 * wpb, p->smem(t)) > 0;
+    };
+    int wpb = waves_per_block;
+    if (wpb <= 0) wpb = (want_block && fits(8)) ? 8 : 4;
+    g.waves_per_block = wpb;
     g.queue_chunk = queue_chunk;
-    const int items_blocks = (g.n_items + waves_per_block - 1) / waves_per_block;
+    g.block_acc = want_block && fits(wpb) ? 1 : 0;
+    const int items_blocks = (g.n_items + wpb - 1) / wpb;
     if (persistent) {
-        const int res = resident_blocks(p, g, 64 * waves_per_block);
+        const int res = resident_blocks(p, g, 64 * wpb);
         g.blocks = std::max(1, std::min(items_blocks, res));
     } else {
         g.queue_chunk = 1;
@@ -316,6 +339,9 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
         a.fallback_after = p->solver.fallback_after > 0 ? p->solver.fallback_after : 2;
         a.infer_policy = p->solver.infer_policy;
         a.count_waves = p->launch.count_wave_attempts;
+        a.block_acc = g.block_acc;
+        a.acc_lds_off = (int)p->acc_lds_off();
+        a.acc_lds_samples = g.acc_samples;
         a.log = p->d_log;
         a.log_count = &p->counters(0)->log_count;
         a.log_cap = p->log_cap;
@@ -328,7 +354,7 @@ hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
             a.h_first = p->d_h_first;
         }
         HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn(), a, std::max(1, blocks),
-                                  64 * g.waves_per_block, p->smem(g.waves_per_block), st),
+                                  64 * g.waves_per_block, p->smem(g), st),
                 "solve launch");
     }
     HHG_TRY(hipEventRecord(p->ev[1], st), "event");
@@ -442,6 +468,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         if (hipGetDeviceProperties(&prop, p->device) != hipSuccess)
             return bail(fail(HHG_ERROR_HIP, "hipGetDeviceProperties failed"));
         p->num_cus = prop.multiProcessorCount;
+        p->lds_per_block = prop.sharedMemPerBlock;
     }
 
     // k-grids, concatenated
@@ -522,6 +549,9 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         for (int j : js)
             for (int c = 0; c < p->jobs[j].n_items; ++c) items.push_back(make_int2(j, c));
         g.n_items = (int)items.size() - g.item_begin;
+        g.acc_samples = 0;
+        for (int j : js)
+            g.acc_samples = std::max(g.acc_samples, (int)(p->offsets[j + 1] - p->offsets[j]));
         if (g.n_items > 0) p->groups.push_back(g);
     }
     if (items.size() > (size_t)0x7fffffff) return bail(fail(HHG_ERROR_UNSUPPORTED, "more than 2^31 work items"));
@@ -640,13 +670,20 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     }
 
     // launch shapes
-    const int wpb = p->launch.waves_per_block > 0 ? p->launch.waves_per_block : 4;
+    const int wpb = p->launch.waves_per_block;  // 0: chosen per group
     const int qc = p->launch.queue_chunk > 0 ? p->launch.queue_chunk : 1;
-    if (wpb > 8 || (wpb & (wpb - 1))) return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block must be 1, 2, 4 or 8"));
+    if (wpb < 0 || wpb > 8 || (wpb & (wpb - 1)))
+        return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block must be 0, 1, 2, 4 or 8"));
+    if (p->launch.block_reduce < 0 || p->launch.block_reduce > 2)
+        return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "block_reduce must be 0, 1 or 2"));
     for (auto& g : p->groups)
-        if (64 * wpb > max_block_threads(p, g))
+        if (64 * std::max(wpb, 8) > max_block_threads(p, g))
             return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block exceeds the kernel's block size limit"));
-    for (auto& g : p->groups) size_group(p, g, wpb, qc, p->launch.persistent);
+    for (auto& g : p->groups) {
+        size_group(p, g, wpb, qc, p->launch.persistent);
+        if (p->launch.block_reduce == 2 && p->launch.persistent && !g.block_acc)
+            return bail(fail(HHG_ERROR_UNSUPPORTED, "block_reduce 2: a job's output does not fit the block's LDS"));
+    }
     *out = p;
     return HHG_SUCCESS;
 }

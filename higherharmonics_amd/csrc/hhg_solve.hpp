@@ -56,18 +56,35 @@ __device__ __forceinline__ T load_uniform(const T* p) {
     return out;
 }
 
-// Per-lane buffering of one sample sum and the 64-sample flush.
+extern __shared__ __align__(16) unsigned char solve_dyn_lds[];
+
+// Per-lane buffering of one sample sum and the 64-sample flush, into the
+// job's accumulators in HBM or, in block-accumulation launches, into the
+// block's LDS copy of them (lds >= 0: its first word's index in the LDS
+// accumulator region at KernelArgs::acc_lds_off).
 struct SampleSink {
     unsigned long long* acc;  // job accumulators, 2 words per sample
     int N;
     double buf;
     bool overflow;
+    int lds;                  // -1: HBM; else the sink's first word in the LDS accumulators
+    int lds_off;              // byte offset of the LDS accumulators in the dynamic LDS
     __device__ __forceinline__ void put(int i, double total, int lane) {
         const int slot = i & (WAVE - 1);
         if (lane == slot) buf = total;
         if (slot == WAVE - 1 || i == N) {
             const int base = i - slot;
-            if (lane <= slot) overflow |= !fix_accumulate(acc + 2 * (int64_t)(base + lane), buf);
+            if (lane <= slot) {
+                unsigned long long lo, hi;
+                if (!fix_split(buf, lo, hi)) {
+                    overflow = true;
+                } else if (lds >= 0) {
+                    unsigned long long* l = reinterpret_cast<unsigned long long*>(solve_dyn_lds + lds_off);
+                    fix_add_lds(l + lds + 2 * (base + lane), lo, hi);
+                } else {
+                    fix_add(acc + 2 * (int64_t)(base + lane), lo, hi);
+                }
+            }
         }
     }
 };
@@ -137,8 +154,6 @@ __device__ __forceinline__ void interval_features(const F& f, JobRef J, const do
     feat[7] = __log10f((float)(last_err > (1.0 / 390625.0) ? last_err : (1.0 / 390625.0)));
 }
 
-extern __shared__ __align__(16) unsigned char solve_dyn_lds[];
-
 // third coordinate of a k-point (3D pi-flux lattice; the other models have none)
 template <int M>
 __device__ __forceinline__ double load_kz(const double* kz, int64_t idx) {
@@ -147,7 +162,7 @@ __device__ __forceinline__ double load_kz(const double* kz, int64_t idx) {
 }
 
 template <int M, int P, int R, int LEARN>
-__device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
+__device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc, bool to_lds) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t kk = (int64_t)chunk * WAVE + lane;
     const bool active = kk < J.n_k;
@@ -169,12 +184,12 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
         gap_k = 2.0 * sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     }
     const int N = J.n_meas;
-    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
+    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false, to_lds ? 0 : -1, a.acc_lds_off};
     const double* At = a.At + J.sample_base;
     const double* Pt = a.Pt + 2 * J.sample_base;  // phase models: (sin f A_i, cos f A_i)
     // 3D pi-flux split current (PiFlux.cpp:998-1006): the samples of points
     // with |m(k - A)| >= split_cut go to the job's second output
-    SampleSink sink_hi{a.acc + 2 * (J.sample_base + N + 1), N, 0.0, false};
+    SampleSink sink_hi{a.acc + 2 * (J.sample_base + N + 1), N, 0.0, false, to_lds ? 2 * (N + 1) : -1, a.acc_lds_off};
     const bool split = M == MODEL_PIFLUX3D && J.split_cut > 0.0;
     auto emit = [&](int i, double g) {
         if constexpr (M == MODEL_PIFLUX3D) {
@@ -289,7 +304,7 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
 }
 
 template <int M>
-__device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
+__device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc, bool to_lds) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t kk = (int64_t)chunk * WAVE + lane;
     const bool active = kk < J.n_k;
@@ -307,7 +322,7 @@ __device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, Lane
     const int N = J.n_meas;
     const int ms = J.magnus_sub;
     const double h = J.delta / ms;
-    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
+    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false, to_lds ? 0 : -1, a.acc_lds_off};
     const double* At = a.At + J.sample_base;
     const double* Am = a.Am + J.magnus_base;
     sink.put(0, wave_sum(active ? w * f.vertex(At[0], s) : 0.0), lane);
@@ -328,7 +343,7 @@ __device__ void solve_item_magnus(JobRef J, int chunk, const KernelArgs& a, Lane
 // The reference's moment-based Magnus step of the 3D Dirac cone, one per
 // sampling interval; Am holds the four pulse moments of every interval.
 template <int M>
-__device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc) {
+__device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc, bool to_lds) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t kk = (int64_t)chunk * WAVE + lane;
     const bool active = kk < J.n_k;
@@ -344,7 +359,7 @@ __device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs&
     f.initial(s);
     const MomentLane L = moment_lane(kx, ky, J.delta);
     const int N = J.n_meas;
-    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false};
+    SampleSink sink{a.acc + 2 * J.sample_base, N, 0.0, false, to_lds ? 0 : -1, a.acc_lds_off};
     const double* Am = a.Am + J.magnus_base;
     sink.put(0, wave_sum(active ? w * s[2] : 0.0), lane);
     for (int i = 1; i <= N; ++i) {
@@ -376,25 +391,69 @@ solve_kernel(KernelArgs a) {
         mlp_stage_lds(a.mlp, solve_dyn_lds);
         __syncthreads();
     }
+    // Work is popped from the group's queue counter.  Wave accumulation: each
+    // wave pops queue_chunk items and adds its sample sums to HBM.  Block
+    // accumulation: the block pops waves x queue_chunk items; its waves add
+    // their sample sums to the block's LDS copy of the current job's
+    // accumulators, which goes to HBM (one 128-bit addition per sample) when
+    // the block moves to another job and at the end; items of another job
+    // than the round's first (at a job boundary) add to HBM directly.
+    __shared__ int s_base;
+    unsigned long long* lacc = reinterpret_cast<unsigned long long*>(solve_dyn_lds + a.acc_lds_off);
+    const bool block = a.block_acc != 0;
+    const int waves = blockDim.x / WAVE, wave = threadIdx.x / WAVE;
+    const int per_pop = block ? waves * a.queue_chunk : a.queue_chunk;
+    int cur_job = -1;
+    auto flush_lds = [&](int job) {  // every thread of the block: LDS words read, zeroed, added to HBM
+        JobRef J = job_ref(a.jobs + job);
+        const int n_s = (J.split_cut > 0.0 ? 2 : 1) * (J.n_meas + 1);
+        for (int q = threadIdx.x; q < n_s; q += blockDim.x) {
+            const unsigned long long lo = lacc[2 * q], hi = lacc[2 * q + 1];
+            lacc[2 * q] = 0ull;
+            lacc[2 * q + 1] = 0ull;
+            if (lo | hi) fix_add(a.acc + 2 * (J.sample_base + q), lo, hi);
+        }
+    };
+    if (block)
+        for (int q = threadIdx.x; q < 2 * a.acc_lds_samples; q += blockDim.x) lacc[q] = 0ull;
     for (;;) {
-        int base = 0;
-        if (lane == 0) base = (int)atomicAdd(&a.ctr->queue, (unsigned long long)a.queue_chunk);
-        base = uniform(base);
+        int base;
+        if (block) {
+            __syncthreads();  // the previous round's waves are done with s_base and the LDS sums
+            if (threadIdx.x == 0) s_base = (int)atomicAdd(&a.ctr->queue, (unsigned long long)per_pop);
+            __syncthreads();
+            base = s_base;
+        } else {
+            base = 0;
+            if (lane == 0) base = (int)atomicAdd(&a.ctr->queue, (unsigned long long)per_pop);
+            base = uniform(base);
+        }
         if (base >= a.n_items) break;
-        const int end = min(base + a.queue_chunk, a.n_items);
-        for (int it = base; it < end; ++it) {
+        if (block) {
+            const int lead = a.items[base].x;
+            if (lead != cur_job) {
+                if (cur_job >= 0) flush_lds(cur_job);
+                cur_job = lead;
+                __syncthreads();  // zeroed before any wave adds
+            }
+        }
+        for (int c = 0; c < a.queue_chunk; ++c) {
+            const int it = block ? base + c * waves + wave : base + c;
+            if (it >= a.n_items) break;
             const int2 item = load_uniform(a.items + it);
             LaneCounters lc;
+            const bool to_lds = block && item.x == cur_job;
             if constexpr (METHOD == METHOD_RKF78)
-                solve_item_rkf78<M, P, R, LEARN>(job_ref(a.jobs + item.x), item.y, a, lc);
+                solve_item_rkf78<M, P, R, LEARN>(job_ref(a.jobs + item.x), item.y, a, lc, to_lds);
             else if constexpr (METHOD == METHOD_MAGNUS4)
-                solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc);
+                solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc, to_lds);
             else
-                solve_item_magnus_moments<M>(job_ref(a.jobs + item.x), item.y, a, lc);
+                solve_item_magnus_moments<M>(job_ref(a.jobs + item.x), item.y, a, lc, to_lds);
             // the item's counts, its failures attributed to its job
             flush_counters(a.ctr, a.job_fail, item.x, lc);
         }
     }
+    if (block && cur_job >= 0) flush_lds(cur_job);
 }
 
 // Per-k trajectories of one job (tests / inspection): one lane per selected
@@ -638,8 +697,8 @@ struct ModelLaunch {
                 *fn = reinterpret_cast<const void*>(&solve_kernel<1, M, PULSE_COS2, RELAX_NONE, LEARN_NONE>);
                 return hipSuccess;
             }
-            hipLaunchKernelGGL((solve_kernel<1, M, PULSE_COS2, RELAX_NONE, LEARN_NONE>), dim3(blocks), dim3(threads), 0, st,
-                               a);
+            hipLaunchKernelGGL((solve_kernel<1, M, PULSE_COS2, RELAX_NONE, LEARN_NONE>), dim3(blocks), dim3(threads), smem,
+                               st, a);
             return hipGetLastError();
         }
         if (method == METHOD_MAGNUS_MOMENTS) {
@@ -649,7 +708,7 @@ struct ModelLaunch {
                     return hipSuccess;
                 }
                 hipLaunchKernelGGL((solve_kernel<2, M, PULSE_CONTINUOUS, RELAX_NONE, LEARN_NONE>), dim3(blocks),
-                                   dim3(threads), 0, st, a);
+                                   dim3(threads), smem, st, a);
                 return hipGetLastError();
             }
             return hipErrorInvalidValue;

@@ -242,13 +242,17 @@ typedef struct hhg_solver_desc {
 } hhg_solver_desc;
 
 typedef struct hhg_launch_desc {
-    int32_t waves_per_block; /* 0 = default (4); 1, 2, 4 or 8 waves of 64 lanes per block */
+    int32_t waves_per_block; /* 0 = 4, or 8 with block accumulation; or 1, 2, 4, 8 */
     int32_t queue_chunk;     /* 0 = default/autotuned: work items grabbed per queue pop */
     int32_t persistent;      /* 1 = persistent grid + device work queue, 0 = one wave per item */
     int32_t group_by_cost;   /* 1 = order k-points inside a job by the network's predicted step count
                                 (needs solver.mlp) so that lanes of a wave need similar work */
     int32_t count_wave_attempts; /* 1 = count hhg_stats.wave_attempts (a few instructions per interval) */
-    int32_t pad;
+    int32_t block_reduce;    /* persistent launches: 0 or 1 = every wave adds its exact sample sums to the job's
+                                accumulators in HBM; 2 = the waves of a block add them to an LDS copy of the
+                                job's accumulators, which goes to HBM once per job and block (far less HBM
+                                traffic, about 10 % more solve time on MI355X; HHG_ERROR_UNSUPPORTED where a
+                                job's output does not fit the block's LDS).  j is bitwise the same either way */
 } hhg_launch_desc;
 
 typedef struct hhg_plan_desc {

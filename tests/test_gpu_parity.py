@@ -841,3 +841,31 @@ def test_full_size_field_reversal(hhg, config):
     assert st.failed_trajectories == 0 and st.nonfinite_trajectories == 0
     assert np.abs(j_plus).max() > 1e-6
     assert err <= 1e-12
+
+
+def test_block_accumulation_bitwise_equal_to_wave_atomics(hhg):
+    """Block accumulation (the waves of a block add their exact 128-bit sample
+    sums in an LDS copy of the job's accumulators, which goes to HBM once per
+    job) against every wave adding to HBM: bitwise the same accumulators for
+    jobs of different lengths in one plan (job boundaries inside a block's
+    round), block shapes 4 and 8 waves with queue chunks 1 and 2, the split
+    current's two outputs, both Magnus methods and a graph replay."""
+    from higherharmonics_amd import workloads as W
+    from tests.problems import DIRAC3D_CONTINUOUS, PIFLUX3D_SPLIT
+    print("[case] block accumulation vs wave atomics", flush=True)
+    jobs = W.mixed_sweep(n=20, n_each=3, per_cycle=64, cos2_cycles=2, sigma_cycles=0.5)
+    with W.build_plan(hhg, jobs, block_reduce=1) as plan:
+        ref = plan.execute_accumulators()
+    for wpb, qc in ((4, 1), (8, 2)):
+        with W.build_plan(hhg, jobs, block_reduce=2, waves_per_block=wpb, queue_chunk=qc) as plan:
+            assert np.array_equal(plan.execute_accumulators(), ref)
+            plan.capture_graph()
+            plan.execute()
+            assert np.array_equal(plan.execute_accumulators(), ref)
+    for pr, solver in ((PIFLUX3D_SPLIT, {}), (DIRAC_COHERENT, dict(method=1, magnus_substeps=2)),
+                       (DIRAC3D_CONTINUOUS, dict(method=2))):
+        _print_case(pr, solver=solver)
+        with pr.h_plan(hhg, solver=solver, block_reduce=1) as plan:
+            ref = plan.execute_accumulators()
+        with pr.h_plan(hhg, solver=solver, block_reduce=2, waves_per_block=8) as plan:
+            assert np.array_equal(plan.execute_accumulators(), ref)
