@@ -275,12 +275,14 @@ def oracle_audit(H, W, jobs, solvers, frac=0.01, threads=None):
             plan.close()
     t0 = time.perf_counter()
     ref = []
-    for job in sub:
+    for i, job in enumerate(sub):
         m, p, tc = oracle_job(O, job)
         g = job["grid"]
         j, _ = O.current_density(m, p, tc, O.solver(rtol=1e-13, atol=1e-13, proposal=O.PROPOSAL_CARRY),
                                  g[1], g[2], g[3], n_threads=threads, fast=False)
         ref.append(j)
+        print(f"[bench] oracle audit job {i + 1}/{len(sub)}: {len(g[1])} k-points, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     cpu_s = time.perf_counter() - t0
     out = {"k_points": W.trajectories(sub), "fraction": frac, "oracle": "oracle (strict IEEE build), "
            "rtol = atol = 1e-13, carried step", "oracle_seconds": cpu_s, "oracle_threads": threads}
@@ -450,67 +452,61 @@ def config2_leg(H, steps=3):
     net.close()
     for name in js:
         out["legs"][name]["rel_l2_vs_same_kernel_tight"] = rel_l2(js[name], j_tight)
-    b = leg_summary(blk, K, f_attempt, vertex)
-    b["bitwise_equal_to_learned_leg"] = all(np.array_equal(x, y) for x, y in zip(blk["j"], legs["learned"]["j"]))
-    b["hbm_write_kib_per_launch"] = {"block_reduce_2": 737608.0, "per_wave_atomics": 26376256.0,
-                                     "source": "rocprofv3 --pmc WRITE_SIZE, one launch each"}
-    out["learned_block_accumulation"] = b
     L = out["legs"]
     out["learned_vs_plain"] = L["learned"]["trajectories_per_s"] / L["plain"]["trajectories_per_s"]
     out["learned_vs_carry"] = L["learned"]["trajectories_per_s"] / L["carry"]["trajectories_per_s"]
     return out
 
 
-def config5_leg(H, n=256, log_n=32):
+def config5_leg(H, n=256, log_n=32, audit_frac=0.01):
     """64 honeycomb (double cos^2) + 64 PiFlux (double Gaussian) seeded pulse
-    configurations on n x n grids in one plan: plain vs learned proposal with
-    step-count grouping, both autotuned and graph-captured.  The network is
-    trained on step logs of the same jobs on log_n x log_n grids.  Audit: a
-    seeded random 1 % of every job's k-points solved with the learned proposal,
-    the plain controller and a tight tolerance."""
-    from higherharmonics_amd import learn
+    configurations on n x n grids in one plan.  Legs: the plain controller in
+    one fused launch over both groups and in one launch per group, the carried
+    controller, the learned proposal, and the learned proposal with step-count
+    grouping; every plan autotuned and graph-captured, one warmup and one
+    timed execution.  The network is trained on step logs of the same jobs on
+    log_n x log_n grids.  Audit: a seeded 1 % of every job's k-points against
+    the CPU oracle at rtol = atol = 1e-13."""
     from higherharmonics_amd import workloads as W
     jobs = W.mixed_sweep(n=n)
     K = W.trajectories(jobs)
     small = W.mixed_sweep(n=log_n)
-    cap = 1 << 21
-    stride = max(1, -(-sum(j["grid"][1] * j["grid"][2] * j["time"][2] for j in small) // cap))
-    t0 = time.perf_counter()
-    net, rec, hist = learn.train_on_solver_logs(lambda sv: W.build_plan(H, small, sv, persistent=1), capacity=cap,
-                                                stride=stride, rounds=2)
-    losses = hist[-1]
+    print(f"[bench] configuration 5: {len(jobs)} jobs, {K} trajectories; training the proposal network",
+          file=sys.stderr, flush=True)
+    net, train = train_on_jobs(H, W, small)
     res = {"workload": f"64 honeycomb double-cos^2 + 64 PiFlux double-Gaussian pulses, GL {n}x{n} each, "
-                       f"256 samples/cycle", "jobs": len(jobs), "trajectories": K,
-           "training": {"records": int(len(rec)), "log_stride": stride, "seconds": time.perf_counter() - t0,
-                        "loss_first": float(losses[0]), "loss_last": float(losses[-1])}}
-    for name, sv, kw in [("plain", H.Solver(), {}),
-                         ("learned_grouped", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {"group_by_cost": 1})]:
-        print(f"[bench] config 5 {name}", file=sys.stderr, flush=True)
-        plan = W.build_plan(H, jobs, sv, persistent=1, **kw)
-        tuned = plan.autotune()
-        plan.capture_graph()
-        plan.execute()
-        t1 = time.perf_counter()
-        plan.execute()
-        wall = time.perf_counter() - t1
-        st = plan.stats.as_dict()
-        plan.close()
-        res[name] = {"trajectories_per_s": K / wall, "wall_s": wall, "attempts": st["attempts"],
-                     "rejects": st["rejects"], "fallbacks": st["proposal_fallbacks"], "launch": tuned}
-    res["learned_vs_plain"] = res["learned_grouped"]["trajectories_per_s"] / res["plain"]["trajectories_per_s"]
-    print("[bench] config 5 audit", file=sys.stderr, flush=True)
-    sub = W.audit_subset(H, jobs, 0.01)
-    tight = [dict(j, time=(j["time"][0], j["time"][1], j["time"][2], 500)) for j in sub]
-    out = {}
-    for name, js, sv in [("learned", sub, H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net)), ("plain", sub, H.Solver()),
-                         ("tight", tight, H.Solver(rtol=1e-13, atol=1e-13))]:
-        plan = W.build_plan(H, js, sv, persistent=1)
-        out[name] = plan.execute()
-        plan.close()
-    res["audit"] = {"k_points": W.trajectories(sub), "fraction": 0.01,
-                    "max_rel_l2_learned_vs_plain": max(rel_l2(a, b) for a, b in zip(out["learned"], out["plain"])),
-                    "max_rel_l2_learned_vs_tight": max(rel_l2(a, b) for a, b in zip(out["learned"], out["tight"])),
-                    "max_rel_l2_plain_vs_tight": max(rel_l2(a, b) for a, b in zip(out["plain"], out["tight"]))}
+                       f"256 samples/cycle, RKF78 rtol=atol=1e-10", "jobs": len(jobs), "trajectories": K,
+           "training": train, "legs": {}}
+    learned = H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net)
+    for name, sv, kw in [("plain", H.Solver(), {}), ("plain_separate_launches", H.Solver(), {"separate_launches": 1}),
+                         ("carry", H.Solver(proposal=H.PROPOSAL_CARRY), {}), ("learned", learned, {}),
+                         ("learned_grouped", learned, {"group_by_cost": 1})]:
+        print(f"[bench] configuration 5 leg {name}", file=sys.stderr, flush=True)
+        plan = W.build_plan(H, jobs, sv, persistent=1, count_wave_attempts=1, **kw)
+        try:
+            tuned = plan.autotune()
+            plan.capture_graph()
+            plan.execute()
+            t1 = time.perf_counter()
+            plan.execute()
+            wall = time.perf_counter() - t1
+            st = plan.stats.as_dict()
+            launches = plan.solve_launches
+        finally:
+            plan.close()
+        res["legs"][name] = {"trajectories_per_s": K / wall, "wall_s": wall, "solve_ms": st["solve_ms"],
+                             "solve_launches": launches, "attempts": st["attempts"], "rejects": st["rejects"],
+                             "wave_attempts": st["wave_attempts"], "fallbacks": st["proposal_fallbacks"],
+                             "launch": tuned}
+    L = res["legs"]
+    res["learned_vs_plain"] = L["learned"]["trajectories_per_s"] / L["plain"]["trajectories_per_s"]
+    res["learned_vs_carry"] = L["learned"]["trajectories_per_s"] / L["carry"]["trajectories_per_s"]
+    res["fused_vs_separate_launches"] = L["plain"]["trajectories_per_s"] / \
+        L["plain_separate_launches"]["trajectories_per_s"]
+    print(f"[bench] configuration 5 oracle audit on {audit_frac:.2%} of every job's k-points", file=sys.stderr,
+          flush=True)
+    res["audit"] = oracle_audit(H, W, jobs, {"learned": learned, "plain": H.Solver(),
+                                             "carry": H.Solver(proposal=H.PROPOSAL_CARRY)}, audit_frac)
     net.close()
     return res
 
@@ -643,8 +639,10 @@ def main():
     ap.add_argument("--sweeps", action="store_true", help="also time reduced configurations 3-5")
     ap.add_argument("--config3-full", action="store_true",
                     help="only configuration 3 at full size (33.5M trajectories, one launch; one JSON object)")
-    ap.add_argument("--config5", action="store_true",
-                    help="also run the mixed 128-pulse sweep, plain vs learned + grouping, with a 1%% audit")
+    ap.add_argument("--config5-only", action="store_true",
+                    help="only the mixed 128-pulse sweep (configuration 5; one JSON object)")
+    ap.add_argument("--profile-config5", choices=["plain", "plain_separate_launches", "carry"], default=None,
+                    help="only time this configuration-5 leg (autotuned, graph-captured; profiler runs)")
     ap.add_argument("--reference-models-only", action="store_true",
                     help="only the reference systems' leg (one JSON object)")
     ap.add_argument("--reference-models", action="store_true",
@@ -672,6 +670,29 @@ def main():
         return
     if args.config3_full:
         print(json.dumps({"config3_full": config3_full_leg(H)}))
+        return
+    if args.profile_config5:
+        from higherharmonics_amd import workloads as W
+        jobs = W.mixed_sweep()
+        leg = args.profile_config5
+        sv = H.Solver(proposal=H.PROPOSAL_CARRY) if leg == "carry" else H.Solver()
+        plan = W.build_plan(H, jobs, sv, persistent=1, separate_launches=int(leg == "plain_separate_launches"))
+        tuned = plan.autotune()
+        plan.capture_graph()
+        for _ in range(args.warmup):
+            plan.execute()
+        t0 = time.perf_counter()
+        ms = []
+        for _ in range(args.steps):
+            plan.execute()
+            ms.append(plan.stats.solve_ms)
+        wall = time.perf_counter() - t0
+        print(json.dumps({"profile_config5": leg, "solve_launches": plan.solve_launches, "launch": tuned,
+                          "trajectories_per_s": args.steps * W.trajectories(jobs) / wall, "graph_ms": ms}))
+        plan.close()
+        return
+    if args.config5_only:
+        print(json.dumps({"config5_mixed_sweep": config5_leg(H, audit_frac=args.audit_frac)}))
         return
     if args.profile_leg:
         from higherharmonics_amd import workloads as W
@@ -734,10 +755,9 @@ def main():
     if not args.headline_only:
         out["uneven_steps_32_per_cycle"] = uneven_steps_leg(H, audit_frac=args.audit_frac)
         out["config2_dirac_1024x512"] = config2_leg(H)
+        out["config5_mixed_sweep"] = config5_leg(H, audit_frac=args.audit_frac)
     if args.sweeps:
         out["sweeps"] = sweep_extras(H)
-    if args.config5:
-        out["config5_mixed_learned"] = config5_leg(H)
     if args.reference_models:
         out["reference_models"] = reference_models_leg(H)
     if not args.no_cpu:

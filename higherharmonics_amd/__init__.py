@@ -96,7 +96,8 @@ class SolverDesc(ctypes.Structure):
 class LaunchDesc(ctypes.Structure):
     _fields_ = [("waves_per_block", ctypes.c_int32), ("queue_chunk", ctypes.c_int32),
                 ("persistent", ctypes.c_int32), ("group_by_cost", ctypes.c_int32),
-                ("count_wave_attempts", ctypes.c_int32), ("block_reduce", ctypes.c_int32)]
+                ("count_wave_attempts", ctypes.c_int32), ("block_reduce", ctypes.c_int32),
+                ("separate_launches", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class PlanDesc(ctypes.Structure):
@@ -128,7 +129,7 @@ EXPORTED_SYMBOLS = [
     "hhg_mlp_forward", "hhg_mlp_forward_mfma", "hhg_mlp_gradients", "hhg_mlp_train", "hhg_welch_spectrum",
     "hhg_spline_coefficients", "hhg_reference_double_laser", "hhg_reference_experimental_laser_size",
     "hhg_reference_experimental_laser", "hhg_occupations", "hhg_plan_job_failures", "hhg_plan_last_stats",
-    "hhg_trajectories_noalloc",
+    "hhg_trajectories_noalloc", "hhg_plan_solve_launches",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -155,6 +156,8 @@ def lib() -> ctypes.CDLL:
     L.hhg_plan_execute_device.argtypes = [P, _DP, P]
     L.hhg_plan_last_stats.argtypes = [P, ctypes.POINTER(Stats)]
     L.hhg_plan_autotune.argtypes = [P, ctypes.POINTER(LaunchDesc)]
+    L.hhg_plan_solve_launches.argtypes = [P]
+    L.hhg_plan_solve_launches.restype = ctypes.c_int32
     L.hhg_plan_capture_graph.argtypes = [P]
     L.hhg_plan_destroy.argtypes = [P]
     L.hhg_trajectories.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP]
@@ -581,7 +584,7 @@ class Plan:
     def __init__(self, jobs: Sequence[Job], kgrids: Sequence[KGrid], solver: Solver = Solver(),
                  rank: int = 0, n_ranks: int = 1, device: int = 0, waves_per_block: int = 0,
                  queue_chunk: int = 0, persistent: int = 1, group_by_cost: int = 0, count_wave_attempts: int = 0,
-                 block_reduce: int = 0):
+                 block_reduce: int = 0, separate_launches: int = 0):
         L = lib()
         self._solver = solver  # keeps the network alive as long as the plan
         self._kgrids = list(kgrids)  # keep explicit arrays alive
@@ -590,7 +593,7 @@ class Plan:
                                      for j in jobs])
         d = PlanDesc(len(kgrids), len(jobs), kg, jd, solver.desc(),
                      LaunchDesc(waves_per_block, queue_chunk, persistent, group_by_cost, count_wave_attempts,
-                                block_reduce),
+                                block_reduce, separate_launches, 0),
                      rank, n_ranks, device, 0)
         h = ctypes.c_void_p()
         _check(L.hhg_plan_create(ctypes.byref(d), ctypes.byref(h)))
@@ -647,6 +650,11 @@ class Plan:
         ld = LaunchDesc()
         _check(lib().hhg_plan_autotune(self._h, ctypes.byref(ld)))
         return {"waves_per_block": ld.waves_per_block, "queue_chunk": ld.queue_chunk}
+
+    @property
+    def solve_launches(self) -> int:
+        """Solve-kernel launches per execution (1 for a fused pair of groups)."""
+        return int(lib().hhg_plan_solve_launches(self._h))
 
     def capture_graph(self) -> None:
         _check(lib().hhg_plan_capture_graph(self._h))

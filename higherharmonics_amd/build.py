@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libhhg_amd.so")
 RUN = os.path.join(HERE, "hhg_run")
 
 SOURCES = ["hhg_solve_dirac.hip", "hhg_solve_piflux.hip", "hhg_solve_honeycomb.hip", "hhg_solve_dirac3d.hip",
-           "hhg_solve_piflux3d.hip", "hhg_kernels.hip", "hhg_mlp.hip", "hhg_plan.hip", "hhg_host.cpp"]
+           "hhg_solve_piflux3d.hip", "hhg_solve_fused.hip", "hhg_kernels.hip", "hhg_mlp.hip", "hhg_plan.hip", "hhg_host.cpp"]
 HEADERS = ["hhg_device.hpp", "hhg_internal.hpp", "hhg_kernels.hpp", "hhg_host.hpp", "hhg_solve.hpp", "hhg_mlp.hpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

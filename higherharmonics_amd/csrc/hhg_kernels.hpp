@@ -69,6 +69,12 @@ hipError_t cost_model(int pulse, const KernelArgs& a, int n_probe, float* cost, 
 hipError_t launch_solve(int method, int model, int pulse, int relax, int learned, const KernelArgs& a, int blocks,
                         int threads, size_t smem, hipStream_t st);
 const void* solve_kernel_fn(int method, int model, int pulse, int relax, int learned);
+// One RKF78 launch over the items of two groups, keys {model, pulse, relax} in
+// either order: the fused kernel's address, or nullptr where none is built;
+// launch_solve_fused returns hipErrorInvalidValue for such a pair.
+const void* solve_fused_fn(const int key1[3], const int key2[3], int learned);
+hipError_t launch_solve_fused(const int key1[3], const int key2[3], int learned, const KernelArgs& a, int blocks,
+                              int threads, size_t smem, hipStream_t st);
 hipError_t launch_trajectories(int method, int model, int pulse, int relax, const TrajArgs& x);
 // 3D pi-flux lattice with equilibrium relaxation, RKF78: x.s_out = occupations [n_sel][N+1][2]
 hipError_t launch_occupations(int pulse, const TrajArgs& x);

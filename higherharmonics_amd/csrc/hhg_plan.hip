@@ -36,6 +36,8 @@ struct Group {
     int waves_per_block, queue_chunk, blocks;
     int acc_samples;  // the largest output of the group's jobs (samples)
     int block_acc;    // block accumulation in LDS (size_group decides)
+    int fused;        // the fused launch over two groups' items: the second group's key follows
+    int model2, pulse2, relax2;
 };
 
 std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
@@ -95,7 +97,8 @@ struct hhg_plan {
     std::vector<int64_t> offsets;  // n_jobs + 1
     int64_t n_out = 0;
     int64_t n_kpts = 0;
-    std::vector<Group> groups;
+    std::vector<Group> groups;     // per (model, pulse, relax): first-step and cost-probe launches
+    std::vector<Group> launches;   // the solve launches: the groups, or one fused launch over two groups
     int num_cus = 0;
     size_t lds_per_block = 0;  // the device's dynamic LDS limit per block
 
@@ -266,15 +269,24 @@ void destroy_device(hhg_plan* p) {
     if (p->stream) hipStreamDestroy(p->stream);
 }
 
+// the solve kernel a group launches (a fused group: the two groups' fused kernel)
+const void* group_fn(const hhg_plan* p, const Group& g) {
+    if (g.fused) {
+        const int k1[3] = {g.model, g.pulse, g.relax}, k2[3] = {g.model2, g.pulse2, g.relax2};
+        return hhg::solve_fused_fn(k1, k2, p->solve_learn());
+    }
+    return hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn());
+}
+
 int max_block_threads(const hhg_plan* p, const Group& g) {
-    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn());
+    const void* fn = group_fn(p, g);
     hipFuncAttributes attr{};
     if (!fn || hipFuncGetAttributes(&attr, fn) != hipSuccess) return 256;
     return attr.maxThreadsPerBlock;
 }
 
 int blocks_per_cu(const hhg_plan* p, const Group& g, int threads, size_t smem) {
-    const void* fn = hhg::solve_kernel_fn(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn());
+    const void* fn = group_fn(p, g);
     int per_cu = 0;
     if (smem > p->lds_per_block) return 0;
     if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, smem) != hipSuccess) return 0;
@@ -311,55 +323,74 @@ void size_group(const hhg_plan* p, Group& g, int waves_per_block, int queue_chun
     }
 }
 
-hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit) {
+hhg::KernelArgs launch_args(const hhg_plan* p, const Group& g, int gi, int n_items_limit) {
+    hhg::KernelArgs a{};
+    a.jobs = p->d_jobs;
+    a.items = p->d_items + g.item_begin;
+    a.n_items = n_items_limit > 0 ? std::min(g.n_items, n_items_limit) : g.n_items;
+    a.queue_chunk = g.queue_chunk;
+    a.kx = p->d_kx;
+    a.ky = p->d_ky;
+    a.kz = p->d_kz;
+    a.w = p->d_w;
+    a.At = p->d_At;
+    a.Pt = p->d_Pt;
+    a.Am = p->d_Am;
+    a.nom = p->d_nom;
+    a.acc = p->acc();
+    a.ctr = p->counters(gi);
+    a.job_fail = p->job_fail();
+    a.perm = p->d_perm;
+    a.mlp = p->d_mlp;
+    a.infer_every = p->solver.mlp_infer_every > 0 ? p->solver.mlp_infer_every : 1;
+    a.fallback_after = p->solver.fallback_after > 0 ? p->solver.fallback_after : 2;
+    a.infer_policy = p->solver.infer_policy;
+    a.count_waves = p->launch.count_wave_attempts;
+    a.block_acc = g.block_acc;
+    a.acc_lds_off = (int)p->acc_lds_off();
+    a.acc_lds_samples = g.acc_samples;
+    a.log = p->d_log;
+    a.log_count = &p->counters(0)->log_count;
+    a.log_cap = p->log_cap;
+    a.log_stride = p->log_stride;
+    a.sp = solver_dev(p->solver);
+    return a;
+}
+
+// The execution's kernels on `st`.  events: the timing events around the
+// solve launches and the reduction (a graph capture leaves them out: run()
+// times a graph replay from outside the graph).
+hhg_status enqueue(hhg_plan* p, hipStream_t st, int n_items_limit, bool events = true) {
     HHG_TRY(hhg::launch_clear(p->d_words, p->n_words, st), "clear");
-    HHG_TRY(hipEventRecord(p->ev[0], st), "event");
-    for (size_t gi = 0; gi < p->groups.size(); ++gi) {
-        const Group& g = p->groups[gi];
-        hhg::KernelArgs a{};
-        a.jobs = p->d_jobs;
-        a.items = p->d_items + g.item_begin;
-        a.n_items = n_items_limit > 0 ? std::min(g.n_items, n_items_limit) : g.n_items;
-        a.queue_chunk = g.queue_chunk;
-        a.kx = p->d_kx;
-        a.ky = p->d_ky;
-        a.kz = p->d_kz;
-        a.w = p->d_w;
-        a.perm = nullptr;
-        a.At = p->d_At;
-        a.Pt = p->d_Pt;
-        a.Am = p->d_Am;
-        a.nom = p->d_nom;
-        a.acc = p->acc();
-        a.ctr = p->counters((int)gi);
-        a.job_fail = p->job_fail();
-        a.perm = p->d_perm;
-        a.mlp = p->d_mlp;
-        a.infer_every = p->solver.mlp_infer_every > 0 ? p->solver.mlp_infer_every : 1;
-        a.fallback_after = p->solver.fallback_after > 0 ? p->solver.fallback_after : 2;
-        a.infer_policy = p->solver.infer_policy;
-        a.count_waves = p->launch.count_wave_attempts;
-        a.block_acc = g.block_acc;
-        a.acc_lds_off = (int)p->acc_lds_off();
-        a.acc_lds_samples = g.acc_samples;
-        a.log = p->d_log;
-        a.log_count = &p->counters(0)->log_count;
-        a.log_cap = p->log_cap;
-        a.log_stride = p->log_stride;
-        a.sp = solver_dev(p->solver);
+    if (events) HHG_TRY(hipEventRecord(p->ev[0], st), "event");
+    if (p->first_step())
+        for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+            const hhg::KernelArgs a = launch_args(p, p->groups[gi], (int)gi, 0);
+            HHG_TRY(hhg::launch_first_step(p->groups[gi].model, p->groups[gi].pulse, a, p->d_h_first,
+                                           hhg::kMlpLdsBytes, st),
+                    "first-step launch");
+        }
+    for (size_t li = 0; li < p->launches.size(); ++li) {
+        const Group& g = p->launches[li];
+        hhg::KernelArgs a = launch_args(p, g, (int)li, n_items_limit);
+        if (p->first_step()) a.h_first = p->d_h_first;
         int blocks = g.blocks;
         if (n_items_limit > 0) blocks = std::min(blocks, (a.n_items + g.waves_per_block - 1) / g.waves_per_block);
-        if (p->first_step()) {
-            HHG_TRY(hhg::launch_first_step(g.model, g.pulse, a, p->d_h_first, hhg::kMlpLdsBytes, st), "first-step launch");
-            a.h_first = p->d_h_first;
+        blocks = std::max(1, blocks);
+        if (g.fused) {
+            const int k1[3] = {g.model, g.pulse, g.relax}, k2[3] = {g.model2, g.pulse2, g.relax2};
+            HHG_TRY(hhg::launch_solve_fused(k1, k2, p->solve_learn(), a, blocks, 64 * g.waves_per_block, p->smem(g),
+                                            st),
+                    "fused solve launch");
+        } else {
+            HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn(), a, blocks,
+                                      64 * g.waves_per_block, p->smem(g), st),
+                    "solve launch");
         }
-        HHG_TRY(hhg::launch_solve(p->solver.method, g.model, g.pulse, g.relax, p->solve_learn(), a, std::max(1, blocks),
-                                  64 * g.waves_per_block, p->smem(g), st),
-                "solve launch");
     }
-    HHG_TRY(hipEventRecord(p->ev[1], st), "event");
+    if (events) HHG_TRY(hipEventRecord(p->ev[1], st), "event");
     HHG_TRY(hhg::launch_finalize(p->acc(), p->n_out, p->d_j, st), "finalize");
-    HHG_TRY(hipEventRecord(p->ev[2], st), "event");
+    if (events) HHG_TRY(hipEventRecord(p->ev[2], st), "event");
     return HHG_SUCCESS;
 }
 
@@ -395,8 +426,13 @@ hhg_status collect_stats(hhg_plan* p, hhg_stats* stats, double wall_ms) {
 
 hhg_status run(hhg_plan* p, hipStream_t st, int n_items_limit = 0) {
     if (p->graph_exec && n_items_limit == 0 && st == p->stream) {
+        // solve_ms of a replay covers the whole graph (the clear and finalize
+        // kernels add microseconds); reduce_ms is 0
         TraceRange range("hhg graph launch");
+        HHG_TRY(hipEventRecord(p->ev[0], st), "event");
         HHG_TRY(hipGraphLaunch(p->graph_exec, st), "graph launch");
+        HHG_TRY(hipEventRecord(p->ev[1], st), "event");
+        HHG_TRY(hipEventRecord(p->ev[2], st), "event");
     } else {
         TraceRange range("hhg enqueue");
         const hhg_status r = enqueue(p, st, n_items_limit);
@@ -676,10 +712,23 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block must be 0, 1, 2, 4 or 8"));
     if (p->launch.block_reduce < 0 || p->launch.block_reduce > 2)
         return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "block_reduce must be 0, 1 or 2"));
-    for (auto& g : p->groups)
+    // one solve launch per group, or one fused launch where the plan has two
+    // RKF78 groups whose fused kernel is built (launch.separate_launches 0)
+    p->launches = p->groups;
+    if (sv.method == HHG_METHOD_RKF78 && p->groups.size() == 2 && !p->launch.separate_launches) {
+        Group f = p->groups[0];
+        const Group& g2 = p->groups[1];
+        f.fused = 1;
+        f.model2 = g2.model, f.pulse2 = g2.pulse, f.relax2 = g2.relax;
+        f.item_begin = 0;
+        f.n_items = g2.item_begin + g2.n_items;
+        f.acc_samples = std::max(f.acc_samples, g2.acc_samples);
+        if (group_fn(p, f)) p->launches.assign(1, f);
+    }
+    for (auto& g : p->launches)
         if (64 * std::max(wpb, 8) > max_block_threads(p, g))
             return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block exceeds the kernel's block size limit"));
-    for (auto& g : p->groups) {
+    for (auto& g : p->launches) {
         size_group(p, g, wpb, qc, p->launch.persistent);
         if (p->launch.block_reduce == 2 && p->launch.persistent && !g.block_acc)
             return bail(fail(HHG_ERROR_UNSUPPORTED, "block_reduce 2: a job's output does not fit the block's LDS"));
@@ -689,6 +738,8 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
 }
 
 int64_t hhg_plan_output_size(const hhg_plan* p) { return p ? p->n_out : -1; }
+
+int32_t hhg_plan_solve_launches(const hhg_plan* p) { return p ? (int32_t)p->launches.size() : -1; }
 
 hhg_status hhg_plan_job_offsets(const hhg_plan* p, int64_t* offsets) {
     hhg::clear_error();
@@ -762,11 +813,11 @@ hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
     const int chunks[] = {1, 2, 4};
     double best = 1e300;
     int best_w = 4, best_c = 1;
-    const std::vector<Group> saved = p->groups;
+    const std::vector<Group> saved = p->launches;
     for (int wpb : wpbs)
         for (int qc : chunks) {
             int limit = 0;
-            for (auto& g : p->groups) {
+            for (auto& g : p->launches) {
                 size_group(p, g, wpb, qc, 1);
                 limit = std::max(limit, 3 * g.blocks * wpb);
             }
@@ -777,15 +828,15 @@ hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
             HHG_TRY(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]), "autotune timing");
             // normalise by the items actually run
             int64_t ran = 0;
-            for (auto& g : p->groups) ran += std::min(g.n_items, limit);
+            for (auto& g : p->launches) ran += std::min(g.n_items, limit);
             const double per_item = ms / std::max<int64_t>(1, ran);
             if (per_item < best) best = per_item, best_w = wpb, best_c = qc;
         }
-    p->groups = saved;
+    p->launches = saved;
     p->launch.waves_per_block = best_w;
     p->launch.queue_chunk = best_c;
     p->launch.persistent = 1;
-    for (auto& g : p->groups) size_group(p, g, best_w, best_c, 1);
+    for (auto& g : p->launches) size_group(p, g, best_w, best_c, 1);
     // the tuning runs were partial solves: leave no counts of them behind
     HHG_TRY(hhg::launch_clear(p->d_words, p->n_words, p->stream), "clear");
     HHG_TRY(hipStreamSynchronize(p->stream), "autotune sync");
@@ -801,7 +852,7 @@ hhg_status hhg_plan_capture_graph(hhg_plan* p) {
     HHG_TRY(hipSetDevice(p->device), "hipSetDevice");
     if (p->graph_exec) return HHG_SUCCESS;
     HHG_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-    const hhg_status st = enqueue(p, p->stream, 0);
+    const hhg_status st = enqueue(p, p->stream, 0, false);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(p->stream, &g);
     if (st != HHG_SUCCESS) {

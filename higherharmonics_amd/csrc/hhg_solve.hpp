@@ -376,28 +376,23 @@ __device__ void solve_item_magnus_moments(JobRef J, int chunk, const KernelArgs&
 
 }  // namespace solve_detail
 
-// One kernel per (method, model, pulse, relaxation, learned); a launch only
-// carries items of its own group.  Every wave pops items from the group's
-// queue counter until the list is exhausted: a grid of one wave per item
-// makes each wave solve one item, a grid of the resident wave count is the
-// persistent schedule.
-template <int METHOD, int M, int P, int R, int LEARN>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SolveOccupancy<METHOD, M, LEARN>::waves,
-                                                                      SolveOccupancy<METHOD, M, LEARN>::waves))) void
-solve_kernel(KernelArgs a) {
-    using namespace solve_detail;
+namespace solve_detail {
+
+// The persistent work loop shared by the solve kernels.  Work is popped from
+// the launch's queue counter.  Wave accumulation: each wave pops queue_chunk
+// items and adds its sample sums to HBM.  Block accumulation: the block pops
+// waves x queue_chunk items; its waves add their sample sums to the block's
+// LDS copy of the current job's accumulators, which goes to HBM (one 128-bit
+// addition per sample) when the block moves to another job and at the end;
+// items of another job than the round's first (at a job boundary) add to HBM
+// directly.  item(J, chunk, lc, to_lds) solves one work item.
+template <int LEARN, class Item>
+__device__ __forceinline__ void solve_loop(const KernelArgs& a, Item&& item_fn) {
     const int lane = threadIdx.x & (WAVE - 1);
     if constexpr (LEARN != LEARN_NONE) {
         mlp_stage_lds(a.mlp, solve_dyn_lds);
         __syncthreads();
     }
-    // Work is popped from the group's queue counter.  Wave accumulation: each
-    // wave pops queue_chunk items and adds its sample sums to HBM.  Block
-    // accumulation: the block pops waves x queue_chunk items; its waves add
-    // their sample sums to the block's LDS copy of the current job's
-    // accumulators, which goes to HBM (one 128-bit addition per sample) when
-    // the block moves to another job and at the end; items of another job
-    // than the round's first (at a job boundary) add to HBM directly.
     __shared__ int s_base;
     unsigned long long* lacc = reinterpret_cast<unsigned long long*>(solve_dyn_lds + a.acc_lds_off);
     const bool block = a.block_acc != 0;
@@ -442,18 +437,55 @@ solve_kernel(KernelArgs a) {
             if (it >= a.n_items) break;
             const int2 item = load_uniform(a.items + it);
             LaneCounters lc;
-            const bool to_lds = block && item.x == cur_job;
-            if constexpr (METHOD == METHOD_RKF78)
-                solve_item_rkf78<M, P, R, LEARN>(job_ref(a.jobs + item.x), item.y, a, lc, to_lds);
-            else if constexpr (METHOD == METHOD_MAGNUS4)
-                solve_item_magnus<M>(job_ref(a.jobs + item.x), item.y, a, lc, to_lds);
-            else
-                solve_item_magnus_moments<M>(job_ref(a.jobs + item.x), item.y, a, lc, to_lds);
+            item_fn(job_ref(a.jobs + item.x), item.y, lc, block && item.x == cur_job);
             // the item's counts, its failures attributed to its job
             flush_counters(a.ctr, a.job_fail, item.x, lc);
         }
     }
     if (block && cur_job >= 0) flush_lds(cur_job);
+}
+
+}  // namespace solve_detail
+
+// One kernel per (method, model, pulse, relaxation, learned); a launch only
+// carries items of its own group.  Every wave pops items from the group's
+// queue counter until the list is exhausted: a grid of one wave per item
+// makes each wave solve one item, a grid of the resident wave count is the
+// persistent schedule.
+template <int METHOD, int M, int P, int R, int LEARN>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SolveOccupancy<METHOD, M, LEARN>::waves,
+                                                                      SolveOccupancy<METHOD, M, LEARN>::waves))) void
+solve_kernel(KernelArgs a) {
+    using namespace solve_detail;
+    solve_loop<LEARN>(a, [&](JobRef J, int chunk, LaneCounters& lc, bool to_lds) {
+        if constexpr (METHOD == METHOD_RKF78)
+            solve_item_rkf78<M, P, R, LEARN>(J, chunk, a, lc, to_lds);
+        else if constexpr (METHOD == METHOD_MAGNUS4)
+            solve_item_magnus<M>(J, chunk, a, lc, to_lds);
+        else
+            solve_item_magnus_moments<M>(J, chunk, a, lc, to_lds);
+    });
+}
+
+// A (model, pulse, relaxation) group of RKF78 jobs.
+template <int M_, int P_, int R_>
+struct GroupKey {
+    static constexpr int M = M_, P = P_, R = R_;
+};
+
+// One RKF78 launch over the items of two groups (mixed sweeps): the waves
+// share one work queue, so the launch has one tail instead of one per group.
+// The item's job record (constant address space, a scalar load) selects the
+// group's solver, a branch that is uniform over the wave.
+template <class G1, class G2, int LEARN>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void solve_kernel_fused(KernelArgs a) {
+    using namespace solve_detail;
+    solve_loop<LEARN>(a, [&](JobRef J, int chunk, LaneCounters& lc, bool to_lds) {
+        if (J.model == G1::M && J.pulse == G1::P && J.relax == G1::R)
+            solve_item_rkf78<G1::M, G1::P, G1::R, LEARN>(J, chunk, a, lc, to_lds);
+        else
+            solve_item_rkf78<G2::M, G2::P, G2::R, LEARN>(J, chunk, a, lc, to_lds);
+    });
 }
 
 // Per-k trajectories of one job (tests / inspection): one lane per selected

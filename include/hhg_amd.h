@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define HHG_AMD_ABI_VERSION 2
+#define HHG_AMD_ABI_VERSION 3
 
 typedef enum hhg_status {
     HHG_SUCCESS = 0,
@@ -253,6 +253,11 @@ typedef struct hhg_launch_desc {
                                 job's accumulators, which goes to HBM once per job and block (far less HBM
                                 traffic, about 10 % more solve time on MI355X; HHG_ERROR_UNSUPPORTED where a
                                 job's output does not fit the block's LDS).  j is bitwise the same either way */
+    int32_t separate_launches; /* 0 = a plan whose RKF78 jobs form two (model, pulse, relaxation) groups with a
+                                  fused kernel (honeycomb double cos^2 with PiFlux-2D double Gaussian, no
+                                  relaxation) solves both in one launch over one work queue; 1 = one launch per
+                                  group.  j is bitwise the same either way */
+    int32_t pad;
 } hhg_launch_desc;
 
 typedef struct hhg_plan_desc {
@@ -280,8 +285,9 @@ typedef struct hhg_stats {
                                     per-trajectory attempt count (0 when not counted) */
     int64_t accumulator_overflows; /* wavefront lanes whose sample sums left the exact accumulators'
                                       range (|sum| >= 2^46): j is wrong where this is nonzero */
-    double solve_ms;             /* device time of the solve kernel(s) */
-    double reduce_ms;            /* device time of the j reduction */
+    double solve_ms;             /* device time of the solve kernel(s); after a graph replay
+                                    (hhg_plan_capture_graph) the time of the whole graph */
+    double reduce_ms;            /* device time of the j reduction (0 after a graph replay) */
     double total_ms;             /* host wall time of hhg_plan_execute */
 } hhg_stats;
 
@@ -325,6 +331,11 @@ hhg_status hhg_plan_last_stats(hhg_plan* plan, hhg_stats* stats);
 /* Tries launch shapes on the plan's own work and keeps the fastest
  * (waves_per_block x queue_chunk); writes the choice into chosen (may be NULL). */
 hhg_status hhg_plan_autotune(hhg_plan* plan, hhg_launch_desc* chosen);
+
+/* Number of solve-kernel launches one execution of the plan makes (one per
+ * (model, pulse, relaxation) group, or 1 for a fused pair); -1 for NULL.
+ * No counterpart in the reference, whose dispatchers run one system per call. */
+int32_t hhg_plan_solve_launches(const hhg_plan* plan);
 
 /* Records the plan's kernels into a HIP graph; later executes replay it. */
 hhg_status hhg_plan_capture_graph(hhg_plan* plan);

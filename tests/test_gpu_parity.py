@@ -206,13 +206,14 @@ def _oracle_job(job):
 def test_mixed_sweep_one_plan_matches_oracle(hhg):
     """Configuration-5 structure at small size: honeycomb (double cos^2) and
     PiFlux (double Gaussian) jobs with seeded amplitudes, CEPs and delays,
-    different N per job, in one plan (one persistent launch per model/pulse
-    group), autotuned and graph-captured."""
+    different N per job, in one plan (one fused persistent launch over both
+    model/pulse groups), autotuned and graph-captured."""
     from higherharmonics_amd import workloads as W
     jobs = W.mixed_sweep(n=12, n_each=3, per_cycle=64, cos2_cycles=2, sigma_cycles=0.5)
     print(f"[case] mixed sweep {len(jobs)} jobs, N = {[j['time'][2] for j in jobs]}", flush=True)
     refs = [_oracle_job(j) for j in jobs]
     with W.build_plan(hhg, jobs) as plan:
+        assert plan.solve_launches == 1
         out = plan.execute()
         for j, r in zip(out, refs):
             assert rel_l2(j, r) <= J_TOL
@@ -225,6 +226,37 @@ def test_mixed_sweep_one_plan_matches_oracle(hhg):
                 assert np.array_equal(a, b)
     for a, b in zip(tuned, out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("proposal", [0, 1], ids=["restart", "carry"])
+def test_mixed_sweep_fused_launch_equals_separate_launches(hhg, proposal):
+    """The mixed sweep's two groups in one fused launch (one work queue) and
+    in one launch per group: the exact accumulators make j bitwise
+    independent of which wave solved which item, so the two must agree bit for
+    bit, with equal attempt counts, also with per-block accumulation and
+    under graph replays of an autotuned plan."""
+    from higherharmonics_amd import workloads as W
+    jobs = W.mixed_sweep(n=16, n_each=4, per_cycle=64, cos2_cycles=2, sigma_cycles=0.5)
+    print(f"[case] fused vs separate launches, {len(jobs)} jobs, proposal {proposal}", flush=True)
+    sv = hhg.Solver(proposal=proposal)
+    with W.build_plan(hhg, jobs, sv, separate_launches=1) as plan:
+        assert plan.solve_launches == 2
+        sep = plan.execute()
+        st_sep = plan.stats.as_dict()
+    for kw in ({}, {"block_reduce": 2}, {"waves_per_block": 1, "queue_chunk": 2}):
+        with W.build_plan(hhg, jobs, sv, **kw) as plan:
+            assert plan.solve_launches == 1
+            fused = plan.execute()
+            st = plan.stats.as_dict()
+        assert st["attempts"] == st_sep["attempts"] and st["rejects"] == st_sep["rejects"]
+        for a, b in zip(fused, sep):
+            assert np.array_equal(a, b)
+    with W.build_plan(hhg, jobs, sv) as plan:
+        plan.autotune()
+        plan.capture_graph()
+        for _ in range(3):
+            for a, b in zip(plan.execute(), sep):
+                assert np.array_equal(a, b)
 
 
 def test_piflux_delay_sweep_matches_oracle(hhg):
