@@ -346,11 +346,40 @@ struct Bloch {
     // A(t + dlt) (t = the prepared step start)
     __device__ __forceinline__ double pulse_at(double dlt) const { return pc.at(J, dlt); }
 
+    // Fermi pair of the equilibrium relaxations: P = f(c + x), M = f(c - x) for
+    // x >= 0, f(e) = 1 / (1 + exp(beta e)) (GlobalDefinitions.hpp:36-41).  With
+    // q = exp(-beta x) in [0, 1] and w = exp(beta c) (JobDev::fermi_w),
+    // P = q / (q + w) and M = 1 / (1 + w q), so
+    //   pm = P - M = w (q^2 - 1) / ((q + w)(1 + w q)),  pp = P M = q / ((q + w)(1 + w q)):
+    // one expm1 and one division instead of two exponentials and two divisions.
+    // q^2 - 1 is formed as (q - 1)(q + 1) from q - 1 = expm1(-beta x), which keeps
+    // its relative accuracy where beta x is small (P - M itself cancels there).
+    // w in [1e-150, 1e150] keeps (q + w)(1 + w q) finite and nonzero; w = 0 (T = 0,
+    // or |beta c| > 345) evaluates the two Fermi functions as they stand.
+    __device__ __forceinline__ void fermi_pair(double c, double x, double& pm, double& pp) const {
+        const double w = J.fermi_w;
+        if (w > 0.0) {
+            const double q = exp(-J.beta * x);
+            const double r = 1.0 / ((q + w) * fma(w, q, 1.0));
+            pm = w * ((q - 1.0) * (q + 1.0)) * r;
+            pp = q * r;
+        } else {
+            const double fp = fermi(c + x, J.beta), fm = fermi(c - x, J.beta);
+            pm = fp - fm;
+            pp = fp * fm;
+        }
+    }
+
     // Relaxation of the 3D pi-flux lattice toward the instantaneous
     // equilibrium at m = (cos kx, cos ky, cz = cos(kz - A)): update_equilibrium_state
     // (PiFlux.cpp:134-166) with the off-diagonal phase of the step-start state
     // in the band frame (diagonal_sigma, PiFlux.cpp:1037-1049), subtracted as
-    // rd (s/3 - D) + ro (2s/3 - O) (PiFlux.cpp:174-179).
+    // rd (s/3 - D) + ro (2s/3 - O) (PiFlux.cpp:174-179).  alpha^2 - beta^2 and
+    // alpha beta come from fermi_pair with c = -E_F, x = 2t|m|.  The band-frame
+    // components g = G / nz (nz = 2|m| z >= 0) enter only through their phase
+    // G / |G|, so G is not divided by nz; the reference's is_zero(|g|) test is
+    // |G| < 2^-44 nz.  Three divisions, three square roots and one expm1 per
+    // call (the reference's form takes eight, three and two exponentials).
     __device__ __forceinline__ void relax_equilibrium(double cz, const double s[3], double out[3]) const {
         double mx = cx;
         if constexpr (STICKY) {
@@ -362,21 +391,20 @@ struct Bloch {
         }
         const double my = cy;
         const double disp = sqrt(mx * mx + my * my + cz * cz);
-        const double a2 = fermi(-J.fermi + 2.0 * J.hop * disp, J.beta);
-        const double b2 = fermi(-J.fermi - 2.0 * J.hop * disp, J.beta);
-        const double diff = a2 - b2;
+        double diff, pp;
+        fermi_pair(-J.fermi, 2.0 * J.hop * disp, diff, pp);
+        const double p2 = 2.0 * sqrt(pp);
         const double z = cz + disp, nz = 2.0 * disp * z;
         const double xz = 2.0 * mx * z, yz = 2.0 * my * z, xy = 2.0 * mx * my;
-        const double g0 = (eq[0] * (z * z - mx * mx + my * my) - xy * eq[1] - xz * eq[2]) / nz;
-        const double g1 = (eq[1] * (z * z + mx * mx - my * my) - xy * eq[0] - yz * eq[2]) / nz;
-        const double xyl = sqrt(g0 * g0 + g1 * g1);
-        const double p2 = 2.0 * sqrt(a2 * b2);
-        const bool flat = ref_is_zero(xyl);
-        const double prod = p2 * (flat ? 1.0 : g0 / xyl);
-        const double imag = p2 * (flat ? 0.0 : g1 / xyl);
-        const double nrm = disp * z;
-        const double fd = diff * z / nrm;
-        const double inv = 1.0 / nrm;
+        const double G0 = eq[0] * (z * z - mx * mx + my * my) - xy * eq[1] - xz * eq[2];
+        const double G1 = eq[1] * (z * z + mx * mx - my * my) - xy * eq[0] - yz * eq[2];
+        const double L = sqrt(G0 * G0 + G1 * G1);
+        const bool flat = L < 0x1p-44 * nz;
+        const double u = flat ? 0.0 : p2 / L;
+        const double prod = flat ? p2 : u * G0;
+        const double imag = u * G1;
+        const double inv = 1.0 / (disp * z);
+        const double fd = diff * z * inv;
         const double O0 = (prod * (my * my + cz * z) + imag * (-mx * my)) * inv;
         const double O1 = (prod * (-mx * my) + imag * (mx * mx + cz * z)) * inv;
         const double O2 = (prod * (-mx * z) + imag * (-my * z)) * inv;
@@ -384,6 +412,21 @@ struct Bloch {
         out[0] -= rd * (s[0] * (1.0 / 3.0) - mx * fd) + ro * (s[0] * (2.0 / 3.0) - O0);
         out[1] -= rd * (s[1] * (1.0 / 3.0) - my * fd) + ro * (s[1] * (2.0 / 3.0) - O1);
         out[2] -= rd * (s[2] * (1.0 / 3.0) - cz * fd) + ro * (s[2] * (2.0 / 3.0) - O2);
+    }
+
+    // The honeycomb's instantaneous equilibrium at d (the relaxation target of
+    // Honeycomb.cpp:169-177): honeycomb_state with alpha^2 - beta^2 and
+    // alpha beta from fermi_pair (c = E_F, x = |d|; alpha^2 = M, beta^2 = P), so
+    // (s0, s1) = (P - M) (d0, d1) / |d|.  Two divisions, two square roots and one
+    // expm1 (honeycomb_state: seven, two and two exponentials).
+    __device__ __forceinline__ void honeycomb_target(const double d[3], double se[3]) const {
+        const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double pm, pp;
+        fermi_pair(J.fermi, e, pm, pp);
+        const double f = e == 0.0 ? 0.0 : pm / e;
+        se[0] = f * d[0];
+        se[1] = f * d[1];
+        se[2] = -2.0 * sqrt(pp);
     }
 
     // STICKY: update_z of the persistent shifted momentum to cos(z - A) = cz and
@@ -521,7 +564,7 @@ struct Bloch {
             out[2] = wx * s[1] - wy * s[0];
             const double r = J.relax_rate;
             double se[3];
-            honeycomb_state(d, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), se);
+            honeycomb_target(d, se);
             out[0] -= r * (s[0] - se[0]);
             out[1] -= r * (s[1] - se[1]);
             out[2] -= r * (s[2] - se[2]);

@@ -84,6 +84,8 @@ struct JobDev {
     double fermi, beta;   // beta <= 0: T = 0
     double relax_rate;    // 1 / relax_time (equilibrium relaxation: the diagonal rate)
     double relax_rate2;   // equilibrium relaxation: the off-diagonal rate
+    double fermi_w;       // equilibrium relaxation: exp(beta c), c = -E_F (3D pi-flux) or E_F (honeycomb);
+                          // 0 where the one-exponential Fermi pair does not apply (T = 0, |beta E_F| > 345)
     // pulse
     int32_t pulse, theta_zero;
     double amp_a, amp_b;

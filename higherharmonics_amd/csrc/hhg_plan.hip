@@ -223,6 +223,13 @@ hhg::JobDev make_job(const hhg_job_desc& jd, const hhg_solver_desc& sv) {
     J.beta = m.beta;
     J.relax_rate = m.relax != HHG_RELAX_NONE ? 1.0 / m.relax_time : 0.0;
     J.relax_rate2 = m.relax == HHG_RELAX_EQUILIBRIUM && m.relax_time_offdiag > 0.0 ? 1.0 / m.relax_time_offdiag : 0.0;
+    // one-exponential Fermi pair of the equilibrium relaxations: w = exp(beta c)
+    // within [1e-150, 1e150], so that (q + w)(1 + w q) stays finite and nonzero
+    {
+        const double c = m.kind == HHG_MODEL_PIFLUX3D ? -m.fermi_energy : m.fermi_energy;
+        const double y = m.beta * c;
+        J.fermi_w = m.relax == HHG_RELAX_EQUILIBRIUM && m.beta > 0.0 && std::fabs(y) <= 345.0 ? std::exp(y) : 0.0;
+    }
     J.split_cut = m.split_dispersion;
     J.pulse = p.kind;
     J.theta_zero = m.theta == 0.0;
@@ -308,8 +315,9 @@ void size_group(const hhg_plan* p, Group& g, int waves_per_block, int queue_chun
         t.block_acc = 1;
         return blocks_per_cu(p, t, 64 * wpb, p->smem(t)) > 0;
     };
+    const int wpb_max = std::max(1, max_block_threads(p, g) / 64);
     int wpb = waves_per_block;
-    if (wpb <= 0) wpb = (want_block && fits(8)) ? 8 : 4;
+    if (wpb <= 0) wpb = (want_block && wpb_max >= 8 && fits(8)) ? 8 : std::min(4, wpb_max);
     g.waves_per_block = wpb;
     g.queue_chunk = queue_chunk;
     g.block_acc = want_block && fits(wpb) ? 1 : 0;
@@ -726,7 +734,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         if (group_fn(p, f)) p->launches.assign(1, f);
     }
     for (auto& g : p->launches)
-        if (64 * std::max(wpb, 8) > max_block_threads(p, g))
+        if (64 * wpb > max_block_threads(p, g))
             return bail(fail(HHG_ERROR_INVALID_ARGUMENT, "waves_per_block exceeds the kernel's block size limit"));
     for (auto& g : p->launches) {
         size_group(p, g, wpb, qc, p->launch.persistent);
@@ -814,8 +822,11 @@ hhg_status hhg_plan_autotune(hhg_plan* p, hhg_launch_desc* chosen) {
     double best = 1e300;
     int best_w = 4, best_c = 1;
     const std::vector<Group> saved = p->launches;
+    int wpb_max = 8;
+    for (const auto& g : p->launches) wpb_max = std::min(wpb_max, max_block_threads(p, g) / 64);
     for (int wpb : wpbs)
         for (int qc : chunks) {
+            if (wpb > wpb_max) continue;  // beyond a kernel's block size limit
             int limit = 0;
             for (auto& g : p->launches) {
                 size_group(p, g, wpb, qc, 1);

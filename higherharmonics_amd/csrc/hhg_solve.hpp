@@ -24,6 +24,11 @@ namespace hhg {
 // run at 2.  At 3 the learned 2D Dirac kernel ran 4-7 % faster but spilled 58
 // VGPRs to scratch in its interval loop (43 GB of scratch writes per solve);
 // at 4 the plain kernel gained 3-6 % and the carried controller lost 11 %.
+// The equilibrium-relaxation kernels spill at 2 (3D pi-flux 468, honeycomb
+// 280 bytes of scratch per lane); at 1 (blocks of at most 4 waves) they spill
+// into AGPRs instead and use no scratch, but ran slower on the reference
+// systems' configurations (3D pi-flux 874 vs 725 ms, honeycomb 751 vs 679 ms,
+// spline double pulse 406 vs 328 ms), so they stay at 2 (HHG_EQ_WAVES).
 // HHG_WAVES_PER_EU overrides all of them (occupancy experiments).
 // Learned-proposal kernels: LEARN_NONE (plain and carried controllers; the
 // learned first-interval policy is first_step_kernel followed by this kernel
@@ -31,13 +36,18 @@ namespace hhg {
 // first step of every infer_every-th interval inside the loop).
 enum : int { LEARN_NONE = 0, LEARN_SCHEDULED = 2 };
 
-template <int METHOD, int M, int LEARN>
+#ifndef HHG_EQ_WAVES
+#define HHG_EQ_WAVES 2
+#endif
+template <int METHOD, int M, int R, int LEARN>
 struct SolveOccupancy {
 #ifdef HHG_WAVES_PER_EU
     static constexpr int waves = HHG_WAVES_PER_EU;
 #else
-    static constexpr int waves = METHOD != METHOD_RKF78 ? 4 : 2;
+    static constexpr int waves = METHOD != METHOD_RKF78 ? 4 : R == RELAX_EQUILIBRIUM ? HHG_EQ_WAVES : 2;
 #endif
+    // one wave per SIMD needs blocks of at most 4 waves
+    static constexpr int max_threads = waves == 1 ? 256 : 512;
 };
 
 namespace solve_detail {
@@ -453,8 +463,9 @@ __device__ __forceinline__ void solve_loop(const KernelArgs& a, Item&& item_fn) 
 // makes each wave solve one item, a grid of the resident wave count is the
 // persistent schedule.
 template <int METHOD, int M, int P, int R, int LEARN>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SolveOccupancy<METHOD, M, LEARN>::waves,
-                                                                      SolveOccupancy<METHOD, M, LEARN>::waves))) void
+__global__ __attribute__((amdgpu_flat_work_group_size(1, SolveOccupancy<METHOD, M, R, LEARN>::max_threads),
+                          amdgpu_waves_per_eu(SolveOccupancy<METHOD, M, R, LEARN>::waves,
+                                              SolveOccupancy<METHOD, M, R, LEARN>::waves))) void
 solve_kernel(KernelArgs a) {
     using namespace solve_detail;
     solve_loop<LEARN>(a, [&](JobRef J, int chunk, LaneCounters& lc, bool to_lds) {

@@ -34,7 +34,7 @@ class Model(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
                 ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
                 ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double),
-                ("relax_time_offdiag", ctypes.c_double), ("ic_rescue", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("relax_time_offdiag", ctypes.c_double), ("ic_rescue", ctypes.c_int32), ("eq_form", ctypes.c_int32),
                 ("split_dispersion", ctypes.c_double)]
 
 
@@ -87,6 +87,7 @@ def lib(fast: bool = False) -> ctypes.CDLL:
     L.or_spline_coefficients.restype = ctypes.c_int
     L.or_spline_coefficients.argtypes = [dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
     L.or_rkf78_tableau.argtypes = [dp, dp, dp, dp]
+    L.or_fermi_pair.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
     d3 = [ctypes.c_double] * 3
     L.or_initial_state.argtypes = [ctypes.POINTER(Model), *d3, dp]
     L.or_rhs.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(Pulse), *d3, dp, dp, ctypes.c_double, dp]
@@ -117,9 +118,13 @@ def _dp(a: np.ndarray):
 
 
 def model(kind=MODEL_DIRAC2D, v=1.0, mass=0.1, hopping=1.0, theta=0.0, fermi_energy=0.0, beta=0.0,
-          relax=RELAX_NONE, relax_time=0.0, relax_time_offdiag=0.0, ic_rescue=0, split_dispersion=0.0) -> Model:
+          relax=RELAX_NONE, relax_time=0.0, relax_time_offdiag=0.0, ic_rescue=0, split_dispersion=0.0,
+          eq_form=0) -> Model:
+    """eq_form 1 evaluates the equilibrium-relaxation targets in the HIP
+    kernels' one-exponential form (a CPU check of that form); 0 is the
+    reference's."""
     return Model(kind, relax, v, mass, hopping, theta, fermi_energy, beta, relax_time, relax_time_offdiag,
-                 ic_rescue, 0, split_dispersion)
+                 ic_rescue, eq_form, split_dispersion)
 
 
 def pulse(kind=PULSE_COS2, amp_a=0.5, amp_b=0.0, n_cycles=8.0, sigma=0.0, t0=0.0, delay=0.0, cep=0.0) -> Pulse:
@@ -256,6 +261,14 @@ def current_density(m: Model, p: Pulse, tc: Time, sv: Solver, kx, ky, w, n_threa
     if r < 0:
         raise ValueError("unsupported model/solver combination")
     return j, st
+
+
+def fermi_pair(form: int, c: float, x: float, beta: float):
+    """(P - M, P M) of P = f(c + x), M = f(c - x): form 0 the reference's two
+    Fermi functions, 1 the HIP kernels' one-exponential form."""
+    pm, pp = ctypes.c_double(), ctypes.c_double()
+    lib().or_fermi_pair(form, c, x, beta, ctypes.byref(pm), ctypes.byref(pp))
+    return pm.value, pp.value
 
 
 def magnus_moments(p: Pulse, t0: float, h: float) -> np.ndarray:

@@ -364,12 +364,79 @@ static void piflux3d_ic(const or_model* m, double cx, double cy, double cz, doub
     s[2] = (diff * cz * ze - prod * cx * ze) / nrm;
 }
 
+/* The HIP kernels' one-exponential Fermi pair (eq_form 1), restated for its
+ * CPU check: P = f(c + x), M = f(c - x) with q = exp(-beta x), w = exp(beta c),
+ * P - M = w (q^2 - 1) / ((q + w)(1 + w q)), P M = q / ((q + w)(1 + w q)),
+ * q^2 - 1 = (q - 1)(q + 1) from q - 1 = expm1(-beta x); outside
+ * beta > 0, |beta c| <= 345 the two Fermi functions as they stand. */
+static void fermi_pair_product(double c, double x, double beta, double* pm, double* pp) {
+    const double y = beta * c;
+    if (beta > 0.0 && fabs(y) <= 345.0) {
+        const double w = exp(y);
+        const double q = exp(-beta * x);
+        const double r = 1.0 / ((q + w) * fma(w, q, 1.0));
+        *pm = w * ((q - 1.0) * (q + 1.0)) * r;
+        *pp = q * r;
+    } else {
+        const double fp = fermi(c + x, beta), fm = fermi(c - x, beta);
+        *pm = fp - fm;
+        *pp = fp * fm;
+    }
+}
+
+/* The Fermi pair (P - M, P M) of P = f(c + x), M = f(c - x): form 0 as the
+ * reference evaluates it (two Fermi functions), 1 in the HIP kernels' form. */
+void or_fermi_pair(int form, double c, double x, double beta, double* pm, double* pp) {
+    if (form == 1) {
+        fermi_pair_product(c, x, beta, pm, pp);
+        return;
+    }
+    const double fp = fermi(c + x, beta), fm = fermi(c - x, beta);
+    *pm = fp - fm;
+    *pp = fp * fm;
+}
+
+/* eq_form 1 of piflux3d_equilibrium: the HIP kernels' arrangement (band-frame
+ * phase from G = nz g without dividing by nz, is_zero(|g|) as |G| < 2^-44 nz,
+ * three divisions). */
+static void piflux3d_equilibrium_product(const or_model* m, double cx, double cy, double cz, const double s_ref[3],
+                                         double D[3], double O[3]) {
+    if (or_is_zero(cx) && or_is_zero(cy) && cz < 0.0) {
+        cx = fabs(cz) * OR_RESCUE;
+        cz *= (1.0 - 0.5 * OR_RESCUE * OR_RESCUE);
+    }
+    const double disp = sqrt(cx * cx + cy * cy + cz * cz);
+    double diff, pp;
+    fermi_pair_product(-m->fermi_energy, 2.0 * m->hopping * disp, m->beta, &diff, &pp);
+    const double p2 = 2.0 * sqrt(pp);
+    const double z = cz + disp, nz = 2.0 * disp * z;
+    const double xz = 2.0 * cx * z, yz = 2.0 * cy * z, xy = 2.0 * cx * cy;
+    const double G0 = s_ref[0] * (z * z - cx * cx + cy * cy) - xy * s_ref[1] - xz * s_ref[2];
+    const double G1 = s_ref[1] * (z * z + cx * cx - cy * cy) - xy * s_ref[0] - yz * s_ref[2];
+    const double L = sqrt(G0 * G0 + G1 * G1);
+    const int flat = L < 0x1p-44 * nz;
+    const double u = flat ? 0.0 : p2 / L;
+    const double prod = flat ? p2 : u * G0, imag = u * G1;
+    const double inv = 1.0 / (disp * z);
+    const double fd = diff * z * inv;
+    D[0] = cx * fd;
+    D[1] = cy * fd;
+    D[2] = cz * fd;
+    O[0] = (prod * (cy * cy + cz * z) + imag * (-cx * cy)) * inv;
+    O[1] = (prod * (-cx * cy) + imag * (cx * cx + cz * z)) * inv;
+    O[2] = (prod * (-cx * z) + imag * (-cy * z)) * inv;
+}
+
 /* Relaxation targets of the 3D pi-flux lattice at shifted m = (cx, cy, cz),
  * update_equilibrium_state of PiFlux.cpp:134-166: the diagonal part is
  * (alpha^2 - beta^2) m/|m|; the off-diagonal part keeps the band-frame phase
  * of s_ref (diagonal_sigma, PiFlux.cpp:1037-1049) with modulus 2 alpha beta. */
 static void piflux3d_equilibrium(const or_model* m, double cx, double cy, double cz, const double s_ref[3],
                                  double D[3], double O[3]) {
+    if (m->eq_form == 1) {
+        piflux3d_equilibrium_product(m, cx, cy, cz, s_ref, D, O);
+        return;
+    }
     if (or_is_zero(cx) && or_is_zero(cy) && cz < 0.0) {
         cx = fabs(cz) * OR_RESCUE;
         cz *= (1.0 - 0.5 * OR_RESCUE * OR_RESCUE);
@@ -489,7 +556,18 @@ void or_rhs(const or_model* m, const or_pulse* p, double kx, double ky, double k
         /* honeycomb: -(s - s_eq(k - A))/tau (Honeycomb.cpp:180-191) */
         const double r = 1.0 / m->relax_time;
         double se[3];
-        honeycomb_state(m, d, se);
+        if (m->eq_form == 1) {
+            /* the HIP kernels' form: (P - M) (d0, d1) / |d|, -2 sqrt(P M), c = E_F, x = |d| */
+            const double e = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            double pm, pp;
+            fermi_pair_product(m->fermi_energy, e, m->beta, &pm, &pp);
+            const double f = e == 0.0 ? 0.0 : pm / e;
+            se[0] = f * d[0];
+            se[1] = f * d[1];
+            se[2] = -2.0 * sqrt(pp);
+        } else {
+            honeycomb_state(m, d, se);
+        }
         for (int c = 0; c < 3; ++c) out[c] -= r * (s[c] - se[c]);
     } else if (m->relax == OR_RELAX_T2) {
         const double r = 1.0 / m->relax_time;

@@ -43,7 +43,8 @@ typedef struct {
     double relax_time;        /* equilibrium relaxation: the diagonal time */
     double relax_time_offdiag; /* equilibrium relaxation: the off-diagonal time (<= 0: none) */
     int32_t ic_rescue;        /* 3D pi-flux rescue of the reference's drivers: 1 continuum grid, 2 uniform grid */
-    int32_t pad;
+    int32_t eq_form;          /* equilibrium relaxation targets: 0 the reference's formulas; 1 the HIP
+                                 kernels' one-exponential form, restated for its CPU check */
     double split_dispersion;  /* 3D pi-flux, > 0: j split at |m(k - A)| into two outputs of N + 1 samples */
 } or_model;
 
@@ -89,6 +90,11 @@ typedef struct {
     int64_t failed_trajectories;
     int64_t rhs_evals;
 } or_stats;
+
+/* The equilibrium relaxations' Fermi pair (P - M, P M) with P = f(c + x),
+ * M = f(c - x), f(e) = 1 / (1 + exp(beta e)): form 0 the reference's two Fermi
+ * functions, 1 the HIP kernels' one-exponential form. */
+void or_fermi_pair(int form, double c, double x, double beta, double* pm, double* pp);
 
 /* Pulse A(t) (vector potential including its amplitude). */
 double or_pulse_A(const or_pulse* p, double t);

@@ -1036,3 +1036,61 @@ def test_dirac_gauss_quadrature_converges():
     assert errs[-1] < 1e-6
     assert errs[0] / errs[1] > 14 and errs[1] / errs[2] > 14
     assert 3.5 < errs_cyl[0] / errs_cyl[1] < 4.5 and errs[-1] < errs_cyl[-1] / 100
+
+
+# --------------------------------------------------------------------------
+# The equilibrium relaxations' Fermi pair and the kernels' form of the targets
+# --------------------------------------------------------------------------
+
+def test_fermi_pair_error_bound_against_mpmath():
+    """P - M and 2 sqrt(P M) of P = f(c + x), M = f(c - x),
+    f(e) = 1 / (1 + exp(beta e)), against 60-digit mpmath: both the reference's
+    two Fermi functions (form 0) and the kernels' one-exponential form (form 1,
+    q = exp(-beta x), w = exp(beta c)) stay within 2 eps (1 + beta (|c| + x) / 4)
+    absolute.  The beta (|c| + x) term is the exponents' own rounding; the
+    reference pays it too once x = 2t|m| is formed from k.  A swapped P/M or a
+    wrong sign of c fails by O(1)."""
+    import mpmath
+    mpmath.mp.dps = 60
+    eps = 2.0 ** -52
+    rng = np.random.default_rng(20261204)
+    worst = {0: 0.0, 1: 0.0}
+    for _ in range(3000):
+        beta = 10 ** rng.uniform(-1, 2.5)
+        c = rng.uniform(-4, 4)
+        x = 10 ** rng.uniform(-9, 1.5)
+        B, C, X = mpmath.mpf(beta), mpmath.mpf(c), mpmath.mpf(x)
+        P = 1 / (1 + mpmath.exp(B * (C + X)))
+        M = 1 / (1 + mpmath.exp(B * (C - X)))
+        bound = 2 * eps * (1 + beta * (abs(c) + x) / 4)
+        for form in (0, 1):
+            pm, pp = O.fermi_pair(form, c, x, beta)
+            e = max(float(abs(pm - (P - M))), float(abs(2 * math.sqrt(pp) - 2 * mpmath.sqrt(P * M))))
+            worst[form] = max(worst[form], e / bound)
+    assert worst[0] <= 1.0 and worst[1] <= 1.0, worst
+    # T = 0 (beta <= 0) and |beta c| > 345 take the two Fermi functions as they stand
+    for c, x, beta in ((0.3, 0.5, 0.0), (-0.2, 0.1, -1.0), (2.0, 0.3, 200.0)):
+        assert O.fermi_pair(1, c, x, beta) == O.fermi_pair(0, c, x, beta)
+
+
+@pytest.mark.parametrize("name", ["PIFLUX3D_EQUILIBRIUM", "HONEYCOMB_EQUILIBRIUM", "HONEYCOMB_EQUILIBRIUM_CONT"])
+def test_kernels_relaxation_form_runs_like_the_reference_form(name):
+    """The kernels' arrangement of the equilibrium targets (one exponential,
+    three divisions; oracle eq_form 1), integrated by the oracle's controller on
+    the CPU: every trajectory finishes with the reference form's attempt count,
+    none fails, and j(t) agrees to 1e-13.  (A formulation that made the step
+    controller stall would show here as failed trajectories or extra attempts.)"""
+    import tests.problems as P
+    pr = getattr(P, name)
+    g = pr.o_grid()
+    kz = g[3] if len(g) > 3 else None
+    out = {}
+    for form in (0, 1):
+        md = dict(pr.model)
+        md["eq_form"] = form
+        out[form] = O.current_density(O.model(**md), pr.o_pulse(), pr.o_time(), pr.o_solver(), g[0], g[1], g[2],
+                                      kz=kz)
+    (j0, s0), (j1, s1) = out[0], out[1]
+    assert s1.failed_trajectories == 0 and s0.failed_trajectories == 0
+    assert s1.attempts == s0.attempts
+    assert rel_l2(j1, j0) <= 1e-13
