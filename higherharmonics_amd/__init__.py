@@ -56,7 +56,8 @@ class ModelDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("relax", ctypes.c_int32), ("v", ctypes.c_double),
                 ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
                 ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double),
-                ("relax_time_offdiag", ctypes.c_double), ("split_dispersion", ctypes.c_double)]
+                ("relax_time_offdiag", ctypes.c_double), ("split_dispersion", ctypes.c_double),
+                ("perp_current", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class PulseDesc(ctypes.Structure):
@@ -751,6 +752,16 @@ class _System:
         with Plan([Job(self.model_desc(), laser, time_config, 0)], [self.kgrid], self.solver, rank, n_ranks) as p:
             out = p.execute()[0]
             self.last_stats = p.stats.as_dict()
+        if getattr(self, "perp_current", False):
+            # j, then j_perp; warn where j_perp does not vanish (Honeycomb.cpp:328-334)
+            n = len(out) // 2
+            out, self.last_j_perp = out[:n], out[n:]
+            bad = (np.abs(self.last_j_perp) > 1e-6 * np.abs(out)) & (np.abs(self.last_j_perp) > 1e-6)
+            if bad.any():
+                import warnings
+                i = int(np.argmax(np.where(bad, np.abs(self.last_j_perp), 0.0)))
+                warnings.warn(f"j_perp = {self.last_j_perp[i]} at sample {i} (j_par = {out[i]}); "
+                              f"{int(bad.sum())} samples exceed 1e-6")
         return out
 
 
@@ -787,11 +798,13 @@ class PiFlux2D(_System):
 
 
 class Honeycomb(_System):
-    """Honeycomb nearest-neighbour model over the hexagonal BZ (Honeycomb.cpp:225-336)."""
+    """Honeycomb nearest-neighbour model over the hexagonal BZ (Honeycomb.cpp:225-336).
+    perp_current: the job's output is j, then the reference's consistency
+    check j_perp (Honeycomb.cpp:328-334), N + 1 samples each."""
     kind = MODEL_HONEYCOMB
 
-    def __init__(self, hopping=1.0, nx=128, ny=128, **kw):
-        self.hopping, self.nx, self.ny = hopping, nx, ny
+    def __init__(self, hopping=1.0, nx=128, ny=128, perp_current=False, **kw):
+        self.hopping, self.nx, self.ny, self.perp_current = hopping, nx, ny, perp_current
         super().__init__(**kw)
 
     def default_grid(self) -> KGrid:
@@ -799,7 +812,7 @@ class Honeycomb(_System):
 
     def model_desc(self) -> ModelDesc:
         return ModelDesc(MODEL_HONEYCOMB, self.relax, 0.0, 0.0, self.hopping, self.theta, self.fermi_energy,
-                         self.beta, self.relax_time)
+                         self.beta, self.relax_time, 0.0, 0.0, int(bool(self.perp_current)))
 
 
 class DiracSystem3D(_System):

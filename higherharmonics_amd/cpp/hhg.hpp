@@ -25,6 +25,7 @@
 #include <cstddef>
 #include <cmath>
 #include <cstdint>
+#include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -325,7 +326,7 @@ class System {
         job.time = time_config.desc();
         job.kgrid = 0;
         Dispatch::Plan plan({kgrid}, {job}, solver, rank, n_ranks);
-        return plan.compute(stats)[0];
+        return take_perp(plan.compute(stats)[0]);
     }
 
     // the same, executing one plan `repeats` times (timing): the solve time of
@@ -347,13 +348,32 @@ class System {
             if (solve_ms) solve_ms->push_back(st.solve_ms);
             if (stats) *stats = st;
         }
-        return j;
+        return take_perp(std::move(j));
     }
+
+    // model.perp_current (honeycomb): j_perp of the last computation, N + 1 samples
+    mutable std::vector<h_float> last_j_perp;
 
     virtual ~System() = default;
 
    protected:
     System() = default;
+
+    // With model.perp_current the job's output is j, then j_perp: keep j_perp
+    // and warn where it does not vanish, as Honeycomb::compute_current_density
+    // does (Honeycomb.cpp:328-334)
+    std::vector<h_float> take_perp(std::vector<h_float> j) const {
+        if (!model.perp_current) return j;
+        const size_t n = j.size() / 2;
+        last_j_perp.assign(j.begin() + (std::ptrdiff_t)n, j.end());
+        j.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            const h_float jp = last_j_perp[i];
+            if (std::abs(jp) > 1e-6 * std::abs(j[i]) && std::abs(jp) > 1e-6)
+                std::cerr << "Warning: j_perp = " << jp << "\t\tj_par = " << j[i] << "!\n";
+        }
+        return j;
+    }
 };
 
 // 2D massive Dirac cone d = (v qx, v qy, Delta) on a GL box |kx|, |ky| <= k_max
@@ -577,6 +597,7 @@ class Honeycomb : public System {
     Honeycomb(h_float temperature, h_float E_F, h_float v_F_, h_float band_width_, h_float photon_energy_,
               h_float diagonal_relaxation_time, int N_k = 64)
         : Honeycomb(band_width_ / 6.0, 2 * N_k, 2 * N_k) {  // hopping W/6 (Honeycomb.cpp:90)
+        model.perp_current = 1;  // the reference's j_perp check on every run (Honeycomb.cpp:328-334)
         v_F = v_F_;
         band_width = band_width_;
         photon_energy = photon_energy_;

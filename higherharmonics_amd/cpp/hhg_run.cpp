@@ -386,6 +386,9 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     o << "  \"momentum_amplitude\": " << num_str(laser->momentum_amplitude) << ", \"trajectories\": " << stats.trajectories
       << ", \"attempted_steps\": " << stats.attempts << ", \"rejects\": " << stats.rejects
       << ", \"solve_ms\": " << num_str(stats.solve_ms) << ",\n";
+    // the honeycomb's consistency check j_perp (Honeycomb.cpp:328-334; the
+    // reference only prints its warnings, which the host library repeats)
+    if (honeycomb) write_array(o, "current_density_perp", sys.last_j_perp);
     write_array(o, "solve_ms_runs", runs_ms, true);
     o << "}\n";
     emit(o.str(), argc, argv);

@@ -596,6 +596,17 @@ struct Bloch {
         }
     }
 
+    // honeycomb j_perp vertex (the y component at theta = 0) divided by
+    // perp_factor(): with G_y = sum_delta delta_y e^{i q.delta}
+    // = i sqrt3 sin(sqrt3/2 ky) e^{-i qx/2}, Re G_y s_x + Im G_y s_y
+    // = sqrt3 sin(sqrt3/2 ky) (sin(qx/2) s_x + cos(qx/2) s_y)
+    __device__ __forceinline__ double vertex_perp_phase(double sa, double ca, const double s[3]) const {
+        double sn, cs;
+        shifted_phase(sa, ca, sn, cs);
+        return sn * s[0] + cs * s[1];
+    }
+    __device__ __forceinline__ double perp_factor() const { return kSqrt3 * sin(kHalfSqrt3 * ky); }
+
     // current vertex g(s, A) along the polarisation
     __device__ __forceinline__ double vertex(double A, const double s[3]) const {
         if constexpr (M == MODEL_PIFLUX3D) {

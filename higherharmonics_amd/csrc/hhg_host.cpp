@@ -698,6 +698,8 @@ hhg_status validate_job(const hhg_job_desc& jd, const hhg_solver_desc& sv, int n
         return fail(HHG_ERROR_UNSUPPORTED, at + "the Magnus step is unitary: relaxation needs RKF78");
     if (m.split_dispersion != 0.0 && (!piflux3d || sv.method != HHG_METHOD_RKF78 || !(m.split_dispersion > 0.0)))
         return fail(HHG_ERROR_UNSUPPORTED, at + "the split current is the 3D pi-flux lattice's with RKF78 and a cut > 0");
+    if (m.perp_current != 0 && (m.perp_current != 1 || m.kind != HHG_MODEL_HONEYCOMB || sv.method != HHG_METHOD_RKF78))
+        return fail(HHG_ERROR_UNSUPPORTED, at + "j_perp (perp_current 1) is the honeycomb's with RKF78");
     if (sv.method == HHG_METHOD_MAGNUS_MOMENTS && m.kind != HHG_MODEL_DIRAC3D)
         return fail(HHG_ERROR_UNSUPPORTED, at + "the moment Magnus step is the 3D Dirac cone's (HHG_MODEL_DIRAC3D)");
     return HHG_SUCCESS;

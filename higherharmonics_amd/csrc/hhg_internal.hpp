@@ -112,12 +112,14 @@ struct JobDev {
     int64_t nominal_base; // offset of this job in the nominal-step table (N rows of kNomRow)
     int64_t item_base;    // first item of this job (items are job-major before reordering)
     int64_t traj_base;    // first trajectory of this job in per-trajectory arrays (first-step proposals)
-    int32_t n_items, pad1;
+    int32_t n_items;
+    int32_t n_outputs;    // currents per sample: 2 with split_cut > 0 or perp_current, else 1
     // corners of the cos^2 envelopes inside sampling intervals (carried and
     // learned proposals stop there): times, interval index i (t_{i-1} < t < t_i)
     double kink_t[4];
     int32_t kink_i[4];
-    int32_t n_kinks, pad2;
+    int32_t n_kinks;
+    int32_t perp_current; // honeycomb: the second output is the reference's j_perp (Honeycomb.cpp:328-334)
 };
 
 struct Counters {

@@ -231,6 +231,8 @@ hhg::JobDev make_job(const hhg_job_desc& jd, const hhg_solver_desc& sv) {
         J.fermi_w = m.relax == HHG_RELAX_EQUILIBRIUM && m.beta > 0.0 && std::fabs(y) <= 345.0 ? std::exp(y) : 0.0;
     }
     J.split_cut = m.split_dispersion;
+    J.perp_current = m.perp_current;
+    J.n_outputs = J.split_cut > 0.0 || J.perp_current ? 2 : 1;
     J.pulse = p.kind;
     J.theta_zero = m.theta == 0.0;
     J.amp_a = p.amp_a;
@@ -554,7 +556,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
         J.sample_base = p->offsets[i];
         J.traj_base = traj_total;
         traj_total += J.n_k;
-        p->offsets[i + 1] = p->offsets[i] + (J.split_cut > 0.0 ? 2 : 1) * (int64_t)(J.n_meas + 1);
+        p->offsets[i + 1] = p->offsets[i] + J.n_outputs * (int64_t)(J.n_meas + 1);
         J.magnus_base = magnus_words;
         if (sv.method == HHG_METHOD_MAGNUS4) magnus_words += 2 * (int64_t)J.n_meas * J.magnus_sub;
         if (sv.method == HHG_METHOD_MAGNUS_MOMENTS) magnus_words += 4 * (int64_t)J.n_meas;

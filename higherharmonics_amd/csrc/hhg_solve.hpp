@@ -199,8 +199,11 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const double* Pt = a.Pt + 2 * J.sample_base;  // phase models: (sin f A_i, cos f A_i)
     // 3D pi-flux split current (PiFlux.cpp:998-1006): the samples of points
     // with |m(k - A)| >= split_cut go to the job's second output
+    // honeycomb j_perp (Honeycomb.cpp:328-334): the second output
     SampleSink sink_hi{a.acc + 2 * (J.sample_base + N + 1), N, 0.0, false, to_lds ? 2 * (N + 1) : -1, a.acc_lds_off};
     const bool split = M == MODEL_PIFLUX3D && J.split_cut > 0.0;
+    const bool perp = M == MODEL_HONEYCOMB && J.perp_current;
+    const double w_perp = perp && active ? w * f.perp_factor() : 0.0;
     auto emit = [&](int i, double g) {
         if constexpr (M == MODEL_PIFLUX3D) {
             if (split) {
@@ -209,6 +212,9 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
                 sink_hi.put(i, wave_sum(lo ? 0.0 : g), lane);
                 return;
             }
+        }
+        if constexpr (M == MODEL_HONEYCOMB) {
+            if (perp) sink_hi.put(i, wave_sum(w_perp * f.vertex_perp_phase(Pt[2 * i], Pt[2 * i + 1], s)), lane);
         }
         sink.put(i, wave_sum(g), lane);
     };
@@ -411,7 +417,7 @@ __device__ __forceinline__ void solve_loop(const KernelArgs& a, Item&& item_fn) 
     int cur_job = -1;
     auto flush_lds = [&](int job) {  // every thread of the block: LDS words read, zeroed, added to HBM
         JobRef J = job_ref(a.jobs + job);
-        const int n_s = (J.split_cut > 0.0 ? 2 : 1) * (J.n_meas + 1);
+        const int n_s = J.n_outputs * (J.n_meas + 1);
         for (int q = threadIdx.x; q < n_s; q += blockDim.x) {
             const unsigned long long lo = lacc[2 * q], hi = lacc[2 * q + 1];
             lacc[2 * q] = 0ull;

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define HHG_AMD_ABI_VERSION 3
+#define HHG_AMD_ABI_VERSION 4
 
 typedef enum hhg_status {
     HHG_SUCCESS = 0,
@@ -97,6 +97,11 @@ typedef struct hhg_model_desc {
     double split_dispersion;   /* HHG_MODEL_PIFLUX3D with RKF78, > 0: the job's output is two currents of N + 1
                                   samples, first the points with |m(k - A(t_i))| below this value, then the rest
                                   (PiFlux::current_per_energy, PiFlux.cpp:954-1035, cut sqrt(3)/2) */
+    int32_t perp_current;      /* HHG_MODEL_HONEYCOMB with RKF78, 1: the job's output is two currents of N + 1
+                                  samples, j along the polarisation, then the reference's consistency check
+                                  j_perp = j_x sin(theta) + j_y cos(theta) (Honeycomb.cpp:328-334; theta = 0,
+                                  so j_y), which vanishes on the mirror-symmetric hexagonal grid */
+    int32_t pad;
 } hhg_model_desc;
 
 /* ---- pulses: A(t) = amplitude * envelope * sin(t + cep) ---- */

@@ -35,7 +35,7 @@ class Model(ctypes.Structure):
                 ("mass", ctypes.c_double), ("hopping", ctypes.c_double), ("theta", ctypes.c_double),
                 ("fermi_energy", ctypes.c_double), ("beta", ctypes.c_double), ("relax_time", ctypes.c_double),
                 ("relax_time_offdiag", ctypes.c_double), ("ic_rescue", ctypes.c_int32), ("eq_form", ctypes.c_int32),
-                ("split_dispersion", ctypes.c_double)]
+                ("split_dispersion", ctypes.c_double), ("perp_current", ctypes.c_int32), ("pad2", ctypes.c_int32)]
 
 
 class Pulse(ctypes.Structure):
@@ -119,12 +119,13 @@ def _dp(a: np.ndarray):
 
 def model(kind=MODEL_DIRAC2D, v=1.0, mass=0.1, hopping=1.0, theta=0.0, fermi_energy=0.0, beta=0.0,
           relax=RELAX_NONE, relax_time=0.0, relax_time_offdiag=0.0, ic_rescue=0, split_dispersion=0.0,
-          eq_form=0) -> Model:
+          eq_form=0, perp_current=0) -> Model:
     """eq_form 1 evaluates the equilibrium-relaxation targets in the HIP
     kernels' one-exponential form (a CPU check of that form); 0 is the
-    reference's."""
+    reference's.  perp_current 1 (honeycomb) appends the reference's j_perp
+    to j."""
     return Model(kind, relax, v, mass, hopping, theta, fermi_energy, beta, relax_time, relax_time_offdiag,
-                 ic_rescue, eq_form, split_dispersion)
+                 ic_rescue, eq_form, split_dispersion, perp_current, 0)
 
 
 def pulse(kind=PULSE_COS2, amp_a=0.5, amp_b=0.0, n_cycles=8.0, sigma=0.0, t0=0.0, delay=0.0, cep=0.0) -> Pulse:
@@ -254,7 +255,8 @@ def current_density(m: Model, p: Pulse, tc: Time, sv: Solver, kx, ky, w, n_threa
         kz = np.ascontiguousarray(kz, dtype=np.float64)
         kzp = _dp(kz)
     split = m.kind == MODEL_PIFLUX3D and m.split_dispersion > 0
-    j = np.zeros((2 if split else 1) * (tc.n_measurements + 1))
+    perp = m.kind == MODEL_HONEYCOMB and m.perp_current == 1
+    j = np.zeros((2 if split or perp else 1) * (tc.n_measurements + 1))
     st = Stats()
     r = lib(fast).or_current_density(ctypes.byref(m), ctypes.byref(p), ctypes.byref(tc), ctypes.byref(sv),
                                      _dp(kx), _dp(ky), kzp, _dp(w), len(kx), _dp(j), ctypes.byref(st), n_threads)

@@ -613,6 +613,25 @@ double or_vertex(const or_model* m, double kx, double ky, double kz, double A, c
     }
 }
 
+/* The honeycomb's j_perp vertex: the reference's check buffer_x sin(theta) +
+ * buffer_y cos(theta) (Honeycomb.cpp:328-334; theta = 0 there, so it is the
+ * y component of the current), with buffer_x, buffer_y the components of
+ * shifted_grad_gamma contracted with (s_x, s_y) (Honeycomb.cpp:273-279). */
+static double or_vertex_perp(const or_model* m, double kx, double ky, double A, const double s[3]) {
+    const double ct = cos(m->theta), st = sin(m->theta);
+    const double qx = kx - A * ct, qy = ky - A * st;
+    double gxr = 0, gxi = 0, gyr = 0, gyi = 0;
+    for (int n = 0; n < 3; ++n) {
+        const double ph = NN[n][0] * qx + NN[n][1] * qy;
+        const double c = cos(ph), sn = sin(ph);
+        gxr += NN[n][0] * c;
+        gxi += NN[n][0] * sn;
+        gyr += NN[n][1] * c;
+        gyi += NN[n][1] * sn;
+    }
+    return st * (gxr * s[0] + gxi * s[1]) + ct * (gyr * s[0] + gyi * s[1]);
+}
+
 /* ------------------------------------------------------------------------- */
 /* RKF78 with the odeint controller                                           */
 /* ------------------------------------------------------------------------- */
@@ -1110,6 +1129,7 @@ typedef struct {
     double kx, ky, kz, w;
     double* row;
     int split_offset; /* 0: no split; else N + 1, the row offset of the part above the cut */
+    int perp_offset;  /* 0: no j_perp; else N + 1, the row offset of the honeycomb's j_perp */
 } acc_user;
 
 static void emit_acc(void* user, int i, const double s[3]) {
@@ -1124,6 +1144,7 @@ static void emit_acc(void* user, int i, const double s[3]) {
         u->row[disp < u->m->split_dispersion ? i : u->split_offset + i] += g;
         return;
     }
+    if (u->perp_offset) u->row[u->perp_offset + i] += u->w * or_vertex_perp(u->m, u->kx, u->ky, u->A[i], s);
     u->row[i] += g;
 }
 
@@ -1134,7 +1155,8 @@ int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* 
                            double* j_out, or_stats* st, int n_threads) {
     const int N = tc->n_measurements;
     const int split = m->kind == OR_MODEL_PIFLUX3D && m->split_dispersion > 0.0;
-    const int W = split ? 2 * (N + 1) : N + 1; /* output samples */
+    const int perp = m->kind == OR_MODEL_HONEYCOMB && m->perp_current == 1;
+    const int W = split || perp ? 2 * (N + 1) : N + 1; /* output samples */
     const int64_t n_chunks = (n_k + OR_CHUNK - 1) / OR_CHUNK;
     double* A = (double*)malloc(sizeof(double) * (size_t)(N + 1));
     double* partial = (double*)calloc((size_t)n_chunks * (size_t)W, sizeof(double));
@@ -1158,7 +1180,7 @@ int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* 
         const int64_t k_end = (c + 1) * OR_CHUNK < n_k ? (c + 1) * OR_CHUNK : n_k;
         for (int64_t k = c * OR_CHUNK; k < k_end; ++k) {
             const double z = kz ? kz[k] : 0.0;
-            acc_user u = {m, A, kx[k], ky[k], z, w[k], row, split ? N + 1 : 0};
+            acc_user u = {m, A, kx[k], ky[k], z, w[k], row, split ? N + 1 : 0, perp ? N + 1 : 0};
             const int r = evolve(m, p, tc, sv, kx[k], ky[k], z, &ls, emit_acc, &u, NULL);
             if (r > 0) failed++;
             if (r < 0) unsupported++;

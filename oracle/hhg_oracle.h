@@ -46,6 +46,9 @@ typedef struct {
     int32_t eq_form;          /* equilibrium relaxation targets: 0 the reference's formulas; 1 the HIP
                                  kernels' one-exponential form, restated for its CPU check */
     double split_dispersion;  /* 3D pi-flux, > 0: j split at |m(k - A)| into two outputs of N + 1 samples */
+    int32_t perp_current;     /* honeycomb, 1: a second output of N + 1 samples, the reference's j_perp
+                                 (Honeycomb.cpp:328-334) */
+    int32_t pad2;
 } or_model;
 
 typedef struct {
@@ -155,8 +158,8 @@ int or_occupations(const or_model* m, const or_pulse* p, const or_time* tc, cons
 
 /* j(t_i) = sum_k w_k g_k(t_i), i = 0..N, deterministic for any thread count
  * (kz may be null: all zero); with split_dispersion > 0, j_out holds 2 (N + 1)
- * samples (below the cut, then the rest).  Returns the number of failed
- * trajectories. */
+ * samples (below the cut, then the rest), with perp_current 1 (honeycomb) j
+ * then j_perp.  Returns the number of failed trajectories. */
 int64_t or_current_density(const or_model* m, const or_pulse* p, const or_time* tc, const or_solver* sv,
                            const double* kx, const double* ky, const double* kz, const double* w, int64_t n_k,
                            double* j_out, or_stats* st, int n_threads);

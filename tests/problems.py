@@ -71,7 +71,7 @@ class Problem:
         return H.ModelDesc(m.get("kind", 0), m.get("relax", 0), m.get("v", 1.0), m.get("mass", 0.1),
                            m.get("hopping", 1.0), m.get("theta", 0.0), m.get("fermi_energy", 0.0),
                            m.get("beta", 0.0), m.get("relax_time", 0.0), m.get("relax_time_offdiag", 0.0),
-                           m.get("split_dispersion", 0.0))
+                           m.get("split_dispersion", 0.0), m.get("perp_current", 0))
 
     def h_laser(self, H):
         p = self.pulse

@@ -544,6 +544,46 @@ def test_cpp_driver_reference_units_honeycomb(hhg, tmp_path):
     err = rel_l2(np.array(d["current_density_time"]), j_ref)
     print(f"  rel_l2={err:.3e}", flush=True)
     assert err <= J_TOL
+    # the reference's j_perp check (Honeycomb.cpp:328-334): recorded, and it
+    # vanishes on the mirror-symmetric grid, so no warning is printed
+    jp = np.array(d["current_density_perp"])
+    assert len(jp) == len(j_ref) and np.abs(jp).max() <= 1e-12 * np.abs(j_ref).max()
+    assert "j_perp" not in r.stderr
+
+
+def test_honeycomb_j_perp_matches_oracle(hhg):
+    """perp_current: the honeycomb job's second output is the reference's
+    j_perp (Honeycomb.cpp:328-334).  On the ky > 0 half of the hexagonal grid
+    (an explicit grid, where j_perp does not vanish) both outputs match the
+    oracle within 1e-9; on the full grid j is bitwise the job's j without
+    j_perp and j_perp vanishes (mirror symmetry)."""
+    pr = HONEYCOMB_SMALL
+    kx, ky, w = pr.o_grid()
+    up = ky > 0
+    m_ref = pr.o_model()
+    m_ref.perp_current = 1
+    md = pr.h_model(hhg)
+    md.perp_current = 1
+    args = (pr.o_pulse(), pr.o_time(), pr.o_solver())
+    n = pr.time[2] + 1
+    print(f"[case] honeycomb j_perp, half grid of {int(up.sum())} points", flush=True)
+    j_ref, _ = O.current_density(m_ref, *args, kx[up], ky[up], w[up])
+    with hhg.Plan([hhg.Job(md, pr.h_laser(hhg), pr.h_time(hhg), 0)], [hhg.KGrid.explicit(kx[up], ky[up], w[up])],
+                  pr.h_solver(hhg)) as plan:
+        assert plan.n_out == 2 * n
+        j = plan.execute()[0]
+    e_par, e_perp = rel_l2(j[:n], j_ref[:n]), rel_l2(j[n:], j_ref[n:])
+    print(f"  half grid: j rel_l2={e_par:.3e}, j_perp rel_l2={e_perp:.3e}, "
+          f"max|j_perp|/max|j|={np.abs(j[n:]).max() / np.abs(j[:n]).max():.2e}", flush=True)
+    assert e_par <= J_TOL and e_perp <= J_TOL
+    print("[case] honeycomb j_perp, full grid", flush=True)
+    with hhg.Plan([hhg.Job(md, pr.h_laser(hhg), pr.h_time(hhg), 0)], [pr.h_grid(hhg)], pr.h_solver(hhg)) as plan:
+        j2 = plan.execute()[0]
+    with pr.h_plan(hhg) as plan:
+        j1 = plan.execute()[0]
+    print(f"  full grid: max|j_perp|/max|j|={np.abs(j2[n:]).max() / np.abs(j1).max():.2e}", flush=True)
+    assert np.array_equal(j2[:n], j1)
+    assert np.abs(j2[n:]).max() <= 1e-12 * np.abs(j1).max()
 
 
 def _run_driver(hhg, tmp_path, text, name):

@@ -23,7 +23,7 @@ def test_exports_every_declared_symbol(hhg):
     L = hhg.lib()
     for name in declared:
         assert hasattr(L, name), name
-    assert L.hhg_abi_version() == 3
+    assert L.hhg_abi_version() == 4
 
 
 def test_gauss_legendre_matches_reference_tables(hhg):
@@ -219,15 +219,19 @@ def test_cpp_driver_rejects_unknown_laser(hhg, tmp_path):
     assert r.returncode == 1 and "not recognized" in r.stderr
 
 
-@pytest.mark.parametrize("kind,relax,proposal,method", [
-    (4, 3, 1, 0),  # equilibrium relaxation with the carried proposal
-    (4, 2, 0, 0),  # the 3D pi-flux lattice has no T2 dephasing
-    (0, 3, 0, 0),  # equilibrium relaxation of another model
-    (4, 0, 0, 2),  # the moment Magnus step is the 3D Dirac cone's
-], ids=["equilibrium-carry", "piflux3d-t2", "dirac2d-equilibrium", "piflux3d-moments"])
-def test_plan_rejects_unsupported_combinations(hhg, kind, relax, proposal, method):
+@pytest.mark.parametrize("kind,relax,proposal,method,perp", [
+    (4, 3, 1, 0, 0),  # equilibrium relaxation with the carried proposal
+    (4, 2, 0, 0, 0),  # the 3D pi-flux lattice has no T2 dephasing
+    (0, 3, 0, 0, 0),  # equilibrium relaxation of another model
+    (4, 0, 0, 2, 0),  # the moment Magnus step is the 3D Dirac cone's
+    (0, 0, 0, 0, 1),  # j_perp is the honeycomb's
+    (2, 0, 0, 1, 1),  # j_perp needs RKF78
+    (2, 0, 0, 0, 2),  # perp_current is 0 or 1
+], ids=["equilibrium-carry", "piflux3d-t2", "dirac2d-equilibrium", "piflux3d-moments", "dirac2d-perp",
+        "honeycomb-magnus-perp", "honeycomb-perp-2"])
+def test_plan_rejects_unsupported_combinations(hhg, kind, relax, proposal, method, perp):
     """Validation runs on the host before any device call and names the reason."""
-    m = hhg.ModelDesc(kind, relax, 1.0, 0.1, 0.6, 0.0, 0.0, 0.0, 10.0, 4.0)
+    m = hhg.ModelDesc(kind, relax, 1.0, 0.1, 0.6, 0.0, 0.0, 0.0, 10.0, 4.0, 0.0, perp)
     job = hhg.Job(m, hhg.ContinuousLaser(0.5), hhg.TimeIntegrationConfig(0.0, 6.0, 64, 50), 0)
     grid = hhg.KGrid.piflux_continuum(2, 1) if kind == 4 else hhg.KGrid.box(4, 4, -1.0, 1.0, -1.0, 1.0)
     with pytest.raises(hhg.HHGError) as ei:
