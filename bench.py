@@ -392,8 +392,9 @@ def uneven_steps_leg(H, n=768, n_pulses=16, per_cycle=32, log_n=96, audit_frac=0
     the quietest, so a wave issues as many attempts per interval as its
     busiest lane.  Legs: plain, carried, learned (first-interval proposal),
     the network proposing every interval's first step, and learned with
-    step-count grouping by the network's cost estimate; each counts the
-    attempts its waves issue; oracle audit of the learned and carried legs."""
+    step-count grouping by the network's cost estimate (64 cost classes in
+    grid order, and a full sort by the estimate); each counts the attempts
+    its waves issue; oracle audit of the learned and carried legs."""
     from higherharmonics_amd import workloads as W
     jobs = W.honeycomb_cos2(n=n, n_pulses=n_pulses, per_cycle=per_cycle)
     small = W.honeycomb_cos2(n=log_n, n_pulses=n_pulses, per_cycle=per_cycle)
@@ -407,7 +408,9 @@ def uneven_steps_leg(H, n=768, n_pulses=16, per_cycle=32, log_n=96, audit_frac=0
     for name, sv, kw in [("plain", H.Solver(), {}), ("carry", H.Solver(proposal=H.PROPOSAL_CARRY), {}),
                          ("learned", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {}),
                          ("learned_every_interval", every, {}),
-                         ("learned_grouped", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {"group_by_cost": 1})]:
+                         ("learned_grouped", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net), {"group_by_cost": 1}),
+                         ("learned_grouped_full_sort", H.Solver(proposal=H.PROPOSAL_LEARNED, mlp=net),
+                          {"group_by_cost": 1 << 30})]:
         print(f"[bench] uneven-step leg {name}", file=sys.stderr, flush=True)
         leg = dict(run_leg(H, W, jobs, sv, 3, 1, count_wave_attempts=1, **kw), steps=3)
         legs[name] = leg_summary(leg, K, f_attempt, vertex)
@@ -423,7 +426,13 @@ def uneven_steps_leg(H, n=768, n_pulses=16, per_cycle=32, log_n=96, audit_frac=0
             "learned_grouped_vs_carry": legs["learned_grouped"]["trajectories_per_s"] /
             legs["carry"]["trajectories_per_s"],
             "wave_attempts_learned_grouped_over_carry": legs["learned_grouped"]["wave_attempts"] /
-            legs["carry"]["wave_attempts"]}
+            legs["carry"]["wave_attempts"],
+            "learned_grouped_full_sort_vs_carry": legs["learned_grouped_full_sort"]["trajectories_per_s"] /
+            legs["carry"]["trajectories_per_s"],
+            "wave_attempts_learned_every_interval_over_carry": legs["learned_every_interval"]["wave_attempts"] /
+            legs["carry"]["wave_attempts"],
+            "rejects_learned_every_interval_over_carry": legs["learned_every_interval"]["rejects"] /
+            max(1, legs["carry"]["rejects"])}
 
 
 def config2_leg(H, steps=3):

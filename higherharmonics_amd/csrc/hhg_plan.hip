@@ -9,6 +9,7 @@
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
+#include <limits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -481,6 +482,8 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
     if (sv.proposal < 0 || sv.proposal > 2) return fail(HHG_ERROR_INVALID_ARGUMENT, "unknown proposal");
     if (sv.proposal == HHG_PROPOSAL_LEARNED && (!sv.mlp || sv.method != HHG_METHOD_RKF78))
         return fail(HHG_ERROR_INVALID_ARGUMENT, "the learned proposal needs RKF78 and a network (solver.mlp)");
+    if (d->launch.group_by_cost < 0)
+        return fail(HHG_ERROR_INVALID_ARGUMENT, "group_by_cost must be >= 0 (the number of cost classes; 1 = 64)");
     if (d->launch.group_by_cost && !sv.mlp)
         return fail(HHG_ERROR_INVALID_ARGUMENT, "grouping by predicted cost needs a network (solver.mlp)");
     if (sv.mlp && hhg::mlp_device(sv.mlp) != d->device)
@@ -670,6 +673,11 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
 
     if (sv.mlp) p->d_mlp = hhg::mlp_device_record(sv.mlp);
 
+    // intervals the cost probe sums the network's attempts output over
+#ifndef HHG_COST_PROBES
+#define HHG_COST_PROBES 16
+#endif
+    constexpr int kCostProbes = HHG_COST_PROBES;
     // lockstep grouping: order each job's k-points by the network's predicted
     // attempts so that the lanes of a wave need similar work
     if (p->launch.group_by_cost) {
@@ -692,7 +700,7 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
             a.At = p->d_At;
             a.mlp = p->d_mlp;
             if (st == HHG_SUCCESS)
-                st = check_hip(hhg::launch_cost_probe(g.model, g.pulse, a, 16, d_cost,
+                st = check_hip(hhg::launch_cost_probe(g.model, g.pulse, a, kCostProbes, d_cost,
                                                       hhg::kMlpLdsBytes, p->stream),
                                "cost probe");
         }
@@ -702,12 +710,26 @@ hhg_status hhg_plan_create(const hhg_plan_desc* d, hhg_plan** out) {
             st = check_hip(hipMemcpy(cost.data(), d_cost, sizeof(float) * p->n_kpts, hipMemcpyDeviceToHost), "cost");
         hipFree(d_cost);
         if (st != HHG_SUCCESS) return bail(st);
+        // cost classes: the quantiles of the job's predictions; costliest class
+        // first, grid order inside a class
+        const int64_t n_cls = p->launch.group_by_cost == 1 ? 64 : p->launch.group_by_cost;
+        for (auto& x : cost)  // a NaN prediction counts as the costliest (keeps the sorts well-ordered)
+            if (x != x) x = std::numeric_limits<float>::infinity();
         std::vector<int64_t> perm(p->n_kpts, 0);
         for (const auto& J : p->jobs) {
-            std::vector<int64_t> order(J.n_k);
-            for (int64_t r = 0; r < J.n_k; ++r) order[r] = r;
+            if (J.n_k == 0) continue;
             const float* c = cost.data() + J.k_base;
-            std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return c[x] > c[y]; });
+            std::vector<float> sorted(c, c + J.n_k);
+            std::sort(sorted.begin(), sorted.end());
+            const int64_t nc = std::min<int64_t>(n_cls, J.n_k);
+            std::vector<float> edge;  // class q holds costs in [edge[q - 1], edge[q])
+            for (int64_t q = 1; q < nc; ++q) edge.push_back(sorted[(size_t)((q * J.n_k) / nc)]);
+            auto cls = [&](int64_t r) {
+                return (int64_t)(std::upper_bound(edge.begin(), edge.end(), c[r]) - edge.begin());
+            };
+            std::vector<int64_t> order(J.n_k), key(J.n_k);
+            for (int64_t r = 0; r < J.n_k; ++r) order[r] = r, key[r] = cls(r);
+            std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return key[x] > key[y]; });
             for (int64_t r = 0; r < J.n_k; ++r) perm[J.k_base + r] = order[r];
         }
         if ((st = check_hip(hipMemcpy(p->d_perm, perm.data(), sizeof(int64_t) * p->n_kpts, hipMemcpyHostToDevice),

@@ -250,8 +250,12 @@ typedef struct hhg_launch_desc {
     int32_t waves_per_block; /* 0 = 4, or 8 with block accumulation; or 1, 2, 4, 8 */
     int32_t queue_chunk;     /* 0 = default/autotuned: work items grabbed per queue pop */
     int32_t persistent;      /* 1 = persistent grid + device work queue, 0 = one wave per item */
-    int32_t group_by_cost;   /* 1 = order k-points inside a job by the network's predicted step count
-                                (needs solver.mlp) so that lanes of a wave need similar work */
+    int32_t group_by_cost;   /* > 0: order k-points inside a job by the network's predicted step count
+                                (needs solver.mlp) so that lanes of a wave need similar work: 1 = 64 cost
+                                classes (quantiles of the prediction), n >= 2 = n classes; inside a class
+                                the k-grid order is kept, so neighbouring points (whose step counts rise
+                                and fall together in time) stay in one wave.  A full sort by the prediction
+                                (n = the job's point count) scatters them and costs more than it saves */
     int32_t count_wave_attempts; /* 1 = count hhg_stats.wave_attempts (a few instructions per interval) */
     int32_t block_reduce;    /* persistent launches: 0 or 1 = every wave adds its exact sample sums to the job's
                                 accumulators in HBM; 2 = the waves of a block add them to an LDS copy of the
