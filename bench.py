@@ -10,11 +10,11 @@ Headline workload (one "step" = one full solve of all 16 jobs):
   9,437,184 trajectories per step.
 
 Legs on that workload: the learned proposal (a BF16 network trained on the
-GPU from step logs of the same pulses on a 96 x 96 grid proposes an
-interval's first step where the carried step did not do; the embedded-error
-controller judges every step), the plain controller (the reference's restart
-at Delta/50 every interval) and the carried controller (the non-learned
-alternative).  ``value`` is the learned leg's trajectories/s when its j(t),
+GPU from step logs of the same pulses on a 96 x 96 grid proposes every
+trajectory's first step; later intervals start from the controller's carried
+step; the embedded-error controller judges every step), the plain controller
+(the reference's restart at Delta/50 every interval) and the carried
+controller (the non-learned alternative).  ``value`` is the learned leg's trajectories/s when its j(t),
 on a seeded 1 % of every pulse's k-points, is within 1e-9 relative L2 of the
 CPU oracle at rtol = atol = 1e-13; else the plain leg's.
 

@@ -280,6 +280,13 @@ class Plan {
         return out;
     }
 
+    // Bloch vectors of job 0 at every sample for the listed k indices: [n_sel][N+1][3]
+    std::vector<h_float> trajectories(const std::vector<int64_t>& k_index, int n_samples) {
+        std::vector<h_float> out(k_index.size() * (size_t)n_samples * 3);
+        check(hhg_trajectories(plan_.get(), k_index.data(), (int64_t)k_index.size(), out.data()));
+        return out;
+    }
+
     // (lower, upper) band occupations of job 0 for the listed k indices: [n_sel][N+1][2]
     std::vector<h_float> occupations(const std::vector<int64_t>& k_index, int n_samples) {
         std::vector<h_float> out(k_index.size() * (size_t)n_samples * 2);
@@ -353,6 +360,33 @@ class System {
 
     // model.perp_current (honeycomb): j_perp of the last computation, N + 1 samples
     mutable std::vector<h_float> last_j_perp;
+
+    // The Bloch vectors of single points at every sample, [point][N+1][3]: the
+    // time evolutions the reference's debug outputs record
+    // (compute_current_density_debug, DiracSystem.cpp:333-363, PiFlux.cpp:320-383);
+    // kz only for the 3D pi-flux lattice (may be empty)
+    std::vector<h_float> trajectories_at(const Laser::Laser* laser, const TimeIntegrationConfig& time_config,
+                                         const std::vector<h_float>& kx, const std::vector<h_float>& ky,
+                                         const std::vector<h_float>& kz) const {
+        std::vector<h_float> w(kx.size(), 1.0);
+        hhg_kgrid_desc g{};
+        g.kind = HHG_KGRID_EXPLICIT;
+        g.kx = kx.data();
+        g.ky = ky.data();
+        g.w = w.data();
+        g.kz = kz.empty() ? nullptr : kz.data();
+        g.n_points = (int64_t)kx.size();
+        hhg_job_desc job{};
+        job.model = model;
+        job.model.perp_current = 0;
+        job.pulse = laser->desc();
+        job.time = time_config.desc();
+        job.kgrid = 0;
+        Dispatch::Plan plan({g}, {job}, solver);
+        std::vector<int64_t> sel(kx.size());
+        for (size_t i = 0; i < sel.size(); ++i) sel[i] = (int64_t)i;
+        return plan.trajectories(sel, time_config.n_measurements + 1);
+    }
 
     virtual ~System() = default;
 

@@ -353,10 +353,20 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
         if (split_current == "only") return 0;
     }
 
+    // debug_data yes | no | only (hhg.cpp:78, 235-238): the time evolutions of 25 single
+    // points beside the current (compute_current_density_debug; only: no current)
+    const std::string debug_data = str(kv, "debug_data", "no");
+    if (debug_data != "no" && debug_data != "yes" && debug_data != "only")
+        throw std::invalid_argument("debug_data is yes, no or only");
+    if (debug_data != "no" && honeycomb)  // Honeycomb.cpp:111-115
+        throw std::runtime_error("Honeycomb::compute_current_density_debug is not implemented yet!");
+
     hhg_stats stats{};
     std::vector<double> runs_ms;
     const int repeats = (int)num(kv, "timing_repeats", 1);  // > 1: execute the plan that often (timing)
-    const std::vector<double> j = sys.compute_current_density_repeated(laser.get(), tc, repeats, &runs_ms, &stats);
+    const std::vector<double> j = debug_data == "only"
+                                      ? std::vector<double>((size_t)N + 1, 0.0)
+                                      : sys.compute_current_density_repeated(laser.get(), tc, repeats, &runs_ms, &stats);
     const Fourier::Spectrum sp = continuous ? Fourier::welch_spectrum(j, tc) : Fourier::trapezoidal_spectrum(j, tc);
     std::vector<double> A(N + (continuous ? 0 : 1));
     for (size_t i = 0; i < A.size(); ++i) A[i] = laser->laser_function(tc.t_begin + (double)i * tc.measure_every());
@@ -392,6 +402,61 @@ int run_reference_units(const std::string& system, const std::map<std::string, s
     write_array(o, "solve_ms_runs", runs_ms, true);
     o << "}\n";
     emit(o.str(), argc, argv);
+    if (debug_data == "no") return 0;
+
+    // the 25 debug points (n_debug_points, GlobalDefinitions.hpp:34) and their
+    // recorded quantity s_z: the 3D Dirac cone at k_z = (n_z/10) dk, kappa_i =
+    // (i + 1) kappa_max(k_z) / 26, weighted by k_z kappa / |k|
+    // (DiracSystem.cpp:333-363, 427-458); the 3D pi-flux lattice at
+    // (pi/2, (0.5 - 0.02 i) pi, 0.49 pi) (PiFlux.cpp:320-383)
+    constexpr int n_debug = 25;
+    std::vector<double> px(n_debug), py(n_debug), pz, weight(n_debug, 1.0);
+    if (dirac) {
+        const double dk = 2.0 * band_width / n_z, k_z = (double)(n_z / 10) * dk;
+        const double kappa_max = std::sqrt(band_width * band_width - k_z * k_z);
+        for (int i = 0; i < n_debug; ++i) {
+            px[i] = k_z;
+            py[i] = (i + 1) * kappa_max / (n_debug + 1);
+            weight[i] = k_z * py[i] / std::hypot(k_z, py[i]);
+        }
+    } else {
+        pz.assign(n_debug, 0.49 * HHG::pi);
+        for (int i = 0; i < n_debug; ++i) px[i] = 0.5 * HHG::pi, py[i] = 0.5 * HHG::pi - i * 0.02 * HHG::pi;
+    }
+    const std::vector<double> traj = sys.trajectories_at(laser.get(), tc, px, py, pz);
+    const size_t n_s = (size_t)N + 1;
+    std::ostringstream d;
+    d << "{\n  \"time\": \"" << time_stamp() << "\",\n";
+    write_array(d, "frequencies", sp.frequencies);
+    std::vector<std::vector<double>> rho(n_debug, std::vector<double>(n_s)), fre(n_debug), fim(n_debug);
+    for (int i = 0; i < n_debug; ++i) {
+        for (size_t t = 0; t < n_s; ++t) rho[i][t] = weight[i] * traj[((size_t)i * n_s + t) * 3 + 2];
+        if (continuous) {  // the reference transforms the debug sets of pulsed runs only (hhg.cpp:287-291)
+            fre[i].assign(n_s, 0.0);
+            fim[i].assign(n_s, 0.0);
+        } else {
+            const Fourier::Spectrum s = Fourier::trapezoidal_spectrum(rho[i], tc, 1);
+            fre[i] = s.real;
+            fim[i] = s.imag;
+        }
+    }
+    auto write_rows = [&](const char* name, const std::vector<std::vector<double>>& rows, bool last) {
+        d << "  \"" << name << "\": [\n";
+        for (int i = 0; i < n_debug; ++i) {
+            std::ostringstream r;
+            write_array(r, "x", rows[i], true);
+            std::string s = r.str();  // "  \"x\": [...]\n"
+            s = s.substr(s.find('['));
+            s.pop_back();
+            d << "    " << s << (i + 1 < n_debug ? ",\n" : "\n");
+        }
+        d << "  ]" << (last ? "\n" : ",\n");
+    };
+    write_rows("time_evolutions", rho, false);
+    write_rows("debug_fft_real", fre, false);
+    write_rows("debug_fft_imag", fim, true);
+    d << "}\n";
+    write_file(sibling(argc, argv, "time_evolution.json.gz"), d.str());
     return 0;
 }
 

@@ -586,6 +586,64 @@ def test_honeycomb_j_perp_matches_oracle(hhg):
     assert np.abs(j2[n:]).max() <= 1e-12 * np.abs(j1).max()
 
 
+@pytest.mark.parametrize("system", ["Dirac", "PiFlux"])
+def test_cpp_driver_debug_data(hhg, tmp_path, system):
+    """debug_data yes (hhg.cpp:78, 235-238, 343-352): hhg_run writes
+    time_evolution.json.gz with the s_z time evolutions of the reference's 25
+    debug points -- the 3D Dirac cone at k_z = (n_z/10) dk, kappa_i =
+    (i + 1) kappa_max / 26 weighted by k_z kappa/|k| (DiracSystem.cpp:333-363),
+    the 3D pi-flux lattice at (pi/2, (0.5 - 0.02 i) pi, 0.49 pi)
+    (PiFlux.cpp:320-383) -- and their trapezoidal transforms; each equals the
+    oracle's trajectory of that point (moment Magnus 1e-12, RKF78 1e-9).
+    debug_data only skips the current."""
+    import gzip
+    import json
+    import math
+    import os
+    import subprocess
+    from oracle import reference_units as R
+    if system == "Dirac":
+        text = ("system_type Dirac\nT 0\nE_F 0\nv_F 1.5e5\nband_width 5\nfield_amplitude 1.2\nphoton_energy 5.25\n"
+                "laser_type cosine\nn_laser_cycles 2\nt0_offset 0\nn_z 40\nn_kappa 6\nn_measurements 512\n")
+    else:
+        text = ("system_type PiFlux\nT 30\nE_F 2\nv_F 1.5e3\nband_width 10\nfield_amplitude 8\nphoton_energy 5.25\n"
+                "laser_type cosine\nn_laser_cycles 1\nt0_offset 0\nz_range 4\nn_coarse 1\nn_measurements 256\n")
+    exe = os.path.join(os.path.dirname(hhg.LIB_PATH), "hhg_run")
+    st = R.setup(R.parse(text))
+    n = st["time"].n_measurements + 1
+    if system == "Dirac":
+        dk = 2 * 5.0 / 40
+        kz0 = (40 // 10) * dk
+        kappa = np.array([(i + 1) * math.sqrt(25.0 - kz0 ** 2) / 26 for i in range(25)])
+        pts = [(kz0, k, 0.0) for k in kappa]
+        wts = kz0 * kappa / np.hypot(kz0, kappa)
+    else:
+        pts = [(0.5 * math.pi, 0.5 * math.pi - 0.02 * math.pi * i, 0.49 * math.pi) for i in range(25)]
+        wts = np.ones(25)
+    m = st["model"]
+    m.ic_rescue = 0
+    ref = np.array([wt * O.trajectory(m, st["pulse"], st["time"], st["solver"], *p)[0][:, 2] for p, wt in zip(pts, wts)])
+    for mode in ("yes", "only"):
+        folder = tmp_path / mode
+        folder.mkdir()
+        cfg = folder / "run.cfg"
+        cfg.write_text(text + f"debug_data {mode}\n")
+        out = folder / "current_density.json.gz"
+        print(f"[case] hhg_run {system} debug_data {mode}", flush=True)
+        r = subprocess.run([exe, str(cfg), str(out)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = json.loads(gzip.open(folder / "time_evolution.json.gz").read())
+        te = np.array(d["time_evolutions"])
+        assert te.shape == (25, n)
+        err = rel_l2(te, ref)
+        print(f"  time evolutions rel_l2={err:.3e}", flush=True)
+        assert err <= (1e-12 if system == "Dirac" else J_TOL)
+        f, re, im = O.trapezoidal_spectrum(te[3], st["time"].t_begin, st["time"].t_end, n - 1, zero_padding=1)
+        assert np.allclose(d["debug_fft_real"][3], re, rtol=1e-12, atol=1e-12 * np.abs(re).max())
+        j = np.array(json.loads(gzip.open(out).read())["current_density_time"])
+        assert (np.abs(j).max() == 0.0) == (mode == "only")
+
+
 def _run_driver(hhg, tmp_path, text, name):
     import gzip
     import json
