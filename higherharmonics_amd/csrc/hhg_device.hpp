@@ -539,6 +539,31 @@ struct Bloch {
         }
     }
 
+    // rhs_w for an in-plane field w = (wx, wy, 0) (the 2D lattice models): the
+    // products with wz = 0, which IEEE arithmetic does not let the compiler
+    // drop, left out (two operations per evaluation fewer)
+    __device__ __forceinline__ void rhs_w0(const double s[3], double wx, double wy, double out[3]) const {
+        out[0] = wy * s[2];
+        out[1] = -(wx * s[2]);
+        out[2] = wx * s[1] - wy * s[0];
+        if constexpr (R == RELAX_TO_INITIAL) {
+            const double r = J.relax_rate;
+            out[0] -= (s[0] - s0[0]) * r;
+            out[1] -= (s[1] - s0[1]) * r;
+            out[2] -= (s[2] - s0[2]) * r;
+        } else if constexpr (R == RELAX_T2) {
+            const double r = J.relax_rate;
+            const double ww = wx * wx + wy * wy;
+            double y = __builtin_amdgcn_rcp(ww);
+            y = fma(fma(-ww, y, 1.0), y, y);
+            y = fma(fma(-ww, y, 1.0), y, y);
+            const double proj = ww > 0.0 ? (wx * s[0] + wy * s[1]) * y : 0.0;
+            out[0] -= (s[0] - proj * wx) * r;
+            out[1] -= (s[1] - proj * wy) * r;
+            out[2] -= s[2] * r;
+        }
+    }
+
     // phase models: sn, cs = sin, cos of f (k - A) (pi-flux: only cs is read)
     __device__ __forceinline__ void rhs_sc(const double s[3], double sn, double cs, double out[3]) const {
         if constexpr (M == MODEL_PIFLUX3D) {
@@ -549,11 +574,13 @@ struct Bloch {
             out[2] = (cx * s[1] - cy * s[0]) * pf;
             if constexpr (R == RELAX_EQUILIBRIUM) relax_equilibrium(cz, s, out);
         } else if constexpr (M == MODEL_PIFLUX2D) {
-            rhs_w(s, (4.0 * J.hop) * cs, 2.0 * cy, 0.0, out);
+            rhs_w0(s, (4.0 * J.hop) * cs, 2.0 * cy, out);
         } else if constexpr (R != RELAX_EQUILIBRIUM) {
-            // honeycomb, w = 2 d with the factor 2 folded into t (exact)
-            const double cq = (cs - sn) * (cs + sn), sq = 2.0 * sn * cs, th = 2.0 * J.hop;
-            rhs_w(s, th * (cq + 2.0 * cs * cy), -th * (sq - 2.0 * sn * cy), 0.0, out);
+            // honeycomb, w = 2 d = th (Re g, -Im g), th = 2t, g = e^{i qx} + 2 cy e^{-i qx/2}:
+            // with a = th cos(qx/2), b = th sin(qx/2),
+            //   th Re g = a (cos(qx/2) + 2 cy) - b sin(qx/2),  -th Im g = b (2 cy - 2 cos(qx/2))
+            const double th = 2.0 * J.hop, a = th * cs, b = th * sn, c2 = 2.0 * cy;
+            rhs_w0(s, fma(a, cs + c2, -(b * sn)), b * fma(-2.0, cs, c2), out);
         } else {
             // honeycomb: -(s - s_eq(k - A))/tau (Honeycomb.cpp:180-191)
             double d[3];
@@ -687,6 +714,37 @@ struct StageVals<true> {
     double S[kPhaseStages], C[kPhaseStages];
 };
 
+// The stage values of a table row (words 2..W-1).
+template <class F>
+__device__ __forceinline__ void stage_values_from_row(const double* cv, StageVals<F::kPhase>& sv) {
+    if constexpr (F::kPhase) {
+#pragma unroll
+        for (int d = 0; d < kPhaseStages; ++d) sv.S[d] = cv[2 + 2 * d], sv.C[d] = cv[3 + 2 * d];
+    } else {
+#pragma unroll
+        for (int j = 0; j < 13; ++j) sv.A[j] = cv[2 + j];
+    }
+}
+
+// The stage values of the step (t, h) from the per-step pulse cache.
+template <class F>
+__device__ __forceinline__ void stage_values_from_cache(F& f, double t, double h, StageVals<F::kPhase>& sv) {
+    f.prepare(t);
+    if constexpr (F::kPhase) {
+        constexpr double fs = model_phase_scale(F::kModel);
+#pragma unroll
+        for (int d = 0; d < kPhaseStages; ++d) sincos(fs * f.pulse_at(F78::c[phase_stage_of(d)] * h), &sv.S[d], &sv.C[d]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 13; ++i) {
+            if (i == 7) sv.A[i] = sv.A[3];
+            else if (i == 11) sv.A[i] = sv.A[0];
+            else if (i == 12) sv.A[i] = sv.A[10];
+            else sv.A[i] = f.pulse_at(F78::c[i] * h);
+        }
+    }
+}
+
 template <class F>
 __device__ __forceinline__ void stage_pulses(F& f, double t, double h, const double* cand,
                                              StageVals<F::kPhase>& sv) {
@@ -697,30 +755,9 @@ __device__ __forceinline__ void stage_pulses(F& f, double t, double h, const dou
 #pragma unroll
         for (int j = 0; j < W; ++j) cv[j] = cand[j];
         hit = (cv[0] == t) && (cv[1] == h);
-        if constexpr (F::kPhase) {
-#pragma unroll
-            for (int d = 0; d < kPhaseStages; ++d) sv.S[d] = cv[2 + 2 * d], sv.C[d] = cv[3 + 2 * d];
-        } else {
-#pragma unroll
-            for (int j = 0; j < 13; ++j) sv.A[j] = cv[2 + j];
-        }
+        stage_values_from_row<F>(cv, sv);
     }
-    if (!hit) {
-        f.prepare(t);
-        if constexpr (F::kPhase) {
-            constexpr double fs = model_phase_scale(F::kModel);
-#pragma unroll
-            for (int d = 0; d < kPhaseStages; ++d) sincos(fs * f.pulse_at(F78::c[phase_stage_of(d)] * h), &sv.S[d], &sv.C[d]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 13; ++i) {
-                if (i == 7) sv.A[i] = sv.A[3];
-                else if (i == 11) sv.A[i] = sv.A[0];
-                else if (i == 12) sv.A[i] = sv.A[10];
-                else sv.A[i] = f.pulse_at(F78::c[i] * h);
-            }
-        }
-    }
+    if (!hit) stage_values_from_cache(f, t, h, sv);
 }
 
 // right-hand side at stage i
@@ -780,7 +817,9 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
         sc[q] = atol + rtol * (fabs(s[q]) + fabs(h) * fabs(k1[q]));
         tiny = tiny && sc[q] >= 0x1p-1000 && fabs(e[q]) < sc[q] * kTinyErrRatio;
     }
-    if (tiny) return 0.0;
+    // a wave-uniform branch: with a per-lane one the compiler computes the
+    // divisions for every lane and selects
+    if (__all(tiny)) return 0.0;
     double err = 0.0;
     bool nan = false;
 #pragma unroll
@@ -789,7 +828,7 @@ __device__ __forceinline__ double rkf78_attempt(F& f, const double s[3], double 
         nan |= (r != r);
         err = r > err ? r : err;
     }
-    return nan ? __builtin_nan("") : err;
+    return tiny ? 0.0 : nan ? __builtin_nan("") : err;
 }
 
 // Options and results of one sampling interval beyond the odeint semantics.
@@ -1007,10 +1046,31 @@ __device__ __forceinline__ void magnus_moments_step(const MomentLane& L, const d
 // ---------------------------------------------------------------------------
 // Butterfly over 64 lanes; IEEE addition is commutative, so every lane ends
 // with the bitwise-identical total.
+// v plus the value DPP control CTRL moves into each lane (rows outside ROWS
+// add 0); the two 32-bit halves move with the same control
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return v + __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a fully active wave, returned wave-uniform: a DPP
+// tree (pairs quad_perm [1,0,3,2], quads [2,3,0,1], half rows row_half_mirror,
+// rows row_mirror, then row_bcast:15 into rows 1 and 3 and row_bcast:31 into
+// rows 2 and 3) leaves the total in lane 63, which v_readlane returns.  All
+// moves are VALU operations: no LDS round trips on the per-sample path.  The
+// order of the additions is fixed, so the sum is deterministic.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
+    v = dpp_add<0xb1, 0xf>(v);
+    v = dpp_add<0x4e, 0xf>(v);
+    v = dpp_add<0x141, 0xf>(v);
+    v = dpp_add<0x140, 0xf>(v);
+    v = dpp_add<0x142, 0xa>(v);
+    v = dpp_add<0x143, 0xc>(v);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
