@@ -5,7 +5,10 @@
 Every translation unit is compiled for gfx950 only, with packed-FP32
 instructions disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``);
 the FP64 path does not use them and the flag keeps hipcc from generating them
-anywhere in the library.
+anywhere in the library.  ``-ffp-contract=on`` fuses a*b + c into an FMA only
+within one source expression, as the C standard's FP_CONTRACT does, instead of
+wherever the backend finds a multiply feeding an add: the kernel variants of
+one solver (with and without step logging, say) then round identically.
 """
 from __future__ import annotations
 
@@ -25,7 +28,7 @@ SOURCES = ["hhg_solve_dirac.hip", "hhg_solve_piflux.hip", "hhg_solve_honeycomb.h
            "hhg_solve_piflux3d.hip", "hhg_solve_fused.hip", "hhg_kernels.hip", "hhg_mlp.hip", "hhg_plan.hip", "hhg_host.cpp"]
 HEADERS = ["hhg_device.hpp", "hhg_internal.hpp", "hhg_kernels.hpp", "hhg_host.hpp", "hhg_solve.hpp", "hhg_mlp.hpp"]
 
-COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=on",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 

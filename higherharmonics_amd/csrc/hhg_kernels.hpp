@@ -8,6 +8,17 @@ namespace hhg {
 
 struct MlpDev;
 
+// Learned-proposal kernels: LEARN_NONE (plain and carried controllers; the
+// learned first-interval policy is first_step_kernel followed by this kernel
+// reading KernelArgs::h_first) and LEARN_SCHEDULED (the network proposes the
+// first step of every infer_every-th interval inside the loop), each with or
+// without LOG_STEPS: only kernels built with it write step logs.  The logging
+// code holds the network's features per lane through the interval; without it
+// the headline's honeycomb kernel needs 205 instead of 237 VGPRs and spills
+// fewer scalar registers into vector lanes.
+enum : int { LEARN_NONE = 0, LEARN_SCHEDULED = 2, LOG_STEPS = 4 };
+__host__ __device__ constexpr int learn_mode(int learn) { return learn & 3; }
+
 struct KernelArgs {
     const JobDev* jobs;
     const int2* items;  // {job, chunk}

@@ -131,10 +131,17 @@ struct hhg_plan {
     bool first_step() const { return learned() && solver.infer_policy != HHG_INFER_SCHEDULED; }
     // the solve kernels' learned-proposal instantiation (LEARN_* of hhg_solve.hpp): the network runs
     // inside the solve kernel only for the scheduled policy
-    int solve_learn() const { return learned() && solver.infer_policy == HHG_INFER_SCHEDULED ? 2 : 0; }
+    // the solve kernels' LEARN argument: the scheduled policy's in-loop network,
+    // and the step-logging variant while a step log is set
+    int solve_learn() const {
+        return (learned() && solver.infer_policy == HHG_INFER_SCHEDULED ? hhg::LEARN_SCHEDULED : hhg::LEARN_NONE) |
+               (d_log ? hhg::LOG_STEPS : 0);
+    }
     // dynamic LDS of a group's solve launch: the network (scheduled learned
     // policy), then the block accumulators (2 words per sample)
-    size_t mlp_lds() const { return solve_learn() ? (size_t)hhg::kMlpLdsBytes : 0; }
+    size_t mlp_lds() const {
+        return hhg::learn_mode(solve_learn()) != hhg::LEARN_NONE ? (size_t)hhg::kMlpLdsBytes : 0;
+    }
     size_t acc_lds_off() const { return (mlp_lds() + 15) & ~(size_t)15; }
     size_t smem(const Group& g) const {
         return g.block_acc ? acc_lds_off() + 16 * (size_t)g.acc_samples : mlp_lds();
@@ -913,7 +920,9 @@ hhg_status hhg_plan_set_step_log(hhg_plan* p, int64_t capacity, int32_t stride) 
         p->log_cap = capacity;
     }
     p->log_stride = stride;
-    // a captured graph holds the old arguments
+    // the solves now run the logging (or the plain) kernel variant: launch
+    // shapes for it, and a captured graph holds the old kernels and arguments
+    for (auto& g : p->launches) size_group(p, g, g.waves_per_block, g.queue_chunk, p->launch.persistent);
     if (p->graph_exec) hipGraphExecDestroy(p->graph_exec), p->graph_exec = nullptr;
     if (p->graph) hipGraphDestroy(p->graph), p->graph = nullptr;
     return HHG_SUCCESS;

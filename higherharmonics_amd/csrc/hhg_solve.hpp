@@ -30,11 +30,6 @@ namespace hhg {
 // systems' configurations (3D pi-flux 874 vs 725 ms, honeycomb 751 vs 679 ms,
 // spline double pulse 406 vs 328 ms), so they stay at 2 (HHG_EQ_WAVES).
 // HHG_WAVES_PER_EU overrides all of them (occupancy experiments).
-// Learned-proposal kernels: LEARN_NONE (plain and carried controllers; the
-// learned first-interval policy is first_step_kernel followed by this kernel
-// reading KernelArgs::h_first) and LEARN_SCHEDULED (the network proposes the
-// first step of every infer_every-th interval inside the loop).
-enum : int { LEARN_NONE = 0, LEARN_SCHEDULED = 2 };
 
 #ifndef HHG_EQ_WAVES
 #define HHG_EQ_WAVES 2
@@ -229,12 +224,12 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const double h_restart = J.delta / J.n_sub;
     // the learned first-interval policy's proposal (first_step_kernel)
     double h_carry = (a.h_first && active) ? a.h_first[J.traj_base + kk] : h_restart;
-    const bool logging = a.log != nullptr;
+    const bool logging = (LEARN & LOG_STEPS) != 0 && a.log != nullptr;
     // step logs record exact error ratios; the network's feature clamps the
     // last ratio at 5^-8, which the tiny-error path's 0 satisfies.  Scheduled
     // learned solves hand an interval whose first step was rejected
     // fallback_after times back to the restart step Delta/n_sub.
-    const IntervalCtl ctl{LEARN == LEARN_SCHEDULED ? a.fallback_after : 0, h_restart,
+    const IntervalCtl ctl{learn_mode(LEARN) == LEARN_SCHEDULED ? a.fallback_after : 0, h_restart,
                           logging ? TINY_EXACT : TINY_ZERO};
     IntervalOut io{};
     io.last_err = 1e-3;
@@ -258,7 +253,7 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
         float feat[kMlpIn];
         const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
         bool featured = false;
-        if constexpr (LEARN == LEARN_SCHEDULED) {
+        if constexpr (learn_mode(LEARN) == LEARN_SCHEDULED) {
             if (i == next_infer) {
                 next_infer += a.infer_every;
                 propose(i, true, feat);
@@ -405,7 +400,7 @@ namespace solve_detail {
 template <int LEARN, class Item>
 __device__ __forceinline__ void solve_loop(const KernelArgs& a, Item&& item_fn) {
     const int lane = threadIdx.x & (WAVE - 1);
-    if constexpr (LEARN != LEARN_NONE) {
+    if constexpr (learn_mode(LEARN) != LEARN_NONE) {
         mlp_stage_lds(a.mlp, solve_dyn_lds);
         __syncthreads();
     }
@@ -724,6 +719,9 @@ struct ModelLaunch {
             switch (learn) {
             case LEARN_NONE: return solve_t<P, R, LEARN_NONE>(a, blocks, threads, smem, st, fn);
             case LEARN_SCHEDULED: return solve_t<P, R, LEARN_SCHEDULED>(a, blocks, threads, smem, st, fn);
+            case LEARN_NONE | LOG_STEPS: return solve_t<P, R, LEARN_NONE | LOG_STEPS>(a, blocks, threads, smem, st, fn);
+            case LEARN_SCHEDULED | LOG_STEPS:
+                return solve_t<P, R, LEARN_SCHEDULED | LOG_STEPS>(a, blocks, threads, smem, st, fn);
             }
         }
         return hipErrorInvalidValue;

@@ -36,11 +36,19 @@ hipError_t fused(const int* k1, const int* k2, int learned, const KernelArgs* a,
     if (is_pair<PiFluxDoubleGauss, HoneyDoubleCos2>(k1, k2)) {
         KernelArgs none{};
         const KernelArgs& args = a ? *a : none;
-        if (learned == LEARN_NONE)
+        switch (learned) {
+        case LEARN_NONE:
             return launch_pair<PiFluxDoubleGauss, HoneyDoubleCos2, LEARN_NONE>(args, blocks, threads, smem, st, fn);
-        if (learned == LEARN_SCHEDULED)
+        case LEARN_SCHEDULED:
             return launch_pair<PiFluxDoubleGauss, HoneyDoubleCos2, LEARN_SCHEDULED>(args, blocks, threads, smem, st,
                                                                                    fn);
+        case LEARN_NONE | LOG_STEPS:
+            return launch_pair<PiFluxDoubleGauss, HoneyDoubleCos2, LEARN_NONE | LOG_STEPS>(args, blocks, threads, smem,
+                                                                                         st, fn);
+        case LEARN_SCHEDULED | LOG_STEPS:
+            return launch_pair<PiFluxDoubleGauss, HoneyDoubleCos2, LEARN_SCHEDULED | LOG_STEPS>(args, blocks, threads,
+                                                                                              smem, st, fn);
+        }
     }
     return hipErrorInvalidValue;
 }
