@@ -35,6 +35,18 @@ __device__ __forceinline__ JobRef job_ref(const JobDev* p) {
     return *reinterpret_cast<ConstJob*>(reinterpret_cast<uintptr_t>(p));
 }
 
+// The same record through an opaque copy of its address.  A field read
+// through the copy cannot be hoisted above the point that makes it, so a loop
+// that makes the copy in every pass fetches the field with a scalar load where
+// it is used; read through the original, every field the loop needs is loaded
+// once before it and held in scalar registers for the whole loop, and the ones
+// that do not fit are spilled to VGPR lanes (a VALU slot at every use).
+__device__ __forceinline__ JobRef fresh_job(JobRef J) {
+    uint64_t p = reinterpret_cast<uint64_t>(&J);
+    asm volatile("" : "+s"(p));
+    return *reinterpret_cast<ConstJob*>(p);
+}
+
 // ---------------------------------------------------------------------------
 // Pulses: A(t) = amp * envelope * sin(t + cep); B = same shape at t - delay.
 // ---------------------------------------------------------------------------
@@ -338,13 +350,9 @@ struct Bloch {
         s0[2] = s[2];
     }
 
+    // the pulse at the stage times of a step: pc.prepare at its start t, then
+    // pc.at(dlt) = A(t + dlt)
     PulseCache<P> pc;
-
-    // start of a step at time t (stage times are t + c_i h)
-    __device__ __forceinline__ void prepare(double t) { pc.prepare(J, t); }
-
-    // A(t + dlt) (t = the prepared step start)
-    __device__ __forceinline__ double pulse_at(double dlt) const { return pc.at(J, dlt); }
 
     // Fermi pair of the equilibrium relaxations: P = f(c + x), M = f(c - x) for
     // x >= 0, f(e) = 1 / (1 + exp(beta e)) (GlobalDefinitions.hpp:36-41).  With
@@ -726,21 +734,23 @@ __device__ __forceinline__ void stage_values_from_row(const double* cv, StageVal
     }
 }
 
-// The stage values of the step (t, h) from the per-step pulse cache.
+// The stage values of the step (t, h) from the per-step pulse cache.  The
+// pulse parameters are read afresh at every step (fresh_job).
 template <class F>
 __device__ __forceinline__ void stage_values_from_cache(F& f, double t, double h, StageVals<F::kPhase>& sv) {
-    f.prepare(t);
+    JobRef J = fresh_job(f.J);
+    f.pc.prepare(J, t);
     if constexpr (F::kPhase) {
         constexpr double fs = model_phase_scale(F::kModel);
 #pragma unroll
-        for (int d = 0; d < kPhaseStages; ++d) sincos(fs * f.pulse_at(F78::c[phase_stage_of(d)] * h), &sv.S[d], &sv.C[d]);
+        for (int d = 0; d < kPhaseStages; ++d) sincos(fs * f.pc.at(J, F78::c[phase_stage_of(d)] * h), &sv.S[d], &sv.C[d]);
     } else {
 #pragma unroll
         for (int i = 0; i < 13; ++i) {
             if (i == 7) sv.A[i] = sv.A[3];
             else if (i == 11) sv.A[i] = sv.A[0];
             else if (i == 12) sv.A[i] = sv.A[10];
-            else sv.A[i] = f.pulse_at(F78::c[i] * h);
+            else sv.A[i] = f.pc.at(J, F78::c[i] * h);
         }
     }
 }

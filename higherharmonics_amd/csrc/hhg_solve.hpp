@@ -166,6 +166,18 @@ __device__ __forceinline__ double load_kz(const double* kz, int64_t idx) {
     else return 0.0;
 }
 
+// The launch arguments through an opaque copy of the kernel-argument segment
+// address (the kernels that solve RKF78 items take KernelArgs as their only
+// parameter, so the segment starts with it).  As with fresh_job, a copy made
+// in every pass of a loop keeps the loop from holding the fields it reads in
+// scalar registers from kernel entry on.
+using ConstArgs = const __attribute__((address_space(4))) KernelArgs;
+__device__ __forceinline__ ConstArgs& fresh_args() {
+    uint64_t p = reinterpret_cast<uint64_t>(__builtin_amdgcn_kernarg_segment_ptr());
+    asm volatile("" : "+s"(p));
+    return *reinterpret_cast<ConstArgs*>(p);
+}
+
 template <int M, int P, int R, int LEARN>
 __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneCounters& lc, bool to_lds) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -250,12 +262,13 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     int next_infer = 1;  // scheduled: interval of the next network proposal
     int kq = 0;  // next envelope corner
     for (int i = 1; i <= N; ++i) {
+        ConstArgs& A = fresh_args();  // the arguments the interval reads
         float feat[kMlpIn];
         const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
         bool featured = false;
         if constexpr (learn_mode(LEARN) == LEARN_SCHEDULED) {
             if (i == next_infer) {
-                next_infer += a.infer_every;
+                next_infer += A.infer_every;
                 propose(i, true, feat);
                 featured = true;
             }
@@ -273,14 +286,15 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
             // controller's carried step; the table's whole-interval step does not apply
             corner = kq < J.n_kinks && J.kink_i[kq] == i;
             const double* nom =
-                (a.nom && !corner) ? a.nom + J.nominal_base + (int64_t)(i - 1) * nom_row(M, R) : nullptr;
+                (A.nom && !corner) ? A.nom + J.nominal_base + (int64_t)(i - 1) * nom_row(M, R) : nullptr;
+            const SolverDev sp{A.sp.rtol, A.sp.atol, A.sp.max_fail, A.sp.proposal, A.sp.max_steps, 0};
             IntervalCtl c = ctl;
             double ta = t0;
             for (int q = kq;; ++q) {
                 const bool stop = q < J.n_kinks && J.kink_i[q] == i;
                 const double tb = stop ? J.kink_t[q] : t1;
                 if (status == 0) {
-                    rkf78_interval(f, s, ta, tb, h, unclipped, a.sp, lc.att, lc.rej, status, nom, c, io);
+                    rkf78_interval(f, s, ta, tb, h, unclipped, sp, lc.att, lc.rej, status, nom, c, io);
                     if (io.fell_back) lc.fallback++;
                 }
                 if (!stop) break;
@@ -305,7 +319,7 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
             }
         }
         // the wave issues as many attempts in this interval as its busiest lane
-        if (a.count_waves) lc.wave_att += wave_max_u32(lc.att - att0);
+        if (A.count_waves) lc.wave_att += wave_max_u32(lc.att - att0);
         while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
         emit(i, (status != 3) ? w * vertex(i) : 0.0);
     }
