@@ -1060,8 +1060,8 @@ __device__ __forceinline__ void magnus_moments_step(const MomentLane& L, const d
 // add 0); the two 32-bit halves move with the same control
 template <int CTRL, int ROWS>
 __device__ __forceinline__ double dpp_add(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, ROWS, 0xf, false);
     return v + __hiloint2double(hi, lo);
 }
 
@@ -1070,7 +1070,11 @@ __device__ __forceinline__ double dpp_add(double v) {
 // rows row_mirror, then row_bcast:15 into rows 1 and 3 and row_bcast:31 into
 // rows 2 and 3) leaves the total in lane 63, which v_readlane returns.  All
 // moves are VALU operations: no LDS round trips on the per-sample path.  The
-// order of the additions is fixed, so the sum is deterministic.
+// order of the additions is fixed, so the sum is deterministic.  The rows a
+// broadcast leaves out keep whatever the moved register held (no zeroing
+// move): they add an undefined value to rows 0 and 2, which lane 63's sum
+// never reads (row_bcast:15 reads lane 15 and row_bcast:31 lane 31 before the
+// addition of their step).
 __device__ __forceinline__ double wave_sum(double v) {
     v = dpp_add<0xb1, 0xf>(v);
     v = dpp_add<0x4e, 0xf>(v);

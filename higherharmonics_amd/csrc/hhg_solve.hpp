@@ -211,6 +211,13 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     const bool split = M == MODEL_PIFLUX3D && J.split_cut > 0.0;
     const bool perp = M == MODEL_HONEYCOMB && J.perp_current;
     const double w_perp = perp && active ? w * f.perp_factor() : 0.0;
+    // what the current vertex reads of the field at sample i: A_i, or
+    // (sin f A_i, cos f A_i) where the table holds phases (kPhase)
+    struct Field { double x, y; };
+    auto field = [&](int i) -> Field {
+        if constexpr (Bloch<M, P, R>::kPhase) return {Pt[2 * i], Pt[2 * i + 1]};
+        else return {At[i], 0.0};
+    };
     auto emit = [&](int i, double g) {
         if constexpr (M == MODEL_PIFLUX3D) {
             if (split) {
@@ -225,17 +232,23 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
         }
         sink.put(i, wave_sum(g), lane);
     };
-    auto vertex = [&](int i) {
-        if constexpr (Bloch<M, P, R>::kPhase) return f.vertex_phase(Pt[2 * i], Pt[2 * i + 1], s);
-        else return f.vertex(At[i], s);
+    auto vertex = [&](const Field& fa) {
+        if constexpr (Bloch<M, P, R>::kPhase) return f.vertex_phase(fa.x, fa.y, s);
+        else return f.vertex(fa.x, s);
     };
-    emit(0, active ? w * vertex(0) : 0.0);
+    {
+        const Field f0 = field(0);
+        emit(0, active ? w * vertex(f0) : 0.0);
+    }
 
     int status = active ? 0 : 3;
     const bool restart = a.sp.proposal == PROPOSAL_RESTART;
     const double h_restart = J.delta / J.n_sub;
     // the learned first-interval policy's proposal (first_step_kernel)
     double h_carry = (a.h_first && active) ? a.h_first[J.traj_base + kk] : h_restart;
+    // an empty statement that reads h_carry: its load completes here, before
+    // the interval loop, so no wait inside the loop has to count it
+    asm volatile("" ::"v"(h_carry));
     const bool logging = (LEARN & LOG_STEPS) != 0 && a.log != nullptr;
     // step logs record exact error ratios; the network's feature clamps the
     // last ratio at 5^-8, which the tiny-error path's 0 satisfies.  Scheduled
@@ -261,8 +274,31 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
     };
     int next_infer = 1;  // scheduled: interval of the next network proposal
     int kq = 0;  // next envelope corner
+    // The table slots the next interval's steps read first (the restart
+    // sequence's, or the whole-interval step's) are touched an interval ahead,
+    // one load per 64-byte line and one of the last word (every line of the
+    // range holds a touched word), so that the steps find them in the caches.
+    // The loaded value is only read by an empty statement at the top of the
+    // next interval, which keeps the load from being removed.  The sample's
+    // field is loaded before the steps, so that it has arrived when they end.
+    // Both hold registers across the steps: the equilibrium kernels, whose
+    // registers are all in use, do neither.
+    constexpr bool kEarlyLoads = R != RELAX_EQUILIBRIUM;
+    constexpr int W = nom_stride(M, R);
+    const int touch_first = restart ? 0 : kNomRestartSlots * W;
+    const int touch_words = restart ? kNomRestartSlots * W : W;
+    const int touch_lines = (touch_words + 7) / 8 + 1;
+    uint32_t touched = 0;
     for (int i = 1; i <= N; ++i) {
         ConstArgs& A = fresh_args();  // the arguments the interval reads
+        asm volatile("" ::"v"(touched));
+        if (kEarlyLoads && A.nom && i < N && lane < touch_lines) {
+            const double* nrow = A.nom + J.nominal_base + (int64_t)i * nom_row(M, R) + touch_first;
+            const int wd = 8 * lane < touch_words ? 8 * lane : touch_words - 1;
+            touched = reinterpret_cast<const uint32_t*>(nrow + wd)[0];
+        }
+        Field fi{};
+        if constexpr (kEarlyLoads) fi = field(i);
         float feat[kMlpIn];
         const bool log_this = logging && status == 0 && ((i + 7 * lane) % a.log_stride) == 0;
         bool featured = false;
@@ -321,7 +357,8 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
         // the wave issues as many attempts in this interval as its busiest lane
         if (A.count_waves) lc.wave_att += wave_max_u32(lc.att - att0);
         while (kq < J.n_kinks && J.kink_i[kq] <= i) ++kq;
-        emit(i, (status != 3) ? w * vertex(i) : 0.0);
+        if constexpr (!kEarlyLoads) fi = field(i);
+        emit(i, (status != 3) ? w * vertex(fi) : 0.0);
     }
     if (status == 1) lc.fail++;
     if (status == 2) lc.nonfin++;
