@@ -46,8 +46,9 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X vector FP64
 F_STEP = 450.0
 # Honeycomb right-hand side as the phase-table kernels evaluate it: sin and cos
 # of (kx - A)/2 by angle addition from the tabulated (sin A/2, cos A/2) (6),
-# cos qx and sin qx (5), w = 2 t (Re g, -Im g) (8), w x s with w_z = 0 (5).
-F_RHS = {"dirac_t2": 47.0, "dirac": 23.0, "piflux": 26.0, "honeycomb": 24.0}
+# w = 2 t (Re g, -Im g) from them with a = 2t cos, b = 2t sin: a (cos + 2 cy)
+# - b sin and b (2 cy - 2 cos) (9), w x s with w_z = 0 (5).
+F_RHS = {"dirac_t2": 47.0, "dirac": 23.0, "piflux": 26.0, "honeycomb": 20.0}
 F_VERTEX = {"dirac": 3.0, "piflux": 4.0, "honeycomb": 20.0}
 F_MAGNUS = 85.0
 # One network inference per lane: 2 x (8*64 + 2*64*64 + 64*2) FLOPs
@@ -60,8 +61,8 @@ HEADLINE = dict(n=768, n_pulses=16, per_cycle=512, log_n=96)
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB per dispatch, the factors
 # a plain 1 GiB copy reads) and the hardware FP64 FLOP (SQ_INSTS_VALU_FMA_F64 x 2
 # + MUL + ADD + TRANS, x 64 lanes).
-TRAFFIC_BYTES_HEADLINE = (420072.8125 * 2.0 + 26376256.0) * 1024
-HW_FP64_FLOP_HEADLINE = (2 * 258522183666 + 96392535742 + 58157531136 + 32615126) * 64
+TRAFFIC_BYTES_HEADLINE = (267518.625 * 2.0 + 26376256.0) * 1024
+HW_FP64_FLOP_HEADLINE = (2 * 239417339958 + 93898906842 + 30749147136 + 32467362) * 64
 
 
 def workload(small: bool):
