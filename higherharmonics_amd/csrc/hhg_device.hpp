@@ -736,14 +736,39 @@ __device__ __forceinline__ void stage_values_from_row(const double* cv, StageVal
 
 // The stage values of the step (t, h) from the per-step pulse cache.  The
 // pulse parameters are read afresh at every step (fresh_job).
+//
+// Phase models: sin and cos of f A at the first stage time (c = 0) in full;
+// the other stage times follow by angle addition, sin(x0 + e) = sin x0 cos e
+// + cos x0 sin e with e = f (A_d - A_0) and small_sincos(e).  Where an e of an
+// active lane is outside small_sincos's range, the active lanes evaluate every
+// stage's sincos in full instead (a branch uniform over them).  Both forms
+// agree to a few ulp.
 template <class F>
 __device__ __forceinline__ void stage_values_from_cache(F& f, double t, double h, StageVals<F::kPhase>& sv) {
     JobRef J = fresh_job(f.J);
     f.pc.prepare(J, t);
     if constexpr (F::kPhase) {
         constexpr double fs = model_phase_scale(F::kModel);
+        static_assert(F78::c[phase_stage_of(0)] == 0.0, "the first distinct stage time is the step's start");
+        const double x0 = fs * f.pc.at(J, 0.0);
+        sincos(x0, &sv.S[0], &sv.C[0]);
+        bool small = true;
 #pragma unroll
-        for (int d = 0; d < kPhaseStages; ++d) sincos(fs * f.pc.at(J, F78::c[phase_stage_of(d)] * h), &sv.S[d], &sv.C[d]);
+        for (int d = 1; d < kPhaseStages; ++d) {
+            const double e = fs * f.pc.at(J, F78::c[phase_stage_of(d)] * h) - x0;
+            small = small && fabs(e) <= kSmallArg;
+            double se, ce;
+            small_sincos(e, se, ce);
+            sv.S[d] = sv.S[0] * ce + sv.C[0] * se;
+            sv.C[d] = sv.C[0] * ce - sv.S[0] * se;
+        }
+        // marked unlikely: placed off the straight-line path, the full form
+        // leaves the surrounding loop's code as it is without it
+        if (__builtin_expect(!__all(small), 0)) {
+#pragma unroll
+            for (int d = 1; d < kPhaseStages; ++d)
+                sincos(fs * f.pc.at(J, F78::c[phase_stage_of(d)] * h), &sv.S[d], &sv.C[d]);
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < 13; ++i) {
