@@ -98,7 +98,7 @@ class LaunchDesc(ctypes.Structure):
     _fields_ = [("waves_per_block", ctypes.c_int32), ("queue_chunk", ctypes.c_int32),
                 ("persistent", ctypes.c_int32), ("group_by_cost", ctypes.c_int32),
                 ("count_wave_attempts", ctypes.c_int32), ("block_reduce", ctypes.c_int32),
-                ("separate_launches", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("separate_launches", ctypes.c_int32), ("loop_only", ctypes.c_int32)]
 
 
 class PlanDesc(ctypes.Structure):
@@ -585,7 +585,7 @@ class Plan:
     def __init__(self, jobs: Sequence[Job], kgrids: Sequence[KGrid], solver: Solver = Solver(),
                  rank: int = 0, n_ranks: int = 1, device: int = 0, waves_per_block: int = 0,
                  queue_chunk: int = 0, persistent: int = 1, group_by_cost: int = 0, count_wave_attempts: int = 0,
-                 block_reduce: int = 0, separate_launches: int = 0):
+                 block_reduce: int = 0, separate_launches: int = 0, loop_only: int = 0):
         L = lib()
         self._solver = solver  # keeps the network alive as long as the plan
         self._kgrids = list(kgrids)  # keep explicit arrays alive
@@ -594,7 +594,7 @@ class Plan:
                                      for j in jobs])
         d = PlanDesc(len(kgrids), len(jobs), kg, jd, solver.desc(),
                      LaunchDesc(waves_per_block, queue_chunk, persistent, group_by_cost, count_wave_attempts,
-                                block_reduce, separate_launches, 0),
+                                block_reduce, separate_launches, loop_only),
                      rank, n_ranks, device, 0)
         h = ctypes.c_void_p()
         _check(L.hhg_plan_create(ctypes.byref(d), ctypes.byref(h)))

@@ -707,6 +707,15 @@ __device__ __forceinline__ double max_growth() {
     return 0.9 / sqrt(sqrt(sqrt(e)));
 }
 
+// The accepted step's growth: for err < 0.5, h 0.9 err^(-1/8), at most
+// max_growth().
+__device__ __forceinline__ void grow_step(double& h, double err) {
+    if (err < 0.5) {
+        if (err > (1.0 / 390625.0)) h *= 0.9 / sqrt(sqrt(sqrt(err)));
+        else h *= max_growth();
+    }
+}
+
 // The pulse does not depend on the state, so the field at the stage times of
 // a step is produced first: from the nominal-step table row `cand` when its
 // (t, h) equal the step's, else from the per-step pulse cache.  All the row's
@@ -964,10 +973,7 @@ __device__ __forceinline__ void rkf78_interval(F& f, double s[3], double t0, dou
             s[2] = snew[2];
             io.last_err = err;
             if (io.h_acc == 0.0) io.h_acc = h, io.err_acc = err;
-            if (err < 0.5) {
-                if (err > (1.0 / 390625.0)) h *= 0.9 / sqrt(sqrt(sqrt(err)));
-                else h *= max_growth();
-            }
+            grow_step(h, err);
             fresh = true;
             if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) {
                 status = 2;

@@ -31,7 +31,7 @@ struct KernelArgs {
     int block_acc;       // nonzero: block accumulation in LDS (the block pops waves x queue_chunk items)
     int acc_lds_off;     // byte offset of the LDS accumulators in the dynamic LDS
     int acc_lds_samples; // samples the LDS accumulators hold (the group's largest job output)
-    int pad2;
+    int loop_only;       // nonzero: no whole-interval attempt outside the controller's loop
     const double* kx;
     const double* ky;
     const double* kz;  // 3D pi-flux lattice only (nullptr otherwise)

@@ -364,6 +364,7 @@ hhg::KernelArgs launch_args(const hhg_plan* p, const Group& g, int gi, int n_ite
     a.fallback_after = p->solver.fallback_after > 0 ? p->solver.fallback_after : 2;
     a.infer_policy = p->solver.infer_policy;
     a.count_waves = p->launch.count_wave_attempts;
+    a.loop_only = p->launch.loop_only;
     a.block_acc = g.block_acc;
     a.acc_lds_off = (int)p->acc_lds_off();
     a.acc_lds_samples = g.acc_samples;

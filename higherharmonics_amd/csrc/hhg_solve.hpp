@@ -325,8 +325,51 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
                 (A.nom && !corner) ? A.nom + J.nominal_base + (int64_t)(i - 1) * nom_row(M, R) : nullptr;
             const SolverDev sp{A.sp.rtol, A.sp.atol, A.sp.max_fail, A.sp.proposal, A.sp.max_steps, 0};
             IntervalCtl c = ctl;
+            // Whole-interval steps: when a carried or proposed step covers the
+            // interval, the loop clips it to the interval and its first attempt
+            // is the table's whole-interval slot.  When that holds for every
+            // active lane, that attempt runs outside the controller's loop, with
+            // one program path for the wave; it stands when every lane accepts
+            // it and it ends the interval, and it then leaves exactly the state
+            // the loop would.  Otherwise the interval runs through the loop from
+            // its start.  (A second copy of the attempt: the equilibrium kernels,
+            // whose registers are all in use, keep the loop alone.)
+            constexpr bool kWholeIntervalAttempt = R != RELAX_EQUILIBRIUM;
+            bool done = false;
+            constexpr int W = nom_stride(M, R);
+            const double hw = t1 - t0;
+            const bool whole = nom != nullptr && !restart && sp.max_steps >= 1 && t1 - t0 > kDblEps &&
+                               (t0 + h) - t1 > kDblEps;
+            if (kWholeIntervalAttempt && !A.loop_only && __all(whole)) {
+                const double* cv = nom + kNomRestartSlots * W;
+                double row[W];
+#pragma unroll
+                for (int j = 0; j < W; ++j) row[j] = cv[j];
+                const bool hit = row[0] == t0 && row[1] == hw;
+                StageVals<Bloch<M, P, R>::kPhase> sv;
+                stage_values_from_row<Bloch<M, P, R>>(row, sv);
+                double snew[3];
+                const double err = rkf78_attempt(f, s, hw, sv, snew, sp.rtol, sp.atol, c.tiny_mode);
+                const double t_next = t0 + hw;
+                const bool ok = hit && !(err > 1.0) && t_next > t0 && !(t1 - t_next > kDblEps);
+                if (__all(ok)) {
+                    s[0] = snew[0];
+                    s[1] = snew[1];
+                    s[2] = snew[2];
+                    ++lc.att;
+                    io.attempts = 1;
+                    io.fell_back = false;
+                    io.last_err = err;
+                    io.h_acc = hw;
+                    io.err_acc = err;
+                    h = hw;
+                    grow_step(h, err);
+                    if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]))) status = 2;
+                    done = true;
+                }
+            }
             double ta = t0;
-            for (int q = kq;; ++q) {
+            for (int q = kq; !done; ++q) {
                 const bool stop = q < J.n_kinks && J.kink_i[q] == i;
                 const double tb = stop ? J.kink_t[q] : t1;
                 if (status == 0) {

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define HHG_AMD_ABI_VERSION 4
+#define HHG_AMD_ABI_VERSION 5
 
 typedef enum hhg_status {
     HHG_SUCCESS = 0,
@@ -266,7 +266,11 @@ typedef struct hhg_launch_desc {
                                   fused kernel (honeycomb double cos^2 with PiFlux-2D double Gaussian, no
                                   relaxation) solves both in one launch over one work queue; 1 = one launch per
                                   group.  j is bitwise the same either way */
-    int32_t pad;
+    int32_t loop_only;         /* RKF78: 0 = where every active lane of a wave takes the nominal-step table's
+                                  whole-interval step first (carried and proposed steps that cover the
+                                  interval), that attempt runs outside the controller's loop and stands when
+                                  every lane accepts it; 1 = every interval through the loop.  j and the
+                                  counters are bitwise the same either way (a check of the first form) */
 } hhg_launch_desc;
 
 typedef struct hhg_plan_desc {

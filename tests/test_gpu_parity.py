@@ -999,3 +999,30 @@ def test_block_accumulation_bitwise_equal_to_wave_atomics(hhg):
             ref = plan.execute_accumulators()
         with pr.h_plan(hhg, solver=solver, block_reduce=2, waves_per_block=8) as plan:
             assert np.array_equal(plan.execute_accumulators(), ref)
+
+
+@pytest.mark.parametrize("per_cycle", [512, 32])
+def test_whole_interval_attempt_bitwise_equal_to_the_loop(hhg, per_cycle):
+    """The whole-interval attempt (a carried step that covers the interval,
+    run outside the controller's loop when every active lane of a wave takes
+    the table's whole-interval slot first) against every interval through the
+    loop (launch option loop_only): the same accumulators and step counters,
+    bitwise.  At 512 samples per cycle every interval takes that attempt; at 32
+    most steps are shorter than the interval and waves with a rejecting lane
+    go back to the loop.  Also the fused two-group kernel of a mixed sweep."""
+    from higherharmonics_amd import workloads as W
+    print(f"[case] whole-interval attempt vs loop, honeycomb 48x48 x 4 pulses at {per_cycle}/cycle", flush=True)
+    sweeps = [W.honeycomb_cos2(n=48, n_pulses=4, per_cycle=per_cycle)]
+    if per_cycle == 512:
+        sweeps.append(W.mixed_sweep(n=24, n_each=2, per_cycle=128, cos2_cycles=2, sigma_cycles=0.5))
+    for jobs in sweeps:
+        out = []
+        for loop_only in (0, 1):
+            with W.build_plan(hhg, jobs, hhg.Solver(proposal=hhg.PROPOSAL_CARRY), loop_only=loop_only) as plan:
+                acc = plan.execute_accumulators()
+                st = plan.stats
+            out.append((acc, st.attempts, st.rejects, st.failed_trajectories))
+        print(f"  attempts {out[0][1]}, rejects {out[0][2]}", flush=True)
+        assert out[0][1] > 0 and out[0][3] == 0
+        assert np.array_equal(out[0][0], out[1][0])
+        assert out[0][1:] == out[1][1:]

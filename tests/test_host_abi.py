@@ -23,7 +23,7 @@ def test_exports_every_declared_symbol(hhg):
     L = hhg.lib()
     for name in declared:
         assert hasattr(L, name), name
-    assert L.hhg_abi_version() == 4
+    assert L.hhg_abi_version() == 5
 
 
 def test_gauss_legendre_matches_reference_tables(hhg):
