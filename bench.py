@@ -61,7 +61,7 @@ HEADLINE = dict(n=768, n_pulses=16, per_cycle=512, log_n=96)
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB per dispatch, the factors
 # a plain 1 GiB copy reads) and the hardware FP64 FLOP (SQ_INSTS_VALU_FMA_F64 x 2
 # + MUL + ADD + TRANS, x 64 lanes).
-TRAFFIC_BYTES_HEADLINE = (267518.625 * 2.0 + 26376256.0) * 1024
+TRAFFIC_BYTES_HEADLINE = (271527.3125 * 2.0 + 26376256.0) * 1024
 HW_FP64_FLOP_HEADLINE = (2 * 239417339958 + 93898906842 + 30749147136 + 32467362) * 64
 
 

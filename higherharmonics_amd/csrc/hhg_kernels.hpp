@@ -14,7 +14,7 @@ struct MlpDev;
 // first step of every infer_every-th interval inside the loop), each with or
 // without LOG_STEPS: only kernels built with it write step logs.  The logging
 // code holds the network's features per lane through the interval; without it
-// the headline's honeycomb kernel needs 205 instead of 237 VGPRs and spills
+// the headline's honeycomb kernel needs 215 instead of 247 VGPRs and spills
 // fewer scalar registers into vector lanes.
 enum : int { LEARN_NONE = 0, LEARN_SCHEDULED = 2, LOG_STEPS = 4 };
 __host__ __device__ constexpr int learn_mode(int learn) { return learn & 3; }
