@@ -336,8 +336,7 @@ __device__ void solve_item_rkf78(JobRef J, int chunk, const KernelArgs& a, LaneC
             // whose registers are all in use, keep the loop alone.)
             constexpr bool kWholeIntervalAttempt = R != RELAX_EQUILIBRIUM;
             bool done = false;
-            constexpr int W = nom_stride(M, R);
-            const double hw = t1 - t0;
+            const double hw = t1 - t0;  // the loop's clip of a step that covers the interval
             const bool whole = nom != nullptr && !restart && sp.max_steps >= 1 && t1 - t0 > kDblEps &&
                                (t0 + h) - t1 > kDblEps;
             if (kWholeIntervalAttempt && !A.loop_only && __all(whole)) {

@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 from tests.problems import (CONFIG1, DIRAC3D_CONTINUOUS, DIRAC3D_COS2_RELAX, DIRAC3D_GAUSS_CONTINUOUS,
-                            DIRAC3D_GAUSS_COS2_RELAX, DIRAC_COHERENT, DIRAC_SMALL,
+                            DIRAC3D_GAUSS_COS2_RELAX, DIRAC_COHERENT, DIRAC_RELAX_INIT, DIRAC_SMALL,
                             HONEYCOMB_CORNERS, HONEYCOMB_EQUILIBRIUM, HONEYCOMB_EQUILIBRIUM_CONT, HONEYCOMB_SMALL,
                             HONEYCOMB_SPLINE,
                             PIFLUX3D_COS2, PIFLUX3D_EQUILIBRIUM, PIFLUX3D_OCCUPATION, PIFLUX3D_OCCUPATION_CONT,
@@ -1026,3 +1026,22 @@ def test_whole_interval_attempt_bitwise_equal_to_the_loop(hhg, per_cycle):
         assert out[0][1] > 0 and out[0][3] == 0
         assert np.array_equal(out[0][0], out[1][0])
         assert out[0][1:] == out[1][1:]
+
+
+@pytest.mark.parametrize("pr", [CONFIG1, DIRAC_RELAX_INIT, PIFLUX_SMALL, HONEYCOMB_SPLINE, HONEYCOMB_CORNERS,
+                                DIRAC3D_COS2_RELAX, PIFLUX3D_COS2], ids=lambda p: p.name)
+def test_whole_interval_attempt_bitwise_equal_to_the_loop_per_model(hhg, pr):
+    """The same check per model, pulse and relaxation with the carried step:
+    the Dirac-2D T2 and relax-to-initial forms, the 2D pi-flux lattice, a
+    splined pulse, envelope corners (whose intervals keep the loop) and both 3D
+    models."""
+    _print_case(pr, solver={"proposal": "carry"})
+    out = []
+    for loop_only in (0, 1):
+        with pr.h_plan(hhg, solver={"proposal": hhg.PROPOSAL_CARRY}, loop_only=loop_only) as plan:
+            acc = plan.execute_accumulators()
+            st = plan.stats
+        out.append((acc, st.attempts, st.rejects, st.failed_trajectories, st.nonfinite_trajectories))
+    assert out[0][1] > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
