@@ -196,7 +196,10 @@ template <int M, int P, int R, bool STICKY = false>
 struct Bloch {
     static constexpr int kModel = M;
     static constexpr int kRelax = R;
-    static constexpr bool kPhase = phase_table(M, R);  // stage values are (sin f A, cos f A)
+    // stage values are (sin f A, cos f A); the occupation numbers' integration
+    // (STICKY) takes them too, so that it computes cos(kz - A) with the same
+    // operations as the plain relaxing trajectory it is checked against
+    static constexpr bool kPhase = phase_table(M, R);
     JobRef J;
     double kx, ky, kz;
     double pk_s = 0.0, pk_c = 0.0;  // phase models: sin and cos of f kx (f kz for the 3D pi-flux lattice)
