@@ -651,6 +651,8 @@ def main():
                     help="only configuration 3 at full size (33.5M trajectories, one launch; one JSON object)")
     ap.add_argument("--config5-only", action="store_true",
                     help="only the mixed 128-pulse sweep (configuration 5; one JSON object)")
+    ap.add_argument("--uneven-only", action="store_true",
+                    help="only the uneven-step leg (the headline's pulses at 32 samples per cycle; one JSON object)")
     ap.add_argument("--profile-config5", choices=["plain", "plain_separate_launches", "carry"], default=None,
                     help="only time this configuration-5 leg (autotuned, graph-captured; profiler runs)")
     ap.add_argument("--reference-models-only", action="store_true",
@@ -703,6 +705,9 @@ def main():
         return
     if args.config5_only:
         print(json.dumps({"config5_mixed_sweep": config5_leg(H, audit_frac=args.audit_frac)}))
+        return
+    if args.uneven_only:
+        print(json.dumps({"uneven_steps_32_per_cycle": uneven_steps_leg(H, audit_frac=args.audit_frac)}))
         return
     if args.profile_leg:
         from higherharmonics_amd import workloads as W
